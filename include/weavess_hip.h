@@ -1,0 +1,163 @@
+/*
+ * weavess_hip.h -- C ABI of libweavess_hip.so, the MI355X (gfx950) kNN-graph
+ * hot path of WEAVESS: NN-Descent graph construction and greedy best-first
+ * search.
+ *
+ * Plain C, POD arguments, no exceptions and no torch types across the
+ * boundary.  Every call returns WV_OK (0) or a negative WV_E* code;
+ * wv_last_error() gives the text of the last failure on that context.
+ * Calls on one context are serialised by an internal mutex.  All calls are
+ * synchronous: when one returns, its outputs are written.  The caller owns
+ * every host and device buffer it passes in.
+ *
+ * Reference interfaces replaced (paths relative to the WEAVESS tree):
+ *   wv_index_build_nndescent / wv_nndescent_build
+ *       ComponentInitRandom::InitInner      src/component_init.cpp:11-45
+ *     + ComponentRefineNNDescent::RefineInner src/component_refine.cpp:12-77
+ *     + ComponentPruneNaive::PruneInner      src/component_prune.cpp:8-18
+ *       (virtual hooks: include/weavess/component.h:32-37 InitInner,
+ *        :361-366 RefineInner)
+ *   wv_index_search / wv_search_greedy
+ *       ComponentSearchEntryRand::SearchEntryInner src/component_search_entry.cpp:9-30
+ *     + ComponentSearchRouteGreedy::RouteInner     src/component_route.cpp:9-53
+ *       (virtual hooks: include/weavess/component.h:752-757, :827-832),
+ *       batched over the query loop of IndexBuilder::search
+ *       (src/builder.cpp:440-458)
+ *   wv_exact_knn
+ *       ground-truth generation, algo/knn/gen_knn.cpp:117-150 and
+ *       dataset/main.cpp:237-270 (FP64, self excluded, ties by id)
+ *
+ * Ordering convention: every list is ascending in (distance, id).  The
+ * reference orders by distance only; ids break its ties here.
+ */
+#ifndef WEAVESS_HIP_H
+#define WEAVESS_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define WV_OK 0
+#define WV_EINVAL (-22)    /* bad argument: null pointer, L < k, S >= N, ... */
+#define WV_ENOMEM (-12)    /* device allocation failed */
+#define WV_EHIP (-5)       /* a HIP runtime call failed */
+#define WV_ENOTSUP (-95)   /* shape outside what the kernels support */
+
+typedef struct wv_ctx wv_ctx;
+typedef struct wv_index wv_index;
+
+/* Join precision of NN-Descent. */
+#define WV_DTYPE_F32 0
+#define WV_DTYPE_BF16 1
+#define WV_DTYPE_FP8 2
+
+typedef struct {
+    uint32_t K;          /* "K": final degree (src/component_refine.cpp:81)          */
+    uint32_t L;          /* "L": pool size (:80)                                     */
+    uint32_t S;          /* "S": initial and per-iteration new samples (component_init.cpp:47-49) */
+    uint32_t R;          /* "R": reverse-sample cap (:83)                            */
+    uint32_t iters;      /* "ITER": (join, update) rounds incl. the empty first join (:84, :121-124) */
+    uint32_t dtype;      /* WV_DTYPE_*: distance precision inside the local join    */
+    uint32_t rerank_mult;/* low precision: the best rerank_mult*K of each pool are re-ranked with
+                            exact FP32 distances at the end (0 -> 2)                */
+    uint32_t cand_slots; /* per-node candidate table, power of two (0 -> automatic)  */
+    uint64_t seed;       /* random initialisation and reverse-sample choice          */
+} wv_nnd_params;
+
+typedef struct {
+    double total_ms;     /* InitRandom + NN-Descent + finalize, device time          */
+    double init_ms, sample_ms, join_ms, merge_ms, finalize_ms;
+    uint64_t pairs;      /* distance evaluations in all joins                        */
+    uint64_t proposals;  /* join results that passed the target's threshold          */
+    uint64_t rows_gathered; /* sample rows loaded by the joins                       */
+    uint32_t iters_run;
+    uint32_t cand_slots;
+} wv_build_stats;
+
+typedef struct {
+    double kernel_ms;    /* device time of the search launch(es)                     */
+    uint64_t dist_evals; /* sum over queries (entries + first-visited neighbours)    */
+    uint64_t hops;       /* sum over queries (expanded pool entries)                 */
+} wv_search_stats;
+
+/* Graph in "padded adjacency" form: ids[N*deg_max], unused slots any value,
+ * deg[N] valid counts.  The reference's on-disk graph (u32 degree, then that
+ * many u32 ids per node; src/builder.cpp:1068-1076) loads into this form. */
+typedef struct {
+    const uint32_t *ids;
+    const uint32_t *deg;
+    uint32_t N;
+    uint32_t deg_max;
+} wv_graph;
+
+/* Search tuning (the autotuner's knobs; results never depend on them). */
+typedef struct {
+    uint32_t waves_per_block;  /* 1, 2 or 4 queries per workgroup (0 -> auto)        */
+    uint32_t row_groups;       /* gather tile: 4*row_groups rows in flight (0 -> auto) */
+    uint32_t visited_slots;    /* LDS visited table size, power of two (0 -> auto)   */
+} wv_search_config;
+
+/* ---------------------------------------------------------------- context */
+int wv_ctx_create(int device, wv_ctx **out);
+int wv_ctx_destroy(wv_ctx *ctx);
+const char *wv_last_error(const wv_ctx *ctx);
+int wv_version(void);
+
+/* ---------------------------------------------------------------- index */
+/* A device-resident base set (and, once built or set, its graph). */
+int wv_index_create(wv_ctx *ctx, const float *base, uint32_t N, uint32_t d, wv_index **out);
+int wv_index_destroy(wv_index *idx);
+/* Replace the graph (host arrays, padded adjacency form). */
+int wv_index_set_graph(wv_index *idx, const uint32_t *ids, const uint32_t *deg, uint32_t deg_max);
+/* Copy the graph to the host: ids/dist are N*deg_max (dist may be NULL). */
+int wv_index_get_graph(wv_index *idx, uint32_t *ids, float *dist, uint32_t *deg, uint32_t *deg_max);
+int wv_index_set_search_config(wv_index *idx, const wv_search_config *cfg);
+
+/* NN-Descent on the resident base; the result becomes the index's graph
+ * (deg_max = K).  init_ids: optional N*S initial neighbour ids (host), as
+ * ComponentRefineNNDescent::init reads them from FinalGraph
+ * (src/component_refine.cpp:96-103); NULL runs InitRandom on the device. */
+int wv_index_build_nndescent(wv_index *idx, const wv_nnd_params *p, const uint32_t *init_ids,
+                             const uint32_t *init_cnt, wv_build_stats *stats);
+
+/* Greedy search of Q queries (host arrays).  entries: Q*L ids or NULL
+ * (seeded GenRandom per query).  L_per_q/k_per_q: NULL or Q values each
+ * (then `entries` must be NULL).  Requires k <= L, and L < N for random
+ * entries.  Outputs Q*k_stride (slots >= k get 0xFFFFFFFF / -1). */
+int wv_index_search(wv_index *idx, const float *queries, uint32_t Q, const uint32_t *entries,
+                    uint64_t seed, uint32_t L, uint32_t k, const uint32_t *L_per_q,
+                    const uint32_t *k_per_q, uint32_t k_stride, uint32_t *out_ids,
+                    float *out_dist, uint32_t *evals, uint32_t *hops, wv_search_stats *stats);
+/* Same with device pointers.  queries: Q rows of stride roundup(d,4) floats,
+ * zero padded.  Everything else as above, on the device. */
+int wv_index_search_dev(wv_index *idx, const float *d_queries, uint32_t Q,
+                        const uint32_t *d_entries, uint64_t seed, uint32_t L, uint32_t k,
+                        const uint32_t *d_L_per_q, const uint32_t *d_k_per_q, uint32_t k_stride,
+                        uint32_t *d_out_ids, float *d_out_dist, uint32_t *d_evals,
+                        uint32_t *d_hops, wv_search_stats *stats);
+
+/* ---------------------------------------------------------------- one-shot */
+/* The two stage entry points as single calls on host arrays. */
+int wv_nndescent_build(wv_ctx *ctx, const float *base, uint32_t N, uint32_t d,
+                       const wv_nnd_params *p, const uint32_t *init_ids, uint32_t *out_ids,
+                       float *out_dist, uint32_t *out_deg, wv_build_stats *stats);
+int wv_search_greedy(wv_ctx *ctx, const wv_graph *g, const float *base, uint32_t d,
+                     const float *queries, uint32_t Q, const uint32_t *entries, uint64_t seed,
+                     uint32_t L, uint32_t k, const uint32_t *L_per_q, const uint32_t *k_per_q,
+                     uint32_t k_stride, uint32_t *out_ids, float *out_dist, uint32_t *evals,
+                     uint32_t *hops, wv_search_stats *stats);
+
+/* ---------------------------------------------------------------- oracle */
+/* Exact kNN in FP64 on the device.  self_offset >= 0: query q is base row
+ * q + self_offset and is skipped (kNN graph); -1: nothing skipped.  k <= 128. */
+int wv_exact_knn(wv_ctx *ctx, const float *base, uint32_t N, const float *queries, uint32_t Q,
+                 uint32_t d, uint32_t k, int64_t self_offset, uint32_t *out_ids,
+                 double *out_dist, double *kernel_ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* WEAVESS_HIP_H */

@@ -1,0 +1,247 @@
+"""ctypes front of the CPU oracle (TEST INFRASTRUCTURE ONLY).
+
+Only tests/, ``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of
+bench.py may import this package; the product path never does.  It loads
+``oracle/libwv_oracle.so`` (built by ``make -C oracle``) and, for pinning the
+restatement against the reference's own headers, ``oracle/_ref/libwv_ref.so``.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libwv_oracle.so")
+REF_LIB_PATH = os.path.join(HERE, "_ref", "libwv_ref.so")
+
+_u32p = np.ctypeslib.ndpointer(np.uint32, flags="C_CONTIGUOUS")
+_f32p = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
+_f64p = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+
+
+class NndParams(C.Structure):
+    _fields_ = [("K", C.c_uint32), ("L", C.c_uint32), ("S", C.c_uint32),
+                ("R", C.c_uint32), ("iters", C.c_uint32),
+                ("cand_slots", C.c_uint32), ("seed", C.c_uint64)]
+
+
+class NndIterStats(C.Structure):
+    _fields_ = [("pairs", C.c_uint64), ("inserts", C.c_uint64)]
+
+
+def build(ref: bool = True) -> None:
+    """Compile the oracle (and the reference shim when the tree is present)."""
+    targets = ["all"]
+    if ref and os.path.isdir("/root/reference/include/weavess"):
+        targets.append("ref")
+    subprocess.run(["make", "-s", "-C", HERE] + targets, check=True)
+
+
+_lib = None
+_ref = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build(ref=False)
+        L = C.CDLL(LIB_PATH)
+        L.wvo_dist_ref.restype = C.c_float
+        L.wvo_dist_ref.argtypes = [_f32p, _f32p, C.c_uint32]
+        L.wvo_dist_canon.restype = C.c_float
+        L.wvo_dist_canon.argtypes = [_f32p, _f32p, C.c_uint32]
+        L.wvo_dist_f64.restype = C.c_double
+        L.wvo_dist_f64.argtypes = [_f32p, _f32p, C.c_uint32]
+        L.wvo_stream.restype = C.c_uint64
+        L.wvo_stream.argtypes = [C.c_uint64, C.c_uint64, C.c_uint64]
+        L.wvo_draw.restype = C.c_uint32
+        L.wvo_draw.argtypes = [C.c_uint64, C.c_uint64]
+        L.wvo_gen_random.restype = C.c_int
+        L.wvo_gen_random.argtypes = [C.c_uint64, _u32p, C.c_uint32, C.c_uint32]
+        L.wvo_gen_random_draws.restype = C.c_int
+        L.wvo_gen_random_draws.argtypes = [_u32p, _u32p, C.c_uint32, C.c_uint32]
+        L.wvo_init_random.restype = C.c_int
+        L.wvo_init_random.argtypes = [_f32p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint64,
+                                      _u32p, _f32p, _u32p, C.c_int]
+        L.wvo_nndescent.restype = C.c_int
+        L.wvo_nndescent.argtypes = [_f32p, C.c_uint32, C.c_uint32, C.POINTER(NndParams),
+                                    C.c_void_p, C.c_void_p, C.c_void_p,
+                                    _u32p, _f32p, _u32p, C.c_void_p, C.c_int]
+        L.wvo_search_greedy.restype = C.c_int
+        L.wvo_search_greedy.argtypes = [_u32p, _u32p, C.c_uint32, _f32p, C.c_uint32, C.c_uint32,
+                                        _f32p, C.c_uint32, C.c_void_p, C.c_uint64,
+                                        C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p,
+                                        C.c_uint32, _u32p, _f32p, _u32p, _u32p, C.c_int]
+        L.wvo_exact_knn.restype = C.c_int
+        L.wvo_exact_knn.argtypes = [_f32p, C.c_uint32, _f32p, C.c_uint32, C.c_uint32,
+                                    C.c_uint32, C.c_int64, _u32p, _f64p, C.c_int]
+        L.wvo_recall.restype = C.c_double
+        L.wvo_recall.argtypes = [_u32p, C.c_uint32, C.c_uint32, C.c_uint32, _u32p, C.c_uint32]
+        L.wvo_graph_quality.restype = C.c_double
+        L.wvo_graph_quality.argtypes = [_u32p, _u32p, C.c_uint32, C.c_void_p, C.c_uint32,
+                                        _u32p, C.c_uint32, C.c_uint32]
+        _lib = L
+    return _lib
+
+
+def ref_lib():
+    """The reference-header shim, or None where it was not built."""
+    global _ref
+    if _ref is None and os.path.exists(REF_LIB_PATH):
+        R = C.CDLL(REF_LIB_PATH)
+        R.wvr_dist.restype = C.c_float
+        R.wvr_dist.argtypes = [_f32p, _f32p, C.c_uint32]
+        R.wvr_gen_random.restype = None
+        R.wvr_gen_random.argtypes = [C.c_uint32, _u32p, C.c_uint32, C.c_uint32]
+        R.wvr_mt19937.restype = None
+        R.wvr_mt19937.argtypes = [C.c_uint32, _u32p, C.c_uint32]
+        _ref = R
+    return _ref
+
+
+def _c(a, dt):
+    return np.ascontiguousarray(a, dtype=dt)
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+# ---------------------------------------------------------------- distances
+def dist_ref(a, b) -> float:
+    a, b = _c(a, np.float32), _c(b, np.float32)
+    return float(lib().wvo_dist_ref(a, b, a.size))
+
+
+def dist_canon(a, b) -> float:
+    a, b = _c(a, np.float32), _c(b, np.float32)
+    return float(lib().wvo_dist_canon(a, b, a.size))
+
+
+def dist_f64(a, b) -> float:
+    a, b = _c(a, np.float32), _c(b, np.float32)
+    return float(lib().wvo_dist_f64(a, b, a.size))
+
+
+# ---------------------------------------------------------------- rng
+TAG_INIT = 0x494E4954
+TAG_ENTRY = 0x454E5452
+
+
+def stream(seed: int, tag: int, idx: int) -> int:
+    return int(lib().wvo_stream(seed, tag, idx))
+
+
+def draw(s: int, k: int) -> int:
+    return int(lib().wvo_draw(s, k))
+
+
+def gen_random(s: int, size: int, N: int) -> np.ndarray:
+    out = np.zeros(size, np.uint32)
+    if lib().wvo_gen_random(s, out, size, N) != 0:
+        raise ValueError("gen_random: size must be < N")
+    return out
+
+
+def gen_random_draws(draws, size: int, N: int) -> np.ndarray:
+    out = np.zeros(size, np.uint32)
+    d = _c(draws, np.uint32)
+    assert d.size >= size + 1
+    if lib().wvo_gen_random_draws(d, out, size, N) != 0:
+        raise ValueError("gen_random_draws: size must be < N")
+    return out
+
+
+# ---------------------------------------------------------------- nn-descent
+def init_random(base, S: int, seed: int, nthreads: int = 0):
+    base = _c(base, np.float32)
+    N, d = base.shape
+    ids = np.zeros((N, S), np.uint32)
+    dist = np.zeros((N, S), np.float32)
+    cnt = np.zeros(N, np.uint32)
+    if lib().wvo_init_random(base, N, d, S, seed, ids, dist, cnt, nthreads) != 0:
+        raise ValueError("init_random: bad arguments")
+    return ids, dist, cnt
+
+
+def nndescent(base, K, L, S, R, iters, seed=0, cand_slots=0, init=None,
+              nthreads: int = 0, with_stats: bool = False):
+    base = _c(base, np.float32)
+    N, d = base.shape
+    p = NndParams(K, L, S, R, iters, cand_slots, seed)
+    ids = np.zeros((N, K), np.uint32)
+    dist = np.zeros((N, K), np.float32)
+    deg = np.zeros(N, np.uint32)
+    st = (NndIterStats * max(iters, 1))()
+    if init is not None:
+        i_ids, i_dist, i_cnt = (_c(init[0], np.uint32), _c(init[1], np.float32),
+                                _c(init[2], np.uint32))
+        args = (_ptr(i_ids), _ptr(i_dist), _ptr(i_cnt))
+    else:
+        args = (None, None, None)
+    rc = lib().wvo_nndescent(base, N, d, C.byref(p), *args, ids, dist, deg,
+                             C.cast(st, C.c_void_p), nthreads)
+    if rc != 0:
+        raise ValueError("nndescent: bad arguments")
+    if with_stats:
+        return ids, dist, deg, [(s.pairs, s.inserts) for s in st[:iters]]
+    return ids, dist, deg
+
+
+# ---------------------------------------------------------------- search
+def search_greedy(graph, deg, base, queries, L, k, seed=0, entries=None,
+                  L_per_q=None, k_per_q=None, k_stride=None, nthreads: int = 0):
+    graph = _c(graph, np.uint32)
+    deg = _c(deg, np.uint32)
+    base = _c(base, np.float32)
+    queries = _c(queries, np.float32)
+    N, d = base.shape
+    Q = queries.shape[0]
+    deg_max = graph.shape[1]
+    if k_stride is None:
+        k_stride = int(np.max(k_per_q)) if k_per_q is not None else k
+    out_ids = np.zeros((Q, k_stride), np.uint32)
+    out_dist = np.zeros((Q, k_stride), np.float32)
+    evals = np.zeros(Q, np.uint32)
+    hops = np.zeros(Q, np.uint32)
+    ent = None if entries is None else _c(entries, np.uint32)
+    lq = None if L_per_q is None else _c(L_per_q, np.uint32)
+    kq = None if k_per_q is None else _c(k_per_q, np.uint32)
+    rc = lib().wvo_search_greedy(graph, deg, deg_max, base, N, d, queries, Q, _ptr(ent), seed,
+                                 L, k, _ptr(lq), _ptr(kq), k_stride, out_ids, out_dist,
+                                 evals, hops, nthreads)
+    if rc != 0:
+        raise ValueError("search_greedy: bad arguments")
+    return out_ids, out_dist, evals, hops
+
+
+# ---------------------------------------------------------------- exact / metrics
+def exact_knn(base, queries, k, self_offset=-1, nthreads: int = 0):
+    base = _c(base, np.float32)
+    queries = _c(queries, np.float32)
+    N, d = base.shape
+    Q = queries.shape[0]
+    ids = np.zeros((Q, k), np.uint32)
+    dist = np.zeros((Q, k), np.float64)
+    if lib().wvo_exact_knn(base, N, queries, Q, d, k, self_offset, ids, dist, nthreads) != 0:
+        raise ValueError("exact_knn: bad arguments")
+    return ids, dist
+
+
+def recall(res, gt, k) -> float:
+    res = _c(res, np.uint32)
+    gt = _c(gt, np.uint32)
+    return float(lib().wvo_recall(res, res.shape[0], k, res.shape[1], gt, gt.shape[1]))
+
+
+def graph_quality(ids, deg, gt, rows=None, control=10) -> float:
+    ids = _c(ids, np.uint32)
+    deg = _c(deg, np.uint32)
+    gt = _c(gt, np.uint32)
+    r = None if rows is None else _c(rows, np.uint32)
+    return float(lib().wvo_graph_quality(ids, deg, ids.shape[1], _ptr(r), gt.shape[0], gt,
+                                         gt.shape[1], control))

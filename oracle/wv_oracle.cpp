@@ -1,0 +1,708 @@
+/*
+ * wv_oracle.cpp -- CPU restatement of WEAVESS's kNN-graph hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY (see wv_oracle.h).  Built with
+ * -O3 -fopenmp -ffp-contract=off so single-precision arithmetic follows the
+ * source order exactly.
+ *
+ * Function                      follows (reference path:line)                         pinned by (tests/)
+ * ----------------------------  ----------------------------------------------------  -----------------------------------------------
+ * wvo_dist_ref                  include/weavess/distance.h:12-36                      oracle/_ref build of distance.h (bit-exact),
+ *                                                                                     closed form on integer data
+ * wvo_dist_canon                build order (D4), reduction tree below                FP64 recompute <=1e-5 rel, integer data exact,
+ *                                                                                     tiny hand-worked vectors
+ * wvo_gen_random                include/weavess/util.h:13-27                          oracle/_ref build of util.h fed the same draws,
+ *                                                                                     invariants (distinct, in range, count)
+ * wvo_init_random               src/component_init.cpp:11-45                          invariants (no self, distinct, sorted, dist)
+ * wvo_nndescent                 src/component_refine.cpp:12-239,                      S>=N-1 gives the exact kNN graph; a 5-point
+ *                               include/weavess/index.h:86-150,                       hand-worked case; invariants; graph recall
+ *                               src/component_prune.cpp:8-18                          vs FP64 exact kNN
+ * wvo_search_greedy             src/component_search_entry.cpp:9-30,                  complete graph / all-node entry set / L>=N give
+ *                               src/component_route.cpp:9-53,                         exact kNN; hand-worked 8-node KAT for the
+ *                               include/weavess/index.h:152-180                       k-pointer restart and duplicate rejection
+ * wvo_exact_knn                 algo/knn/gen_knn.cpp:117-150 (self skipped properly)  numpy argsort brute force on small inputs
+ * wvo_recall                    src/builder.cpp:465-480                               hand-computed examples
+ * wvo_graph_quality             src/builder.cpp:760-776                               hand-computed examples
+ */
+#include "wv_oracle.h"
+
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <cstring>
+#include <mutex>
+#include <vector>
+#include <omp.h>
+
+namespace {
+
+/* ---------------------------------------------------------------------- */
+/* Keys: (dist, id) packed so that unsigned order == (dist, id) order for  */
+/* non-negative finite dist.                                               */
+/* ---------------------------------------------------------------------- */
+inline uint64_t make_key(float d, uint32_t id) {
+    uint32_t b;
+    std::memcpy(&b, &d, 4);
+    return ((uint64_t)b << 32) | id;
+}
+inline uint32_t key_id(uint64_t k) { return (uint32_t)k; }
+inline float key_dist(uint64_t k) {
+    uint32_t b = (uint32_t)(k >> 32);
+    float d;
+    std::memcpy(&d, &b, 4);
+    return d;
+}
+constexpr uint64_t KEY_EMPTY = ~0ull;
+
+/* splitmix64 finaliser (Steele, Lea, Flood 2014). */
+inline uint64_t fmix64(uint64_t z) {
+    z ^= z >> 30;
+    z *= 0xBF58476D1CE4E5B9ull;
+    z ^= z >> 27;
+    z *= 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    return z;
+}
+constexpr uint64_t GOLDEN = 0x9E3779B97F4A7C15ull;
+
+/* murmur3 fmix32. */
+inline uint32_t fmix32(uint32_t h) {
+    h ^= h >> 16;
+    h *= 0x85EBCA6Bu;
+    h ^= h >> 13;
+    h *= 0xC2B2AE35u;
+    h ^= h >> 16;
+    return h;
+}
+
+/* Stream tags (shared with the GPU path by value). */
+constexpr uint64_t TAG_INIT = 0x494E4954ull;    /* "INIT" */
+constexpr uint64_t TAG_ENTRY = 0x454E5452ull;   /* "ENTR" */
+constexpr uint64_t TAG_REVNEW = 0x52564E57ull;  /* "RVNW" */
+constexpr uint64_t TAG_REVOLD = 0x52564F4Cull;  /* "RVOL" */
+
+inline uint64_t stream_of(uint64_t seed, uint64_t tag, uint64_t idx) {
+    uint64_t h = fmix64(seed + GOLDEN);
+    h = fmix64(h ^ (tag * GOLDEN));
+    h = fmix64(h ^ (idx + 0x632BE59BD9B4E019ull));
+    return h;
+}
+inline uint32_t draw(uint64_t stream, uint64_t k) {
+    return (uint32_t)(fmix64(stream + (k + 1) * GOLDEN) >> 32);
+}
+
+/* Slot of candidate `id` in target `t`'s table (GPU: same formula). */
+inline uint32_t cand_slot(uint32_t t, uint32_t id, uint32_t mask) {
+    return fmix32(id * 0x9E3779B1u + t * 0x7F4A7C15u) & mask;
+}
+
+/* Build-order distance (D4).  Chunks of 4 consecutive elements; chunk c
+ * contributes s = ((x0^2 + x1^2) + x2^2) + x3^2 (the grouping of
+ * distance.h:24) to accumulator c % 16; accumulators are then folded
+ * acc[j] += acc[j + h] for h = 8, 4, 2, 1.  A trailing partial chunk is
+ * zero-padded, which adds exact zeros.  A GPU tile of 16/m lanes per row in
+ * which lane j owns accumulators j, j + 16/m, ... reproduces this exactly. */
+inline float dist_canon(const float *a, const float *b, uint32_t d) {
+    float acc[16];
+    for (int j = 0; j < 16; ++j) acc[j] = 0.0f;
+    uint32_t nch = d / 4;
+    for (uint32_t c = 0; c < nch; ++c) {
+        const float *x = a + 4 * c, *y = b + 4 * c;
+        float d0 = x[0] - y[0], d1 = x[1] - y[1], d2 = x[2] - y[2], d3 = x[3] - y[3];
+        float s = d0 * d0 + d1 * d1;
+        s = s + d2 * d2;
+        s = s + d3 * d3;
+        acc[c & 15] = acc[c & 15] + s;
+    }
+    uint32_t rem = d & 3;
+    if (rem) {
+        const float *x = a + 4 * nch, *y = b + 4 * nch;
+        float s = 0.0f;
+        float d0 = x[0] - y[0];
+        s = d0 * d0;
+        if (rem > 1) { float d1 = x[1] - y[1]; s = s + d1 * d1; }
+        if (rem > 2) { float d2 = x[2] - y[2]; s = s + d2 * d2; }
+        acc[nch & 15] = acc[nch & 15] + s;
+    }
+    for (int h = 8; h >= 1; h >>= 1)
+        for (int j = 0; j < h; ++j) acc[j] = acc[j] + acc[j + h];
+    return acc[0];
+}
+
+int nthreads_or_default(int n) { return n > 0 ? n : omp_get_max_threads(); }
+
+/* GenRandom (include/weavess/util.h:13-27): `size` draws reduced mod
+ * (N - size), sorted, forced strictly increasing, then rotated by one more
+ * draw mod N.  Draw i is next(i). */
+template <class Next>
+void gen_random_with(Next next, uint32_t *addr, uint32_t size, uint32_t N) {
+    for (uint32_t i = 0; i < size; ++i) addr[i] = next(i) % (N - size);
+    std::sort(addr, addr + size);
+    for (uint32_t i = 1; i < size; ++i)
+        if (addr[i] <= addr[i - 1]) addr[i] = addr[i - 1] + 1;
+    uint32_t off = next(size) % N;
+    for (uint32_t i = 0; i < size; ++i) addr[i] = (uint32_t)(((uint64_t)addr[i] + off) % N);
+}
+/* ... on the counter stream (D2). */
+void gen_random(uint64_t stream, uint32_t *addr, uint32_t size, uint32_t N) {
+    gen_random_with([stream](uint32_t i) { return draw(stream, i); }, addr, size, N);
+}
+
+}  // namespace
+
+extern "C" {
+
+float wvo_dist_ref(const float *a, const float *b, uint32_t d) {
+    /* distance.h:16-35: groups of four while at least four remain, each
+     * group summed left to right and then added to the running total; the
+     * remaining 0-3 elements are added one at a time. */
+    float result = 0.0f;
+    uint32_t i = 0;
+    for (; i + 4 <= d; i += 4) {
+        float d0 = a[i] - b[i], d1 = a[i + 1] - b[i + 1];
+        float d2 = a[i + 2] - b[i + 2], d3 = a[i + 3] - b[i + 3];
+        float g = d0 * d0 + d1 * d1;
+        g = g + d2 * d2;
+        g = g + d3 * d3;
+        result = result + g;
+    }
+    for (; i < d; ++i) {
+        float t = a[i] - b[i];
+        result = result + t * t;
+    }
+    return result;
+}
+
+float wvo_dist_canon(const float *a, const float *b, uint32_t d) { return dist_canon(a, b, d); }
+
+double wvo_dist_f64(const float *a, const float *b, uint32_t d) {
+    double s = 0.0;
+    for (uint32_t i = 0; i < d; ++i) {
+        double t = (double)a[i] - (double)b[i];
+        s += t * t;
+    }
+    return s;
+}
+
+uint64_t wvo_stream(uint64_t seed, uint64_t tag, uint64_t idx) { return stream_of(seed, tag, idx); }
+uint32_t wvo_draw(uint64_t stream, uint64_t k) { return draw(stream, k); }
+
+int wvo_gen_random(uint64_t stream, uint32_t *addr, uint32_t size, uint32_t N) {
+    if (!addr || size >= N) return -1;
+    gen_random(stream, addr, size, N);
+    return 0;
+}
+
+int wvo_gen_random_draws(const uint32_t *draws, uint32_t *addr, uint32_t size, uint32_t N) {
+    if (!draws || !addr || size >= N) return -1;
+    gen_random_with([draws](uint32_t i) { return draws[i]; }, addr, size, N);
+    return 0;
+}
+
+int wvo_init_random(const float *base, uint32_t N, uint32_t d, uint32_t S, uint64_t seed,
+                    uint32_t *ids, float *dist, uint32_t *cnt, int nthreads) {
+    /* src/component_init.cpp:11-45: per node, GenRandom(S) over [0,N), skip
+     * self, FP32 distances, ascending sort (here by (dist,id), D1). */
+    if (!base || !ids || !dist || !cnt || S == 0 || S >= N) return -1;
+#pragma omp parallel for num_threads(nthreads_or_default(nthreads)) schedule(static)
+    for (int64_t ii = 0; ii < (int64_t)N; ++ii) {
+        uint32_t i = (uint32_t)ii;
+        std::vector<uint32_t> tmp(S);
+        gen_random(stream_of(seed, TAG_INIT, i), tmp.data(), S, N);
+        std::vector<uint64_t> keys;
+        keys.reserve(S);
+        for (uint32_t j = 0; j < S; ++j) {
+            if (tmp[j] == i) continue;
+            float dd = dist_canon(base + (size_t)i * d, base + (size_t)tmp[j] * d, d);
+            keys.push_back(make_key(dd, tmp[j]));
+        }
+        std::sort(keys.begin(), keys.end());
+        cnt[i] = (uint32_t)keys.size();
+        for (uint32_t j = 0; j < S; ++j) {
+            if (j < keys.size()) {
+                ids[(size_t)i * S + j] = key_id(keys[j]);
+                dist[(size_t)i * S + j] = key_dist(keys[j]);
+            } else {
+                ids[(size_t)i * S + j] = 0xFFFFFFFFu;
+                dist[(size_t)i * S + j] = -1.0f;
+            }
+        }
+    }
+    return 0;
+}
+
+/* ---------------------------------------------------------------------- */
+/* NN-Descent                                                              */
+/* ---------------------------------------------------------------------- */
+namespace {
+
+struct Ent {
+    uint64_t key;
+    uint8_t flag;
+};
+struct EntLess {
+    bool operator()(const Ent &a, const Ent &b) const { return a.key < b.key; }
+};
+
+struct NndState {
+    uint32_t N, d, L, S, R;
+    const float *base;
+    std::vector<std::vector<Ent>> pool;      /* max-heap during join, sorted otherwise */
+    std::vector<std::mutex> locks;
+    std::vector<uint32_t> M;
+    std::vector<std::vector<uint32_t>> nn_new, nn_old;
+    /* slot mode */
+    uint32_t C = 0;
+    std::vector<uint64_t> slots;             /* N*C */
+    std::vector<uint64_t> thr;               /* per-node threshold key */
+    std::vector<std::vector<uint32_t>> pool_ids;  /* sorted pool ids at join start */
+    NndState(uint32_t n) : pool(n), locks(n), M(n), nn_new(n), nn_old(n) {}
+};
+
+/* nhood::insert, include/weavess/index.h:120-135, with (dist,id) order (D1):
+ * reject if worse than the current worst; reject a duplicate id; push while
+ * below capacity L, else replace the worst. */
+bool insert_exact(NndState &st, uint32_t t, uint32_t id, float dd) {
+    uint64_t key = make_key(dd, id);
+    std::lock_guard<std::mutex> g(st.locks[t]);
+    auto &p = st.pool[t];
+    if (!p.empty() && key > p.front().key) return false;
+    for (const Ent &e : p)
+        if (key_id(e.key) == id) return false;
+    if (p.size() < st.L) {
+        p.push_back(Ent{key, 1});
+        std::push_heap(p.begin(), p.end(), EntLess());
+    } else {
+        std::pop_heap(p.begin(), p.end(), EntLess());
+        p.back() = Ent{key, 1};
+        std::push_heap(p.begin(), p.end(), EntLess());
+    }
+    return true;
+}
+
+/* Lossy slot-table insertion mirrored by the GPU: keep the smaller key per
+ * slot; the threshold is the target's worst key at the start of the join. */
+bool insert_slot(NndState &st, uint32_t t, uint32_t id, float dd) {
+    uint64_t key = make_key(dd, id);
+    if (key > st.thr[t]) return false;
+    /* an id already in the target's pool is rejected before it can take a
+     * slot (the duplicate test of index.h:123-125, done up front) */
+    const auto &ids = st.pool_ids[t];
+    if (std::binary_search(ids.begin(), ids.end(), id)) return false;
+    uint64_t *slot = &st.slots[(size_t)t * st.C + cand_slot(t, id, st.C - 1)];
+    uint64_t cur = __atomic_load_n(slot, __ATOMIC_RELAXED);
+    while (key < cur && !__atomic_compare_exchange_n(slot, &cur, key, true, __ATOMIC_RELAXED,
+                                                      __ATOMIC_RELAXED)) {
+    }
+    return true;
+}
+
+/* join(), src/component_refine.cpp:127-143 with nhood::join,
+ * include/weavess/index.h:137-149. */
+void nnd_join(NndState &st, int nth, uint64_t &pairs, uint64_t &accepted) {
+    const bool slots = st.C > 0;
+    if (slots) {
+#pragma omp parallel for num_threads(nth) schedule(static)
+        for (int64_t t = 0; t < (int64_t)st.N; ++t) {
+            const auto &p = st.pool[t];
+            st.thr[t] = p.empty() ? KEY_EMPTY : p.back().key;  /* pools are sorted here */
+            auto &ids = st.pool_ids[t];
+            ids.clear();
+            for (const Ent &e : p) ids.push_back(key_id(e.key));
+            std::sort(ids.begin(), ids.end());
+        }
+    } else {
+#pragma omp parallel for num_threads(nth) schedule(static)
+        for (int64_t t = 0; t < (int64_t)st.N; ++t)
+            std::make_heap(st.pool[t].begin(), st.pool[t].end(), EntLess());
+    }
+    uint64_t pc = 0, ac = 0;
+#pragma omp parallel for num_threads(nth) schedule(dynamic, 100) reduction(+ : pc, ac)
+    for (int64_t nn = 0; nn < (int64_t)st.N; ++nn) {
+        const auto &nw = st.nn_new[nn];
+        const auto &od = st.nn_old[nn];
+        auto cb = [&](uint32_t i, uint32_t j) {
+            if (i == j) return;
+            float dd = dist_canon(st.base + (size_t)i * st.d, st.base + (size_t)j * st.d, st.d);
+            ++pc;
+            if (slots) {
+                ac += insert_slot(st, i, j, dd);
+                ac += insert_slot(st, j, i, dd);
+            } else {
+                ac += insert_exact(st, i, j, dd);
+                ac += insert_exact(st, j, i, dd);
+            }
+        };
+        for (uint32_t i : nw) {
+            for (uint32_t j : nw)
+                if (i < j) cb(i, j);
+            for (uint32_t j : od) cb(i, j);
+        }
+    }
+    pairs = pc;
+    accepted = ac;
+    if (slots) {
+        /* merge: union of the pool and the slot winners, duplicates by id
+         * dropped (the pool entry wins), ascending, first L kept; winners are
+         * flagged new.  Slots are reset for the next join. */
+#pragma omp parallel for num_threads(nth) schedule(dynamic, 256)
+        for (int64_t tt = 0; tt < (int64_t)st.N; ++tt) {
+            auto &p = st.pool[tt];
+            uint64_t *sl = &st.slots[(size_t)tt * st.C];
+            std::vector<Ent> u(p.begin(), p.end());
+            for (uint32_t s = 0; s < st.C; ++s) {
+                uint64_t k = sl[s];
+                sl[s] = KEY_EMPTY;
+                if (k == KEY_EMPTY) continue;
+                bool dup = false;
+                for (const Ent &e : p)
+                    if (key_id(e.key) == key_id(k)) { dup = true; break; }
+                if (!dup) u.push_back(Ent{k, 1});
+            }
+            std::sort(u.begin(), u.end(), EntLess());
+            if (u.size() > st.L) u.resize(st.L);
+            p.swap(u);
+        }
+    }
+}
+
+/* Reverse-sample priority key of proposer n at node x: a 32-bit hash in the
+ * high word, n in the low word; the R smallest keys are kept (D3). */
+uint64_t rev_key(uint64_t seed, uint64_t tag, uint32_t iter, uint32_t x, uint32_t n) {
+    uint64_t h = fmix64(stream_of(seed, tag, ((uint64_t)iter << 32) | x) ^ ((uint64_t)n * GOLDEN));
+    return ((h >> 32) << 32) | n;
+}
+
+/* update(), src/component_refine.cpp:145-239, two-phase (all pools sorted
+ * before any reverse test) and with the reverse cap of D3. */
+void nnd_update(NndState &st, int nth, uint32_t iter, uint64_t seed) {
+    const uint32_t N = st.N, S = st.S, R = st.R;
+    std::vector<float> worst(N);
+    /* :157-175 sort, truncate to L, new M */
+#pragma omp parallel for num_threads(nth) schedule(static)
+    for (int64_t n = 0; n < (int64_t)N; ++n) {
+        auto &p = st.pool[n];
+        std::sort(p.begin(), p.end(), EntLess());
+        if (p.size() > st.L) p.resize(st.L);
+        uint32_t maxl = std::min(st.M[n] + S, (uint32_t)p.size());
+        uint32_t c = 0, l = 0;
+        while (l < maxl && c < S) {
+            if (p[l].flag) ++c;
+            ++l;
+        }
+        st.M[n] = l;
+        worst[n] = p.empty() ? -1.0f : key_dist(p.back().key);
+        st.nn_new[n].clear();
+        st.nn_old[n].clear();
+    }
+    /* :176-209 forward samples and reverse proposals */
+    std::vector<std::atomic<uint32_t>> cnt_new(N), cnt_old(N);
+    for (uint32_t n = 0; n < N; ++n) { cnt_new[n] = 0; cnt_old[n] = 0; }
+#pragma omp parallel for num_threads(nth) schedule(static)
+    for (int64_t n = 0; n < (int64_t)N; ++n) {
+        auto &p = st.pool[n];
+        for (uint32_t l = 0; l < st.M[n]; ++l) {
+            uint32_t id = key_id(p[l].key);
+            bool rev = key_dist(p[l].key) > worst[id];
+            if (p[l].flag) {
+                st.nn_new[n].push_back(id);
+                if (rev) cnt_new[id].fetch_add(1, std::memory_order_relaxed);
+            } else {
+                st.nn_old[n].push_back(id);
+                if (rev) cnt_old[id].fetch_add(1, std::memory_order_relaxed);
+            }
+        }
+    }
+    std::vector<uint64_t> off_new(N + 1, 0), off_old(N + 1, 0);
+    for (uint32_t n = 0; n < N; ++n) {
+        off_new[n + 1] = off_new[n] + cnt_new[n].load();
+        off_old[n + 1] = off_old[n] + cnt_old[n].load();
+        cnt_new[n] = 0;
+        cnt_old[n] = 0;
+    }
+    std::vector<uint32_t> rv_new(off_new[N]), rv_old(off_old[N]);
+#pragma omp parallel for num_threads(nth) schedule(static)
+    for (int64_t n = 0; n < (int64_t)N; ++n) {
+        auto &p = st.pool[n];
+        for (uint32_t l = 0; l < st.M[n]; ++l) {
+            uint32_t id = key_id(p[l].key);
+            if (!(key_dist(p[l].key) > worst[id])) continue;
+            if (p[l].flag)
+                rv_new[off_new[id] + cnt_new[id].fetch_add(1, std::memory_order_relaxed)] = (uint32_t)n;
+            else
+                rv_old[off_old[id] + cnt_old[id].fetch_add(1, std::memory_order_relaxed)] = (uint32_t)n;
+        }
+        for (uint32_t l = 0; l < st.M[n]; ++l) p[l].flag = 0;  /* :189 nn.flag = false */
+    }
+    /* :213-238 append the (capped) reverse samples; cap old at 2R */
+#pragma omp parallel for num_threads(nth) schedule(dynamic, 256)
+    for (int64_t x = 0; x < (int64_t)N; ++x) {
+        for (int which = 0; which < 2; ++which) {
+            uint32_t *b = which == 0 ? &rv_new[off_new[x]] : &rv_old[off_old[x]];
+            uint64_t c = which == 0 ? off_new[x + 1] - off_new[x] : off_old[x + 1] - off_old[x];
+            std::vector<uint32_t> sel(b, b + c);
+            if (c > R) {
+                uint64_t tag = which == 0 ? TAG_REVNEW : TAG_REVOLD;
+                std::vector<uint64_t> pr(c);
+                for (uint64_t q = 0; q < c; ++q) pr[q] = rev_key(seed, tag, iter, (uint32_t)x, b[q]);
+                std::nth_element(pr.begin(), pr.begin() + R, pr.end());
+                sel.resize(R);
+                for (uint32_t q = 0; q < R; ++q) sel[q] = (uint32_t)pr[q];
+            }
+            std::sort(sel.begin(), sel.end());
+            auto &dst = which == 0 ? st.nn_new[x] : st.nn_old[x];
+            dst.insert(dst.end(), sel.begin(), sel.end());
+        }
+        if (st.nn_old[x].size() > 2 * (size_t)R) st.nn_old[x].resize(2 * (size_t)R);
+    }
+}
+
+}  // namespace
+
+int wvo_nndescent(const float *base, uint32_t N, uint32_t d, const wvo_nnd_params *p,
+                  const uint32_t *init_ids, const float *init_dist, const uint32_t *init_cnt,
+                  uint32_t *out_ids, float *out_dist, uint32_t *out_deg,
+                  wvo_nnd_iter_stats *iter_stats, int nthreads) {
+    if (!base || !p || !out_ids || !out_dist || !out_deg) return -1;
+    if (N < 2 || d == 0 || p->S == 0 || p->S >= N || p->L == 0 || p->K == 0 || p->K > p->L) return -1;
+    if (p->S > p->L) return -1;
+    if (p->cand_slots && (p->cand_slots & (p->cand_slots - 1))) return -1;
+    const int nth = nthreads_or_default(nthreads);
+    const uint32_t S = p->S;
+
+    std::vector<uint32_t> ids0, cnt0;
+    std::vector<float> dist0;
+    if (!init_ids) {
+        ids0.resize((size_t)N * S);
+        dist0.resize((size_t)N * S);
+        cnt0.resize(N);
+        wvo_init_random(base, N, d, S, p->seed, ids0.data(), dist0.data(), cnt0.data(), nth);
+        init_ids = ids0.data();
+        init_dist = dist0.data();
+        init_cnt = cnt0.data();
+    }
+
+    NndState st(N);
+    st.N = N; st.d = d; st.L = p->L; st.S = S; st.R = p->R; st.base = base;
+    st.C = p->cand_slots;
+    if (st.C) {
+        st.slots.assign((size_t)N * st.C, KEY_EMPTY);
+        st.thr.assign(N, KEY_EMPTY);
+        st.pool_ids.resize(N);
+    }
+    /* init(), src/component_refine.cpp:87-104: pools seeded from the initial
+     * graph, all flagged new, M = S. */
+    for (uint32_t i = 0; i < N; ++i) {
+        auto &pl = st.pool[i];
+        pl.reserve(p->L);
+        uint32_t c = init_cnt ? init_cnt[i] : S;
+        for (uint32_t j = 0; j < c && j < p->L; ++j)
+            pl.push_back(Ent{make_key(init_dist[(size_t)i * S + j], init_ids[(size_t)i * S + j]), 1});
+        std::sort(pl.begin(), pl.end(), EntLess());
+        st.M[i] = S;
+    }
+    /* NNDescent(), :106-125.  Iteration 0's join sees nn_new = 2S copies of
+     * id 0 and no nn_old (index.h:98-103), so it evaluates nothing. */
+    for (uint32_t it = 0; it < p->iters; ++it) {
+        uint64_t pairs = 0, acc = 0;
+        if (it > 0) nnd_join(st, nth, pairs, acc);
+        nnd_update(st, nth, it, p->seed);
+        if (iter_stats) {
+            iter_stats[it].pairs = pairs;
+            iter_stats[it].inserts = acc;
+        }
+    }
+    /* RefineInner tail :22-74 + PruneNaive: sort, keep first K. */
+#pragma omp parallel for num_threads(nth) schedule(static)
+    for (int64_t i = 0; i < (int64_t)N; ++i) {
+        auto &pl = st.pool[i];
+        std::sort(pl.begin(), pl.end(), EntLess());
+        uint32_t deg = std::min<uint32_t>(p->K, (uint32_t)pl.size());
+        out_deg[i] = deg;
+        for (uint32_t j = 0; j < p->K; ++j) {
+            if (j < deg) {
+                out_ids[(size_t)i * p->K + j] = key_id(pl[j].key);
+                out_dist[(size_t)i * p->K + j] = key_dist(pl[j].key);
+            } else {
+                out_ids[(size_t)i * p->K + j] = 0xFFFFFFFFu;
+                out_dist[(size_t)i * p->K + j] = -1.0f;
+            }
+        }
+    }
+    return 0;
+}
+
+/* ---------------------------------------------------------------------- */
+/* Greedy search                                                           */
+/* ---------------------------------------------------------------------- */
+int wvo_search_greedy(const uint32_t *graph, const uint32_t *deg, uint32_t deg_max,
+                      const float *base, uint32_t N, uint32_t d, const float *queries, uint32_t Q,
+                      const uint32_t *entries, uint64_t seed, uint32_t L, uint32_t k,
+                      const uint32_t *L_per_q, const uint32_t *k_per_q, uint32_t k_stride,
+                      uint32_t *out_ids, float *out_dist, uint32_t *evals, uint32_t *hops,
+                      int nthreads) {
+    if (!graph || !deg || !base || !queries || !out_ids || N == 0 || d == 0) return -1;
+    /* validate all queries first: L >= k (src/builder.cpp:435-438), L < N
+     * for random entries (GenRandom needs N - L > 0), L <= N otherwise */
+    for (uint32_t q = 0; q < Q; ++q) {
+        uint32_t Lq = L_per_q ? L_per_q[q] : L, kq = k_per_q ? k_per_q[q] : k;
+        if (Lq == 0 || kq == 0 || kq > Lq || kq > k_stride) return -1;
+        if (entries ? Lq > N : Lq >= N) return -1;
+        if (entries && L_per_q) return -1;  /* explicit entries need a fixed L */
+    }
+    const int nth = nthreads_or_default(nthreads);
+#pragma omp parallel num_threads(nth)
+    {
+        std::vector<uint32_t> stamp(N, 0);
+        uint32_t epoch = 0;
+        std::vector<uint64_t> key;
+        std::vector<uint8_t> flag;
+        std::vector<uint32_t> ent;
+#pragma omp for schedule(dynamic, 16)
+        for (int64_t qq = 0; qq < (int64_t)Q; ++qq) {
+            const uint32_t q = (uint32_t)qq;
+            const uint32_t Lq = L_per_q ? L_per_q[q] : L, kq = k_per_q ? k_per_q[q] : k;
+            const float *qv = queries + (size_t)q * d;
+            if (++epoch == 0) { std::fill(stamp.begin(), stamp.end(), 0); epoch = 1; }
+            key.assign(Lq + 1, KEY_EMPTY);
+            flag.assign(Lq + 1, 0);
+            ent.resize(Lq);
+            uint32_t ev = 0, hp = 0;
+            /* SearchEntryRand: L random ids, distances, sort; entries are not
+             * marked visited (component_search_entry.cpp:18-27). */
+            if (entries) std::memcpy(ent.data(), entries + (size_t)q * Lq, Lq * 4);
+            else gen_random(stream_of(seed, TAG_ENTRY, q), ent.data(), Lq, N);
+            for (uint32_t i = 0; i < Lq; ++i) {
+                key[i] = make_key(dist_canon(qv, base + (size_t)ent[i] * d, d), ent[i]);
+                flag[i] = 1;
+                ++ev;
+            }
+            std::sort(key.begin(), key.begin() + Lq);
+            /* RouteGreedy, component_route.cpp:16-47 */
+            uint32_t kk = 0;
+            while (kk < Lq) {
+                uint32_t nk = Lq;
+                if (flag[kk]) {
+                    flag[kk] = 0;
+                    ++hp;
+                    uint32_t u = key_id(key[kk]);
+                    uint32_t du = deg[u] < deg_max ? deg[u] : deg_max;
+                    for (uint32_t m = 0; m < du; ++m) {
+                        uint32_t id = graph[(size_t)u * deg_max + m];
+                        if (id >= N) continue;
+                        if (stamp[id] == epoch) continue;
+                        stamp[id] = epoch;
+                        float dd = dist_canon(qv, base + (size_t)id * d, d);
+                        ++ev;
+                        uint64_t nkey = make_key(dd, id);
+                        if (nkey >= key[Lq - 1]) continue;
+                        /* InsertIntoPool, index.h:152-180, on (dist,id):
+                         * position = number of smaller keys; an equal key is
+                         * the same id -> duplicate (returns L+1). */
+                        uint32_t pos = (uint32_t)(std::lower_bound(key.begin(), key.begin() + Lq, nkey) - key.begin());
+                        uint32_t r;
+                        if (key[pos] == nkey) {
+                            r = Lq + 1;
+                        } else {
+                            std::memmove(&key[pos + 1], &key[pos], (Lq - 1 - pos) * sizeof(uint64_t));
+                            std::memmove(&flag[pos + 1], &flag[pos], (Lq - 1 - pos));
+                            key[pos] = nkey;
+                            flag[pos] = 1;
+                            r = pos;
+                        }
+                        if (r < nk) nk = r;
+                    }
+                }
+                kk = (nk <= kk) ? nk : kk + 1;
+            }
+            for (uint32_t i = 0; i < k_stride; ++i) {
+                out_ids[(size_t)q * k_stride + i] = i < kq ? key_id(key[i]) : 0xFFFFFFFFu;
+                if (out_dist) out_dist[(size_t)q * k_stride + i] = i < kq ? key_dist(key[i]) : -1.0f;
+            }
+            if (evals) evals[q] = ev;
+            if (hops) hops[q] = hp;
+        }
+    }
+    return 0;
+}
+
+/* ---------------------------------------------------------------------- */
+/* Exact kNN (FP64)                                                        */
+/* ---------------------------------------------------------------------- */
+int wvo_exact_knn(const float *base, uint32_t N, const float *queries, uint32_t Q, uint32_t d,
+                  uint32_t k, int64_t self_offset, uint32_t *out_ids, double *out_dist,
+                  int nthreads) {
+    if (!base || !queries || !out_ids || k == 0 || k > N) return -1;
+    const int nth = nthreads_or_default(nthreads);
+#pragma omp parallel num_threads(nth)
+    {
+        std::vector<std::pair<double, uint32_t>> h;
+#pragma omp for schedule(dynamic, 4)
+        for (int64_t qq = 0; qq < (int64_t)Q; ++qq) {
+            const float *qv = queries + (size_t)qq * d;
+            int64_t self = self_offset >= 0 ? qq + self_offset : -1;
+            h.clear();
+            for (uint32_t j = 0; j < N; ++j) {
+                if ((int64_t)j == self) continue;
+                std::pair<double, uint32_t> e(wvo_dist_f64(qv, base + (size_t)j * d, d), j);
+                if (h.size() < k) {
+                    h.push_back(e);
+                    std::push_heap(h.begin(), h.end());
+                } else if (e < h.front()) {
+                    std::pop_heap(h.begin(), h.end());
+                    h.back() = e;
+                    std::push_heap(h.begin(), h.end());
+                }
+            }
+            std::sort_heap(h.begin(), h.end());
+            for (uint32_t i = 0; i < k; ++i) {
+                out_ids[(size_t)qq * k + i] = i < h.size() ? h[i].second : 0xFFFFFFFFu;
+                if (out_dist) out_dist[(size_t)qq * k + i] = i < h.size() ? h[i].first : -1.0;
+            }
+        }
+    }
+    return 0;
+}
+
+double wvo_recall(const uint32_t *res, uint32_t Q, uint32_t k, uint32_t res_stride,
+                  const uint32_t *gt, uint32_t gt_dim) {
+    /* src/builder.cpp:465-480: count result ids that are not among the first
+     * k ground-truth ids; accuracy = 1 - misses / (Q * k). */
+    if (Q == 0 || k == 0 || k > gt_dim) return 0.0;
+    uint64_t miss = 0;
+    for (uint32_t i = 0; i < Q; ++i)
+        for (uint32_t j = 0; j < k; ++j) {
+            uint32_t r = res[(size_t)i * res_stride + j];
+            bool hit = false;
+            for (uint32_t t = 0; t < k; ++t)
+                if (gt[(size_t)i * gt_dim + t] == r) { hit = true; break; }
+            miss += !hit;
+        }
+    return 1.0 - (double)miss / ((double)Q * k);
+}
+
+double wvo_graph_quality(const uint32_t *ids, const uint32_t *deg, uint32_t deg_stride,
+                         const uint32_t *rows, uint32_t nrows, const uint32_t *gt, uint32_t gt_dim,
+                         uint32_t control) {
+    /* src/builder.cpp:760-776: for each node, the fraction of its first
+     * `control` exact neighbours present anywhere in its adjacency list,
+     * normalised by min(control, deg, |exact|); averaged over nodes.  gt row
+     * r belongs to node rows[r] (rows NULL: r itself). */
+    if (nrows == 0) return 0.0;
+    double mean = 0.0;
+    for (uint32_t r = 0; r < nrows; ++r) {
+        uint32_t node = rows ? rows[r] : r;
+        const uint32_t *g = ids + (size_t)node * deg_stride;
+        uint32_t dg = deg[node];
+        const uint32_t *v = gt + (size_t)r * gt_dim;
+        uint32_t q = 0;
+        for (uint32_t j = 0; j < control && j < gt_dim; ++j)
+            for (uint32_t t = 0; t < dg; ++t)
+                if (g[t] == v[j]) { ++q; break; }
+        uint32_t den = std::min(std::min(control, dg), gt_dim);
+        if (den) mean += (double)q / den;
+    }
+    return mean / nrows;
+}
+
+}  // extern "C"

@@ -1,0 +1,117 @@
+/*
+ * wv_oracle.h -- CPU restatement of WEAVESS's kNN-graph hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker for the HIP path in
+ * weavess_amd/: only tests/, __graft_entry__.smoke() and bench.py's
+ * cpu_baseline leg may load it.  Nothing in the product links or calls it.
+ *
+ * Every function follows a reference routine, cited as path:line relative to
+ * the WEAVESS source tree.  Documented deviations (shared with the GPU path):
+ *   D1  ties are broken by id: every ordering is on (dist, id), never dist only
+ *       (reference orders by dist only, include/weavess/index.h:74-80).
+ *   D2  random numbers come from a counter-based generator (splitmix64
+ *       finaliser on seed/stream/counter) instead of glibc rand()/mt19937
+ *       (src/component_init.cpp:18, src/component_search_entry.cpp:15), so
+ *       that a GPU wavefront can reproduce any draw without sequential state.
+ *   D3  reverse samples above the cap R are chosen as the R smallest hash
+ *       priorities instead of rand()%R overwrites in arrival order
+ *       (src/component_refine.cpp:190-208), which makes the choice
+ *       independent of thread order.
+ *   D4  "build order" distance: see wvo_dist_canon.
+ *
+ * Parity status of every exported function is listed in the table at the top
+ * of wv_oracle.cpp.
+ */
+#ifndef WV_ORACLE_H
+#define WV_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Distances ------------------------------------------------------------ */
+/* Reference order: include/weavess/distance.h:12-36. */
+float wvo_dist_ref(const float *a, const float *b, uint32_t d);
+/* Build order shared with the GPU kernels. */
+float wvo_dist_canon(const float *a, const float *b, uint32_t d);
+/* FP64 squared L2 (exact-kNN oracle). */
+double wvo_dist_f64(const float *a, const float *b, uint32_t d);
+
+/* Random numbers -------------------------------------------------------- */
+uint64_t wvo_stream(uint64_t seed, uint64_t tag, uint64_t idx);
+uint32_t wvo_draw(uint64_t stream, uint64_t k);
+/* GenRandom algorithm (include/weavess/util.h:13-27) on a counter stream. */
+int wvo_gen_random(uint64_t stream, uint32_t *addr, uint32_t size, uint32_t N);
+/* Same algorithm fed explicit draws (size+1 of them), for pinning against
+ * the reference's own GenRandom driven by std::mt19937. */
+int wvo_gen_random_draws(const uint32_t *draws, uint32_t *addr, uint32_t size, uint32_t N);
+
+/* NN-Descent ------------------------------------------------------------ */
+typedef struct {
+    uint32_t K, L, S, R, iters;
+    uint32_t cand_slots;   /* 0: exact bounded insertion (reference);      */
+                           /* >0: power-of-two lossy slot table per node   */
+    uint64_t seed;
+} wvo_nnd_params;
+
+typedef struct {
+    uint64_t pairs;        /* distance evaluations in the joins            */
+    uint64_t inserts;      /* accepted list updates (new entries kept)     */
+} wvo_nnd_iter_stats;
+
+/* Random S-NN initialisation (src/component_init.cpp:11-45).
+ * ids/dist are N*S row-major, cnt[i] = S or S-1 (self skipped). */
+int wvo_init_random(const float *base, uint32_t N, uint32_t d, uint32_t S,
+                    uint64_t seed, uint32_t *ids, float *dist, uint32_t *cnt,
+                    int nthreads);
+
+/* InitRandom + RefineNNDescent + PruneNaive.
+ * init_ids/init_dist/init_cnt: optional N*S initial graph (NULL: run
+ * wvo_init_random with p->seed).  Outputs N*K ids/dist and deg[N].
+ * iter_stats (optional) has p->iters entries. */
+int wvo_nndescent(const float *base, uint32_t N, uint32_t d,
+                  const wvo_nnd_params *p,
+                  const uint32_t *init_ids, const float *init_dist,
+                  const uint32_t *init_cnt,
+                  uint32_t *out_ids, float *out_dist, uint32_t *out_deg,
+                  wvo_nnd_iter_stats *iter_stats, int nthreads);
+
+/* Greedy search --------------------------------------------------------- */
+/* SearchEntryRand + RouteGreedy (src/component_search_entry.cpp:9-30,
+ * src/component_route.cpp:9-53).  graph is N*deg_max ids, deg[N] valid
+ * counts.  entries: Q*L ids or NULL (generated from seed).  L_per_q/k_per_q
+ * may be NULL (then L/k are used for every query).  out_ids/out_dist are
+ * Q*k_stride.  evals/hops may be NULL. */
+int wvo_search_greedy(const uint32_t *graph, const uint32_t *deg,
+                      uint32_t deg_max, const float *base, uint32_t N,
+                      uint32_t d, const float *queries, uint32_t Q,
+                      const uint32_t *entries, uint64_t seed,
+                      uint32_t L, uint32_t k,
+                      const uint32_t *L_per_q, const uint32_t *k_per_q,
+                      uint32_t k_stride,
+                      uint32_t *out_ids, float *out_dist,
+                      uint32_t *evals, uint32_t *hops, int nthreads);
+
+/* Exact kNN ------------------------------------------------------------- */
+/* FP64 brute force, ties by id.  self_offset >= 0: query q is base row
+ * q+self_offset and is excluded (kNN graph); -1: no exclusion. */
+int wvo_exact_knn(const float *base, uint32_t N, const float *queries,
+                  uint32_t Q, uint32_t d, uint32_t k, int64_t self_offset,
+                  uint32_t *out_ids, double *out_dist, int nthreads);
+
+/* Metrics --------------------------------------------------------------- */
+/* recall@k, src/builder.cpp:465-480. */
+double wvo_recall(const uint32_t *res, uint32_t Q, uint32_t k,
+                  uint32_t res_stride, const uint32_t *gt, uint32_t gt_dim);
+/* graph quality, src/builder.cpp:760-776 (self already excluded in gt). */
+double wvo_graph_quality(const uint32_t *ids, const uint32_t *deg,
+                         uint32_t deg_stride, const uint32_t *rows,
+                         uint32_t nrows, const uint32_t *gt, uint32_t gt_dim,
+                         uint32_t control);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,101 @@
+"""Exact kNN (FP64) and NN-Descent on the GPU vs the CPU oracle."""
+import numpy as np
+import pytest
+
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+# ------------------------------------------------------------------ exact kNN
+def test_exact_knn_bit_exact(ctx, small_float):
+    base, q = small_float
+    ids, dist, ms = ctx.exact_knn(base, q, 50)
+    rid, rdist = O.exact_knn(base, q, 50)
+    assert np.array_equal(ids, rid) and np.array_equal(dist, rdist)
+
+
+def test_exact_knn_graph_self_excluded(ctx, small_int):
+    base, _ = small_int
+    ids, dist, _ = ctx.exact_knn(base, base[:300], 20, self_offset=0)
+    rid, rdist = O.exact_knn(base, base[:300], 20, self_offset=0)
+    assert np.array_equal(ids, rid) and np.array_equal(dist, rdist)
+    assert not np.any(ids == np.arange(300)[:, None])
+
+
+def test_exact_knn_d960(ctx):
+    rng = np.random.default_rng(1)
+    base = np.abs(rng.normal(size=(700, 960))).astype(np.float32)
+    q = np.abs(rng.normal(size=(9, 960))).astype(np.float32)
+    ids, dist, _ = ctx.exact_knn(base, q, 100)
+    rid, rdist = O.exact_knn(base, q, 100)
+    assert np.array_equal(ids, rid) and np.array_equal(dist, rdist)
+
+
+# ------------------------------------------------------------------ NN-Descent
+def _invariants(base, ids, dist, deg, K):
+    N = len(base)
+    assert np.all(deg <= K)
+    for i in range(0, N, max(1, N // 97)):
+        row = ids[i, :deg[i]].tolist()
+        assert i not in row and len(set(row)) == len(row)
+        keys = list(zip(dist[i, :deg[i]].tolist(), row))
+        assert keys == sorted(keys)
+        for j in range(0, deg[i], 7):
+            assert dist[i, j] == O.dist_canon(base[i], base[row[j]])
+
+
+@pytest.mark.parametrize("slots", [64, 256])
+def test_nndescent_bit_exact_on_integer_data(ctx, small_int, slots):
+    """On integer data every FP32 distance (MFMA Gram + norms included) is an
+    exact integer, so the GPU build must equal the oracle's slot-table build
+    (same seeds, same candidate-slot hash) bit for bit."""
+    base, _ = small_int
+    K, L, S, R, it = 24, 32, 10, 32, 6
+    gi, gd, gdeg, st = ctx.nndescent_build(base, K, L, S, R, it, seed=3, cand_slots=slots)
+    ri, rd, rdeg, rst = O.nndescent(base, K, L, S, R, it, seed=3, cand_slots=slots, with_stats=True)
+    assert st["pairs"] == sum(p for p, _ in rst)
+    assert st["proposals"] == sum(a for _, a in rst)
+    assert np.array_equal(gdeg, rdeg)
+    assert np.array_equal(gi, ri)
+    assert np.array_equal(gd, rd)
+
+
+def test_nndescent_full_init_is_exact(ctx, small_float):
+    base, _ = small_float
+    base = base[:60]
+    ids, dist, deg, _ = ctx.nndescent_build(base, 8, 64, 59, 16, 3, seed=1)
+    gt, _ = O.exact_knn(base, base, 8, self_offset=0)
+    assert np.array_equal(ids, gt)
+
+
+@pytest.mark.parametrize("dtype", ["f32", "bf16", "fp8"])
+def test_nndescent_quality_and_invariants(ctx, small_float, dtype):
+    base, _ = small_float
+    K, L, S, R, it = 20, 32, 10, 32, 8
+    ids, dist, deg, st = ctx.nndescent_build(base, K, L, S, R, it, seed=5, dtype=dtype)
+    _invariants(base, ids, dist, deg, K)
+    gt, _ = O.exact_knn(base, base, 10, self_offset=0)
+    q_gpu = O.graph_quality(ids, deg, gt)
+    ri, rd, rdeg = O.nndescent(base, K, L, S, R, it, seed=5)
+    q_ref = O.graph_quality(ri, rdeg, gt)
+    print(dtype, "graph recall gpu", q_gpu, "oracle", q_ref, st)
+    tol = 0.005 if dtype == "f32" else 0.02
+    assert q_gpu >= q_ref - tol
+
+
+def test_nndescent_deterministic(ctx, small_float):
+    base, _ = small_float
+    a = ctx.nndescent_build(base, 16, 24, 8, 24, 5, seed=2)
+    b = ctx.nndescent_build(base, 16, 24, 8, 24, 5, seed=2)
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+
+
+def test_nndescent_rejects_bad_params(ctx, small_float):
+    from weavess_amd import WvError
+    base, _ = small_float
+    for bad in [dict(K=40, L=32), dict(S=0), dict(S=len(base))]:
+        p = dict(K=16, L=32, S=8, R=16)
+        p.update(bad)
+        with pytest.raises(WvError):
+            ctx.nndescent_build(base, p["K"], p["L"], p["S"], p["R"], 3)

@@ -1,0 +1,132 @@
+"""Greedy search on the GPU vs the CPU oracle (parity: bit-exact ids).
+
+The oracle and the kernel share the (dist, id) order and the build-order
+FP32 distance, so ids AND distances must match exactly; evaluation and hop
+counts must match whenever the visited table does not overflow.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from tests.conftest import chain_points
+from tests.test_oracle import _chain_graph, kat_cases
+
+pytestmark = pytest.mark.gpu
+
+
+def _gpu_search(ctx, graph, deg, base, q, L, k, **kw):
+    ids, dist, ev, hp, st = ctx.search_greedy(graph, deg, base, q, L, k, **kw)
+    return ids, dist, ev, hp
+
+
+@pytest.mark.parametrize("L", [10, 20, 64, 100, 200])
+def test_search_parity_integer(ctx, small_int, small_int_graph, L):
+    base, q = small_int
+    g, _, deg = small_int_graph
+    ref = O.search_greedy(g, deg, base, q, L, 10, seed=21)
+    got = _gpu_search(ctx, g, deg, base, q, L, 10, seed=21)
+    print("L", L, "evals", got[2].mean())
+    assert np.array_equal(got[0], ref[0])
+    assert np.array_equal(got[1], ref[1])
+    assert np.array_equal(got[2], ref[2]) and np.array_equal(got[3], ref[3])
+
+
+@pytest.mark.parametrize("L", [16, 48, 130])
+def test_search_parity_float(ctx, small_float, small_float_graph, L):
+    base, q = small_float
+    g, _, deg = small_float_graph
+    ref = O.search_greedy(g, deg, base, q, L, L, seed=5)
+    got = _gpu_search(ctx, g, deg, base, q, L, L, seed=5)
+    assert np.array_equal(got[0], ref[0])
+    # same build-order arithmetic: distances agree bit for bit (<= 1e-5 rel is the stated bar)
+    np.testing.assert_allclose(got[1], ref[1], rtol=1e-5, atol=0)
+    assert np.array_equal(got[1], ref[1])
+    assert np.array_equal(got[2], ref[2]) and np.array_equal(got[3], ref[3])
+
+
+@pytest.mark.parametrize("case", kat_cases())
+def test_search_kat_gpu(ctx, case):
+    base = chain_points()
+    g, deg = _chain_graph(8, case["order"])
+    q = np.zeros((1, 4), np.float32)
+    q[0, 0] = 10
+    ent = np.array([case["entries"]], np.uint32)
+    ids, dist, ev, hp = _gpu_search(ctx, g, deg, base, q, case["L"], case["L"], entries=ent)
+    assert ids[0].tolist() == case["ids"] and dist[0].tolist() == case["dist"]
+    assert ev[0] == case["evals"] and hp[0] == case["hops"]
+
+
+def test_search_explicit_entries_and_strides(ctx, small_float, small_float_graph):
+    base, q = small_float
+    g, _, deg = small_float_graph
+    rng = np.random.default_rng(3)
+    ent = np.stack([rng.choice(len(base), 40, replace=False) for _ in range(len(q))]).astype(np.uint32)
+    ref = O.search_greedy(g, deg, base, q, 40, 10, entries=ent, k_stride=12)
+    got = _gpu_search(ctx, g, deg, base, q, 40, 10, entries=ent, k_stride=12)
+    assert np.array_equal(got[0], ref[0]) and np.all(got[0][:, 10:] == 0xFFFFFFFF)
+
+
+def test_search_per_query_budgets(ctx, small_float, small_float_graph):
+    base, q = small_float
+    g, _, deg = small_float_graph
+    rng = np.random.default_rng(4)
+    Lq = rng.integers(16, 300, len(q)).astype(np.uint32)
+    kq = rng.choice([1, 10, 100], len(q)).astype(np.uint32)
+    Lq = np.maximum(Lq, kq)
+    ref = O.search_greedy(g, deg, base, q, 0, 0, seed=8, L_per_q=Lq, k_per_q=kq)
+    got = _gpu_search(ctx, g, deg, base, q, 0, 0, seed=8, L_per_q=Lq, k_per_q=kq)
+    assert np.array_equal(got[0], ref[0]) and np.array_equal(got[1], ref[1])
+
+
+@pytest.mark.parametrize("d", [96, 99, 960])
+def test_search_other_dims(ctx, d):
+    rng = np.random.default_rng(d)
+    base = rng.normal(size=(800, d)).astype(np.float32)
+    q = rng.normal(size=(40, d)).astype(np.float32)
+    g, _, deg = O.nndescent(base, 16, 24, 8, 24, 5, seed=1)
+    ref = O.search_greedy(g, deg, base, q, 32, 10, seed=2)
+    got = _gpu_search(ctx, g, deg, base, q, 32, 10, seed=2)
+    assert np.array_equal(got[0], ref[0]) and np.array_equal(got[1], ref[1])
+
+
+def test_search_complete_graph_exact(ctx, small_float):
+    base, q = small_float
+    base = base[:200]
+    n = len(base)
+    g = np.array([[j for j in range(n) if j != i] for i in range(n)], np.uint32)
+    deg = np.full(n, n - 1, np.uint32)
+    gt, _ = O.exact_knn(base, q, 10)
+    ids, *_ = _gpu_search(ctx, g, deg, base, q, 10, 10, seed=5)
+    assert np.array_equal(ids, gt)
+
+
+def test_search_config_changes_speed_only(ctx, small_float, small_float_graph):
+    """waves per block, gather tile and a tiny (forgetful) visited table
+    change nothing but the evaluation count."""
+    from weavess_amd import Index
+    base, q = small_float
+    g, _, deg = small_float_graph
+    ix = Index(ctx, base)
+    ix.set_graph(g, deg)
+    ref = ix.search(q, 64, 10, seed=3)
+    for cfg in [dict(waves_per_block=2), dict(waves_per_block=4, row_groups=4),
+                dict(row_groups=16), dict(visited_slots=64)]:
+        ix.set_search_config(**cfg)
+        got = ix.search(q, 64, 10, seed=3)
+        assert np.array_equal(got[0], ref[0]) and np.array_equal(got[1], ref[1]), cfg
+        if cfg.get("visited_slots"):
+            assert np.all(got[2] >= ref[2])
+        else:
+            assert np.array_equal(got[2], ref[2])
+    ix.close()
+
+
+def test_search_rejects_bad_arguments(ctx, small_float, small_float_graph):
+    from weavess_amd import WvError
+    base, q = small_float
+    g, _, deg = small_float_graph
+    with pytest.raises(WvError) as e:
+        _gpu_search(ctx, g, deg, base, q, 5, 10)  # L < k
+    assert e.value.code == -22
+    with pytest.raises(WvError):
+        _gpu_search(ctx, g, deg, base, q, len(base), 10)  # L >= N, random entries

@@ -1,0 +1,552 @@
+// wv_capi.hip -- the C ABI (include/weavess_hip.h): contexts, device-resident
+// indexes, argument checking and launch plumbing around the kernels.
+#include "../../include/weavess_hip.h"
+#include "wv_device.hpp"
+#include "wv_internal.hpp"
+#include "wv_nnd.hpp"
+
+#include <algorithm>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <new>
+#include <string>
+#include <vector>
+
+struct wv_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    std::mutex mu;
+    std::string err;
+    int cu_count = 0;
+};
+
+struct wv_index {
+    wv_ctx *ctx = nullptr;
+    uint32_t N = 0, d = 0, dp = 0;
+    float *base = nullptr;      // N x dp
+    uint32_t *graph = nullptr;  // N x deg_max
+    float *gdist = nullptr;     // N x deg_max (NaN when unknown)
+    uint32_t *deg = nullptr;    // N
+    uint32_t deg_max = 0;
+    wv_search_config cfg = {0, 0, 0};
+};
+
+namespace {
+
+int fail(wv_ctx *c, int code, const char *fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    if (c) c->err = buf;
+    return code;
+}
+
+#define WV_HIP(ctx, call)                                                                        \
+    do {                                                                                         \
+        hipError_t e_ = (call);                                                                  \
+        if (e_ != hipSuccess)                                                                    \
+            return fail((ctx), e_ == hipErrorOutOfMemory ? WV_ENOMEM : WV_EHIP, "%s: %s (%s:%d)", \
+                        #call, hipGetErrorString(e_), __FILE__, __LINE__);                       \
+    } while (0)
+
+uint32_t next_pow2(uint32_t v) {
+    uint32_t p = 1;
+    while (p < v) p <<= 1;
+    return p;
+}
+
+// Device buffer released at scope exit (used for per-call staging).
+struct DevBuf {
+    void *p = nullptr;
+    ~DevBuf() {
+        if (p) (void)hipFree(p);
+    }
+    hipError_t alloc(size_t bytes) { return hipMalloc(&p, bytes ? bytes : 16); }
+    template <class T> T *as() const { return static_cast<T *>(p); }
+};
+
+// Copy `rows` host rows of d floats into a zero-padded device matrix of
+// stride dp.
+hipError_t upload_padded(float *dst, const float *src, uint32_t rows, uint32_t d, uint32_t dp,
+                         hipStream_t s) {
+    hipError_t e = hipSuccess;
+    if (dp != d) e = hipMemsetAsync(dst, 0, (size_t)rows * dp * 4, s);
+    if (e != hipSuccess) return e;
+    if (rows == 0) return hipSuccess;
+    return hipMemcpy2DAsync(dst, (size_t)dp * 4, src, (size_t)d * 4, (size_t)d * 4, rows,
+                            hipMemcpyHostToDevice, s);
+}
+
+int chunks_per_lane(uint32_t nch) {
+    if (nch <= 16) return 1;
+    if (nch <= 32) return 2;
+    if (nch <= 64) return 4;
+    if (nch <= 128) return 8;
+    if (nch <= 256) return 16;
+    return 0;
+}
+
+struct SearchPlan {
+    uint32_t P, H, wpb, groups, Lmax;
+};
+
+int plan_search(wv_index *ix, uint32_t Lmax, SearchPlan &pl) {
+    pl.Lmax = Lmax;
+    pl.P = std::max<uint32_t>(64, next_pow2(Lmax));
+    uint32_t H = ix->cfg.visited_slots;
+    if (H == 0) H = std::min<uint32_t>(16384, std::max<uint32_t>(2048, next_pow2(Lmax * 32)));
+    if (H & (H - 1)) return fail(ix->ctx, WV_EINVAL, "visited_slots must be a power of two");
+    pl.H = H;
+    uint32_t wpb = ix->cfg.waves_per_block ? ix->cfg.waves_per_block : 1;
+    if (wpb != 1 && wpb != 2 && wpb != 4) return fail(ix->ctx, WV_EINVAL, "waves_per_block must be 1, 2 or 4");
+    size_t lds = wv::search_wave_lds(pl.P, pl.H);
+    while (wpb > 1 && lds * wpb > 160 * 1024) wpb >>= 1;
+    if (lds > 160 * 1024) return fail(ix->ctx, WV_ENOTSUP, "pool/visited table need %zu B of LDS", lds);
+    pl.wpb = wpb;
+    pl.groups = ix->cfg.row_groups ? ix->cfg.row_groups : 8;
+    return WV_OK;
+}
+
+// Core search on device buffers (caller holds the context lock).
+int search_core(wv_index *ix, const float *d_q, uint32_t Q, const uint32_t *d_entries,
+                uint64_t seed, uint32_t L, uint32_t k, const uint32_t *d_L, const uint32_t *d_k,
+                uint32_t Lmax, uint32_t k_stride, uint32_t *d_ids, float *d_dist,
+                uint32_t *d_evals, uint32_t *d_hops, wv_search_stats *stats) {
+    wv_ctx *c = ix->ctx;
+    if (!ix->graph) return fail(c, WV_EINVAL, "index has no graph (build or set one first)");
+    if (Q == 0) {
+        if (stats) *stats = wv_search_stats{0.0, 0, 0};
+        return WV_OK;
+    }
+    SearchPlan pl;
+    int rc = plan_search(ix, Lmax, pl);
+    if (rc) return rc;
+    const int cpl = chunks_per_lane(ix->dp / 4);
+    if (!cpl) return fail(c, WV_ENOTSUP, "dimension %u above 1024", ix->d);
+
+    DevBuf ev_tmp, hop_tmp;
+    if (stats && !d_evals) {
+        WV_HIP(c, ev_tmp.alloc((size_t)Q * 4));
+        d_evals = ev_tmp.as<uint32_t>();
+    }
+    if (stats && !d_hops) {
+        WV_HIP(c, hop_tmp.alloc((size_t)Q * 4));
+        d_hops = hop_tmp.as<uint32_t>();
+    }
+    wv::SearchArgs a;
+    a.base = ix->base;
+    a.queries = d_q;
+    a.graph = ix->graph;
+    a.deg = ix->deg;
+    a.entries = d_entries;
+    a.L_per_q = d_L;
+    a.k_per_q = d_k;
+    a.out_ids = d_ids;
+    a.out_dist = d_dist;
+    a.evals = d_evals;
+    a.hops = d_hops;
+    a.seed = seed;
+    a.N = ix->N;
+    a.dp = ix->dp;
+    a.nch = ix->dp / 4;
+    a.deg_max = ix->deg_max;
+    a.Q = Q;
+    a.L = L;
+    a.k = k;
+    a.k_stride = k_stride;
+    a.P = pl.P;
+    a.H = pl.H;
+    a.wave_lds = (uint32_t)wv::search_wave_lds(pl.P, pl.H);
+    WV_HIP(c, hipEventRecord(c->ev0, c->stream));
+    WV_HIP(c, wv::launch_greedy_search(a, cpl, (int)pl.groups, (int)pl.wpb, c->stream));
+    WV_HIP(c, hipEventRecord(c->ev1, c->stream));
+    WV_HIP(c, hipStreamSynchronize(c->stream));
+    if (stats) {
+        float ms = 0.f;
+        WV_HIP(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+        std::vector<uint32_t> ev(Q), hp(Q);
+        WV_HIP(c, hipMemcpy(ev.data(), d_evals, (size_t)Q * 4, hipMemcpyDeviceToHost));
+        WV_HIP(c, hipMemcpy(hp.data(), d_hops, (size_t)Q * 4, hipMemcpyDeviceToHost));
+        stats->kernel_ms = ms;
+        stats->dist_evals = 0;
+        stats->hops = 0;
+        for (uint32_t i = 0; i < Q; ++i) {
+            stats->dist_evals += ev[i];
+            stats->hops += hp[i];
+        }
+    }
+    return WV_OK;
+}
+
+// Host-side argument checks shared by the search entry points.
+int check_search_args(wv_ctx *c, uint32_t N, uint32_t Q, const void *entries, uint32_t L,
+                      uint32_t k, const uint32_t *L_per_q, const uint32_t *k_per_q,
+                      uint32_t k_stride, uint32_t &Lmax) {
+    if ((L_per_q || k_per_q) && entries)
+        return fail(c, WV_EINVAL, "explicit entries need one L for all queries");
+    Lmax = 0;
+    for (uint32_t q = 0; q < Q; ++q) {
+        uint32_t Lq = L_per_q ? L_per_q[q] : L, kq = k_per_q ? k_per_q[q] : k;
+        if (Lq == 0 || kq == 0)
+            return fail(c, WV_EINVAL, "query %u: L and k must be positive", q);
+        if (kq > Lq)  // src/builder.cpp:435-438
+            return fail(c, WV_EINVAL, "query %u: search_L (%u) cannot be smaller than search_K (%u)", q, Lq, kq);
+        if (kq > k_stride) return fail(c, WV_EINVAL, "query %u: k (%u) above k_stride (%u)", q, kq, k_stride);
+        if (entries ? Lq > N : Lq >= N)
+            return fail(c, WV_EINVAL, "query %u: L (%u) must be below N (%u) for random entries", q, Lq, N);
+        Lmax = std::max(Lmax, Lq);
+    }
+    if (Lmax > 1024) return fail(c, WV_ENOTSUP, "L above 1024");
+    return WV_OK;
+}
+
+// Explicit entry ids (host copy) must name base rows.
+int check_entries(wv_ctx *c, const uint32_t *entries, size_t n, uint32_t N) {
+    for (size_t i = 0; i < n; ++i)
+        if (entries[i] >= N) return fail(c, WV_EINVAL, "entry %zu is %u, not below N=%u", i, entries[i], N);
+    return WV_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int wv_version(void) { return 1; }
+
+int wv_ctx_create(int device, wv_ctx **out) {
+    if (!out) return WV_EINVAL;
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return WV_EHIP;
+    if (device < 0 || device >= n) return WV_EINVAL;
+    wv_ctx *c = new (std::nothrow) wv_ctx;
+    if (!c) return WV_ENOMEM;
+    c->device = device;
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
+        delete c;
+        return WV_EHIP;
+    }
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->cu_count = prop.multiProcessorCount;
+    *out = c;
+    return WV_OK;
+}
+
+int wv_ctx_destroy(wv_ctx *c) {
+    if (!c) return WV_EINVAL;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    if (c->ev0) (void)hipEventDestroy(c->ev0);
+    if (c->ev1) (void)hipEventDestroy(c->ev1);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return WV_OK;
+}
+
+const char *wv_last_error(const wv_ctx *c) { return c ? c->err.c_str() : "null context"; }
+
+int wv_index_create(wv_ctx *c, const float *base, uint32_t N, uint32_t d, wv_index **out) {
+    if (!c || !out) return WV_EINVAL;
+    std::lock_guard<std::mutex> g(c->mu);
+    *out = nullptr;
+    if (!base) return fail(c, WV_EINVAL, "base is null");
+    if (N < 2 || d == 0) return fail(c, WV_EINVAL, "need N >= 2 and d >= 1 (N=%u d=%u)", N, d);
+    if (d > 1024) return fail(c, WV_ENOTSUP, "dimension %u above 1024", d);
+    WV_HIP(c, hipSetDevice(c->device));
+    wv_index *ix = new (std::nothrow) wv_index;
+    if (!ix) return fail(c, WV_ENOMEM, "host allocation");
+    ix->ctx = c;
+    ix->N = N;
+    ix->d = d;
+    ix->dp = (d + 3) & ~3u;
+    hipError_t e = hipMalloc(&ix->base, (size_t)N * ix->dp * 4);
+    if (e == hipSuccess) e = upload_padded(ix->base, base, N, d, ix->dp, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) {
+        if (ix->base) (void)hipFree(ix->base);
+        delete ix;
+        return fail(c, e == hipErrorOutOfMemory ? WV_ENOMEM : WV_EHIP, "index upload: %s", hipGetErrorString(e));
+    }
+    *out = ix;
+    return WV_OK;
+}
+
+int wv_index_destroy(wv_index *ix) {
+    if (!ix) return WV_EINVAL;
+    std::lock_guard<std::mutex> g(ix->ctx->mu);
+    (void)hipSetDevice(ix->ctx->device);
+    if (ix->base) (void)hipFree(ix->base);
+    if (ix->graph) (void)hipFree(ix->graph);
+    if (ix->gdist) (void)hipFree(ix->gdist);
+    if (ix->deg) (void)hipFree(ix->deg);
+    delete ix;
+    return WV_OK;
+}
+
+}  // extern "C"
+
+namespace wv {
+// Replace the index's graph storage with freshly allocated N x deg_max
+// buffers (contents undefined).
+int index_alloc_graph(wv_index *ix, uint32_t deg_max) {
+    wv_ctx *c = ix->ctx;
+    if (ix->graph) (void)hipFree(ix->graph);
+    if (ix->gdist) (void)hipFree(ix->gdist);
+    if (ix->deg) (void)hipFree(ix->deg);
+    ix->graph = nullptr;
+    ix->gdist = nullptr;
+    ix->deg = nullptr;
+    ix->deg_max = 0;
+    WV_HIP(c, hipMalloc(&ix->graph, (size_t)ix->N * std::max<uint32_t>(deg_max, 1) * 4));
+    WV_HIP(c, hipMalloc(&ix->gdist, (size_t)ix->N * std::max<uint32_t>(deg_max, 1) * 4));
+    WV_HIP(c, hipMalloc(&ix->deg, (size_t)ix->N * 4));
+    ix->deg_max = deg_max;
+    return WV_OK;
+}
+IndexView index_view(wv_index *ix) {
+    IndexView v;
+    v.base = ix->base;
+    v.N = ix->N;
+    v.d = ix->d;
+    v.dp = ix->dp;
+    v.stream = ix->ctx->stream;
+    return v;
+}
+int index_set_error(wv_index *ix, int code, const char *msg) { return fail(ix->ctx, code, "%s", msg); }
+}  // namespace wv
+
+extern "C" {
+
+int wv_index_set_graph(wv_index *ix, const uint32_t *ids, const uint32_t *deg, uint32_t deg_max) {
+    if (!ix) return WV_EINVAL;
+    wv_ctx *c = ix->ctx;
+    std::lock_guard<std::mutex> g(c->mu);
+    if (!ids || !deg) return fail(c, WV_EINVAL, "graph ids/deg are null");
+    if (deg_max == 0) return fail(c, WV_EINVAL, "deg_max must be positive");
+    WV_HIP(c, hipSetDevice(c->device));
+    int rc = wv::index_alloc_graph(ix, deg_max);
+    if (rc) return rc;
+    WV_HIP(c, hipMemcpyAsync(ix->graph, ids, (size_t)ix->N * deg_max * 4, hipMemcpyHostToDevice, c->stream));
+    WV_HIP(c, hipMemcpyAsync(ix->deg, deg, (size_t)ix->N * 4, hipMemcpyHostToDevice, c->stream));
+    WV_HIP(c, hipMemsetAsync(ix->gdist, 0xFF, (size_t)ix->N * deg_max * 4, c->stream));
+    WV_HIP(c, hipStreamSynchronize(c->stream));
+    return WV_OK;
+}
+
+int wv_index_get_graph(wv_index *ix, uint32_t *ids, float *dist, uint32_t *deg, uint32_t *deg_max) {
+    if (!ix) return WV_EINVAL;
+    wv_ctx *c = ix->ctx;
+    std::lock_guard<std::mutex> g(c->mu);
+    if (!ix->graph) return fail(c, WV_EINVAL, "index has no graph");
+    if (deg_max) *deg_max = ix->deg_max;
+    WV_HIP(c, hipSetDevice(c->device));
+    if (ids) WV_HIP(c, hipMemcpy(ids, ix->graph, (size_t)ix->N * ix->deg_max * 4, hipMemcpyDeviceToHost));
+    if (dist) WV_HIP(c, hipMemcpy(dist, ix->gdist, (size_t)ix->N * ix->deg_max * 4, hipMemcpyDeviceToHost));
+    if (deg) WV_HIP(c, hipMemcpy(deg, ix->deg, (size_t)ix->N * 4, hipMemcpyDeviceToHost));
+    return WV_OK;
+}
+
+int wv_index_set_search_config(wv_index *ix, const wv_search_config *cfg) {
+    if (!ix || !cfg) return WV_EINVAL;
+    std::lock_guard<std::mutex> g(ix->ctx->mu);
+    if (cfg->visited_slots & (cfg->visited_slots - 1))
+        return fail(ix->ctx, WV_EINVAL, "visited_slots must be a power of two");
+    if (cfg->waves_per_block > 4 || cfg->waves_per_block == 3)
+        return fail(ix->ctx, WV_EINVAL, "waves_per_block must be 0, 1, 2 or 4");
+    ix->cfg = *cfg;
+    return WV_OK;
+}
+
+int wv_index_search(wv_index *ix, const float *queries, uint32_t Q, const uint32_t *entries,
+                    uint64_t seed, uint32_t L, uint32_t k, const uint32_t *L_per_q,
+                    const uint32_t *k_per_q, uint32_t k_stride, uint32_t *out_ids, float *out_dist,
+                    uint32_t *evals, uint32_t *hops, wv_search_stats *stats) {
+    if (!ix) return WV_EINVAL;
+    wv_ctx *c = ix->ctx;
+    std::lock_guard<std::mutex> g(c->mu);
+    if (!queries || !out_ids) return fail(c, WV_EINVAL, "queries/out_ids are null");
+    uint32_t Lmax = 0;
+    int rc = check_search_args(c, ix->N, Q, entries, L, k, L_per_q, k_per_q, k_stride, Lmax);
+    if (rc) return rc;
+    if (entries && (rc = check_entries(c, entries, (size_t)Q * L, ix->N))) return rc;
+    WV_HIP(c, hipSetDevice(c->device));
+    if (Q == 0) {
+        if (stats) *stats = wv_search_stats{0.0, 0, 0};
+        return WV_OK;
+    }
+    DevBuf dq, de, dl, dk, di, dd, dev, dh;
+    WV_HIP(c, dq.alloc((size_t)Q * ix->dp * 4));
+    WV_HIP(c, upload_padded(dq.as<float>(), queries, Q, ix->d, ix->dp, c->stream));
+    if (entries) {
+        WV_HIP(c, de.alloc((size_t)Q * L * 4));
+        WV_HIP(c, hipMemcpyAsync(de.p, entries, (size_t)Q * L * 4, hipMemcpyHostToDevice, c->stream));
+    }
+    if (L_per_q) {
+        WV_HIP(c, dl.alloc((size_t)Q * 4));
+        WV_HIP(c, hipMemcpyAsync(dl.p, L_per_q, (size_t)Q * 4, hipMemcpyHostToDevice, c->stream));
+    }
+    if (k_per_q) {
+        WV_HIP(c, dk.alloc((size_t)Q * 4));
+        WV_HIP(c, hipMemcpyAsync(dk.p, k_per_q, (size_t)Q * 4, hipMemcpyHostToDevice, c->stream));
+    }
+    WV_HIP(c, di.alloc((size_t)Q * k_stride * 4));
+    if (out_dist) WV_HIP(c, dd.alloc((size_t)Q * k_stride * 4));
+    if (evals) WV_HIP(c, dev.alloc((size_t)Q * 4));
+    if (hops) WV_HIP(c, dh.alloc((size_t)Q * 4));
+    rc = search_core(ix, dq.as<float>(), Q, de.as<uint32_t>(), seed, L, k, dl.as<uint32_t>(),
+                     dk.as<uint32_t>(), Lmax, k_stride, di.as<uint32_t>(), dd.as<float>(),
+                     dev.as<uint32_t>(), dh.as<uint32_t>(), stats);
+    if (rc) return rc;
+    WV_HIP(c, hipMemcpy(out_ids, di.p, (size_t)Q * k_stride * 4, hipMemcpyDeviceToHost));
+    if (out_dist) WV_HIP(c, hipMemcpy(out_dist, dd.p, (size_t)Q * k_stride * 4, hipMemcpyDeviceToHost));
+    if (evals) WV_HIP(c, hipMemcpy(evals, dev.p, (size_t)Q * 4, hipMemcpyDeviceToHost));
+    if (hops) WV_HIP(c, hipMemcpy(hops, dh.p, (size_t)Q * 4, hipMemcpyDeviceToHost));
+    return WV_OK;
+}
+
+int wv_index_search_dev(wv_index *ix, const float *d_queries, uint32_t Q, const uint32_t *d_entries,
+                        uint64_t seed, uint32_t L, uint32_t k, const uint32_t *d_L_per_q,
+                        const uint32_t *d_k_per_q, uint32_t k_stride, uint32_t *d_out_ids,
+                        float *d_out_dist, uint32_t *d_evals, uint32_t *d_hops,
+                        wv_search_stats *stats) {
+    if (!ix) return WV_EINVAL;
+    wv_ctx *c = ix->ctx;
+    std::lock_guard<std::mutex> g(c->mu);
+    if (!d_queries || !d_out_ids) return fail(c, WV_EINVAL, "queries/out_ids are null");
+    WV_HIP(c, hipSetDevice(c->device));
+    // per-query L/k are checked on the host before any launch
+    std::vector<uint32_t> hl, hk;
+    if (d_L_per_q) {
+        hl.resize(Q);
+        WV_HIP(c, hipMemcpy(hl.data(), d_L_per_q, (size_t)Q * 4, hipMemcpyDeviceToHost));
+    }
+    if (d_k_per_q) {
+        hk.resize(Q);
+        WV_HIP(c, hipMemcpy(hk.data(), d_k_per_q, (size_t)Q * 4, hipMemcpyDeviceToHost));
+    }
+    uint32_t Lmax = 0;
+    int rc = check_search_args(c, ix->N, Q, d_entries, L, k, d_L_per_q ? hl.data() : nullptr,
+                               d_k_per_q ? hk.data() : nullptr, k_stride, Lmax);
+    if (rc) return rc;
+    if (d_entries && Q) {
+        std::vector<uint32_t> he((size_t)Q * L);
+        WV_HIP(c, hipMemcpy(he.data(), d_entries, he.size() * 4, hipMemcpyDeviceToHost));
+        if ((rc = check_entries(c, he.data(), he.size(), ix->N))) return rc;
+    }
+    return search_core(ix, d_queries, Q, d_entries, seed, L, k, d_L_per_q, d_k_per_q, Lmax,
+                       k_stride, d_out_ids, d_out_dist, d_evals, d_hops, stats);
+}
+
+int wv_index_build_nndescent(wv_index *ix, const wv_nnd_params *p, const uint32_t *init_ids,
+                             const uint32_t *init_cnt, wv_build_stats *stats) {
+    if (!ix) return WV_EINVAL;
+    wv_ctx *c = ix->ctx;
+    std::lock_guard<std::mutex> g(c->mu);
+    if (!p) return fail(c, WV_EINVAL, "params are null");
+    if (p->K == 0 || p->L < p->K)
+        return fail(c, WV_EINVAL, "need 1 <= K <= L (K=%u L=%u)", p->K, p->L);
+    if (p->S == 0 || p->S >= ix->N || p->S > p->L)
+        return fail(c, WV_EINVAL, "need 1 <= S < N and S <= L (S=%u N=%u L=%u)", p->S, ix->N, p->L);
+    if (p->S > 64) return fail(c, WV_ENOTSUP, "S above 64");
+    if (p->L > 1024 || p->R > 1024) return fail(c, WV_ENOTSUP, "L or R above 1024");
+    if (p->dtype > WV_DTYPE_FP8) return fail(c, WV_EINVAL, "unknown dtype %u", p->dtype);
+    if (p->cand_slots & (p->cand_slots - 1)) return fail(c, WV_EINVAL, "cand_slots must be a power of two");
+    WV_HIP(c, hipSetDevice(c->device));
+    int rc = wv::index_alloc_graph(ix, p->K);
+    if (rc) return rc;
+    rc = wv::nndescent_build(ix, p, init_ids, init_cnt, stats, ix->graph, ix->gdist, ix->deg);
+    if (rc) {
+        (void)hipFree(ix->graph);
+        (void)hipFree(ix->gdist);
+        (void)hipFree(ix->deg);
+        ix->graph = nullptr;
+        ix->gdist = nullptr;
+        ix->deg = nullptr;
+        ix->deg_max = 0;
+    }
+    return rc;
+}
+
+int wv_nndescent_build(wv_ctx *c, const float *base, uint32_t N, uint32_t d,
+                       const wv_nnd_params *p, const uint32_t *init_ids, uint32_t *out_ids,
+                       float *out_dist, uint32_t *out_deg, wv_build_stats *stats) {
+    if (!c) return WV_EINVAL;
+    if (!out_ids || !out_deg) return fail(c, WV_EINVAL, "out_ids/out_deg are null");
+    wv_index *ix = nullptr;
+    int rc = wv_index_create(c, base, N, d, &ix);
+    if (rc) return rc;
+    rc = wv_index_build_nndescent(ix, p, init_ids, nullptr, stats);
+    if (!rc) rc = wv_index_get_graph(ix, out_ids, out_dist, out_deg, nullptr);
+    wv_index_destroy(ix);
+    return rc;
+}
+
+int wv_search_greedy(wv_ctx *c, const wv_graph *gr, const float *base, uint32_t d,
+                     const float *queries, uint32_t Q, const uint32_t *entries, uint64_t seed,
+                     uint32_t L, uint32_t k, const uint32_t *L_per_q, const uint32_t *k_per_q,
+                     uint32_t k_stride, uint32_t *out_ids, float *out_dist, uint32_t *evals,
+                     uint32_t *hops, wv_search_stats *stats) {
+    if (!c) return WV_EINVAL;
+    if (!gr) return fail(c, WV_EINVAL, "graph is null");
+    wv_index *ix = nullptr;
+    int rc = wv_index_create(c, base, gr->N, d, &ix);
+    if (rc) return rc;
+    rc = wv_index_set_graph(ix, gr->ids, gr->deg, gr->deg_max);
+    if (!rc)
+        rc = wv_index_search(ix, queries, Q, entries, seed, L, k, L_per_q, k_per_q, k_stride,
+                             out_ids, out_dist, evals, hops, stats);
+    wv_index_destroy(ix);
+    return rc;
+}
+
+int wv_exact_knn(wv_ctx *c, const float *base, uint32_t N, const float *queries, uint32_t Q,
+                 uint32_t d, uint32_t k, int64_t self_offset, uint32_t *out_ids, double *out_dist,
+                 double *kernel_ms) {
+    if (!c) return WV_EINVAL;
+    std::lock_guard<std::mutex> g(c->mu);
+    if (!base || !queries || !out_ids || !out_dist) return fail(c, WV_EINVAL, "null argument");
+    if (k == 0 || k > 128) return fail(c, WV_ENOTSUP, "exact kNN supports 1 <= k <= 128 (k=%u)", k);
+    if (k > N - (self_offset >= 0 ? 1u : 0u)) return fail(c, WV_EINVAL, "k above the number of candidates");
+    if (d == 0 || d > 1024) return fail(c, WV_ENOTSUP, "dimension must be 1..1024");
+    WV_HIP(c, hipSetDevice(c->device));
+    if (Q == 0) return WV_OK;
+    DevBuf db, dq, di, dd;
+    WV_HIP(c, db.alloc((size_t)N * d * 4));
+    WV_HIP(c, dq.alloc((size_t)Q * d * 4));
+    WV_HIP(c, di.alloc((size_t)Q * k * 4));
+    WV_HIP(c, dd.alloc((size_t)Q * k * 8));
+    WV_HIP(c, hipMemcpyAsync(db.p, base, (size_t)N * d * 4, hipMemcpyHostToDevice, c->stream));
+    WV_HIP(c, hipMemcpyAsync(dq.p, queries, (size_t)Q * d * 4, hipMemcpyHostToDevice, c->stream));
+    wv::ExactArgs a;
+    a.base = db.as<float>();
+    a.queries = dq.as<float>();
+    a.N = N;
+    a.Q = Q;
+    a.d = d;
+    a.ld_base = d;
+    a.ld_query = d;
+    a.k = k;
+    a.self_offset = self_offset;
+    a.out_ids = di.as<uint32_t>();
+    a.out_dist = dd.as<double>();
+    WV_HIP(c, hipEventRecord(c->ev0, c->stream));
+    WV_HIP(c, wv::launch_exact_knn(a, c->stream));
+    WV_HIP(c, hipEventRecord(c->ev1, c->stream));
+    WV_HIP(c, hipStreamSynchronize(c->stream));
+    if (kernel_ms) {
+        float ms = 0.f;
+        WV_HIP(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+        *kernel_ms = ms;
+    }
+    WV_HIP(c, hipMemcpy(out_ids, di.p, (size_t)Q * k * 4, hipMemcpyDeviceToHost));
+    WV_HIP(c, hipMemcpy(out_dist, dd.p, (size_t)Q * k * 8, hipMemcpyDeviceToHost));
+    return WV_OK;
+}
+
+}  // extern "C"

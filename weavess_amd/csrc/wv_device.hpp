@@ -1,0 +1,234 @@
+// wv_device.hpp -- device-side building blocks shared by the HIP kernels.
+//
+// Everything here is written for gfx950 (CDNA4): 64-lane wavefronts, LDS
+// atomics, 64-bit keys moved across lanes as two 32-bit shuffles.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace wv {
+
+constexpr int WAVE = 64;
+constexpr uint64_t KEY_EMPTY = ~0ull;
+constexpr uint32_t ID_NONE = 0xFFFFFFFFu;
+constexpr uint64_t GOLDEN = 0x9E3779B97F4A7C15ull;
+
+// Stream tags; the values are part of the seeded-output contract.
+constexpr uint64_t TAG_INIT = 0x494E4954ull;    // "INIT"
+constexpr uint64_t TAG_ENTRY = 0x454E5452ull;   // "ENTR"
+constexpr uint64_t TAG_REVNEW = 0x52564E57ull;  // "RVNW"
+constexpr uint64_t TAG_REVOLD = 0x52564F4Cull;  // "RVOL"
+
+// ---------------------------------------------------------------- keys
+// (dist, id) packed so that unsigned 64-bit order is (dist, id) order for
+// non-negative finite distances; ties between equal distances go to the
+// smaller id.
+__host__ __device__ inline uint64_t make_key(float d, uint32_t id) {
+    return ((uint64_t)__float_as_uint(d) << 32) | id;
+}
+__host__ __device__ inline uint32_t key_id(uint64_t k) { return (uint32_t)k; }
+__host__ __device__ inline float key_dist(uint64_t k) { return __uint_as_float((uint32_t)(k >> 32)); }
+
+// ---------------------------------------------------------------- hashing
+// splitmix64 finaliser (Steele, Lea, Flood 2014) and murmur3's fmix32.
+__host__ __device__ inline uint64_t fmix64(uint64_t z) {
+    z ^= z >> 30;
+    z *= 0xBF58476D1CE4E5B9ull;
+    z ^= z >> 27;
+    z *= 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    return z;
+}
+__host__ __device__ inline uint32_t fmix32(uint32_t h) {
+    h ^= h >> 16;
+    h *= 0x85EBCA6Bu;
+    h ^= h >> 13;
+    h *= 0xC2B2AE35u;
+    h ^= h >> 16;
+    return h;
+}
+// Counter-based random stream: draw k of stream s is independent of every
+// other draw, so a wavefront can produce draws in any order.
+__host__ __device__ inline uint64_t stream_of(uint64_t seed, uint64_t tag, uint64_t idx) {
+    uint64_t h = fmix64(seed + GOLDEN);
+    h = fmix64(h ^ (tag * GOLDEN));
+    h = fmix64(h ^ (idx + 0x632BE59BD9B4E019ull));
+    return h;
+}
+__host__ __device__ inline uint32_t draw(uint64_t s, uint64_t k) {
+    return (uint32_t)(fmix64(s + (k + 1) * GOLDEN) >> 32);
+}
+// Slot of candidate `id` in target `t`'s candidate table.
+__host__ __device__ inline uint32_t cand_slot(uint32_t t, uint32_t id, uint32_t mask) {
+    return fmix32(id * 0x9E3779B1u + t * 0x7F4A7C15u) & mask;
+}
+
+// ---------------------------------------------------------------- lanes
+__device__ inline int lane_id() { return (int)(threadIdx.x & (WAVE - 1)); }
+
+// Make this wave's LDS writes visible to its own lanes (no cross-wave
+// barrier: waves of one block run independent queries/nodes).
+__device__ inline void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ inline uint64_t shfl_xor64(uint64_t v, int m) {
+    uint32_t lo = (uint32_t)v, hi = (uint32_t)(v >> 32);
+    lo = (uint32_t)__shfl_xor((int)lo, m);
+    hi = (uint32_t)__shfl_xor((int)hi, m);
+    return ((uint64_t)hi << 32) | lo;
+}
+__device__ inline uint64_t shfl64(uint64_t v, int src) {
+    uint32_t lo = (uint32_t)v, hi = (uint32_t)(v >> 32);
+    lo = (uint32_t)__shfl((int)lo, src);
+    hi = (uint32_t)__shfl((int)hi, src);
+    return ((uint64_t)hi << 32) | lo;
+}
+
+// Number of set bits of `mask` below this lane.
+__device__ inline uint32_t lanes_below(uint64_t mask) {
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
+                                     __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+}
+
+// Ascending bitonic sort of one 64-bit key per lane across the wavefront.
+__device__ inline uint64_t wave_sort64(uint64_t key) {
+    const int lane = lane_id();
+#pragma unroll
+    for (int k = 2; k <= WAVE; k <<= 1) {
+#pragma unroll
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            uint64_t o = shfl_xor64(key, j);
+            bool up = (lane & k) == 0;
+            bool lower = (lane & j) == 0;
+            bool keep_min = (lower == up);
+            uint64_t mn = key < o ? key : o, mx = key < o ? o : key;
+            key = keep_min ? mn : mx;
+        }
+    }
+    return key;
+}
+
+// Ascending bitonic sort of P (power of two, >= 2) 64-bit keys in LDS by one
+// wavefront.
+__device__ inline void lds_sort64(uint64_t *a, int P) {
+    const int lane = lane_id();
+    for (int k = 2; k <= P; k <<= 1) {
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            for (int t = lane; t < P / 2; t += WAVE) {
+                int lo = ((t & ~(j - 1)) << 1) | (t & (j - 1));
+                int hi = lo | j;
+                bool up = (lo & k) == 0;
+                uint64_t x = a[lo], y = a[hi];
+                if ((x > y) == up) {
+                    a[lo] = y;
+                    a[hi] = x;
+                }
+            }
+            wave_sync();
+        }
+    }
+}
+__device__ inline void lds_sort32(uint32_t *a, int P) {
+    const int lane = lane_id();
+    for (int k = 2; k <= P; k <<= 1) {
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            for (int t = lane; t < P / 2; t += WAVE) {
+                int lo = ((t & ~(j - 1)) << 1) | (t & (j - 1));
+                int hi = lo | j;
+                bool up = (lo & k) == 0;
+                uint32_t x = a[lo], y = a[hi];
+                if ((x > y) == up) {
+                    a[lo] = y;
+                    a[hi] = x;
+                }
+            }
+            wave_sync();
+        }
+    }
+}
+
+// Number of elements of sorted a[0..n) that are < key.
+__device__ inline int lower_bound64(const uint64_t *a, int n, uint64_t key) {
+    int lo = 0, hi = n;
+    while (lo < hi) {
+        int mid = (lo + hi) >> 1;
+        if (a[mid] < key) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// Inclusive wave-wide max scan of a signed 64-bit value.
+__device__ inline int64_t wave_scan_max64(int64_t v) {
+    const int lane = lane_id();
+#pragma unroll
+    for (int o = 1; o < WAVE; o <<= 1) {
+        uint32_t lo = (uint32_t)__shfl_up((int)(uint32_t)v, o);
+        uint32_t hi = (uint32_t)__shfl_up((int)(uint32_t)((uint64_t)v >> 32), o);
+        int64_t u = (int64_t)(((uint64_t)hi << 32) | lo);
+        if (lane >= o && u > v) v = u;
+    }
+    return v;
+}
+
+// ---------------------------------------------------------------- distance
+// One chunk (four consecutive elements) of the build-order distance:
+// ((x0^2 + x1^2) + x2^2) + x3^2 with no fused multiply-add (the library is
+// compiled with -ffp-contract=off).
+__device__ inline float chunk_sq(const float4 a, const float4 b) {
+    float d0 = a.x - b.x, d1 = a.y - b.y, d2 = a.z - b.z, d3 = a.w - b.w;
+    float s = d0 * d0 + d1 * d1;
+    s = s + d2 * d2;
+    s = s + d3 * d3;
+    return s;
+}
+// Fold of the 16 accumulators of a 16-lane group: acc[j] += acc[j+h] for
+// h = 8, 4, 2, 1.  Every lane of the group ends with the same value.
+__device__ inline float fold16(float acc) {
+    acc = acc + __shfl_xor(acc, 8);
+    acc = acc + __shfl_xor(acc, 4);
+    acc = acc + __shfl_xor(acc, 2);
+    acc = acc + __shfl_xor(acc, 1);
+    return acc;
+}
+
+// keys_out[r] = (dist(query, base[ids[r]]), ids[r]) for r < m.  16 lanes per
+// row; lane j of a row group owns accumulator j of the build order, i.e.
+// chunks j, j+16, ...; G row groups of 4 are gathered before any is reduced
+// so that 4*G rows of loads are in flight.
+template <int CPL, int G>
+__device__ inline void rows_dist(const float *__restrict__ base, uint32_t dp, uint32_t nch,
+                                 const float4 (&qv)[CPL], const uint32_t *ids, int m,
+                                 uint64_t *keys_out) {
+    const int lane = lane_id(), sub = lane & 15, grp = lane >> 4;
+    for (int r0 = 0; r0 < m; r0 += 4 * G) {
+        float4 v[G][CPL];
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+            const int row = r0 + 4 * g + grp;
+            const uint32_t id = row < m ? ids[row] : 0u;
+            const float4 *p = reinterpret_cast<const float4 *>(base + (size_t)id * dp);
+#pragma unroll
+            for (int c = 0; c < CPL; ++c) {
+                const int ch = sub + 16 * c;
+                v[g][c] = (row < m && ch < (int)nch) ? p[ch] : make_float4(0.f, 0.f, 0.f, 0.f);
+            }
+        }
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+            const int row = r0 + 4 * g + grp;
+            float acc = 0.f;
+#pragma unroll
+            for (int c = 0; c < CPL; ++c)
+                if (sub + 16 * c < (int)nch) acc = acc + chunk_sq(qv[c], v[g][c]);
+            acc = fold16(acc);
+            if (sub == 0 && row < m) keys_out[row] = make_key(acc, ids[row]);
+        }
+    }
+}
+
+}  // namespace wv

@@ -1,0 +1,48 @@
+// wv_internal.hpp -- kernel argument blocks and launchers shared between the
+// HIP translation units and the C-ABI layer (wv_capi.hip).  Not installed.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+namespace wv {
+
+// ------------------------------------------------------------ greedy search
+struct SearchArgs {
+    const float *base;        // N x dp, rows zero-padded to dp = roundup(d, 4)
+    const float *queries;     // Q x dp, same padding
+    const uint32_t *graph;    // N x deg_max
+    const uint32_t *deg;      // N
+    const uint32_t *entries;  // Q x L or null (seeded GenRandom)
+    const uint32_t *L_per_q;  // Q or null
+    const uint32_t *k_per_q;  // Q or null
+    uint32_t *out_ids;        // Q x k_stride
+    float *out_dist;          // Q x k_stride or null
+    uint32_t *evals;          // Q or null
+    uint32_t *hops;           // Q or null
+    uint64_t seed;
+    uint32_t N, dp, nch, deg_max;
+    uint32_t Q, L, k, k_stride;
+    uint32_t P;               // power of two >= every L
+    uint32_t H;               // visited-table slots (power of two)
+    uint32_t wave_lds;        // LDS bytes per wavefront
+};
+
+size_t search_wave_lds(uint32_t P, uint32_t H);
+hipError_t launch_greedy_search(const SearchArgs &a, int chunks_per_lane, int row_groups,
+                                int waves_per_block, hipStream_t s);
+
+// ------------------------------------------------------------ exact kNN
+struct ExactArgs {
+    const float *base;     // N x d (unpadded rows of stride ld_base)
+    const float *queries;  // Q x d (stride ld_query)
+    uint32_t N, Q, d, ld_base, ld_query;
+    uint32_t k;
+    int64_t self_offset;   // >= 0: query q is base row q+self_offset, skipped
+    uint32_t *out_ids;     // Q x k
+    double *out_dist;      // Q x k
+};
+hipError_t launch_exact_knn(const ExactArgs &a, hipStream_t s);
+
+}  // namespace wv

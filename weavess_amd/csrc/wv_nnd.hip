@@ -1,0 +1,1069 @@
+// wv_nnd.hip -- NN-Descent graph construction on one GPU.
+//
+// Replaces ComponentInitRandom::InitInner (src/component_init.cpp:11-45),
+// ComponentRefineNNDescent::RefineInner (src/component_refine.cpp:12-239)
+// and ComponentPruneNaive::PruneInner (src/component_prune.cpp:8-18).
+//
+// State per node n, all in HBM:
+//   pool[n][0..L)   ascending (dist,id) keys of its neighbour pool, unused
+//                   slots KEY_EMPTY; pflag = "new" flags; pcnt = size
+//   M[n]            sampling depth of update() (component_refine.cpp:167-175)
+//   thr[n]          worst key of the pool (join acceptance threshold)
+//   worst[n]        its distance (reverse-sample test, :191/:202)
+//   fnew/fold       forward new/old samples (keys) of the last update
+//   rv_*            reverse proposals in CSR form, rsel_* the kept ones
+//   phash[n]        open-addressing table of the pool's ids, probed by the
+//                   join so that an id already pooled never takes a slot
+//                   (the duplicate test of index.h:123-125, done up front)
+//   cand[n][0..C)   candidate slots: a join result (id, d) for target t goes
+//                   to slot hash(t, id) via 64-bit atomicMin on its key, so
+//                   each slot keeps the best proposal that hashed there and
+//                   the outcome does not depend on arrival order.
+// Iteration it (0-based; it = 0's join is empty, component_refine.cpp:90
+// with include/weavess/index.h:98-103):
+//   join   one wavefront per node: new x new (i < j) and new x old Gram
+//          blocks on MFMA from LDS-staged 16-row tiles, d = |a|^2 + |b|^2 -
+//          2 a.b with precomputed squared norms; proposals at or below the
+//          target's threshold go to its candidate slots;
+//   merge  one wavefront per node: pool U slot winners (duplicates by id
+//          dropped, pool entry wins), bitonic sort, first L kept, winners
+//          flagged new, slots cleared, new M;
+//   sample forward samples + reverse counts, exclusive scans, reverse
+//          scatter, reverse selection (R smallest hash keys, sorted by id).
+// The last iteration's sampling is skipped: it changes only flags and M,
+// which nothing reads afterwards.  finalize re-ranks the first
+// rerank_mult*K pool entries with exact FP32 build-order distances and
+// keeps the first K (PruneNaive).
+#include "wv_device.hpp"
+#include "wv_nnd.hpp"
+
+#include <algorithm>
+#include <cstdio>
+#include <vector>
+
+namespace wv {
+namespace {
+
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+
+constexpr int DT_F32 = 0, DT_BF16 = 1, DT_FP8 = 2;
+constexpr float FP8_MAX = 448.0f;  // largest finite e4m3 magnitude
+constexpr int TILE_ST = 528;       // LDS bytes per staged tile row (512 + 16 pad)
+
+struct NndDev {
+    const float *base;     // N x dp FP32 rows (zero padded)
+    uint32_t dp;
+    const uint8_t *xq;     // join rows in join precision
+    uint32_t xq_stride;    // bytes per join row
+    uint32_t kdim;         // elements per join row (FP32: dp; BF16/FP8: multiple of 32)
+    const float *norm;     // squared norms of the join rows
+    const float *scale;    // FP8 per-row scale, else null
+    uint32_t N, L, S, R, K, C, rerank;
+    uint32_t iter;
+    uint64_t seed;
+    uint64_t *pool;
+    uint8_t *pflag;
+    uint32_t *pcnt, *M;
+    uint64_t *thr;
+    float *worst;
+    uint64_t *fnew, *fold;
+    uint32_t *fnew_cnt, *fold_cnt;
+    uint32_t *rcnt_new, *rcnt_old, *rcur_new, *rcur_old, *roff_new, *roff_old, *rv_new, *rv_old;
+    uint32_t *rsel_new, *rsel_old, *rsel_new_cnt, *rsel_old_cnt;
+    uint64_t *cand;
+    uint32_t *phash;          // N x HP open-addressing table of the pool's ids
+    uint32_t HP;
+    unsigned long long *ctr;  // [0] pairs [1] proposals [2] rows gathered [3] pool updates
+    uint32_t *out_ids;
+    float *out_dist;
+    uint32_t *out_deg;
+    uint32_t lds_join_list;   // bytes of list area in the join's per-wave LDS
+    uint32_t nw_pow2;         // power of two >= S + R
+};
+
+__device__ inline uint32_t wave_sum_u32(uint32_t v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += (uint32_t)__shfl_xor((int)v, o);
+    return v;
+}
+
+// ------------------------------------------------------------------ fill
+__global__ void fill_u64_kernel(uint64_t *p, uint64_t v, size_t n) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = v;
+}
+__global__ void fill_u32_kernel(uint32_t *p, uint32_t v, size_t n) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = v;
+}
+
+// ------------------------------------------------------------------ scan
+// Exclusive scan of n u32 counts into out[0..n] (out[n] = total): per-block
+// sums, one block scanning the sums, per-block rescan with the offset.
+constexpr int SCAN_T = 256, SCAN_IT = 16, SCAN_B = SCAN_T * SCAN_IT;
+
+__device__ inline uint32_t block_excl_scan(uint32_t v, uint32_t *tmp, uint32_t &total) {
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    uint32_t x = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        uint32_t y = (uint32_t)__shfl_up((int)x, o);
+        if (lane >= o) x += y;
+    }
+    if (lane == 63) tmp[w] = x;
+    __syncthreads();
+    uint32_t off = 0, tot = 0;
+    for (int i = 0; i < (int)(blockDim.x >> 6); ++i) {
+        if (i < w) off += tmp[i];
+        tot += tmp[i];
+    }
+    __syncthreads();
+    total = tot;
+    return off + x - v;
+}
+
+__global__ void scan_sums_kernel(const uint32_t *in, uint32_t n, uint32_t *bsum) {
+    __shared__ uint32_t tmp[SCAN_T / 64];
+    const size_t b0 = (size_t)blockIdx.x * SCAN_B;
+    uint32_t s = 0;
+    for (int i = 0; i < SCAN_IT; ++i) {
+        size_t j = b0 + (size_t)i * SCAN_T + threadIdx.x;
+        if (j < n) s += in[j];
+    }
+    uint32_t tot;
+    (void)block_excl_scan(s, tmp, tot);
+    if (threadIdx.x == 0) bsum[blockIdx.x] = tot;
+}
+__global__ void scan_bsum_kernel(uint32_t *bsum, uint32_t nb) {
+    __shared__ uint32_t tmp[SCAN_T / 64];
+    uint32_t carry = 0;
+    for (uint32_t c0 = 0; c0 < nb; c0 += SCAN_T) {
+        uint32_t j = c0 + threadIdx.x;
+        uint32_t v = j < nb ? bsum[j] : 0;
+        uint32_t tot;
+        uint32_t ex = block_excl_scan(v, tmp, tot);
+        if (j < nb) bsum[j] = carry + ex;
+        carry += tot;
+    }
+}
+__global__ void scan_final_kernel(const uint32_t *in, uint32_t n, const uint32_t *bsum, uint32_t *out) {
+    __shared__ uint32_t tmp[SCAN_T / 64];
+    const size_t b0 = (size_t)blockIdx.x * SCAN_B;
+    // each thread owns SCAN_IT consecutive elements
+    const size_t j0 = b0 + (size_t)threadIdx.x * SCAN_IT;
+    uint32_t v[SCAN_IT], s = 0;
+#pragma unroll
+    for (int i = 0; i < SCAN_IT; ++i) {
+        v[i] = (j0 + i < n) ? in[j0 + i] : 0;
+        s += v[i];
+    }
+    uint32_t tot;
+    uint32_t run = bsum[blockIdx.x] + block_excl_scan(s, tmp, tot);
+#pragma unroll
+    for (int i = 0; i < SCAN_IT; ++i) {
+        if (j0 + i < n) out[j0 + i] = run;
+        run += v[i];
+    }
+    if (j0 < n && j0 + SCAN_IT >= n) out[n] = run;  // thread holding the last element
+}
+
+// ------------------------------------------------------------------ prep
+// Squared norms of the join rows; for BF16/FP8 also the quantised rows.
+// 16 lanes per row, chunk c of lane j = j + 16c.
+template <int DT>
+__global__ void nnd_prep_kernel(NndDev a, uint8_t *xq_out, float *norm_out, float *scale_out) {
+    const uint32_t row = (blockIdx.x * blockDim.x + threadIdx.x) >> 4;
+    const int sub = threadIdx.x & 15;
+    const bool live = row < a.N;
+    const uint32_t r = live ? row : 0;
+    const float4 *src = reinterpret_cast<const float4 *>(a.base + (size_t)r * a.dp);
+    const uint32_t nsrc = a.dp / 4;           // chunks present in the FP32 row
+    const uint32_t nch = (DT == DT_F32) ? nsrc : a.kdim / 4;
+    float acc = 0.f, amax = 0.f;
+    if (DT == DT_FP8) {
+        for (uint32_t c = sub; c < nsrc; c += 16) {
+            float4 v = src[c];
+            amax = fmaxf(amax, fmaxf(fmaxf(fabsf(v.x), fabsf(v.y)), fmaxf(fabsf(v.z), fabsf(v.w))));
+        }
+        amax = fmaxf(amax, __shfl_xor(amax, 8));
+        amax = fmaxf(amax, __shfl_xor(amax, 4));
+        amax = fmaxf(amax, __shfl_xor(amax, 2));
+        amax = fmaxf(amax, __shfl_xor(amax, 1));
+    }
+    const float sc = (DT == DT_FP8) ? (amax > 0.f ? amax / FP8_MAX : 1.0f) : 1.0f;
+    const float inv = 1.0f / sc;
+    for (uint32_t c = sub; c < nch; c += 16) {
+        float4 v = c < nsrc ? src[c] : make_float4(0.f, 0.f, 0.f, 0.f);
+        if (DT == DT_F32) {
+            acc = acc + (v.x * v.x + v.y * v.y + v.z * v.z + v.w * v.w);
+        } else if (DT == DT_BF16) {
+            __bf16 b0 = (__bf16)v.x, b1 = (__bf16)v.y, b2 = (__bf16)v.z, b3 = (__bf16)v.w;
+            float f0 = (float)b0, f1 = (float)b1, f2 = (float)b2, f3 = (float)b3;
+            acc = acc + (f0 * f0 + f1 * f1 + f2 * f2 + f3 * f3);
+            if (live) {
+                __bf16 *dst = reinterpret_cast<__bf16 *>(xq_out + (size_t)r * a.xq_stride) + 4 * c;
+                dst[0] = b0; dst[1] = b1; dst[2] = b2; dst[3] = b3;
+            }
+        } else {
+            float x0 = fminf(fmaxf(v.x * inv, -FP8_MAX), FP8_MAX), x1 = fminf(fmaxf(v.y * inv, -FP8_MAX), FP8_MAX);
+            float x2 = fminf(fmaxf(v.z * inv, -FP8_MAX), FP8_MAX), x3 = fminf(fmaxf(v.w * inv, -FP8_MAX), FP8_MAX);
+            int w = __builtin_amdgcn_cvt_pk_fp8_f32(x0, x1, 0, false);
+            w = __builtin_amdgcn_cvt_pk_fp8_f32(x2, x3, w, true);
+            float f0 = __builtin_amdgcn_cvt_f32_fp8(w, 0), f1 = __builtin_amdgcn_cvt_f32_fp8(w, 1);
+            float f2 = __builtin_amdgcn_cvt_f32_fp8(w, 2), f3 = __builtin_amdgcn_cvt_f32_fp8(w, 3);
+            acc = acc + (f0 * f0 + f1 * f1 + f2 * f2 + f3 * f3);
+            if (live) *reinterpret_cast<int *>(xq_out + (size_t)r * a.xq_stride + 4 * c) = w;
+        }
+    }
+    acc = fold16(acc);
+    if (live && sub == 0) {
+        norm_out[r] = (DT == DT_FP8) ? acc * sc * sc : acc;
+        if (DT == DT_FP8) scale_out[r] = sc;
+    }
+}
+
+// ------------------------------------------------------------------ M
+// update()'s sampling depth, src/component_refine.cpp:167-175: scan from the
+// front while l < min(M+S, size) and fewer than S new entries were seen.
+__device__ inline uint32_t compute_M(const uint8_t *flags, uint32_t Mprev, uint32_t S, uint32_t cnt) {
+    const int lane = lane_id();
+    const uint32_t maxl = min(Mprev + S, cnt);
+    uint32_t c = 0;
+    if (S == 0) return 0;
+    for (uint32_t t0 = 0; t0 < maxl; t0 += WAVE) {
+        const uint32_t p = t0 + lane;
+        const uint64_t b = __ballot(p < maxl && flags[p]);
+        const uint32_t pc = (uint32_t)__popcll(b);
+        if (c + pc >= S) {
+            uint64_t m = b;
+            for (uint32_t i = 1; i < S - c; ++i) m &= m - 1;
+            return t0 + (uint32_t)__ffsll((unsigned long long)m);  // position + 1
+        }
+        c += pc;
+    }
+    return maxl;
+}
+
+// ------------------------------------------------------------------ init
+// InitRandom: per node, S draws of GenRandom on the node's stream (as in
+// SearchEntryRand, a prefix-max scan forces them strictly increasing), self
+// skipped, build-order FP32 distances, ascending pool, all new, M = S.
+template <int CPL>
+__global__ void __launch_bounds__(256) nnd_init_kernel(NndDev a, const uint32_t *init_ids,
+                                                       const uint32_t *init_cnt) {
+    extern __shared__ __align__(16) uint8_t smem[];
+    const int wid = threadIdx.x >> 6, lane = lane_id();
+    const uint32_t n = blockIdx.x * (blockDim.x >> 6) + wid;
+    if (n >= a.N) return;
+    uint32_t *cid = reinterpret_cast<uint32_t *>(smem) + wid * 64;
+    uint64_t *ckey = reinterpret_cast<uint64_t *>(smem + (blockDim.x >> 6) * 256) + wid * 64;
+    const uint32_t S = a.S, N = a.N;
+    uint32_t id = ID_NONE;
+    if (init_ids) {
+        const uint32_t c = init_cnt ? init_cnt[n] : S;
+        if ((uint32_t)lane < c && (uint32_t)lane < S) id = init_ids[(size_t)n * S + lane];
+    } else {
+        const uint64_t s = stream_of(a.seed, TAG_INIT, n);
+        uint64_t v = (uint32_t)lane < S ? (uint64_t)(draw(s, lane) % (N - S)) : KEY_EMPTY;
+        v = wave_sort64(v);
+        int64_t b = (uint32_t)lane < S ? (int64_t)v - lane : INT64_MIN;
+        int64_t pm = wave_scan_max64(b);
+        const uint64_t off = draw(s, S) % N;
+        if ((uint32_t)lane < S) id = (uint32_t)(((uint64_t)(pm + lane) + off) % N);
+    }
+    const bool ok = id < N && id != n;
+    const uint64_t bm = __ballot(ok);
+    const int m = __popcll(bm);
+    if (ok) cid[lanes_below(bm)] = id;
+    wave_sync();
+    float4 qv[CPL];
+    {
+        const float4 *qp = reinterpret_cast<const float4 *>(a.base + (size_t)n * a.dp);
+#pragma unroll
+        for (int c = 0; c < CPL; ++c) {
+            const int ch = (lane & 15) + 16 * c;
+            qv[c] = ch < (int)(a.dp / 4) ? qp[ch] : make_float4(0.f, 0.f, 0.f, 0.f);
+        }
+    }
+    rows_dist<CPL, 4>(a.base, a.dp, a.dp / 4, qv, cid, m, ckey);
+    wave_sync();
+    uint64_t key = lane < m ? ckey[lane] : KEY_EMPTY;
+    key = wave_sort64(key);
+    const uint64_t last = shfl64(key, m > 0 ? m - 1 : 0);
+    // slot i < 64 is lane i's sorted key (KEY_EMPTY past m); the rest empty
+    for (uint32_t i = lane; i < a.L; i += WAVE) {
+        a.pool[(size_t)n * a.L + i] = i < 64 ? key : KEY_EMPTY;
+        a.pflag[(size_t)n * a.L + i] = i < (uint32_t)m ? 1 : 0;
+    }
+    if (lane == 0) {
+        a.pcnt[n] = (uint32_t)m;
+        a.M[n] = S;
+        a.thr[n] = m > 0 ? last : KEY_EMPTY;
+        a.worst[n] = m > 0 ? key_dist(last) : -1.0f;
+    }
+}
+
+__global__ void __launch_bounds__(256) nnd_m0_kernel(NndDev a) {
+    const int wid = threadIdx.x >> 6;
+    const uint32_t n = blockIdx.x * (blockDim.x >> 6) + wid;
+    if (n >= a.N) return;
+    const uint32_t m = compute_M(a.pflag + (size_t)n * a.L, a.M[n], a.S, a.pcnt[n]);
+    if (lane_id() == 0) a.M[n] = m;
+}
+
+// ------------------------------------------------------------------ sample
+// update() :176-209 forward part: first M pool entries, flagged ones to the
+// new list (flag cleared), the rest to the old list, in pool order; count a
+// reverse proposal at the neighbour when dist > neighbour's worst.
+__global__ void __launch_bounds__(256) nnd_sample_kernel(NndDev a) {
+    const int wid = threadIdx.x >> 6, lane = lane_id();
+    const uint32_t n = blockIdx.x * (blockDim.x >> 6) + wid;
+    if (n >= a.N) return;
+    const uint32_t Mn = a.M[n];
+    const uint64_t *pr = a.pool + (size_t)n * a.L;
+    uint8_t *fl = a.pflag + (size_t)n * a.L;
+    uint32_t nf = 0, no = 0;
+    for (uint32_t t0 = 0; t0 < Mn; t0 += WAVE) {
+        const uint32_t p = t0 + lane;
+        const bool act = p < Mn;
+        const uint64_t key = act ? pr[p] : KEY_EMPTY;
+        const bool f = act && fl[p];
+        const uint64_t bn = __ballot(f), bo = __ballot(act && !f);
+        if (f) a.fnew[(size_t)n * a.S + nf + lanes_below(bn)] = key;
+        if (act && !f) a.fold[(size_t)n * a.L + no + lanes_below(bo)] = key;
+        if (act) {
+            const uint32_t id = key_id(key);
+            if (key_dist(key) > a.worst[id]) atomicAdd(f ? &a.rcnt_new[id] : &a.rcnt_old[id], 1u);
+            fl[p] = 0;
+        }
+        nf += (uint32_t)__popcll(bn);
+        no += (uint32_t)__popcll(bo);
+    }
+    if (lane == 0) {
+        a.fnew_cnt[n] = nf;
+        a.fold_cnt[n] = no;
+    }
+}
+
+__global__ void __launch_bounds__(256) nnd_rev_scatter_kernel(NndDev a) {
+    const int wid = threadIdx.x >> 6, lane = lane_id();
+    const uint32_t n = blockIdx.x * (blockDim.x >> 6) + wid;
+    if (n >= a.N) return;
+    const uint32_t nf = a.fnew_cnt[n], no = a.fold_cnt[n];
+    for (uint32_t i = lane; i < nf + no; i += WAVE) {
+        const bool isnew = i < nf;
+        const uint64_t key = isnew ? a.fnew[(size_t)n * a.S + i] : a.fold[(size_t)n * a.L + (i - nf)];
+        const uint32_t id = key_id(key);
+        if (!(key_dist(key) > a.worst[id])) continue;
+        if (isnew) {
+            const uint32_t pos = atomicAdd(&a.rcur_new[id], 1u);
+            a.rv_new[a.roff_new[id] + pos] = n;
+        } else {
+            const uint32_t pos = atomicAdd(&a.rcur_old[id], 1u);
+            a.rv_old[a.roff_old[id] + pos] = n;
+        }
+    }
+}
+
+// Reverse-sample priority key of proposer p at node x (hash high word, p low
+// word); the R smallest are kept.
+__device__ inline uint64_t rev_key(uint64_t seed, uint64_t tag, uint32_t iter, uint32_t x, uint32_t p) {
+    uint64_t h = fmix64(stream_of(seed, tag, ((uint64_t)iter << 32) | x) ^ ((uint64_t)p * GOLDEN));
+    return ((h >> 32) << 32) | p;
+}
+
+// Keep at most R reverse proposals of node x (update() :190-208, :213-228):
+// all when there are at most R, else the R smallest rev_key (radix select
+// of the R-th smallest over the segment); the kept ids are sorted ascending.
+__global__ void __launch_bounds__(256) nnd_rev_select_kernel(NndDev a, uint32_t rp2) {
+    extern __shared__ __align__(16) uint8_t smem[];
+    const int wid = threadIdx.x >> 6, lane = lane_id();
+    const uint32_t x = blockIdx.x * (blockDim.x >> 6) + wid;
+    if (x >= a.N) return;
+    uint8_t *my = smem + (size_t)wid * (rp2 * 4 + 256 * 4 + 16);
+    uint32_t *ids = reinterpret_cast<uint32_t *>(my);
+    uint32_t *hist = ids + rp2;
+    uint32_t *cnt = hist + 256;
+    for (int which = 0; which < 2; ++which) {
+        const uint32_t *off = which ? a.roff_old : a.roff_new;
+        const uint32_t *seg = (which ? a.rv_old : a.rv_new) + off[x];
+        const uint32_t c = off[x + 1] - off[x];
+        const uint64_t tag = which ? TAG_REVOLD : TAG_REVNEW;
+        const uint32_t R = a.R;
+        uint32_t kept;
+        if (c <= R) {
+            for (uint32_t i = lane; i < rp2; i += WAVE) ids[i] = i < c ? seg[i] : ID_NONE;
+            kept = c;
+        } else {
+            uint64_t prefix = 0, pmask = 0;
+            uint32_t need = R;
+            for (int shift = 56; shift >= 0; shift -= 8) {
+                for (int i = lane; i < 256; i += WAVE) hist[i] = 0;
+                wave_sync();
+                for (uint32_t i = lane; i < c; i += WAVE) {
+                    const uint64_t k = rev_key(a.seed, tag, a.iter, x, seg[i]);
+                    if ((k & pmask) == prefix) atomicAdd(&hist[(k >> shift) & 255], 1u);
+                }
+                wave_sync();
+                // bin where the running count reaches `need`
+                uint32_t h4[4], s4 = 0;
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    h4[j] = hist[lane * 4 + j];
+                    s4 += h4[j];
+                }
+                uint32_t incl = s4;
+#pragma unroll
+                for (int o = 1; o < 64; o <<= 1) {
+                    uint32_t y = (uint32_t)__shfl_up((int)incl, o);
+                    if (lane >= o) incl += y;
+                }
+                uint32_t before = incl - s4;
+                int bin = -1;
+                uint32_t below = 0;
+                if (before < need && need <= incl) {
+                    uint32_t run = before;
+                    for (int j = 0; j < 4; ++j) {
+                        if (run + h4[j] >= need) {
+                            bin = lane * 4 + j;
+                            below = run;
+                            break;
+                        }
+                        run += h4[j];
+                    }
+                }
+                const uint64_t bb = __ballot(bin >= 0);
+                const int src = __ffsll((unsigned long long)bb) - 1;
+                bin = __shfl(bin, src);
+                below = (uint32_t)__shfl((int)below, src);
+                prefix |= (uint64_t)bin << shift;
+                pmask |= (uint64_t)255 << shift;
+                need -= below;
+                wave_sync();
+            }
+            // keys are distinct (low word = proposer id): exactly R are <= prefix
+            if (lane == 0) *cnt = 0;
+            for (uint32_t i = lane; i < rp2; i += WAVE) ids[i] = ID_NONE;
+            wave_sync();
+            for (uint32_t i = lane; i < c; i += WAVE) {
+                const uint32_t p = seg[i];
+                if (rev_key(a.seed, tag, a.iter, x, p) <= prefix) {
+                    const uint32_t pos = atomicAdd(cnt, 1u);
+                    if (pos < rp2) ids[pos] = p;
+                }
+            }
+            kept = R;
+        }
+        wave_sync();
+        lds_sort32(ids, (int)rp2);
+        uint32_t *dst = (which ? a.rsel_old : a.rsel_new) + (size_t)x * R;
+        for (uint32_t i = lane; i < kept; i += WAVE) dst[i] = ids[i];
+        if (lane == 0) (which ? a.rsel_old_cnt : a.rsel_new_cnt)[x] = kept;
+        wave_sync();
+    }
+}
+
+// ------------------------------------------------------------------ join
+template <int DT> struct JoinT;
+template <> struct JoinT<DT_F32> { static constexpr int ES = 4, KC = 128, KSTEP = 16; };
+template <> struct JoinT<DT_BF16> { static constexpr int ES = 2, KC = 256, KSTEP = 32; };
+template <> struct JoinT<DT_FP8> { static constexpr int ES = 1, KC = 512, KSTEP = 32; };
+
+// Stage join rows list[r0 .. r0+16) (zero rows past `count`), elements
+// [k0, k0 + klen) with klen a multiple of KSTEP, into a 16 x TILE_ST tile.
+template <int DT>
+__device__ inline void stage_tile(const NndDev &a, uint8_t *tile, const uint32_t *list, int r0,
+                                  int count, uint32_t k0, uint32_t klen) {
+    constexpr int ES = JoinT<DT>::ES;
+    const int lane = lane_id();
+    const int vpr = (int)(klen * ES / 16);  // 16-byte vectors per row
+    const int epv = 16 / ES;                // elements per vector
+    for (int v = lane; v < 16 * vpr; v += WAVE) {
+        const int r = v / vpr, cv = v - r * vpr;
+        const uint32_t e0 = k0 + (uint32_t)(cv * epv);
+        uint4 val = make_uint4(0, 0, 0, 0);
+        if (r0 + r < count && e0 < a.kdim) {
+            const uint32_t id = list[r0 + r];
+            val = *reinterpret_cast<const uint4 *>(a.xq + (size_t)id * a.xq_stride + (size_t)e0 * ES);
+        }
+        *reinterpret_cast<uint4 *>(tile + r * TILE_ST + cv * 16) = val;
+    }
+}
+
+template <int DT>
+__device__ inline floatx4 tile_mfma(const uint8_t *A, const uint8_t *B, uint32_t klen, floatx4 acc) {
+    const int lane = lane_id(), r = lane & 15, g = lane >> 4;
+    if (DT == DT_F32) {
+        for (uint32_t t = 0; t < klen / 16; ++t) {
+            const float4 x = *reinterpret_cast<const float4 *>(A + r * TILE_ST + (16 * t + 4 * g) * 4);
+            const float4 y = *reinterpret_cast<const float4 *>(B + r * TILE_ST + (16 * t + 4 * g) * 4);
+            acc = __builtin_amdgcn_mfma_f32_16x16x4f32(x.x, y.x, acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_16x16x4f32(x.y, y.y, acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_16x16x4f32(x.z, y.z, acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_16x16x4f32(x.w, y.w, acc, 0, 0, 0);
+        }
+    } else if (DT == DT_BF16) {
+        for (uint32_t t = 0; t < klen / 32; ++t) {
+            const bf16x8 x = *reinterpret_cast<const bf16x8 *>(A + r * TILE_ST + (32 * t + 8 * g) * 2);
+            const bf16x8 y = *reinterpret_cast<const bf16x8 *>(B + r * TILE_ST + (32 * t + 8 * g) * 2);
+            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(x, y, acc, 0, 0, 0);
+        }
+    } else {
+        for (uint32_t t = 0; t < klen / 32; ++t) {
+            const long x = *reinterpret_cast<const long *>(A + r * TILE_ST + 32 * t + 8 * g);
+            const long y = *reinterpret_cast<const long *>(B + r * TILE_ST + 32 * t + 8 * g);
+            acc = __builtin_amdgcn_mfma_f32_16x16x32_fp8_fp8(x, y, acc, 0, 0, 0);
+        }
+    }
+    return acc;
+}
+
+__device__ inline bool pooled(const NndDev &a, uint32_t t, uint32_t id) {
+    const uint32_t *h = a.phash + (size_t)t * a.HP;
+    const uint32_t mask = a.HP - 1;
+    uint32_t i = fmix32(id) & mask;
+    for (;;) {
+        const uint32_t e = h[i];
+        if (e == id) return true;
+        if (e == ID_NONE) return false;
+        i = (i + 1) & mask;
+    }
+}
+
+__device__ inline bool propose(const NndDev &a, uint32_t t, uint32_t id, float d, uint64_t thr) {
+    const uint64_t key = make_key(d, id);
+    if (key > thr) return false;
+    if (pooled(a, t, id)) return false;
+    unsigned long long *slot =
+        reinterpret_cast<unsigned long long *>(a.cand + (size_t)t * a.C + cand_slot(t, id, a.C - 1));
+    if (*slot > key) atomicMin(slot, (unsigned long long)key);
+    return true;
+}
+
+// join(), src/component_refine.cpp:127-143 / nhood::join,
+// include/weavess/index.h:137-149, one wavefront per node.
+template <int DT>
+__global__ void __launch_bounds__(256) nnd_join_kernel(NndDev a) {
+    extern __shared__ __align__(16) uint8_t smem[];
+    constexpr int KC = JoinT<DT>::KC, KSTEP = JoinT<DT>::KSTEP;
+    const int wid = threadIdx.x >> 6, lane = lane_id();
+    const uint32_t n = blockIdx.x * (blockDim.x >> 6) + wid;
+    if (n >= a.N) return;
+    const uint32_t ncap = a.S + 3 * a.R;  // |new| <= S+R, |old| <= 2R
+    uint8_t *my = smem + (size_t)wid * (a.lds_join_list + 2 * 16 * TILE_ST);
+    uint32_t *nw = reinterpret_cast<uint32_t *>(my);              // nw_pow2
+    uint32_t *col = nw + a.nw_pow2;                               // ncap
+    uint64_t *cthr = reinterpret_cast<uint64_t *>(col + ((ncap + 1) & ~1u));
+    float *cnorm = reinterpret_cast<float *>(cthr + ncap);
+    float *cscale = cnorm + ncap;
+    uint8_t *tA = my + a.lds_join_list;
+    uint8_t *tB = tA + 16 * TILE_ST;
+
+    const uint32_t fnc = a.fnew_cnt[n], rnc = a.rsel_new_cnt[n];
+    const uint32_t foc = a.fold_cnt[n], roc = a.rsel_old_cnt[n];
+    const uint32_t nn = fnc + rnc;
+    const uint32_t no = min(foc + roc, 2 * a.R);
+    if (nn == 0) return;
+    for (uint32_t i = lane; i < a.nw_pow2; i += WAVE)
+        nw[i] = i < fnc ? key_id(a.fnew[(size_t)n * a.S + i])
+              : i < nn ? a.rsel_new[(size_t)n * a.R + (i - fnc)] : ID_NONE;
+    wave_sync();
+    lds_sort32(nw, (int)a.nw_pow2);  // ascending ids: position order == id order
+    const uint32_t nc = nn + no;
+    for (uint32_t i = lane; i < nc; i += WAVE) {
+        uint32_t id;
+        if (i < nn) id = nw[i];
+        else if (i - nn < foc) id = key_id(a.fold[(size_t)n * a.L + (i - nn)]);
+        else id = a.rsel_old[(size_t)n * a.R + (i - nn - foc)];
+        col[i] = id;
+        cthr[i] = a.thr[id];
+        cnorm[i] = a.norm[id];
+        cscale[i] = (DT == DT_FP8) ? a.scale[id] : 1.0f;
+    }
+    wave_sync();
+
+    const uint32_t kdim = a.kdim;
+    const uint32_t nkc = (kdim + KC - 1) / KC;
+    const int nrt = (int)((nn + 15) / 16), nct = (int)((nc + 15) / 16);
+    const int g = lane >> 4, cl = lane & 15;
+    uint32_t pairs = 0, props = 0;
+    for (int rt = 0; rt < nrt; ++rt) {
+        if (nkc == 1) {
+            stage_tile<DT>(a, tA, col, rt * 16, (int)nn, 0, ((kdim + KSTEP - 1) / KSTEP) * KSTEP);
+        }
+        for (int ct = rt; ct < nct; ++ct) {
+            floatx4 acc = {0.f, 0.f, 0.f, 0.f};
+            for (uint32_t kc = 0; kc < nkc; ++kc) {
+                const uint32_t k0 = kc * KC;
+                uint32_t klen = min((uint32_t)KC, kdim - k0);
+                klen = ((klen + KSTEP - 1) / KSTEP) * KSTEP;
+                if (nkc > 1) stage_tile<DT>(a, tA, col, rt * 16, (int)nn, k0, klen);
+                stage_tile<DT>(a, tB, col, ct * 16, (int)nc, k0, klen);
+                wave_sync();
+                acc = tile_mfma<DT>(tA, tB, klen, acc);
+                wave_sync();
+            }
+            // epilogue: lane holds rows 16rt+4g+i, column 16ct+cl
+            const uint32_t b = (uint32_t)(ct * 16 + cl);
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const uint32_t ar = (uint32_t)(rt * 16 + 4 * g + i);
+                if (ar >= nn || b >= nc) continue;
+                if (b < nn && !(ar < b)) continue;  // new x new: i < j once
+                const uint32_t ia = col[ar], ib = col[b];
+                if (ia == ib) continue;
+                float dot = acc[i];
+                if (DT == DT_FP8) dot = dot * (cscale[ar] * cscale[b]);
+                float d = (cnorm[ar] + cnorm[b]) - 2.0f * dot;
+                d = fmaxf(d, 0.0f);
+                ++pairs;
+                props += propose(a, ia, ib, d, cthr[ar]);
+                props += propose(a, ib, ia, d, cthr[b]);
+            }
+        }
+    }
+    pairs = wave_sum_u32(pairs);
+    props = wave_sum_u32(props);
+    if (lane == 0) {
+        atomicAdd(&a.ctr[0], (unsigned long long)pairs);
+        atomicAdd(&a.ctr[1], (unsigned long long)props);
+        atomicAdd(&a.ctr[2], (unsigned long long)nc);
+    }
+}
+
+// ------------------------------------------------------------------ pool hash
+// Rebuild phash[n] from the pool (load factor <= 1/2, so probes are short
+// and every lookup terminates at an empty slot).
+__global__ void __launch_bounds__(256) nnd_hash_kernel(NndDev a) {
+    extern __shared__ __align__(16) uint8_t smem[];
+    const int wid = threadIdx.x >> 6, lane = lane_id();
+    const uint32_t n = blockIdx.x * (blockDim.x >> 6) + wid;
+    if (n >= a.N) return;
+    uint32_t *hk = reinterpret_cast<uint32_t *>(smem) + (size_t)wid * a.HP;
+    const uint32_t mask = a.HP - 1;
+    for (uint32_t i = lane; i < a.HP; i += WAVE) hk[i] = ID_NONE;
+    wave_sync();
+    const uint32_t cnt = a.pcnt[n];
+    for (uint32_t i = lane; i < cnt; i += WAVE) {
+        const uint32_t id = key_id(a.pool[(size_t)n * a.L + i]);
+        uint32_t h = fmix32(id) & mask;
+        while (atomicCAS(&hk[h], ID_NONE, id) != ID_NONE) h = (h + 1) & mask;
+    }
+    wave_sync();
+    uint32_t *dst = a.phash + (size_t)n * a.HP;
+    for (uint32_t i = lane; i < a.HP; i += WAVE) dst[i] = hk[i];
+}
+
+// ------------------------------------------------------------------ merge
+__host__ __device__ inline size_t merge_wave_lds(uint32_t up2, uint32_t hp2, uint32_t L) {
+    return ((size_t)up2 * 8 + (size_t)hp2 * 5 + L + 16 + 15) & ~(size_t)15;
+}
+// Fold each node's slot winners into its pool (index.h:120-135 applied to
+// the whole batch at once; see header comment), then compute the next M.
+__global__ void __launch_bounds__(256) nnd_merge_kernel(NndDev a, uint32_t up2, uint32_t hp2) {
+    extern __shared__ __align__(16) uint8_t smem[];
+    const int wid = threadIdx.x >> 6, lane = lane_id();
+    const uint32_t n = blockIdx.x * (blockDim.x >> 6) + wid;
+    if (n >= a.N) return;
+    uint8_t *my = smem + (size_t)wid * merge_wave_lds(up2, hp2, a.L);
+    uint64_t *u = reinterpret_cast<uint64_t *>(my);
+    uint32_t *hk = reinterpret_cast<uint32_t *>(u + up2);
+    uint8_t *hf = reinterpret_cast<uint8_t *>(hk + hp2);
+    uint8_t *nf = hf + hp2;
+    uint32_t *ucnt = reinterpret_cast<uint32_t *>(nf + ((a.L + 3) & ~3u));
+    const uint32_t hmask = hp2 - 1;
+    const uint32_t cnt = a.pcnt[n];
+    const uint64_t *pr = a.pool + (size_t)n * a.L;
+    const uint8_t *pf = a.pflag + (size_t)n * a.L;
+    for (uint32_t i = lane; i < hp2; i += WAVE) hk[i] = ID_NONE;
+    if (lane == 0) *ucnt = 0;
+    wave_sync();
+    for (uint32_t i = lane; i < cnt; i += WAVE) {
+        const uint64_t k = pr[i];
+        u[i] = k;
+        uint32_t h = fmix32(key_id(k)) & hmask;
+        while (atomicCAS(&hk[h], ID_NONE, key_id(k)) != ID_NONE) h = (h + 1) & hmask;
+        hf[h] = pf[i];
+    }
+    wave_sync();
+    uint64_t *cs = a.cand + (size_t)n * a.C;
+    for (uint32_t s = lane; s < a.C; s += WAVE) {
+        const uint64_t k = cs[s];
+        if (k == KEY_EMPTY) continue;
+        cs[s] = KEY_EMPTY;
+        const uint32_t id = key_id(k);
+        uint32_t h = fmix32(id) & hmask;
+        bool dup = false;
+        for (;;) {
+            const uint32_t e = hk[h];
+            if (e == id) { dup = true; break; }
+            if (e == ID_NONE) break;
+            h = (h + 1) & hmask;
+        }
+        if (!dup) u[cnt + atomicAdd(ucnt, 1u)] = k;
+    }
+    wave_sync();
+    const uint32_t total = cnt + *ucnt;
+    uint32_t P = 64;
+    while (P < total) P <<= 1;
+    for (uint32_t i = total + lane; i < P; i += WAVE) u[i] = KEY_EMPTY;
+    wave_sync();
+    lds_sort64(u, (int)P);
+    const uint32_t keep = min(total, a.L);
+    uint32_t fresh = 0;
+    for (uint32_t i = lane; i < a.L; i += WAVE) {
+        const uint64_t k = i < keep ? u[i] : KEY_EMPTY;
+        uint8_t f = 0;
+        if (i < keep) {
+            const uint32_t id = key_id(k);
+            uint32_t h = fmix32(id) & hmask;
+            f = 1;
+            for (;;) {
+                const uint32_t e = hk[h];
+                if (e == id) { f = hf[h]; break; }
+                if (e == ID_NONE) { ++fresh; break; }
+                h = (h + 1) & hmask;
+            }
+        }
+        a.pool[(size_t)n * a.L + i] = k;
+        a.pflag[(size_t)n * a.L + i] = f;
+        nf[i] = f;
+    }
+    wave_sync();
+    const uint32_t m = compute_M(nf, a.M[n], a.S, keep);
+    fresh = wave_sum_u32(fresh);
+    if (lane == 0) {
+        a.pcnt[n] = keep;
+        a.thr[n] = keep ? u[keep - 1] : KEY_EMPTY;
+        a.worst[n] = keep ? key_dist(u[keep - 1]) : -1.0f;
+        a.M[n] = m;
+        atomicAdd(&a.ctr[3], (unsigned long long)fresh);
+    }
+}
+
+// ------------------------------------------------------------------ finalize
+// RefineInner :22-74 + PruneNaive: exact FP32 build-order distances for the
+// first `rerank` pool entries, ascending, first K kept.
+template <int CPL>
+__global__ void __launch_bounds__(256) nnd_finalize_kernel(NndDev a, uint32_t rp2) {
+    extern __shared__ __align__(16) uint8_t smem[];
+    const int wid = threadIdx.x >> 6, lane = lane_id();
+    const uint32_t n = blockIdx.x * (blockDim.x >> 6) + wid;
+    if (n >= a.N) return;
+    uint8_t *my = smem + (size_t)wid * (rp2 * 12 + 16);
+    uint64_t *keys = reinterpret_cast<uint64_t *>(my);
+    uint32_t *ids = reinterpret_cast<uint32_t *>(keys + rp2);
+    const uint32_t cnt = a.pcnt[n];
+    const uint32_t rr = min(cnt, a.rerank);
+    for (uint32_t i = lane; i < rr; i += WAVE) ids[i] = key_id(a.pool[(size_t)n * a.L + i]);
+    wave_sync();
+    float4 qv[CPL];
+    {
+        const float4 *qp = reinterpret_cast<const float4 *>(a.base + (size_t)n * a.dp);
+#pragma unroll
+        for (int c = 0; c < CPL; ++c) {
+            const int ch = (lane & 15) + 16 * c;
+            qv[c] = ch < (int)(a.dp / 4) ? qp[ch] : make_float4(0.f, 0.f, 0.f, 0.f);
+        }
+    }
+    rows_dist<CPL, 4>(a.base, a.dp, a.dp / 4, qv, ids, (int)rr, keys);
+    for (uint32_t i = rr + lane; i < rp2; i += WAVE) keys[i] = KEY_EMPTY;
+    wave_sync();
+    lds_sort64(keys, (int)rp2);
+    const uint32_t deg = min(a.K, rr);
+    for (uint32_t i = lane; i < a.K; i += WAVE) {
+        const bool ok = i < deg;
+        a.out_ids[(size_t)n * a.K + i] = ok ? key_id(keys[i]) : ID_NONE;
+        a.out_dist[(size_t)n * a.K + i] = ok ? key_dist(keys[i]) : -1.0f;
+    }
+    if (lane == 0) a.out_deg[n] = deg;
+}
+
+// ------------------------------------------------------------------ host
+uint32_t pow2_at_least(uint32_t v, uint32_t lo) {
+    uint32_t p = lo;
+    while (p < v) p <<= 1;
+    return p;
+}
+
+struct Bufs {
+    std::vector<void *> ptrs;
+    ~Bufs() {
+        for (void *p : ptrs) (void)hipFree(p);
+    }
+    template <class T> hipError_t get(T **p, size_t n) {
+        void *q = nullptr;
+        hipError_t e = hipMalloc(&q, std::max<size_t>(n * sizeof(T), 16));
+        if (e == hipSuccess) {
+            ptrs.push_back(q);
+            *p = static_cast<T *>(q);
+        }
+        return e;
+    }
+};
+
+constexpr int WPB = 4;       // wavefronts (nodes) per workgroup
+constexpr int JOIN_WPB = 2;  // the join holds ~22 KB of LDS per wavefront
+
+hipError_t scan_u32(const uint32_t *in, uint32_t n, uint32_t *out, uint32_t *bsum, hipStream_t s) {
+    const uint32_t nb = (n + SCAN_B - 1) / SCAN_B;
+    hipLaunchKernelGGL(scan_sums_kernel, dim3(nb), dim3(SCAN_T), 0, s, in, n, bsum);
+    hipLaunchKernelGGL(scan_bsum_kernel, dim3(1), dim3(SCAN_T), 0, s, bsum, nb);
+    hipLaunchKernelGGL(scan_final_kernel, dim3(nb), dim3(SCAN_T), 0, s, in, n, bsum, out);
+    return hipGetLastError();
+}
+
+template <class K>
+hipError_t set_lds(K kernel, size_t bytes) {
+    if (bytes <= 65536) return hipSuccess;
+    return hipFuncSetAttribute(reinterpret_cast<const void *>(kernel),
+                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+}
+
+#define NND_HIP(call)                                                                        \
+    do {                                                                                     \
+        hipError_t e_ = (call);                                                              \
+        if (e_ != hipSuccess) {                                                              \
+            char m_[256];                                                                    \
+            snprintf(m_, sizeof m_, "%s: %s (%s:%d)", #call, hipGetErrorString(e_), __FILE__, \
+                     __LINE__);                                                              \
+            return index_set_error(ix, e_ == hipErrorOutOfMemory ? WV_ENOMEM : WV_EHIP, m_); \
+        }                                                                                    \
+    } while (0)
+
+template <int CPL>
+hipError_t launch_init(const NndDev &a, const uint32_t *ii, const uint32_t *ic, uint32_t blocks, hipStream_t s) {
+    hipLaunchKernelGGL(nnd_init_kernel<CPL>, dim3(blocks), dim3(64 * WPB), WPB * (256 + 512), s, a, ii, ic);
+    return hipGetLastError();
+}
+template <int CPL>
+hipError_t launch_final(const NndDev &a, uint32_t rp2, uint32_t blocks, hipStream_t s) {
+    const size_t lds = (size_t)WPB * (rp2 * 12 + 16);
+    hipError_t e = set_lds(nnd_finalize_kernel<CPL>, lds);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(nnd_finalize_kernel<CPL>, dim3(blocks), dim3(64 * WPB), lds, s, a, rp2);
+    return hipGetLastError();
+}
+
+}  // namespace
+
+int nndescent_build(wv_index *ix, const wv_nnd_params *p, const uint32_t *init_ids,
+                    const uint32_t *init_cnt, wv_build_stats *stats, uint32_t *graph_out,
+                    float *gdist_out, uint32_t *deg_out) {
+    const IndexView v = index_view(ix);
+    const hipStream_t s = v.stream;
+    const uint32_t N = v.N;
+    if (p->dtype > 2) return index_set_error(ix, WV_EINVAL, "dtype must be WV_DTYPE_F32, _BF16 or _FP8");
+    const int dt = (int)p->dtype;
+    const uint32_t L = p->L, S = p->S, R = p->R, K = p->K;
+    const uint32_t nch = v.dp / 4;
+    const int cpl = nch <= 16 ? 1 : nch <= 32 ? 2 : nch <= 64 ? 4 : nch <= 128 ? 8 : nch <= 256 ? 16 : 0;
+    if (!cpl) return index_set_error(ix, WV_ENOTSUP, "dimension above 1024");
+    uint32_t C = p->cand_slots ? p->cand_slots : std::min<uint32_t>(1024, pow2_at_least(4 * L, 64));
+    if (C & (C - 1)) return index_set_error(ix, WV_EINVAL, "cand_slots must be a power of two");
+    const uint32_t rer = std::min(L, std::max(K, (p->rerank_mult ? p->rerank_mult : (dt ? 2u : 1u)) * K));
+
+    Bufs B;
+    NndDev a{};
+    a.base = v.base;
+    a.dp = v.dp;
+    a.N = N; a.L = L; a.S = S; a.R = R; a.K = K; a.C = C; a.rerank = rer;
+    a.seed = p->seed;
+    float *norm = nullptr, *scale = nullptr;
+    uint8_t *xq = nullptr;
+    NND_HIP(B.get(&norm, N));
+    if (dt == DT_F32) {
+        a.xq = reinterpret_cast<const uint8_t *>(v.base);
+        a.xq_stride = v.dp * 4;
+        a.kdim = v.dp;
+    } else {
+        a.kdim = (v.dp + 31) & ~31u;
+        a.xq_stride = a.kdim * (dt == DT_BF16 ? 2 : 1);
+        NND_HIP(B.get(&xq, (size_t)N * a.xq_stride));
+        NND_HIP(hipMemsetAsync(xq, 0, (size_t)N * a.xq_stride, s));
+        a.xq = xq;
+        if (dt == DT_FP8) NND_HIP(B.get(&scale, N));
+    }
+    a.norm = norm;
+    a.scale = scale;
+    NND_HIP(B.get(&a.pool, (size_t)N * L));
+    NND_HIP(B.get(&a.pflag, (size_t)N * L));
+    NND_HIP(B.get(&a.pcnt, N));
+    NND_HIP(B.get(&a.M, N));
+    NND_HIP(B.get(&a.thr, N));
+    NND_HIP(B.get(&a.worst, N));
+    NND_HIP(B.get(&a.fnew, (size_t)N * S));
+    NND_HIP(B.get(&a.fold, (size_t)N * L));
+    NND_HIP(B.get(&a.fnew_cnt, N));
+    NND_HIP(B.get(&a.fold_cnt, N));
+    NND_HIP(B.get(&a.rcnt_new, N));
+    NND_HIP(B.get(&a.rcnt_old, N));
+    NND_HIP(B.get(&a.rcur_new, N));
+    NND_HIP(B.get(&a.rcur_old, N));
+    NND_HIP(B.get(&a.roff_new, (size_t)N + 1));
+    NND_HIP(B.get(&a.roff_old, (size_t)N + 1));
+    NND_HIP(B.get(&a.rv_new, (size_t)N * L));
+    NND_HIP(B.get(&a.rv_old, (size_t)N * L));
+    NND_HIP(B.get(&a.rsel_new, (size_t)N * std::max(R, 1u)));
+    NND_HIP(B.get(&a.rsel_old, (size_t)N * std::max(R, 1u)));
+    NND_HIP(B.get(&a.rsel_new_cnt, N));
+    NND_HIP(B.get(&a.rsel_old_cnt, N));
+    NND_HIP(B.get(&a.cand, (size_t)N * C));
+    a.HP = pow2_at_least(2 * L, 64);
+    NND_HIP(B.get(&a.phash, (size_t)N * a.HP));
+    const size_t hash_lds = (size_t)WPB * a.HP * 4;
+    NND_HIP(B.get(&a.ctr, 4));
+    uint32_t *bsum = nullptr;
+    NND_HIP(B.get(&bsum, (N + SCAN_B - 1) / SCAN_B + 1));
+    a.out_ids = graph_out;
+    a.out_dist = gdist_out;
+    a.out_deg = deg_out;
+
+    const uint32_t ncap = S + 3 * R;
+    a.nw_pow2 = pow2_at_least(S + R, 64);
+    a.lds_join_list = (uint32_t)(((a.nw_pow2 * 4 + ((ncap + 1) & ~1u) * 4 + ncap * 16) + 15) & ~15u);
+    const size_t join_lds = (size_t)JOIN_WPB * (a.lds_join_list + 2 * 16 * TILE_ST);
+    if (join_lds > 160 * 1024)
+        return index_set_error(ix, WV_ENOTSUP, "S/R too large for the join's LDS lists");
+    const uint32_t up2 = pow2_at_least(L + C, 64), hp2 = pow2_at_least(2 * L, 64);
+    const size_t merge_lds = (size_t)WPB * merge_wave_lds(up2, hp2, L);
+    if (merge_lds > 160 * 1024) return index_set_error(ix, WV_ENOTSUP, "L + cand_slots too large for LDS");
+    const uint32_t rp2 = pow2_at_least(R, 64);
+    const size_t sel_lds = (size_t)WPB * (rp2 * 4 + 256 * 4 + 16);
+    const uint32_t fp2 = pow2_at_least(rer, 64);
+
+    const uint32_t blocks = (N + WPB - 1) / WPB;
+    hipEvent_t ev[8];
+    for (auto &e : ev) NND_HIP(hipEventCreate(&e));
+    struct EvGuard {
+        hipEvent_t *e;
+        ~EvGuard() {
+            for (int i = 0; i < 8; ++i) (void)hipEventDestroy(e[i]);
+        }
+    } evg{ev};
+    double t_sample = 0, t_join = 0, t_merge = 0;
+
+    NND_HIP(hipEventRecord(ev[0], s));
+    hipLaunchKernelGGL(fill_u64_kernel, dim3(2048), dim3(256), 0, s, a.cand, KEY_EMPTY, (size_t)N * C);
+    hipLaunchKernelGGL(fill_u64_kernel, dim3(1), dim3(64), 0, s, reinterpret_cast<uint64_t *>(a.ctr), 0ull, (size_t)4);
+    {
+        const uint32_t pb = (N * 16 + 255) / 256;
+        if (dt == DT_F32)
+            hipLaunchKernelGGL(nnd_prep_kernel<DT_F32>, dim3(pb), dim3(256), 0, s, a, xq, norm, scale);
+        else if (dt == DT_BF16)
+            hipLaunchKernelGGL(nnd_prep_kernel<DT_BF16>, dim3(pb), dim3(256), 0, s, a, xq, norm, scale);
+        else
+            hipLaunchKernelGGL(nnd_prep_kernel<DT_FP8>, dim3(pb), dim3(256), 0, s, a, xq, norm, scale);
+        NND_HIP(hipGetLastError());
+    }
+    uint32_t *d_init = nullptr, *d_icnt = nullptr;
+    if (init_ids) {
+        NND_HIP(B.get(&d_init, (size_t)N * S));
+        NND_HIP(hipMemcpyAsync(d_init, init_ids, (size_t)N * S * 4, hipMemcpyHostToDevice, s));
+        if (init_cnt) {
+            NND_HIP(B.get(&d_icnt, N));
+            NND_HIP(hipMemcpyAsync(d_icnt, init_cnt, (size_t)N * 4, hipMemcpyHostToDevice, s));
+        }
+    }
+    switch (cpl) {
+    case 1: NND_HIP(launch_init<1>(a, d_init, d_icnt, blocks, s)); break;
+    case 2: NND_HIP(launch_init<2>(a, d_init, d_icnt, blocks, s)); break;
+    case 4: NND_HIP(launch_init<4>(a, d_init, d_icnt, blocks, s)); break;
+    case 8: NND_HIP(launch_init<8>(a, d_init, d_icnt, blocks, s)); break;
+    default: NND_HIP(launch_init<16>(a, d_init, d_icnt, blocks, s)); break;
+    }
+    NND_HIP(set_lds(nnd_hash_kernel, hash_lds));
+    hipLaunchKernelGGL(nnd_hash_kernel, dim3(blocks), dim3(64 * WPB), hash_lds, s, a);
+    NND_HIP(hipGetLastError());
+    NND_HIP(hipEventRecord(ev[1], s));
+    NND_HIP(hipEventSynchronize(ev[1]));
+    float ms_init = 0.f;
+    NND_HIP(hipEventElapsedTime(&ms_init, ev[0], ev[1]));
+
+    NND_HIP(set_lds(nnd_join_kernel<DT_F32>, join_lds));
+    NND_HIP(set_lds(nnd_join_kernel<DT_BF16>, join_lds));
+    NND_HIP(set_lds(nnd_join_kernel<DT_FP8>, join_lds));
+    NND_HIP(set_lds(nnd_merge_kernel, merge_lds));
+    NND_HIP(set_lds(nnd_rev_select_kernel, sel_lds));
+
+    for (uint32_t it = 0; it < p->iters; ++it) {
+        a.iter = it;
+        float ms = 0.f;
+        if (it > 0) {
+            NND_HIP(hipEventRecord(ev[2], s));
+            const uint32_t jb = (N + JOIN_WPB - 1) / JOIN_WPB;
+            if (dt == DT_F32) hipLaunchKernelGGL(nnd_join_kernel<DT_F32>, dim3(jb), dim3(64 * JOIN_WPB), join_lds, s, a);
+            else if (dt == DT_BF16) hipLaunchKernelGGL(nnd_join_kernel<DT_BF16>, dim3(jb), dim3(64 * JOIN_WPB), join_lds, s, a);
+            else hipLaunchKernelGGL(nnd_join_kernel<DT_FP8>, dim3(jb), dim3(64 * JOIN_WPB), join_lds, s, a);
+            NND_HIP(hipGetLastError());
+            NND_HIP(hipEventRecord(ev[3], s));
+            hipLaunchKernelGGL(nnd_merge_kernel, dim3(blocks), dim3(64 * WPB), merge_lds, s, a, up2, hp2);
+            NND_HIP(hipGetLastError());
+            if (it + 1 < p->iters) {
+                hipLaunchKernelGGL(nnd_hash_kernel, dim3(blocks), dim3(64 * WPB), hash_lds, s, a);
+                NND_HIP(hipGetLastError());
+            }
+            NND_HIP(hipEventRecord(ev[4], s));
+        } else {
+            hipLaunchKernelGGL(nnd_m0_kernel, dim3(blocks), dim3(64 * WPB), 0, s, a);
+            NND_HIP(hipGetLastError());
+        }
+        if (it + 1 < p->iters) {
+            NND_HIP(hipEventRecord(ev[5], s));
+            hipLaunchKernelGGL(fill_u32_kernel, dim3(1024), dim3(256), 0, s, a.rcnt_new, 0u, (size_t)N);
+            hipLaunchKernelGGL(fill_u32_kernel, dim3(1024), dim3(256), 0, s, a.rcnt_old, 0u, (size_t)N);
+            hipLaunchKernelGGL(fill_u32_kernel, dim3(1024), dim3(256), 0, s, a.rcur_new, 0u, (size_t)N);
+            hipLaunchKernelGGL(fill_u32_kernel, dim3(1024), dim3(256), 0, s, a.rcur_old, 0u, (size_t)N);
+            hipLaunchKernelGGL(nnd_sample_kernel, dim3(blocks), dim3(64 * WPB), 0, s, a);
+            NND_HIP(hipGetLastError());
+            NND_HIP(scan_u32(a.rcnt_new, N, a.roff_new, bsum, s));
+            NND_HIP(scan_u32(a.rcnt_old, N, a.roff_old, bsum, s));
+            hipLaunchKernelGGL(nnd_rev_scatter_kernel, dim3(blocks), dim3(64 * WPB), 0, s, a);
+            NND_HIP(hipGetLastError());
+            hipLaunchKernelGGL(nnd_rev_select_kernel, dim3(blocks), dim3(64 * WPB), sel_lds, s, a, rp2);
+            NND_HIP(hipGetLastError());
+            NND_HIP(hipEventRecord(ev[6], s));
+        }
+        NND_HIP(hipStreamSynchronize(s));
+        if (it > 0) {
+            NND_HIP(hipEventElapsedTime(&ms, ev[2], ev[3]));
+            t_join += ms;
+            NND_HIP(hipEventElapsedTime(&ms, ev[3], ev[4]));
+            t_merge += ms;
+        }
+        if (it + 1 < p->iters) {
+            NND_HIP(hipEventElapsedTime(&ms, ev[5], ev[6]));
+            t_sample += ms;
+        }
+    }
+    NND_HIP(hipEventRecord(ev[2], s));
+    switch (cpl) {
+    case 1: NND_HIP(launch_final<1>(a, fp2, blocks, s)); break;
+    case 2: NND_HIP(launch_final<2>(a, fp2, blocks, s)); break;
+    case 4: NND_HIP(launch_final<4>(a, fp2, blocks, s)); break;
+    case 8: NND_HIP(launch_final<8>(a, fp2, blocks, s)); break;
+    default: NND_HIP(launch_final<16>(a, fp2, blocks, s)); break;
+    }
+    NND_HIP(hipEventRecord(ev[7], s));
+    NND_HIP(hipStreamSynchronize(s));
+    if (stats) {
+        float ms = 0.f;
+        unsigned long long c[4];
+        NND_HIP(hipMemcpy(c, a.ctr, sizeof c, hipMemcpyDeviceToHost));
+        NND_HIP(hipEventElapsedTime(&ms, ev[2], ev[7]));
+        stats->finalize_ms = ms;
+        NND_HIP(hipEventElapsedTime(&ms, ev[0], ev[7]));
+        stats->total_ms = ms;
+        stats->init_ms = ms_init;
+        stats->sample_ms = t_sample;
+        stats->join_ms = t_join;
+        stats->merge_ms = t_merge;
+        stats->pairs = c[0];
+        stats->proposals = c[1];
+        stats->rows_gathered = c[2];
+        stats->iters_run = p->iters;
+        stats->cand_slots = C;
+    }
+    return WV_OK;
+}
+
+}  // namespace wv
