@@ -1,0 +1,234 @@
+#!/usr/bin/env python3
+"""Benchmark: greedy-search QPS at recall@10 = 0.95 on one MI355X, with the
+NN-Descent build that produced the graph (seconds, graph recall).
+
+Workload (default "c2"): 1,000,000 base + 10,000 query vectors, dim 128,
+seeded 1,000-cluster Gaussian mixture with Zipf(1.1) cluster sizes; the
+graph is built on the GPU by NN-Descent (K=64, L=64, S=16, R=64, 12 rounds,
+FP32 join; a BF16 build is timed beside it); search is fixed-L greedy
+search, k=10, from seeded random entry points, with L the smallest value
+whose recall@10 against exact FP64 kNN reaches the target.
+
+A "step" is one search pass over all queries, with base, graph, queries and
+result buffers resident in HBM.  Exactly --steps steps are timed after
+--warmup untimed ones, bracketed by device synchronisation.  Prints ONE JSON
+line on stdout; progress goes to stderr.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "QPS @ recall@10=0.95 (learned vs plain); NN-Descent build s @ 0.95 graph recall"
+HBM_PEAK_GBS = 8000.0
+
+WORKLOADS = {
+    "c2": dict(desc="1M base x 10k queries, d=128, 1000-cluster Zipf(1.1) GMM; NN-Descent K=64 L=64 "
+                    "S=16 R=64 iters=12; greedy search k=10 from random entries",
+               gen="gmm_c2", K=64, L=64, S=16, R=64, iters=12, k=10, sweep=[20, 40, 80, 160],
+               control=10000),
+    "c1": dict(desc="10k base x 1k queries, d=128, 20-cluster integer GMM; NN-Descent K=50 L=70 "
+                    "S=10 R=100 iters=10; greedy search k=10 from random entries",
+               gen="gmm_c1", K=50, L=70, S=10, R=100, iters=10, k=10, sweep=[20, 40, 80, 100],
+               control=10000),
+}
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def recall_at_k(res, gt, k):
+    """src/builder.cpp:465-480: 1 - misses / (Q*k), GT truncated to k."""
+    hit = (res[:, :k, None] == gt[:, None, :k]).any(axis=2)
+    return float(hit.mean())
+
+
+def graph_recall(ids, deg, gt, control=10):
+    """src/builder.cpp:760-776 over the sampled rows (self excluded in gt)."""
+    kk = min(control, gt.shape[1])
+    m = ids.shape[1]
+    valid = np.arange(m)[None, :] < deg[:, None]
+    hits = np.zeros(len(gt))
+    for j in range(kk):
+        hits += ((ids == gt[:, j:j + 1]) & valid).any(axis=1)
+    den = np.minimum(np.minimum(kk, deg), gt.shape[1]).astype(np.float64)
+    return float(np.mean(np.where(den > 0, hits / np.maximum(den, 1), 0.0)))
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="c2", choices=sorted(WORKLOADS))
+    ap.add_argument("--target", type=float, default=0.95)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the host-core baseline")
+    ap.add_argument("--no-bf16", action="store_true", help="skip the BF16 build")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path "
+              "(queries would shard with no collective; see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    from weavess_amd import Context, Index, datagen
+    from weavess_amd.devmem import DeviceArray, host_cores, pad_rows, synchronize
+
+    w = WORKLOADS[args.workload]
+    t0 = time.time()
+    base, queries = getattr(datagen, w["gen"])()
+    N, d = base.shape
+    Q = len(queries)
+    k = w["k"]
+    log(f"data {N}x{d}, {Q} queries in {time.time() - t0:.1f}s")
+
+    ctx = Context(0)
+    ix = Index(ctx, base)
+    nnd = {}
+    ctl = min(w["control"], N)
+    gt_graph, _, _ = ctx.exact_knn(base, base[:ctl], 10, self_offset=0)
+    for dt in ([] if args.no_bf16 else ["bf16"]) + ["f32"]:
+        t1 = time.time()
+        st = ix.build_nndescent(w["K"], w["L"], w["S"], w["R"], w["iters"], seed=1, dtype=dt)
+        wall = time.time() - t1
+        gids, gdist, gdeg = ix.get_graph()
+        gr = graph_recall(gids[:ctl], gdeg[:ctl], gt_graph)
+        nnd[dt] = dict(build_s=st["total_ms"] / 1e3, wall_s=wall, graph_recall=gr,
+                       join_ms=st["join_ms"], merge_ms=st["merge_ms"], sample_ms=st["sample_ms"],
+                       pairs=st["pairs"], proposals=st["proposals"])
+        log(f"nndescent {dt}: {st['total_ms']:.1f} ms device, graph recall {gr:.4f}")
+    graph, gdist, gdeg = gids, gdist, gdeg  # the FP32 graph stays in the index
+
+    gt, _, gt_ms = ctx.exact_knn(base, queries, k)
+    log(f"exact FP64 ground truth {gt_ms:.1f} ms")
+
+    dq = DeviceArray.from_host(pad_rows(queries))
+    d_ids = DeviceArray((Q, k), np.uint32)
+    d_dist = DeviceArray((Q, k), np.float32)
+    d_ev = DeviceArray((Q,), np.uint32)
+    d_hp = DeviceArray((Q,), np.uint32)
+
+    def run(L, seed=7):
+        return ix.search_dev(dq.ptr, Q, L, k, d_ids.ptr, d_dist.ptr, seed=seed,
+                             d_evals=d_ev.ptr, d_hops=d_hp.ptr)
+
+    cache = {}
+
+    def measure(L):
+        if L not in cache:
+            st = run(L)
+            r = recall_at_k(d_ids.to_host(), gt, k)
+            cache[L] = dict(L=L, recall=r, kernel_ms=st["kernel_ms"], qps=Q / (st["kernel_ms"] / 1e3),
+                            evals=st["dist_evals"] / Q, hops=st["hops"] / Q)
+            log(f"L={L}: recall@{k} {r:.4f}, {cache[L]['qps']:.0f} QPS (kernel), "
+                f"{cache[L]['evals']:.0f} evals/query")
+        return cache[L]
+
+    sweep = [measure(L) for L in w["sweep"]]
+    lo, hi = k, None
+    for s in sweep:
+        if s["recall"] >= args.target:
+            hi = s["L"]
+            break
+        lo = s["L"]
+    if hi is None:
+        hi = w["sweep"][-1]
+        while measure(hi)["recall"] < args.target and hi < 1024:
+            lo, hi = hi, min(1024, hi * 2)
+    while hi - lo > 1:
+        mid = (lo + hi) // 2
+        if measure(mid)["recall"] >= args.target:
+            hi = mid
+        else:
+            lo = mid
+    Ls = hi
+    op = measure(Ls)
+    log(f"operating point L={Ls}, recall {op['recall']:.4f}")
+
+    for _ in range(args.warmup):
+        run(Ls)
+    synchronize()
+    kern = []
+    evals = hops = 0
+    t1 = time.perf_counter()
+    for _ in range(args.steps):
+        st = run(Ls)
+        kern.append(st["kernel_ms"])
+        evals, hops = st["dist_evals"], st["hops"]
+    synchronize()
+    t2 = time.perf_counter()
+    ms_step = (t2 - t1) * 1e3 / args.steps
+    kern_ms = float(np.mean(kern))
+    value = Q / (ms_step / 1e3)
+    # algorithmic bytes per launch: rows gathered + adjacency lists + query + results
+    deg = w["K"]
+    alg_bytes = 4 * d * evals + 4 * deg * hops + Q * (4 * d + 8 * k)
+    achieved = alg_bytes / (kern_ms / 1e3) / 1e9
+    traffic = None
+    tp = os.path.join(ROOT, "profiles", f"greedy_search_{args.workload}_traffic.json")
+    if os.path.exists(tp):
+        try:
+            tj = json.load(open(tp))
+            if tj.get("L") == Ls:
+                traffic = tj.get("hbm_bytes_per_launch")
+        except (ValueError, OSError):
+            traffic = None
+
+    out = {
+        "metric": METRIC, "value": round(value, 1), "unit": "QPS", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_step, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": f"synthetic, seeded ({w['gen']}), {N}x{d} base, {Q} queries",
+        "config": {"workload": args.workload + ": " + w["desc"], "L_search": Ls, "k": k,
+                   "recall_at_10": round(op["recall"], 4), "target_recall": args.target,
+                   "entries": "seeded GenRandom per query", "method": "plain fixed-L"},
+        "roofline": {"bound": "hbm", "kernel": "greedy_search_kernel", "achieved": round(achieved, 1),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": traffic, "kernel_ms": round(kern_ms, 4),
+                     "alg_bytes_per_launch": int(alg_bytes),
+                     "dist_evals_per_query": round(evals / Q, 1), "hops_per_query": round(hops / Q, 1)},
+        "nndescent": nnd,
+        "l_sweep": [dict(L=s["L"], recall=round(s["recall"], 4), qps=round(s["qps"], 1)) for s in sweep],
+    }
+    if "bf16" in nnd:
+        out["nndescent"]["bf16_speedup_vs_f32"] = round(nnd["f32"]["build_s"] / nnd["bf16"]["build_s"], 3)
+
+    if not args.no_cpu:
+        import oracle  # the CPU baseline leg: the oracle is timed here, never the product
+        cores = host_cores()
+        sample = min(Q, 10000)
+        res, _, _, _ = oracle.search_greedy(graph, gdeg, base, queries[:sample], Ls, k, seed=7,
+                                            nthreads=cores)
+        times = []
+        tstart = time.time()
+        while len(times) < 3 or (time.time() - tstart < 10 and len(times) < 10):
+            c0 = time.perf_counter()
+            oracle.search_greedy(graph, gdeg, base, queries[:sample], Ls, k, seed=7, nthreads=cores)
+            times.append(time.perf_counter() - c0)
+            if time.time() - tstart > 30:
+                break
+        cpu_qps = sample / float(np.median(times))
+        gpu_ids = d_ids.to_host()[:sample]
+        out["cpu_baseline"] = {"value": round(cpu_qps, 1), "unit": "QPS", "cores": cores, "kind": "port",
+                               "sample": f"{sample} queries at L={Ls}, median of {len(times)} runs "
+                                         f"(oracle greedy search, OpenMP over queries, -O3)",
+                               "ids_bit_exact_vs_gpu": bool(np.array_equal(res, gpu_ids))}
+        log(f"cpu baseline {cpu_qps:.0f} QPS on {cores} threads; ids equal: "
+            f"{out['cpu_baseline']['ids_bit_exact_vs_gpu']}")
+    print(json.dumps(out), flush=True)
+    for a in (dq, d_ids, d_dist, d_ev, d_hp):
+        a.free()
+    ix.close()
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,101 @@
+"""Minimal HIP runtime plumbing over ctypes: device buffers and copies.
+
+Used by bench.py to keep queries and results resident in HBM for the
+device-pointer entry points (wv_index_search_dev) without importing torch.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_hip = None
+
+
+def hip():
+    global _hip
+    if _hip is None:
+        for name in ("libamdhip64.so", "/opt/rocm/lib/libamdhip64.so"):
+            try:
+                _hip = C.CDLL(name)
+                break
+            except OSError:
+                continue
+        if _hip is None:
+            raise ImportError("libamdhip64.so not found")
+        _hip.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+        _hip.hipFree.argtypes = [C.c_void_p]
+        _hip.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+        _hip.hipMemset.argtypes = [C.c_void_p, C.c_int, C.c_size_t]
+        _hip.hipDeviceSynchronize.argtypes = []
+        _hip.hipSetDevice.argtypes = [C.c_int]
+    return _hip
+
+
+H2D, D2H = 1, 2
+
+
+def _check(rc, what):
+    if rc != 0:
+        raise RuntimeError(f"{what} failed with hipError {rc}")
+
+
+class DeviceArray:
+    """A device allocation holding a numpy-shaped array."""
+
+    def __init__(self, shape, dtype):
+        self.shape = tuple(shape)
+        self.dtype = np.dtype(dtype)
+        self.nbytes = int(np.prod(self.shape)) * self.dtype.itemsize
+        p = C.c_void_p()
+        _check(hip().hipMalloc(C.byref(p), max(self.nbytes, 16)), "hipMalloc")
+        self.ptr = p.value
+
+    @classmethod
+    def from_host(cls, a: np.ndarray):
+        a = np.ascontiguousarray(a)
+        d = cls(a.shape, a.dtype)
+        _check(hip().hipMemcpy(C.c_void_p(d.ptr), a.ctypes.data_as(C.c_void_p), a.nbytes, H2D),
+               "hipMemcpy H2D")
+        return d
+
+    def to_host(self) -> np.ndarray:
+        out = np.empty(self.shape, self.dtype)
+        _check(hip().hipMemcpy(out.ctypes.data_as(C.c_void_p), C.c_void_p(self.ptr), self.nbytes, D2H),
+               "hipMemcpy D2H")
+        return out
+
+    def free(self):
+        if self.ptr:
+            hip().hipFree(C.c_void_p(self.ptr))
+            self.ptr = 0
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def synchronize():
+    _check(hip().hipDeviceSynchronize(), "hipDeviceSynchronize")
+
+
+def pad_rows(a: np.ndarray) -> np.ndarray:
+    """Zero-pad rows to a multiple of 4 floats (the device row layout)."""
+    a = np.ascontiguousarray(a, np.float32)
+    d = a.shape[1]
+    dp = (d + 3) & ~3
+    if dp == d:
+        return a
+    out = np.zeros((a.shape[0], dp), np.float32)
+    out[:, :d] = a
+    return out
+
+
+def host_cores() -> int:
+    try:
+        return len(os.sched_getaffinity(0))
+    except AttributeError:
+        return os.cpu_count() or 1

@@ -1,0 +1,204 @@
+// components.cpp -- the HIP stages behind the reference's plug-in hooks.
+#include <algorithm>
+#include <fstream>
+#include <stdexcept>
+
+#include "weavess/component.h"
+
+namespace weavess {
+
+namespace {
+inline uint64_t fmix64(uint64_t z) {  // splitmix64 finaliser
+    z ^= z >> 30;
+    z *= 0xBF58476D1CE4E5B9ull;
+    z ^= z >> 27;
+    z *= 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    return z;
+}
+constexpr uint64_t GOLDEN = 0x9E3779B97F4A7C15ull;
+constexpr uint64_t TAG_ENTRY = 0x454E5452ull;
+}  // namespace
+
+void GenRandomStream(uint64_t seed, uint64_t tag, uint64_t idx, unsigned *addr, unsigned size, unsigned N) {
+    if (size >= N) throw std::invalid_argument("GenRandom needs size < N");
+    uint64_t s = fmix64(seed + GOLDEN);
+    s = fmix64(s ^ (tag * GOLDEN));
+    s = fmix64(s ^ (idx + 0x632BE59BD9B4E019ull));
+    auto draw = [s](uint64_t k) { return (uint32_t)(fmix64(s + (k + 1) * GOLDEN) >> 32); };
+    for (unsigned i = 0; i < size; ++i) addr[i] = draw(i) % (N - size);
+    std::sort(addr, addr + size);
+    for (unsigned i = 1; i < size; ++i)
+        if (addr[i] <= addr[i - 1]) addr[i] = addr[i - 1] + 1;
+    const uint64_t off = draw(size) % N;
+    for (unsigned i = 0; i < size; ++i) addr[i] = (unsigned)(((uint64_t)addr[i] + off) % N);
+}
+
+// ------------------------------------------------------------------ load
+namespace {
+// fvecs/ivecs: per row an int32 dimension then dim 4-byte values
+// (src/component_load.cpp:9-28).
+template <class T>
+void load_vecs(const char *path, std::vector<T> &data, unsigned &num, unsigned &dim) {
+    std::ifstream in(path, std::ios::binary);
+    if (!in.is_open()) throw std::runtime_error(std::string("open file error: ") + path);
+    int32_t d = 0;
+    in.read(reinterpret_cast<char *>(&d), 4);
+    if (!in || d <= 0) throw std::runtime_error(std::string("bad vecs header: ") + path);
+    in.seekg(0, std::ios::end);
+    const size_t bytes = (size_t)in.tellg();
+    dim = (unsigned)d;
+    num = (unsigned)(bytes / ((size_t)dim + 1) / 4);
+    data.resize((size_t)num * dim);
+    in.seekg(0, std::ios::beg);
+    for (size_t i = 0; i < num; ++i) {
+        in.seekg(4, std::ios::cur);
+        in.read(reinterpret_cast<char *>(data.data() + i * dim), (std::streamsize)dim * sizeof(T));
+    }
+    if (!in) throw std::runtime_error(std::string("short vecs file: ") + path);
+}
+}  // namespace
+
+void ComponentLoad::LoadInner(const char *data_file, const char *query_file, const char *ground_file,
+                              Parameters &parameters) {
+    std::vector<float> b, q;
+    std::vector<unsigned> g;
+    unsigned n, d, qn, qd, gn = 0, gd = 0;
+    load_vecs(data_file, b, n, d);
+    load_vecs(query_file, q, qn, qd);
+    if (qd != d) throw std::runtime_error("query dimension differs from base dimension");
+    if (ground_file && *ground_file) load_vecs(ground_file, g, gn, gd);
+    index->setBase(std::move(b), n, d);
+    index->setQuery(std::move(q), qn, qd);
+    index->setGround(std::move(g), gn, gd);
+    index->setParam(parameters);
+}
+
+// ------------------------------------------------------------------ init / refine
+void ComponentInitRandomHIP::InitInner() {
+    const unsigned S = index->getParam().get<unsigned>("S");  // component_init.cpp:47-49
+    index->setInitEdgesNum(S);
+    wv_nnd_params p{};
+    p.K = S;
+    p.L = S;
+    p.S = S;
+    p.R = 0;
+    p.iters = 0;  // InitRandom only
+    p.dtype = WV_DTYPE_F32;
+    p.seed = index->seed();
+    wv_index *ix = index->deviceIndex();
+    index->check(wv_index_build_nndescent(ix, &p, nullptr, nullptr, nullptr));
+    const unsigned N = index->getBaseLen();
+    std::vector<uint32_t> ids((size_t)N * S), deg(N);
+    std::vector<float> dist((size_t)N * S);
+    index->check(wv_index_get_graph(ix, ids.data(), dist.data(), deg.data(), nullptr));
+    auto &fg = index->getFinalGraph();
+    fg.assign(N, {});
+    for (unsigned i = 0; i < N; ++i) {
+        fg[i].reserve(S);
+        for (unsigned j = 0; j < deg[i]; ++j)
+            fg[i].emplace_back(ids[(size_t)i * S + j], dist[(size_t)i * S + j]);
+    }
+}
+
+void ComponentRefineNNDescentHIP::RefineInner() {
+    auto &P = index->getParam();
+    // SetConfigs, component_refine.cpp:79-85
+    index->setCandidatesEdgesNum(P.get<unsigned>("L"));
+    index->setResultEdgesNum(P.get<unsigned>("K"));
+    wv_nnd_params p{};
+    p.L = index->getCandidatesEdgesNum();
+    p.K = index->getResultEdgesNum();
+    p.R = P.get<unsigned>("R");
+    p.iters = P.get<unsigned>("ITER");
+    p.S = index->getInitEdgesNum() ? index->getInitEdgesNum() : P.get<unsigned>("S");
+    p.dtype = dtype_;
+    p.rerank_mult = P.has("rerank_mult") ? P.get<unsigned>("rerank_mult") : 0;
+    p.cand_slots = P.has("cand_slots") ? P.get<unsigned>("cand_slots") : 0;
+    p.seed = index->seed();
+    const unsigned N = index->getBaseLen();
+    auto &fg = index->getFinalGraph();
+    std::vector<uint32_t> init, cnt;
+    if (fg.size() == N) {  // initial lists from the init stage (component_refine.cpp:96-103)
+        init.assign((size_t)N * p.S, 0xFFFFFFFFu);
+        cnt.assign(N, 0);
+        for (unsigned i = 0; i < N; ++i) {
+            const unsigned c = std::min<unsigned>(p.S, (unsigned)fg[i].size());
+            for (unsigned j = 0; j < c; ++j) init[(size_t)i * p.S + j] = fg[i][j].id;
+            cnt[i] = c;
+        }
+    }
+    wv_index *ix = index->deviceIndex();
+    index->check(wv_index_build_nndescent(ix, &p, init.empty() ? nullptr : init.data(),
+                                          cnt.empty() ? nullptr : cnt.data(), &stats_));
+    std::vector<uint32_t> ids((size_t)N * p.K), deg(N);
+    std::vector<float> dist((size_t)N * p.K);
+    index->check(wv_index_get_graph(ix, ids.data(), dist.data(), deg.data(), nullptr));
+    fg.assign(N, {});
+    for (unsigned i = 0; i < N; ++i) {
+        fg[i].reserve(deg[i]);
+        for (unsigned j = 0; j < deg[i]; ++j) fg[i].emplace_back(ids[(size_t)i * p.K + j], dist[(size_t)i * p.K + j]);
+    }
+    index->markLoadGraphChanged();  // the device graph is now the built one
+}
+
+// ------------------------------------------------------------------ search
+void ComponentSearchEntryRand::SearchEntryInner(unsigned query, std::vector<Index::Neighbor> &pool) {
+    const unsigned L = index->getParam().get<unsigned>("L_search");
+    pool.resize(L + 1);
+    std::vector<unsigned> ids(L);
+    GenRandomStream(index->seed(), TAG_ENTRY, query, ids.data(), L, index->getBaseLen());
+    for (unsigned i = 0; i < L; ++i) pool[i] = Index::Neighbor(ids[i], 0.0f, true);
+}
+
+void ComponentSearchRoute::RouteBatch(unsigned nq, const std::vector<std::vector<Index::Neighbor>> *pools,
+                                      std::vector<std::vector<unsigned>> &res) {
+    if (!pools) throw std::invalid_argument("RouteBatch needs entry pools");
+    res.resize(nq);
+    for (unsigned i = 0; i < nq; ++i) {
+        auto pool = (*pools)[i];
+        RouteInner(i, pool, res[i]);
+    }
+}
+
+void ComponentSearchRouteGreedyHIP::RouteInner(unsigned query, std::vector<Index::Neighbor> &pool,
+                                               std::vector<unsigned> &res) {
+    const unsigned L = index->getParam().get<unsigned>("L_search");  // route.cpp:11-12
+    const unsigned K = index->getParam().get<unsigned>("K_search");
+    if (pool.size() < L) throw std::invalid_argument("entry pool shorter than L_search");
+    index->uploadLoadGraph();
+    std::vector<uint32_t> ent(L), ids(L), ev(1), hp(1);
+    std::vector<float> dist(L);
+    for (unsigned i = 0; i < L; ++i) ent[i] = pool[i].id;
+    const float *q = index->getQueryData() + (size_t)query * index->getQueryDim();
+    wv_search_stats st{};
+    index->check(wv_index_search(index->deviceIndex(), q, 1, ent.data(), 0, L, K, nullptr, nullptr, L,
+                                 ids.data(), dist.data(), ev.data(), hp.data(), &st));
+    for (unsigned i = 0; i < L; ++i) pool[i] = Index::Neighbor(ids[i], dist[i], false);
+    res.assign(ids.begin(), ids.begin() + K);
+    index->addDistCount(ev[0]);
+    index->addHopCount(hp[0]);
+    kernel_ms_ = st.kernel_ms;
+}
+
+void ComponentSearchRouteGreedyHIP::RouteBatch(unsigned nq, const std::vector<std::vector<Index::Neighbor>> *pools,
+                                               std::vector<std::vector<unsigned>> &res) {
+    if (pools) {  // caller-supplied entries: per-query path
+        ComponentSearchRoute::RouteBatch(nq, pools, res);
+        return;
+    }
+    const unsigned L = index->getParam().get<unsigned>("L_search");
+    const unsigned K = index->getParam().get<unsigned>("K_search");
+    index->uploadLoadGraph();
+    std::vector<uint32_t> ids((size_t)nq * K), ev(nq), hp(nq);
+    wv_search_stats st{};
+    index->check(wv_index_search(index->deviceIndex(), index->getQueryData(), nq, nullptr, index->seed(), L, K,
+                                 nullptr, nullptr, K, ids.data(), nullptr, ev.data(), hp.data(), &st));
+    res.assign(nq, {});
+    for (unsigned i = 0; i < nq; ++i) res[i].assign(ids.begin() + (size_t)i * K, ids.begin() + (size_t)(i + 1) * K);
+    index->addDistCount(st.dist_evals);
+    index->addHopCount(st.hops);
+    kernel_ms_ = st.kernel_ms;
+}
+
+}  // namespace weavess

@@ -1,0 +1,117 @@
+// component.h -- the pipeline's plug-in stages.
+//
+// Stage bases keep the reference's virtual hooks and signatures
+// (include/weavess/component.h:12-37, 361-366, 752-757, 827-832).  Unlike
+// the reference, ~Component does not delete the shared Index (component.h:16
+// there deletes it, which is why its builder leaks every component); the
+// Index is owned by IndexBuilder alone and device resources by the Index.
+#pragma once
+
+#include <vector>
+
+#include "index.h"
+
+namespace weavess {
+
+class Component {
+public:
+    explicit Component(Index *index) : index(index) {}
+    virtual ~Component() = default;
+
+protected:
+    Index *index = nullptr;
+};
+
+class ComponentLoad : public Component {
+public:
+    explicit ComponentLoad(Index *index) : Component(index) {}
+    // src/component_load.cpp:67-104
+    virtual void LoadInner(const char *data_file, const char *query_file, const char *ground_file,
+                           Parameters &parameters);
+};
+
+class ComponentInit : public Component {
+public:
+    explicit ComponentInit(Index *index) : Component(index) {}
+    virtual void InitInner() = 0;
+};
+
+// Random S-NN initialisation on the GPU (src/component_init.cpp:11-45):
+// FinalGraph[i] = S random distinct ids != i, ascending (dist, id).
+class ComponentInitRandomHIP : public ComponentInit {
+public:
+    explicit ComponentInitRandomHIP(Index *index) : ComponentInit(index) {}
+    void InitInner() override;
+};
+
+class ComponentRefine : public Component {
+public:
+    explicit ComponentRefine(Index *index) : Component(index) {}
+    virtual void RefineInner() = 0;
+};
+
+// NN-Descent + PruneNaive on the GPU (src/component_refine.cpp:12-239,
+// src/component_prune.cpp:8-18).  Reads "L","K","R","ITER" like
+// ComponentRefineNNDescent::SetConfigs (:79-85), S from the init stage and
+// the initial lists from FinalGraph; writes FinalGraph (<= K per node).
+class ComponentRefineNNDescentHIP : public ComponentRefine {
+public:
+    ComponentRefineNNDescentHIP(Index *index, unsigned dtype = WV_DTYPE_F32)
+        : ComponentRefine(index), dtype_(dtype) {}
+    void RefineInner() override;
+    const wv_build_stats &stats() const { return stats_; }
+
+private:
+    unsigned dtype_;
+    wv_build_stats stats_{};
+};
+
+class ComponentSearchEntry : public Component {
+public:
+    explicit ComponentSearchEntry(Index *index) : Component(index) {}
+    virtual void SearchEntryInner(unsigned query, std::vector<Index::Neighbor> &pool) = 0;
+};
+
+// SearchEntryRand (src/component_search_entry.cpp:9-30): pool sized L+1,
+// the first L slots hold GenRandom ids (same seeded stream the device uses),
+// flagged.  Distances are left 0 here: the GPU router computes them.
+class ComponentSearchEntryRand : public ComponentSearchEntry {
+public:
+    explicit ComponentSearchEntryRand(Index *index) : ComponentSearchEntry(index) {}
+    void SearchEntryInner(unsigned query, std::vector<Index::Neighbor> &pool) override;
+};
+
+class ComponentSearchRoute : public Component {
+public:
+    explicit ComponentSearchRoute(Index *index) : Component(index) {}
+    virtual void RouteInner(unsigned query, std::vector<Index::Neighbor> &pool,
+                            std::vector<unsigned> &res) = 0;
+    // Whole query set at once; the default loops over RouteInner.
+    virtual void RouteBatch(unsigned nq, const std::vector<std::vector<Index::Neighbor>> *pools,
+                            std::vector<std::vector<unsigned>> &res);
+};
+
+// Greedy best-first search on the GPU (src/component_route.cpp:9-53).
+// RouteInner takes its entries from `pool` and writes the final sorted pool
+// back into it; RouteBatch searches every query in one launch with entries
+// generated on the device from the same seeded stream as
+// ComponentSearchEntryRand.
+class ComponentSearchRouteGreedyHIP : public ComponentSearchRoute {
+public:
+    explicit ComponentSearchRouteGreedyHIP(Index *index) : ComponentSearchRoute(index) {}
+    void RouteInner(unsigned query, std::vector<Index::Neighbor> &pool,
+                    std::vector<unsigned> &res) override;
+    void RouteBatch(unsigned nq, const std::vector<std::vector<Index::Neighbor>> *pools,
+                    std::vector<std::vector<unsigned>> &res) override;
+    double last_kernel_ms() const { return kernel_ms_; }
+
+private:
+    double kernel_ms_ = 0.0;
+};
+
+// GenRandom (include/weavess/util.h:13-27) on the counter-based stream of
+// (seed, tag, idx) that the device kernels use.
+void GenRandomStream(uint64_t seed, uint64_t tag, uint64_t idx, unsigned *addr, unsigned size,
+                     unsigned N);
+
+}  // namespace weavess

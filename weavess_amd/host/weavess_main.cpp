@@ -1,0 +1,102 @@
+// weavess_main.cpp -- KGraph driver of the pipeline (test/main.cpp:6-34 and
+// :480-546 of the reference, KGraph only, with the HIP stages).
+//
+//   weavess_main kgraph <prefix> build|search|info [L_search]
+//
+// <prefix>_base.fvecs, <prefix>_query.fvecs and <prefix>_groundtruth.ivecs
+// are read; the graph file is <prefix>.kgraph.graph (WEAVESS_GRAPH
+// overrides).  K, L, ITER, S, R come from the per-dataset table below (keyed
+// by the prefix's file name) and can be overridden with WEAVESS_K, _L,
+// _ITER, _S, _R; WEAVESS_SEED, WEAVESS_DTYPE (f32|bf16|fp8) and
+// WEAVESS_EXACT (exact-kNN graph for "info") are read too.  Without
+// L_search, search tunes L to WEAVESS_ACC_SET (default 0.95) recall.
+#include <cstdlib>
+#include <cstring>
+#include <iostream>
+#include <map>
+#include <string>
+
+#include "weavess/builder.h"
+
+namespace {
+
+struct KG {
+    unsigned K, L, ITER, S, R;
+};
+
+// KGraph parameters per dataset (include/weavess/exp_data.h:57-98), plus
+// the synthetic workloads of this project.
+const std::map<std::string, KG> &table() {
+    static const std::map<std::string, KG> t = {
+        {"siftsmall", {25, 50, 6, 10, 100}}, {"sift1M", {90, 130, 12, 20, 50}},
+        {"gist", {100, 120, 12, 25, 100}},   {"glove-100", {100, 150, 12, 35, 150}},
+        {"c1", {50, 70, 10, 10, 100}},       {"c2", {64, 64, 12, 16, 64}},
+    };
+    return t;
+}
+
+unsigned env_u(const char *name, unsigned dflt) {
+    const char *v = std::getenv(name);
+    return v && *v ? (unsigned)std::strtoul(v, nullptr, 10) : dflt;
+}
+
+}  // namespace
+
+int main(int argc, char **argv) {
+    if (argc < 4 || argc > 5 || std::string(argv[1]) != "kgraph") {
+        std::cout << "./weavess_main kgraph dataset_prefix build|search|info [L_search]" << std::endl;
+        return 1;
+    }
+    const std::string prefix(argv[2]), exc(argv[3]);
+    std::string name = prefix.substr(prefix.find_last_of('/') + 1);
+    KG kg = table().count(name) ? table().at(name) : table().at("c1");
+    weavess::Parameters P;
+    P.set<unsigned>("K", env_u("WEAVESS_K", kg.K));
+    P.set<unsigned>("L", env_u("WEAVESS_L", kg.L));
+    P.set<unsigned>("ITER", env_u("WEAVESS_ITER", kg.ITER));
+    P.set<unsigned>("S", env_u("WEAVESS_S", kg.S));
+    P.set<unsigned>("R", env_u("WEAVESS_R", kg.R));
+    P.set<unsigned long long>("seed", env_u("WEAVESS_SEED", 0));
+    if (const char *a = std::getenv("WEAVESS_ACC_SET")) P.set<std::string>("acc_set", a);
+    else P.set<float>("acc_set", 0.95f);
+    if (const char *e = std::getenv("WEAVESS_EXACT")) P.set<std::string>("exact_knng_path", e);
+    const char *gf = std::getenv("WEAVESS_GRAPH");
+    const std::string graph = gf ? gf : prefix + ".kgraph.graph";
+    const std::string base = prefix + "_base.fvecs", query = prefix + "_query.fvecs",
+                      ground = prefix + "_groundtruth.ivecs";
+    const std::string dtype = std::getenv("WEAVESS_DTYPE") ? std::getenv("WEAVESS_DTYPE") : "f32";
+    const weavess::TYPE refine = dtype == "bf16"  ? weavess::REFINE_NN_DESCENT_HIP_BF16
+                                 : dtype == "fp8" ? weavess::REFINE_NN_DESCENT_HIP_FP8
+                                                  : weavess::REFINE_NN_DESCENT_HIP;
+    try {
+        weavess::IndexBuilder builder(8);
+        if (exc == "build") {
+            builder.load(base.c_str(), query.c_str(), ground.c_str(), P)->init(weavess::INIT_RANDOM);
+            std::cout << "Init cost: " << builder.GetBuildTime().count() << std::endl;
+            builder.refine(refine)->save_graph(weavess::INDEX_KGRAPH, graph.c_str());
+            std::cout << "Build cost: " << builder.GetBuildTime().count() << std::endl;
+        } else if (exc == "search") {
+            builder.load(base.c_str(), query.c_str(), ground.c_str(), P)->load_graph(weavess::INDEX_KGRAPH, graph.c_str());
+            if (argc == 5) {
+                builder.index()->getParam().set<unsigned>("L_search", (unsigned)std::atoi(argv[4]));
+                builder.search(weavess::SEARCH_ENTRY_RAND, weavess::ROUTER_GREEDY_HIP, weavess::L_SEARCH_ASSIGN);
+            } else {
+                builder.search(weavess::SEARCH_ENTRY_RAND, weavess::ROUTER_GREEDY_HIP, weavess::L_SEARCH_SET_RECALL);
+            }
+            std::cout << "QPS: " << builder.last_qps() << " recall: " << builder.last_recall()
+                      << " L: " << builder.last_L() << std::endl;
+            builder.peak_memory_footprint();
+        } else if (exc == "info") {
+            builder.load(base.c_str(), query.c_str(), ground.c_str(), P)
+                ->load_graph(weavess::INDEX_KGRAPH, graph.c_str())
+                ->print_index_info(weavess::INDEX_KGRAPH);
+        } else {
+            std::cout << "exc_type input error!" << std::endl;
+            return 1;
+        }
+    } catch (const std::exception &e) {
+        std::cerr << "error: " << e.what() << std::endl;
+        return 2;
+    }
+    return 0;
+}
