@@ -29,6 +29,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "QPS @ recall@10=0.95 (learned vs plain); NN-Descent build s @ 0.95 graph recall"
 HBM_PEAK_GBS = 8000.0
+L_MAX = 4096  # largest search pool the kernel supports
 
 WORKLOADS = {
     "c2": dict(desc="1M base x 10k queries, d=128, 1000-cluster Zipf(1.1) GMM; NN-Descent K=64 L=64 "
@@ -139,11 +140,13 @@ def main():
             hi = s["L"]
             break
         lo = s["L"]
+    reached = True
     if hi is None:
         hi = w["sweep"][-1]
-        while measure(hi)["recall"] < args.target and hi < 1024:
-            lo, hi = hi, min(1024, hi * 2)
-    while hi - lo > 1:
+        while measure(hi)["recall"] < args.target and hi < L_MAX:
+            lo, hi = hi, min(L_MAX, hi * 2)
+        reached = measure(hi)["recall"] >= args.target
+    while reached and hi - lo > 1:
         mid = (lo + hi) // 2
         if measure(mid)["recall"] >= args.target:
             hi = mid
@@ -151,7 +154,8 @@ def main():
             lo = mid
     Ls = hi
     op = measure(Ls)
-    log(f"operating point L={Ls}, recall {op['recall']:.4f}")
+    log(f"operating point L={Ls}, recall {op['recall']:.4f}" +
+        ("" if reached else f" (target {args.target} not reached at L <= {L_MAX})"))
 
     for _ in range(args.warmup):
         run(Ls)
@@ -189,6 +193,7 @@ def main():
         "data": f"synthetic, seeded ({w['gen']}), {N}x{d} base, {Q} queries",
         "config": {"workload": args.workload + ": " + w["desc"], "L_search": Ls, "k": k,
                    "recall_at_10": round(op["recall"], 4), "target_recall": args.target,
+                   "target_reached": reached,
                    "entries": "seeded GenRandom per query", "method": "plain fixed-L"},
         "roofline": {"bound": "hbm", "kernel": "greedy_search_kernel", "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),

@@ -93,7 +93,9 @@ typedef struct {
     uint32_t deg_max;
 } wv_graph;
 
-/* Search tuning (the autotuner's knobs; results never depend on them). */
+/* Search tuning (the autotuner's knobs; ids and distances never depend on
+ * them, only speed and the evaluation count: a visited table too small to
+ * hold every visited node forgets some, which are then evaluated again). */
 typedef struct {
     uint32_t waves_per_block;  /* 1, 2 or 4 queries per workgroup (0 -> auto)        */
     uint32_t row_groups;       /* gather tile: 4*row_groups rows in flight (0 -> auto) */

@@ -115,9 +115,30 @@ def test_search_config_changes_speed_only(ctx, small_float, small_float_graph):
         got = ix.search(q, 64, 10, seed=3)
         assert np.array_equal(got[0], ref[0]) and np.array_equal(got[1], ref[1]), cfg
         if cfg.get("visited_slots"):
-            assert np.all(got[2] >= ref[2])
+            assert np.all(got[2] >= ref[2])      # a forgetting table only re-evaluates
         else:
-            assert np.array_equal(got[2], ref[2])
+            assert np.array_equal(got[2], ref[2]), cfg
+        assert np.array_equal(got[3], ref[3])
+    ix.close()
+
+
+@pytest.mark.skip(reason="persistent work-queue search with an HBM visited bitmap is switched off: "
+                         "its first GPU run faulted and the cause was not found by reading the code")
+def test_search_persistent_queue_many_queries(ctx, small_float, small_float_graph):
+    """More queries than resident wavefronts: every query is taken exactly once
+    from the device work queue, with the exact bitmap behind a tiny table."""
+    from weavess_amd import Index
+    base, q = small_float
+    g, _, deg = small_float_graph
+    rng = np.random.default_rng(9)
+    qq = (base[rng.integers(0, len(base), 3000)] + rng.normal(0, 0.05, (3000, base.shape[1]))).astype(np.float32)
+    ref = O.search_greedy(g, deg, base, qq, 96, 10, seed=4)
+    ix = Index(ctx, base)
+    ix.set_graph(g, deg)
+    ix.set_search_config(waves_per_cu=1, visited_slots=128, visited_bitmap=2)
+    got = ix.search(qq, 96, 10, seed=4)
+    assert np.array_equal(got[0], ref[0]) and np.array_equal(got[1], ref[1])
+    assert np.array_equal(got[2], ref[2]) and np.array_equal(got[3], ref[3])
     ix.close()
 
 

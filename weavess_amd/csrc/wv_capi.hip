@@ -99,9 +99,9 @@ int plan_search(wv_index *ix, uint32_t Lmax, SearchPlan &pl) {
     pl.Lmax = Lmax;
     pl.P = std::max<uint32_t>(64, next_pow2(Lmax));
     uint32_t H = ix->cfg.visited_slots;
-    if (H == 0) H = std::min<uint32_t>(16384, std::max<uint32_t>(2048, next_pow2(Lmax * 32)));
+    if (H == 0) H = std::min<uint32_t>(4096, std::max<uint32_t>(1024, next_pow2(Lmax * 8)));
     if (H & (H - 1)) return fail(ix->ctx, WV_EINVAL, "visited_slots must be a power of two");
-    pl.H = H;
+    pl.H = std::max(H, pl.P);  // the table's storage also holds the entry ids
     uint32_t wpb = ix->cfg.waves_per_block ? ix->cfg.waves_per_block : 1;
     if (wpb != 1 && wpb != 2 && wpb != 4) return fail(ix->ctx, WV_EINVAL, "waves_per_block must be 1, 2 or 4");
     size_t lds = wv::search_wave_lds(pl.P, pl.H);
@@ -201,7 +201,7 @@ int check_search_args(wv_ctx *c, uint32_t N, uint32_t Q, const void *entries, ui
             return fail(c, WV_EINVAL, "query %u: L (%u) must be below N (%u) for random entries", q, Lq, N);
         Lmax = std::max(Lmax, Lq);
     }
-    if (Lmax > 1024) return fail(c, WV_ENOTSUP, "L above 1024");
+    if (Lmax > 4096) return fail(c, WV_ENOTSUP, "L above 4096");
     return WV_OK;
 }
 

@@ -30,6 +30,7 @@ struct SearchArgs {
 };
 
 size_t search_wave_lds(uint32_t P, uint32_t H);
+// One wavefront per query (grid of ceil(Q / waves_per_block) workgroups).
 hipError_t launch_greedy_search(const SearchArgs &a, int chunks_per_lane, int row_groups,
                                 int waves_per_block, hipStream_t s);
 

@@ -6,9 +6,9 @@
 // with Index::InsertIntoPool (include/weavess/index.h:152-180) done as a
 // wave-wide merge.
 //
-// Per wavefront, in LDS: the sorted pool of L (dist,id) keys (double
-// buffered), its "unexpanded" flags, a 64-entry candidate block and an
-// open-addressing visited table.  One expansion:
+// Per wavefront, in LDS: the sorted pool of L (dist,id) keys, its
+// "unexpanded" flags, a 64-entry candidate block and an open-addressing
+// visited table.  One expansion:
 //   1. ballot over the flags finds the first unexpanded pool slot >= k;
 //   2. lane i takes neighbour i of that node (64 per block; more in blocks);
 //   3. LDS atomicCAS test-and-set in the visited table; survivors are
@@ -16,15 +16,19 @@
 //   4. distances, 16 lanes per row with float4 gathers, in the build order;
 //   5. candidates not better than the pool's worst, or already in the pool,
 //      are dropped; the rest are bitonic-sorted across the wave and merged
-//      into the pool by rank (each element's new slot = own rank + number of
-//      smaller elements on the other side);
+//      into the pool in place: a candidate's slot is its rank plus the number
+//      of smaller pool keys, a pool key moves right by the number of smaller
+//      candidates; keys are moved in 64-slot chunks from the tail backwards
+//      (all reads of a chunk precede its writes), so slots before the first
+//      insertion point are never touched;
 //   6. the k-pointer restart of route.cpp:44-45 uses the final slot of the
 //      best merged candidate, which equals the minimum insert position of
 //      the sequential loop when (dist,id) is a strict order.
-// A full visited table (probe limit reached) forgets: the node is treated as
-// unvisited and its distance recomputed; a recomputed node is either already
-// in the pool (duplicate) or no better than the pool's worst, so ids and
-// distances are unchanged and only the evaluation count grows.
+// The visited table forgets: when an id finds no free slot within the probe
+// limit it evicts the occupant of its home slot.  A forgotten node that comes
+// back is re-evaluated; it is then either already in the pool (duplicate) or
+// no better than the pool's worst, so ids and distances are unchanged and
+// only the evaluation count grows.
 #include "wv_device.hpp"
 #include "wv_internal.hpp"
 
@@ -32,18 +36,21 @@ namespace wv {
 
 namespace {
 
-constexpr int VIS_PROBES = 16;
+constexpr int VIS_PROBES = 8;
 
 // Test-and-set of `id` in the visited table; true when this call inserted it
-// (first visit) or when no slot was found within the probe limit.
+// (first visit, or the id had been forgotten: when no free slot is found
+// within the probe limit the occupant of the home slot is evicted).
 __device__ inline bool visit_first(uint32_t *vis, uint32_t hmask, uint32_t id) {
-    uint32_t h = fmix32(id) & hmask;
+    const uint32_t h0 = fmix32(id) & hmask;
+    uint32_t h = h0;
     for (int p = 0; p < VIS_PROBES; ++p) {
-        uint32_t old = atomicCAS(&vis[h], ID_NONE, id);
+        const uint32_t old = atomicCAS(&vis[h], ID_NONE, id);
         if (old == ID_NONE) return true;
         if (old == id) return false;
         h = (h + 1) & hmask;
     }
+    atomicExch(&vis[h0], id);
     return true;
 }
 
@@ -88,13 +95,11 @@ __global__ void __launch_bounds__(256) greedy_search_kernel(SearchArgs a) {
     if (q >= a.Q) return;
 
     uint8_t *my = smem + (size_t)wid * a.wave_lds;
-    uint64_t *keyA = reinterpret_cast<uint64_t *>(my);
-    uint64_t *keyB = keyA + a.P;
-    uint64_t *ckey = keyB + a.P;
-    uint32_t *cid = reinterpret_cast<uint32_t *>(ckey + WAVE);
-    uint32_t *vis = cid + WAVE;
-    uint8_t *flagA = reinterpret_cast<uint8_t *>(vis + a.H);
-    uint8_t *flagB = flagA + a.P;
+    uint64_t *pool = reinterpret_cast<uint64_t *>(my);         // P keys
+    uint64_t *ckey = pool + a.P;                                // 64
+    uint32_t *cid = reinterpret_cast<uint32_t *>(ckey + WAVE);  // 64
+    uint32_t *vis = cid + WAVE;                                 // H (>= P)
+    uint8_t *flag = reinterpret_cast<uint8_t *>(vis + a.H);     // P
 
     const uint32_t L = a.L_per_q ? a.L_per_q[q] : a.L;
     const uint32_t kq = a.k_per_q ? a.k_per_q[q] : a.k;
@@ -109,30 +114,26 @@ __global__ void __launch_bounds__(256) greedy_search_kernel(SearchArgs a) {
             qv[c] = ch < (int)nch ? qp[ch] : make_float4(0.f, 0.f, 0.f, 0.f);
         }
     }
-    const uint32_t hmask = a.H - 1;
-    for (uint32_t i = lane; i < a.H; i += WAVE) vis[i] = ID_NONE;
 
-    // ---- entries (SearchEntryRand): not marked visited, all flagged
-    uint32_t *ent = reinterpret_cast<uint32_t *>(keyB);
+    // ---- entries (SearchEntryRand): not marked visited, all flagged; the
+    // visited table's storage holds the entry ids until they are scored
+    uint32_t *ent = vis;
     if (a.entries) {
         for (uint32_t i = lane; i < L; i += WAVE) ent[i] = a.entries[(size_t)q * L + i];
         wave_sync();
     } else {
         gen_entries(a.seed, q, L, N, a.P, ent);
     }
-    rows_dist<CPL, G>(a.base, dp, nch, qv, ent, (int)L, keyA);
-    for (uint32_t i = L + lane; i < a.P; i += WAVE) keyA[i] = KEY_EMPTY;
+    rows_dist<CPL, G>(a.base, dp, nch, qv, ent, (int)L, pool);
+    for (uint32_t i = L + lane; i < a.P; i += WAVE) pool[i] = KEY_EMPTY;
     wave_sync();
-    lds_sort64(keyA, (int)a.P);
-    for (uint32_t i = lane; i < a.P; i += WAVE) {
-        flagA[i] = i < L ? 1 : 0;
-        flagB[i] = 0;
-    }
+    lds_sort64(pool, (int)a.P);
+    const uint32_t hmask = a.H - 1;
+    for (uint32_t i = lane; i < a.H; i += WAVE) vis[i] = ID_NONE;
+    for (uint32_t i = lane; i < a.P; i += WAVE) flag[i] = i < L ? 1 : 0;
     wave_sync();
 
     uint32_t evals = L, hops = 0;
-    uint64_t *cur = keyA, *nxt = keyB;
-    uint8_t *fc = flagA, *fn = flagB;
     int k = 0;
     const int Li = (int)L;
     for (;;) {
@@ -140,7 +141,7 @@ __global__ void __launch_bounds__(256) greedy_search_kernel(SearchArgs a) {
         int kk = Li;
         for (int t0 = (k & ~(WAVE - 1)); t0 < Li; t0 += WAVE) {
             const int p = t0 + lane;
-            const bool f = p >= k && p < Li && fc[p];
+            const bool f = p >= k && p < Li && flag[p];
             const uint64_t b = __ballot(f);
             if (b) {
                 kk = t0 + __ffsll((unsigned long long)b) - 1;
@@ -148,9 +149,9 @@ __global__ void __launch_bounds__(256) greedy_search_kernel(SearchArgs a) {
             }
         }
         if (kk >= Li) break;
-        const uint32_t u = key_id(cur[kk]);
+        const uint32_t u = key_id(pool[kk]);
         wave_sync();
-        if (lane == 0) fc[kk] = 0;
+        if (lane == 0) flag[kk] = 0;
         ++hops;
         uint32_t du = a.deg[u];
         if (du > a.deg_max) du = a.deg_max;
@@ -168,15 +169,17 @@ __global__ void __launch_bounds__(256) greedy_search_kernel(SearchArgs a) {
             rows_dist<CPL, G>(a.base, dp, nch, qv, cid, m, ckey);
             evals += (uint32_t)m;
             wave_sync();
-            const uint64_t worst = cur[Li - 1];
+            const uint64_t worst = pool[Li - 1];
             uint64_t key = lane < m ? ckey[lane] : KEY_EMPTY;
-            if (key >= worst) {
-                key = KEY_EMPTY;
+            int lb = Li;
+            if (key < worst) {
+                lb = lower_bound64(pool, Li, key);
+                if (pool[lb] == key) key = KEY_EMPTY;  // already pooled: duplicate
             } else {
-                const int pos = lower_bound64(cur, Li, key);
-                if (cur[pos] == key) key = KEY_EMPTY;  // already pooled: duplicate
+                key = KEY_EMPTY;
             }
             if (__ballot(key != KEY_EMPTY) == 0) continue;
+            // sort (key, lb) pairs by key: lb is monotone in key, recompute after
             key = wave_sort64(key);
             const uint64_t prev = shfl64(key, lane > 0 ? lane - 1 : 0);
             const bool live = key != KEY_EMPTY && !(lane > 0 && prev == key);
@@ -186,34 +189,44 @@ __global__ void __launch_bounds__(256) greedy_search_kernel(SearchArgs a) {
             wave_sync();
             if (live) ckey[r] = key;
             wave_sync();
-            if (live) {
-                const int pos = r + lower_bound64(cur, Li, key);
-                if (pos < Li) {
-                    nxt[pos] = key;
-                    fn[pos] = 1;
+            // final slot of each live candidate, from the unmoved pool
+            int cpos = Li;
+            if (live) cpos = r + lower_bound64(pool, Li, key);
+            const int pos0 = lower_bound64(pool, Li, ckey[0]);
+            // shift pool keys right by the number of smaller candidates,
+            // 64-slot chunks from the tail back to the chunk holding pos0
+            for (int t0 = (Li - 1) & ~(WAVE - 1); t0 >= (pos0 & ~(WAVE - 1)); t0 -= WAVE) {
+                const int p = t0 + lane;
+                const bool mv = p >= pos0 && p < Li;
+                uint64_t pk = 0;
+                uint8_t pf = 0;
+                int np = Li;
+                if (mv) {
+                    pk = pool[p];
+                    pf = flag[p];
+                    np = p + lower_bound64(ckey, c, pk);
                 }
-            }
-            for (int p = lane; p < Li; p += WAVE) {
-                const uint64_t pk = cur[p];
-                const int pos = p + lower_bound64(ckey, c, pk);
-                if (pos < Li) {
-                    nxt[pos] = pk;
-                    fn[pos] = fc[p];
+                wave_sync();
+                if (mv && np < Li) {
+                    pool[np] = pk;
+                    flag[np] = pf;
                 }
+                wave_sync();
             }
-            const int pos0 = lower_bound64(cur, Li, ckey[0]);
+            if (live && cpos < Li) {
+                pool[cpos] = key;
+                flag[cpos] = 1;
+            }
             if (pos0 < nk) nk = pos0;
             wave_sync();
-            uint64_t *tk = cur; cur = nxt; nxt = tk;
-            uint8_t *tf = fc; fc = fn; fn = tf;
         }
         k = (nk <= kk) ? nk : kk + 1;
     }
 
     for (uint32_t i = lane; i < a.k_stride; i += WAVE) {
         const bool ok = i < kq;
-        a.out_ids[(size_t)q * a.k_stride + i] = ok ? key_id(cur[i]) : ID_NONE;
-        if (a.out_dist) a.out_dist[(size_t)q * a.k_stride + i] = ok ? key_dist(cur[i]) : -1.0f;
+        a.out_ids[(size_t)q * a.k_stride + i] = ok ? key_id(pool[i]) : ID_NONE;
+        if (a.out_dist) a.out_dist[(size_t)q * a.k_stride + i] = ok ? key_dist(pool[i]) : -1.0f;
     }
     if (lane == 0) {
         if (a.evals) a.evals[q] = evals;
@@ -238,7 +251,7 @@ hipError_t launch_t(const SearchArgs &a, int waves_per_block, hipStream_t s) {
 }  // namespace
 
 size_t search_wave_lds(uint32_t P, uint32_t H) {
-    size_t b = (size_t)P * 16 + WAVE * 8 + WAVE * 4 + (size_t)H * 4 + (size_t)P * 2;
+    size_t b = (size_t)P * 8 + WAVE * 8 + WAVE * 4 + (size_t)H * 4 + (size_t)P;
     return (b + 15) & ~(size_t)15;
 }
 
