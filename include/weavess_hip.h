@@ -102,6 +102,54 @@ typedef struct {
     uint32_t visited_slots;    /* LDS visited table size, power of two (0 -> auto)   */
 } wv_search_config;
 
+/* Learned early termination: a 2-layer MLP (8 -> 32 ReLU -> 1) over
+ * per-hop search-state features, evaluated in BF16 (FP32 accumulation)
+ * after every expansion.  Search stops when the logit exceeds
+ * logit_threshold and at least min_hops expansions were made.  Features
+ * g(state), standardised as (g - mean) * inv_std:
+ *   0 log2(1 + hops)          1 kth / entry     2 best / entry
+ *   3 log2(entry)             4 (kth 4 hops ago - kth) / entry
+ *   5 log2(L)                 6 log2(k)         7 log2(|query|)
+ * where entry = best entry distance, kth/best = current k-th/best pool
+ * distance (squared L2, the library's distances). */
+typedef struct {
+    float w1[32][8];
+    float b1[32];
+    float w2[32];
+    float b2;
+    float mean[8];
+    float inv_std[8];
+    float logit_threshold;
+    uint32_t min_hops;
+} wv_term_model;
+
+/* Optional extras of wv_index_search_ext[_dev]. */
+typedef struct {
+    const wv_term_model *model;  /* learned termination (NULL: plain fixed-L)      */
+    float *trace;                /* Q x trace_cap x 8 features g after each hop,
+                                    or NULL (same memory space as the outputs)      */
+    uint32_t *trace_len;         /* Q: rows written (min(hops, trace_cap))          */
+    uint32_t *settle_hop;        /* Q: expansions after which the top-k never
+                                    changed again (0: the entries' top-k is final)  */
+    uint32_t trace_cap;
+} wv_search_ext;
+
+/* MLP training (FP32, Adam, mean binary cross-entropy on logits). */
+typedef struct {
+    uint32_t epochs;
+    uint32_t batch;              /* multiple of 64                                   */
+    float lr;
+    float beta1, beta2, eps;
+    float pos_weight;            /* weight of label-1 samples in the loss            */
+    uint64_t seed;               /* minibatch order                                   */
+} wv_train_params;
+
+typedef struct {
+    double first_loss, last_loss;  /* mean loss of the first and last epoch          */
+    double kernel_ms;
+    uint32_t steps;
+} wv_train_stats;
+
 /* ---------------------------------------------------------------- context */
 int wv_ctx_create(int device, wv_ctx **out);
 int wv_ctx_destroy(wv_ctx *ctx);
@@ -140,6 +188,31 @@ int wv_index_search_dev(wv_index *idx, const float *d_queries, uint32_t Q,
                         const uint32_t *d_L_per_q, const uint32_t *d_k_per_q, uint32_t k_stride,
                         uint32_t *d_out_ids, float *d_out_dist, uint32_t *d_evals,
                         uint32_t *d_hops, wv_search_stats *stats);
+
+/* Greedy search with learned termination and/or per-hop traces. */
+int wv_index_search_ext(wv_index *idx, const float *queries, uint32_t Q, const uint32_t *entries,
+                        uint64_t seed, uint32_t L, uint32_t k, const uint32_t *L_per_q,
+                        const uint32_t *k_per_q, uint32_t k_stride, uint32_t *out_ids,
+                        float *out_dist, uint32_t *evals, uint32_t *hops,
+                        const wv_search_ext *ext, wv_search_stats *stats);
+int wv_index_search_ext_dev(wv_index *idx, const float *d_queries, uint32_t Q,
+                            const uint32_t *d_entries, uint64_t seed, uint32_t L, uint32_t k,
+                            const uint32_t *d_L_per_q, const uint32_t *d_k_per_q,
+                            uint32_t k_stride, uint32_t *d_out_ids, float *d_out_dist,
+                            uint32_t *d_evals, uint32_t *d_hops, const wv_search_ext *ext,
+                            wv_search_stats *stats);
+
+/* ---------------------------------------------------------------- MLP */
+/* X: n x 8 raw features g (standardised with the model's mean/inv_std
+ * inside), y: n labels in {0,1}.  Host arrays. */
+int wv_mlp_grad(wv_ctx *ctx, const wv_term_model *m, const float *X, const float *y, uint32_t n,
+                float pos_weight, float *grad /* 321: w1, b1, w2, b2 */, double *loss);
+int wv_mlp_train(wv_ctx *ctx, const float *X, const float *y, uint32_t n,
+                 const wv_train_params *tp, wv_term_model *model /* in: init; out */,
+                 wv_train_stats *stats);
+/* Logits exactly as the fused search step computes them (BF16 products,
+ * FP32 accumulation). */
+int wv_mlp_predict(wv_ctx *ctx, const wv_term_model *m, const float *X, uint32_t n, float *logits);
 
 /* ---------------------------------------------------------------- one-shot */
 /* The two stage entry points as single calls on host arrays. */

@@ -28,6 +28,13 @@ class NndParams(C.Structure):
                 ("cand_slots", C.c_uint32), ("seed", C.c_uint64)]
 
 
+class TermModel(C.Structure):
+    """Same layout as wv_term_model / wvo_term_model."""
+    _fields_ = [("w1", (C.c_float * 8) * 32), ("b1", C.c_float * 32), ("w2", C.c_float * 32),
+                ("b2", C.c_float), ("mean", C.c_float * 8), ("inv_std", C.c_float * 8),
+                ("logit_threshold", C.c_float), ("min_hops", C.c_uint32)]
+
+
 class NndIterStats(C.Structure):
     _fields_ = [("pairs", C.c_uint64), ("inserts", C.c_uint64)]
 
@@ -76,6 +83,19 @@ def lib():
                                         _f32p, C.c_uint32, C.c_void_p, C.c_uint64,
                                         C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p,
                                         C.c_uint32, _u32p, _f32p, _u32p, _u32p, C.c_int]
+        L.wvo_search_greedy_ext.restype = C.c_int
+        L.wvo_search_greedy_ext.argtypes = [_u32p, _u32p, C.c_uint32, _f32p, C.c_uint32, C.c_uint32,
+                                            _f32p, C.c_uint32, C.c_void_p, C.c_uint64,
+                                            C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p,
+                                            C.c_uint32, _u32p, _f32p, _u32p, _u32p, C.c_void_p,
+                                            C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_int]
+        L.wvo_term_features.restype = None
+        L.wvo_term_features.argtypes = [C.c_float] * 8 + [_f32p]
+        L.wvo_mlp_grad.restype = C.c_int
+        L.wvo_mlp_grad.argtypes = [C.c_void_p, _f32p, _f32p, C.c_uint32, C.c_float, _f64p,
+                                   C.POINTER(C.c_double)]
+        L.wvo_mlp_logit.restype = C.c_int
+        L.wvo_mlp_logit.argtypes = [C.c_void_p, _f32p, C.c_uint32, C.c_int, _f64p]
         L.wvo_exact_knn.restype = C.c_int
         L.wvo_exact_knn.argtypes = [_f32p, C.c_uint32, _f32p, C.c_uint32, C.c_uint32,
                                     C.c_uint32, C.c_int64, _u32p, _f64p, C.c_int]
@@ -245,3 +265,57 @@ def graph_quality(ids, deg, gt, rows=None, control=10) -> float:
     r = None if rows is None else _c(rows, np.uint32)
     return float(lib().wvo_graph_quality(ids, deg, ids.shape[1], _ptr(r), gt.shape[0], gt,
                                          gt.shape[1], control))
+
+
+# ---------------------------------------------------------------- learned termination
+def search_greedy_ext(graph, deg, base, queries, L, k, seed=0, entries=None, model=None,
+                      trace_cap=0, L_per_q=None, k_per_q=None, nthreads: int = 0):
+    """Greedy search with per-hop feature traces and/or the learned stop."""
+    graph = _c(graph, np.uint32)
+    deg = _c(deg, np.uint32)
+    base = _c(base, np.float32)
+    queries = _c(queries, np.float32)
+    N, d = base.shape
+    Q = queries.shape[0]
+    k_stride = int(np.max(k_per_q)) if k_per_q is not None else k
+    out_ids = np.zeros((Q, k_stride), np.uint32)
+    out_dist = np.zeros((Q, k_stride), np.float32)
+    evals = np.zeros(Q, np.uint32)
+    hops = np.zeros(Q, np.uint32)
+    ent = None if entries is None else _c(entries, np.uint32)
+    lq = None if L_per_q is None else _c(L_per_q, np.uint32)
+    kq = None if k_per_q is None else _c(k_per_q, np.uint32)
+    trace = np.zeros((Q, max(trace_cap, 1), 8), np.float32) if trace_cap else None
+    tlen = np.zeros(Q, np.uint32)
+    settle = np.zeros(Q, np.uint32)
+    rc = lib().wvo_search_greedy_ext(graph, deg, graph.shape[1], base, N, d, queries, Q, _ptr(ent), seed,
+                                     L, k, _ptr(lq), _ptr(kq), k_stride, out_ids, out_dist, evals, hops,
+                                     None if model is None else C.cast(C.byref(model), C.c_void_p),
+                                     _ptr(trace), _ptr(tlen), _ptr(settle), trace_cap, nthreads)
+    if rc != 0:
+        raise ValueError("search_greedy_ext: bad arguments")
+    return out_ids, out_dist, evals, hops, trace, tlen, settle
+
+
+def term_features(hops, kth, best, entry, kth4, L, k, qnorm):
+    f = np.zeros(8, np.float32)
+    lib().wvo_term_features(hops, kth, best, entry, kth4, L, k, qnorm, f)
+    return f
+
+
+def mlp_grad(model, X, y, pos_weight=1.0):
+    X = _c(X, np.float32)
+    y = _c(y, np.float32)
+    g = np.zeros(321, np.float64)
+    loss = C.c_double(0.0)
+    if lib().wvo_mlp_grad(C.cast(C.byref(model), C.c_void_p), X, y, X.shape[0], pos_weight, g,
+                          C.byref(loss)) != 0:
+        raise ValueError("mlp_grad: bad arguments")
+    return g, loss.value
+
+
+def mlp_logit(model, X, bf16=True):
+    X = _c(X, np.float32)
+    out = np.zeros(X.shape[0], np.float64)
+    lib().wvo_mlp_logit(C.cast(C.byref(model), C.c_void_p), X, X.shape[0], 1 if bf16 else 0, out)
+    return out

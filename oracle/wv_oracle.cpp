@@ -535,12 +535,117 @@ int wvo_nndescent(const float *base, uint32_t N, uint32_t d, const wvo_nnd_param
 /* ---------------------------------------------------------------------- */
 /* Greedy search                                                           */
 /* ---------------------------------------------------------------------- */
+namespace {
+inline float bf16r(float x) {
+    /* round to nearest even on the upper 16 bits (finite inputs) */
+    uint32_t u;
+    std::memcpy(&u, &x, 4);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    u &= 0xFFFF0000u;
+    float r;
+    std::memcpy(&r, &u, 4);
+    return r;
+}
+/* The device's BF16 inference step (wv_mlp.hpp term_logit). */
+float logit_bf16(const wvo_term_model *m, const float *f) {
+    float part[32];
+    for (int j = 0; j < 32; ++j) {
+        float acc = m->b1[j];
+        for (int i = 0; i < 8; ++i) acc = acc + bf16r(m->w1[j][i]) * bf16r((f[i] - m->mean[i]) * m->inv_std[i]);
+        const float h = acc > 0.0f ? acc : 0.0f;
+        part[j] = bf16r(m->w2[j]) * bf16r(h);
+    }
+    for (int h = 16; h >= 1; h >>= 1)
+        for (int j = 0; j < h; ++j) part[j] = part[j] + part[j + h];
+    return part[0] + m->b2;
+}
+}  // namespace
+
+void wvo_term_features(float hops, float kth, float best, float entry, float kth4, float L, float k,
+                       float qnorm, float *f) {
+    /* weavess_hip.h: g = [log2(1+hops), kth/entry, best/entry, log2(entry),
+     * (kth4-kth)/entry, log2(L), log2(k), log2(|q|)] */
+    const float e = entry > 1e-30f ? entry : 1e-30f;
+    f[0] = std::log2(1.0f + hops);
+    f[1] = kth / e;
+    f[2] = best / e;
+    f[3] = std::log2(e);
+    f[4] = (kth4 - kth) / e;
+    f[5] = std::log2(L);
+    f[6] = std::log2(k);
+    f[7] = std::log2(qnorm > 1e-30f ? qnorm : 1e-30f);
+}
+
+int wvo_mlp_logit(const wvo_term_model *m, const float *X, uint32_t n, int bf16, double *out) {
+    if (!m || !X || !out) return -1;
+    for (uint32_t s = 0; s < n; ++s) {
+        const float *f = X + (size_t)s * 8;
+        if (bf16) {
+            out[s] = logit_bf16(m, f);
+            continue;
+        }
+        double z = m->b2;
+        for (int j = 0; j < 32; ++j) {
+            double a = m->b1[j];
+            for (int i = 0; i < 8; ++i) a += (double)m->w1[j][i] * (((double)f[i] - m->mean[i]) * m->inv_std[i]);
+            z += (double)m->w2[j] * (a > 0 ? a : 0);
+        }
+        out[s] = z;
+    }
+    return 0;
+}
+
+int wvo_mlp_grad(const wvo_term_model *m, const float *X, const float *y, uint32_t n, float pos_weight,
+                 double *grad, double *loss) {
+    if (!m || !X || !y || !grad || n == 0) return -1;
+    for (int p = 0; p < 321; ++p) grad[p] = 0.0;
+    double L = 0.0;
+    for (uint32_t s = 0; s < n; ++s) {
+        double x[8], z1[32], h[32];
+        for (int i = 0; i < 8; ++i) x[i] = ((double)X[(size_t)s * 8 + i] - m->mean[i]) * m->inv_std[i];
+        double z = m->b2;
+        for (int j = 0; j < 32; ++j) {
+            double a = m->b1[j];
+            for (int i = 0; i < 8; ++i) a += (double)m->w1[j][i] * x[i];
+            z1[j] = a;
+            h[j] = a > 0 ? a : 0;
+            z += (double)m->w2[j] * h[j];
+        }
+        const double t = y[s], w = t > 0.5 ? pos_weight : 1.0;
+        L += w * (std::max(z, 0.0) - z * t + std::log1p(std::exp(-std::fabs(z))));
+        const double dz = w * (1.0 / (1.0 + std::exp(-z)) - t);
+        for (int j = 0; j < 32; ++j) {
+            const double dz1 = z1[j] > 0 ? dz * m->w2[j] : 0.0;
+            for (int i = 0; i < 8; ++i) grad[j * 8 + i] += dz1 * x[i];
+            grad[256 + j] += dz1;
+            grad[288 + j] += dz * h[j];
+        }
+        grad[320] += dz;
+    }
+    for (int p = 0; p < 321; ++p) grad[p] /= n;
+    if (loss) *loss = L / n;
+    return 0;
+}
+
 int wvo_search_greedy(const uint32_t *graph, const uint32_t *deg, uint32_t deg_max,
                       const float *base, uint32_t N, uint32_t d, const float *queries, uint32_t Q,
                       const uint32_t *entries, uint64_t seed, uint32_t L, uint32_t k,
                       const uint32_t *L_per_q, const uint32_t *k_per_q, uint32_t k_stride,
                       uint32_t *out_ids, float *out_dist, uint32_t *evals, uint32_t *hops,
                       int nthreads) {
+    return wvo_search_greedy_ext(graph, deg, deg_max, base, N, d, queries, Q, entries, seed, L, k,
+                                 L_per_q, k_per_q, k_stride, out_ids, out_dist, evals, hops, nullptr,
+                                 nullptr, nullptr, nullptr, 0, nthreads);
+}
+
+int wvo_search_greedy_ext(const uint32_t *graph, const uint32_t *deg, uint32_t deg_max,
+                          const float *base, uint32_t N, uint32_t d, const float *queries,
+                          uint32_t Q, const uint32_t *entries, uint64_t seed, uint32_t L,
+                          uint32_t k, const uint32_t *L_per_q, const uint32_t *k_per_q,
+                          uint32_t k_stride, uint32_t *out_ids, float *out_dist,
+                          uint32_t *evals, uint32_t *hops, const wvo_term_model *model,
+                          float *trace, uint32_t *trace_len, uint32_t *settle, uint32_t cap,
+                          int nthreads) {
     if (!graph || !deg || !base || !queries || !out_ids || N == 0 || d == 0) return -1;
     /* validate all queries first: L >= k (src/builder.cpp:435-438), L < N
      * for random entries (GenRandom needs N - L > 0), L <= N otherwise */
@@ -578,11 +683,22 @@ int wvo_search_greedy(const uint32_t *graph, const uint32_t *deg, uint32_t deg_m
                 ++ev;
             }
             std::sort(key.begin(), key.begin() + Lq);
+            /* state features for traces / the learned stop (weavess_hip.h) */
+            const bool watch = model || trace;
+            const uint32_t kidx = (kq < Lq ? kq : Lq) - 1;
+            float entry_d = key_dist(key[0]);
+            std::vector<float> zero(d, 0.0f);
+            const float qnorm = watch ? std::sqrt(dist_canon(qv, zero.data(), d)) : 0.0f;
+            float hist[4];  /* k-th distance 1..4 expansions ago */
+            for (float &x : hist) x = key_dist(key[kidx]);
+            uint32_t st = 0;
             /* RouteGreedy, component_route.cpp:16-47 */
             uint32_t kk = 0;
             while (kk < Lq) {
                 uint32_t nk = Lq;
+                bool expanded = false;
                 if (flag[kk]) {
+                    expanded = true;
                     flag[kk] = 0;
                     ++hp;
                     uint32_t u = key_id(key[kk]);
@@ -611,10 +727,25 @@ int wvo_search_greedy(const uint32_t *graph, const uint32_t *deg, uint32_t deg_m
                             r = pos;
                         }
                         if (r < nk) nk = r;
+                        if (r <= kidx) st = hp;
                     }
+                }
+                if (expanded && watch) {
+                    float f[8];
+                    const float kth = key_dist(key[kidx]);
+                    wvo_term_features((float)hp, kth, key_dist(key[0]), entry_d, hist[3], (float)Lq, (float)kq, qnorm, f);
+                    hist[3] = hist[2];
+                    hist[2] = hist[1];
+                    hist[1] = hist[0];
+                    hist[0] = kth;
+                    if (trace && hp <= cap)
+                        for (int i = 0; i < 8; ++i) trace[((size_t)q * cap + (hp - 1)) * 8 + i] = f[i];
+                    if (model && hp >= model->min_hops && logit_bf16(model, f) > model->logit_threshold) break;
                 }
                 kk = (nk <= kk) ? nk : kk + 1;
             }
+            if (trace_len) trace_len[q] = hp < cap ? hp : cap;
+            if (settle) settle[q] = st;
             for (uint32_t i = 0; i < k_stride; ++i) {
                 out_ids[(size_t)q * k_stride + i] = i < kq ? key_id(key[i]) : 0xFFFFFFFFu;
                 if (out_dist) out_dist[(size_t)q * k_stride + i] = i < kq ? key_dist(key[i]) : -1.0f;

@@ -94,6 +94,39 @@ int wvo_search_greedy(const uint32_t *graph, const uint32_t *deg,
                       uint32_t *out_ids, float *out_dist,
                       uint32_t *evals, uint32_t *hops, int nthreads);
 
+/* Learned termination (same layout as wv_term_model) --------------------- */
+typedef struct {
+    float w1[32][8];
+    float b1[32];
+    float w2[32];
+    float b2;
+    float mean[8];
+    float inv_std[8];
+    float logit_threshold;
+    uint32_t min_hops;
+} wvo_term_model;
+
+/* Search-state features g, FP32 (restates the device formula). */
+void wvo_term_features(float hops, float kth, float best, float entry, float kth4, float L,
+                       float k, float qnorm, float *f8);
+/* Gradient (321: w1, b1, w2, b2) and mean weighted BCE in FP64. */
+int wvo_mlp_grad(const wvo_term_model *m, const float *X, const float *y, uint32_t n,
+                 float pos_weight, double *grad, double *loss);
+/* Logits: bf16 != 0 follows the device's BF16-product / FP32-sum order,
+ * bf16 == 0 is plain FP64. */
+int wvo_mlp_logit(const wvo_term_model *m, const float *X, uint32_t n, int bf16, double *out);
+
+/* Greedy search with per-hop traces and/or the learned stop rule.  model,
+ * trace (Q*cap*8), trace_len, settle may be NULL. */
+int wvo_search_greedy_ext(const uint32_t *graph, const uint32_t *deg, uint32_t deg_max,
+                          const float *base, uint32_t N, uint32_t d, const float *queries,
+                          uint32_t Q, const uint32_t *entries, uint64_t seed, uint32_t L,
+                          uint32_t k, const uint32_t *L_per_q, const uint32_t *k_per_q,
+                          uint32_t k_stride, uint32_t *out_ids, float *out_dist,
+                          uint32_t *evals, uint32_t *hops, const wvo_term_model *model,
+                          float *trace, uint32_t *trace_len, uint32_t *settle, uint32_t cap,
+                          int nthreads);
+
 /* Exact kNN ------------------------------------------------------------- */
 /* FP64 brute force, ties by id.  self_offset >= 0: query q is base row
  * q+self_offset and is excluded (kNN graph); -1: no exclusion. */

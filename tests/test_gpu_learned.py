@@ -1,0 +1,141 @@
+"""Learned early termination on the GPU: traces, MLP kernels vs the FP64
+oracle, the fused BF16 stop step, and what the network gains."""
+import ctypes
+
+import numpy as np
+import pytest
+
+import oracle as O
+from weavess_amd import Index, learned
+from weavess_amd._capi import TermModel
+
+pytestmark = pytest.mark.gpu
+
+
+def _as_oracle(m):
+    o = O.TermModel()
+    ctypes.memmove(ctypes.addressof(o), ctypes.addressof(m), ctypes.sizeof(o))
+    return o
+
+
+def _random_model(seed):
+    g = np.random.default_rng(seed)
+    m = TermModel.init(seed)
+    for i in range(8):
+        m.mean[i] = float(g.normal(0, 0.2))
+        m.inv_std[i] = float(g.uniform(0.5, 2.0))
+    p = m.params()
+    p[256:288] = g.normal(0, 0.1, 32)
+    p[320] = 0.1
+    m.set_params(p)
+    return m
+
+
+def test_traces_match_oracle(ctx, small_float, small_float_graph):
+    base, q = small_float
+    g, _, deg = small_float_graph
+    ix = Index(ctx, base)
+    ix.set_graph(g, deg)
+    ids, dist, ev, hp, st, trace, tlen, settle = ix.search_ext(q, 48, 10, seed=5, trace_cap=160)
+    rids, _, rev, rhp, rtrace, rtlen, rsettle = O.search_greedy_ext(g, deg, base, q, 48, 10, seed=5,
+                                                                    trace_cap=160)
+    assert np.array_equal(ids, rids) and np.array_equal(hp, rhp)
+    assert np.array_equal(tlen, rtlen) and np.array_equal(settle, rsettle)
+    for r in range(len(q)):
+        n = tlen[r]
+        np.testing.assert_allclose(trace[r, :n], rtrace[r, :n], rtol=2e-6, atol=1e-6)
+    ix.close()
+
+
+def test_mlp_grad_matches_fp64_oracle(ctx):
+    g = np.random.default_rng(11)
+    X = g.normal(size=(1000, 8)).astype(np.float32)
+    y = (g.uniform(size=1000) < 0.3).astype(np.float32)
+    m = _random_model(3)
+    gg, gl = ctx.mlp_grad(m, X, y, pos_weight=2.0)
+    rg, rl = O.mlp_grad(_as_oracle(m), X, y, 2.0)
+    assert abs(gl - rl) <= 1e-5 * max(1.0, abs(rl))
+    np.testing.assert_allclose(gg, rg, rtol=1e-4, atol=1e-6)
+
+
+def test_mlp_predict_is_the_bf16_step(ctx):
+    g = np.random.default_rng(12)
+    X = g.normal(size=(777, 8)).astype(np.float32)
+    m = _random_model(4)
+    z = ctx.mlp_predict(m, X)
+    rz = O.mlp_logit(_as_oracle(m), X, bf16=True)
+    np.testing.assert_allclose(z, rz, rtol=1e-6, atol=1e-6)
+
+
+def test_mlp_training_learns_a_separable_rule(ctx):
+    g = np.random.default_rng(13)
+    X = g.normal(size=(20000, 8)).astype(np.float32)
+    y = ((X[:, 0] - 0.5 * X[:, 3] + 0.2 * X[:, 5] ** 2) > 0.3).astype(np.float32)
+    m = TermModel.init(5)
+    st = ctx.mlp_train(m, X, y, epochs=15, batch=1024, lr=3e-3, seed=1)
+    assert st["last_loss"] < 0.5 * st["first_loss"]
+    acc = ((ctx.mlp_predict(m, X) > 0) == (y > 0.5)).mean()
+    assert acc > 0.95, acc
+    # the device's final loss equals the FP64 oracle loss of the trained weights
+    _, rl = O.mlp_grad(_as_oracle(m), X, y, 1.0)
+    _, gl = ctx.mlp_grad(m, X, y)
+    assert abs(gl - rl) < 1e-4
+
+
+def _c4_like(n=20000, nq=600, seed=21):
+    g = np.random.default_rng(seed)
+    c = g.normal(size=(200, 96))
+    c /= np.linalg.norm(c, axis=1, keepdims=True)
+
+    def draw(m):
+        x = c[g.integers(0, 200, m)] + g.normal(0, 0.3, (m, 96))
+        return (x / np.linalg.norm(x, axis=1, keepdims=True)).astype(np.float32)
+    return draw(n), draw(nq)
+
+
+@pytest.fixture(scope="module")
+def c4_small(ctx):
+    base, q = _c4_like()
+    ix = Index(ctx, base)
+    ix.build_nndescent(32, 48, 16, 48, 10, seed=1)
+    gt, _ = O.exact_knn(base, q, 10)
+    yield base, q, gt, ix
+    ix.close()
+
+
+def test_learned_search_matches_oracle_stop_rule(ctx, c4_small):
+    base, q, gt, ix = c4_small
+    g, _, deg = ix.get_graph()
+    X, y, _ = learned.collect_traces(ix, q[:300], 200, 10, seed=3)
+    m, _ = learned.train(ctx, X, y, epochs=10, batch=1024, seed=2)
+    m.logit_threshold = 0.0
+    ids, _, ev, hp, *_ = ix.search_ext(q[300:], 200, 10, seed=4, model=m)
+    rids, _, rev, rhp, *_ = O.search_greedy_ext(g, deg, base, q[300:], 200, 10, seed=4, model=_as_oracle(m))
+    # features use the device's / host's log2: a logit within rounding of the
+    # threshold may stop one expansion apart; everything else must agree
+    same = (hp == rhp)
+    assert same.mean() > 0.97
+    assert np.array_equal(ids[same], rids[same])
+
+
+def test_learned_termination_gains_at_equal_recall(ctx, c4_small):
+    """Plain fixed-L search needs L_plain for the target recall; the learned
+    stop rule with a larger pool reaches the same recall with fewer distance
+    evaluations (the work the network saves)."""
+    base, q, gt, ix = c4_small
+    train_q, cal_q, test_q = q[:200], q[200:400], q[400:]
+    target = 0.95
+    plain = {}
+    for L in (40, 60, 80, 100, 130, 160, 200, 260):
+        ids, _, ev, *_ = ix.search(test_q, L, 10, seed=9)
+        plain[L] = (learned.recall_at_k(ids, gt[400:], 10), float(ev.mean()))
+    L_plain = min(L for L, (r, _) in plain.items() if r >= target)
+    X, y, _ = learned.collect_traces(ix, train_q, 260, 10, seed=3)
+    m, st = learned.train(ctx, X, y, epochs=20, batch=1024, seed=1)
+    cal = learned.calibrate(ix, cal_q, gt[200:400], m, 260, 10, target + 0.01, seed=5)
+    assert cal is not None
+    ids, _, ev, *_ = ix.search_ext(test_q, 260, 10, seed=9, model=m)
+    r_learned = learned.recall_at_k(ids, gt[400:], 10)
+    print("plain", L_plain, plain[L_plain], "learned", r_learned, ev.mean(), "threshold", cal)
+    assert r_learned >= target - 0.01
+    assert ev.mean() < 0.9 * plain[L_plain][1]

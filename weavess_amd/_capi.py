@@ -29,6 +29,8 @@ EXPORTS = [
     "wv_index_create", "wv_index_destroy", "wv_index_set_graph", "wv_index_get_graph",
     "wv_index_set_search_config", "wv_index_build_nndescent", "wv_index_search",
     "wv_index_search_dev", "wv_nndescent_build", "wv_search_greedy", "wv_exact_knn",
+    "wv_index_search_ext", "wv_index_search_ext_dev", "wv_mlp_grad", "wv_mlp_train",
+    "wv_mlp_predict",
 ]
 
 
@@ -71,6 +73,53 @@ class SearchConfig(C.Structure):
                 ("visited_slots", C.c_uint32)]
 
 
+class TermModel(C.Structure):
+    """wv_term_model: 8 -> 32 (ReLU) -> 1 network, standardisation, stop rule."""
+    _fields_ = [("w1", (C.c_float * 8) * 32), ("b1", C.c_float * 32), ("w2", C.c_float * 32),
+                ("b2", C.c_float), ("mean", C.c_float * 8), ("inv_std", C.c_float * 8),
+                ("logit_threshold", C.c_float), ("min_hops", C.c_uint32)]
+
+    def params(self) -> np.ndarray:
+        """The 321 trainable parameters (w1, b1, w2, b2) as float32."""
+        return np.frombuffer(bytes(self), np.float32, 321).copy()
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, np.float32)
+        assert p.size == 321
+        C.memmove(C.addressof(self), p.ctypes.data, 321 * 4)
+
+    @classmethod
+    def init(cls, seed=0):
+        """He-initialised weights, zero biases, no standardisation yet."""
+        g = np.random.default_rng(seed)
+        m = cls()
+        p = np.zeros(321, np.float32)
+        p[:256] = g.normal(0, np.sqrt(2.0 / 8), 256)
+        p[288:320] = g.normal(0, np.sqrt(1.0 / 32), 32)
+        m.set_params(p)
+        m.logit_threshold = 1e30
+        return m
+
+
+class SearchExt(C.Structure):
+    _fields_ = [("model", C.c_void_p), ("trace", C.c_void_p), ("trace_len", C.c_void_p),
+                ("settle_hop", C.c_void_p), ("trace_cap", C.c_uint32)]
+
+
+class TrainParams(C.Structure):
+    _fields_ = [("epochs", C.c_uint32), ("batch", C.c_uint32), ("lr", C.c_float),
+                ("beta1", C.c_float), ("beta2", C.c_float), ("eps", C.c_float),
+                ("pos_weight", C.c_float), ("seed", C.c_uint64)]
+
+
+class TrainStats(C.Structure):
+    _fields_ = [("first_loss", C.c_double), ("last_loss", C.c_double), ("kernel_ms", C.c_double),
+                ("steps", C.c_uint32)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
 _lib = None
 
 
@@ -100,6 +149,12 @@ def load() -> C.CDLL:
         "wv_search_greedy": ([vp, vp, vp, u32, vp, u32, vp, u64, u32, u32, vp, vp, u32, vp, vp, vp,
                               vp, vp], i32),
         "wv_exact_knn": ([vp, vp, u32, vp, u32, u32, u32, C.c_int64, vp, vp, vp], i32),
+        "wv_index_search_ext": ([vp, vp, u32, vp, u64, u32, u32, vp, vp, u32, vp, vp, vp, vp, vp, vp], i32),
+        "wv_index_search_ext_dev": ([vp, vp, u32, vp, u64, u32, u32, vp, vp, u32, vp, vp, vp, vp, vp, vp],
+                                    i32),
+        "wv_mlp_grad": ([vp, vp, vp, vp, u32, C.c_float, vp, C.POINTER(C.c_double)], i32),
+        "wv_mlp_train": ([vp, vp, vp, u32, vp, vp, vp], i32),
+        "wv_mlp_predict": ([vp, vp, vp, u32, vp], i32),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)
@@ -154,6 +209,32 @@ class Context:
         self.check(self.lib.wv_exact_knn(self.h, _p(base), N, _p(queries), Q, d, k, self_offset,
                                          _p(ids), _p(dist), C.byref(ms)))
         return ids, dist, ms.value
+
+    def mlp_grad(self, model: "TermModel", X, y, pos_weight=1.0):
+        X = _c(X, np.float32)
+        y = _c(y, np.float32)
+        g = np.zeros(321, np.float32)
+        loss = C.c_double(0.0)
+        self.check(self.lib.wv_mlp_grad(self.h, C.byref(model), _p(X), _p(y), X.shape[0], pos_weight,
+                                        _p(g), C.byref(loss)))
+        return g, loss.value
+
+    def mlp_train(self, model: "TermModel", X, y, epochs=20, batch=4096, lr=3e-3, pos_weight=1.0,
+                  seed=0, beta1=0.9, beta2=0.999, eps=1e-8):
+        """Train `model` in place on the device; returns the training statistics."""
+        X = _c(X, np.float32)
+        y = _c(y, np.float32)
+        tp = TrainParams(epochs, batch, lr, beta1, beta2, eps, pos_weight, seed)
+        st = TrainStats()
+        self.check(self.lib.wv_mlp_train(self.h, _p(X), _p(y), X.shape[0], C.byref(tp), C.byref(model),
+                                         C.byref(st)))
+        return st.as_dict()
+
+    def mlp_predict(self, model: "TermModel", X):
+        X = _c(X, np.float32)
+        z = np.zeros(X.shape[0], np.float32)
+        self.check(self.lib.wv_mlp_predict(self.h, C.byref(model), _p(X), X.shape[0], _p(z)))
+        return z
 
     def nndescent_build(self, base, K, L, S, R, iters, seed=0, dtype="f32", rerank_mult=0,
                         cand_slots=0, init_ids=None):
@@ -259,6 +340,46 @@ class Index:
                                                     _p(lq), _p(kq), k_stride, _p(ids), _p(dist),
                                                     _p(ev), _p(hp), C.byref(st)))
         return ids, dist, ev, hp, st.as_dict()
+
+    def search_ext(self, queries, L, k, seed=0, model: Optional["TermModel"] = None, trace_cap=0,
+                   want_settle=False, L_per_q=None, k_per_q=None, k_stride=None, entries=None):
+        """Search with the learned stop rule and/or per-hop feature traces.
+
+        Returns ids, dist, evals, hops, stats, trace (Q x cap x 8 or None),
+        trace_len, settle_hop (or None)."""
+        queries = _c(queries, np.float32)
+        Q = queries.shape[0]
+        if k_stride is None:
+            k_stride = int(np.max(k_per_q)) if k_per_q is not None else k
+        ids = np.zeros((Q, k_stride), np.uint32)
+        dist = np.zeros((Q, k_stride), np.float32)
+        ev = np.zeros(Q, np.uint32)
+        hp = np.zeros(Q, np.uint32)
+        trace = np.zeros((Q, trace_cap, 8), np.float32) if trace_cap else None
+        tlen = np.zeros(Q, np.uint32) if trace_cap else None
+        settle = np.zeros(Q, np.uint32) if (want_settle or trace_cap) else None
+        ext = SearchExt(C.addressof(model) if model is not None else None, _p(trace) and _p(trace).value,
+                        _p(tlen) and _p(tlen).value, _p(settle) and _p(settle).value, trace_cap)
+        st = SearchStats()
+        e, lq, kq = _c(entries, np.uint32), _c(L_per_q, np.uint32), _c(k_per_q, np.uint32)
+        self.ctx.check(self.ctx.lib.wv_index_search_ext(self.h, _p(queries), Q, _p(e), seed, L, k, _p(lq),
+                                                        _p(kq), k_stride, _p(ids), _p(dist), _p(ev), _p(hp),
+                                                        C.byref(ext), C.byref(st)))
+        return ids, dist, ev, hp, st.as_dict(), trace, tlen, settle
+
+    def search_ext_dev(self, d_queries: int, Q: int, L: int, k: int, d_out_ids: int,
+                       model: Optional["TermModel"] = None, d_out_dist: int = 0, seed: int = 0,
+                       d_L: int = 0, d_k: int = 0, k_stride: Optional[int] = None, d_evals: int = 0,
+                       d_hops: int = 0, want_stats: bool = True):
+        """search_dev with the learned stop rule (model is a host struct)."""
+        st = SearchStats()
+        vp = lambda x: C.c_void_p(x) if x else None
+        ext = SearchExt(C.addressof(model) if model is not None else None, None, None, None, 0)
+        self.ctx.check(self.ctx.lib.wv_index_search_ext_dev(
+            self.h, vp(d_queries), Q, None, seed, L, k, vp(d_L), vp(d_k),
+            k_stride if k_stride is not None else k, vp(d_out_ids), vp(d_out_dist),
+            vp(d_evals), vp(d_hops), C.byref(ext), C.byref(st) if want_stats else None))
+        return st.as_dict()
 
     def search_dev(self, d_queries: int, Q: int, L: int, k: int, d_out_ids: int,
                    d_out_dist: int = 0, seed: int = 0, d_entries: int = 0, d_L: int = 0,

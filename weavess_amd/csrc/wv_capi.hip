@@ -3,14 +3,19 @@
 #include "../../include/weavess_hip.h"
 #include "wv_device.hpp"
 #include "wv_internal.hpp"
+#include "wv_mlp.hpp"
 #include "wv_nnd.hpp"
 
 #include <algorithm>
+#include <cmath>
+#include <cstddef>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
 #include <mutex>
 #include <new>
+#include <numeric>
+#include <random>
 #include <string>
 #include <vector>
 
@@ -113,10 +118,18 @@ int plan_search(wv_index *ix, uint32_t Lmax, SearchPlan &pl) {
 }
 
 // Core search on device buffers (caller holds the context lock).
+struct SearchExtDev {
+    const wv_term_model *model = nullptr;  // device copy
+    float *trace = nullptr;
+    uint32_t *trace_len = nullptr, *settle = nullptr;
+    uint32_t cap = 0;
+};
+
 int search_core(wv_index *ix, const float *d_q, uint32_t Q, const uint32_t *d_entries,
                 uint64_t seed, uint32_t L, uint32_t k, const uint32_t *d_L, const uint32_t *d_k,
                 uint32_t Lmax, uint32_t k_stride, uint32_t *d_ids, float *d_dist,
-                uint32_t *d_evals, uint32_t *d_hops, wv_search_stats *stats) {
+                uint32_t *d_evals, uint32_t *d_hops, wv_search_stats *stats,
+                const SearchExtDev &ext = SearchExtDev()) {
     wv_ctx *c = ix->ctx;
     if (!ix->graph) return fail(c, WV_EINVAL, "index has no graph (build or set one first)");
     if (Q == 0) {
@@ -162,6 +175,11 @@ int search_core(wv_index *ix, const float *d_q, uint32_t Q, const uint32_t *d_en
     a.P = pl.P;
     a.H = pl.H;
     a.wave_lds = (uint32_t)wv::search_wave_lds(pl.P, pl.H);
+    a.model = ext.model;
+    a.trace = ext.trace;
+    a.trace_len = ext.trace_len;
+    a.settle = ext.settle;
+    a.trace_cap = ext.cap;
     WV_HIP(c, hipEventRecord(c->ev0, c->stream));
     WV_HIP(c, wv::launch_greedy_search(a, cpl, (int)pl.groups, (int)pl.wpb, c->stream));
     WV_HIP(c, hipEventRecord(c->ev1, c->stream));
@@ -210,6 +228,11 @@ int check_entries(wv_ctx *c, const uint32_t *entries, size_t n, uint32_t N) {
     for (size_t i = 0; i < n; ++i)
         if (entries[i] >= N) return fail(c, WV_EINVAL, "entry %zu is %u, not below N=%u", i, entries[i], N);
     return WV_OK;
+}
+
+// Address of a member of a device-resident model (no host dereference).
+const float *model_field(const void *dev_model, size_t offset) {
+    return reinterpret_cast<const float *>(static_cast<const char *>(dev_model) + offset);
 }
 
 }  // namespace
@@ -441,6 +464,268 @@ int wv_index_search_dev(wv_index *ix, const float *d_queries, uint32_t Q, const 
     }
     return search_core(ix, d_queries, Q, d_entries, seed, L, k, d_L_per_q, d_k_per_q, Lmax,
                        k_stride, d_out_ids, d_out_dist, d_evals, d_hops, stats);
+}
+
+int wv_index_search_ext(wv_index *ix, const float *queries, uint32_t Q, const uint32_t *entries,
+                        uint64_t seed, uint32_t L, uint32_t k, const uint32_t *L_per_q,
+                        const uint32_t *k_per_q, uint32_t k_stride, uint32_t *out_ids,
+                        float *out_dist, uint32_t *evals, uint32_t *hops,
+                        const wv_search_ext *ext, wv_search_stats *stats) {
+    if (!ix) return WV_EINVAL;
+    wv_ctx *c = ix->ctx;
+    std::lock_guard<std::mutex> g(c->mu);
+    if (!queries || !out_ids) return fail(c, WV_EINVAL, "queries/out_ids are null");
+    if (ext && ext->trace && (!ext->trace_len || ext->trace_cap == 0))
+        return fail(c, WV_EINVAL, "trace needs trace_len and trace_cap > 0");
+    uint32_t Lmax = 0;
+    int rc = check_search_args(c, ix->N, Q, entries, L, k, L_per_q, k_per_q, k_stride, Lmax);
+    if (rc) return rc;
+    if (entries && (rc = check_entries(c, entries, (size_t)Q * L, ix->N))) return rc;
+    WV_HIP(c, hipSetDevice(c->device));
+    if (Q == 0) {
+        if (stats) *stats = wv_search_stats{0.0, 0, 0};
+        return WV_OK;
+    }
+    DevBuf dq, de, dl, dk, di, dd, dev, dh, dm, dtr, dtl, dse;
+    WV_HIP(c, dq.alloc((size_t)Q * ix->dp * 4));
+    WV_HIP(c, upload_padded(dq.as<float>(), queries, Q, ix->d, ix->dp, c->stream));
+    if (entries) {
+        WV_HIP(c, de.alloc((size_t)Q * L * 4));
+        WV_HIP(c, hipMemcpyAsync(de.p, entries, (size_t)Q * L * 4, hipMemcpyHostToDevice, c->stream));
+    }
+    if (L_per_q) {
+        WV_HIP(c, dl.alloc((size_t)Q * 4));
+        WV_HIP(c, hipMemcpyAsync(dl.p, L_per_q, (size_t)Q * 4, hipMemcpyHostToDevice, c->stream));
+    }
+    if (k_per_q) {
+        WV_HIP(c, dk.alloc((size_t)Q * 4));
+        WV_HIP(c, hipMemcpyAsync(dk.p, k_per_q, (size_t)Q * 4, hipMemcpyHostToDevice, c->stream));
+    }
+    WV_HIP(c, di.alloc((size_t)Q * k_stride * 4));
+    if (out_dist) WV_HIP(c, dd.alloc((size_t)Q * k_stride * 4));
+    if (evals) WV_HIP(c, dev.alloc((size_t)Q * 4));
+    if (hops) WV_HIP(c, dh.alloc((size_t)Q * 4));
+    SearchExtDev x;
+    if (ext && ext->model) {
+        WV_HIP(c, dm.alloc(sizeof(wv_term_model)));
+        WV_HIP(c, hipMemcpyAsync(dm.p, ext->model, sizeof(wv_term_model), hipMemcpyHostToDevice, c->stream));
+        x.model = dm.as<wv_term_model>();
+    }
+    if (ext && ext->trace) {
+        x.cap = ext->trace_cap;
+        WV_HIP(c, dtr.alloc((size_t)Q * x.cap * 8 * 4));
+        WV_HIP(c, dtl.alloc((size_t)Q * 4));
+        x.trace = dtr.as<float>();
+        x.trace_len = dtl.as<uint32_t>();
+    }
+    if (ext && ext->settle_hop) {
+        WV_HIP(c, dse.alloc((size_t)Q * 4));
+        x.settle = dse.as<uint32_t>();
+    }
+    rc = search_core(ix, dq.as<float>(), Q, de.as<uint32_t>(), seed, L, k, dl.as<uint32_t>(),
+                     dk.as<uint32_t>(), Lmax, k_stride, di.as<uint32_t>(), dd.as<float>(),
+                     dev.as<uint32_t>(), dh.as<uint32_t>(), stats, x);
+    if (rc) return rc;
+    WV_HIP(c, hipMemcpy(out_ids, di.p, (size_t)Q * k_stride * 4, hipMemcpyDeviceToHost));
+    if (out_dist) WV_HIP(c, hipMemcpy(out_dist, dd.p, (size_t)Q * k_stride * 4, hipMemcpyDeviceToHost));
+    if (evals) WV_HIP(c, hipMemcpy(evals, dev.p, (size_t)Q * 4, hipMemcpyDeviceToHost));
+    if (hops) WV_HIP(c, hipMemcpy(hops, dh.p, (size_t)Q * 4, hipMemcpyDeviceToHost));
+    if (x.trace) {
+        WV_HIP(c, hipMemcpy(ext->trace, dtr.p, (size_t)Q * x.cap * 8 * 4, hipMemcpyDeviceToHost));
+        WV_HIP(c, hipMemcpy(ext->trace_len, dtl.p, (size_t)Q * 4, hipMemcpyDeviceToHost));
+    }
+    if (x.settle) WV_HIP(c, hipMemcpy(ext->settle_hop, dse.p, (size_t)Q * 4, hipMemcpyDeviceToHost));
+    return WV_OK;
+}
+
+int wv_index_search_ext_dev(wv_index *ix, const float *d_queries, uint32_t Q, const uint32_t *d_entries,
+                            uint64_t seed, uint32_t L, uint32_t k, const uint32_t *d_L_per_q,
+                            const uint32_t *d_k_per_q, uint32_t k_stride, uint32_t *d_out_ids,
+                            float *d_out_dist, uint32_t *d_evals, uint32_t *d_hops,
+                            const wv_search_ext *ext, wv_search_stats *stats) {
+    if (!ix) return WV_EINVAL;
+    wv_ctx *c = ix->ctx;
+    std::lock_guard<std::mutex> g(c->mu);
+    if (!d_queries || !d_out_ids) return fail(c, WV_EINVAL, "queries/out_ids are null");
+    if (ext && ext->trace && (!ext->trace_len || ext->trace_cap == 0))
+        return fail(c, WV_EINVAL, "trace needs trace_len and trace_cap > 0");
+    WV_HIP(c, hipSetDevice(c->device));
+    std::vector<uint32_t> hl, hk;
+    if (d_L_per_q) {
+        hl.resize(Q);
+        WV_HIP(c, hipMemcpy(hl.data(), d_L_per_q, (size_t)Q * 4, hipMemcpyDeviceToHost));
+    }
+    if (d_k_per_q) {
+        hk.resize(Q);
+        WV_HIP(c, hipMemcpy(hk.data(), d_k_per_q, (size_t)Q * 4, hipMemcpyDeviceToHost));
+    }
+    uint32_t Lmax = 0;
+    int rc = check_search_args(c, ix->N, Q, d_entries, L, k, d_L_per_q ? hl.data() : nullptr,
+                               d_k_per_q ? hk.data() : nullptr, k_stride, Lmax);
+    if (rc) return rc;
+    if (d_entries && Q) {
+        std::vector<uint32_t> he((size_t)Q * L);
+        WV_HIP(c, hipMemcpy(he.data(), d_entries, he.size() * 4, hipMemcpyDeviceToHost));
+        if ((rc = check_entries(c, he.data(), he.size(), ix->N))) return rc;
+    }
+    DevBuf dm;
+    SearchExtDev x;
+    if (ext && ext->model) {
+        WV_HIP(c, dm.alloc(sizeof(wv_term_model)));
+        WV_HIP(c, hipMemcpy(dm.p, ext->model, sizeof(wv_term_model), hipMemcpyHostToDevice));
+        x.model = dm.as<wv_term_model>();
+    }
+    if (ext) {
+        x.trace = ext->trace;
+        x.trace_len = ext->trace ? ext->trace_len : nullptr;
+        x.cap = ext->trace ? ext->trace_cap : 0;
+        x.settle = ext->settle_hop;
+    }
+    return search_core(ix, d_queries, Q, d_entries, seed, L, k, d_L_per_q, d_k_per_q, Lmax, k_stride,
+                       d_out_ids, d_out_dist, d_evals, d_hops, stats, x);
+}
+
+int wv_mlp_grad(wv_ctx *c, const wv_term_model *m, const float *X, const float *y, uint32_t n,
+                float pos_weight, float *grad, double *loss) {
+    if (!c) return WV_EINVAL;
+    std::lock_guard<std::mutex> g(c->mu);
+    if (!m || !X || !y || !grad || n == 0) return fail(c, WV_EINVAL, "null argument or no samples");
+    WV_HIP(c, hipSetDevice(c->device));
+    const uint32_t nb = (n + 63) / 64;
+    DevBuf dp, dx, dy, dpart, dg, dl;
+    WV_HIP(c, dp.alloc(sizeof(wv_term_model)));
+    WV_HIP(c, dx.alloc((size_t)n * 8 * 4));
+    WV_HIP(c, dy.alloc((size_t)n * 4));
+    WV_HIP(c, dpart.alloc((size_t)nb * wv::MLP_PARTIAL_STRIDE * 4));
+    WV_HIP(c, dg.alloc(wv::MLP_PARAMS * 4));
+    WV_HIP(c, dl.alloc(16));
+    WV_HIP(c, hipMemcpyAsync(dp.p, m, sizeof(wv_term_model), hipMemcpyHostToDevice, c->stream));
+    WV_HIP(c, hipMemcpyAsync(dx.p, X, (size_t)n * 8 * 4, hipMemcpyHostToDevice, c->stream));
+    WV_HIP(c, hipMemcpyAsync(dy.p, y, (size_t)n * 4, hipMemcpyHostToDevice, c->stream));
+    const float *params = dp.as<float>();
+    WV_HIP(c, wv::mlp_grad_launch(params, model_field(dp.p, offsetof(wv_term_model, mean)),
+                                  model_field(dp.p, offsetof(wv_term_model, inv_std)), dx.as<float>(),
+                                  dy.as<float>(), nullptr, n,
+                                  pos_weight, dpart.as<float>(), c->stream));
+    WV_HIP(c, wv::mlp_update_launch(dpart.as<float>(), nb, 1.0f / n, nullptr, nullptr, nullptr, dg.as<float>(),
+                                    dl.as<float>(), 0, 0.f, 0.f, 0.f, 0.f, 1.f, 1.f, c->stream));
+    WV_HIP(c, hipStreamSynchronize(c->stream));
+    WV_HIP(c, hipMemcpy(grad, dg.p, wv::MLP_PARAMS * 4, hipMemcpyDeviceToHost));
+    if (loss) {
+        float l = 0.f;
+        WV_HIP(c, hipMemcpy(&l, dl.p, 4, hipMemcpyDeviceToHost));
+        *loss = l;
+    }
+    return WV_OK;
+}
+
+int wv_mlp_train(wv_ctx *c, const float *X, const float *y, uint32_t n, const wv_train_params *tp,
+                 wv_term_model *model, wv_train_stats *stats) {
+    if (!c) return WV_EINVAL;
+    std::lock_guard<std::mutex> g(c->mu);
+    if (!X || !y || !tp || !model || n == 0) return fail(c, WV_EINVAL, "null argument or no samples");
+    if (tp->batch == 0 || tp->batch % 64) return fail(c, WV_EINVAL, "batch must be a positive multiple of 64");
+    WV_HIP(c, hipSetDevice(c->device));
+    // standardisation from the data unless the model carries one
+    bool have_std = false;
+    for (int i = 0; i < 8; ++i) have_std |= model->inv_std[i] != 0.f;
+    if (!have_std) {
+        for (int i = 0; i < 8; ++i) {
+            double s = 0, s2 = 0;
+            for (uint32_t r = 0; r < n; ++r) {
+                const double v = X[(size_t)r * 8 + i];
+                s += v;
+                s2 += v * v;
+            }
+            const double mu = s / n, var = std::max(s2 / n - mu * mu, 0.0);
+            model->mean[i] = (float)mu;
+            model->inv_std[i] = var > 1e-20 ? (float)(1.0 / std::sqrt(var)) : 1.0f;
+        }
+    }
+    const uint32_t B = tp->batch;
+    const uint32_t steps_per_epoch = (n + B - 1) / B;
+    const uint32_t nbmax = B / 64;
+    DevBuf dp, dx, dy, didx, dpart, dmo, dvo, dloss;
+    WV_HIP(c, dp.alloc(sizeof(wv_term_model)));
+    WV_HIP(c, dx.alloc((size_t)n * 8 * 4));
+    WV_HIP(c, dy.alloc((size_t)n * 4));
+    WV_HIP(c, didx.alloc((size_t)n * 4));
+    WV_HIP(c, dpart.alloc((size_t)nbmax * wv::MLP_PARTIAL_STRIDE * 4));
+    WV_HIP(c, dmo.alloc(wv::MLP_PARAMS * 4));
+    WV_HIP(c, dvo.alloc(wv::MLP_PARAMS * 4));
+    const size_t total_steps = (size_t)tp->epochs * steps_per_epoch;
+    WV_HIP(c, dloss.alloc(std::max<size_t>(total_steps, 1) * 4));
+    WV_HIP(c, hipMemcpyAsync(dp.p, model, sizeof(wv_term_model), hipMemcpyHostToDevice, c->stream));
+    WV_HIP(c, hipMemcpyAsync(dx.p, X, (size_t)n * 8 * 4, hipMemcpyHostToDevice, c->stream));
+    WV_HIP(c, hipMemcpyAsync(dy.p, y, (size_t)n * 4, hipMemcpyHostToDevice, c->stream));
+    WV_HIP(c, hipMemsetAsync(dmo.p, 0, wv::MLP_PARAMS * 4, c->stream));
+    WV_HIP(c, hipMemsetAsync(dvo.p, 0, wv::MLP_PARAMS * 4, c->stream));
+    float *params = dp.as<float>();
+    const float *dmean = model_field(dp.p, offsetof(wv_term_model, mean));
+    const float *dinv = model_field(dp.p, offsetof(wv_term_model, inv_std));
+    std::vector<uint32_t> perm(n);
+    std::iota(perm.begin(), perm.end(), 0u);
+    std::mt19937_64 rng(tp->seed);
+    WV_HIP(c, hipEventRecord(c->ev0, c->stream));
+    size_t step = 0;
+    std::vector<uint32_t> batch_n(total_steps);
+    for (uint32_t e = 0; e < tp->epochs; ++e) {
+        std::shuffle(perm.begin(), perm.end(), rng);
+        WV_HIP(c, hipStreamSynchronize(c->stream));  // perm buffer reuse
+        WV_HIP(c, hipMemcpyAsync(didx.p, perm.data(), (size_t)n * 4, hipMemcpyHostToDevice, c->stream));
+        for (uint32_t b0 = 0; b0 < n; b0 += B, ++step) {
+            const uint32_t nbt = std::min(B, n - b0);
+            batch_n[step] = nbt;
+            const float bc1 = 1.f - std::pow(tp->beta1, (float)(step + 1));
+            const float bc2 = 1.f - std::pow(tp->beta2, (float)(step + 1));
+            WV_HIP(c, wv::mlp_grad_launch(params, dmean, dinv, dx.as<float>(), dy.as<float>(),
+                                          didx.as<uint32_t>() + b0, nbt, tp->pos_weight, dpart.as<float>(),
+                                          c->stream));
+            WV_HIP(c, wv::mlp_update_launch(dpart.as<float>(), (nbt + 63) / 64, 1.0f / nbt, params, dmo.as<float>(),
+                                            dvo.as<float>(), nullptr, dloss.as<float>() + step, 1, tp->lr,
+                                            tp->beta1, tp->beta2, tp->eps, bc1, bc2, c->stream));
+        }
+    }
+    WV_HIP(c, hipEventRecord(c->ev1, c->stream));
+    WV_HIP(c, hipStreamSynchronize(c->stream));
+    std::vector<float> losses(total_steps);
+    if (total_steps) WV_HIP(c, hipMemcpy(losses.data(), dloss.p, total_steps * 4, hipMemcpyDeviceToHost));
+    WV_HIP(c, hipMemcpy(model, dp.p, wv::MLP_PARAMS * 4, hipMemcpyDeviceToHost));  // w1, b1, w2, b2
+    if (stats) {
+        float ms = 0.f;
+        WV_HIP(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+        stats->kernel_ms = ms;
+        stats->steps = (uint32_t)total_steps;
+        auto epoch_loss = [&](uint32_t e) {
+            double s = 0;
+            for (uint32_t t = 0; t < steps_per_epoch; ++t) {
+                const size_t i = (size_t)e * steps_per_epoch + t;
+                s += (double)losses[i] * batch_n[i];
+            }
+            return s / n;
+        };
+        stats->first_loss = tp->epochs ? epoch_loss(0) : 0.0;
+        stats->last_loss = tp->epochs ? epoch_loss(tp->epochs - 1) : 0.0;
+    }
+    return WV_OK;
+}
+
+int wv_mlp_predict(wv_ctx *c, const wv_term_model *m, const float *X, uint32_t n, float *logits) {
+    if (!c) return WV_EINVAL;
+    std::lock_guard<std::mutex> g(c->mu);
+    if (!m || !X || !logits) return fail(c, WV_EINVAL, "null argument");
+    if (n == 0) return WV_OK;
+    WV_HIP(c, hipSetDevice(c->device));
+    DevBuf dm, dx, dz;
+    WV_HIP(c, dm.alloc(sizeof(wv_term_model)));
+    WV_HIP(c, dx.alloc((size_t)n * 8 * 4));
+    WV_HIP(c, dz.alloc((size_t)n * 4));
+    WV_HIP(c, hipMemcpyAsync(dm.p, m, sizeof(wv_term_model), hipMemcpyHostToDevice, c->stream));
+    WV_HIP(c, hipMemcpyAsync(dx.p, X, (size_t)n * 8 * 4, hipMemcpyHostToDevice, c->stream));
+    WV_HIP(c, wv::mlp_predict_launch(dm.as<wv_term_model>(), dx.as<float>(), n, dz.as<float>(), c->stream));
+    WV_HIP(c, hipStreamSynchronize(c->stream));
+    WV_HIP(c, hipMemcpy(logits, dz.p, (size_t)n * 4, hipMemcpyDeviceToHost));
+    return WV_OK;
 }
 
 int wv_index_build_nndescent(wv_index *ix, const wv_nnd_params *p, const uint32_t *init_ids,

@@ -6,6 +6,8 @@
 #include <stddef.h>
 #include <stdint.h>
 
+#include "../../include/weavess_hip.h"
+
 namespace wv {
 
 // ------------------------------------------------------------ greedy search
@@ -27,12 +29,27 @@ struct SearchArgs {
     uint32_t P;               // power of two >= every L
     uint32_t H;               // visited-table slots (power of two)
     uint32_t wave_lds;        // LDS bytes per wavefront
+    const wv_term_model *model;  // learned termination (device copy) or null
+    float *trace;             // Q x trace_cap x 8, or null
+    uint32_t *trace_len;      // Q
+    uint32_t *settle;         // Q
+    uint32_t trace_cap;
 };
 
 size_t search_wave_lds(uint32_t P, uint32_t H);
 // One wavefront per query (grid of ceil(Q / waves_per_block) workgroups).
 hipError_t launch_greedy_search(const SearchArgs &a, int chunks_per_lane, int row_groups,
                                 int waves_per_block, hipStream_t s);
+
+// ------------------------------------------------------------ MLP (wv_mlp.hip)
+hipError_t mlp_grad_launch(const float *params, const float *mean, const float *inv_std, const float *X,
+                           const float *y, const uint32_t *idx, uint32_t n_batch, float pos_weight,
+                           float *partial, hipStream_t s);
+hipError_t mlp_update_launch(const float *partial, uint32_t nblocks, float inv_n, float *params, float *m,
+                             float *v, float *grad_out, float *loss_out, int adam, float lr, float beta1,
+                             float beta2, float eps, float bc1, float bc2, hipStream_t s);
+hipError_t mlp_predict_launch(const wv_term_model *m, const float *X, uint32_t n, float *out, hipStream_t s);
+constexpr int MLP_PARTIAL_STRIDE = 324;
 
 // ------------------------------------------------------------ exact kNN
 struct ExactArgs {

@@ -1,0 +1,65 @@
+// wv_mlp.hpp -- the learned-termination network on the device: search-state
+// features and the BF16 inference step fused into the greedy search.
+#pragma once
+
+#include "wv_device.hpp"
+#include "../../include/weavess_hip.h"
+
+namespace wv {
+
+constexpr int MLP_IN = 8, MLP_HID = 32, MLP_PARAMS = MLP_HID * MLP_IN + MLP_HID + MLP_HID + 1;  // 321
+
+__device__ inline float bf16_round(float x) { return (float)(__bf16)x; }
+
+// Feature vector g of a search state (see wv_term_model in weavess_hip.h).
+// Every lane computes the same values.
+__device__ inline void term_features(float hops, float kth, float best, float entry, float kth4,
+                                     float L, float k, float qnorm, float f[MLP_IN]) {
+    const float e = entry > 1e-30f ? entry : 1e-30f;
+    f[0] = log2f(1.0f + hops);
+    f[1] = kth / e;
+    f[2] = best / e;
+    f[3] = log2f(e);
+    f[4] = (kth4 - kth) / e;
+    f[5] = log2f(L);
+    f[6] = log2f(k);
+    f[7] = log2f(qnorm > 1e-30f ? qnorm : 1e-30f);
+}
+
+// Logit of the network for features f, in BF16 products with FP32
+// accumulation: lane j < 32 forms hidden unit j as
+//   b1[j] + sum_i bf16(w1[j][i]) * bf16(x_i)   (i ascending),
+// ReLU, times bf16(w2[j]) after rounding the activation to BF16; the 32
+// products are summed by an xor butterfly (16, 8, 4, 2, 1) and b2 is added
+// last.  x_i = (f_i - mean_i) * inv_std_i.  Returns the same value on every
+// lane of the wavefront.
+__device__ inline float term_logit(const wv_term_model *m, const float f[MLP_IN]) {
+    const int lane = lane_id();
+    float part = 0.0f;
+    if (lane < MLP_HID) {
+        float acc = m->b1[lane];
+#pragma unroll
+        for (int i = 0; i < MLP_IN; ++i) {
+            const float x = bf16_round((f[i] - m->mean[i]) * m->inv_std[i]);
+            acc = acc + bf16_round(m->w1[lane][i]) * x;
+        }
+        const float h = acc > 0.0f ? acc : 0.0f;
+        part = bf16_round(m->w2[lane]) * bf16_round(h);
+    }
+    part = part + __shfl_xor(part, 16);
+    part = part + __shfl_xor(part, 8);
+    part = part + __shfl_xor(part, 4);
+    part = part + __shfl_xor(part, 2);
+    part = part + __shfl_xor(part, 1);
+    return __shfl(part, 0) + m->b2;
+}
+
+// Value of feature `i` from the register array without dynamic indexing.
+__device__ inline float pick8(const float f[MLP_IN], int i) {
+    float v = f[0];
+#pragma unroll
+    for (int j = 1; j < MLP_IN; ++j) v = (i == j) ? f[j] : v;
+    return v;
+}
+
+}  // namespace wv

@@ -12,7 +12,8 @@ large sets are generated in independent chunks so memory stays bounded).
       centre_k = |N(0,1)| * s with per-dimension s ~ U[0.1,1],
       x = |centre_k + 0.3 * N(0,1) * s|.
   c4  2,000,000 base + 200,000 queries, dim 96, unit norm: 2,000 clusters,
-      centres N(0,I), x = centre + 0.3 * N(0,I), then x / |x|; per-query
+      centres uniform on the unit sphere, x = centre + 0.3 * N(0,I) (sigma
+      0.3 per dimension before normalisation), then x / |x|; per-query
       L ~ U{16..400}, k ~ U{1,10,100}.
 All query sets come from the same mixture as their base set.
 """
@@ -79,6 +80,7 @@ def gmm_c3(n_base=1_000_000, n_query=1_000, d=960, clusters=500, noise=0.3, seed
 def gmm_c4(n_base=2_000_000, n_query=200_000, d=96, clusters=2_000, sigma=0.3, seed=4):
     g = _rng(seed, 0)
     centres = g.normal(0.0, 1.0, size=(clusters, d))
+    centres /= np.linalg.norm(centres, axis=1, keepdims=True)
     probs = np.full(clusters, 1.0 / clusters)
 
     def sample(gg, c):
