@@ -561,19 +561,21 @@ float logit_bf16(const wvo_term_model *m, const float *f) {
 }
 }  // namespace
 
-void wvo_term_features(float hops, float kth, float best, float entry, float kth4, float L, float k,
-                       float qnorm, float *f) {
-    /* weavess_hip.h: g = [log2(1+hops), kth/entry, best/entry, log2(entry),
-     * (kth4-kth)/entry, log2(L), log2(k), log2(|q|)] */
+void wvo_term_features(float hops, float kth, float best, float entry, float next, float stale, float L,
+                       float k, float *f) {
+    /* weavess_hip.h: g = [log2(1+hops), kth/entry, best/entry, log2(next/kth),
+     * log2(1+stale), log2(L), log2(k), log2(entry)] */
     const float e = entry > 1e-30f ? entry : 1e-30f;
+    const float kt = kth > 1e-30f ? kth : 1e-30f;
+    const float nx = next > 1e-30f ? next : 1e-30f;
     f[0] = std::log2(1.0f + hops);
     f[1] = kth / e;
     f[2] = best / e;
-    f[3] = std::log2(e);
-    f[4] = (kth4 - kth) / e;
+    f[3] = std::log2(nx / kt);
+    f[4] = std::log2(1.0f + stale);
     f[5] = std::log2(L);
     f[6] = std::log2(k);
-    f[7] = std::log2(qnorm > 1e-30f ? qnorm : 1e-30f);
+    f[7] = std::log2(e);
 }
 
 int wvo_mlp_logit(const wvo_term_model *m, const float *X, uint32_t n, int bf16, double *out) {
@@ -687,10 +689,6 @@ int wvo_search_greedy_ext(const uint32_t *graph, const uint32_t *deg, uint32_t d
             const bool watch = model || trace;
             const uint32_t kidx = (kq < Lq ? kq : Lq) - 1;
             float entry_d = key_dist(key[0]);
-            std::vector<float> zero(d, 0.0f);
-            const float qnorm = watch ? std::sqrt(dist_canon(qv, zero.data(), d)) : 0.0f;
-            float hist[4];  /* k-th distance 1..4 expansions ago */
-            for (float &x : hist) x = key_dist(key[kidx]);
             uint32_t st = 0;
             /* RouteGreedy, component_route.cpp:16-47 */
             uint32_t kk = 0;
@@ -730,19 +728,21 @@ int wvo_search_greedy_ext(const uint32_t *graph, const uint32_t *deg, uint32_t d
                         if (r <= kidx) st = hp;
                     }
                 }
+                kk = (nk <= kk) ? nk : kk + 1;
                 if (expanded && watch) {
+                    /* the next expansion is the first unexpanded entry (all
+                     * entries before kk are expanded) */
+                    uint32_t nx = kk;
+                    while (nx < Lq && !flag[nx]) ++nx;
                     float f[8];
-                    const float kth = key_dist(key[kidx]);
-                    wvo_term_features((float)hp, kth, key_dist(key[0]), entry_d, hist[3], (float)Lq, (float)kq, qnorm, f);
-                    hist[3] = hist[2];
-                    hist[2] = hist[1];
-                    hist[1] = hist[0];
-                    hist[0] = kth;
+                    wvo_term_features((float)hp, key_dist(key[kidx]), key_dist(key[0]), entry_d,
+                                      key_dist(key[nx < Lq ? nx : Lq - 1]), (float)(hp - st), (float)Lq,
+                                      (float)kq, f);
                     if (trace && hp <= cap)
                         for (int i = 0; i < 8; ++i) trace[((size_t)q * cap + (hp - 1)) * 8 + i] = f[i];
-                    if (model && hp >= model->min_hops && logit_bf16(model, f) > model->logit_threshold) break;
+                    if (model && nx < Lq && hp >= model->min_hops && logit_bf16(model, f) > model->logit_threshold)
+                        break;
                 }
-                kk = (nk <= kk) ? nk : kk + 1;
             }
             if (trace_len) trace_len[q] = hp < cap ? hp : cap;
             if (settle) settle[q] = st;

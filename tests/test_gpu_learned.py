@@ -6,7 +6,7 @@ import numpy as np
 import pytest
 
 import oracle as O
-from weavess_amd import Index, learned
+from weavess_amd import Index, datagen, learned
 from weavess_amd._capi import TermModel
 
 pytestmark = pytest.mark.gpu
@@ -118,24 +118,37 @@ def test_learned_search_matches_oracle_stop_rule(ctx, c4_small):
     assert np.array_equal(ids[same], rids[same])
 
 
-def test_learned_termination_gains_at_equal_recall(ctx, c4_small):
-    """Plain fixed-L search needs L_plain for the target recall; the learned
-    stop rule with a larger pool reaches the same recall with fewer distance
-    evaluations (the work the network saves)."""
-    base, q, gt, ix = c4_small
-    train_q, cal_q, test_q = q[:200], q[200:400], q[400:]
-    target = 0.95
+@pytest.fixture(scope="module")
+def c5_small(ctx):
+    base, q = datagen.gmm_c5(n_base=20000, n_query=1000, clusters=200)
+    ix = Index(ctx, base)
+    ix.build_nndescent(32, 48, 16, 48, 10, seed=1)
+    gt, _ = O.exact_knn(base, q, 10)
+    yield base, q, gt, ix
+    ix.close()
+
+
+def test_learned_termination_gains_at_equal_recall(ctx, c5_small):
+    """Clusters of different spreads make query difficulty vary: plain
+    fixed-L search sizes L for the hard queries and spends it on every query;
+    the learned stop rule over the same-size pool stops the easy ones early.
+    At equal recall on held-out queries it makes fewer distance evaluations."""
+    base, q, gt, ix = c5_small
+    train_q, cal_q, test_q, test_gt = q[:300], q[300:600], q[600:], gt[600:]
+    target, cap = 0.95, 400
     plain = {}
-    for L in (40, 60, 80, 100, 130, 160, 200, 260):
+    for L in (200, 240, 280, 320, 360, 400):
         ids, _, ev, *_ = ix.search(test_q, L, 10, seed=9)
-        plain[L] = (learned.recall_at_k(ids, gt[400:], 10), float(ev.mean()))
-    L_plain = min(L for L, (r, _) in plain.items() if r >= target)
-    X, y, _ = learned.collect_traces(ix, train_q, 260, 10, seed=3)
+        plain[L] = (learned.recall_at_k(ids, test_gt, 10), float(ev.mean()))
+    reached = [L for L, (r, _) in plain.items() if r >= target]
+    assert reached, plain
+    L_plain = min(reached)
+    X, y, _ = learned.collect_traces(ix, train_q, cap, 10, seed=3)
     m, st = learned.train(ctx, X, y, epochs=20, batch=1024, seed=1)
-    cal = learned.calibrate(ix, cal_q, gt[200:400], m, 260, 10, target + 0.01, seed=5)
+    cal = learned.calibrate(ix, cal_q, gt[300:600], m, cap, 10, target + 0.005, seed=5)
     assert cal is not None
-    ids, _, ev, *_ = ix.search_ext(test_q, 260, 10, seed=9, model=m)
-    r_learned = learned.recall_at_k(ids, gt[400:], 10)
-    print("plain", L_plain, plain[L_plain], "learned", r_learned, ev.mean(), "threshold", cal)
+    ids, _, ev, *_ = ix.search_ext(test_q, cap, 10, seed=9, model=m)
+    r_learned = learned.recall_at_k(ids, test_gt, 10)
+    print("plain", L_plain, plain[L_plain], "learned", r_learned, ev.mean(), "threshold", cal, st)
     assert r_learned >= target - 0.01
-    assert ev.mean() < 0.9 * plain[L_plain][1]
+    assert ev.mean() < 0.85 * plain[L_plain][1]

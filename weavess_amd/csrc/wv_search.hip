@@ -140,16 +140,8 @@ __global__ void __launch_bounds__(256) greedy_search_kernel(SearchArgs a) {
     // learned termination / traces: state features (identical on every lane)
     const bool watch = a.model != nullptr || a.trace != nullptr;
     const int kidx = (int)(kq < L ? kq : L) - 1;
-    float entry_d = 0.f, qnorm = 0.f, r0 = 0.f, r1 = 0.f, r2 = 0.f, r3 = 0.f;
-    uint32_t settle = 0;
-    if (watch) {
-        float sq = 0.f;
-#pragma unroll
-        for (int c = 0; c < CPL; ++c) sq = sq + (qv[c].x * qv[c].x + qv[c].y * qv[c].y + qv[c].z * qv[c].z + qv[c].w * qv[c].w);
-        qnorm = sqrtf(fold16(sq));
-        entry_d = key_dist(pool[0]);
-        r0 = r1 = r2 = r3 = key_dist(pool[kidx]);
-    }
+    float entry_d = watch ? key_dist(pool[0]) : 0.f;
+    uint32_t settle = 0;  // last expansion that changed the top-k
     for (;;) {
         // first unexpanded slot >= k
         int kk = Li;
@@ -161,6 +153,16 @@ __global__ void __launch_bounds__(256) greedy_search_kernel(SearchArgs a) {
                 kk = t0 + __ffsll((unsigned long long)b) - 1;
                 break;
             }
+        }
+        if (watch && hops > 0) {
+            // state after `hops` expansions; pool[kk] is the next candidate
+            float f[MLP_IN];
+            term_features((float)hops, key_dist(pool[kidx]), key_dist(pool[0]), entry_d,
+                          key_dist(pool[kk < Li ? kk : Li - 1]), (float)(hops - settle), (float)L, (float)kq, f);
+            if (a.trace && hops <= a.trace_cap && lane < MLP_IN)
+                a.trace[((size_t)q * a.trace_cap + (hops - 1)) * MLP_IN + lane] = pick8(f, lane);
+            if (kk < Li && a.model && hops >= a.model->min_hops && term_logit(a.model, f) > a.model->logit_threshold)
+                break;
         }
         if (kk >= Li) break;
         const uint32_t u = key_id(pool[kk]);
@@ -234,18 +236,6 @@ __global__ void __launch_bounds__(256) greedy_search_kernel(SearchArgs a) {
             if (pos0 < nk) nk = pos0;
             if (pos0 <= kidx) settle = hops;  // a candidate entered the top-k
             wave_sync();
-        }
-        if (watch) {
-            float f[MLP_IN];
-            const float kth = key_dist(pool[kidx]);
-            term_features((float)hops, kth, key_dist(pool[0]), entry_d, r3, (float)L, (float)kq, qnorm, f);
-            r3 = r2;
-            r2 = r1;
-            r1 = r0;
-            r0 = kth;
-            if (a.trace && hops <= a.trace_cap && lane < MLP_IN)
-                a.trace[((size_t)q * a.trace_cap + (hops - 1)) * MLP_IN + lane] = pick8(f, lane);
-            if (a.model && hops >= a.model->min_hops && term_logit(a.model, f) > a.model->logit_threshold) break;
         }
         k = (nk <= kk) ? nk : kk + 1;
     }

@@ -15,6 +15,9 @@ large sets are generated in independent chunks so memory stays bounded).
       centres uniform on the unit sphere, x = centre + 0.3 * N(0,I) (sigma
       0.3 per dimension before normalisation), then x / |x|; per-query
       L ~ U{16..400}, k ~ U{1,10,100}.
+  c5  as c4 but each cluster has its own spread, sigma_c log-uniform in
+      [0.05, 0.6]: query difficulty varies across clusters, the setting an
+      adaptive (learned) stop rule is for.
 All query sets come from the same mixture as their base set.
 """
 from __future__ import annotations
@@ -28,8 +31,8 @@ def _rng(seed: int, *key: int) -> np.random.Generator:
     return np.random.Generator(np.random.PCG64(np.random.SeedSequence([seed, *key])))
 
 
-def _mixture(n: int, centres: np.ndarray, probs: np.ndarray, sample, seed: int, stream: int):
-    d = centres.shape[1]
+def _mixture(n: int, centres: np.ndarray, probs: np.ndarray, sample, seed: int, stream: int, d=None):
+    d = centres.shape[1] if d is None else d
     out = np.empty((n, d), np.float32)
     for ci, s0 in enumerate(range(0, n, CHUNK)):
         m = min(CHUNK, n - s0)
@@ -90,6 +93,23 @@ def gmm_c4(n_base=2_000_000, n_query=200_000, d=96, clusters=2_000, sigma=0.3, s
 
     return (_mixture(n_base, centres, probs, sample, seed, 1),
             _mixture(n_query, centres, probs, sample, seed, 2))
+
+
+def gmm_c5(n_base=2_000_000, n_query=200_000, d=96, clusters=2_000, sigma_lo=0.05, sigma_hi=0.6, seed=5):
+    g = _rng(seed, 0)
+    centres = g.normal(0.0, 1.0, size=(clusters, d))
+    centres /= np.linalg.norm(centres, axis=1, keepdims=True)
+    sig = np.exp(g.uniform(np.log(sigma_lo), np.log(sigma_hi), size=clusters))
+    table = np.concatenate([centres, sig[:, None]], axis=1)  # last column: the cluster's sigma
+    probs = np.full(clusters, 1.0 / clusters)
+
+    def sample(gg, c):
+        x = c[:, :d] + gg.normal(0.0, 1.0, size=(c.shape[0], d)) * c[:, d:]
+        x /= np.linalg.norm(x, axis=1, keepdims=True)
+        return x.astype(np.float32)
+
+    return (_mixture(n_base, table, probs, sample, seed, 1, d),
+            _mixture(n_query, table, probs, sample, seed, 2, d))
 
 
 def c4_query_budgets(n_query: int, seed=4, L_lo=16, L_hi=400, ks=(1, 10, 100)):
