@@ -108,13 +108,13 @@ typedef struct {
  * logit_threshold and at least min_hops expansions were made.  Features
  * g(state), standardised as (g - mean) * inv_std:
  *   0 log2(1 + hops)          1 kth / entry     2 best / entry
- *   3 log2(next / kth)        4 log2(1 + stale)
- *   5 log2(L)                 6 log2(k)         7 log2(entry)
+ *   3 log2(entry)             4 (kth 4 hops ago - kth) / entry
+ *   5 log2(L)                 6 log2(k)         7 log2(1 + stale)
  * where entry = best entry distance, kth/best = current k-th/best pool
- * distance, next = distance of the candidate the next expansion takes (the
- * pool's last if none is left), stale = expansions since a candidate last
- * entered the top-k (distances are squared L2; each is clamped to >= 1e-30
- * inside a ratio or log). */
+ * distance (squared L2, the library's distances; entry is clamped to
+ * >= 1e-30), "kth 4 hops ago" is the entries' k-th distance for the first
+ * 4 expansions, and stale = expansions since a candidate last entered the
+ * top-k. */
 typedef struct {
     float w1[32][8];
     float b1[32];

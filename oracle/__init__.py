@@ -297,9 +297,9 @@ def search_greedy_ext(graph, deg, base, queries, L, k, seed=0, entries=None, mod
     return out_ids, out_dist, evals, hops, trace, tlen, settle
 
 
-def term_features(hops, kth, best, entry, next, stale, L, k):
+def term_features(hops, kth, best, entry, kth4, stale, L, k):
     f = np.zeros(8, np.float32)
-    lib().wvo_term_features(hops, kth, best, entry, next, stale, L, k, f)
+    lib().wvo_term_features(hops, kth, best, entry, kth4, stale, L, k, f)
     return f
 
 

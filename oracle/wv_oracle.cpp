@@ -561,21 +561,19 @@ float logit_bf16(const wvo_term_model *m, const float *f) {
 }
 }  // namespace
 
-void wvo_term_features(float hops, float kth, float best, float entry, float next, float stale, float L,
+void wvo_term_features(float hops, float kth, float best, float entry, float kth4, float stale, float L,
                        float k, float *f) {
-    /* weavess_hip.h: g = [log2(1+hops), kth/entry, best/entry, log2(next/kth),
-     * log2(1+stale), log2(L), log2(k), log2(entry)] */
+    /* weavess_hip.h: g = [log2(1+hops), kth/entry, best/entry, log2(entry),
+     * (kth4-kth)/entry, log2(L), log2(k), log2(1+stale)] */
     const float e = entry > 1e-30f ? entry : 1e-30f;
-    const float kt = kth > 1e-30f ? kth : 1e-30f;
-    const float nx = next > 1e-30f ? next : 1e-30f;
     f[0] = std::log2(1.0f + hops);
     f[1] = kth / e;
     f[2] = best / e;
-    f[3] = std::log2(nx / kt);
-    f[4] = std::log2(1.0f + stale);
+    f[3] = std::log2(e);
+    f[4] = (kth4 - kth) / e;
     f[5] = std::log2(L);
     f[6] = std::log2(k);
-    f[7] = std::log2(e);
+    f[7] = std::log2(1.0f + stale);
 }
 
 int wvo_mlp_logit(const wvo_term_model *m, const float *X, uint32_t n, int bf16, double *out) {
@@ -689,6 +687,8 @@ int wvo_search_greedy_ext(const uint32_t *graph, const uint32_t *deg, uint32_t d
             const bool watch = model || trace;
             const uint32_t kidx = (kq < Lq ? kq : Lq) - 1;
             float entry_d = key_dist(key[0]);
+            float hist[4];  /* k-th distance 1..4 expansions ago */
+            for (float &x : hist) x = key_dist(key[kidx]);
             uint32_t st = 0;
             /* RouteGreedy, component_route.cpp:16-47 */
             uint32_t kk = 0;
@@ -730,17 +730,17 @@ int wvo_search_greedy_ext(const uint32_t *graph, const uint32_t *deg, uint32_t d
                 }
                 kk = (nk <= kk) ? nk : kk + 1;
                 if (expanded && watch) {
-                    /* the next expansion is the first unexpanded entry (all
-                     * entries before kk are expanded) */
-                    uint32_t nx = kk;
-                    while (nx < Lq && !flag[nx]) ++nx;
                     float f[8];
-                    wvo_term_features((float)hp, key_dist(key[kidx]), key_dist(key[0]), entry_d,
-                                      key_dist(key[nx < Lq ? nx : Lq - 1]), (float)(hp - st), (float)Lq,
-                                      (float)kq, f);
+                    const float kth = key_dist(key[kidx]);
+                    wvo_term_features((float)hp, kth, key_dist(key[0]), entry_d, hist[3], (float)(hp - st),
+                                      (float)Lq, (float)kq, f);
+                    hist[3] = hist[2];
+                    hist[2] = hist[1];
+                    hist[1] = hist[0];
+                    hist[0] = kth;
                     if (trace && hp <= cap)
                         for (int i = 0; i < 8; ++i) trace[((size_t)q * cap + (hp - 1)) * 8 + i] = f[i];
-                    if (model && nx < Lq && hp >= model->min_hops && logit_bf16(model, f) > model->logit_threshold)
+                    if (model && kk < Lq && hp >= model->min_hops && logit_bf16(model, f) > model->logit_threshold)
                         break;
                 }
             }

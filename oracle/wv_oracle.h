@@ -107,7 +107,7 @@ typedef struct {
 } wvo_term_model;
 
 /* Search-state features g, FP32 (restates the device formula). */
-void wvo_term_features(float hops, float kth, float best, float entry, float next, float stale,
+void wvo_term_features(float hops, float kth, float best, float entry, float kth4, float stale,
                        float L, float k, float *f8);
 /* Gradient (321: w1, b1, w2, b2) and mean weighted BCE in FP64. */
 int wvo_mlp_grad(const wvo_term_model *m, const float *X, const float *y, uint32_t n,

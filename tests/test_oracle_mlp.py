@@ -58,12 +58,12 @@ def test_mlp_logit_bf16_close_to_fp64():
 
 
 def test_term_features_hand_worked():
-    f = O.term_features(hops=3, kth=8.0, best=2.0, entry=4.0, next=2.0, stale=7, L=64, k=10)
-    assert f[0] == 2.0 and f[1] == 2.0 and f[2] == 0.5 and f[3] == -2.0
-    assert f[4] == 3.0 and f[5] == 6.0 and f[6] == pytest.approx(math.log2(10)) and f[7] == 2.0
-    # zero distances are clamped, not infinite
-    z = O.term_features(hops=0, kth=0.0, best=0.0, entry=0.0, next=0.0, stale=0, L=1, k=1)
-    assert np.all(np.isfinite(z)) and z[3] == 0.0
+    f = O.term_features(hops=3, kth=8.0, best=2.0, entry=4.0, kth4=12.0, stale=7, L=64, k=10)
+    assert f[0] == 2.0 and f[1] == 2.0 and f[2] == 0.5 and f[3] == 2.0
+    assert f[4] == 1.0 and f[5] == 6.0 and f[6] == pytest.approx(math.log2(10)) and f[7] == 3.0
+    # a zero entry distance is clamped, not infinite
+    z = O.term_features(hops=0, kth=0.0, best=0.0, entry=0.0, kth4=0.0, stale=0, L=1, k=1)
+    assert np.all(np.isfinite(z))
 
 
 def test_oracle_traces_and_settle(small_float, small_float_graph):

@@ -13,22 +13,20 @@ __device__ inline float bf16_round(float x) { return (float)(__bf16)x; }
 
 // Feature vector g of a search state (see wv_term_model in weavess_hip.h):
 // hops = expansions so far, kth/best = k-th/best pool distance, entry = best
-// entry distance, next = distance of the next expansion's candidate, stale =
-// expansions since a candidate last entered the top-k.  Every lane computes
-// the same values.
-__device__ inline void term_features(float hops, float kth, float best, float entry, float next,
+// entry distance, kth4 = k-th distance 4 expansions earlier (the entries'
+// k-th before the 4th), stale = expansions since a candidate last entered
+// the top-k.  Every lane computes the same values.
+__device__ inline void term_features(float hops, float kth, float best, float entry, float kth4,
                                      float stale, float L, float k, float f[MLP_IN]) {
     const float e = entry > 1e-30f ? entry : 1e-30f;
-    const float kt = kth > 1e-30f ? kth : 1e-30f;
-    const float nx = next > 1e-30f ? next : 1e-30f;
     f[0] = log2f(1.0f + hops);
     f[1] = kth / e;
     f[2] = best / e;
-    f[3] = log2f(nx / kt);
-    f[4] = log2f(1.0f + stale);
+    f[3] = log2f(e);
+    f[4] = (kth4 - kth) / e;
     f[5] = log2f(L);
     f[6] = log2f(k);
-    f[7] = log2f(e);
+    f[7] = log2f(1.0f + stale);
 }
 
 // Logit of the network for features f, in BF16 products with FP32

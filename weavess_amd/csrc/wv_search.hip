@@ -140,7 +140,9 @@ __global__ void __launch_bounds__(256) greedy_search_kernel(SearchArgs a) {
     // learned termination / traces: state features (identical on every lane)
     const bool watch = a.model != nullptr || a.trace != nullptr;
     const int kidx = (int)(kq < L ? kq : L) - 1;
-    float entry_d = watch ? key_dist(pool[0]) : 0.f;
+    const float entry_d = watch ? key_dist(pool[0]) : 0.f;
+    // k-th distance 1..4 expansions ago
+    float r0 = watch ? key_dist(pool[kidx]) : 0.f, r1 = r0, r2 = r0, r3 = r0;
     uint32_t settle = 0;  // last expansion that changed the top-k
     for (;;) {
         // first unexpanded slot >= k
@@ -155,10 +157,15 @@ __global__ void __launch_bounds__(256) greedy_search_kernel(SearchArgs a) {
             }
         }
         if (watch && hops > 0) {
-            // state after `hops` expansions; pool[kk] is the next candidate
+            // state after `hops` expansions
             float f[MLP_IN];
-            term_features((float)hops, key_dist(pool[kidx]), key_dist(pool[0]), entry_d,
-                          key_dist(pool[kk < Li ? kk : Li - 1]), (float)(hops - settle), (float)L, (float)kq, f);
+            const float kth = key_dist(pool[kidx]);
+            term_features((float)hops, kth, key_dist(pool[0]), entry_d, r3, (float)(hops - settle), (float)L,
+                          (float)kq, f);
+            r3 = r2;
+            r2 = r1;
+            r1 = r0;
+            r0 = kth;
             if (a.trace && hops <= a.trace_cap && lane < MLP_IN)
                 a.trace[((size_t)q * a.trace_cap + (hops - 1)) * MLP_IN + lane] = pick8(f, lane);
             if (kk < Li && a.model && hops >= a.model->min_hops && term_logit(a.model, f) > a.model->logit_threshold)
