@@ -99,7 +99,11 @@ typedef struct {
 typedef struct {
     uint32_t waves_per_block;  /* 1, 2 or 4 queries per workgroup (0 -> auto)        */
     uint32_t row_groups;       /* gather tile: 4*row_groups rows in flight (0 -> auto) */
-    uint32_t visited_slots;    /* LDS visited table size, power of two (0 -> auto)   */
+    uint32_t visited_slots;    /* LDS visited table size, power of two >= 64
+                                  (0 -> 8L rounded up to a power of two, clamped
+                                  to [1024, 2048])                                   */
+    uint32_t merge_buffer;     /* keys buffered before a pool merge: 0 auto (256 from
+                                  L = 1536 up), 1 none, or 64/128/192/256           */
 } wv_search_config;
 
 /* Learned early termination: a 2-layer MLP (8 -> 32 ReLU -> 1) over
@@ -185,7 +189,8 @@ int wv_index_search(wv_index *idx, const float *queries, uint32_t Q, const uint3
                     const uint32_t *k_per_q, uint32_t k_stride, uint32_t *out_ids,
                     float *out_dist, uint32_t *evals, uint32_t *hops, wv_search_stats *stats);
 /* Same with device pointers.  queries: Q rows of stride roundup(d,4) floats,
- * zero padded.  Everything else as above, on the device. */
+ * zero padded.  Everything else as above, on the device (d_L_per_q,
+ * d_k_per_q and d_entries are copied back once to check them). */
 int wv_index_search_dev(wv_index *idx, const float *d_queries, uint32_t Q,
                         const uint32_t *d_entries, uint64_t seed, uint32_t L, uint32_t k,
                         const uint32_t *d_L_per_q, const uint32_t *d_k_per_q, uint32_t k_stride,
@@ -204,6 +209,18 @@ int wv_index_search_ext_dev(wv_index *idx, const float *d_queries, uint32_t Q,
                             uint32_t k_stride, uint32_t *d_out_ids, float *d_out_dist,
                             uint32_t *d_evals, uint32_t *d_hops, const wv_search_ext *ext,
                             wv_search_stats *stats);
+
+/* Search autotuner: times the search of Q sample queries (device rows as
+ * for wv_index_search_dev; random entries from `seed`, one L and k, and the
+ * model of `ext` when given) under candidate tunings, by coordinate descent
+ * from the default plan over visited_slots {256..4096}, row_groups {4,8,16},
+ * merge_buffer {none,128,256} and waves_per_block {1,2,4}, each candidate
+ * timed as the best of 2 launches after a warm-up.  The fastest tuning is
+ * kept in the index (as wv_index_set_search_config would) and written to
+ * *best with its kernel time.  Results never depend on the tuning. */
+int wv_index_autotune(wv_index *idx, const float *d_queries, uint32_t Q, uint64_t seed, uint32_t L,
+                      uint32_t k, const wv_search_ext *ext, wv_search_config *best,
+                      double *best_ms);
 
 /* ---------------------------------------------------------------- MLP */
 /* X: n x 8 raw features g (standardised with the model's mean/inv_std

@@ -38,7 +38,7 @@ def test_struct_layouts_match_header():
     assert ctypes.sizeof(_capi.BuildStats) == 80
     assert ctypes.sizeof(_capi.SearchStats) == 24
     assert ctypes.sizeof(_capi.Graph) == 24
-    assert ctypes.sizeof(_capi.SearchConfig) == 12
+    assert ctypes.sizeof(_capi.SearchConfig) == 16
 
 
 def test_null_context_is_rejected():

@@ -31,13 +31,17 @@ def _random_model(seed):
     return m
 
 
-def test_traces_match_oracle(ctx, small_float, small_float_graph):
+@pytest.mark.parametrize("L,mb", [(48, 0), (48, 64), (600, 0)])
+def test_traces_match_oracle(ctx, small_float, small_float_graph, L, mb):
+    """Per-expansion features and settle points, with and without the merge
+    buffer (L = 600 buffers by default)."""
     base, q = small_float
     g, _, deg = small_float_graph
     ix = Index(ctx, base)
     ix.set_graph(g, deg)
-    ids, dist, ev, hp, st, trace, tlen, settle = ix.search_ext(q, 48, 10, seed=5, trace_cap=160)
-    rids, _, rev, rhp, rtrace, rtlen, rsettle = O.search_greedy_ext(g, deg, base, q, 48, 10, seed=5,
+    ix.set_search_config(merge_buffer=mb)
+    ids, dist, ev, hp, st, trace, tlen, settle = ix.search_ext(q, L, 10, seed=5, trace_cap=160)
+    rids, _, rev, rhp, rtrace, rtlen, rsettle = O.search_greedy_ext(g, deg, base, q, L, 10, seed=5,
                                                                     trace_cap=160)
     assert np.array_equal(ids, rids) and np.array_equal(hp, rhp)
     assert np.array_equal(tlen, rtlen) and np.array_equal(settle, rsettle)
@@ -119,8 +123,8 @@ def test_learned_search_matches_oracle_stop_rule(ctx, c4_small):
 
 
 @pytest.fixture(scope="module")
-def c5_small(ctx):
-    base, q = datagen.gmm_c5(n_base=20000, n_query=1000, clusters=200)
+def c4m_small(ctx):
+    base, q = datagen.gmm_c4m(n_base=20000, n_query=1000, clusters=200)
     ix = Index(ctx, base)
     ix.build_nndescent(32, 48, 16, 48, 10, seed=1)
     gt, _ = O.exact_knn(base, q, 10)
@@ -128,12 +132,12 @@ def c5_small(ctx):
     ix.close()
 
 
-def test_learned_termination_gains_at_equal_recall(ctx, c5_small):
+def test_learned_termination_gains_at_equal_recall(ctx, c4m_small):
     """Clusters of different spreads make query difficulty vary: plain
     fixed-L search sizes L for the hard queries and spends it on every query;
     the learned stop rule over the same-size pool stops the easy ones early.
     At equal recall on held-out queries it makes fewer distance evaluations."""
-    base, q, gt, ix = c5_small
+    base, q, gt, ix = c4m_small
     train_q, cal_q, test_q, test_gt = q[:300], q[300:600], q[600:], gt[600:]
     target, cap = 0.95, 400
     plain = {}

@@ -110,7 +110,8 @@ def test_search_config_changes_speed_only(ctx, small_float, small_float_graph):
     ix.set_graph(g, deg)
     ref = ix.search(q, 64, 10, seed=3)
     for cfg in [dict(waves_per_block=2), dict(waves_per_block=4, row_groups=4),
-                dict(row_groups=16), dict(visited_slots=64)]:
+                dict(row_groups=16), dict(visited_slots=64), dict(merge_buffer=64),
+                dict(merge_buffer=128, visited_slots=64, waves_per_block=2)]:
         ix.set_search_config(**cfg)
         got = ix.search(q, 64, 10, seed=3)
         assert np.array_equal(got[0], ref[0]) and np.array_equal(got[1], ref[1]), cfg
@@ -120,6 +121,66 @@ def test_search_config_changes_speed_only(ctx, small_float, small_float_graph):
             assert np.array_equal(got[2], ref[2]), cfg
         assert np.array_equal(got[3], ref[3])
     ix.close()
+
+
+@pytest.mark.parametrize("L", [40, 700])
+def test_search_autotune_changes_speed_only(ctx, small_float, small_float_graph, L):
+    """The autotuner keeps a valid tuning in the index; ids, distances and
+    hops are unchanged (evaluations only grow where a smaller visited table
+    forgets)."""
+    from weavess_amd import Index
+    base, q = small_float
+    g, _, deg = small_float_graph
+    ref = O.search_greedy(g, deg, base, q, L, 10, seed=6)
+    ix = Index(ctx, base)
+    ix.set_graph(g, deg)
+    cfg, ms = ix.autotune(q, L, 10, seed=6)
+    assert ms > 0 and cfg["row_groups"] in (0, 4, 8, 16) and cfg["waves_per_block"] in (0, 1, 2, 4)
+    assert cfg["visited_slots"] in (0, 256, 512, 1024, 2048, 4096)
+    got = ix.search(q, L, 10, seed=6)
+    ix.close()
+    assert np.array_equal(got[0], ref[0]) and np.array_equal(got[1], ref[1])
+    assert np.array_equal(got[3], ref[3]) and np.all(got[2] >= ref[2])
+
+
+@pytest.mark.parametrize("L", [48, 130, 600, 1500])
+@pytest.mark.parametrize("mb", [1, 64, 256])
+def test_search_merge_buffer_parity(ctx, small_float, small_float_graph, L, mb):
+    """The side buffer that delays pool merges (none, 64 or 256 keys) leaves
+    ids, distances, evaluations and hops bit-identical to the oracle, at small
+    pools and at pools most of the base (a visited table large enough for the
+    whole base keeps the evaluation counts exact)."""
+    from weavess_amd import Index
+    base, q = small_float
+    g, _, deg = small_float_graph
+    ref = O.search_greedy(g, deg, base, q, L, 10, seed=8)
+    ix = Index(ctx, base)
+    ix.set_graph(g, deg)
+    ix.set_search_config(merge_buffer=mb, visited_slots=16384)
+    got = ix.search(q, L, 10, seed=8)
+    ix.close()
+    assert np.array_equal(got[0], ref[0]) and np.array_equal(got[1], ref[1])
+    assert np.array_equal(got[2], ref[2]) and np.array_equal(got[3], ref[3])
+
+
+@pytest.mark.parametrize("L", [100, 700, 1900])
+def test_search_merge_buffer_integer_ties(ctx, small_int, small_int_graph, L):
+    """Integer data (many equal distances, ordered by id) through the default
+    plan, which buffers merges from L = 384 up; k = L returns the whole pool."""
+    from weavess_amd import Index
+    base, q = small_int
+    g, _, deg = small_int_graph
+    ref = O.search_greedy(g, deg, base, q[:50], L, L, seed=21)
+    ix = Index(ctx, base)
+    ix.set_graph(g, deg)
+    got = ix.search(q[:50], L, L, seed=21)
+    assert np.array_equal(got[0], ref[0]) and np.array_equal(got[1], ref[1])
+    assert np.array_equal(got[3], ref[3]) and np.all(got[2] >= ref[2])  # default table may forget
+    ix.set_search_config(visited_slots=16384)
+    got = ix.search(q[:50], L, L, seed=21)
+    ix.close()
+    assert np.array_equal(got[0], ref[0]) and np.array_equal(got[1], ref[1])
+    assert np.array_equal(got[2], ref[2]) and np.array_equal(got[3], ref[3])
 
 
 @pytest.mark.skip(reason="persistent work-queue search with an HBM visited bitmap is switched off: "

@@ -29,7 +29,7 @@ EXPORTS = [
     "wv_index_create", "wv_index_destroy", "wv_index_set_graph", "wv_index_get_graph",
     "wv_index_set_search_config", "wv_index_build_nndescent", "wv_index_search",
     "wv_index_search_dev", "wv_nndescent_build", "wv_search_greedy", "wv_exact_knn",
-    "wv_index_search_ext", "wv_index_search_ext_dev", "wv_mlp_grad", "wv_mlp_train",
+    "wv_index_search_ext", "wv_index_search_ext_dev", "wv_index_autotune", "wv_mlp_grad", "wv_mlp_train",
     "wv_mlp_predict",
 ]
 
@@ -70,7 +70,7 @@ class Graph(C.Structure):
 
 class SearchConfig(C.Structure):
     _fields_ = [("waves_per_block", C.c_uint32), ("row_groups", C.c_uint32),
-                ("visited_slots", C.c_uint32)]
+                ("visited_slots", C.c_uint32), ("merge_buffer", C.c_uint32)]
 
 
 class TermModel(C.Structure):
@@ -152,6 +152,7 @@ def load() -> C.CDLL:
         "wv_index_search_ext": ([vp, vp, u32, vp, u64, u32, u32, vp, vp, u32, vp, vp, vp, vp, vp, vp], i32),
         "wv_index_search_ext_dev": ([vp, vp, u32, vp, u64, u32, u32, vp, vp, u32, vp, vp, vp, vp, vp, vp],
                                     i32),
+        "wv_index_autotune": ([vp, vp, u32, u64, u32, u32, vp, vp, C.POINTER(C.c_double)], i32),
         "wv_mlp_grad": ([vp, vp, vp, vp, u32, C.c_float, vp, C.POINTER(C.c_double)], i32),
         "wv_mlp_train": ([vp, vp, vp, u32, vp, vp, vp], i32),
         "wv_mlp_predict": ([vp, vp, vp, u32, vp], i32),
@@ -311,8 +312,8 @@ class Index:
         self.ctx.check(self.ctx.lib.wv_index_get_graph(self.h, _p(ids), _p(dist), _p(deg), None))
         return ids, dist, deg
 
-    def set_search_config(self, waves_per_block=0, row_groups=0, visited_slots=0):
-        cfg = SearchConfig(waves_per_block, row_groups, visited_slots)
+    def set_search_config(self, waves_per_block=0, row_groups=0, visited_slots=0, merge_buffer=0):
+        cfg = SearchConfig(waves_per_block, row_groups, visited_slots, merge_buffer)
         self.ctx.check(self.ctx.lib.wv_index_set_search_config(self.h, C.byref(cfg)))
 
     def build_nndescent(self, K, L, S, R, iters, seed=0, dtype="f32", rerank_mult=0,
@@ -380,6 +381,28 @@ class Index:
             k_stride if k_stride is not None else k, vp(d_out_ids), vp(d_out_dist),
             vp(d_evals), vp(d_hops), C.byref(ext), C.byref(st) if want_stats else None))
         return st.as_dict()
+
+    def autotune_dev(self, d_queries: int, Q: int, L: int, k: int, seed: int = 0,
+                     model: Optional["TermModel"] = None):
+        """Pick the fastest search tuning for (L, k[, model]) on Q sample
+        queries (device rows); the index keeps it.  Returns (config dict,
+        kernel ms of the sample)."""
+        cfg = SearchConfig()
+        ms = C.c_double(0.0)
+        ext = SearchExt(C.addressof(model), None, None, None, 0) if model is not None else None
+        self.ctx.check(self.ctx.lib.wv_index_autotune(self.h, C.c_void_p(d_queries), Q, seed, L, k,
+                                                      C.byref(ext) if ext is not None else None,
+                                                      C.byref(cfg), C.byref(ms)))
+        return {f: getattr(cfg, f) for f, _ in cfg._fields_}, ms.value
+
+    def autotune(self, queries, L: int, k: int, seed: int = 0, model: Optional["TermModel"] = None):
+        """autotune_dev on host sample queries (copied to the device)."""
+        from .devmem import DeviceArray, pad_rows
+        dq = DeviceArray.from_host(pad_rows(_c(queries, np.float32)))
+        try:
+            return self.autotune_dev(dq.ptr, len(queries), L, k, seed=seed, model=model)
+        finally:
+            dq.free()
 
     def search_dev(self, d_queries: int, Q: int, L: int, k: int, d_out_ids: int,
                    d_out_dist: int = 0, seed: int = 0, d_entries: int = 0, d_L: int = 0,

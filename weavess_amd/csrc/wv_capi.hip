@@ -36,7 +36,7 @@ struct wv_index {
     float *gdist = nullptr;     // N x deg_max (NaN when unknown)
     uint32_t *deg = nullptr;    // N
     uint32_t deg_max = 0;
-    wv_search_config cfg = {0, 0, 0};
+    wv_search_config cfg = {0, 0, 0, 0};
 };
 
 namespace {
@@ -97,23 +97,38 @@ int chunks_per_lane(uint32_t nch) {
 }
 
 struct SearchPlan {
-    uint32_t P, H, wpb, groups, Lmax;
+    uint32_t P, H, B, wpb, groups, Lmax;
 };
+
+// Pools from this size up get the 256-key merge buffer by default.
+constexpr uint32_t MERGE_BUFFER_MIN_L = 2560;
 
 int plan_search(wv_index *ix, uint32_t Lmax, SearchPlan &pl) {
     pl.Lmax = Lmax;
-    pl.P = std::max<uint32_t>(64, next_pow2(Lmax));
+    pl.P = std::max<uint32_t>(64, (Lmax + 63) / 64 * 64);
+    // The visited table forgets under pressure (costing re-evaluations only);
+    // LDS per wavefront sets how many queries are in flight, and that, not
+    // the re-evaluations, bounds throughput on the benchmark graphs.
+    // Default: large enough to stay exact for small pools; small for large
+    // ones, where a bigger table buys < 1 % fewer evaluations but costs
+    // resident queries.
     uint32_t H = ix->cfg.visited_slots;
-    if (H == 0) H = std::min<uint32_t>(4096, std::max<uint32_t>(1024, next_pow2(Lmax * 8)));
-    if (H & (H - 1)) return fail(ix->ctx, WV_EINVAL, "visited_slots must be a power of two");
-    pl.H = std::max(H, pl.P);  // the table's storage also holds the entry ids
+    if (H == 0)
+        H = Lmax <= 256 ? std::min<uint32_t>(2048, std::max<uint32_t>(1024, next_pow2(Lmax * 8)))
+                        : (Lmax <= 512 ? 1024u : 256u);
+    if (H < 64 || (H & (H - 1))) return fail(ix->ctx, WV_EINVAL, "visited_slots must be a power of two >= 64");
+    pl.H = H;
+    // merge buffer: pays off once a pool merge costs several 64-slot chunks
+    const uint32_t mb = ix->cfg.merge_buffer;
+    pl.B = mb == 0 ? (Lmax >= MERGE_BUFFER_MIN_L ? 256u : 0u) : (mb == 1 ? 0u : mb);
     uint32_t wpb = ix->cfg.waves_per_block ? ix->cfg.waves_per_block : 1;
     if (wpb != 1 && wpb != 2 && wpb != 4) return fail(ix->ctx, WV_EINVAL, "waves_per_block must be 1, 2 or 4");
-    size_t lds = wv::search_wave_lds(pl.P, pl.H);
+    size_t lds = wv::search_wave_lds(pl.P, pl.H, pl.B);
     while (wpb > 1 && lds * wpb > 160 * 1024) wpb >>= 1;
     if (lds > 160 * 1024) return fail(ix->ctx, WV_ENOTSUP, "pool/visited table need %zu B of LDS", lds);
     pl.wpb = wpb;
-    pl.groups = ix->cfg.row_groups ? ix->cfg.row_groups : 8;
+    // gather tile: 4 row groups keep VGPRs (and so residency) low for d <= 256
+    pl.groups = ix->cfg.row_groups ? ix->cfg.row_groups : (chunks_per_lane(ix->dp / 4) <= 4 ? 4u : 8u);
     return WV_OK;
 }
 
@@ -174,7 +189,8 @@ int search_core(wv_index *ix, const float *d_q, uint32_t Q, const uint32_t *d_en
     a.k_stride = k_stride;
     a.P = pl.P;
     a.H = pl.H;
-    a.wave_lds = (uint32_t)wv::search_wave_lds(pl.P, pl.H);
+    a.B = pl.B;
+    a.wave_lds = (uint32_t)wv::search_wave_lds(pl.P, pl.H, pl.B);
     a.model = ext.model;
     a.trace = ext.trace;
     a.trace_len = ext.trace_len;
@@ -378,10 +394,12 @@ int wv_index_get_graph(wv_index *ix, uint32_t *ids, float *dist, uint32_t *deg, 
 int wv_index_set_search_config(wv_index *ix, const wv_search_config *cfg) {
     if (!ix || !cfg) return WV_EINVAL;
     std::lock_guard<std::mutex> g(ix->ctx->mu);
-    if (cfg->visited_slots & (cfg->visited_slots - 1))
-        return fail(ix->ctx, WV_EINVAL, "visited_slots must be a power of two");
+    if (cfg->visited_slots && (cfg->visited_slots < 64 || (cfg->visited_slots & (cfg->visited_slots - 1))))
+        return fail(ix->ctx, WV_EINVAL, "visited_slots must be 0 or a power of two >= 64");
     if (cfg->waves_per_block > 4 || cfg->waves_per_block == 3)
         return fail(ix->ctx, WV_EINVAL, "waves_per_block must be 0, 1, 2 or 4");
+    if (cfg->merge_buffer > 1 && (cfg->merge_buffer % 64 || cfg->merge_buffer > 256))
+        return fail(ix->ctx, WV_EINVAL, "merge_buffer must be 0, 1, 64, 128, 192 or 256");
     ix->cfg = *cfg;
     return WV_OK;
 }
@@ -583,6 +601,80 @@ int wv_index_search_ext_dev(wv_index *ix, const float *d_queries, uint32_t Q, co
     }
     return search_core(ix, d_queries, Q, d_entries, seed, L, k, d_L_per_q, d_k_per_q, Lmax, k_stride,
                        d_out_ids, d_out_dist, d_evals, d_hops, stats, x);
+}
+
+int wv_index_autotune(wv_index *ix, const float *d_queries, uint32_t Q, uint64_t seed, uint32_t L, uint32_t k,
+                      const wv_search_ext *ext, wv_search_config *best, double *best_ms) {
+    if (!ix) return WV_EINVAL;
+    wv_ctx *c = ix->ctx;
+    std::lock_guard<std::mutex> g(c->mu);
+    if (!d_queries || Q == 0) return fail(c, WV_EINVAL, "autotune needs sample queries");
+    uint32_t Lmax = 0;
+    int rc = check_search_args(c, ix->N, Q, nullptr, L, k, nullptr, nullptr, k, Lmax);
+    if (rc) return rc;
+    WV_HIP(c, hipSetDevice(c->device));
+    DevBuf dm, di;
+    SearchExtDev x;
+    if (ext && ext->model) {
+        WV_HIP(c, dm.alloc(sizeof(wv_term_model)));
+        WV_HIP(c, hipMemcpy(dm.p, ext->model, sizeof(wv_term_model), hipMemcpyHostToDevice));
+        x.model = dm.as<wv_term_model>();
+    }
+    WV_HIP(c, di.alloc((size_t)Q * k * 4));
+    const wv_search_config saved = ix->cfg;
+    // best of 2 timed launches after a warm-up; planning errors reject the
+    // candidate (rc EINVAL/ENOTSUP), launch errors end the tuning
+    auto time_cfg = [&](const wv_search_config &cfg, double &ms) -> int {
+        ix->cfg = cfg;
+        wv_search_stats st;
+        ms = 1e30;
+        for (int r = 0; r < 3; ++r) {
+            const int e = search_core(ix, d_queries, Q, nullptr, seed, L, k, nullptr, nullptr, Lmax, k,
+                                      di.as<uint32_t>(), nullptr, nullptr, nullptr, &st, x);
+            if (e) return e;
+            if (r > 0) ms = std::min(ms, st.kernel_ms);
+        }
+        return WV_OK;
+    };
+    wv_search_config cur = {0, 0, 0, 0};
+    double cur_ms = 0;
+    if ((rc = time_cfg(cur, cur_ms))) {
+        ix->cfg = saved;
+        return rc;
+    }
+    static const uint32_t hs[] = {256, 512, 1024, 2048, 4096}, rgs[] = {4, 8, 16}, mbs[] = {1, 128, 256},
+                          wps[] = {1, 2, 4};
+    struct Axis {
+        uint32_t wv_search_config::*field;
+        const uint32_t *vals;
+        int n;
+    };
+    const Axis axes[] = {{&wv_search_config::visited_slots, hs, 5},
+                         {&wv_search_config::row_groups, rgs, 3},
+                         {&wv_search_config::merge_buffer, mbs, 3},
+                         {&wv_search_config::waves_per_block, wps, 3}};
+    for (const Axis &ax : axes) {
+        const wv_search_config base = cur;
+        for (int i = 0; i < ax.n; ++i) {
+            wv_search_config t = base;
+            t.*(ax.field) = ax.vals[i];
+            double ms = 0;
+            const int e = time_cfg(t, ms);
+            if (e == WV_EINVAL || e == WV_ENOTSUP) continue;
+            if (e) {
+                ix->cfg = saved;
+                return e;
+            }
+            if (ms < cur_ms) {
+                cur = t;
+                cur_ms = ms;
+            }
+        }
+    }
+    ix->cfg = cur;
+    if (best) *best = cur;
+    if (best_ms) *best_ms = cur_ms;
+    return WV_OK;
 }
 
 int wv_mlp_grad(wv_ctx *c, const wv_term_model *m, const float *X, const float *y, uint32_t n,

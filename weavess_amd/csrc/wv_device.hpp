@@ -162,6 +162,17 @@ __device__ inline int lower_bound64(const uint64_t *a, int n, uint64_t key) {
     return lo;
 }
 
+// Number of keys <= key in the sorted a[0..n).
+__device__ inline int upper_bound64(const uint64_t *a, int n, uint64_t key) {
+    int lo = 0, hi = n;
+    while (lo < hi) {
+        int mid = (lo + hi) >> 1;
+        if (a[mid] <= key) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
 // Inclusive wave-wide max scan of a signed 64-bit value.
 __device__ inline int64_t wave_scan_max64(int64_t v) {
     const int lane = lane_id();

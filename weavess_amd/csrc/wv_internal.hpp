@@ -26,8 +26,9 @@ struct SearchArgs {
     uint64_t seed;
     uint32_t N, dp, nch, deg_max;
     uint32_t Q, L, k, k_stride;
-    uint32_t P;               // power of two >= every L
+    uint32_t P;               // pool slots: every L rounded up to a multiple of 64
     uint32_t H;               // visited-table slots (power of two)
+    uint32_t B;               // merge-buffer keys per wavefront (0: none; <= 256)
     uint32_t wave_lds;        // LDS bytes per wavefront
     const wv_term_model *model;  // learned termination (device copy) or null
     float *trace;             // Q x trace_cap x 8, or null
@@ -36,7 +37,7 @@ struct SearchArgs {
     uint32_t trace_cap;
 };
 
-size_t search_wave_lds(uint32_t P, uint32_t H);
+size_t search_wave_lds(uint32_t P, uint32_t H, uint32_t B);
 // One wavefront per query (grid of ceil(Q / waves_per_block) workgroups).
 hipError_t launch_greedy_search(const SearchArgs &a, int chunks_per_lane, int row_groups,
                                 int waves_per_block, hipStream_t s);

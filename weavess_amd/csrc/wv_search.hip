@@ -7,28 +7,33 @@
 // wave-wide merge.
 //
 // Per wavefront, in LDS: the sorted pool of L (dist,id) keys, its
-// "unexpanded" flags, a 64-entry candidate block and an open-addressing
-// visited table.  One expansion:
-//   1. ballot over the flags finds the first unexpanded pool slot >= k;
+// "unexpanded" flags, a sorted side buffer of at most B keys with flags, a
+// 64-entry candidate block and an open-addressing visited table.  The
+// search's candidate set is U = the L smallest keys of pool + buffer; the
+// buffer only delays merges (with B = 0 candidates go straight into the
+// pool).  One expansion:
+//   1. ballots over the flags find the first unexpanded key of the pool and
+//      of the buffer; the smaller one is expanded if fewer than L keys of
+//      pool + buffer are smaller (it is in U), otherwise the search ends;
 //   2. lane i takes neighbour i of that node (64 per block; more in blocks);
 //   3. LDS atomicCAS test-and-set in the visited table; survivors are
 //      compacted;
 //   4. distances, 16 lanes per row with float4 gathers, in the build order;
-//   5. candidates not better than the pool's worst, or already in the pool,
-//      are dropped; the rest are bitonic-sorted across the wave and merged
-//      into the pool in place: a candidate's slot is its rank plus the number
-//      of smaller pool keys, a pool key moves right by the number of smaller
-//      candidates; keys are moved in 64-slot chunks from the tail backwards
-//      (all reads of a chunk precede its writes), so slots before the first
-//      insertion point are never touched;
-//   6. the k-pointer restart of route.cpp:44-45 uses the final slot of the
-//      best merged candidate, which equals the minimum insert position of
-//      the sequential loop when (dist,id) is a strict order.
+//   5. candidates not below the L-th key of pool + buffer, or already in
+//      either, are dropped; the rest are bitonic-sorted across the wave and
+//      merged in place into the buffer (into the pool when B = 0); a full
+//      buffer is first merged into the pool, keeping the L smallest.
+// Selecting the L smallest keys is associative, so U, and with it the
+// expansion order, is the same as inserting every candidate into an L-slot
+// pool at once (Index::InsertIntoPool, include/weavess/index.h:152-180); the
+// next expansion is always U's first unexpanded key, which is where the
+// k-pointer restart of src/component_route.cpp:44-45 lands when (dist,id) is
+// a strict order.
 // The visited table forgets: when an id finds no free slot within the probe
 // limit it evicts the occupant of its home slot.  A forgotten node that comes
-// back is re-evaluated; it is then either already in the pool (duplicate) or
-// no better than the pool's worst, so ids and distances are unchanged and
-// only the evaluation count grows.
+// back is re-evaluated; it is then either already in pool or buffer
+// (duplicate) or not below the L-th key, so ids and distances are unchanged
+// and only the evaluation count grows.
 #include "wv_device.hpp"
 #include "wv_internal.hpp"
 #include "wv_mlp.hpp"
@@ -55,23 +60,115 @@ __device__ inline bool visit_first(uint32_t *vis, uint32_t hmask, uint32_t id) {
     return true;
 }
 
+// Merge c sorted keys src[0..c) (flags sflag[], or 1 when null) into the
+// sorted array arr[0..len) of capacity cap, in place: a key of arr moves
+// right by the number of smaller src keys, src[j] lands at j + the number of
+// arr keys <= it (so equal keys never collide), and keys pushed to slots
+// >= cap are dropped.  Keys are moved in 64-slot chunks from the tail
+// backwards (all reads of a chunk precede its writes), so slots before the
+// first insertion point are never touched.  1 <= c <= 4 * WAVE.  Returns the
+// final slot of src[0] (>= cap when it was dropped).
+__device__ inline int merge_into(uint64_t *arr, uint8_t *aflag, int len, int cap, const uint64_t *src,
+                                 const uint8_t *sflag, int c) {
+    const int lane = lane_id();
+    int cpos[4];
+    uint64_t ck[4];
+    uint8_t cf[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const int j = lane + r * WAVE;
+        cpos[r] = cap;
+        ck[r] = 0;
+        cf[r] = 0;
+        if (j < c) {
+            ck[r] = src[j];
+            cf[r] = sflag ? sflag[j] : (uint8_t)1;
+            cpos[r] = j + upper_bound64(arr, len, ck[r]);
+        }
+    }
+    const int pos0 = upper_bound64(arr, len, src[0]);
+    for (int t0 = (len - 1) & ~(WAVE - 1); t0 >= (pos0 & ~(WAVE - 1)); t0 -= WAVE) {
+        const int p = t0 + lane;
+        const bool mv = p >= pos0 && p < len;
+        uint64_t pk = 0;
+        uint8_t pf = 0;
+        int np = cap;
+        if (mv) {
+            pk = arr[p];
+            pf = aflag[p];
+            np = p + lower_bound64(src, c, pk);
+        }
+        wave_sync();
+        if (mv && np < cap) {
+            arr[np] = pk;
+            aflag[np] = pf;
+        }
+        wave_sync();
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+        if (cpos[r] < cap) {
+            arr[cpos[r]] = ck[r];
+            aflag[cpos[r]] = cf[r];
+        }
+    wave_sync();
+    return pos0;
+}
+
+// First slot >= from of f[0..n) whose flag is set, or n.
+__device__ inline int first_flag(const uint8_t *f, int from, int n) {
+    const int lane = lane_id();
+    for (int t0 = from & ~(WAVE - 1); t0 < n; t0 += WAVE) {
+        const int p = t0 + lane;
+        const uint64_t b = __ballot(p >= from && p < n && f[p]);
+        if (b) return t0 + __ffsll((unsigned long long)b) - 1;
+    }
+    return n;
+}
+
+// The k-th smallest (0-based) key of the union of the sorted, disjoint key
+// arrays a[0..na) and b[0..nb), k < na + nb: the split that takes i keys of b
+// and k+1-i of a, with i the largest value for which b[i-1] < a[k+1-i].
+// Every lane reads the same slots and returns the same value.
+__device__ inline uint64_t kth_union(const uint64_t *a, int na, const uint64_t *b, int nb, int k) {
+    int lo = max(0, k + 1 - na), hi = min(nb, k + 1);
+    while (lo < hi) {
+        const int i = (lo + hi + 1) >> 1;
+        const int j = k + 1 - i;
+        if (j >= na || b[i - 1] < a[j]) lo = i;
+        else hi = i - 1;
+    }
+    const int j = k + 1 - lo;
+    const uint64_t x = lo > 0 ? b[lo - 1] : 0, y = j > 0 ? a[j - 1] : 0;
+    return x > y ? x : y;
+}
+
 // SearchEntryRand's GenRandom (include/weavess/util.h:13-27) on the query's
 // counter stream: L draws mod (N-L), sorted, made strictly increasing
 // (a'[i] = i + max_{j<=i}(a[j]-j), a prefix-max scan), rotated by one more
-// draw mod N.  a has P >= L slots.
-__device__ inline void gen_entries(uint64_t seed, uint32_t q, uint32_t L, uint32_t N, uint32_t P,
-                                   uint32_t *a) {
+// draw mod N.  The draws are sorted as (value, index) keys, 64 at a time,
+// merged into a[0..L) (aflag is scratch); on return a[i] holds entry i's id.
+// tmp: 64 keys of scratch.
+__device__ inline void gen_entries(uint64_t seed, uint32_t q, uint32_t L, uint32_t N, uint64_t *a,
+                                   uint8_t *aflag, uint64_t *tmp) {
     const int lane = lane_id();
     const uint64_t s = stream_of(seed, TAG_ENTRY, q);
     const uint32_t mod = N - L;
-    for (uint32_t i = lane; i < P; i += WAVE) a[i] = i < L ? draw(s, i) % mod : 0xFFFFFFFFu;
-    wave_sync();
-    lds_sort32(a, (int)P);
+    int len = 0;
+    for (uint32_t c0 = 0; c0 < L; c0 += WAVE) {
+        const uint32_t i = c0 + lane;
+        const uint64_t key = wave_sort64(i < L ? ((uint64_t)(draw(s, i) % mod) << 32) | i : KEY_EMPTY);
+        const int cnt = (int)min((uint32_t)WAVE, L - c0);
+        tmp[lane] = key;
+        wave_sync();
+        merge_into(a, aflag, len, (int)L, tmp, nullptr, cnt);
+        len += cnt;
+    }
     const int seg = (int)((L + WAVE - 1) / WAVE);
     const int s0 = lane * seg, s1 = min(s0 + seg, (int)L);
     int64_t mx = INT64_MIN;
     for (int i = s0; i < s1; ++i) {
-        int64_t v = (int64_t)a[i] - i;
+        int64_t v = (int64_t)(a[i] >> 32) - i;
         if (v > mx) mx = v;
     }
     int64_t incl = wave_scan_max64(mx);
@@ -80,9 +177,9 @@ __device__ inline void gen_entries(uint64_t seed, uint32_t q, uint32_t L, uint32
     int64_t run = lane == 0 ? INT64_MIN : (int64_t)(((uint64_t)hi << 32) | lo);
     const uint64_t off = draw(s, L) % N;
     for (int i = s0; i < s1; ++i) {
-        int64_t v = (int64_t)a[i] - i;
+        int64_t v = (int64_t)(a[i] >> 32) - i;
         if (v > run) run = v;
-        a[i] = (uint32_t)(((uint64_t)(run + i) + off) % N);
+        a[i] = ((uint64_t)(run + i) + off) % N;
     }
     wave_sync();
 }
@@ -96,11 +193,13 @@ __global__ void __launch_bounds__(256) greedy_search_kernel(SearchArgs a) {
     if (q >= a.Q) return;
 
     uint8_t *my = smem + (size_t)wid * a.wave_lds;
-    uint64_t *pool = reinterpret_cast<uint64_t *>(my);         // P keys
+    uint64_t *pool = reinterpret_cast<uint64_t *>(my);         // P keys (P >= L, multiple of 64)
     uint64_t *ckey = pool + a.P;                                // 64
     uint32_t *cid = reinterpret_cast<uint32_t *>(ckey + WAVE);  // 64
-    uint32_t *vis = cid + WAVE;                                 // H (>= P)
+    uint32_t *vis = cid + WAVE;                                 // H
     uint8_t *flag = reinterpret_cast<uint8_t *>(vis + a.H);     // P
+    uint64_t *buf = reinterpret_cast<uint64_t *>(flag + a.P);   // B (P is a multiple of 64)
+    uint8_t *bflag = reinterpret_cast<uint8_t *>(buf + a.B);    // B
 
     const uint32_t L = a.L_per_q ? a.L_per_q[q] : a.L;
     const uint32_t kq = a.k_per_q ? a.k_per_q[q] : a.k;
@@ -116,27 +215,48 @@ __global__ void __launch_bounds__(256) greedy_search_kernel(SearchArgs a) {
         }
     }
 
-    // ---- entries (SearchEntryRand): not marked visited, all flagged; the
-    // visited table's storage holds the entry ids until they are scored
-    uint32_t *ent = vis;
+    // ---- entries (SearchEntryRand): not marked visited, all flagged.  The
+    // pool's storage first holds the L entry ids (one per 8-byte slot), then
+    // their keys: distances 64 at a time, each sorted chunk merged into the
+    // growing pool.  A merge writes only slots below the merged length, and
+    // the ids of later chunks sit above it.
     if (a.entries) {
-        for (uint32_t i = lane; i < L; i += WAVE) ent[i] = a.entries[(size_t)q * L + i];
+        for (uint32_t i = lane; i < L; i += WAVE) {
+            const uint32_t e = a.entries[(size_t)q * L + i];
+            pool[i] = e < N ? e : N - 1;  // out-of-range ids (unchecked device input) are clamped
+        }
         wave_sync();
     } else {
-        gen_entries(a.seed, q, L, N, a.P, ent);
+        gen_entries(a.seed, q, L, N, pool, flag, ckey);
     }
-    rows_dist<CPL, G>(a.base, dp, nch, qv, ent, (int)L, pool);
+    {
+        int len = 0;
+        for (uint32_t c0 = 0; c0 < L; c0 += WAVE) {
+            const int cnt = (int)min((uint32_t)WAVE, L - c0);
+            if (lane < cnt) cid[lane] = (uint32_t)pool[c0 + lane];
+            wave_sync();
+            rows_dist<CPL, G>(a.base, dp, nch, qv, cid, cnt, ckey);
+            wave_sync();
+            const uint64_t key = wave_sort64(lane < cnt ? ckey[lane] : KEY_EMPTY);
+            wave_sync();
+            ckey[lane] = key;
+            wave_sync();
+            merge_into(pool, flag, len, (int)L, ckey, nullptr, cnt);
+            len += cnt;
+        }
+    }
     for (uint32_t i = L + lane; i < a.P; i += WAVE) pool[i] = KEY_EMPTY;
-    wave_sync();
-    lds_sort64(pool, (int)a.P);
     const uint32_t hmask = a.H - 1;
     for (uint32_t i = lane; i < a.H; i += WAVE) vis[i] = ID_NONE;
     for (uint32_t i = lane; i < a.P; i += WAVE) flag[i] = i < L ? 1 : 0;
     wave_sync();
 
     uint32_t evals = L, hops = 0;
-    int k = 0;
     const int Li = (int)L;
+    const int B = (int)a.B;    // buffer capacity; 0: merge straight into the pool
+    int kp = 0;                // pool slots before kp are expanded
+    int nb = 0, kb = 0;        // buffer length; buffer slots before kb are expanded
+    uint64_t worst = pool[Li - 1];  // the L-th key of pool + buffer
     // learned termination / traces: state features (identical on every lane)
     const bool watch = a.model != nullptr || a.trace != nullptr;
     const int kidx = (int)(kq < L ? kq : L) - 1;
@@ -145,40 +265,44 @@ __global__ void __launch_bounds__(256) greedy_search_kernel(SearchArgs a) {
     float r0 = watch ? key_dist(pool[kidx]) : 0.f, r1 = r0, r2 = r0, r3 = r0;
     uint32_t settle = 0;  // last expansion that changed the top-k
     for (;;) {
-        // first unexpanded slot >= k
-        int kk = Li;
-        for (int t0 = (k & ~(WAVE - 1)); t0 < Li; t0 += WAVE) {
-            const int p = t0 + lane;
-            const bool f = p >= k && p < Li && flag[p];
-            const uint64_t b = __ballot(f);
-            if (b) {
-                kk = t0 + __ffsll((unsigned long long)b) - 1;
-                break;
-            }
-        }
+        // U's first unexpanded key: the smaller of the pool's and the buffer's
+        const int kk = first_flag(flag, kp, Li);
+        const int jb = nb ? first_flag(bflag, kb, nb) : 0;
+        const uint64_t ka = kk < Li ? pool[kk] : KEY_EMPTY;
+        const uint64_t kbk = jb < nb ? buf[jb] : KEY_EMPTY;
+        const bool from_buf = kbk < ka;
+        const uint64_t cur = from_buf ? kbk : ka;
+        bool done = cur == KEY_EMPTY;
+        if (!done && nb)  // in U only if fewer than L keys are smaller
+            done = (from_buf ? jb + lower_bound64(pool, Li, cur) : kk + lower_bound64(buf, nb, cur)) >= Li;
         if (watch && hops > 0) {
             // state after `hops` expansions
             float f[MLP_IN];
-            const float kth = key_dist(pool[kidx]);
-            term_features((float)hops, kth, key_dist(pool[0]), entry_d, r3, (float)(hops - settle), (float)L,
-                          (float)kq, f);
+            const uint64_t kth = nb ? kth_union(pool, Li, buf, nb, kidx) : pool[kidx];
+            const uint64_t best = (nb && buf[0] < pool[0]) ? buf[0] : pool[0];
+            term_features((float)hops, key_dist(kth), key_dist(best), entry_d, r3, (float)(hops - settle),
+                          (float)L, (float)kq, f);
             r3 = r2;
             r2 = r1;
             r1 = r0;
-            r0 = kth;
+            r0 = key_dist(kth);
             if (a.trace && hops <= a.trace_cap && lane < MLP_IN)
                 a.trace[((size_t)q * a.trace_cap + (hops - 1)) * MLP_IN + lane] = pick8(f, lane);
-            if (kk < Li && a.model && hops >= a.model->min_hops && term_logit(a.model, f) > a.model->logit_threshold)
+            if (!done && a.model && hops >= a.model->min_hops && term_logit(a.model, f) > a.model->logit_threshold)
                 break;
         }
-        if (kk >= Li) break;
-        const uint32_t u = key_id(pool[kk]);
+        if (done) break;
+        const uint32_t u = key_id(cur);
         wave_sync();
-        if (lane == 0) flag[kk] = 0;
+        if (lane == 0) {
+            if (from_buf) bflag[jb] = 0;
+            else flag[kk] = 0;
+        }
+        if (from_buf) kb = jb + 1;
+        else kp = kk + 1;
         ++hops;
         uint32_t du = a.deg[u];
         if (du > a.deg_max) du = a.deg_max;
-        int nk = Li;
         const uint32_t *adj = a.graph + (size_t)u * a.deg_max;
         for (uint32_t m0 = 0; m0 < du; m0 += WAVE) {
             const uint32_t mi = m0 + lane;
@@ -192,17 +316,19 @@ __global__ void __launch_bounds__(256) greedy_search_kernel(SearchArgs a) {
             rows_dist<CPL, G>(a.base, dp, nch, qv, cid, m, ckey);
             evals += (uint32_t)m;
             wave_sync();
-            const uint64_t worst = pool[Li - 1];
             uint64_t key = lane < m ? ckey[lane] : KEY_EMPTY;
-            int lb = Li;
             if (key < worst) {
-                lb = lower_bound64(pool, Li, key);
+                // worst <= pool[L-1], so the pool slot read is in range
+                const int lb = lower_bound64(pool, Li, key);
                 if (pool[lb] == key) key = KEY_EMPTY;  // already pooled: duplicate
+                if (nb) {
+                    const int lbb = lower_bound64(buf, nb, key);
+                    if (lbb < nb && buf[lbb] == key) key = KEY_EMPTY;
+                }
             } else {
                 key = KEY_EMPTY;
             }
             if (__ballot(key != KEY_EMPTY) == 0) continue;
-            // sort (key, lb) pairs by key: lb is monotone in key, recompute after
             key = wave_sort64(key);
             const uint64_t prev = shfl64(key, lane > 0 ? lane - 1 : 0);
             const bool live = key != KEY_EMPTY && !(lane > 0 && prev == key);
@@ -212,40 +338,27 @@ __global__ void __launch_bounds__(256) greedy_search_kernel(SearchArgs a) {
             wave_sync();
             if (live) ckey[r] = key;
             wave_sync();
-            // final slot of each live candidate, from the unmoved pool
-            int cpos = Li;
-            if (live) cpos = r + lower_bound64(pool, Li, key);
-            const int pos0 = lower_bound64(pool, Li, ckey[0]);
-            // shift pool keys right by the number of smaller candidates,
-            // 64-slot chunks from the tail back to the chunk holding pos0
-            for (int t0 = (Li - 1) & ~(WAVE - 1); t0 >= (pos0 & ~(WAVE - 1)); t0 -= WAVE) {
-                const int p = t0 + lane;
-                const bool mv = p >= pos0 && p < Li;
-                uint64_t pk = 0;
-                uint8_t pf = 0;
-                int np = Li;
-                if (mv) {
-                    pk = pool[p];
-                    pf = flag[p];
-                    np = p + lower_bound64(ckey, c, pk);
+            if (B == 0) {
+                const int pos0 = merge_into(pool, flag, Li, Li, ckey, nullptr, c);
+                if (pos0 < kp) kp = pos0;
+                if (pos0 <= kidx) settle = hops;  // a candidate entered the top-k
+                worst = pool[Li - 1];
+            } else {
+                if (nb + c > B) {  // full: merge the buffer into the pool
+                    const int pos0 = merge_into(pool, flag, Li, Li, buf, bflag, nb);
+                    if (pos0 < kp) kp = pos0;
+                    nb = 0;
+                    kb = 0;
                 }
-                wave_sync();
-                if (mv && np < Li) {
-                    pool[np] = pk;
-                    flag[np] = pf;
-                }
-                wave_sync();
+                const int bp = merge_into(buf, bflag, nb, B, ckey, nullptr, c);
+                nb += c;
+                if (bp < kb) kb = bp;
+                if (bp + lower_bound64(pool, Li, ckey[0]) <= kidx) settle = hops;
+                worst = kth_union(pool, Li, buf, nb, Li - 1);
             }
-            if (live && cpos < Li) {
-                pool[cpos] = key;
-                flag[cpos] = 1;
-            }
-            if (pos0 < nk) nk = pos0;
-            if (pos0 <= kidx) settle = hops;  // a candidate entered the top-k
-            wave_sync();
         }
-        k = (nk <= kk) ? nk : kk + 1;
     }
+    if (nb) merge_into(pool, flag, Li, Li, buf, bflag, nb);
 
     for (uint32_t i = lane; i < a.k_stride; i += WAVE) {
         const bool ok = i < kq;
@@ -276,8 +389,8 @@ hipError_t launch_t(const SearchArgs &a, int waves_per_block, hipStream_t s) {
 
 }  // namespace
 
-size_t search_wave_lds(uint32_t P, uint32_t H) {
-    size_t b = (size_t)P * 8 + WAVE * 8 + WAVE * 4 + (size_t)H * 4 + (size_t)P;
+size_t search_wave_lds(uint32_t P, uint32_t H, uint32_t B) {
+    size_t b = (size_t)P * 8 + WAVE * 8 + WAVE * 4 + (size_t)H * 4 + (size_t)P + (size_t)B * 9;
     return (b + 15) & ~(size_t)15;
 }
 

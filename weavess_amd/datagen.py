@@ -15,7 +15,7 @@ large sets are generated in independent chunks so memory stays bounded).
       centres uniform on the unit sphere, x = centre + 0.3 * N(0,I) (sigma
       0.3 per dimension before normalisation), then x / |x|; per-query
       L ~ U{16..400}, k ~ U{1,10,100}.
-  c5  as c4 but each cluster has its own spread, sigma_c log-uniform in
+  c4m as c4 but each cluster has its own spread, sigma_c log-uniform in
       [0.05, 0.6]: query difficulty varies across clusters, the setting an
       adaptive (learned) stop rule is for.
 All query sets come from the same mixture as their base set.
@@ -95,7 +95,7 @@ def gmm_c4(n_base=2_000_000, n_query=200_000, d=96, clusters=2_000, sigma=0.3, s
             _mixture(n_query, centres, probs, sample, seed, 2))
 
 
-def gmm_c5(n_base=2_000_000, n_query=200_000, d=96, clusters=2_000, sigma_lo=0.05, sigma_hi=0.6, seed=5):
+def gmm_c4m(n_base=2_000_000, n_query=200_000, d=96, clusters=2_000, sigma_lo=0.05, sigma_hi=0.6, seed=5):
     g = _rng(seed, 0)
     centres = g.normal(0.0, 1.0, size=(clusters, d))
     centres /= np.linalg.norm(centres, axis=1, keepdims=True)

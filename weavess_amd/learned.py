@@ -61,22 +61,43 @@ def recall_at_k(res, gt, k):
 
 
 def calibrate(index: Index, queries, gt, model: TermModel, L, k, target, seed=0,
-              thresholds=None, L_per_q=None, k_per_q=None):
+              thresholds=None, L_per_q=None, k_per_q=None, bisect=False):
     """Lowest logit threshold with recall@k >= target on (queries, gt).
 
-    Returns (threshold, recall, mean evaluations) or None if even the most
-    conservative threshold misses the target."""
+    Scans from the most conservative threshold down and stops at the first
+    miss; with ``bisect`` it binary-searches the sorted thresholds instead
+    (recall is taken as non-decreasing in the threshold: a higher threshold
+    only stops later).  Returns (threshold, recall, mean evaluations) or None
+    if even the most conservative threshold misses the target."""
     if thresholds is None:
         thresholds = np.arange(-6.0, 8.01, 0.5)
-    best = None
+    ts = sorted(float(t) for t in thresholds)
     saved = model.logit_threshold
-    for t in sorted(thresholds, reverse=True):  # conservative -> aggressive
-        model.logit_threshold = float(t)
+
+    def trial(t):
+        model.logit_threshold = t
         ids, _, ev, hp, st, *_ = index.search_ext(queries, L, k, seed=seed, model=model,
                                                   L_per_q=L_per_q, k_per_q=k_per_q)
         r = recall_at_k(ids, gt, min(k, gt.shape[1])) if L_per_q is None else None
-        if r is None or r < target:
-            break
-        best = (float(t), r, float(ev.mean()))
+        return (t, r, float(ev.mean())) if r is not None and r >= target else None
+
+    best = None
+    if bisect:
+        lo, hi = -1, len(ts) - 1  # ts[hi] is the candidate; everything <= lo misses
+        best = trial(ts[hi])
+        if best is not None:
+            while hi - lo > 1:
+                mid = (lo + hi) // 2
+                got = trial(ts[mid])
+                if got is not None:
+                    hi, best = mid, got
+                else:
+                    lo = mid
+    else:
+        for t in reversed(ts):  # conservative -> aggressive
+            got = trial(t)
+            if got is None:
+                break
+            best = got
     model.logit_threshold = saved if best is None else best[0]
     return best
