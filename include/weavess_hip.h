@@ -134,7 +134,8 @@ typedef struct {
 typedef struct {
     const wv_term_model *model;  /* learned termination (NULL: plain fixed-L)      */
     float *trace;                /* Q x trace_cap x 8 features g after each hop,
-                                    or NULL (same memory space as the outputs)      */
+                                    or NULL (same memory space as the outputs;
+                                    16-byte aligned)                                 */
     uint32_t *trace_len;         /* Q: rows written (min(hops, trace_cap))          */
     uint32_t *settle_hop;        /* Q: expansions after which the top-k never
                                     changed again (0: the entries' top-k is final)  */

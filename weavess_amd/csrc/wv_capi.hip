@@ -567,6 +567,8 @@ int wv_index_search_ext_dev(wv_index *ix, const float *d_queries, uint32_t Q, co
     if (!d_queries || !d_out_ids) return fail(c, WV_EINVAL, "queries/out_ids are null");
     if (ext && ext->trace && (!ext->trace_len || ext->trace_cap == 0))
         return fail(c, WV_EINVAL, "trace needs trace_len and trace_cap > 0");
+    if (ext && ext->trace && (reinterpret_cast<uintptr_t>(ext->trace) & 15))
+        return fail(c, WV_EINVAL, "trace must be 16-byte aligned");
     WV_HIP(c, hipSetDevice(c->device));
     std::vector<uint32_t> hl, hk;
     if (d_L_per_q) {

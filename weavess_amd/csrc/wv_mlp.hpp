@@ -57,12 +57,4 @@ __device__ inline float term_logit(const wv_term_model *m, const float f[MLP_IN]
     return __shfl(part, 0) + m->b2;
 }
 
-// Value of feature `i` from the register array without dynamic indexing.
-__device__ inline float pick8(const float f[MLP_IN], int i) {
-    float v = f[0];
-#pragma unroll
-    for (int j = 1; j < MLP_IN; ++j) v = (i == j) ? f[j] : v;
-    return v;
-}
-
 }  // namespace wv

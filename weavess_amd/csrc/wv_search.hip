@@ -286,8 +286,13 @@ __global__ void __launch_bounds__(256) greedy_search_kernel(SearchArgs a) {
             r2 = r1;
             r1 = r0;
             r0 = key_dist(kth);
-            if (a.trace && hops <= a.trace_cap && lane < MLP_IN)
-                a.trace[((size_t)q * a.trace_cap + (hops - 1)) * MLP_IN + lane] = pick8(f, lane);
+            // one lane stores the row as two float4s: a lane-indexed pick
+            // would put f in scratch memory for the whole loop
+            if (a.trace && hops <= a.trace_cap && lane == 0) {
+                float4 *t = reinterpret_cast<float4 *>(a.trace + ((size_t)q * a.trace_cap + (hops - 1)) * MLP_IN);
+                t[0] = make_float4(f[0], f[1], f[2], f[3]);
+                t[1] = make_float4(f[4], f[5], f[6], f[7]);
+            }
             if (!done && a.model && hops >= a.model->min_hops && term_logit(a.model, f) > a.model->logit_threshold)
                 break;
         }
