@@ -23,6 +23,17 @@
  * wvo_exact_knn                 algo/knn/gen_knn.cpp:117-150 (self skipped properly)  numpy argsort brute force on small inputs
  * wvo_recall                    src/builder.cpp:465-480                               hand-computed examples
  * wvo_graph_quality             src/builder.cpp:760-776                               hand-computed examples
+ *
+ * Learned early termination has no reference counterpart (ml/README.md only
+ * links papers); these restate the device's formulas and are pinned as follows:
+ * wvo_term_features             weavess_hip.h wv_term_model feature list              hand-worked vector; zero-distance clamps
+ * wvo_search_greedy_ext         wvo_search_greedy + features / settle / stop          plain-Python sequential search on integer
+ *                                                                                     data (features, settle, results); always-stop
+ *                                                                                     model stops after one expansion
+ * wvo_mlp_grad                  FP64 backprop of weighted BCE on the logit            closed form (zero weights); central finite
+ *                                                                                     differences of its own loss
+ * wvo_mlp_logit                 FP64 forward; BF16-product order of the device        closed form (one ReLU unit, BF16-exact
+ *                                                                                     inputs); BF16 within 5 % of FP64
  */
 #include "wv_oracle.h"
 
