@@ -32,6 +32,10 @@ Prints ONE JSON line on stdout; progress goes to stderr.
 
 --workload c2 / c1: plain fixed-L search only, on the 1M x 128 (K=64,
 L=R=64 graph) or the 10k x 128 integer mixture (K=50, L=70, S=10, R=100).
+--workload c3: NN-Descent precision on 1M x 960 (FP32 / BF16 / FP8 joins):
+graph recall and distance error against FP64.
+--workload c4: one batched launch over 200,000 queries with per-query
+L ~ U{16..400} and k in {1, 10, 100} on the 2M x 96 graph.
 """
 from __future__ import annotations
 
@@ -117,12 +121,170 @@ def nndescent_leg(ctx, Index, datagen):
     return out
 
 
+def bf16_round(x):
+    """float32 -> bfloat16 -> float32, round to nearest even (the device's
+    conversion in the join's row preparation)."""
+    b = np.ascontiguousarray(x, np.float32).view(np.uint32).astype(np.uint64)
+    b = (b + 0x7FFF + ((b >> 16) & 1)) & 0xFFFF0000
+    return b.astype(np.uint32).view(np.float32)
+
+
+def e4m3_round(x):
+    """Round to OCP FP8 E4M3 (3 mantissa bits, exponent bias 7, subnormal
+    step 2^-9), nearest even; |x| <= 448 already (the join clamps)."""
+    x = np.asarray(x, np.float64)
+    ax = np.abs(x)
+    e = np.floor(np.log2(np.maximum(ax, 2.0 ** -6)))
+    step = 2.0 ** (e - 3)
+    return np.sign(x) * np.round(ax / step) * step
+
+
+def join_distance_error(base, ids, deg, dtype, sample, seed=0):
+    """Max / mean relative error of the join's distance formula
+    |a|^2 + |b|^2 - 2 a.b, in FP32 arithmetic on quantised rows (BF16, or FP8
+    with per-row absmax/448 scales; FP32 rows for f32), against FP64 on the
+    original rows,
+    over the graph edges of `sample` random nodes (the near pairs, where the
+    formula cancels most)."""
+    g = np.random.default_rng(seed)
+    nodes = g.choice(len(base), sample, replace=False)
+    a_idx = np.repeat(nodes, [int(deg[n]) for n in nodes])
+    b_idx = np.concatenate([ids[n, :deg[n]] for n in nodes]).astype(np.int64)
+    A, B = base[a_idx].astype(np.float64), base[b_idx].astype(np.float64)
+    exact = ((A - B) ** 2).sum(1)
+
+    def quant(X):
+        if dtype == "bf16":
+            return bf16_round(X).astype(np.float64)
+        if dtype == "fp8":
+            sc = np.abs(X).max(1, keepdims=True) / 448.0
+            sc[sc == 0] = 1.0
+            return e4m3_round(np.clip(X / sc, -448, 448)) * sc
+        return X.astype(np.float64)
+
+    # products of BF16 / FP8 values are exact in FP32; sums are FP32, as on MFMA
+    qa, qb = quant(base[a_idx]).astype(np.float32), quant(base[b_idx]).astype(np.float32)
+    na = (qa * qa).sum(1, dtype=np.float32)
+    nb = (qb * qb).sum(1, dtype=np.float32)
+    dot = (qa * qb).sum(1, dtype=np.float32)
+    approx = np.maximum((na + nb) - np.float32(2.0) * dot, np.float32(0.0)).astype(np.float64)
+    ok = exact > 0
+    rel = np.abs(approx[ok] - exact[ok]) / exact[ok]
+    return float(rel.max()), float(rel.mean()), int(ok.sum())
+
+
+def run_c3(args, ctx, Index, datagen):
+    """Precision: NN-Descent on 1M x 960 with FP32, BF16 and FP8 joins (FP32
+    rerank of each pool's top 2K), graph recall and distance error vs FP64."""
+    t0 = time.time()
+    base, _ = datagen.gmm_c3(n_query=1)
+    N, d = base.shape
+    log(f"data {N}x{d} in {time.time() - t0:.1f}s")
+    ix = Index(ctx, base)
+    ctl = 10_000
+    gt, _, gt_ms = ctx.exact_knn(base, base[:ctl], 10, self_offset=0)
+    log(f"exact FP64 10-NN of {ctl} control nodes: {gt_ms:.1f} ms")
+    K, L, S, R, iters = 64, 64, 16, 64, 12
+    per = {}
+    for dt in ("f32", "bf16", "fp8"):
+        st = ix.build_nndescent(K, L, S, R, iters, seed=1, dtype=dt, rerank_mult=2)
+        ids, dist, deg = ix.get_graph()
+        gr = graph_recall(ids[:ctl], deg[:ctl], gt)
+        jmax, jmean, npairs = join_distance_error(base, ids, deg, dt, 2000)
+        # stored (reranked FP32) graph distances vs FP64
+        sel = np.random.default_rng(1).choice(N, 2000, replace=False)
+        a_idx = np.repeat(sel, [int(deg[n]) for n in sel])
+        b_idx = np.concatenate([ids[n, :deg[n]] for n in sel]).astype(np.int64)
+        ex = ((base[a_idx].astype(np.float64) - base[b_idx].astype(np.float64)) ** 2).sum(1)
+        stored = np.concatenate([dist[n, :deg[n]] for n in sel]).astype(np.float64)
+        ok = ex > 0
+        srel = np.abs(stored[ok] - ex[ok]) / ex[ok]
+        per[dt] = dict(build_s=round(st["total_ms"] / 1e3, 4), join_ms=round(st["join_ms"], 1),
+                       graph_recall=round(gr, 4),
+                       join_distance_rel_err_max=jmax, join_distance_rel_err_mean=jmean,
+                       graph_distance_rel_err_max=float(srel.max()),
+                       graph_distance_rel_err_mean=float(srel.mean()), error_pairs=npairs)
+        log(f"c3 {dt}: {st['total_ms']:.1f} ms, graph recall {gr:.4f}, join distance rel err max {jmax:.3e} "
+            f"mean {jmean:.3e}; graph distances (FP32 rerank) max {srel.max():.3e}")
+    ix.close()
+    out = {"metric": "NN-Descent precision: graph recall and distance error vs FP64 per join dtype",
+           "value": per["bf16"]["graph_recall"], "unit": "graph recall@10 (BF16 join)", "n_gpus": 1,
+           "higher_is_better": True, "data": f"synthetic, seeded (gmm_c3), {N}x{d}",
+           "config": {"workload": "c3: 1M x 960 non-negative 500-cluster GMM; NN-Descent K=64 L=64 S=16 R=64 "
+                                  "iters=12; FP32 rerank of each pool's top 2K",
+                      "control_nodes": ctl,
+                      "error_sample": "graph edges of 2,000 random nodes; join formula emulated on the host "
+                                      "(BF16 RNE; FP8 E4M3 with per-row absmax/448 scale)"},
+           "per_dtype": per}
+    print(json.dumps(out), flush=True)
+
+
+def run_c4(args, ctx, Index, datagen):
+    """Batching: one launch over 200,000 queries with per-query L ~ U{16..400}
+    and k in {1, 10, 100} (L_eff = max(L, k)) on the 2M x 96 graph."""
+    from weavess_amd.devmem import DeviceArray, pad_rows, synchronize
+    t0 = time.time()
+    base, queries = datagen.gmm_c4(n_query=200_000)
+    N, d = base.shape
+    Q = len(queries)
+    Lq, kq = datagen.c4_query_budgets(Q)
+    log(f"data {N}x{d}, {Q} queries in {time.time() - t0:.1f}s")
+    ix = Index(ctx, base)
+    st = ix.build_nndescent(48, 128, 32, 128, 20, seed=1, dtype="bf16")
+    log(f"graph: {st['total_ms']:.1f} ms")
+    ns = 20_000
+    gt, _, _ = ctx.exact_knn(base, queries[:ns], 100)
+    kmax = 100
+    dq = DeviceArray.from_host(pad_rows(queries))
+    dL = DeviceArray.from_host(Lq)
+    dk = DeviceArray.from_host(kq)
+    d_ids = DeviceArray((Q, kmax), np.uint32)
+    d_ev = DeviceArray((Q,), np.uint32)
+    d_hp = DeviceArray((Q,), np.uint32)
+
+    def run():
+        return ix.search_dev(dq.ptr, Q, 0, 0, d_ids.ptr, seed=7, d_L=dL.ptr, d_k=dk.ptr, k_stride=kmax,
+                             d_evals=d_ev.ptr, d_hops=d_hp.ptr)
+
+    tuning, _ = ix.autotune_dev(dq.ptr, 8192, int(Lq.max()), 100, seed=7)
+    log(f"autotuned at L={int(Lq.max())}, k=100: {tuning}")
+    for _ in range(args.warmup):
+        run()
+    synchronize()
+    kern = []
+    t1 = time.perf_counter()
+    for _ in range(args.steps):
+        s_ = run()
+        kern.append(s_["kernel_ms"])
+    synchronize()
+    ms = (time.perf_counter() - t1) * 1e3 / args.steps
+    ids = d_ids.to_host()[:ns]
+    rec = {}
+    for kk in (1, 10, 100):
+        m = kq[:ns] == kk
+        hit = (ids[m][:, :kk, None] == gt[m][:, None, :kk]).any(axis=2)
+        rec[f"k={kk}"] = dict(queries=int(m.sum()), recall=round(float(hit.mean()), 4))
+    ev, hp = d_ev.to_host(), d_hp.to_host()
+    out = {"metric": "QPS, batched search with per-query L and k", "value": round(Q / (ms / 1e3), 1),
+           "unit": "QPS", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 3),
+           "higher_is_better": True, "dtype": "f32", "data": f"synthetic, seeded (gmm_c4), {N}x{d}, {Q} queries",
+           "config": {"workload": "c4: 2M x 96 unit-norm GMM; NN-Descent K=48 (L=128 S=32 R=128 iters=20, BF16); "
+                                  "per-query L ~ U{16..400}, k in {1,10,100}, L_eff = max(L, k); one launch",
+                      "tuning": tuning},
+           "kernel_ms": round(float(np.mean(kern)), 3), "recall_by_k": rec,
+           "dist_evals_per_query": round(float(ev.mean()), 1), "hops_per_query": round(float(hp.mean()), 1)}
+    print(json.dumps(out), flush=True)
+    for a in (dq, dL, dk, d_ids, d_ev, d_hp):
+        a.free()
+    ix.close()
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="c5", choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", default="c5", choices=sorted(WORKLOADS) + ["c3", "c4"])
     ap.add_argument("--target", type=float, default=0.95)
     ap.add_argument("--no-cpu", action="store_true", help="skip the host-core baseline")
     ap.add_argument("--no-nnd-leg", action="store_true", help="skip the 1M x 128 NN-Descent leg")
@@ -135,8 +297,12 @@ def main():
     from weavess_amd import Context, Index, datagen, learned
     from weavess_amd.devmem import DeviceArray, host_cores, pad_rows, synchronize
 
-    w = WORKLOADS[args.workload]
     ctx = Context(0)
+    if args.workload in ("c3", "c4"):
+        (run_c3 if args.workload == "c3" else run_c4)(args, ctx, Index, datagen)
+        ctx.close()
+        return
+    w = WORKLOADS[args.workload]
     nnd_leg = None
     if not args.no_nnd_leg and args.workload == "c5":
         nnd_leg = nndescent_leg(ctx, Index, datagen)

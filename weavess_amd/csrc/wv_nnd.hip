@@ -489,6 +489,43 @@ __device__ inline void stage_tile(const NndDev &a, uint8_t *tile, const uint32_t
     }
 }
 
+// The same tile in two steps: issue the global loads into registers (at most
+// 8 16-byte vectors per lane for klen <= KC), store them to LDS later.  Lets
+// the next column tile load while the current one's proposals are probed.
+constexpr int TILE_REGS = 8;
+template <int DT>
+__device__ inline void load_tile_regs(const NndDev &a, const uint32_t *list, int r0, int count, uint32_t klen,
+                                      uint4 regs[TILE_REGS]) {
+    constexpr int ES = JoinT<DT>::ES;
+    const int lane = lane_id();
+    const int vpr = (int)(klen * ES / 16), epv = 16 / ES;
+#pragma unroll
+    for (int it = 0; it < TILE_REGS; ++it) {
+        const int v = lane + it * WAVE;
+        regs[it] = make_uint4(0, 0, 0, 0);
+        if (v < 16 * vpr) {
+            const int r = v / vpr, cv = v - r * vpr;
+            const uint32_t e0 = (uint32_t)(cv * epv);
+            if (r0 + r < count && e0 < a.kdim)
+                regs[it] = *reinterpret_cast<const uint4 *>(a.xq + (size_t)list[r0 + r] * a.xq_stride + (size_t)e0 * ES);
+        }
+    }
+}
+template <int DT>
+__device__ inline void store_tile_regs(uint8_t *tile, uint32_t klen, const uint4 regs[TILE_REGS]) {
+    constexpr int ES = JoinT<DT>::ES;
+    const int lane = lane_id();
+    const int vpr = (int)(klen * ES / 16);
+#pragma unroll
+    for (int it = 0; it < TILE_REGS; ++it) {
+        const int v = lane + it * WAVE;
+        if (v < 16 * vpr) {
+            const int r = v / vpr, cv = v - r * vpr;
+            *reinterpret_cast<uint4 *>(tile + r * TILE_ST + cv * 16) = regs[it];
+        }
+    }
+}
+
 template <int DT>
 __device__ inline floatx4 tile_mfma(const uint8_t *A, const uint8_t *B, uint32_t klen, floatx4 acc) {
     const int lane = lane_id(), r = lane & 15, g = lane >> 4;
@@ -517,26 +554,17 @@ __device__ inline floatx4 tile_mfma(const uint8_t *A, const uint8_t *B, uint32_t
     return acc;
 }
 
-__device__ inline bool pooled(const NndDev &a, uint32_t t, uint32_t id) {
+// Is `id` in target t's pool?  Probing of t's pool-id hash from slot `i`,
+// whose entry `e` was already read.
+__device__ inline bool pooled_from(const NndDev &a, uint32_t t, uint32_t id, uint32_t i, uint32_t e) {
     const uint32_t *h = a.phash + (size_t)t * a.HP;
     const uint32_t mask = a.HP - 1;
-    uint32_t i = fmix32(id) & mask;
     for (;;) {
-        const uint32_t e = h[i];
         if (e == id) return true;
         if (e == ID_NONE) return false;
         i = (i + 1) & mask;
+        e = h[i];
     }
-}
-
-__device__ inline bool propose(const NndDev &a, uint32_t t, uint32_t id, float d, uint64_t thr) {
-    const uint64_t key = make_key(d, id);
-    if (key > thr) return false;
-    if (pooled(a, t, id)) return false;
-    unsigned long long *slot =
-        reinterpret_cast<unsigned long long *>(a.cand + (size_t)t * a.C + cand_slot(t, id, a.C - 1));
-    if (*slot > key) atomicMin(slot, (unsigned long long)key);
-    return true;
 }
 
 // join(), src/component_refine.cpp:127-143 / nhood::join,
@@ -586,27 +614,47 @@ __global__ void __launch_bounds__(256) nnd_join_kernel(NndDev a) {
     const int nrt = (int)((nn + 15) / 16), nct = (int)((nc + 15) / 16);
     const int g = lane >> 4, cl = lane & 15;
     uint32_t pairs = 0, props = 0;
+    const uint32_t klen1 = ((kdim + KSTEP - 1) / KSTEP) * KSTEP;  // one chunk when nkc == 1
+    uint4 nxt[TILE_REGS];
     for (int rt = 0; rt < nrt; ++rt) {
         if (nkc == 1) {
-            stage_tile<DT>(a, tA, col, rt * 16, (int)nn, 0, ((kdim + KSTEP - 1) / KSTEP) * KSTEP);
+            stage_tile<DT>(a, tA, col, rt * 16, (int)nn, 0, klen1);
+            load_tile_regs<DT>(a, col, rt * 16, (int)nc, klen1, nxt);  // column tile rt
         }
         for (int ct = rt; ct < nct; ++ct) {
             floatx4 acc = {0.f, 0.f, 0.f, 0.f};
-            for (uint32_t kc = 0; kc < nkc; ++kc) {
+            if (nkc == 1) {
+                store_tile_regs<DT>(tB, klen1, nxt);
+                wave_sync();
+                acc = tile_mfma<DT>(tA, tB, klen1, acc);
+                wave_sync();
+                // next column tile in flight during this tile's proposals
+                if (ct + 1 < nct) load_tile_regs<DT>(a, col, (ct + 1) * 16, (int)nc, klen1, nxt);
+            }
+            for (uint32_t kc = 0; nkc > 1 && kc < nkc; ++kc) {
                 const uint32_t k0 = kc * KC;
                 uint32_t klen = min((uint32_t)KC, kdim - k0);
                 klen = ((klen + KSTEP - 1) / KSTEP) * KSTEP;
-                if (nkc > 1) stage_tile<DT>(a, tA, col, rt * 16, (int)nn, k0, klen);
+                stage_tile<DT>(a, tA, col, rt * 16, (int)nn, k0, klen);
                 stage_tile<DT>(a, tB, col, ct * 16, (int)nc, k0, klen);
                 wave_sync();
                 acc = tile_mfma<DT>(tA, tB, klen, acc);
                 wave_sync();
             }
-            // epilogue: lane holds rows 16rt+4g+i, column 16ct+cl
+            // epilogue: lane holds rows 16rt+4g+i, column 16ct+cl.  Up to 8
+            // proposals (both directions of 4 pairs); those at or below the
+            // target's worst (LDS) probe the target's pool-id hash all at
+            // once, and the survivors compete for their slot by atomicMin.
             const uint32_t b = (uint32_t)(ct * 16 + cl);
+            uint32_t pt[8], pid[8], ph[8], pe[8];
+            uint64_t pk[8];
+            bool pv[8];
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
                 const uint32_t ar = (uint32_t)(rt * 16 + 4 * g + i);
+                pv[2 * i] = pv[2 * i + 1] = false;
+                pt[2 * i] = pt[2 * i + 1] = pid[2 * i] = pid[2 * i + 1] = 0;
+                pk[2 * i] = pk[2 * i + 1] = 0;
                 if (ar >= nn || b >= nc) continue;
                 if (b < nn && !(ar < b)) continue;  // new x new: i < j once
                 const uint32_t ia = col[ar], ib = col[b];
@@ -616,8 +664,28 @@ __global__ void __launch_bounds__(256) nnd_join_kernel(NndDev a) {
                 float d = (cnorm[ar] + cnorm[b]) - 2.0f * dot;
                 d = fmaxf(d, 0.0f);
                 ++pairs;
-                props += propose(a, ia, ib, d, cthr[ar]);
-                props += propose(a, ib, ia, d, cthr[b]);
+                pt[2 * i] = ia;
+                pid[2 * i] = ib;
+                pk[2 * i] = make_key(d, ib);
+                pv[2 * i] = pk[2 * i] <= cthr[ar];
+                pt[2 * i + 1] = ib;
+                pid[2 * i + 1] = ia;
+                pk[2 * i + 1] = make_key(d, ia);
+                pv[2 * i + 1] = pk[2 * i + 1] <= cthr[b];
+            }
+            const uint32_t hmask = a.HP - 1;
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                ph[j] = fmix32(pid[j]) & hmask;
+                pe[j] = pv[j] ? a.phash[(size_t)pt[j] * a.HP + ph[j]] : ID_NONE;
+            }
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                if (!pv[j] || pooled_from(a, pt[j], pid[j], ph[j], pe[j])) continue;
+                ++props;
+                atomicMin(reinterpret_cast<unsigned long long *>(a.cand + (size_t)pt[j] * a.C +
+                                                                 cand_slot(pt[j], pid[j], a.C - 1)),
+                          (unsigned long long)pk[j]);
             }
         }
     }
