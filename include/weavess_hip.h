@@ -63,7 +63,8 @@ typedef struct {
     uint32_t dtype;      /* WV_DTYPE_*: distance precision inside the local join    */
     uint32_t rerank_mult;/* low precision: the best rerank_mult*K of each pool are re-ranked with
                             exact FP32 distances at the end (0 -> 2)                */
-    uint32_t cand_slots; /* per-node candidate table, power of two (0 -> automatic)  */
+    uint32_t cand_slots; /* per-node candidate table, power of two in [64, 1024]
+                            (0 -> automatic)                                        */
     uint64_t seed;       /* random initialisation and reverse-sample choice          */
 } wv_nnd_params;
 

@@ -48,8 +48,14 @@ typedef float floatx4 __attribute__((ext_vector_type(4)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 
 constexpr int DT_F32 = 0, DT_BF16 = 1, DT_FP8 = 2;
+// Statistics counters are spread over slots picked by workgroup and summed on
+// the host: a million same-address atomics per launch serialise.
+constexpr uint32_t CTR_SLOTS = 1024;
+__device__ inline unsigned long long *ctr_slot(unsigned long long *ctr) {
+    return ctr + (size_t)(blockIdx.x & (CTR_SLOTS - 1)) * 4;
+}
 constexpr float FP8_MAX = 448.0f;  // largest finite e4m3 magnitude
-constexpr int TILE_ST = 528;       // LDS bytes per staged tile row (512 + 16 pad)
+constexpr int TILE_ST_MAX = 528;   // LDS bytes per staged tile row at most (512 + 16 pad)
 
 struct NndDev {
     const float *base;     // N x dp FP32 rows (zero padded)
@@ -74,11 +80,12 @@ struct NndDev {
     uint64_t *cand;
     uint32_t *phash;          // N x HP open-addressing table of the pool's ids
     uint32_t HP;
-    unsigned long long *ctr;  // [0] pairs [1] proposals [2] rows gathered [3] pool updates
+    unsigned long long *ctr;  // CTR_SLOTS x [0] pairs [1] proposals [2] rows gathered [3] pool updates
     uint32_t *out_ids;
     float *out_dist;
     uint32_t *out_deg;
     uint32_t lds_join_list;   // bytes of list area in the join's per-wave LDS
+    uint32_t tile_st;         // LDS bytes per staged tile row: the chunk's bytes + 16 pad
     uint32_t nw_pow2;         // power of two >= S + R
 };
 
@@ -469,7 +476,7 @@ template <> struct JoinT<DT_BF16> { static constexpr int ES = 2, KC = 256, KSTEP
 template <> struct JoinT<DT_FP8> { static constexpr int ES = 1, KC = 512, KSTEP = 32; };
 
 // Stage join rows list[r0 .. r0+16) (zero rows past `count`), elements
-// [k0, k0 + klen) with klen a multiple of KSTEP, into a 16 x TILE_ST tile.
+// [k0, k0 + klen) with klen a multiple of KSTEP, into a 16 x tile_st tile.
 template <int DT>
 __device__ inline void stage_tile(const NndDev &a, uint8_t *tile, const uint32_t *list, int r0,
                                   int count, uint32_t k0, uint32_t klen) {
@@ -485,54 +492,61 @@ __device__ inline void stage_tile(const NndDev &a, uint8_t *tile, const uint32_t
             const uint32_t id = list[r0 + r];
             val = *reinterpret_cast<const uint4 *>(a.xq + (size_t)id * a.xq_stride + (size_t)e0 * ES);
         }
-        *reinterpret_cast<uint4 *>(tile + r * TILE_ST + cv * 16) = val;
+        *reinterpret_cast<uint4 *>(tile + r * a.tile_st + cv * 16) = val;
     }
 }
 
-// The same tile in two steps: issue the global loads into registers (at most
-// 8 16-byte vectors per lane for klen <= KC), store them to LDS later.  Lets
-// the next column tile load while the current one's proposals are probed.
+// The same tile in two steps for one-chunk rows (klen <= KC): issue the
+// global loads into registers (at most 8 16-byte vectors per lane), store
+// them to LDS later.  Lets the next column tile load while the current one's
+// proposals are probed.  map[it] packs the row (low 8 bits) and the byte
+// column (<< 8) of this lane's vector it, or TILE_NONE; it depends only on
+// the lane and the row width, so it is computed once per wavefront.
 constexpr int TILE_REGS = 8;
+constexpr uint32_t TILE_NONE = 0xFFFFFFFFu;
 template <int DT>
-__device__ inline void load_tile_regs(const NndDev &a, const uint32_t *list, int r0, int count, uint32_t klen,
-                                      uint4 regs[TILE_REGS]) {
+__device__ inline void tile_map(uint32_t klen, uint32_t kdim, uint32_t map[TILE_REGS]) {
     constexpr int ES = JoinT<DT>::ES;
     const int lane = lane_id();
     const int vpr = (int)(klen * ES / 16), epv = 16 / ES;
 #pragma unroll
     for (int it = 0; it < TILE_REGS; ++it) {
         const int v = lane + it * WAVE;
-        regs[it] = make_uint4(0, 0, 0, 0);
-        if (v < 16 * vpr) {
-            const int r = v / vpr, cv = v - r * vpr;
-            const uint32_t e0 = (uint32_t)(cv * epv);
-            if (r0 + r < count && e0 < a.kdim)
-                regs[it] = *reinterpret_cast<const uint4 *>(a.xq + (size_t)list[r0 + r] * a.xq_stride + (size_t)e0 * ES);
-        }
+        const int r = v / vpr, cv = v - r * vpr;
+        map[it] = v < 16 * vpr ? (uint32_t)r | ((uint32_t)cv * 16u << 8) | ((uint32_t)(cv * epv) < kdim ? 0u : 0x80u)
+                               : TILE_NONE;
     }
 }
-template <int DT>
-__device__ inline void store_tile_regs(uint8_t *tile, uint32_t klen, const uint4 regs[TILE_REGS]) {
-    constexpr int ES = JoinT<DT>::ES;
-    const int lane = lane_id();
-    const int vpr = (int)(klen * ES / 16);
+// bit 7 of a map entry: the vector lies past the row's data (zero fill)
+__device__ inline void load_tile_regs(const NndDev &a, const uint32_t *list, int r0, int count,
+                                      const uint32_t map[TILE_REGS], uint4 regs[TILE_REGS]) {
 #pragma unroll
     for (int it = 0; it < TILE_REGS; ++it) {
-        const int v = lane + it * WAVE;
-        if (v < 16 * vpr) {
-            const int r = v / vpr, cv = v - r * vpr;
-            *reinterpret_cast<uint4 *>(tile + r * TILE_ST + cv * 16) = regs[it];
-        }
+        regs[it] = make_uint4(0, 0, 0, 0);
+        const uint32_t m = map[it];
+        if (m == TILE_NONE || (m & 0x80u)) continue;
+        const int r = (int)(m & 0x7Fu);
+        if (r0 + r < count)
+            regs[it] = *reinterpret_cast<const uint4 *>(a.xq + (size_t)list[r0 + r] * a.xq_stride + (m >> 8));
+    }
+}
+__device__ inline void store_tile_regs(uint8_t *tile, uint32_t st, const uint32_t map[TILE_REGS],
+                                       const uint4 regs[TILE_REGS]) {
+#pragma unroll
+    for (int it = 0; it < TILE_REGS; ++it) {
+        const uint32_t m = map[it];
+        if (m == TILE_NONE) continue;
+        *reinterpret_cast<uint4 *>(tile + (m & 0x7Fu) * st + (m >> 8)) = regs[it];
     }
 }
 
 template <int DT>
-__device__ inline floatx4 tile_mfma(const uint8_t *A, const uint8_t *B, uint32_t klen, floatx4 acc) {
+__device__ inline floatx4 tile_mfma(const uint8_t *A, const uint8_t *B, uint32_t st, uint32_t klen, floatx4 acc) {
     const int lane = lane_id(), r = lane & 15, g = lane >> 4;
     if (DT == DT_F32) {
         for (uint32_t t = 0; t < klen / 16; ++t) {
-            const float4 x = *reinterpret_cast<const float4 *>(A + r * TILE_ST + (16 * t + 4 * g) * 4);
-            const float4 y = *reinterpret_cast<const float4 *>(B + r * TILE_ST + (16 * t + 4 * g) * 4);
+            const float4 x = *reinterpret_cast<const float4 *>(A + r * st + (16 * t + 4 * g) * 4);
+            const float4 y = *reinterpret_cast<const float4 *>(B + r * st + (16 * t + 4 * g) * 4);
             acc = __builtin_amdgcn_mfma_f32_16x16x4f32(x.x, y.x, acc, 0, 0, 0);
             acc = __builtin_amdgcn_mfma_f32_16x16x4f32(x.y, y.y, acc, 0, 0, 0);
             acc = __builtin_amdgcn_mfma_f32_16x16x4f32(x.z, y.z, acc, 0, 0, 0);
@@ -540,14 +554,14 @@ __device__ inline floatx4 tile_mfma(const uint8_t *A, const uint8_t *B, uint32_t
         }
     } else if (DT == DT_BF16) {
         for (uint32_t t = 0; t < klen / 32; ++t) {
-            const bf16x8 x = *reinterpret_cast<const bf16x8 *>(A + r * TILE_ST + (32 * t + 8 * g) * 2);
-            const bf16x8 y = *reinterpret_cast<const bf16x8 *>(B + r * TILE_ST + (32 * t + 8 * g) * 2);
+            const bf16x8 x = *reinterpret_cast<const bf16x8 *>(A + r * st + (32 * t + 8 * g) * 2);
+            const bf16x8 y = *reinterpret_cast<const bf16x8 *>(B + r * st + (32 * t + 8 * g) * 2);
             acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(x, y, acc, 0, 0, 0);
         }
     } else {
         for (uint32_t t = 0; t < klen / 32; ++t) {
-            const long x = *reinterpret_cast<const long *>(A + r * TILE_ST + 32 * t + 8 * g);
-            const long y = *reinterpret_cast<const long *>(B + r * TILE_ST + 32 * t + 8 * g);
+            const long x = *reinterpret_cast<const long *>(A + r * st + 32 * t + 8 * g);
+            const long y = *reinterpret_cast<const long *>(B + r * st + 32 * t + 8 * g);
             acc = __builtin_amdgcn_mfma_f32_16x16x32_fp8_fp8(x, y, acc, 0, 0, 0);
         }
     }
@@ -577,14 +591,14 @@ __global__ void __launch_bounds__(256) nnd_join_kernel(NndDev a) {
     const uint32_t n = blockIdx.x * (blockDim.x >> 6) + wid;
     if (n >= a.N) return;
     const uint32_t ncap = a.S + 3 * a.R;  // |new| <= S+R, |old| <= 2R
-    uint8_t *my = smem + (size_t)wid * (a.lds_join_list + 2 * 16 * TILE_ST);
+    uint8_t *my = smem + (size_t)wid * (a.lds_join_list + 2 * 16 * a.tile_st);
     uint32_t *nw = reinterpret_cast<uint32_t *>(my);              // nw_pow2
     uint32_t *col = nw + a.nw_pow2;                               // ncap
     uint64_t *cthr = reinterpret_cast<uint64_t *>(col + ((ncap + 1) & ~1u));
     float *cnorm = reinterpret_cast<float *>(cthr + ncap);
     float *cscale = cnorm + ncap;
     uint8_t *tA = my + a.lds_join_list;
-    uint8_t *tB = tA + 16 * TILE_ST;
+    uint8_t *tB = tA + 16 * a.tile_st;
 
     const uint32_t fnc = a.fnew_cnt[n], rnc = a.rsel_new_cnt[n];
     const uint32_t foc = a.fold_cnt[n], roc = a.rsel_old_cnt[n];
@@ -615,21 +629,24 @@ __global__ void __launch_bounds__(256) nnd_join_kernel(NndDev a) {
     const int g = lane >> 4, cl = lane & 15;
     uint32_t pairs = 0, props = 0;
     const uint32_t klen1 = ((kdim + KSTEP - 1) / KSTEP) * KSTEP;  // one chunk when nkc == 1
+    uint32_t map[TILE_REGS];
+    tile_map<DT>(klen1, kdim, map);
     uint4 nxt[TILE_REGS];
     for (int rt = 0; rt < nrt; ++rt) {
         if (nkc == 1) {
-            stage_tile<DT>(a, tA, col, rt * 16, (int)nn, 0, klen1);
-            load_tile_regs<DT>(a, col, rt * 16, (int)nc, klen1, nxt);  // column tile rt
+            load_tile_regs(a, col, rt * 16, (int)nn, map, nxt);  // row tile rt
+            store_tile_regs(tA, a.tile_st, map, nxt);
+            load_tile_regs(a, col, rt * 16, (int)nc, map, nxt);  // column tile rt
         }
         for (int ct = rt; ct < nct; ++ct) {
             floatx4 acc = {0.f, 0.f, 0.f, 0.f};
             if (nkc == 1) {
-                store_tile_regs<DT>(tB, klen1, nxt);
+                store_tile_regs(tB, a.tile_st, map, nxt);
                 wave_sync();
-                acc = tile_mfma<DT>(tA, tB, klen1, acc);
+                acc = tile_mfma<DT>(tA, tB, a.tile_st, klen1, acc);
                 wave_sync();
                 // next column tile in flight during this tile's proposals
-                if (ct + 1 < nct) load_tile_regs<DT>(a, col, (ct + 1) * 16, (int)nc, klen1, nxt);
+                if (ct + 1 < nct) load_tile_regs(a, col, (ct + 1) * 16, (int)nc, map, nxt);
             }
             for (uint32_t kc = 0; nkc > 1 && kc < nkc; ++kc) {
                 const uint32_t k0 = kc * KC;
@@ -638,7 +655,7 @@ __global__ void __launch_bounds__(256) nnd_join_kernel(NndDev a) {
                 stage_tile<DT>(a, tA, col, rt * 16, (int)nn, k0, klen);
                 stage_tile<DT>(a, tB, col, ct * 16, (int)nc, k0, klen);
                 wave_sync();
-                acc = tile_mfma<DT>(tA, tB, klen, acc);
+                acc = tile_mfma<DT>(tA, tB, a.tile_st, klen, acc);
                 wave_sync();
             }
             // epilogue: lane holds rows 16rt+4g+i, column 16ct+cl.  Up to 8
@@ -692,9 +709,10 @@ __global__ void __launch_bounds__(256) nnd_join_kernel(NndDev a) {
     pairs = wave_sum_u32(pairs);
     props = wave_sum_u32(props);
     if (lane == 0) {
-        atomicAdd(&a.ctr[0], (unsigned long long)pairs);
-        atomicAdd(&a.ctr[1], (unsigned long long)props);
-        atomicAdd(&a.ctr[2], (unsigned long long)nc);
+        unsigned long long *c = ctr_slot(a.ctr);
+        atomicAdd(&c[0], (unsigned long long)pairs);
+        atomicAdd(&c[1], (unsigned long long)props);
+        atomicAdd(&c[2], (unsigned long long)nc);
     }
 }
 
@@ -723,63 +741,105 @@ __global__ void __launch_bounds__(256) nnd_hash_kernel(NndDev a) {
 
 // ------------------------------------------------------------------ merge
 __host__ __device__ inline size_t merge_wave_lds(uint32_t up2, uint32_t hp2, uint32_t L) {
-    return ((size_t)up2 * 8 + (size_t)hp2 * 5 + L + 16 + 15) & ~(size_t)15;
+    return ((size_t)up2 * 8 + (size_t)L * 8 + (size_t)hp2 * 5 + L + 16 + 15) & ~(size_t)15;
 }
 // Fold each node's slot winners into its pool (index.h:120-135 applied to
-// the whole batch at once; see header comment), then compute the next M.
-__global__ void __launch_bounds__(256) nnd_merge_kernel(NndDev a, uint32_t up2, uint32_t hp2) {
+// the whole batch at once; see header comment), then compute the next M and,
+// when `rehash`, rebuild the node's pool-id hash for the next join.  All of
+// the node's global loads (C slots, L pool keys and flags; C, L <= 1024, so
+// at most 16 per lane each) are issued before any of them is used.
+constexpr int MERGE_REGS = 16;
+__global__ void __launch_bounds__(256) nnd_merge_kernel(NndDev a, uint32_t up2, uint32_t hp2, int rehash) {
     extern __shared__ __align__(16) uint8_t smem[];
     const int wid = threadIdx.x >> 6, lane = lane_id();
     const uint32_t n = blockIdx.x * (blockDim.x >> 6) + wid;
     if (n >= a.N) return;
     uint8_t *my = smem + (size_t)wid * merge_wave_lds(up2, hp2, a.L);
-    uint64_t *u = reinterpret_cast<uint64_t *>(my);
-    uint32_t *hk = reinterpret_cast<uint32_t *>(u + up2);
+    uint64_t *u = reinterpret_cast<uint64_t *>(my);   // pool keys, then the winners
+    uint64_t *w = u + up2;                             // the merged, truncated pool
+    uint32_t *hk = reinterpret_cast<uint32_t *>(w + a.L);
     uint8_t *hf = reinterpret_cast<uint8_t *>(hk + hp2);
     uint8_t *nf = hf + hp2;
-    uint32_t *ucnt = reinterpret_cast<uint32_t *>(nf + ((a.L + 3) & ~3u));
     const uint32_t hmask = hp2 - 1;
     const uint32_t cnt = a.pcnt[n];
     const uint64_t *pr = a.pool + (size_t)n * a.L;
     const uint8_t *pf = a.pflag + (size_t)n * a.L;
-    for (uint32_t i = lane; i < hp2; i += WAVE) hk[i] = ID_NONE;
-    if (lane == 0) *ucnt = 0;
-    wave_sync();
-    for (uint32_t i = lane; i < cnt; i += WAVE) {
-        const uint64_t k = pr[i];
-        u[i] = k;
-        uint32_t h = fmix32(key_id(k)) & hmask;
-        while (atomicCAS(&hk[h], ID_NONE, key_id(k)) != ID_NONE) h = (h + 1) & hmask;
-        hf[h] = pf[i];
-    }
-    wave_sync();
     uint64_t *cs = a.cand + (size_t)n * a.C;
-    for (uint32_t s = lane; s < a.C; s += WAVE) {
-        const uint64_t k = cs[s];
-        if (k == KEY_EMPTY) continue;
-        cs[s] = KEY_EMPTY;
-        const uint32_t id = key_id(k);
-        uint32_t h = fmix32(id) & hmask;
-        bool dup = false;
-        for (;;) {
-            const uint32_t e = hk[h];
-            if (e == id) { dup = true; break; }
-            if (e == ID_NONE) break;
-            h = (h + 1) & hmask;
-        }
-        if (!dup) u[cnt + atomicAdd(ucnt, 1u)] = k;
+    const uint32_t cper = a.C / WAVE;
+    uint64_t ck[MERGE_REGS], pk[MERGE_REGS];
+    uint8_t pfl[MERGE_REGS];
+#pragma unroll
+    for (int r = 0; r < MERGE_REGS; ++r) {
+        const uint32_t i = lane + r * WAVE;
+        ck[r] = (uint32_t)r < cper ? cs[i] : KEY_EMPTY;
+        pk[r] = i < cnt ? pr[i] : KEY_EMPTY;
+        pfl[r] = i < cnt ? pf[i] : (uint8_t)0;
+    }
+    for (uint32_t i = lane; i < hp2; i += WAVE) hk[i] = ID_NONE;
+    wave_sync();
+#pragma unroll
+    for (int r = 0; r < MERGE_REGS; ++r) {
+        const uint32_t i = lane + r * WAVE;
+        if (i >= cnt) continue;
+        u[i] = pk[r];
+        uint32_t h = fmix32(key_id(pk[r])) & hmask;
+        while (atomicCAS(&hk[h], ID_NONE, key_id(pk[r])) != ID_NONE) h = (h + 1) & hmask;
+        hf[h] = pfl[r];
     }
     wave_sync();
-    const uint32_t total = cnt + *ucnt;
-    uint32_t P = 64;
-    while (P < total) P <<= 1;
-    for (uint32_t i = total + lane; i < P; i += WAVE) u[i] = KEY_EMPTY;
+    // reset every slot for the next join with whole-line stores (resetting
+    // only the filled ones leaves scattered partial-line writes)
+#pragma unroll
+    for (int r = 0; r < MERGE_REGS; ++r)
+        if ((uint32_t)r < cper) cs[lane + r * WAVE] = KEY_EMPTY;
+    // winners not already pooled, compacted behind the pool keys by ballot
+    // (slot hashing gives each id one slot, so they are distinct)
+    uint32_t nw = 0;
+#pragma unroll
+    for (int r = 0; r < MERGE_REGS; ++r) {
+        if ((uint32_t)r >= cper) break;
+        const uint64_t k = ck[r];
+        bool take = false;
+        if (k != KEY_EMPTY) {
+            const uint32_t id = key_id(k);
+            uint32_t h = fmix32(id) & hmask;
+            take = true;
+            for (;;) {
+                const uint32_t e = hk[h];
+                if (e == id) { take = false; break; }
+                if (e == ID_NONE) break;
+                h = (h + 1) & hmask;
+            }
+        }
+        const uint64_t bm = __ballot(take);
+        if (take) u[cnt + nw + lanes_below(bm)] = k;
+        nw += (uint32_t)__popcll(bm);
+    }
+    // sort the winners only (the pool part is sorted), then merge the two runs
+    // by rank: a key's slot is its index plus the other run's keys below it
+    if (nw > 0) {
+        uint32_t pw = 64;
+        while (pw < nw) pw <<= 1;  // <= C, so cnt + pw <= L + C <= up2
+        wave_sync();
+        for (uint32_t i = nw + lane; i < pw; i += WAVE) u[cnt + i] = KEY_EMPTY;
+        wave_sync();
+        lds_sort64(u + cnt, (int)pw);
+    }
     wave_sync();
-    lds_sort64(u, (int)P);
-    const uint32_t keep = min(total, a.L);
+    const uint64_t *bw = u + cnt;
+    for (uint32_t i = lane; i < cnt; i += WAVE) {
+        const uint32_t p = i + (uint32_t)lower_bound64(bw, (int)nw, u[i]);
+        if (p < a.L) w[p] = u[i];
+    }
+    for (uint32_t j = lane; j < nw; j += WAVE) {
+        const uint32_t p = j + (uint32_t)lower_bound64(u, (int)cnt, bw[j]);
+        if (p < a.L) w[p] = bw[j];
+    }
+    wave_sync();
+    const uint32_t keep = min(cnt + nw, a.L);
     uint32_t fresh = 0;
     for (uint32_t i = lane; i < a.L; i += WAVE) {
-        const uint64_t k = i < keep ? u[i] : KEY_EMPTY;
+        const uint64_t k = i < keep ? w[i] : KEY_EMPTY;
         uint8_t f = 0;
         if (i < keep) {
             const uint32_t id = key_id(k);
@@ -801,10 +861,23 @@ __global__ void __launch_bounds__(256) nnd_merge_kernel(NndDev a, uint32_t up2, 
     fresh = wave_sum_u32(fresh);
     if (lane == 0) {
         a.pcnt[n] = keep;
-        a.thr[n] = keep ? u[keep - 1] : KEY_EMPTY;
-        a.worst[n] = keep ? key_dist(u[keep - 1]) : -1.0f;
+        a.thr[n] = keep ? w[keep - 1] : KEY_EMPTY;
+        a.worst[n] = keep ? key_dist(w[keep - 1]) : -1.0f;
         a.M[n] = m;
-        atomicAdd(&a.ctr[3], (unsigned long long)fresh);
+        atomicAdd(&ctr_slot(a.ctr)[3], (unsigned long long)fresh);
+    }
+    if (rehash) {
+        // the new pool's id hash (hp2 == HP: load factor <= 1/2)
+        for (uint32_t i = lane; i < hp2; i += WAVE) hk[i] = ID_NONE;
+        wave_sync();
+        for (uint32_t i = lane; i < keep; i += WAVE) {
+            const uint32_t id = key_id(w[i]);
+            uint32_t h = fmix32(id) & hmask;
+            while (atomicCAS(&hk[h], ID_NONE, id) != ID_NONE) h = (h + 1) & hmask;
+        }
+        wave_sync();
+        uint32_t *dst = a.phash + (size_t)n * a.HP;
+        for (uint32_t i = lane; i < hp2; i += WAVE) dst[i] = hk[i];
     }
 }
 
@@ -927,7 +1000,8 @@ int nndescent_build(wv_index *ix, const wv_nnd_params *p, const uint32_t *init_i
     const int cpl = nch <= 16 ? 1 : nch <= 32 ? 2 : nch <= 64 ? 4 : nch <= 128 ? 8 : nch <= 256 ? 16 : 0;
     if (!cpl) return index_set_error(ix, WV_ENOTSUP, "dimension above 1024");
     uint32_t C = p->cand_slots ? p->cand_slots : std::min<uint32_t>(1024, pow2_at_least(4 * L, 64));
-    if (C & (C - 1)) return index_set_error(ix, WV_EINVAL, "cand_slots must be a power of two");
+    if ((C & (C - 1)) || C < 64 || C > 1024)
+        return index_set_error(ix, WV_EINVAL, "cand_slots must be a power of two in [64, 1024]");
     const uint32_t rer = std::min(L, std::max(K, (p->rerank_mult ? p->rerank_mult : (dt ? 2u : 1u)) * K));
 
     Bufs B;
@@ -979,7 +1053,7 @@ int nndescent_build(wv_index *ix, const wv_nnd_params *p, const uint32_t *init_i
     a.HP = pow2_at_least(2 * L, 64);
     NND_HIP(B.get(&a.phash, (size_t)N * a.HP));
     const size_t hash_lds = (size_t)WPB * a.HP * 4;
-    NND_HIP(B.get(&a.ctr, 4));
+    NND_HIP(B.get(&a.ctr, (size_t)CTR_SLOTS * 4));
     uint32_t *bsum = nullptr;
     NND_HIP(B.get(&bsum, (N + SCAN_B - 1) / SCAN_B + 1));
     a.out_ids = graph_out;
@@ -989,7 +1063,15 @@ int nndescent_build(wv_index *ix, const wv_nnd_params *p, const uint32_t *init_i
     const uint32_t ncap = S + 3 * R;
     a.nw_pow2 = pow2_at_least(S + R, 64);
     a.lds_join_list = (uint32_t)(((a.nw_pow2 * 4 + ((ncap + 1) & ~1u) * 4 + ncap * 16) + 15) & ~15u);
-    const size_t join_lds = (size_t)JOIN_WPB * (a.lds_join_list + 2 * 16 * TILE_ST);
+    {
+        // tile rows as wide as one K chunk of this row length: BF16 / FP8 rows
+        // of d <= 256 / 512 need a fraction of the 512-byte maximum
+        const uint32_t es = dt == DT_F32 ? 4 : (dt == DT_BF16 ? 2 : 1);
+        const uint32_t kc = 512 / es, kstep = dt == DT_F32 ? 16 : 32;
+        const uint32_t klen = a.kdim <= kc ? (a.kdim + kstep - 1) / kstep * kstep : kc;
+        a.tile_st = klen * es + 16;
+    }
+    const size_t join_lds = (size_t)JOIN_WPB * (a.lds_join_list + 2 * 16 * a.tile_st);
     if (join_lds > 160 * 1024)
         return index_set_error(ix, WV_ENOTSUP, "S/R too large for the join's LDS lists");
     const uint32_t up2 = pow2_at_least(L + C, 64), hp2 = pow2_at_least(2 * L, 64);
@@ -1012,7 +1094,8 @@ int nndescent_build(wv_index *ix, const wv_nnd_params *p, const uint32_t *init_i
 
     NND_HIP(hipEventRecord(ev[0], s));
     hipLaunchKernelGGL(fill_u64_kernel, dim3(2048), dim3(256), 0, s, a.cand, KEY_EMPTY, (size_t)N * C);
-    hipLaunchKernelGGL(fill_u64_kernel, dim3(1), dim3(64), 0, s, reinterpret_cast<uint64_t *>(a.ctr), 0ull, (size_t)4);
+    hipLaunchKernelGGL(fill_u64_kernel, dim3(16), dim3(256), 0, s, reinterpret_cast<uint64_t *>(a.ctr), 0ull,
+                       (size_t)CTR_SLOTS * 4);
     {
         const uint32_t pb = (N * 16 + 255) / 256;
         if (dt == DT_F32)
@@ -1064,12 +1147,9 @@ int nndescent_build(wv_index *ix, const wv_nnd_params *p, const uint32_t *init_i
             else hipLaunchKernelGGL(nnd_join_kernel<DT_FP8>, dim3(jb), dim3(64 * JOIN_WPB), join_lds, s, a);
             NND_HIP(hipGetLastError());
             NND_HIP(hipEventRecord(ev[3], s));
-            hipLaunchKernelGGL(nnd_merge_kernel, dim3(blocks), dim3(64 * WPB), merge_lds, s, a, up2, hp2);
+            hipLaunchKernelGGL(nnd_merge_kernel, dim3(blocks), dim3(64 * WPB), merge_lds, s, a, up2, hp2,
+                               (int)(it + 1 < p->iters));
             NND_HIP(hipGetLastError());
-            if (it + 1 < p->iters) {
-                hipLaunchKernelGGL(nnd_hash_kernel, dim3(blocks), dim3(64 * WPB), hash_lds, s, a);
-                NND_HIP(hipGetLastError());
-            }
             NND_HIP(hipEventRecord(ev[4], s));
         } else {
             hipLaunchKernelGGL(nnd_m0_kernel, dim3(blocks), dim3(64 * WPB), 0, s, a);
@@ -1115,8 +1195,11 @@ int nndescent_build(wv_index *ix, const wv_nnd_params *p, const uint32_t *init_i
     NND_HIP(hipStreamSynchronize(s));
     if (stats) {
         float ms = 0.f;
-        unsigned long long c[4];
-        NND_HIP(hipMemcpy(c, a.ctr, sizeof c, hipMemcpyDeviceToHost));
+        std::vector<unsigned long long> cs((size_t)CTR_SLOTS * 4);
+        NND_HIP(hipMemcpy(cs.data(), a.ctr, cs.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+        unsigned long long c[4] = {0, 0, 0, 0};
+        for (uint32_t i = 0; i < CTR_SLOTS; ++i)
+            for (int j = 0; j < 4; ++j) c[j] += cs[(size_t)i * 4 + j];
         NND_HIP(hipEventElapsedTime(&ms, ev[2], ev[7]));
         stats->finalize_ms = ms;
         NND_HIP(hipEventElapsedTime(&ms, ev[0], ev[7]));
