@@ -224,6 +224,23 @@ int wv_index_autotune(wv_index *idx, const float *d_queries, uint32_t Q, uint64_
                       uint32_t k, const wv_search_ext *ext, wv_search_config *best,
                       double *best_ms);
 
+/* hipGraph-captured search for serving loops: the launch of
+ * wv_index_search_ext_dev (random entries, with the stop model of `ext` when
+ * given; no traces) planned, checked and captured once with the index's
+ * current tuning, then replayed on the context's stream with the same device
+ * buffers.  The caller may rewrite the query rows (and valid per-query L/k,
+ * within the L and k_stride checked here) between launches; the kernel
+ * clamps per-query L and k to what was planned.  Creation runs the search
+ * once eagerly.  launch: sync != 0 waits for the replay.  No counterpart in
+ * the reference (a serving-loop convenience). */
+typedef struct wv_search_graph wv_search_graph;
+int wv_search_graph_create(wv_index *idx, const float *d_queries, uint32_t Q, uint64_t seed, uint32_t L,
+                           uint32_t k, const uint32_t *d_L_per_q, const uint32_t *d_k_per_q,
+                           uint32_t k_stride, uint32_t *d_out_ids, float *d_out_dist, uint32_t *d_evals,
+                           uint32_t *d_hops, const wv_search_ext *ext, wv_search_graph **out);
+int wv_search_graph_launch(wv_search_graph *g, int sync);
+int wv_search_graph_destroy(wv_search_graph *g);
+
 /* ---------------------------------------------------------------- MLP */
 /* X: n x 8 raw features g (standardised with the model's mean/inv_std
  * inside), y: n labels in {0,1}.  Host arrays. */

@@ -212,3 +212,70 @@ def test_search_rejects_bad_arguments(ctx, small_float, small_float_graph):
     assert e.value.code == -22
     with pytest.raises(WvError):
         _gpu_search(ctx, g, deg, base, q, len(base), 10)  # L >= N, random entries
+
+
+def test_search_graph_replays_match_eager(ctx, small_float, small_float_graph):
+    """The captured launch, replayed with the program's own eager searches and
+    synchronisations in between and the query rows rewritten between replays,
+    returns on every replay exactly what the eager search returns."""
+    from weavess_amd import Index
+    from weavess_amd.devmem import DeviceArray, pad_rows
+    base, q = small_float
+    g, _, deg = small_float_graph
+    ix = Index(ctx, base)
+    ix.set_graph(g, deg)
+    q1, q2 = q, q[::-1].copy()
+    refs = [ix.search(q1, 64, 10, seed=4), ix.search(q2, 64, 10, seed=4)]
+    Q = len(q)
+    dq = DeviceArray.from_host(pad_rows(q1))
+    d_ids, d_dist = DeviceArray((Q, 10), np.uint32), DeviceArray((Q, 10), np.float32)
+    d_ev, d_hp = DeviceArray((Q,), np.uint32), DeviceArray((Q,), np.uint32)
+    sg = ix.search_graph(dq.ptr, Q, 64, 10, d_ids.ptr, seed=4, d_out_dist=d_dist.ptr, d_evals=d_ev.ptr,
+                         d_hops=d_hp.ptr)
+    for rep in range(4):
+        ix.search(q2[:50], 32, 5, seed=1)  # eager work between replays
+        dq.copy_from_host(pad_rows(q1 if rep % 2 == 0 else q2))
+        sg.launch()
+        ref = refs[rep % 2]
+        assert np.array_equal(d_ids.to_host(), ref[0]) and np.array_equal(d_dist.to_host(), ref[1]), rep
+        assert np.array_equal(d_ev.to_host(), ref[2]) and np.array_equal(d_hp.to_host(), ref[3]), rep
+    sg.close()
+    for a in (dq, d_ids, d_dist, d_ev, d_hp):
+        a.free()
+    ix.close()
+
+
+def test_search_graph_per_query_budgets_and_model(ctx, small_float, small_float_graph):
+    """Per-query L and k, and the learned stop, through the captured launch."""
+    from weavess_amd import Index
+    from weavess_amd._capi import TermModel
+    from weavess_amd.devmem import DeviceArray, pad_rows
+    base, q = small_float
+    g, _, deg = small_float_graph
+    ix = Index(ctx, base)
+    ix.set_graph(g, deg)
+    Q = len(q)
+    rng = np.random.default_rng(2)
+    Lq = rng.integers(16, 200, Q).astype(np.uint32)
+    kq = np.minimum(rng.choice([1, 10, 16], Q), Lq).astype(np.uint32)
+    ref = ix.search(q, 0, 0, seed=5, L_per_q=Lq, k_per_q=kq, k_stride=16)
+    m = TermModel.init(3)
+    m.b2 = -1.0
+    m.logit_threshold = 0.0
+    mref = ix.search_ext(q, 120, 10, seed=5, model=m)
+    dq = DeviceArray.from_host(pad_rows(q))
+    dL, dk = DeviceArray.from_host(Lq), DeviceArray.from_host(kq)
+    d_ids, d_ev = DeviceArray((Q, 16), np.uint32), DeviceArray((Q,), np.uint32)
+    sg = ix.search_graph(dq.ptr, Q, 0, 0, d_ids.ptr, seed=5, d_L=dL.ptr, d_k=dk.ptr, k_stride=16, d_evals=d_ev.ptr)
+    m_ids = DeviceArray((Q, 10), np.uint32)
+    sm = ix.search_graph(dq.ptr, Q, 120, 10, m_ids.ptr, seed=5, model=m)
+    for _ in range(3):
+        sg.launch()
+        sm.launch()
+        assert np.array_equal(d_ids.to_host(), ref[0]) and np.array_equal(d_ev.to_host(), ref[2])
+        assert np.array_equal(m_ids.to_host(), mref[0])
+    sg.close()
+    sm.close()
+    for a in (dq, dL, dk, d_ids, d_ev, m_ids):
+        a.free()
+    ix.close()

@@ -29,7 +29,8 @@ EXPORTS = [
     "wv_index_create", "wv_index_destroy", "wv_index_set_graph", "wv_index_get_graph",
     "wv_index_set_search_config", "wv_index_build_nndescent", "wv_index_search",
     "wv_index_search_dev", "wv_nndescent_build", "wv_search_greedy", "wv_exact_knn",
-    "wv_index_search_ext", "wv_index_search_ext_dev", "wv_index_autotune", "wv_mlp_grad", "wv_mlp_train",
+    "wv_index_search_ext", "wv_index_search_ext_dev", "wv_index_autotune", "wv_search_graph_create",
+    "wv_search_graph_launch", "wv_search_graph_destroy", "wv_mlp_grad", "wv_mlp_train",
     "wv_mlp_predict",
 ]
 
@@ -153,6 +154,10 @@ def load() -> C.CDLL:
         "wv_index_search_ext_dev": ([vp, vp, u32, vp, u64, u32, u32, vp, vp, u32, vp, vp, vp, vp, vp, vp],
                                     i32),
         "wv_index_autotune": ([vp, vp, u32, u64, u32, u32, vp, vp, C.POINTER(C.c_double)], i32),
+        "wv_search_graph_create": ([vp, vp, u32, u64, u32, u32, vp, vp, u32, vp, vp, vp, vp, vp,
+                                    C.POINTER(vp)], i32),
+        "wv_search_graph_launch": ([vp, i32], i32),
+        "wv_search_graph_destroy": ([vp], i32),
         "wv_mlp_grad": ([vp, vp, vp, vp, u32, C.c_float, vp, C.POINTER(C.c_double)], i32),
         "wv_mlp_train": ([vp, vp, vp, u32, vp, vp, vp], i32),
         "wv_mlp_predict": ([vp, vp, vp, u32, vp], i32),
@@ -272,6 +277,29 @@ class Context:
                                              seed, L, k, _p(lq), _p(kq), k_stride, _p(ids),
                                              _p(dist), _p(ev), _p(hp), C.byref(st)))
         return ids, dist, ev, hp, st.as_dict()
+
+
+class SearchGraph:
+    """A captured search launch: ``launch()`` replays it on the context's
+    stream with the buffers given at capture (rewrite the query rows between
+    replays to serve new queries)."""
+
+    def __init__(self, ctx: "Context", h):
+        self.ctx, self.h = ctx, h
+
+    def launch(self, sync: bool = True):
+        self.ctx.check(self.ctx.lib.wv_search_graph_launch(self.h, 1 if sync else 0))
+
+    def close(self):
+        if self.h:
+            self.ctx.lib.wv_search_graph_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class Index:
@@ -394,6 +422,19 @@ class Index:
                                                       C.byref(ext) if ext is not None else None,
                                                       C.byref(cfg), C.byref(ms)))
         return {f: getattr(cfg, f) for f, _ in cfg._fields_}, ms.value
+
+    def search_graph(self, d_queries: int, Q: int, L: int, k: int, d_out_ids: int, seed: int = 0,
+                     d_out_dist: int = 0, d_L: int = 0, d_k: int = 0, k_stride: Optional[int] = None,
+                     d_evals: int = 0, d_hops: int = 0, model: Optional["TermModel"] = None) -> "SearchGraph":
+        """Capture the device-buffer search once as a hipGraph (see SearchGraph)."""
+        vp = lambda x: C.c_void_p(x) if x else None
+        ext = SearchExt(C.addressof(model), None, None, None, 0) if model is not None else None
+        h = C.c_void_p()
+        self.ctx.check(self.ctx.lib.wv_search_graph_create(
+            self.h, vp(d_queries), Q, seed, L, k, vp(d_L), vp(d_k), k_stride if k_stride is not None else k,
+            vp(d_out_ids), vp(d_out_dist), vp(d_evals), vp(d_hops), C.byref(ext) if ext is not None else None,
+            C.byref(h)))
+        return SearchGraph(self.ctx, h)
 
     def autotune(self, queries, L: int, k: int, seed: int = 0, model: Optional["TermModel"] = None):
         """autotune_dev on host sample queries (copied to the device)."""

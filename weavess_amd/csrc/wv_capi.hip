@@ -12,6 +12,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <memory>
 #include <mutex>
 #include <new>
 #include <numeric>
@@ -37,6 +38,13 @@ struct wv_index {
     uint32_t *deg = nullptr;    // N
     uint32_t deg_max = 0;
     wv_search_config cfg = {0, 0, 0, 0};
+};
+
+struct wv_search_graph {
+    wv_index *ix = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    void *model = nullptr;  // device copy of the stop model, or null
 };
 
 namespace {
@@ -140,33 +148,17 @@ struct SearchExtDev {
     uint32_t cap = 0;
 };
 
-int search_core(wv_index *ix, const float *d_q, uint32_t Q, const uint32_t *d_entries,
-                uint64_t seed, uint32_t L, uint32_t k, const uint32_t *d_L, const uint32_t *d_k,
-                uint32_t Lmax, uint32_t k_stride, uint32_t *d_ids, float *d_dist,
-                uint32_t *d_evals, uint32_t *d_hops, wv_search_stats *stats,
-                const SearchExtDev &ext = SearchExtDev()) {
+// Launch arguments of one search over device buffers (no allocation).
+int prepare_search(wv_index *ix, const float *d_q, uint32_t Q, const uint32_t *d_entries, uint64_t seed,
+                   uint32_t L, uint32_t k, const uint32_t *d_L, const uint32_t *d_k, uint32_t Lmax,
+                   uint32_t k_stride, uint32_t *d_ids, float *d_dist, uint32_t *d_evals, uint32_t *d_hops,
+                   const SearchExtDev &ext, wv::SearchArgs &a, SearchPlan &pl, int &cpl) {
     wv_ctx *c = ix->ctx;
     if (!ix->graph) return fail(c, WV_EINVAL, "index has no graph (build or set one first)");
-    if (Q == 0) {
-        if (stats) *stats = wv_search_stats{0.0, 0, 0};
-        return WV_OK;
-    }
-    SearchPlan pl;
     int rc = plan_search(ix, Lmax, pl);
     if (rc) return rc;
-    const int cpl = chunks_per_lane(ix->dp / 4);
+    cpl = chunks_per_lane(ix->dp / 4);
     if (!cpl) return fail(c, WV_ENOTSUP, "dimension %u above 1024", ix->d);
-
-    DevBuf ev_tmp, hop_tmp;
-    if (stats && !d_evals) {
-        WV_HIP(c, ev_tmp.alloc((size_t)Q * 4));
-        d_evals = ev_tmp.as<uint32_t>();
-    }
-    if (stats && !d_hops) {
-        WV_HIP(c, hop_tmp.alloc((size_t)Q * 4));
-        d_hops = hop_tmp.as<uint32_t>();
-    }
-    wv::SearchArgs a;
     a.base = ix->base;
     a.queries = d_q;
     a.graph = ix->graph;
@@ -196,6 +188,35 @@ int search_core(wv_index *ix, const float *d_q, uint32_t Q, const uint32_t *d_en
     a.trace_len = ext.trace_len;
     a.settle = ext.settle;
     a.trace_cap = ext.cap;
+    return WV_OK;
+}
+
+int search_core(wv_index *ix, const float *d_q, uint32_t Q, const uint32_t *d_entries,
+                uint64_t seed, uint32_t L, uint32_t k, const uint32_t *d_L, const uint32_t *d_k,
+                uint32_t Lmax, uint32_t k_stride, uint32_t *d_ids, float *d_dist,
+                uint32_t *d_evals, uint32_t *d_hops, wv_search_stats *stats,
+                const SearchExtDev &ext = SearchExtDev()) {
+    wv_ctx *c = ix->ctx;
+    if (!ix->graph) return fail(c, WV_EINVAL, "index has no graph (build or set one first)");
+    if (Q == 0) {
+        if (stats) *stats = wv_search_stats{0.0, 0, 0};
+        return WV_OK;
+    }
+    DevBuf ev_tmp, hop_tmp;
+    if (stats && !d_evals) {
+        WV_HIP(c, ev_tmp.alloc((size_t)Q * 4));
+        d_evals = ev_tmp.as<uint32_t>();
+    }
+    if (stats && !d_hops) {
+        WV_HIP(c, hop_tmp.alloc((size_t)Q * 4));
+        d_hops = hop_tmp.as<uint32_t>();
+    }
+    wv::SearchArgs a;
+    SearchPlan pl;
+    int cpl = 0;
+    int rc = prepare_search(ix, d_q, Q, d_entries, seed, L, k, d_L, d_k, Lmax, k_stride, d_ids, d_dist, d_evals,
+                            d_hops, ext, a, pl, cpl);
+    if (rc) return rc;
     WV_HIP(c, hipEventRecord(c->ev0, c->stream));
     WV_HIP(c, wv::launch_greedy_search(a, cpl, (int)pl.groups, (int)pl.wpb, c->stream));
     WV_HIP(c, hipEventRecord(c->ev1, c->stream));
@@ -676,6 +697,92 @@ int wv_index_autotune(wv_index *ix, const float *d_queries, uint32_t Q, uint64_t
     ix->cfg = cur;
     if (best) *best = cur;
     if (best_ms) *best_ms = cur_ms;
+    return WV_OK;
+}
+
+int wv_search_graph_create(wv_index *ix, const float *d_queries, uint32_t Q, uint64_t seed, uint32_t L,
+                           uint32_t k, const uint32_t *d_L_per_q, const uint32_t *d_k_per_q, uint32_t k_stride,
+                           uint32_t *d_out_ids, float *d_out_dist, uint32_t *d_evals, uint32_t *d_hops,
+                           const wv_search_ext *ext, wv_search_graph **out) {
+    if (!ix || !out) return WV_EINVAL;
+    *out = nullptr;
+    wv_ctx *c = ix->ctx;
+    std::lock_guard<std::mutex> g(c->mu);
+    if (!d_queries || !d_out_ids || Q == 0) return fail(c, WV_EINVAL, "queries/out_ids are null or Q is 0");
+    if (ext && (ext->trace || ext->trace_len || ext->settle_hop))
+        return fail(c, WV_ENOTSUP, "a search graph records no traces");
+    WV_HIP(c, hipSetDevice(c->device));
+    std::vector<uint32_t> hl, hk;
+    if (d_L_per_q) {
+        hl.resize(Q);
+        WV_HIP(c, hipMemcpy(hl.data(), d_L_per_q, (size_t)Q * 4, hipMemcpyDeviceToHost));
+    }
+    if (d_k_per_q) {
+        hk.resize(Q);
+        WV_HIP(c, hipMemcpy(hk.data(), d_k_per_q, (size_t)Q * 4, hipMemcpyDeviceToHost));
+    }
+    uint32_t Lmax = 0;
+    int rc = check_search_args(c, ix->N, Q, nullptr, L, k, d_L_per_q ? hl.data() : nullptr,
+                               d_k_per_q ? hk.data() : nullptr, k_stride, Lmax);
+    if (rc) return rc;
+    std::unique_ptr<wv_search_graph> sg(new wv_search_graph);
+    sg->ix = ix;
+    SearchExtDev x;
+    if (ext && ext->model) {
+        WV_HIP(c, hipMalloc(&sg->model, sizeof(wv_term_model)));
+        WV_HIP(c, hipMemcpy(sg->model, ext->model, sizeof(wv_term_model), hipMemcpyHostToDevice));
+        x.model = static_cast<const wv_term_model *>(sg->model);
+    }
+    wv::SearchArgs a;
+    SearchPlan pl;
+    int cpl = 0;
+    rc = prepare_search(ix, d_queries, Q, nullptr, seed, L, k, d_L_per_q, d_k_per_q, Lmax, k_stride, d_out_ids,
+                        d_out_dist, d_evals, d_hops, x, a, pl, cpl);
+    if (rc) {
+        if (sg->model) (void)hipFree(sg->model);
+        return rc;
+    }
+    // one eager launch sets the kernel's LDS attribute outside the capture and
+    // fills the outputs once
+    hipError_t e = wv::launch_greedy_search(a, cpl, (int)pl.groups, (int)pl.wpb, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e == hipSuccess) e = hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal);
+    if (e == hipSuccess) {
+        hipError_t le = wv::launch_greedy_search(a, cpl, (int)pl.groups, (int)pl.wpb, c->stream);
+        e = hipStreamEndCapture(c->stream, &sg->graph);
+        if (e == hipSuccess) e = le;
+    }
+    if (e == hipSuccess) e = hipGraphInstantiate(&sg->exec, sg->graph, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        if (sg->exec) (void)hipGraphExecDestroy(sg->exec);
+        if (sg->graph) (void)hipGraphDestroy(sg->graph);
+        if (sg->model) (void)hipFree(sg->model);
+        return fail(c, WV_EHIP, "search graph capture: %s", hipGetErrorString(e));
+    }
+    *out = sg.release();
+    return WV_OK;
+}
+
+int wv_search_graph_launch(wv_search_graph *sg, int sync) {
+    if (!sg) return WV_EINVAL;
+    wv_ctx *c = sg->ix->ctx;
+    std::lock_guard<std::mutex> g(c->mu);
+    WV_HIP(c, hipSetDevice(c->device));
+    WV_HIP(c, hipGraphLaunch(sg->exec, c->stream));
+    if (sync) WV_HIP(c, hipStreamSynchronize(c->stream));
+    return WV_OK;
+}
+
+int wv_search_graph_destroy(wv_search_graph *sg) {
+    if (!sg) return WV_OK;
+    wv_ctx *c = sg->ix->ctx;
+    std::lock_guard<std::mutex> g(c->mu);
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    if (sg->exec) (void)hipGraphExecDestroy(sg->exec);
+    if (sg->graph) (void)hipGraphDestroy(sg->graph);
+    if (sg->model) (void)hipFree(sg->model);
+    delete sg;
     return WV_OK;
 }
 

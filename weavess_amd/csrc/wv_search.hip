@@ -201,9 +201,12 @@ __global__ void __launch_bounds__(256) greedy_search_kernel(SearchArgs a) {
     uint64_t *buf = reinterpret_cast<uint64_t *>(flag + a.P);   // B (P is a multiple of 64)
     uint8_t *bflag = reinterpret_cast<uint8_t *>(buf + a.B);    // B
 
-    const uint32_t L = a.L_per_q ? a.L_per_q[q] : a.L;
-    const uint32_t kq = a.k_per_q ? a.k_per_q[q] : a.k;
     const uint32_t N = a.N, nch = a.nch, dp = a.dp;
+    // per-query budgets were checked on the host when the launch was planned;
+    // clamping to the plan keeps a buffer rewritten since inside the pool
+    const uint32_t lcap = min(a.P, a.entries ? N : (N > 1 ? N - 1 : 1u));
+    const uint32_t L = min(max(a.L_per_q ? a.L_per_q[q] : a.L, 1u), lcap);
+    const uint32_t kq = min(max(a.k_per_q ? a.k_per_q[q] : a.k, 1u), min(L, a.k_stride));
 
     float4 qv[CPL];
     {

@@ -60,6 +60,14 @@ class DeviceArray:
                "hipMemcpy H2D")
         return d
 
+    def copy_from_host(self, a: np.ndarray):
+        """Overwrite the allocation with a host array of the same size."""
+        a = np.ascontiguousarray(a, self.dtype)
+        if a.nbytes != self.nbytes:
+            raise ValueError(f"size mismatch: {a.nbytes} bytes into {self.nbytes}")
+        _check(hip().hipMemcpy(C.c_void_p(self.ptr), a.ctypes.data_as(C.c_void_p), a.nbytes, H2D),
+               "hipMemcpy H2D")
+
     def to_host(self) -> np.ndarray:
         out = np.empty(self.shape, self.dtype)
         _check(hip().hipMemcpy(out.ctypes.data_as(C.c_void_p), C.c_void_p(self.ptr), self.nbytes, D2H),
