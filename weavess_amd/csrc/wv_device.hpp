@@ -112,6 +112,23 @@ __device__ inline uint64_t wave_sort64(uint64_t key) {
     return key;
 }
 
+// Ascending bitonic sort of the keys in lanes [0, w), w a power of two in
+// [2, 64] (lanes from w up keep KEY_EMPTY-sorted garbage of their own block).
+__device__ inline uint64_t wave_sort64_n(uint64_t key, int w) {
+    const int lane = lane_id();
+    for (int k = 2; k <= w; k <<= 1) {
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            uint64_t o = shfl_xor64(key, j);
+            bool up = (lane & k) == 0;
+            bool lower = (lane & j) == 0;
+            bool keep_min = (lower == up);
+            uint64_t mn = key < o ? key : o, mx = key < o ? o : key;
+            key = keep_min ? mn : mx;
+        }
+    }
+    return key;
+}
+
 // Ascending bitonic sort of P (power of two, >= 2) 64-bit keys in LDS by one
 // wavefront.
 __device__ inline void lds_sort64(uint64_t *a, int P) {

@@ -336,15 +336,20 @@ __global__ void __launch_bounds__(256) greedy_search_kernel(SearchArgs a) {
             } else {
                 key = KEY_EMPTY;
             }
-            if (__ballot(key != KEY_EMPTY) == 0) continue;
-            key = wave_sort64(key);
-            const uint64_t prev = shfl64(key, lane > 0 ? lane - 1 : 0);
-            const bool live = key != KEY_EMPTY && !(lane > 0 && prev == key);
-            const uint64_t lm = __ballot(live);
+            // the batch's keys are distinct (the visited table admits an id
+            // once per batch): compact the survivors to the front lanes and
+            // sort only the smallest power-of-two width holding them
+            const uint64_t lm = __ballot(key != KEY_EMPTY);
+            if (lm == 0) continue;
             const int c = __popcll(lm);
-            const int r = (int)lanes_below(lm);
             wave_sync();
-            if (live) ckey[r] = key;
+            if (key != KEY_EMPTY) ckey[lanes_below(lm)] = key;
+            wave_sync();
+            int w = 2;
+            while (w < c) w <<= 1;
+            key = wave_sort64_n(lane < c ? ckey[lane] : KEY_EMPTY, w);
+            wave_sync();
+            if (lane < c) ckey[lane] = key;
             wave_sync();
             if (B == 0) {
                 const int pos0 = merge_into(pool, flag, Li, Li, ckey, nullptr, c);
