@@ -217,7 +217,8 @@ int wv_index_search_ext_dev(wv_index *idx, const float *d_queries, uint32_t Q,
  * model of `ext` when given) under candidate tunings, by coordinate descent
  * from the default plan over visited_slots {256..4096}, row_groups {4,8,16},
  * merge_buffer {none,128,256} and waves_per_block {1,2,4}, each candidate
- * timed as the best of 2 launches after a warm-up.  The fastest tuning is
+ * timed as the best of 3 launches after a warm-up and taken only when 2 %
+ * faster than the current choice.  The fastest tuning is
  * kept in the index (as wv_index_set_search_config would) and written to
  * *best with its kernel time.  Results never depend on the tuning. */
 int wv_index_autotune(wv_index *idx, const float *d_queries, uint32_t Q, uint64_t seed, uint32_t L,

@@ -645,13 +645,13 @@ int wv_index_autotune(wv_index *ix, const float *d_queries, uint32_t Q, uint64_t
     }
     WV_HIP(c, di.alloc((size_t)Q * k * 4));
     const wv_search_config saved = ix->cfg;
-    // best of 2 timed launches after a warm-up; planning errors reject the
+    // best of 3 timed launches after a warm-up; planning errors reject the
     // candidate (rc EINVAL/ENOTSUP), launch errors end the tuning
     auto time_cfg = [&](const wv_search_config &cfg, double &ms) -> int {
         ix->cfg = cfg;
         wv_search_stats st;
         ms = 1e30;
-        for (int r = 0; r < 3; ++r) {
+        for (int r = 0; r < 4; ++r) {
             const int e = search_core(ix, d_queries, Q, nullptr, seed, L, k, nullptr, nullptr, Lmax, k,
                                       di.as<uint32_t>(), nullptr, nullptr, nullptr, &st, x);
             if (e) return e;
@@ -688,7 +688,7 @@ int wv_index_autotune(wv_index *ix, const float *d_queries, uint32_t Q, uint64_t
                 ix->cfg = saved;
                 return e;
             }
-            if (ms < cur_ms) {
+            if (ms < 0.98 * cur_ms) {  // a move must beat timing noise
                 cur = t;
                 cur_ms = ms;
             }

@@ -86,7 +86,7 @@ __device__ inline int merge_into(uint64_t *arr, uint8_t *aflag, int len, int cap
             cpos[r] = j + upper_bound64(arr, len, ck[r]);
         }
     }
-    const int pos0 = upper_bound64(arr, len, src[0]);
+    const int pos0 = __shfl(cpos[0], 0);  // lane 0 holds src[0]: j = 0
     for (int t0 = (len - 1) & ~(WAVE - 1); t0 >= (pos0 & ~(WAVE - 1)); t0 -= WAVE) {
         const int p = t0 + lane;
         const bool mv = p >= pos0 && p < len;
@@ -326,13 +326,23 @@ __global__ void __launch_bounds__(256) greedy_search_kernel(SearchArgs a) {
             wave_sync();
             uint64_t key = lane < m ? ckey[lane] : KEY_EMPTY;
             if (key < worst) {
-                // worst <= pool[L-1], so the pool slot read is in range
-                const int lb = lower_bound64(pool, Li, key);
-                if (pool[lb] == key) key = KEY_EMPTY;  // already pooled: duplicate
-                if (nb) {
-                    const int lbb = lower_bound64(buf, nb, key);
-                    if (lbb < nb && buf[lbb] == key) key = KEY_EMPTY;
+                // already in pool or buffer: a duplicate.  Both binary searches
+                // advance in one loop so their LDS reads overlap; worst <=
+                // pool[L-1], so the pool slot read is in range.
+                int plo = 0, phi = Li, blo = 0, bhi = nb;
+                while (plo < phi || blo < bhi) {
+                    if (plo < phi) {
+                        const int mid = (plo + phi) >> 1;
+                        if (pool[mid] < key) plo = mid + 1;
+                        else phi = mid;
+                    }
+                    if (blo < bhi) {
+                        const int mid = (blo + bhi) >> 1;
+                        if (buf[mid] < key) blo = mid + 1;
+                        else bhi = mid;
+                    }
                 }
+                if (pool[plo] == key || (blo < nb && buf[blo] == key)) key = KEY_EMPTY;
             } else {
                 key = KEY_EMPTY;
             }
@@ -366,7 +376,7 @@ __global__ void __launch_bounds__(256) greedy_search_kernel(SearchArgs a) {
                 const int bp = merge_into(buf, bflag, nb, B, ckey, nullptr, c);
                 nb += c;
                 if (bp < kb) kb = bp;
-                if (bp + lower_bound64(pool, Li, ckey[0]) <= kidx) settle = hops;
+                if (watch && bp + lower_bound64(pool, Li, ckey[0]) <= kidx) settle = hops;
                 worst = kth_union(pool, Li, buf, nb, Li - 1);
             }
         }
