@@ -30,7 +30,8 @@
 // k-pointer restart of src/component_route.cpp:44-45 lands when (dist,id) is
 // a strict order.
 // The visited table forgets: when an id finds no free slot within the probe
-// limit it evicts the occupant of its home slot.  A forgotten node that comes
+// limit (8 slots until the table first overflows, 2 after) it evicts the
+// occupant of its home slot.  A forgotten node that comes
 // back is re-evaluated; it is then either already in pool or buffer
 // (duplicate) or not below the L-th key, so ids and distances are unchanged
 // and only the evaluation count grows.
@@ -42,22 +43,24 @@ namespace wv {
 
 namespace {
 
-constexpr int VIS_PROBES = 8;
+constexpr int VIS_PROBES = 8;      // probes while the table has never overflowed
+constexpr int VIS_PROBES_FULL = 2; // after its first eviction
 
-// Test-and-set of `id` in the visited table; true when this call inserted it
-// (first visit, or the id had been forgotten: when no free slot is found
-// within the probe limit the occupant of the home slot is evicted).
-__device__ inline bool visit_first(uint32_t *vis, uint32_t hmask, uint32_t id) {
+// Test-and-set of `id` in the visited table, probing at most `probes` slots.
+// Returns 0 when the id was there, 1 when this call inserted it into a free
+// slot, 2 when it found no free slot and evicted the occupant of its home
+// slot (the table forgets).  Both 1 and 2 mean "evaluate".
+__device__ inline int visit_first(uint32_t *vis, uint32_t hmask, uint32_t id, int probes) {
     const uint32_t h0 = fmix32(id) & hmask;
     uint32_t h = h0;
-    for (int p = 0; p < VIS_PROBES; ++p) {
+    for (int p = 0; p < probes; ++p) {
         const uint32_t old = atomicCAS(&vis[h], ID_NONE, id);
-        if (old == ID_NONE) return true;
-        if (old == id) return false;
+        if (old == ID_NONE) return 1;
+        if (old == id) return 0;
         h = (h + 1) & hmask;
     }
     atomicExch(&vis[h0], id);
-    return true;
+    return 2;
 }
 
 // Merge c sorted keys src[0..c) (flags sflag[], or 1 when null) into the
@@ -267,6 +270,7 @@ __global__ void __launch_bounds__(256) greedy_search_kernel(SearchArgs a) {
     // k-th distance 1..4 expansions ago
     float r0 = watch ? key_dist(pool[kidx]) : 0.f, r1 = r0, r2 = r0, r3 = r0;
     uint32_t settle = 0;  // last expansion that changed the top-k
+    int probes = VIS_PROBES;
     for (;;) {
         // U's first unexpanded key: the smaller of the pool's and the buffer's
         const int kk = first_flag(flag, kp, Li);
@@ -275,9 +279,8 @@ __global__ void __launch_bounds__(256) greedy_search_kernel(SearchArgs a) {
         const uint64_t kbk = jb < nb ? buf[jb] : KEY_EMPTY;
         const bool from_buf = kbk < ka;
         const uint64_t cur = from_buf ? kbk : ka;
-        bool done = cur == KEY_EMPTY;
-        if (!done && nb)  // in U only if fewer than L keys are smaller
-            done = (from_buf ? jb + lower_bound64(pool, Li, cur) : kk + lower_bound64(buf, nb, cur)) >= Li;
+        // cur is in U iff it is not above U's L-th key (`worst` is kept exact)
+        const bool done = cur == KEY_EMPTY || cur > worst;
         if (watch && hops > 0) {
             // state after `hops` expansions
             float f[MLP_IN];
@@ -315,7 +318,10 @@ __global__ void __launch_bounds__(256) greedy_search_kernel(SearchArgs a) {
         for (uint32_t m0 = 0; m0 < du; m0 += WAVE) {
             const uint32_t mi = m0 + lane;
             const uint32_t id = mi < du ? adj[mi] : ID_NONE;
-            const bool fresh = id < N && visit_first(vis, hmask, id);
+            const int vr = id < N ? visit_first(vis, hmask, id, probes) : 0;
+            const bool fresh = vr != 0;
+            // a full table stops probing deeply: it forgets either way
+            if (__ballot(vr == 2)) probes = VIS_PROBES_FULL;
             const uint64_t bm = __ballot(fresh);
             const int m = __popcll(bm);
             if (fresh) cid[lanes_below(bm)] = id;
