@@ -619,7 +619,7 @@ __global__ void __launch_bounds__(256) nnd_join_kernel(NndDev a) {
         col[i] = id;
         cthr[i] = a.thr[id];
         cnorm[i] = a.norm[id];
-        cscale[i] = (DT == DT_FP8) ? a.scale[id] : 1.0f;
+        if (DT == DT_FP8) cscale[i] = a.scale[id];  // only the FP8 list has this array
     }
     wave_sync();
 
@@ -1062,7 +1062,9 @@ int nndescent_build(wv_index *ix, const wv_nnd_params *p, const uint32_t *init_i
 
     const uint32_t ncap = S + 3 * R;
     a.nw_pow2 = pow2_at_least(S + R, 64);
-    a.lds_join_list = (uint32_t)(((a.nw_pow2 * 4 + ((ncap + 1) & ~1u) * 4 + ncap * 16) + 15) & ~15u);
+    // list area: nw ids, col ids, cthr keys, cnorm, and FP8 row scales
+    a.lds_join_list = (uint32_t)(((a.nw_pow2 * 4 + ((ncap + 1) & ~1u) * 4 + ncap * (dt == DT_FP8 ? 16 : 12)) + 15) &
+                                 ~15u);
     {
         // tile rows as wide as one K chunk of this row length: BF16 / FP8 rows
         // of d <= 256 / 512 need a fraction of the 512-byte maximum
