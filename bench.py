@@ -17,8 +17,9 @@ Default workload "c5" (the configuration the metric is quoted on):
     1.5 x L_plain, threshold calibrated on the calibration queries for
     recall 0.95 + 0.003; searched with the same cap.
   * NN-Descent leg: the 1M x 128 Zipf mixture (K=64, S=16, 12 rounds) with
-    L = R = 96, built in FP32 and in BF16 (FP32 rerank); graph recall on
-    10,000 control nodes against exact FP64 kNN.
+    L = R = 96 and 256 candidate slots per node (the smallest settings tried
+    that reach graph recall 0.95), built in FP32 and in BF16 (FP32 rerank);
+    graph recall on 10,000 control nodes against exact FP64 kNN.
 
 A "step" is one search pass over the 200,000 workload queries, with base,
 graph, queries, model and outputs resident in HBM.  Each method's search
@@ -72,8 +73,8 @@ WORKLOADS = {
                sweep=[20, 40, 80, 100], control=10_000, learned=False),
 }
 # NN-Descent leg of the metric: build seconds at graph recall >= 0.95
-NND_LEG = dict(gen="gmm_c2", K=64, L=96, S=16, R=96, iters=12, control=10_000,
-               desc="1M x 128 Zipf(1.1) GMM, K=64 L=96 S=16 R=96 iters=12")
+NND_LEG = dict(gen="gmm_c2", K=64, L=96, S=16, R=96, iters=12, cand_slots=256, control=10_000,
+               desc="1M x 128 Zipf(1.1) GMM, K=64 L=96 S=16 R=96 iters=12, 256 candidate slots per node")
 
 
 def log(*a):
@@ -108,7 +109,8 @@ def nndescent_leg(ctx, Index, datagen):
     out = {"workload": w["desc"]}
     for dt in ("bf16", "f32"):
         t1 = time.time()
-        st = ix.build_nndescent(w["K"], w["L"], w["S"], w["R"], w["iters"], seed=1, dtype=dt)
+        st = ix.build_nndescent(w["K"], w["L"], w["S"], w["R"], w["iters"], seed=1, dtype=dt,
+                                cand_slots=w["cand_slots"])
         wall = time.time() - t1
         g, _, dg = ix.get_graph()
         gr = graph_recall(g[:ctl], dg[:ctl], gt)

@@ -373,14 +373,18 @@ __global__ void __launch_bounds__(256) nnd_rev_scatter_kernel(NndDev a) {
 
 // Reverse-sample priority key of proposer p at node x (hash high word, p low
 // word); the R smallest are kept.
-__device__ inline uint64_t rev_key(uint64_t seed, uint64_t tag, uint32_t iter, uint32_t x, uint32_t p) {
-    uint64_t h = fmix64(stream_of(seed, tag, ((uint64_t)iter << 32) | x) ^ ((uint64_t)p * GOLDEN));
+__device__ inline uint64_t rev_stream(uint64_t seed, uint64_t tag, uint32_t iter, uint32_t x) {
+    return stream_of(seed, tag, ((uint64_t)iter << 32) | x);
+}
+__device__ inline uint64_t rev_key(uint64_t sx, uint32_t p) {
+    uint64_t h = fmix64(sx ^ ((uint64_t)p * GOLDEN));
     return ((h >> 32) << 32) | p;
 }
 
 // Keep at most R reverse proposals of node x (update() :190-208, :213-228):
 // all when there are at most R, else the R smallest rev_key (radix select
-// of the R-th smallest over the segment); the kept ids are sorted ascending.
+// of the R-th smallest over the segment, a byte per pass, stopping once every
+// key of the chosen bin is needed); the kept ids are sorted ascending.
 __global__ void __launch_bounds__(256) nnd_rev_select_kernel(NndDev a, uint32_t rp2) {
     extern __shared__ __align__(16) uint8_t smem[];
     const int wid = threadIdx.x >> 6, lane = lane_id();
@@ -394,7 +398,7 @@ __global__ void __launch_bounds__(256) nnd_rev_select_kernel(NndDev a, uint32_t 
         const uint32_t *off = which ? a.roff_old : a.roff_new;
         const uint32_t *seg = (which ? a.rv_old : a.rv_new) + off[x];
         const uint32_t c = off[x + 1] - off[x];
-        const uint64_t tag = which ? TAG_REVOLD : TAG_REVNEW;
+        const uint64_t sx = rev_stream(a.seed, which ? TAG_REVOLD : TAG_REVNEW, a.iter, x);
         const uint32_t R = a.R;
         uint32_t kept;
         if (c <= R) {
@@ -407,7 +411,7 @@ __global__ void __launch_bounds__(256) nnd_rev_select_kernel(NndDev a, uint32_t 
                 for (int i = lane; i < 256; i += WAVE) hist[i] = 0;
                 wave_sync();
                 for (uint32_t i = lane; i < c; i += WAVE) {
-                    const uint64_t k = rev_key(a.seed, tag, a.iter, x, seg[i]);
+                    const uint64_t k = rev_key(sx, seg[i]);
                     if ((k & pmask) == prefix) atomicAdd(&hist[(k >> shift) & 255], 1u);
                 }
                 wave_sync();
@@ -445,15 +449,18 @@ __global__ void __launch_bounds__(256) nnd_rev_select_kernel(NndDev a, uint32_t 
                 prefix |= (uint64_t)bin << shift;
                 pmask |= (uint64_t)255 << shift;
                 need -= below;
+                const uint32_t inbin = hist[bin];
                 wave_sync();
+                if (inbin == need) break;  // the whole bin is kept: no finer pass needed
             }
-            // keys are distinct (low word = proposer id): exactly R are <= prefix
+            // keys are distinct (low word = proposer id): exactly R have
+            // (key & pmask) <= prefix
             if (lane == 0) *cnt = 0;
             for (uint32_t i = lane; i < rp2; i += WAVE) ids[i] = ID_NONE;
             wave_sync();
             for (uint32_t i = lane; i < c; i += WAVE) {
                 const uint32_t p = seg[i];
-                if (rev_key(a.seed, tag, a.iter, x, p) <= prefix) {
+                if ((rev_key(sx, p) & pmask) <= prefix) {
                     const uint32_t pos = atomicAdd(cnt, 1u);
                     if (pos < rp2) ids[pos] = p;
                 }
