@@ -30,8 +30,8 @@
 // k-pointer restart of src/component_route.cpp:44-45 lands when (dist,id) is
 // a strict order.
 // The visited table forgets: when an id finds no free slot within the probe
-// limit (8 slots until the table first overflows, 2 after) it evicts the
-// occupant of its home slot.  A forgotten node that comes
+// limit (8 slots until more ids than slots were inserted, 2 after) it evicts
+// the occupant of its home slot.  A forgotten node that comes
 // back is re-evaluated; it is then either already in pool or buffer
 // (duplicate) or not below the L-th key, so ids and distances are unchanged
 // and only the evaluation count grows.
@@ -43,8 +43,8 @@ namespace wv {
 
 namespace {
 
-constexpr int VIS_PROBES = 8;      // probes while the table has never overflowed
-constexpr int VIS_PROBES_FULL = 2; // after its first eviction
+constexpr int VIS_PROBES = 8;      // probes while fewer ids than slots were inserted
+constexpr int VIS_PROBES_FULL = 2; // after that
 
 // Test-and-set of `id` in the visited table, probing at most `probes` slots.
 // Returns 0 when the id was there, 1 when this call inserted it into a free
@@ -318,10 +318,7 @@ __global__ void __launch_bounds__(256) greedy_search_kernel(SearchArgs a) {
         for (uint32_t m0 = 0; m0 < du; m0 += WAVE) {
             const uint32_t mi = m0 + lane;
             const uint32_t id = mi < du ? adj[mi] : ID_NONE;
-            const int vr = id < N ? visit_first(vis, hmask, id, probes) : 0;
-            const bool fresh = vr != 0;
-            // a full table stops probing deeply: it forgets either way
-            if (__ballot(vr == 2)) probes = VIS_PROBES_FULL;
+            const bool fresh = id < N && visit_first(vis, hmask, id, probes) != 0;
             const uint64_t bm = __ballot(fresh);
             const int m = __popcll(bm);
             if (fresh) cid[lanes_below(bm)] = id;
@@ -329,6 +326,9 @@ __global__ void __launch_bounds__(256) greedy_search_kernel(SearchArgs a) {
             if (m == 0) continue;
             rows_dist<CPL, G>(a.base, dp, nch, qv, cid, m, ckey);
             evals += (uint32_t)m;
+            // more ids inserted than the table has slots: it forgets either
+            // way, so stop probing deeply
+            if (evals - L > a.H) probes = VIS_PROBES_FULL;
             wave_sync();
             uint64_t key = lane < m ? ckey[lane] : KEY_EMPTY;
             if (key < worst) {
