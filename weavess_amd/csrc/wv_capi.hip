@@ -317,6 +317,8 @@ int wv_index_create(wv_ctx *c, const float *base, uint32_t N, uint32_t d, wv_ind
     *out = nullptr;
     if (!base) return fail(c, WV_EINVAL, "base is null");
     if (N < 2 || d == 0) return fail(c, WV_EINVAL, "need N >= 2 and d >= 1 (N=%u d=%u)", N, d);
+    // the search keeps a flag in bit 31 of each pooled id
+    if (N > 0x7FFFFFFFu) return fail(c, WV_ENOTSUP, "N above 2^31 - 1");
     if (d > 1024) return fail(c, WV_ENOTSUP, "dimension %u above 1024", d);
     WV_HIP(c, hipSetDevice(c->device));
     wv_index *ix = new (std::nothrow) wv_index;

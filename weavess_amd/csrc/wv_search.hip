@@ -63,30 +63,55 @@ __device__ inline int visit_first(uint32_t *vis, uint32_t hmask, uint32_t id, in
     return 2;
 }
 
-// Merge c sorted keys src[0..c) (flags sflag[], or 1 when null) into the
-// sorted array arr[0..len) of capacity cap, in place: a key of arr moves
-// right by the number of smaller src keys, src[j] lands at j + the number of
-// arr keys <= it (so equal keys never collide), and keys pushed to slots
-// >= cap are dropped.  Keys are moved in 64-slot chunks from the tail
-// backwards (all reads of a chunk precede its writes), so slots before the
-// first insertion point are never touched.  1 <= c <= 4 * WAVE.  Returns the
-// final slot of src[0] (>= cap when it was dropped).
-__device__ inline int merge_into(uint64_t *arr, uint8_t *aflag, int len, int cap, const uint64_t *src,
-                                 const uint8_t *sflag, int c) {
+// Pool and buffer entries are keys with bit 31 (the top bit of the id word;
+// ids are below 2^31) set while the entry is unexpanded.  Comparisons use the
+// key with that bit cleared, so the (dist, id) order is unchanged, and a merge
+// moves the flag with its key.
+constexpr uint64_t FBIT = 1ull << 31;
+__device__ inline uint64_t canon(uint64_t e) { return e & ~FBIT; }
+__device__ inline uint32_t entry_id(uint64_t e) { return (uint32_t)e & 0x7FFFFFFFu; }
+
+// Number of entries of the sorted a[0..n) whose key is < key (<= key).
+__device__ inline int lower_bound_e(const uint64_t *a, int n, uint64_t key) {
+    int lo = 0, hi = n;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (canon(a[mid]) < key) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+__device__ inline int upper_bound_e(const uint64_t *a, int n, uint64_t key) {
+    int lo = 0, hi = n;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (canon(a[mid]) <= key) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// Merge c sorted entries src[0..c) into the sorted array arr[0..len) of
+// capacity cap, in place: an entry of arr moves right by the number of
+// smaller src keys, src[j] lands at j + the number of arr keys <= it (so
+// equal keys never collide), and entries pushed to slots >= cap are dropped.
+// With `mark`, the src entries are stored unexpanded (bit 31 set); without,
+// as they are.  Entries are moved in 64-slot chunks from the tail backwards
+// (all reads of a chunk precede its writes), so slots before the first
+// insertion point are never touched.  1 <= c <= 4 * WAVE.  Returns the final
+// slot of src[0] (>= cap when it was dropped).
+__device__ inline int merge_into(uint64_t *arr, int len, int cap, const uint64_t *src, int c, bool mark) {
     const int lane = lane_id();
     int cpos[4];
     uint64_t ck[4];
-    uint8_t cf[4];
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
         const int j = lane + r * WAVE;
         cpos[r] = cap;
         ck[r] = 0;
-        cf[r] = 0;
         if (j < c) {
-            ck[r] = src[j];
-            cf[r] = sflag ? sflag[j] : (uint8_t)1;
-            cpos[r] = j + upper_bound64(arr, len, ck[r]);
+            ck[r] = mark ? (src[j] | FBIT) : src[j];
+            cpos[r] = j + upper_bound_e(arr, len, canon(ck[r]));
         }
     }
     const int pos0 = __shfl(cpos[0], 0);  // lane 0 holds src[0]: j = 0
@@ -94,55 +119,47 @@ __device__ inline int merge_into(uint64_t *arr, uint8_t *aflag, int len, int cap
         const int p = t0 + lane;
         const bool mv = p >= pos0 && p < len;
         uint64_t pk = 0;
-        uint8_t pf = 0;
         int np = cap;
         if (mv) {
             pk = arr[p];
-            pf = aflag[p];
-            np = p + lower_bound64(src, c, pk);
+            np = p + lower_bound_e(src, c, canon(pk));
         }
         wave_sync();
-        if (mv && np < cap) {
-            arr[np] = pk;
-            aflag[np] = pf;
-        }
+        if (mv && np < cap) arr[np] = pk;
         wave_sync();
     }
 #pragma unroll
     for (int r = 0; r < 4; ++r)
-        if (cpos[r] < cap) {
-            arr[cpos[r]] = ck[r];
-            aflag[cpos[r]] = cf[r];
-        }
+        if (cpos[r] < cap) arr[cpos[r]] = ck[r];
     wave_sync();
     return pos0;
 }
 
-// First slot >= from of f[0..n) whose flag is set, or n.
-__device__ inline int first_flag(const uint8_t *f, int from, int n) {
+// First slot >= from of a[0..n) holding an unexpanded entry, or n.
+__device__ inline int first_flag(const uint64_t *a, int from, int n) {
     const int lane = lane_id();
     for (int t0 = from & ~(WAVE - 1); t0 < n; t0 += WAVE) {
         const int p = t0 + lane;
-        const uint64_t b = __ballot(p >= from && p < n && f[p]);
+        const uint64_t b = __ballot(p >= from && p < n && (a[p] & FBIT));
         if (b) return t0 + __ffsll((unsigned long long)b) - 1;
     }
     return n;
 }
 
-// The k-th smallest (0-based) key of the union of the sorted, disjoint key
+// The k-th smallest (0-based) key of the union of the sorted, disjoint entry
 // arrays a[0..na) and b[0..nb), k < na + nb: the split that takes i keys of b
 // and k+1-i of a, with i the largest value for which b[i-1] < a[k+1-i].
-// Every lane reads the same slots and returns the same value.
+// Every lane reads the same slots and returns the same (canonical) key.
 __device__ inline uint64_t kth_union(const uint64_t *a, int na, const uint64_t *b, int nb, int k) {
     int lo = max(0, k + 1 - na), hi = min(nb, k + 1);
     while (lo < hi) {
         const int i = (lo + hi + 1) >> 1;
         const int j = k + 1 - i;
-        if (j >= na || b[i - 1] < a[j]) lo = i;
+        if (j >= na || canon(b[i - 1]) < canon(a[j])) lo = i;
         else hi = i - 1;
     }
     const int j = k + 1 - lo;
-    const uint64_t x = lo > 0 ? b[lo - 1] : 0, y = j > 0 ? a[j - 1] : 0;
+    const uint64_t x = lo > 0 ? canon(b[lo - 1]) : 0, y = j > 0 ? canon(a[j - 1]) : 0;
     return x > y ? x : y;
 }
 
@@ -150,10 +167,9 @@ __device__ inline uint64_t kth_union(const uint64_t *a, int na, const uint64_t *
 // counter stream: L draws mod (N-L), sorted, made strictly increasing
 // (a'[i] = i + max_{j<=i}(a[j]-j), a prefix-max scan), rotated by one more
 // draw mod N.  The draws are sorted as (value, index) keys, 64 at a time,
-// merged into a[0..L) (aflag is scratch); on return a[i] holds entry i's id.
-// tmp: 64 keys of scratch.
-__device__ inline void gen_entries(uint64_t seed, uint32_t q, uint32_t L, uint32_t N, uint64_t *a,
-                                   uint8_t *aflag, uint64_t *tmp) {
+// merged into a[0..L) (index i < 2^31, so bit 31 stays clear); on return
+// a[i] holds entry i's id.  tmp: 64 keys of scratch.
+__device__ inline void gen_entries(uint64_t seed, uint32_t q, uint32_t L, uint32_t N, uint64_t *a, uint64_t *tmp) {
     const int lane = lane_id();
     const uint64_t s = stream_of(seed, TAG_ENTRY, q);
     const uint32_t mod = N - L;
@@ -164,7 +180,7 @@ __device__ inline void gen_entries(uint64_t seed, uint32_t q, uint32_t L, uint32
         const int cnt = (int)min((uint32_t)WAVE, L - c0);
         tmp[lane] = key;
         wave_sync();
-        merge_into(a, aflag, len, (int)L, tmp, nullptr, cnt);
+        merge_into(a, len, (int)L, tmp, cnt, false);
         len += cnt;
     }
     const int seg = (int)((L + WAVE - 1) / WAVE);
@@ -199,10 +215,8 @@ __global__ void __launch_bounds__(256) greedy_search_kernel(SearchArgs a) {
     uint64_t *pool = reinterpret_cast<uint64_t *>(my);         // P keys (P >= L, multiple of 64)
     uint64_t *ckey = pool + a.P;                                // 64
     uint32_t *cid = reinterpret_cast<uint32_t *>(ckey + WAVE);  // 64
-    uint32_t *vis = cid + WAVE;                                 // H
-    uint8_t *flag = reinterpret_cast<uint8_t *>(vis + a.H);     // P
-    uint64_t *buf = reinterpret_cast<uint64_t *>(flag + a.P);   // B (P is a multiple of 64)
-    uint8_t *bflag = reinterpret_cast<uint8_t *>(buf + a.B);    // B
+    uint32_t *vis = cid + WAVE;                                 // H (a power of two >= 64)
+    uint64_t *buf = reinterpret_cast<uint64_t *>(vis + a.H);    // B
 
     const uint32_t N = a.N, nch = a.nch, dp = a.dp;
     // per-query budgets were checked on the host when the launch was planned;
@@ -233,7 +247,7 @@ __global__ void __launch_bounds__(256) greedy_search_kernel(SearchArgs a) {
         }
         wave_sync();
     } else {
-        gen_entries(a.seed, q, L, N, pool, flag, ckey);
+        gen_entries(a.seed, q, L, N, pool, ckey);
     }
     {
         int len = 0;
@@ -247,14 +261,13 @@ __global__ void __launch_bounds__(256) greedy_search_kernel(SearchArgs a) {
             wave_sync();
             ckey[lane] = key;
             wave_sync();
-            merge_into(pool, flag, len, (int)L, ckey, nullptr, cnt);
+            merge_into(pool, len, (int)L, ckey, cnt, true);  // entries start unexpanded
             len += cnt;
         }
     }
     for (uint32_t i = L + lane; i < a.P; i += WAVE) pool[i] = KEY_EMPTY;
     const uint32_t hmask = a.H - 1;
     for (uint32_t i = lane; i < a.H; i += WAVE) vis[i] = ID_NONE;
-    for (uint32_t i = lane; i < a.P; i += WAVE) flag[i] = i < L ? 1 : 0;
     wave_sync();
 
     uint32_t evals = L, hops = 0;
@@ -262,21 +275,21 @@ __global__ void __launch_bounds__(256) greedy_search_kernel(SearchArgs a) {
     const int B = (int)a.B;    // buffer capacity; 0: merge straight into the pool
     int kp = 0;                // pool slots before kp are expanded
     int nb = 0, kb = 0;        // buffer length; buffer slots before kb are expanded
-    uint64_t worst = pool[Li - 1];  // the L-th key of pool + buffer
+    uint64_t worst = canon(pool[Li - 1]);  // the L-th key of pool + buffer
     // learned termination / traces: state features (identical on every lane)
     const bool watch = a.model != nullptr || a.trace != nullptr;
     const int kidx = (int)(kq < L ? kq : L) - 1;
     const float entry_d = watch ? key_dist(pool[0]) : 0.f;
     // k-th distance 1..4 expansions ago
-    float r0 = watch ? key_dist(pool[kidx]) : 0.f, r1 = r0, r2 = r0, r3 = r0;
+    float r0 = watch ? key_dist(pool[kidx]) : 0.f, r1 = r0, r2 = r0, r3 = r0;  // bit 31 is not in the distance
     uint32_t settle = 0;  // last expansion that changed the top-k
     int probes = VIS_PROBES;
     for (;;) {
         // U's first unexpanded key: the smaller of the pool's and the buffer's
-        const int kk = first_flag(flag, kp, Li);
-        const int jb = nb ? first_flag(bflag, kb, nb) : 0;
-        const uint64_t ka = kk < Li ? pool[kk] : KEY_EMPTY;
-        const uint64_t kbk = jb < nb ? buf[jb] : KEY_EMPTY;
+        const int kk = first_flag(pool, kp, Li);
+        const int jb = nb ? first_flag(buf, kb, nb) : 0;
+        const uint64_t ka = kk < Li ? canon(pool[kk]) : KEY_EMPTY;
+        const uint64_t kbk = jb < nb ? canon(buf[jb]) : KEY_EMPTY;
         const bool from_buf = kbk < ka;
         const uint64_t cur = from_buf ? kbk : ka;
         // cur is in U iff it is not above U's L-th key (`worst` is kept exact)
@@ -284,8 +297,8 @@ __global__ void __launch_bounds__(256) greedy_search_kernel(SearchArgs a) {
         if (watch && hops > 0) {
             // state after `hops` expansions
             float f[MLP_IN];
-            const uint64_t kth = nb ? kth_union(pool, Li, buf, nb, kidx) : pool[kidx];
-            const uint64_t best = (nb && buf[0] < pool[0]) ? buf[0] : pool[0];
+            const uint64_t kth = nb ? kth_union(pool, Li, buf, nb, kidx) : canon(pool[kidx]);
+            const uint64_t best = (nb && canon(buf[0]) < canon(pool[0])) ? canon(buf[0]) : canon(pool[0]);
             term_features((float)hops, key_dist(kth), key_dist(best), entry_d, r3, (float)(hops - settle),
                           (float)L, (float)kq, f);
             r3 = r2;
@@ -303,11 +316,11 @@ __global__ void __launch_bounds__(256) greedy_search_kernel(SearchArgs a) {
                 break;
         }
         if (done) break;
-        const uint32_t u = key_id(cur);
+        const uint32_t u = key_id(cur);  // cur is canonical
         wave_sync();
-        if (lane == 0) {
-            if (from_buf) bflag[jb] = 0;
-            else flag[kk] = 0;
+        if (lane == 0) {  // expanded: store the key without bit 31
+            if (from_buf) buf[jb] = cur;
+            else pool[kk] = cur;
         }
         if (from_buf) kb = jb + 1;
         else kp = kk + 1;
@@ -339,16 +352,16 @@ __global__ void __launch_bounds__(256) greedy_search_kernel(SearchArgs a) {
                 while (plo < phi || blo < bhi) {
                     if (plo < phi) {
                         const int mid = (plo + phi) >> 1;
-                        if (pool[mid] < key) plo = mid + 1;
+                        if (canon(pool[mid]) < key) plo = mid + 1;
                         else phi = mid;
                     }
                     if (blo < bhi) {
                         const int mid = (blo + bhi) >> 1;
-                        if (buf[mid] < key) blo = mid + 1;
+                        if (canon(buf[mid]) < key) blo = mid + 1;
                         else bhi = mid;
                     }
                 }
-                if (pool[plo] == key || (blo < nb && buf[blo] == key)) key = KEY_EMPTY;
+                if (canon(pool[plo]) == key || (blo < nb && canon(buf[blo]) == key)) key = KEY_EMPTY;
             } else {
                 key = KEY_EMPTY;
             }
@@ -368,30 +381,30 @@ __global__ void __launch_bounds__(256) greedy_search_kernel(SearchArgs a) {
             if (lane < c) ckey[lane] = key;
             wave_sync();
             if (B == 0) {
-                const int pos0 = merge_into(pool, flag, Li, Li, ckey, nullptr, c);
+                const int pos0 = merge_into(pool, Li, Li, ckey, c, true);
                 if (pos0 < kp) kp = pos0;
                 if (pos0 <= kidx) settle = hops;  // a candidate entered the top-k
-                worst = pool[Li - 1];
+                worst = canon(pool[Li - 1]);
             } else {
                 if (nb + c > B) {  // full: merge the buffer into the pool
-                    const int pos0 = merge_into(pool, flag, Li, Li, buf, bflag, nb);
+                    const int pos0 = merge_into(pool, Li, Li, buf, nb, false);
                     if (pos0 < kp) kp = pos0;
                     nb = 0;
                     kb = 0;
                 }
-                const int bp = merge_into(buf, bflag, nb, B, ckey, nullptr, c);
+                const int bp = merge_into(buf, nb, B, ckey, c, true);
                 nb += c;
                 if (bp < kb) kb = bp;
-                if (watch && bp + lower_bound64(pool, Li, ckey[0]) <= kidx) settle = hops;
+                if (watch && bp + lower_bound_e(pool, Li, ckey[0]) <= kidx) settle = hops;
                 worst = kth_union(pool, Li, buf, nb, Li - 1);
             }
         }
     }
-    if (nb) merge_into(pool, flag, Li, Li, buf, bflag, nb);
+    if (nb) merge_into(pool, Li, Li, buf, nb, false);
 
     for (uint32_t i = lane; i < a.k_stride; i += WAVE) {
         const bool ok = i < kq;
-        a.out_ids[(size_t)q * a.k_stride + i] = ok ? key_id(pool[i]) : ID_NONE;
+        a.out_ids[(size_t)q * a.k_stride + i] = ok ? entry_id(pool[i]) : ID_NONE;
         if (a.out_dist) a.out_dist[(size_t)q * a.k_stride + i] = ok ? key_dist(pool[i]) : -1.0f;
     }
     if (lane == 0) {
@@ -419,7 +432,7 @@ hipError_t launch_t(const SearchArgs &a, int waves_per_block, hipStream_t s) {
 }  // namespace
 
 size_t search_wave_lds(uint32_t P, uint32_t H, uint32_t B) {
-    size_t b = (size_t)P * 8 + WAVE * 8 + WAVE * 4 + (size_t)H * 4 + (size_t)P + (size_t)B * 9;
+    size_t b = (size_t)P * 8 + WAVE * 8 + WAVE * 4 + (size_t)H * 4 + (size_t)B * 8;
     return (b + 15) & ~(size_t)15;
 }
 
