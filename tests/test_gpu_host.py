@@ -42,3 +42,82 @@ def test_pipeline_build_search_matches_c_abi(ctx, tmp_path, small_int):
     out = _run(["kgraph", pre, "search"], {**env, "WEAVESS_ACC_SET": "0.9"})
     m = re.search(r"QPS: ([0-9.e+]+) recall: ([0-9.]+) L: (\d+)", out)
     assert float(m.group(2)) >= 0.9
+
+
+def test_pipeline_centroid_entry(ctx, tmp_path, small_int):
+    """WEAVESS_ENTRY=centroid: ep is the exact nearest base point to the
+    centroid, and search from ep's neighbours plus the counter-stream fill
+    gives the oracle's recall with the same explicit entry sets."""
+    from weavess_amd.entries import centroid, centroid_entries, centroid_ep
+    base, q = small_int
+    pre = str(tmp_path / "c1")
+    io.write_fvecs(pre + "_base.fvecs", base)
+    io.write_fvecs(pre + "_query.fvecs", q)
+    gt, _ = O.exact_knn(base, q, 10)
+    io.write_ivecs(pre + "_groundtruth.ivecs", gt)
+    env = dict(WEAVESS_K="24", WEAVESS_L="32", WEAVESS_ITER="6", WEAVESS_S="10", WEAVESS_R="32",
+               WEAVESS_SEED="5", WEAVESS_ENTRY="centroid")
+    _run(["kgraph", pre, "build"], env)
+    gids, gdeg = io.read_graph(pre + ".kgraph.graph")
+    out = _run(["kgraph", pre, "search", "40"], env)
+    ep = int(re.search(r"ep_ (\d+)", out).group(1))
+    d2 = ((base.astype(np.float64) - centroid(base).astype(np.float64)) ** 2).sum(axis=1)
+    assert d2[ep] == d2.min()
+    assert ep == centroid_ep(ctx, base)
+    ent = centroid_entries(ep, gids, gdeg, len(q), 40, len(base), seed=5)
+    assert np.array_equal(ent[:, :min(40, int(gdeg[ep]))],
+                          np.broadcast_to(gids[ep, :min(40, int(gdeg[ep]))], (len(q), min(40, int(gdeg[ep])))))
+    res, *_ = O.search_greedy(gids, gdeg, base, q, 40, 10, entries=ent)
+    rec = float(re.search(r"10 NN accuracy: ([0-9.]+)", out).group(1))
+    assert rec == pytest.approx(O.recall(res, gt, 10), abs=1e-6)
+    # and the C ABI with the same entries returns the oracle's ids
+    from weavess_amd import Index
+    ix = Index(ctx, base)
+    ix.set_graph(gids, gdeg)
+    gres, *_ = ix.search(q, 40, 10, entries=ent)
+    ix.close()
+    assert np.array_equal(gres, res)
+
+
+def test_pipeline_nsg_and_ssg_entries(ctx, tmp_path, small_int):
+    """A build written as an NSG file carries the centroid entry point and a
+    search of it reuses that ep; an SSG file's entry set eps leads every
+    query's entries (SubCentroid).  Both give the oracle's recall with the
+    same explicit entry sets."""
+    from weavess_amd.entries import centroid_entries, centroid_ep, subcentroid_entries
+    base, q = small_int
+    pre = str(tmp_path / "c1")
+    io.write_fvecs(pre + "_base.fvecs", base)
+    io.write_fvecs(pre + "_query.fvecs", q)
+    gt, _ = O.exact_knn(base, q, 10)
+    io.write_ivecs(pre + "_groundtruth.ivecs", gt)
+    gn = pre + ".nsg.graph"
+    env = dict(WEAVESS_K="24", WEAVESS_L="32", WEAVESS_ITER="6", WEAVESS_S="10", WEAVESS_R="32",
+               WEAVESS_SEED="9", WEAVESS_GRAPH=gn, WEAVESS_FORMAT="nsg")
+    _run(["kgraph", pre, "build"], env)
+    gids, gdeg, ep = io.read_graph(gn, fmt="nsg")
+    assert ep == centroid_ep(ctx, base)
+    out = _run(["kgraph", pre, "search", "48"], {**env, "WEAVESS_ENTRY": "centroid"})
+    assert "__ENTRY : CENTROID__" not in out  # ep came from the file
+    ent = centroid_entries(ep, gids, gdeg, len(q), 48, len(base), seed=9)
+    res, *_ = O.search_greedy(gids, gdeg, base, q, 48, 10, entries=ent)
+    assert float(re.search(r"10 NN accuracy: ([0-9.]+)", out).group(1)) == pytest.approx(
+        O.recall(res, gt, 10), abs=1e-6)
+
+    gs = pre + ".ssg.graph"
+    eps = np.random.default_rng(4).choice(len(base), 8, replace=False).astype(np.uint32)
+    io.write_graph(gs, gids, gdeg, fmt="ssg", ep=eps)
+    senv = {**env, "WEAVESS_GRAPH": gs, "WEAVESS_FORMAT": "ssg", "WEAVESS_ENTRY": "subcentroid"}
+    out = _run(["kgraph", pre, "search", "32"], senv)
+    ent = subcentroid_entries(eps, len(q), 32, len(base), seed=9)
+    res, *_ = O.search_greedy(gids, gdeg, base, q, 32, 10, entries=ent)
+    assert float(re.search(r"10 NN accuracy: ([0-9.]+)", out).group(1)) == pytest.approx(
+        O.recall(res, gt, 10), abs=1e-6)
+    # SET_RECALL's L sweep runs with the same entry component
+    out = _run(["kgraph", pre, "search"], {**senv, "WEAVESS_ACC_SET": "0.9"})
+    assert float(re.search(r"recall: ([0-9.]+)", out).group(1)) >= 0.9
+    # more entry points than L_search is an error, as the reference asserts
+    io.write_graph(gs, gids, gdeg, fmt="ssg", ep=np.arange(12, dtype=np.uint32))
+    r = subprocess.run([os.path.join(HOST, "weavess_main"), "kgraph", pre, "search", "10"], capture_output=True,
+                       text=True, env={**os.environ, **senv}, timeout=300)
+    assert r.returncode == 2 and "more entry points" in r.stderr

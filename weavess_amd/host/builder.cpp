@@ -77,8 +77,12 @@ double IndexBuilder::accuracy(unsigned K) {
 double IndexBuilder::run_queries(ComponentSearchEntry *a, ComponentSearchRoute *b, bool batch, double &secs) {
     const unsigned nq = index_->getQueryLen();
     auto t0 = std::chrono::high_resolution_clock::now();
-    if (batch) {
-        b->RouteBatch(nq, nullptr, res_);
+    if (batch && dynamic_cast<ComponentSearchEntryRand *>(a)) {
+        b->RouteBatch(nq, nullptr, res_);  // entries generated on the device
+    } else if (batch) {
+        std::vector<std::vector<Index::Neighbor>> pools(nq);
+        for (unsigned i = 0; i < nq; ++i) a->SearchEntryInner(i, pools[i]);
+        b->RouteBatch(nq, &pools, res_);
     } else {
         res_.assign(nq, {});
         for (unsigned i = 0; i < nq; ++i) {
@@ -96,7 +100,9 @@ IndexBuilder *IndexBuilder::search(TYPE entry_type, TYPE route_type, TYPE L_type
     std::cout << "__SEARCH__" << std::endl;
     const unsigned K = 10;  // src/builder.cpp:190-192
     index_->getParam().set<unsigned>("K_search", K);
-    if (entry_type != SEARCH_ENTRY_RAND) throw std::invalid_argument("__SEARCH ENTRY : WRONG TYPE__");
+    if (entry_type != SEARCH_ENTRY_RAND && entry_type != SEARCH_ENTRY_CENTROID &&
+        entry_type != SEARCH_ENTRY_SUB_CENTROID)
+        throw std::invalid_argument("__SEARCH ENTRY : WRONG TYPE__");
     if (route_type != ROUTER_GREEDY && route_type != ROUTER_GREEDY_HIP)
         throw std::invalid_argument("__ROUTER : WRONG TYPE__");
     if (index_->getLoadGraph().empty() && !index_->getFinalGraph().empty()) {
@@ -109,7 +115,16 @@ IndexBuilder *IndexBuilder::search(TYPE entry_type, TYPE route_type, TYPE L_type
         }
         index_->markLoadGraphChanged();
     }
-    ComponentSearchEntryRand a(index_);
+    ComponentSearchEntryRand a_rand(index_);
+    ComponentSearchEntryCentroid a_cent(index_);
+    ComponentSearchEntrySubCentroid a_sub(index_);
+    ComponentSearchEntry &a = entry_type == SEARCH_ENTRY_CENTROID       ? static_cast<ComponentSearchEntry &>(a_cent)
+                              : entry_type == SEARCH_ENTRY_SUB_CENTROID ? static_cast<ComponentSearchEntry &>(a_sub)
+                                                                        : static_cast<ComponentSearchEntry &>(a_rand);
+    if (entry_type == SEARCH_ENTRY_CENTROID && !index_->ep_set_) {
+        std::cout << "__ENTRY : CENTROID__" << std::endl;
+        ComponentRefineEntryCentroidHIP(index_).EntryInner();
+    }
     ComponentSearchRouteGreedyHIP b(index_);
     const bool batch = !index_->getParam().has("per_query") || index_->getParam().get<unsigned>("per_query") == 0;
     const unsigned nq = index_->getQueryLen();
@@ -184,9 +199,18 @@ IndexBuilder *IndexBuilder::search(TYPE entry_type, TYPE route_type, TYPE L_type
 }
 
 IndexBuilder *IndexBuilder::save_graph(TYPE type, const char *graph_file) {
-    if (type != INDEX_KGRAPH) throw std::invalid_argument("only INDEX_KGRAPH graphs are written here");
+    if (type != INDEX_KGRAPH && type != INDEX_NSG && type != INDEX_VAMANA && type != INDEX_SSG)
+        throw std::invalid_argument("graph type not written here");
     std::ofstream out(graph_file, std::ios::binary);
     if (!out.is_open()) throw std::runtime_error(std::string("cannot write ") + graph_file);
+    if (type == INDEX_NSG || type == INDEX_VAMANA) {  // src/builder.cpp:920-921
+        if (!index_->ep_set_) throw std::invalid_argument("no entry point to write");
+        out.write(reinterpret_cast<const char *>(&index_->ep_), sizeof(unsigned));
+    } else if (type == INDEX_SSG) {  // src/builder.cpp:922-925
+        const unsigned n_ep = (unsigned)index_->eps_.size();
+        out.write(reinterpret_cast<const char *>(&n_ep), sizeof n_ep);
+        out.write(reinterpret_cast<const char *>(index_->eps_.data()), (std::streamsize)n_ep * sizeof(unsigned));
+    }
     for (const auto &row : index_->getFinalGraph()) {  // src/builder.cpp:1068-1076
         const unsigned GK = (unsigned)row.size();
         out.write(reinterpret_cast<const char *>(&GK), sizeof GK);
@@ -197,8 +221,25 @@ IndexBuilder *IndexBuilder::save_graph(TYPE type, const char *graph_file) {
 }
 
 IndexBuilder *IndexBuilder::load_graph(TYPE type, const char *graph_file) {
+    if (type != INDEX_KGRAPH && type != INDEX_EXACT_KNNG && type != INDEX_NSG && type != INDEX_VAMANA &&
+        type != INDEX_SSG)
+        throw std::invalid_argument("graph type not read here");
     std::ifstream in(graph_file, std::ios::binary);
     if (!in.is_open()) throw std::runtime_error(std::string("load graph error: ") + graph_file);
+    if (type == INDEX_NSG || type == INDEX_VAMANA) {  // src/builder.cpp:1089-1090
+        unsigned ep = 0;
+        in.read(reinterpret_cast<char *>(&ep), sizeof ep);
+        if (!in) throw std::runtime_error("truncated graph header");
+        index_->ep_ = ep;
+        index_->ep_set_ = true;
+    } else if (type == INDEX_SSG) {  // src/builder.cpp:1091-1095
+        unsigned n_ep = 0;
+        in.read(reinterpret_cast<char *>(&n_ep), sizeof n_ep);
+        if (!in || n_ep > index_->getBaseLen()) throw std::runtime_error("bad graph header");
+        index_->eps_.resize(n_ep);
+        in.read(reinterpret_cast<char *>(index_->eps_.data()), (std::streamsize)n_ep * sizeof(unsigned));
+        if (!in) throw std::runtime_error("truncated graph header");
+    }
     auto &dst = type == INDEX_EXACT_KNNG ? index_->getExactGraph() : index_->getLoadGraph();
     dst.clear();
     while (true) {  // src/builder.cpp:1301-1308
@@ -250,6 +291,8 @@ IndexBuilder *IndexBuilder::print_index_info(TYPE type) {
     }
     std::cout << "nodes: " << G.size() << " edges: " << edges << " max degree: " << mx << " min degree: " << mn
               << std::endl;
+    if ((type == INDEX_NSG || type == INDEX_VAMANA) && index_->ep_set_) std::cout << "ep: " << index_->ep_ << std::endl;
+    if (type == INDEX_SSG) std::cout << "eps: " << index_->eps_.size() << std::endl;
     if (index_->getParam().has("exact_knng_path") || !index_->getExactGraph().empty()) graph_quality(type);
     return this;
 }

@@ -1,6 +1,7 @@
 // components.cpp -- the HIP stages behind the reference's plug-in hooks.
 #include <algorithm>
 #include <fstream>
+#include <iostream>
 #include <stdexcept>
 
 #include "weavess/component.h"
@@ -18,13 +19,44 @@ inline uint64_t fmix64(uint64_t z) {  // splitmix64 finaliser
 }
 constexpr uint64_t GOLDEN = 0x9E3779B97F4A7C15ull;
 constexpr uint64_t TAG_ENTRY = 0x454E5452ull;
+constexpr uint64_t TAG_CENTROID = 0x43454E54ull;     // "CENT": centroid-entry fill
+constexpr uint64_t TAG_SUBCENTROID = 0x53554243ull;  // "SUBC": sub-centroid-entry fill
+uint64_t stream_of(uint64_t seed, uint64_t tag, uint64_t idx) {
+    uint64_t s = fmix64(seed + GOLDEN);
+    s = fmix64(s ^ (tag * GOLDEN));
+    return fmix64(s ^ (idx + 0x632BE59BD9B4E019ull));
+}
+
+// L distinct entry ids: the ids of `head` (below N, first occurrence, at
+// most L of them), then draws t mod N, t = 0, 1, ..., of the query's
+// counter stream, skipping ids already taken.
+void head_then_fill(const std::vector<unsigned> &head, uint64_t stream, unsigned L, unsigned N,
+                    std::vector<Index::Neighbor> &pool) {
+    if (L > N) throw std::invalid_argument("L_search above the base size");
+    std::vector<unsigned> ids;
+    ids.reserve(L);
+    std::vector<bool> taken(N, false);
+    for (unsigned nb : head) {
+        if (ids.size() == L) break;
+        if (nb < N && !taken[nb]) {
+            taken[nb] = true;
+            ids.push_back(nb);
+        }
+    }
+    for (uint64_t t = 0; ids.size() < L; ++t) {
+        const unsigned id = (unsigned)((uint32_t)(fmix64(stream + (t + 1) * GOLDEN) >> 32) % N);
+        if (taken[id]) continue;
+        taken[id] = true;
+        ids.push_back(id);
+    }
+    pool.resize(L + 1);
+    for (unsigned i = 0; i < L; ++i) pool[i] = Index::Neighbor(ids[i], 0.0f, true);
+}
 }  // namespace
 
 void GenRandomStream(uint64_t seed, uint64_t tag, uint64_t idx, unsigned *addr, unsigned size, unsigned N) {
     if (size >= N) throw std::invalid_argument("GenRandom needs size < N");
-    uint64_t s = fmix64(seed + GOLDEN);
-    s = fmix64(s ^ (tag * GOLDEN));
-    s = fmix64(s ^ (idx + 0x632BE59BD9B4E019ull));
+    const uint64_t s = stream_of(seed, tag, idx);
     auto draw = [s](uint64_t k) { return (uint32_t)(fmix64(s + (k + 1) * GOLDEN) >> 32); };
     for (unsigned i = 0; i < size; ++i) addr[i] = draw(i) % (N - size);
     std::sort(addr, addr + size);
@@ -151,6 +183,38 @@ void ComponentSearchEntryRand::SearchEntryInner(unsigned query, std::vector<Inde
     for (unsigned i = 0; i < L; ++i) pool[i] = Index::Neighbor(ids[i], 0.0f, true);
 }
 
+void ComponentRefineEntryCentroidHIP::EntryInner() {
+    const unsigned N = index->getBaseLen(), d = index->getBaseDim();
+    std::vector<double> sum(d, 0.0);
+    const float *b = index->getBaseData();
+    for (size_t i = 0; i < N; ++i)
+        for (unsigned j = 0; j < d; ++j) sum[j] += b[i * d + j];
+    std::vector<float> center(d);
+    for (unsigned j = 0; j < d; ++j) center[j] = (float)(sum[j] / N);
+    uint32_t id = 0;
+    double dist = 0.0;
+    index->check(wv_exact_knn(index->ctx(), b, N, center.data(), 1, d, 1, -1, &id, &dist, nullptr));
+    index->ep_ = id;
+    index->ep_set_ = true;
+    std::cout << "ep_ " << index->ep_ << std::endl;
+}
+
+void ComponentSearchEntryCentroid::SearchEntryInner(unsigned query, std::vector<Index::Neighbor> &pool) {
+    const unsigned L = index->getParam().get<unsigned>("L_search");
+    if (!index->ep_set_)
+        throw std::invalid_argument("no entry point: load an NSG/Vamana graph or run ComponentRefineEntryCentroidHIP");
+    const auto &lg = index->getLoadGraph();
+    static const std::vector<unsigned> none;
+    head_then_fill(index->ep_ < lg.size() ? lg[index->ep_] : none, stream_of(index->seed(), TAG_CENTROID, query), L,
+                   index->getBaseLen(), pool);
+}
+
+void ComponentSearchEntrySubCentroid::SearchEntryInner(unsigned query, std::vector<Index::Neighbor> &pool) {
+    const unsigned L = index->getParam().get<unsigned>("L_search");
+    if (index->eps_.size() > L) throw std::invalid_argument("more entry points (eps_) than L_search");
+    head_then_fill(index->eps_, stream_of(index->seed(), TAG_SUBCENTROID, query), L, index->getBaseLen(), pool);
+}
+
 void ComponentSearchRoute::RouteBatch(unsigned nq, const std::vector<std::vector<Index::Neighbor>> *pools,
                                       std::vector<std::vector<unsigned>> &res) {
     if (!pools) throw std::invalid_argument("RouteBatch needs entry pools");
@@ -183,17 +247,23 @@ void ComponentSearchRouteGreedyHIP::RouteInner(unsigned query, std::vector<Index
 
 void ComponentSearchRouteGreedyHIP::RouteBatch(unsigned nq, const std::vector<std::vector<Index::Neighbor>> *pools,
                                                std::vector<std::vector<unsigned>> &res) {
-    if (pools) {  // caller-supplied entries: per-query path
-        ComponentSearchRoute::RouteBatch(nq, pools, res);
-        return;
-    }
     const unsigned L = index->getParam().get<unsigned>("L_search");
     const unsigned K = index->getParam().get<unsigned>("K_search");
     index->uploadLoadGraph();
+    std::vector<uint32_t> ent;
+    if (pools) {  // caller-supplied entries: one launch with explicit entry sets
+        if (pools->size() < nq) throw std::invalid_argument("fewer entry pools than queries");
+        ent.resize((size_t)nq * L);
+        for (unsigned i = 0; i < nq; ++i) {
+            if ((*pools)[i].size() < L) throw std::invalid_argument("entry pool shorter than L_search");
+            for (unsigned j = 0; j < L; ++j) ent[(size_t)i * L + j] = (*pools)[i][j].id;
+        }
+    }
     std::vector<uint32_t> ids((size_t)nq * K), ev(nq), hp(nq);
     wv_search_stats st{};
-    index->check(wv_index_search(index->deviceIndex(), index->getQueryData(), nq, nullptr, index->seed(), L, K,
-                                 nullptr, nullptr, K, ids.data(), nullptr, ev.data(), hp.data(), &st));
+    index->check(wv_index_search(index->deviceIndex(), index->getQueryData(), nq, pools ? ent.data() : nullptr,
+                                 index->seed(), L, K, nullptr, nullptr, K, ids.data(), nullptr, ev.data(), hp.data(),
+                                 &st));
     res.assign(nq, {});
     for (unsigned i = 0; i < nq; ++i) res[i].assign(ids.begin() + (size_t)i * K, ids.begin() + (size_t)(i + 1) * K);
     index->addDistCount(st.dist_evals);

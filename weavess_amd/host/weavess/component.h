@@ -81,6 +81,41 @@ public:
     void SearchEntryInner(unsigned query, std::vector<Index::Neighbor> &pool) override;
 };
 
+// ComponentRefineEntryCentroid (src/component_entry.cpp:9-28): ep_ = the
+// node nearest to the mean of the base.  The reference finds it with an
+// approximate greedy search from a random node; here it is the exact FP64
+// nearest neighbour of the centroid, found on the GPU (the limit of that
+// search).  The centroid is summed in double and rounded to float.
+class ComponentRefineEntryCentroidHIP : public Component {
+public:
+    explicit ComponentRefineEntryCentroidHIP(Index *index) : Component(index) {}
+    void EntryInner();
+};
+
+// SearchEntryCentroid (src/component_search_entry.cpp:33-67): the first
+// min(L, degree) neighbours of ep_ in the load graph, then distinct random
+// ids until L.  The reference draws the fill with rand(); here it comes from
+// the query's counter stream (draw t mod N, t = 0, 1, ..., skipping ids
+// already taken).
+class ComponentSearchEntryCentroid : public ComponentSearchEntry {
+public:
+    explicit ComponentSearchEntryCentroid(Index *index) : ComponentSearchEntry(index) {}
+    void SearchEntryInner(unsigned query, std::vector<Index::Neighbor> &pool) override;
+};
+
+// SearchEntrySubCentroid (src/component_search_entry.cpp:70-99): the entry
+// set eps_ first, then random ids until L.  The reference overwrites the
+// head of a GenRandom set with eps_, so an id can appear twice, and it
+// reads each base row one float late (its `x++`); here the ids are
+// distinct (eps_ deduplicated, the fill drawn like the centroid entry's
+// from its own stream) and distances are plain.  eps_ longer than L is
+// rejected, as the reference's assert does.
+class ComponentSearchEntrySubCentroid : public ComponentSearchEntry {
+public:
+    explicit ComponentSearchEntrySubCentroid(Index *index) : ComponentSearchEntry(index) {}
+    void SearchEntryInner(unsigned query, std::vector<Index::Neighbor> &pool) override;
+};
+
 class ComponentSearchRoute : public Component {
 public:
     explicit ComponentSearchRoute(Index *index) : Component(index) {}
@@ -93,9 +128,9 @@ public:
 
 // Greedy best-first search on the GPU (src/component_route.cpp:9-53).
 // RouteInner takes its entries from `pool` and writes the final sorted pool
-// back into it; RouteBatch searches every query in one launch with entries
-// generated on the device from the same seeded stream as
-// ComponentSearchEntryRand.
+// back into it; RouteBatch searches every query in one launch, with the
+// entries of `pools` when given, else with entries generated on the device
+// from the same seeded stream as ComponentSearchEntryRand.
 class ComponentSearchRouteGreedyHIP : public ComponentSearchRoute {
 public:
     explicit ComponentSearchRouteGreedyHIP(Index *index) : ComponentSearchRoute(index) {}

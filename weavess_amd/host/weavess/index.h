@@ -80,6 +80,13 @@ public:
     LoadGraph &getLoadGraph() { return load_graph_; }
     LoadGraph &getExactGraph() { return exact_graph_; }
 
+    // navigating (entry) point, ep_ of include/weavess/index.h; set by
+    // ComponentRefineEntryCentroidHIP or read from an NSG/Vamana graph file
+    unsigned ep_ = 0;
+    bool ep_set_ = false;
+    // entry set eps_ of include/weavess/index.h, read from an SSG graph file
+    std::vector<unsigned> eps_;
+
     // counters (src/component_search_entry.cpp:25, src/component_route.cpp:24,34)
     void addDistCount(uint64_t n = 1) { dist_count_ += n; }
     void addHopCount(uint64_t n = 1) { hop_count_ += n; }

@@ -7,6 +7,9 @@ namespace weavess {
 enum TYPE {
     INDEX_KGRAPH,
     INDEX_EXACT_KNNG,
+    INDEX_NSG,              // graph file headed by ep_ (src/builder.cpp:920-921)
+    INDEX_VAMANA,           // same layout as INDEX_NSG
+    INDEX_SSG,              // graph file headed by n_ep and eps_ (src/builder.cpp:922-925)
 
     INIT_RANDOM,            // ComponentInitRandom (host, reference semantics)
     INIT_RANDOM_HIP,        // random S-NN initialisation on the GPU
@@ -17,6 +20,8 @@ enum TYPE {
     REFINE_NN_DESCENT_HIP_FP8,
 
     SEARCH_ENTRY_RAND,      // L random entries per query (GenRandom)
+    SEARCH_ENTRY_CENTROID,  // the entry point's neighbours, then distinct random ids
+    SEARCH_ENTRY_SUB_CENTROID,  // the entry set eps_, then distinct random ids
 
     L_SEARCH_ASCEND, L_SEARCH_SET_RECALL, L_SEARCH_ASSIGN,
 

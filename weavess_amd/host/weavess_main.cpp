@@ -8,8 +8,13 @@
 // overrides).  K, L, ITER, S, R come from the per-dataset table below (keyed
 // by the prefix's file name) and can be overridden with WEAVESS_K, _L,
 // _ITER, _S, _R; WEAVESS_SEED, WEAVESS_DTYPE (f32|bf16|fp8) and
-// WEAVESS_EXACT (exact-kNN graph for "info") are read too.  Without
-// L_search, search tunes L to WEAVESS_ACC_SET (default 0.95) recall.
+// WEAVESS_EXACT (exact-kNN graph for "info") are read too;
+// WEAVESS_ENTRY=centroid|subcentroid searches from the centroid entry point
+// or the entry set eps_ instead of random entries; WEAVESS_FORMAT=nsg|ssg
+// reads and writes graph files headed by ep_ (NSG/Vamana layout) or eps_
+// (SSG layout) instead of the plain KGraph layout.  A build with
+// WEAVESS_FORMAT=nsg finds the centroid entry point and writes it.  Without L_search, search tunes L to WEAVESS_ACC_SET
+// (default 0.95) recall.
 #include <cstdlib>
 #include <cstring>
 #include <iostream>
@@ -17,6 +22,7 @@
 #include <string>
 
 #include "weavess/builder.h"
+#include "weavess/component.h"
 
 namespace {
 
@@ -68,28 +74,38 @@ int main(int argc, char **argv) {
     const weavess::TYPE refine = dtype == "bf16"  ? weavess::REFINE_NN_DESCENT_HIP_BF16
                                  : dtype == "fp8" ? weavess::REFINE_NN_DESCENT_HIP_FP8
                                                   : weavess::REFINE_NN_DESCENT_HIP;
+    const std::string ent = std::getenv("WEAVESS_ENTRY") ? std::getenv("WEAVESS_ENTRY") : "rand";
+    const weavess::TYPE entry = ent == "centroid"      ? weavess::SEARCH_ENTRY_CENTROID
+                                : ent == "subcentroid" ? weavess::SEARCH_ENTRY_SUB_CENTROID
+                                                       : weavess::SEARCH_ENTRY_RAND;
+    const std::string fmt = std::getenv("WEAVESS_FORMAT") ? std::getenv("WEAVESS_FORMAT") : "kgraph";
+    const weavess::TYPE gtype = fmt == "nsg" ? weavess::INDEX_NSG : fmt == "ssg" ? weavess::INDEX_SSG
+                                                                                 : weavess::INDEX_KGRAPH;
     try {
         weavess::IndexBuilder builder(8);
         if (exc == "build") {
+            if (gtype == weavess::INDEX_SSG) throw std::invalid_argument("SSG graphs are not built here");
             builder.load(base.c_str(), query.c_str(), ground.c_str(), P)->init(weavess::INIT_RANDOM);
             std::cout << "Init cost: " << builder.GetBuildTime().count() << std::endl;
-            builder.refine(refine)->save_graph(weavess::INDEX_KGRAPH, graph.c_str());
+            builder.refine(refine);
+            if (gtype == weavess::INDEX_NSG) weavess::ComponentRefineEntryCentroidHIP(builder.index()).EntryInner();
+            builder.save_graph(gtype, graph.c_str());
             std::cout << "Build cost: " << builder.GetBuildTime().count() << std::endl;
         } else if (exc == "search") {
-            builder.load(base.c_str(), query.c_str(), ground.c_str(), P)->load_graph(weavess::INDEX_KGRAPH, graph.c_str());
+            builder.load(base.c_str(), query.c_str(), ground.c_str(), P)->load_graph(gtype, graph.c_str());
             if (argc == 5) {
                 builder.index()->getParam().set<unsigned>("L_search", (unsigned)std::atoi(argv[4]));
-                builder.search(weavess::SEARCH_ENTRY_RAND, weavess::ROUTER_GREEDY_HIP, weavess::L_SEARCH_ASSIGN);
+                builder.search(entry, weavess::ROUTER_GREEDY_HIP, weavess::L_SEARCH_ASSIGN);
             } else {
-                builder.search(weavess::SEARCH_ENTRY_RAND, weavess::ROUTER_GREEDY_HIP, weavess::L_SEARCH_SET_RECALL);
+                builder.search(entry, weavess::ROUTER_GREEDY_HIP, weavess::L_SEARCH_SET_RECALL);
             }
             std::cout << "QPS: " << builder.last_qps() << " recall: " << builder.last_recall()
                       << " L: " << builder.last_L() << std::endl;
             builder.peak_memory_footprint();
         } else if (exc == "info") {
             builder.load(base.c_str(), query.c_str(), ground.c_str(), P)
-                ->load_graph(weavess::INDEX_KGRAPH, graph.c_str())
-                ->print_index_info(weavess::INDEX_KGRAPH);
+                ->load_graph(gtype, graph.c_str())
+                ->print_index_info(gtype);
         } else {
             std::cout << "exc_type input error!" << std::endl;
             return 1;
