@@ -182,6 +182,18 @@ int wv_index_set_search_config(wv_index *idx, const wv_search_config *cfg);
 int wv_index_build_nndescent(wv_index *idx, const wv_nnd_params *p, const uint32_t *init_ids,
                              const uint32_t *init_cnt, wv_build_stats *stats);
 
+/* EFANNA's NN-Descent (ComponentRefineEFANNA, src/component_refine.cpp:
+ * 959-1183) on the resident base; the result becomes the index's graph.
+ * init_ids: optional N*init_stride ids of an initial graph (host; rows of
+ * init_cnt[i] valid ids, or all init_stride when init_cnt is NULL;
+ * 1 <= init_stride <= L), as EFANNA's init reads FinalGraph (:1036-1058):
+ * each row is sorted by distance with self, out-of-range and repeated ids
+ * dropped.  NULL starts from InitRandom's S random neighbours.  Unlike
+ * KGraph's refine, the first join runs over 2S GenRandom ids per node
+ * (include/weavess/index.h:105-110), so it needs 2S < N. */
+int wv_index_build_efanna(wv_index *idx, const wv_nnd_params *p, const uint32_t *init_ids,
+                          const uint32_t *init_cnt, uint32_t init_stride, wv_build_stats *stats);
+
 /* Greedy search of Q queries (host arrays).  entries: Q*L ids or NULL
  * (seeded GenRandom per query).  L_per_q/k_per_q: NULL or Q values each
  * (then `entries` must be NULL).  Requires k <= L, and L < N for random

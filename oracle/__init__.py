@@ -78,6 +78,10 @@ def lib():
         L.wvo_nndescent.argtypes = [_f32p, C.c_uint32, C.c_uint32, C.POINTER(NndParams),
                                     C.c_void_p, C.c_void_p, C.c_void_p,
                                     _u32p, _f32p, _u32p, C.c_void_p, C.c_int]
+        L.wvo_nndescent_efanna.restype = C.c_int
+        L.wvo_nndescent_efanna.argtypes = [_f32p, C.c_uint32, C.c_uint32, C.POINTER(NndParams),
+                                           C.c_void_p, C.c_void_p, C.c_uint32,
+                                           _u32p, _f32p, _u32p, C.c_void_p, C.c_int]
         L.wvo_search_greedy.restype = C.c_int
         L.wvo_search_greedy.argtypes = [_u32p, _u32p, C.c_uint32, _f32p, C.c_uint32, C.c_uint32,
                                         _f32p, C.c_uint32, C.c_void_p, C.c_uint64,
@@ -207,6 +211,32 @@ def nndescent(base, K, L, S, R, iters, seed=0, cand_slots=0, init=None,
                              C.cast(st, C.c_void_p), nthreads)
     if rc != 0:
         raise ValueError("nndescent: bad arguments")
+    if with_stats:
+        return ids, dist, deg, [(s.pairs, s.inserts) for s in st[:iters]]
+    return ids, dist, deg
+
+
+def nndescent_efanna(base, K, L, S, R, iters, seed=0, cand_slots=0, init=None,
+                     nthreads: int = 0, with_stats: bool = False):
+    """EFANNA's refine.  init: None (InitRandom's graph) or (ids N x stride,
+    cnt[N] or None) of any initial graph with stride <= L."""
+    base = _c(base, np.float32)
+    N, d = base.shape
+    p = NndParams(K, L, S, R, iters, cand_slots, seed)
+    ids = np.zeros((N, K), np.uint32)
+    dist = np.zeros((N, K), np.float32)
+    deg = np.zeros(N, np.uint32)
+    st = (NndIterStats * max(iters, 1))()
+    if init is not None:
+        i_ids = _c(init[0], np.uint32)
+        i_cnt = None if init[1] is None else _c(init[1], np.uint32)
+        args = (_ptr(i_ids), _ptr(i_cnt), i_ids.shape[1])
+    else:
+        args = (None, None, 0)
+    rc = lib().wvo_nndescent_efanna(base, N, d, C.byref(p), *args, ids, dist, deg,
+                                    C.cast(st, C.c_void_p), nthreads)
+    if rc != 0:
+        raise ValueError("nndescent_efanna: bad arguments")
     if with_stats:
         return ids, dist, deg, [(s.pairs, s.inserts) for s in st[:iters]]
     return ids, dist, deg

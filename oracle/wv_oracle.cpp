@@ -17,6 +17,10 @@
  * wvo_nndescent                 src/component_refine.cpp:12-239,                      S>=N-1 gives the exact kNN graph; a 5-point
  *                               include/weavess/index.h:86-150,                       hand-worked case; invariants; graph recall
  *                               src/component_prune.cpp:8-18                          vs FP64 exact kNN
+ * wvo_nndescent_efanna          src/component_refine.cpp:959-1183,                    ITER=0 returns the initial rows sorted, self
+ *                               include/weavess/index.h:105-110                       and repeats dropped (numpy); 2S=N-1 with exact
+ *                                                                                     insertion gives the exact kNN graph in one
+ *                                                                                     round (the samples are joined); graph recall
  * wvo_search_greedy             src/component_search_entry.cpp:9-30,                  complete graph / all-node entry set / L>=N give
  *                               src/component_route.cpp:9-53,                         exact kNN; hand-worked 8-node KAT for the
  *                               include/weavess/index.h:152-180                       k-pointer restart and duplicate rejection
@@ -91,6 +95,7 @@ constexpr uint64_t TAG_INIT = 0x494E4954ull;    /* "INIT" */
 constexpr uint64_t TAG_ENTRY = 0x454E5452ull;   /* "ENTR" */
 constexpr uint64_t TAG_REVNEW = 0x52564E57ull;  /* "RVNW" */
 constexpr uint64_t TAG_REVOLD = 0x52564F4Cull;  /* "RVOL" */
+constexpr uint64_t TAG_EFANNA = 0x45464E41ull;  /* "EFNA": first-round samples */
 
 inline uint64_t stream_of(uint64_t seed, uint64_t tag, uint64_t idx) {
     uint64_t h = fmix64(seed + GOLDEN);
@@ -468,6 +473,38 @@ void nnd_update(NndState &st, int nth, uint32_t iter, uint64_t seed) {
     }
 }
 
+/* ITER rounds of join() + update(), then RefineInner's tail (:22-74) and
+ * PruneNaive: sort, keep the first K.  first_join: round 0 joins the new
+ * lists already set (EFANNA's samples); else round 0 has nothing to join. */
+void nnd_rounds(NndState &st, const wvo_nnd_params *p, int nth, bool first_join, uint32_t *out_ids,
+                float *out_dist, uint32_t *out_deg, wvo_nnd_iter_stats *iter_stats) {
+    for (uint32_t it = 0; it < p->iters; ++it) {
+        uint64_t pairs = 0, acc = 0;
+        if (it > 0 || first_join) nnd_join(st, nth, pairs, acc);
+        nnd_update(st, nth, it, p->seed);
+        if (iter_stats) {
+            iter_stats[it].pairs = pairs;
+            iter_stats[it].inserts = acc;
+        }
+    }
+#pragma omp parallel for num_threads(nth) schedule(static)
+    for (int64_t i = 0; i < (int64_t)st.N; ++i) {
+        auto &pl = st.pool[i];
+        std::sort(pl.begin(), pl.end(), EntLess());
+        uint32_t deg = std::min<uint32_t>(p->K, (uint32_t)pl.size());
+        out_deg[i] = deg;
+        for (uint32_t j = 0; j < p->K; ++j) {
+            if (j < deg) {
+                out_ids[(size_t)i * p->K + j] = key_id(pl[j].key);
+                out_dist[(size_t)i * p->K + j] = key_dist(pl[j].key);
+            } else {
+                out_ids[(size_t)i * p->K + j] = 0xFFFFFFFFu;
+                out_dist[(size_t)i * p->K + j] = -1.0f;
+            }
+        }
+    }
+}
+
 }  // namespace
 
 int wvo_nndescent(const float *base, uint32_t N, uint32_t d, const wvo_nnd_params *p,
@@ -514,32 +551,66 @@ int wvo_nndescent(const float *base, uint32_t N, uint32_t d, const wvo_nnd_param
     }
     /* NNDescent(), :106-125.  Iteration 0's join sees nn_new = 2S copies of
      * id 0 and no nn_old (index.h:98-103), so it evaluates nothing. */
-    for (uint32_t it = 0; it < p->iters; ++it) {
-        uint64_t pairs = 0, acc = 0;
-        if (it > 0) nnd_join(st, nth, pairs, acc);
-        nnd_update(st, nth, it, p->seed);
-        if (iter_stats) {
-            iter_stats[it].pairs = pairs;
-            iter_stats[it].inserts = acc;
-        }
+    nnd_rounds(st, p, nth, false, out_ids, out_dist, out_deg, iter_stats);
+    return 0;
+}
+
+int wvo_nndescent_efanna(const float *base, uint32_t N, uint32_t d, const wvo_nnd_params *p,
+                         const uint32_t *init_ids, const uint32_t *init_cnt, uint32_t init_stride,
+                         uint32_t *out_ids, float *out_dist, uint32_t *out_deg,
+                         wvo_nnd_iter_stats *iter_stats, int nthreads) {
+    if (!base || !p || !out_ids || !out_dist || !out_deg) return -1;
+    if (N < 2 || d == 0 || p->S == 0 || 2 * (uint64_t)p->S >= N || p->L == 0 || p->K == 0 || p->K > p->L) return -1;
+    if (p->cand_slots && (p->cand_slots & (p->cand_slots - 1))) return -1;
+    if (init_ids && (init_stride == 0 || init_stride > p->L)) return -1;
+    const int nth = nthreads_or_default(nthreads);
+    const uint32_t S = p->S;
+    std::vector<uint32_t> ids0, cnt0;
+    std::vector<float> dist0;
+    if (!init_ids) {  /* no initial graph given: InitRandom's */
+        ids0.resize((size_t)N * S);
+        dist0.resize((size_t)N * S);
+        cnt0.resize(N);
+        wvo_init_random(base, N, d, S, p->seed, ids0.data(), dist0.data(), cnt0.data(), nth);
+        init_ids = ids0.data();
+        init_cnt = cnt0.data();
+        init_stride = S;
     }
-    /* RefineInner tail :22-74 + PruneNaive: sort, keep first K. */
+    NndState st(N);
+    st.N = N; st.d = d; st.L = p->L; st.S = S; st.R = p->R; st.base = base;
+    st.C = p->cand_slots;
+    if (st.C) {
+        st.slots.assign((size_t)N * st.C, KEY_EMPTY);
+        st.thr.assign(N, KEY_EMPTY);
+        st.pool_ids.resize(N);
+    }
+    /* ComponentRefineEFANNA::init, src/component_refine.cpp:1036-1058: the
+     * pool is the initial graph's row sorted, self and repeated ids skipped
+     * (here every repeat, not only adjacent ones), all flagged new, M = S;
+     * nn_new is GenRandom(2S) over [0, N) (index.h:105-110), nn_old empty. */
 #pragma omp parallel for num_threads(nth) schedule(static)
-    for (int64_t i = 0; i < (int64_t)N; ++i) {
-        auto &pl = st.pool[i];
-        std::sort(pl.begin(), pl.end(), EntLess());
-        uint32_t deg = std::min<uint32_t>(p->K, (uint32_t)pl.size());
-        out_deg[i] = deg;
-        for (uint32_t j = 0; j < p->K; ++j) {
-            if (j < deg) {
-                out_ids[(size_t)i * p->K + j] = key_id(pl[j].key);
-                out_dist[(size_t)i * p->K + j] = key_dist(pl[j].key);
-            } else {
-                out_ids[(size_t)i * p->K + j] = 0xFFFFFFFFu;
-                out_dist[(size_t)i * p->K + j] = -1.0f;
-            }
+    for (int64_t ii = 0; ii < (int64_t)N; ++ii) {
+        const uint32_t i = (uint32_t)ii;
+        const uint32_t c = std::min(init_cnt ? init_cnt[i] : init_stride, init_stride);
+        std::vector<uint64_t> keys;
+        for (uint32_t j = 0; j < c; ++j) {
+            const uint32_t id = init_ids[(size_t)i * init_stride + j];
+            if (id >= N || id == i) continue;
+            keys.push_back(make_key(dist_canon(base + (size_t)i * d, base + (size_t)id * d, d), id));
         }
+        std::sort(keys.begin(), keys.end());
+        keys.erase(std::unique(keys.begin(), keys.end()), keys.end());
+        if (keys.size() > p->L) keys.resize(p->L);
+        auto &pl = st.pool[i];
+        pl.clear();
+        for (uint64_t k : keys) pl.push_back(Ent{k, 1});
+        st.M[i] = S;
+        st.nn_new[i].resize(2 * S);
+        gen_random(stream_of(p->seed, TAG_EFANNA, i), st.nn_new[i].data(), 2 * S, N);
+        st.nn_old[i].clear();
     }
+    /* NNDescent(), :1059-1078: the first join runs over those samples. */
+    nnd_rounds(st, p, nth, true, out_ids, out_dist, out_deg, iter_stats);
     return 0;
 }
 

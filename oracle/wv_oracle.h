@@ -78,6 +78,18 @@ int wvo_nndescent(const float *base, uint32_t N, uint32_t d,
                   uint32_t *out_ids, float *out_dist, uint32_t *out_deg,
                   wvo_nnd_iter_stats *iter_stats, int nthreads);
 
+/* ComponentRefineEFANNA + PruneNaive (src/component_refine.cpp:959-1183):
+ * pools from an initial graph (init_ids rows of init_stride <= L ids,
+ * init_cnt[i] valid, or NULL for wvo_init_random's graph), sorted with self
+ * and repeated ids dropped; the first join runs over 2S GenRandom samples
+ * per node (stream tag "EFNA").  Needs 2S < N.  Outputs as wvo_nndescent. */
+int wvo_nndescent_efanna(const float *base, uint32_t N, uint32_t d,
+                         const wvo_nnd_params *p,
+                         const uint32_t *init_ids, const uint32_t *init_cnt,
+                         uint32_t init_stride,
+                         uint32_t *out_ids, float *out_dist, uint32_t *out_deg,
+                         wvo_nnd_iter_stats *iter_stats, int nthreads);
+
 /* Greedy search --------------------------------------------------------- */
 /* SearchEntryRand + RouteGreedy (src/component_search_entry.cpp:9-30,
  * src/component_route.cpp:9-53).  graph is N*deg_max ids, deg[N] valid

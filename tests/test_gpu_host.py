@@ -121,3 +121,26 @@ def test_pipeline_nsg_and_ssg_entries(ctx, tmp_path, small_int):
     r = subprocess.run([os.path.join(HOST, "weavess_main"), "kgraph", pre, "search", "10"], capture_output=True,
                        text=True, env={**os.environ, **senv}, timeout=300)
     assert r.returncode == 2 and "more entry points" in r.stderr
+
+
+def test_pipeline_efanna_build_matches_oracle(ctx, tmp_path, small_int):
+    """weavess_main efanna: INIT_RANDOM on the GPU, then REFINE_EFANNA from
+    those rows, equals the oracle's EFANNA build from InitRandom bit for bit
+    (integer data), and its search gives the oracle's recall."""
+    base, q = small_int
+    pre = str(tmp_path / "c1")
+    io.write_fvecs(pre + "_base.fvecs", base)
+    io.write_fvecs(pre + "_query.fvecs", q)
+    gt, _ = O.exact_knn(base, q, 10)
+    io.write_ivecs(pre + "_groundtruth.ivecs", gt)
+    env = dict(WEAVESS_K="24", WEAVESS_L="32", WEAVESS_ITER="4", WEAVESS_S="10", WEAVESS_R="32",
+               WEAVESS_SEED="6")
+    out = _run(["efanna", pre, "build"], env)
+    assert "__REFINE : EFANNA (HIP)__" in out
+    gids, gdeg = io.read_graph(pre + ".efanna.graph")
+    ri, _, rdeg = O.nndescent_efanna(base, 24, 32, 10, 32, 4, seed=6, cand_slots=128)
+    assert np.array_equal(gdeg, rdeg) and np.array_equal(gids[:, :24], ri)
+    out = _run(["efanna", pre, "search", "40"], env)
+    res, *_ = O.search_greedy(ri, rdeg, base, q, 40, 10, seed=6)
+    assert float(re.search(r"10 NN accuracy: ([0-9.]+)", out).group(1)) == pytest.approx(
+        O.recall(res, gt, 10), abs=1e-6)

@@ -31,7 +31,7 @@ EXPORTS = [
     "wv_index_search_dev", "wv_nndescent_build", "wv_search_greedy", "wv_exact_knn",
     "wv_index_search_ext", "wv_index_search_ext_dev", "wv_index_autotune", "wv_search_graph_create",
     "wv_search_graph_launch", "wv_search_graph_destroy", "wv_mlp_grad", "wv_mlp_train",
-    "wv_mlp_predict",
+    "wv_mlp_predict", "wv_index_build_efanna",
 ]
 
 
@@ -144,6 +144,7 @@ def load() -> C.CDLL:
         "wv_index_get_graph": ([vp, vp, vp, vp, vp], i32),
         "wv_index_set_search_config": ([vp, vp], i32),
         "wv_index_build_nndescent": ([vp, vp, vp, vp, vp], i32),
+        "wv_index_build_efanna": ([vp, vp, vp, vp, u32, vp], i32),
         "wv_index_search": ([vp, vp, u32, vp, u64, u32, u32, vp, vp, u32, vp, vp, vp, vp, vp], i32),
         "wv_index_search_dev": ([vp, vp, u32, vp, u64, u32, u32, vp, vp, u32, vp, vp, vp, vp, vp], i32),
         "wv_nndescent_build": ([vp, vp, u32, u32, vp, vp, vp, vp, vp, vp], i32),
@@ -351,6 +352,17 @@ class Index:
         ii, ic = _c(init_ids, np.uint32), _c(init_cnt, np.uint32)
         self.ctx.check(self.ctx.lib.wv_index_build_nndescent(self.h, C.byref(p), _p(ii), _p(ic),
                                                              C.byref(st)))
+        return st.as_dict()
+
+    def build_efanna(self, K, L, S, R, iters, seed=0, dtype="f32", rerank_mult=0,
+                     cand_slots=0, init_ids=None, init_cnt=None):
+        """EFANNA's refine; init_ids: None or an N x stride initial graph."""
+        p = NndParams(K, L, S, R, iters, DTYPES[dtype], rerank_mult, cand_slots, seed)
+        st = BuildStats()
+        ii, ic = _c(init_ids, np.uint32), _c(init_cnt, np.uint32)
+        stride = 0 if ii is None else ii.shape[1]
+        self.ctx.check(self.ctx.lib.wv_index_build_efanna(self.h, C.byref(p), _p(ii), _p(ic), stride,
+                                                          C.byref(st)))
         return st.as_dict()
 
     def search(self, queries, L, k, seed=0, entries=None, L_per_q=None, k_per_q=None,

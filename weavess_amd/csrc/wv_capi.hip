@@ -931,11 +931,8 @@ int wv_mlp_predict(wv_ctx *c, const wv_term_model *m, const float *X, uint32_t n
     return WV_OK;
 }
 
-int wv_index_build_nndescent(wv_index *ix, const wv_nnd_params *p, const uint32_t *init_ids,
-                             const uint32_t *init_cnt, wv_build_stats *stats) {
-    if (!ix) return WV_EINVAL;
-    wv_ctx *c = ix->ctx;
-    std::lock_guard<std::mutex> g(c->mu);
+namespace {
+int check_nnd_params(wv_ctx *c, const wv_index *ix, const wv_nnd_params *p) {
     if (!p) return fail(c, WV_EINVAL, "params are null");
     if (p->K == 0 || p->L < p->K)
         return fail(c, WV_EINVAL, "need 1 <= K <= L (K=%u L=%u)", p->K, p->L);
@@ -945,10 +942,16 @@ int wv_index_build_nndescent(wv_index *ix, const wv_nnd_params *p, const uint32_
     if (p->L > 1024 || p->R > 1024) return fail(c, WV_ENOTSUP, "L or R above 1024");
     if (p->dtype > WV_DTYPE_FP8) return fail(c, WV_EINVAL, "unknown dtype %u", p->dtype);
     if (p->cand_slots & (p->cand_slots - 1)) return fail(c, WV_EINVAL, "cand_slots must be a power of two");
+    return WV_OK;
+}
+
+int run_nnd(wv_index *ix, const wv_nnd_params *p, const uint32_t *init_ids, const uint32_t *init_cnt,
+            wv_build_stats *stats, bool efanna, uint32_t init_stride) {
+    wv_ctx *c = ix->ctx;
     WV_HIP(c, hipSetDevice(c->device));
     int rc = wv::index_alloc_graph(ix, p->K);
     if (rc) return rc;
-    rc = wv::nndescent_build(ix, p, init_ids, init_cnt, stats, ix->graph, ix->gdist, ix->deg);
+    rc = wv::nndescent_build(ix, p, init_ids, init_cnt, stats, ix->graph, ix->gdist, ix->deg, efanna, init_stride);
     if (rc) {
         (void)hipFree(ix->graph);
         (void)hipFree(ix->gdist);
@@ -959,6 +962,30 @@ int wv_index_build_nndescent(wv_index *ix, const wv_nnd_params *p, const uint32_
         ix->deg_max = 0;
     }
     return rc;
+}
+}  // namespace
+
+int wv_index_build_nndescent(wv_index *ix, const wv_nnd_params *p, const uint32_t *init_ids,
+                             const uint32_t *init_cnt, wv_build_stats *stats) {
+    if (!ix) return WV_EINVAL;
+    wv_ctx *c = ix->ctx;
+    std::lock_guard<std::mutex> g(c->mu);
+    int rc = check_nnd_params(c, ix, p);
+    if (rc) return rc;
+    return run_nnd(ix, p, init_ids, init_cnt, stats, false, 0);
+}
+
+int wv_index_build_efanna(wv_index *ix, const wv_nnd_params *p, const uint32_t *init_ids,
+                          const uint32_t *init_cnt, uint32_t init_stride, wv_build_stats *stats) {
+    if (!ix) return WV_EINVAL;
+    wv_ctx *c = ix->ctx;
+    std::lock_guard<std::mutex> g(c->mu);
+    int rc = check_nnd_params(c, ix, p);
+    if (rc) return rc;
+    if (2ull * p->S >= ix->N) return fail(c, WV_EINVAL, "EFANNA samples 2S ids: need 2S < N (S=%u N=%u)", p->S, ix->N);
+    if (init_ids && (init_stride == 0 || init_stride > p->L))
+        return fail(c, WV_EINVAL, "need 1 <= init_stride <= L (init_stride=%u L=%u)", init_stride, p->L);
+    return run_nnd(ix, p, init_ids, init_cnt, stats, true, init_stride);
 }
 
 int wv_nndescent_build(wv_ctx *c, const float *base, uint32_t N, uint32_t d,

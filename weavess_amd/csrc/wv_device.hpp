@@ -19,6 +19,7 @@ constexpr uint64_t TAG_INIT = 0x494E4954ull;    // "INIT"
 constexpr uint64_t TAG_ENTRY = 0x454E5452ull;   // "ENTR"
 constexpr uint64_t TAG_REVNEW = 0x52564E57ull;  // "RVNW"
 constexpr uint64_t TAG_REVOLD = 0x52564F4Cull;  // "RVOL"
+constexpr uint64_t TAG_EFANNA = 0x45464E41ull;  // "EFNA": EFANNA's first-round samples
 
 // ---------------------------------------------------------------- keys
 // (dist, id) packed so that unsigned 64-bit order is (dist, id) order for

@@ -86,7 +86,9 @@ struct NndDev {
     uint32_t *out_deg;
     uint32_t lds_join_list;   // bytes of list area in the join's per-wave LDS
     uint32_t tile_st;         // LDS bytes per staged tile row: the chunk's bytes + 16 pad
-    uint32_t nw_pow2;         // power of two >= S + R
+    uint32_t nw_pow2;         // power of two >= the longest new list (S + R; 2S in EFANNA's first round)
+    uint32_t fst;             // ids per fnew row: S, or 2S for EFANNA's first-round samples
+    uint32_t ncap;            // longest new + old list: S + 3R, or at least 2S for EFANNA
 };
 
 __device__ inline uint32_t wave_sum_u32(uint32_t v) {
@@ -251,6 +253,21 @@ __device__ inline uint32_t compute_M(const uint8_t *flags, uint32_t Mprev, uint3
 }
 
 // ------------------------------------------------------------------ init
+// GenRandom (include/weavess/util.h:13-27) of size <= 64 on counter stream
+// s: lane i < size returns the i-th id (ascending before the rotation),
+// other lanes ID_NONE.  The draws are reduced mod (N - size), sorted and
+// forced strictly increasing by a prefix max of (v_i - i), then rotated by
+// one more draw mod N.
+__device__ inline uint32_t wave_gen_random(uint64_t s, uint32_t size, uint32_t N) {
+    const int lane = lane_id();
+    uint64_t v = (uint32_t)lane < size ? (uint64_t)(draw(s, lane) % (N - size)) : KEY_EMPTY;
+    v = wave_sort64(v);
+    const int64_t b = (uint32_t)lane < size ? (int64_t)v - lane : INT64_MIN;
+    const int64_t pm = wave_scan_max64(b);
+    const uint64_t off = draw(s, size) % N;
+    return (uint32_t)lane < size ? (uint32_t)(((uint64_t)(pm + lane) + off) % N) : ID_NONE;
+}
+
 // InitRandom: per node, S draws of GenRandom on the node's stream (as in
 // SearchEntryRand, a prefix-max scan forces them strictly increasing), self
 // skipped, build-order FP32 distances, ascending pool, all new, M = S.
@@ -269,13 +286,7 @@ __global__ void __launch_bounds__(256) nnd_init_kernel(NndDev a, const uint32_t 
         const uint32_t c = init_cnt ? init_cnt[n] : S;
         if ((uint32_t)lane < c && (uint32_t)lane < S) id = init_ids[(size_t)n * S + lane];
     } else {
-        const uint64_t s = stream_of(a.seed, TAG_INIT, n);
-        uint64_t v = (uint32_t)lane < S ? (uint64_t)(draw(s, lane) % (N - S)) : KEY_EMPTY;
-        v = wave_sort64(v);
-        int64_t b = (uint32_t)lane < S ? (int64_t)v - lane : INT64_MIN;
-        int64_t pm = wave_scan_max64(b);
-        const uint64_t off = draw(s, S) % N;
-        if ((uint32_t)lane < S) id = (uint32_t)(((uint64_t)(pm + lane) + off) % N);
+        id = wave_gen_random(stream_of(a.seed, TAG_INIT, n), S, N);
     }
     const bool ok = id < N && id != n;
     const uint64_t bm = __ballot(ok);
@@ -317,6 +328,120 @@ __global__ void __launch_bounds__(256) nnd_m0_kernel(NndDev a) {
     if (lane_id() == 0) a.M[n] = m;
 }
 
+// ComponentRefineEFANNA::init (src/component_refine.cpp:1036-1058): the pool
+// is the initial graph's row (up to `stride` <= L ids, cnt[n] valid) sorted
+// by (dist, id) with self, out-of-range and repeated ids dropped, first L
+// kept, all new, M = S.  The row's keys are gathered 64 at a time into a
+// P-key LDS array (P = power of two >= stride), sorted, repeats (equal keys,
+// hence adjacent) cleared and sorted again.
+template <int CPL>
+__global__ void __launch_bounds__(256) nnd_init_rows_kernel(NndDev a, const uint32_t *ids, const uint32_t *cnt,
+                                                            uint32_t stride, uint32_t P) {
+    extern __shared__ __align__(16) uint8_t smem[];
+    const int wid = threadIdx.x >> 6, lane = lane_id();
+    const int wpb = blockDim.x >> 6;
+    const uint32_t n = blockIdx.x * wpb + wid;
+    if (n >= a.N) return;
+    uint32_t *cid = reinterpret_cast<uint32_t *>(smem) + wid * 64;
+    uint64_t *ckey = reinterpret_cast<uint64_t *>(smem + wpb * 256) + wid * 64;
+    uint64_t *keys = reinterpret_cast<uint64_t *>(smem + wpb * (256 + 512)) + (size_t)wid * P;
+    const uint32_t N = a.N;
+    const uint32_t c = cnt ? min(cnt[n], stride) : stride;
+    float4 qv[CPL];
+    {
+        const float4 *qp = reinterpret_cast<const float4 *>(a.base + (size_t)n * a.dp);
+#pragma unroll
+        for (int k = 0; k < CPL; ++k) {
+            const int ch = (lane & 15) + 16 * k;
+            qv[k] = ch < (int)(a.dp / 4) ? qp[ch] : make_float4(0.f, 0.f, 0.f, 0.f);
+        }
+    }
+    uint32_t tot = 0;
+    for (uint32_t c0 = 0; c0 < c; c0 += WAVE) {
+        const uint32_t id = c0 + lane < c ? ids[(size_t)n * stride + c0 + lane] : ID_NONE;
+        const bool ok = id < N && id != n;
+        const uint64_t bm = __ballot(ok);
+        const int m = __popcll(bm);
+        if (ok) cid[lanes_below(bm)] = id;
+        wave_sync();
+        rows_dist<CPL, 4>(a.base, a.dp, a.dp / 4, qv, cid, m, ckey);
+        wave_sync();
+        if (lane < m) keys[tot + lane] = ckey[lane];
+        tot += (uint32_t)m;
+        wave_sync();
+    }
+    for (uint32_t i = tot + lane; i < P; i += WAVE) keys[i] = KEY_EMPTY;
+    wave_sync();
+    lds_sort64(keys, (int)P);
+    uint32_t rep = 0;  // bit j: slot lane + 64 j repeats its predecessor (P <= 1024)
+    for (uint32_t j = 0; j * WAVE < P; ++j) {
+        const uint32_t i = lane + j * WAVE;
+        if (i > 0 && keys[i] != KEY_EMPTY && keys[i] == keys[i - 1]) rep |= 1u << j;
+    }
+    wave_sync();
+    for (uint32_t j = 0; j * WAVE < P; ++j)
+        if (rep >> j & 1u) keys[lane + j * WAVE] = KEY_EMPTY;
+    wave_sync();
+    if (__ballot(rep != 0)) lds_sort64(keys, (int)P);
+    uint32_t uniq = 0;
+    for (uint32_t j = 0; j * WAVE < P; ++j) uniq += (uint32_t)__popcll(__ballot(keys[lane + j * WAVE] != KEY_EMPTY));
+    const uint32_t m = min(uniq, a.L);
+    for (uint32_t i = lane; i < a.L; i += WAVE) {
+        a.pool[(size_t)n * a.L + i] = i < m ? keys[i] : KEY_EMPTY;
+        a.pflag[(size_t)n * a.L + i] = i < m ? 1 : 0;
+    }
+    if (lane == 0) {
+        const uint64_t last = m > 0 ? keys[m - 1] : KEY_EMPTY;
+        a.pcnt[n] = m;
+        a.M[n] = a.S;
+        a.thr[n] = last;
+        a.worst[n] = m > 0 ? key_dist(last) : -1.0f;
+    }
+}
+
+// EFANNA's first-round lists (include/weavess/index.h:105-110): nn_new is
+// GenRandom(2S) over [0, N) on the node's "EFNA" stream (it may hold the
+// node itself, as in the reference), nn_old and the reverse lists empty.
+// 2S <= 128: the draws are sorted in a 128-key LDS array, and lane l forces
+// elements 2l and 2l+1 strictly increasing with the prefix max of (v_i - i)
+// carried in from the lanes below.
+__global__ void __launch_bounds__(256) nnd_efanna_seed_kernel(NndDev a) {
+    extern __shared__ __align__(16) uint8_t smem[];
+    const int wid = threadIdx.x >> 6, lane = lane_id();
+    const uint32_t n = blockIdx.x * (blockDim.x >> 6) + wid;
+    if (n >= a.N) return;
+    uint64_t *v = reinterpret_cast<uint64_t *>(smem) + wid * 128;
+    const uint32_t S2 = 2 * a.S, N = a.N;
+    const uint64_t st = stream_of(a.seed, TAG_EFANNA, n);
+    for (uint32_t i = lane; i < 128; i += WAVE) v[i] = i < S2 ? (uint64_t)(draw(st, i) % (N - S2)) : KEY_EMPTY;
+    wave_sync();
+    lds_sort64(v, 128);
+    const uint32_t i0 = 2 * lane, i1 = i0 + 1;
+    int64_t mx = INT64_MIN;
+    if (i0 < S2) mx = (int64_t)v[i0] - (int64_t)i0;
+    if (i1 < S2) mx = max(mx, (int64_t)v[i1] - (int64_t)i1);
+    const int64_t incl = wave_scan_max64(mx);
+    const uint32_t lo = (uint32_t)__shfl_up((int)(uint32_t)incl, 1);
+    const uint32_t hi = (uint32_t)__shfl_up((int)(uint32_t)((uint64_t)incl >> 32), 1);
+    int64_t run = lane == 0 ? INT64_MIN : (int64_t)(((uint64_t)hi << 32) | lo);
+    const uint64_t off = draw(st, S2) % N;
+    uint64_t *dst = a.fnew + (size_t)n * a.fst;
+    if (i0 < S2) {
+        run = max(run, (int64_t)v[i0] - (int64_t)i0);
+        dst[i0] = ((uint64_t)(run + i0) + off) % N;
+    }
+    if (i1 < S2) {
+        run = max(run, (int64_t)v[i1] - (int64_t)i1);
+        dst[i1] = ((uint64_t)(run + i1) + off) % N;
+    }
+    if (lane == 0) {
+        a.fnew_cnt[n] = S2;
+        a.fold_cnt[n] = 0;
+        a.rsel_new_cnt[n] = 0;
+        a.rsel_old_cnt[n] = 0;
+    }
+}
+
 // ------------------------------------------------------------------ sample
 // update() :176-209 forward part: first M pool entries, flagged ones to the
 // new list (flag cleared), the rest to the old list, in pool order; count a
@@ -335,7 +460,7 @@ __global__ void __launch_bounds__(256) nnd_sample_kernel(NndDev a) {
         const uint64_t key = act ? pr[p] : KEY_EMPTY;
         const bool f = act && fl[p];
         const uint64_t bn = __ballot(f), bo = __ballot(act && !f);
-        if (f) a.fnew[(size_t)n * a.S + nf + lanes_below(bn)] = key;
+        if (f) a.fnew[(size_t)n * a.fst + nf + lanes_below(bn)] = key;
         if (act && !f) a.fold[(size_t)n * a.L + no + lanes_below(bo)] = key;
         if (act) {
             const uint32_t id = key_id(key);
@@ -358,7 +483,7 @@ __global__ void __launch_bounds__(256) nnd_rev_scatter_kernel(NndDev a) {
     const uint32_t nf = a.fnew_cnt[n], no = a.fold_cnt[n];
     for (uint32_t i = lane; i < nf + no; i += WAVE) {
         const bool isnew = i < nf;
-        const uint64_t key = isnew ? a.fnew[(size_t)n * a.S + i] : a.fold[(size_t)n * a.L + (i - nf)];
+        const uint64_t key = isnew ? a.fnew[(size_t)n * a.fst + i] : a.fold[(size_t)n * a.L + (i - nf)];
         const uint32_t id = key_id(key);
         if (!(key_dist(key) > a.worst[id])) continue;
         if (isnew) {
@@ -597,7 +722,7 @@ __global__ void __launch_bounds__(256) nnd_join_kernel(NndDev a) {
     const int wid = threadIdx.x >> 6, lane = lane_id();
     const uint32_t n = blockIdx.x * (blockDim.x >> 6) + wid;
     if (n >= a.N) return;
-    const uint32_t ncap = a.S + 3 * a.R;  // |new| <= S+R, |old| <= 2R
+    const uint32_t ncap = a.ncap;  // |new| <= S+R (2S in EFANNA's first round), |old| <= 2R
     uint8_t *my = smem + (size_t)wid * (a.lds_join_list + 2 * 16 * a.tile_st);
     uint32_t *nw = reinterpret_cast<uint32_t *>(my);              // nw_pow2
     uint32_t *col = nw + a.nw_pow2;                               // ncap
@@ -613,7 +738,7 @@ __global__ void __launch_bounds__(256) nnd_join_kernel(NndDev a) {
     const uint32_t no = min(foc + roc, 2 * a.R);
     if (nn == 0) return;
     for (uint32_t i = lane; i < a.nw_pow2; i += WAVE)
-        nw[i] = i < fnc ? key_id(a.fnew[(size_t)n * a.S + i])
+        nw[i] = i < fnc ? key_id(a.fnew[(size_t)n * a.fst + i])
               : i < nn ? a.rsel_new[(size_t)n * a.R + (i - fnc)] : ID_NONE;
     wave_sync();
     lds_sort32(nw, (int)a.nw_pow2);  // ascending ids: position order == id order
@@ -984,6 +1109,16 @@ hipError_t launch_init(const NndDev &a, const uint32_t *ii, const uint32_t *ic, 
     return hipGetLastError();
 }
 template <int CPL>
+hipError_t launch_init_rows(const NndDev &a, const uint32_t *ii, const uint32_t *ic, uint32_t stride, uint32_t blocks,
+                            hipStream_t s) {
+    const uint32_t P = pow2_at_least(stride, 64);
+    const size_t lds = (size_t)WPB * (256 + 512 + (size_t)P * 8);
+    hipError_t e = set_lds(nnd_init_rows_kernel<CPL>, lds);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(nnd_init_rows_kernel<CPL>, dim3(blocks), dim3(64 * WPB), lds, s, a, ii, ic, stride, P);
+    return hipGetLastError();
+}
+template <int CPL>
 hipError_t launch_final(const NndDev &a, uint32_t rp2, uint32_t blocks, hipStream_t s) {
     const size_t lds = (size_t)WPB * (rp2 * 12 + 16);
     hipError_t e = set_lds(nnd_finalize_kernel<CPL>, lds);
@@ -996,7 +1131,7 @@ hipError_t launch_final(const NndDev &a, uint32_t rp2, uint32_t blocks, hipStrea
 
 int nndescent_build(wv_index *ix, const wv_nnd_params *p, const uint32_t *init_ids,
                     const uint32_t *init_cnt, wv_build_stats *stats, uint32_t *graph_out,
-                    float *gdist_out, uint32_t *deg_out) {
+                    float *gdist_out, uint32_t *deg_out, bool efanna, uint32_t init_stride) {
     const IndexView v = index_view(ix);
     const hipStream_t s = v.stream;
     const uint32_t N = v.N;
@@ -1040,7 +1175,8 @@ int nndescent_build(wv_index *ix, const wv_nnd_params *p, const uint32_t *init_i
     NND_HIP(B.get(&a.M, N));
     NND_HIP(B.get(&a.thr, N));
     NND_HIP(B.get(&a.worst, N));
-    NND_HIP(B.get(&a.fnew, (size_t)N * S));
+    a.fst = efanna ? 2 * S : S;
+    NND_HIP(B.get(&a.fnew, (size_t)N * a.fst));
     NND_HIP(B.get(&a.fold, (size_t)N * L));
     NND_HIP(B.get(&a.fnew_cnt, N));
     NND_HIP(B.get(&a.fold_cnt, N));
@@ -1067,8 +1203,9 @@ int nndescent_build(wv_index *ix, const wv_nnd_params *p, const uint32_t *init_i
     a.out_dist = gdist_out;
     a.out_deg = deg_out;
 
-    const uint32_t ncap = S + 3 * R;
-    a.nw_pow2 = pow2_at_least(S + R, 64);
+    const uint32_t ncap = efanna ? std::max(S + 3 * R, 2 * S) : S + 3 * R;
+    a.ncap = ncap;
+    a.nw_pow2 = pow2_at_least(efanna ? std::max(S + R, 2 * S) : S + R, 64);
     // list area: nw ids, col ids, cthr keys, cnorm, and FP8 row scales
     a.lds_join_list = (uint32_t)(((a.nw_pow2 * 4 + ((ncap + 1) & ~1u) * 4 + ncap * (dt == DT_FP8 ? 16 : 12)) + 15) &
                                  ~15u);
@@ -1116,20 +1253,31 @@ int nndescent_build(wv_index *ix, const wv_nnd_params *p, const uint32_t *init_i
         NND_HIP(hipGetLastError());
     }
     uint32_t *d_init = nullptr, *d_icnt = nullptr;
+    const uint32_t ist = efanna && init_ids ? init_stride : S;  // ids per initial row
     if (init_ids) {
-        NND_HIP(B.get(&d_init, (size_t)N * S));
-        NND_HIP(hipMemcpyAsync(d_init, init_ids, (size_t)N * S * 4, hipMemcpyHostToDevice, s));
+        NND_HIP(B.get(&d_init, (size_t)N * ist));
+        NND_HIP(hipMemcpyAsync(d_init, init_ids, (size_t)N * ist * 4, hipMemcpyHostToDevice, s));
         if (init_cnt) {
             NND_HIP(B.get(&d_icnt, N));
             NND_HIP(hipMemcpyAsync(d_icnt, init_cnt, (size_t)N * 4, hipMemcpyHostToDevice, s));
         }
     }
-    switch (cpl) {
-    case 1: NND_HIP(launch_init<1>(a, d_init, d_icnt, blocks, s)); break;
-    case 2: NND_HIP(launch_init<2>(a, d_init, d_icnt, blocks, s)); break;
-    case 4: NND_HIP(launch_init<4>(a, d_init, d_icnt, blocks, s)); break;
-    case 8: NND_HIP(launch_init<8>(a, d_init, d_icnt, blocks, s)); break;
-    default: NND_HIP(launch_init<16>(a, d_init, d_icnt, blocks, s)); break;
+    if (efanna && init_ids) {
+        switch (cpl) {
+        case 1: NND_HIP(launch_init_rows<1>(a, d_init, d_icnt, ist, blocks, s)); break;
+        case 2: NND_HIP(launch_init_rows<2>(a, d_init, d_icnt, ist, blocks, s)); break;
+        case 4: NND_HIP(launch_init_rows<4>(a, d_init, d_icnt, ist, blocks, s)); break;
+        case 8: NND_HIP(launch_init_rows<8>(a, d_init, d_icnt, ist, blocks, s)); break;
+        default: NND_HIP(launch_init_rows<16>(a, d_init, d_icnt, ist, blocks, s)); break;
+        }
+    } else {
+        switch (cpl) {
+        case 1: NND_HIP(launch_init<1>(a, d_init, d_icnt, blocks, s)); break;
+        case 2: NND_HIP(launch_init<2>(a, d_init, d_icnt, blocks, s)); break;
+        case 4: NND_HIP(launch_init<4>(a, d_init, d_icnt, blocks, s)); break;
+        case 8: NND_HIP(launch_init<8>(a, d_init, d_icnt, blocks, s)); break;
+        default: NND_HIP(launch_init<16>(a, d_init, d_icnt, blocks, s)); break;
+        }
     }
     NND_HIP(set_lds(nnd_hash_kernel, hash_lds));
     hipLaunchKernelGGL(nnd_hash_kernel, dim3(blocks), dim3(64 * WPB), hash_lds, s, a);
@@ -1148,7 +1296,11 @@ int nndescent_build(wv_index *ix, const wv_nnd_params *p, const uint32_t *init_i
     for (uint32_t it = 0; it < p->iters; ++it) {
         a.iter = it;
         float ms = 0.f;
-        if (it > 0) {
+        if (it > 0 || efanna) {
+            if (it == 0) {  // EFANNA: the first join runs over random samples
+                hipLaunchKernelGGL(nnd_efanna_seed_kernel, dim3(blocks), dim3(64 * WPB), WPB * 128 * 8, s, a);
+                NND_HIP(hipGetLastError());
+            }
             NND_HIP(hipEventRecord(ev[2], s));
             const uint32_t jb = (N + JOIN_WPB - 1) / JOIN_WPB;
             if (dt == DT_F32) hipLaunchKernelGGL(nnd_join_kernel<DT_F32>, dim3(jb), dim3(64 * JOIN_WPB), join_lds, s, a);
@@ -1181,7 +1333,7 @@ int nndescent_build(wv_index *ix, const wv_nnd_params *p, const uint32_t *init_i
             NND_HIP(hipEventRecord(ev[6], s));
         }
         NND_HIP(hipStreamSynchronize(s));
-        if (it > 0) {
+        if (it > 0 || efanna) {
             NND_HIP(hipEventElapsedTime(&ms, ev[2], ev[3]));
             t_join += ms;
             NND_HIP(hipEventElapsedTime(&ms, ev[3], ev[4]));

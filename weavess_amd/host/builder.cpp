@@ -48,6 +48,11 @@ IndexBuilder *IndexBuilder::refine(TYPE type, bool) {
         a.reset(new ComponentRefineNNDescentHIP(index_, WV_DTYPE_BF16));
     } else if (type == REFINE_NN_DESCENT_HIP_FP8) {
         a.reset(new ComponentRefineNNDescentHIP(index_, WV_DTYPE_FP8));
+    } else if (type == REFINE_EFANNA || type == REFINE_EFANNA_HIP_BF16 || type == REFINE_EFANNA_HIP_FP8) {
+        std::cout << "__REFINE : EFANNA (HIP)__" << std::endl;  // src/builder.cpp:137-139
+        a.reset(new ComponentRefineNNDescentHIP(
+            index_, type == REFINE_EFANNA ? WV_DTYPE_F32 : type == REFINE_EFANNA_HIP_BF16 ? WV_DTYPE_BF16 : WV_DTYPE_FP8,
+            true));
     } else {
         throw std::invalid_argument("__REFINE : WRONG TYPE__");
     }

@@ -143,7 +143,7 @@ void ComponentRefineNNDescentHIP::RefineInner() {
     p.K = index->getResultEdgesNum();
     p.R = P.get<unsigned>("R");
     p.iters = P.get<unsigned>("ITER");
-    p.S = index->getInitEdgesNum() ? index->getInitEdgesNum() : P.get<unsigned>("S");
+    p.S = index->getInitEdgesNum() && !efanna_ ? index->getInitEdgesNum() : P.get<unsigned>("S");
     p.dtype = dtype_;
     p.rerank_mult = P.has("rerank_mult") ? P.get<unsigned>("rerank_mult") : 0;
     p.cand_slots = P.has("cand_slots") ? P.get<unsigned>("cand_slots") : 0;
@@ -151,18 +151,34 @@ void ComponentRefineNNDescentHIP::RefineInner() {
     const unsigned N = index->getBaseLen();
     auto &fg = index->getFinalGraph();
     std::vector<uint32_t> init, cnt;
-    if (fg.size() == N) {  // initial lists from the init stage (component_refine.cpp:96-103)
-        init.assign((size_t)N * p.S, 0xFFFFFFFFu);
+    unsigned stride = p.S;
+    if (fg.size() == N) {  // initial lists from the init stage (component_refine.cpp:96-103, :1046-1056)
+        if (efanna_) {
+            stride = 1;
+            for (const auto &r : fg) stride = std::max<unsigned>(stride, (unsigned)r.size());
+            stride = std::min(stride, p.L);
+        }
+        init.assign((size_t)N * stride, 0xFFFFFFFFu);
         cnt.assign(N, 0);
         for (unsigned i = 0; i < N; ++i) {
-            const unsigned c = std::min<unsigned>(p.S, (unsigned)fg[i].size());
-            for (unsigned j = 0; j < c; ++j) init[(size_t)i * p.S + j] = fg[i][j].id;
+            std::vector<Index::SimpleNeighbor> row(fg[i].begin(), fg[i].end());
+            if (efanna_ && row.size() > stride)
+                std::partial_sort(row.begin(), row.begin() + stride, row.end(),
+                                  [](const Index::SimpleNeighbor &a, const Index::SimpleNeighbor &b) {
+                                      return a.distance < b.distance || (a.distance == b.distance && a.id < b.id);
+                                  });
+            const unsigned c = std::min<unsigned>(stride, (unsigned)row.size());
+            for (unsigned j = 0; j < c; ++j) init[(size_t)i * stride + j] = row[j].id;
             cnt[i] = c;
         }
     }
     wv_index *ix = index->deviceIndex();
-    index->check(wv_index_build_nndescent(ix, &p, init.empty() ? nullptr : init.data(),
-                                          cnt.empty() ? nullptr : cnt.data(), &stats_));
+    if (efanna_)
+        index->check(wv_index_build_efanna(ix, &p, init.empty() ? nullptr : init.data(),
+                                           cnt.empty() ? nullptr : cnt.data(), stride, &stats_));
+    else
+        index->check(wv_index_build_nndescent(ix, &p, init.empty() ? nullptr : init.data(),
+                                              cnt.empty() ? nullptr : cnt.data(), &stats_));
     std::vector<uint32_t> ids((size_t)N * p.K), deg(N);
     std::vector<float> dist((size_t)N * p.K);
     index->check(wv_index_get_graph(ix, ids.data(), dist.data(), deg.data(), nullptr));

@@ -54,15 +54,20 @@ public:
 // src/component_prune.cpp:8-18).  Reads "L","K","R","ITER" like
 // ComponentRefineNNDescent::SetConfigs (:79-85), S from the init stage and
 // the initial lists from FinalGraph; writes FinalGraph (<= K per node).
+// efanna: ComponentRefineEFANNA (:959-1183) instead, which reads S from the
+// parameters (:1028-1034), takes whole FinalGraph rows as initial pools (a
+// row longer than L is cut to its L nearest) and joins 2S random samples
+// per node in its first round.
 class ComponentRefineNNDescentHIP : public ComponentRefine {
 public:
-    ComponentRefineNNDescentHIP(Index *index, unsigned dtype = WV_DTYPE_F32)
-        : ComponentRefine(index), dtype_(dtype) {}
+    ComponentRefineNNDescentHIP(Index *index, unsigned dtype = WV_DTYPE_F32, bool efanna = false)
+        : ComponentRefine(index), dtype_(dtype), efanna_(efanna) {}
     void RefineInner() override;
     const wv_build_stats &stats() const { return stats_; }
 
 private:
     unsigned dtype_;
+    bool efanna_;
     wv_build_stats stats_{};
 };
 

@@ -18,6 +18,9 @@ enum TYPE {
     REFINE_NN_DESCENT_HIP,  // NN-Descent on the GPU, FP32 join
     REFINE_NN_DESCENT_HIP_BF16,
     REFINE_NN_DESCENT_HIP_FP8,
+    REFINE_EFANNA,          // EFANNA's NN-Descent on the GPU, FP32 join
+    REFINE_EFANNA_HIP_BF16,
+    REFINE_EFANNA_HIP_FP8,
 
     SEARCH_ENTRY_RAND,      // L random entries per query (GenRandom)
     SEARCH_ENTRY_CENTROID,  // the entry point's neighbours, then distinct random ids

@@ -1,20 +1,23 @@
-// weavess_main.cpp -- KGraph driver of the pipeline (test/main.cpp:6-34 and
-// :480-546 of the reference, KGraph only, with the HIP stages).
+// weavess_main.cpp -- KGraph and EFANNA drivers of the pipeline
+// (test/main.cpp:6-34, :190-217 and :480-546 of the reference, with the HIP
+// stages).
 //
-//   weavess_main kgraph <prefix> build|search|info [L_search]
+//   weavess_main kgraph|efanna <prefix> build|search|info [L_search]
 //
 // <prefix>_base.fvecs, <prefix>_query.fvecs and <prefix>_groundtruth.ivecs
-// are read; the graph file is <prefix>.kgraph.graph (WEAVESS_GRAPH
-// overrides).  K, L, ITER, S, R come from the per-dataset table below (keyed
-// by the prefix's file name) and can be overridden with WEAVESS_K, _L,
-// _ITER, _S, _R; WEAVESS_SEED, WEAVESS_DTYPE (f32|bf16|fp8) and
-// WEAVESS_EXACT (exact-kNN graph for "info") are read too;
+// are read; the graph file is <prefix>.kgraph.graph or <prefix>.efanna.graph
+// (WEAVESS_GRAPH overrides).  K, L, ITER, S, R come from the per-dataset
+// tables below (keyed by the prefix's file name) and can be overridden with
+// WEAVESS_K, _L, _ITER, _S, _R; WEAVESS_SEED, WEAVESS_DTYPE (f32|bf16|fp8)
+// and WEAVESS_EXACT (exact-kNN graph for "info") are read too.
 // WEAVESS_ENTRY=centroid|subcentroid searches from the centroid entry point
-// or the entry set eps_ instead of random entries; WEAVESS_FORMAT=nsg|ssg
+// or the entry set eps_ instead of random entries.  WEAVESS_FORMAT=nsg|ssg
 // reads and writes graph files headed by ep_ (NSG/Vamana layout) or eps_
-// (SSG layout) instead of the plain KGraph layout.  A build with
-// WEAVESS_FORMAT=nsg finds the centroid entry point and writes it.  Without L_search, search tunes L to WEAVESS_ACC_SET
-// (default 0.95) recall.
+// (SSG layout) instead of the plain layout; a build with WEAVESS_FORMAT=nsg
+// finds the centroid entry point and writes it.  Without L_search, search
+// tunes L to WEAVESS_ACC_SET (default 0.95) recall.  EFANNA builds start
+// from InitRandom's graph (the reference's KD-tree initialisation is not
+// built) and are written in the plain layout.
 #include <cstdlib>
 #include <cstring>
 #include <iostream>
@@ -41,6 +44,19 @@ const std::map<std::string, KG> &table() {
     return t;
 }
 
+// EFANNA's K, L, Iter, S, R per dataset (include/weavess/exp_data.h:315-340;
+// its KD-tree nTrees / mLevel have no use here, the initial graph is
+// InitRandom's).
+const std::map<std::string, KG> &table_efanna() {
+    static const std::map<std::string, KG> t = {
+        {"siftsmall", {30, 50, 10, 15, 150}}, {"sift1M", {60, 70, 10, 15, 150}},
+        {"gist", {100, 190, 7, 30, 50}},      {"glove-100", {100, 170, 7, 10, 100}},
+        {"crawl", {100, 120, 8, 25, 100}},    {"msong", {50, 130, 7, 10, 150}},
+        {"c1", {50, 70, 10, 10, 100}},        {"c2", {64, 64, 12, 16, 64}},
+    };
+    return t;
+}
+
 unsigned env_u(const char *name, unsigned dflt) {
     const char *v = std::getenv(name);
     return v && *v ? (unsigned)std::strtoul(v, nullptr, 10) : dflt;
@@ -49,13 +65,16 @@ unsigned env_u(const char *name, unsigned dflt) {
 }  // namespace
 
 int main(int argc, char **argv) {
-    if (argc < 4 || argc > 5 || std::string(argv[1]) != "kgraph") {
-        std::cout << "./weavess_main kgraph dataset_prefix build|search|info [L_search]" << std::endl;
+    const std::string alg = argc > 1 ? argv[1] : "";
+    if (argc < 4 || argc > 5 || (alg != "kgraph" && alg != "efanna")) {
+        std::cout << "./weavess_main kgraph|efanna dataset_prefix build|search|info [L_search]" << std::endl;
         return 1;
     }
+    const bool efanna = alg == "efanna";
     const std::string prefix(argv[2]), exc(argv[3]);
     std::string name = prefix.substr(prefix.find_last_of('/') + 1);
-    KG kg = table().count(name) ? table().at(name) : table().at("c1");
+    const auto &tab = efanna ? table_efanna() : table();
+    KG kg = tab.count(name) ? tab.at(name) : tab.at("c1");
     weavess::Parameters P;
     P.set<unsigned>("K", env_u("WEAVESS_K", kg.K));
     P.set<unsigned>("L", env_u("WEAVESS_L", kg.L));
@@ -67,13 +86,17 @@ int main(int argc, char **argv) {
     else P.set<float>("acc_set", 0.95f);
     if (const char *e = std::getenv("WEAVESS_EXACT")) P.set<std::string>("exact_knng_path", e);
     const char *gf = std::getenv("WEAVESS_GRAPH");
-    const std::string graph = gf ? gf : prefix + ".kgraph.graph";
+    const std::string graph = gf ? gf : prefix + (efanna ? ".efanna.graph" : ".kgraph.graph");
     const std::string base = prefix + "_base.fvecs", query = prefix + "_query.fvecs",
                       ground = prefix + "_groundtruth.ivecs";
     const std::string dtype = std::getenv("WEAVESS_DTYPE") ? std::getenv("WEAVESS_DTYPE") : "f32";
-    const weavess::TYPE refine = dtype == "bf16"  ? weavess::REFINE_NN_DESCENT_HIP_BF16
-                                 : dtype == "fp8" ? weavess::REFINE_NN_DESCENT_HIP_FP8
-                                                  : weavess::REFINE_NN_DESCENT_HIP;
+    const weavess::TYPE refine =
+        efanna ? (dtype == "bf16"  ? weavess::REFINE_EFANNA_HIP_BF16
+                  : dtype == "fp8" ? weavess::REFINE_EFANNA_HIP_FP8
+                                   : weavess::REFINE_EFANNA)
+               : (dtype == "bf16"  ? weavess::REFINE_NN_DESCENT_HIP_BF16
+                  : dtype == "fp8" ? weavess::REFINE_NN_DESCENT_HIP_FP8
+                                   : weavess::REFINE_NN_DESCENT_HIP);
     const std::string ent = std::getenv("WEAVESS_ENTRY") ? std::getenv("WEAVESS_ENTRY") : "rand";
     const weavess::TYPE entry = ent == "centroid"      ? weavess::SEARCH_ENTRY_CENTROID
                                 : ent == "subcentroid" ? weavess::SEARCH_ENTRY_SUB_CENTROID
