@@ -194,6 +194,35 @@ int wv_index_build_nndescent(wv_index *idx, const wv_nnd_params *p, const uint32
 int wv_index_build_efanna(wv_index *idx, const wv_nnd_params *p, const uint32_t *init_ids,
                           const uint32_t *init_cnt, uint32_t init_stride, wv_build_stats *stats);
 
+/* NSG refine parameters (ComponentRefineNSG::SetConfigs,
+ * src/component_refine.cpp:430-436). */
+typedef struct {
+    uint32_t L;     /* "L_refine": pool of each node's candidate search, 1..min(N-1, 1024) */
+    uint32_t R;     /* "R_refine": degree bound of the pruned lists, 1..256       */
+    uint32_t C;     /* "C_refine": candidates the prune considers, 1..1024         */
+    uint32_t ep;    /* navigating node (ComponentRefineEntryCentroid's ep_)        */
+    uint64_t seed;  /* random entry fill and fallback roots                        */
+} wv_nsg_params;
+
+typedef struct {
+    double total_ms, search_ms, prune_ms, insert_ms, conn_ms;
+    uint64_t evals;       /* distance evaluations of the candidate searches */
+    uint32_t conn_edges;  /* edges added to reach every node from ep        */
+    uint32_t width;       /* widest row of the result                       */
+} wv_nsg_stats;
+
+/* NSG refine of the index's graph, which must be its kNN graph
+ * (ComponentRefineNSG::RefineInner, src/component_refine.cpp:393-428, after
+ * the centroid entry): per node a greedy search for the node from ep's
+ * neighbours plus random ids gathers candidates
+ * (ComponentCandidateNSG, component_candidate.cpp:10-85), PruneNSG's
+ * occlusion keeps up to R of the C nearest (component_prune.cpp:20-66),
+ * reverse edges are inserted and re-pruned (InterInsert, :474-540, as one
+ * batch), and nodes unreachable from ep are linked (ComponentConnNSGDFS,
+ * component_conn.cpp:10-140).  The result replaces the index's graph; rows
+ * hold at most R ids plus the links. */
+int wv_index_build_nsg(wv_index *idx, const wv_nsg_params *p, wv_nsg_stats *stats);
+
 /* Greedy search of Q queries (host arrays).  entries: Q*L ids or NULL
  * (seeded GenRandom per query).  L_per_q/k_per_q: NULL or Q values each
  * (then `entries` must be NULL).  Requires k <= L, and L < N for random

@@ -82,6 +82,14 @@ def lib():
         L.wvo_nndescent_efanna.argtypes = [_f32p, C.c_uint32, C.c_uint32, C.POINTER(NndParams),
                                            C.c_void_p, C.c_void_p, C.c_uint32,
                                            _u32p, _f32p, _u32p, C.c_void_p, C.c_int]
+        for fn in ("wvo_nsg", "wvo_nsg_conn"):
+            getattr(L, fn).restype = C.c_int
+        L.wvo_nsg.argtypes = [_f32p, C.c_uint32, C.c_uint32, _u32p, _u32p, C.c_uint32, C.c_uint32,
+                              C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint64, _u32p, _f32p, _u32p,
+                              C.c_uint32, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.c_int]
+        L.wvo_nsg_conn.argtypes = [_f32p, C.c_uint32, C.c_uint32, _u32p, _u32p, C.c_uint32, C.c_uint32,
+                                   C.c_uint32, C.c_uint32, C.c_uint64, _u32p, _f32p, _u32p, C.c_uint32,
+                                   C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.c_int]
         L.wvo_search_greedy.restype = C.c_int
         L.wvo_search_greedy.argtypes = [_u32p, _u32p, C.c_uint32, _f32p, C.c_uint32, C.c_uint32,
                                         _f32p, C.c_uint32, C.c_void_p, C.c_uint64,
@@ -240,6 +248,42 @@ def nndescent_efanna(base, K, L, S, R, iters, seed=0, cand_slots=0, init=None,
     if with_stats:
         return ids, dist, deg, [(s.pairs, s.inserts) for s in st[:iters]]
     return ids, dist, deg
+
+
+def _graph_call(fn, N, first_width, *args):
+    wcap = first_width
+    while True:
+        ids = np.zeros((N, wcap), np.uint32)
+        dist = np.zeros((N, wcap), np.float32)
+        deg = np.zeros(N, np.uint32)
+        w, e = C.c_uint32(0), C.c_uint32(0)
+        rc = fn(*args, ids, dist, deg, wcap, C.byref(w), C.byref(e), 0)
+        if rc == -2:
+            wcap = w.value
+            continue
+        if rc != 0:
+            raise ValueError(f"{fn.__name__}: bad arguments")
+        return ids[:, :w.value], dist[:, :w.value], deg, e.value
+
+
+def nsg(base, knn_ids, knn_deg, L, R, C_, ep, seed=0):
+    """NSG refine of a kNN graph; returns (ids, dist, deg, conn_edges)."""
+    base = _c(base, np.float32)
+    knn_ids = _c(knn_ids, np.uint32)
+    knn_deg = _c(knn_deg, np.uint32)
+    N, d = base.shape
+    return _graph_call(lib().wvo_nsg, N, R + 8, base, N, d, knn_ids, knn_deg, knn_ids.shape[1], L, R, C_, ep,
+                       seed)
+
+
+def nsg_conn(base, ids, deg, L, C_, ep, seed=0):
+    """NSG's connectivity pass alone; returns (ids, dist, deg, edges)."""
+    base = _c(base, np.float32)
+    ids = _c(ids, np.uint32)
+    deg = _c(deg, np.uint32)
+    N, d = base.shape
+    return _graph_call(lib().wvo_nsg_conn, N, ids.shape[1] + 8, base, N, d, ids, deg, ids.shape[1], L, C_, ep,
+                       seed)
 
 
 # ---------------------------------------------------------------- search

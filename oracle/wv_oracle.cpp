@@ -919,3 +919,292 @@ double wvo_graph_quality(const uint32_t *ids, const uint32_t *deg, uint32_t deg_
 }
 
 }  // extern "C"
+
+/* ---------------------------------------------------------------------- */
+/* NSG refine (src/component_refine.cpp:393-540, component_candidate.cpp:  */
+/* 10-85, component_prune.cpp:20-66, component_conn.cpp:10-140)            */
+/* ---------------------------------------------------------------------- */
+namespace {
+constexpr uint64_t TAG_NSGC = 0x4E534743ull;  /* "NSGC": candidate-search entries */
+constexpr uint64_t TAG_NSGR = 0x4E534752ull;  /* "NSGR": connectivity fallback root */
+constexpr uint32_t NSG_TCAP = 1024;           /* candidates of one reverse-insert re-prune */
+
+using Adj = std::vector<std::vector<uint32_t>>;
+
+/* ep's first min(L, degree) distinct in-range ids: the head of every
+ * candidate search's entries. */
+std::vector<uint32_t> nsg_head(const Adj &g, uint32_t ep, uint32_t L, uint32_t N) {
+    std::vector<uint32_t> h;
+    for (uint32_t x : g[ep]) {
+        if (h.size() == L) break;
+        if (x < N && std::find(h.begin(), h.end(), x) == h.end()) h.push_back(x);
+    }
+    return h;
+}
+
+/* Entries of node n's candidate search: the head, then the ids of
+ * GenRandom(L) on n's "NSGC" stream that are not in the head, in order, up
+ * to L (the reference fills with rand(), component_candidate.cpp:17-31). */
+void nsg_entries(const std::vector<uint32_t> &head, const std::vector<uint32_t> &head_sorted, uint32_t L,
+                 uint32_t N, uint64_t seed, uint32_t n, std::vector<uint32_t> &out) {
+    out.assign(head.begin(), head.end());
+    std::vector<uint32_t> gen(L);
+    gen_random(stream_of(seed, TAG_NSGC, n), gen.data(), L, N);
+    for (uint32_t x : gen) {
+        if (out.size() == L) break;
+        if (!std::binary_search(head_sorted.begin(), head_sorted.end(), x)) out.push_back(x);
+    }
+}
+
+/* CandidateInner (component_candidate.cpp:10-85): greedy search for point
+ * qv from `ent` with pool L, as RouteGreedy; returns the C smallest
+ * distinct keys of everything evaluated, entries included. */
+void nsg_collect(const Adj &g, const float *base, uint32_t d, uint32_t N, const float *qv,
+                 const std::vector<uint32_t> &ent, uint32_t L, uint32_t C, std::vector<uint32_t> &stamp,
+                 uint32_t &epoch, std::vector<uint64_t> &out) {
+    if (++epoch == 0) {
+        std::fill(stamp.begin(), stamp.end(), 0);
+        epoch = 1;
+    }
+    std::vector<uint64_t> key(L + 1, KEY_EMPTY), all;
+    std::vector<uint8_t> flag(L + 1, 0);
+    for (uint32_t i = 0; i < L; ++i) {
+        key[i] = make_key(dist_canon(qv, base + (size_t)ent[i] * d, d), ent[i]);
+        flag[i] = 1;
+        all.push_back(key[i]);
+    }
+    std::sort(key.begin(), key.begin() + L);
+    uint32_t kk = 0;
+    while (kk < L) {
+        uint32_t nk = L;
+        if (flag[kk]) {
+            flag[kk] = 0;
+            for (uint32_t id : g[key_id(key[kk])]) {
+                if (id >= N || stamp[id] == epoch) continue;
+                stamp[id] = epoch;
+                const uint64_t nkey = make_key(dist_canon(qv, base + (size_t)id * d, d), id);
+                all.push_back(nkey);
+                if (nkey >= key[L - 1]) continue;
+                const uint32_t pos = (uint32_t)(std::lower_bound(key.begin(), key.begin() + L, nkey) - key.begin());
+                if (key[pos] == nkey) continue;
+                std::memmove(&key[pos + 1], &key[pos], (L - 1 - pos) * sizeof(uint64_t));
+                std::memmove(&flag[pos + 1], &flag[pos], L - 1 - pos);
+                key[pos] = nkey;
+                flag[pos] = 1;
+                if (pos < nk) nk = pos;
+            }
+        }
+        kk = (nk <= kk) ? nk : kk + 1;
+    }
+    std::sort(all.begin(), all.end());
+    all.erase(std::unique(all.begin(), all.end()), all.end());
+    if (all.size() > C) all.resize(C);
+    out.swap(all);
+}
+
+/* The occlusion rule of PruneNSG (component_prune.cpp:40-58) over sorted
+ * distinct keys c[0..m) of node `self`: self skipped; each next key is kept
+ * unless a kept one is strictly closer to it than the node is; at most R.
+ * Written as: every kept key removes the later keys it occludes. */
+void nsg_occlude(const float *base, uint32_t d, const uint64_t *c, uint32_t m, uint32_t self, uint32_t R,
+                 std::vector<uint64_t> &res) {
+    res.clear();
+    std::vector<uint8_t> alive(m, 1);
+    for (uint32_t i = 0; i < m && res.size() < R; ++i) {
+        if (!alive[i] || key_id(c[i]) == self) continue;
+        res.push_back(c[i]);
+        const float *ri = base + (size_t)key_id(c[i]) * d;
+        for (uint32_t j = i + 1; j < m; ++j)
+            if (alive[j] && dist_canon(ri, base + (size_t)key_id(c[j]) * d, d) < key_dist(c[j])) alive[j] = 0;
+    }
+}
+
+/* Link (component_refine.cpp:438-472) without the connectivity pass:
+ * candidates, prune, then InterInsert (:474-540) as one batch: node t's
+ * list grows by every s with t in s's pruned list (ascending s, skipping
+ * ones already there); a list above R is sorted, cut to its NSG_TCAP
+ * nearest and re-pruned. */
+std::vector<std::vector<uint64_t>> nsg_link(const Adj &g, const float *base, uint32_t N, uint32_t d, uint32_t ep,
+                                            uint32_t L, uint32_t R, uint32_t C, uint64_t seed, int nth) {
+    const std::vector<uint32_t> head = nsg_head(g, ep, L, N);
+    std::vector<uint32_t> hs(head);
+    std::sort(hs.begin(), hs.end());
+    std::vector<std::vector<uint64_t>> pr(N);
+#pragma omp parallel num_threads(nth)
+    {
+        std::vector<uint32_t> stamp(N, 0), ent;
+        uint32_t epoch = 0;
+        std::vector<uint64_t> F;
+#pragma omp for schedule(dynamic, 16)
+        for (int64_t nn = 0; nn < (int64_t)N; ++nn) {
+            const uint32_t n = (uint32_t)nn;
+            const float *qv = base + (size_t)n * d;
+            nsg_entries(head, hs, L, N, seed, n, ent);
+            nsg_collect(g, base, d, N, qv, ent, L, C, stamp, epoch, F);
+            for (uint32_t id : g[n])  /* PruneNSG adds the node's own neighbours (:26-34) */
+                if (id < N) F.push_back(make_key(dist_canon(qv, base + (size_t)id * d, d), id));
+            std::sort(F.begin(), F.end());
+            F.erase(std::unique(F.begin(), F.end()), F.end());
+            if (F.size() > C) F.resize(C);
+            nsg_occlude(base, d, F.data(), (uint32_t)F.size(), n, R, pr[n]);
+        }
+    }
+    std::vector<std::vector<uint64_t>> inc(N);
+    for (uint32_t n = 0; n < N; ++n)
+        for (uint64_t k : pr[n]) inc[key_id(k)].push_back(make_key(key_dist(k), n));
+    std::vector<std::vector<uint64_t>> out(N);
+#pragma omp parallel for num_threads(nth) schedule(dynamic, 64)
+    for (int64_t tt = 0; tt < (int64_t)N; ++tt) {
+        const uint32_t t = (uint32_t)tt;
+        std::vector<uint64_t> temp(pr[t]);
+        for (uint64_t k : inc[t]) {
+            bool dup = false;
+            for (uint64_t o : pr[t])
+                if (key_id(o) == key_id(k)) { dup = true; break; }
+            if (!dup) temp.push_back(k);
+        }
+        if (temp.size() <= R) {
+            out[t].swap(temp);
+        } else {
+            std::sort(temp.begin(), temp.end());
+            if (temp.size() > NSG_TCAP) temp.resize(NSG_TCAP);
+            nsg_occlude(base, d, temp.data(), (uint32_t)temp.size(), t, R, out[t]);
+        }
+    }
+    return out;
+}
+
+void nsg_mark(const std::vector<std::vector<uint64_t>> &G, uint32_t root, std::vector<uint8_t> &flag) {
+    std::vector<uint32_t> st{root};
+    flag[root] = 1;
+    while (!st.empty()) {
+        const uint32_t u = st.back();
+        st.pop_back();
+        for (uint64_t k : G[u]) {
+            const uint32_t v = key_id(k);
+            if (!flag[v]) {
+                flag[v] = 1;
+                st.push_back(v);
+            }
+        }
+    }
+}
+
+/* tree_grow (component_conn.cpp:14-98): nodes unreachable from ep are linked
+ * in ascending id order.  Each gets one candidate search on the graph as it
+ * stands before any link (the reference searches the growing graph); its
+ * root is the first reached node among the C nearest it evaluated, else the
+ * first reached draw of its "NSGR" stream; the edge root -> node is
+ * appended and the node's reach is marked. */
+uint32_t nsg_conn(std::vector<std::vector<uint64_t>> &G, const float *base, uint32_t N, uint32_t d, uint32_t ep,
+                  uint32_t L, uint32_t C, uint64_t seed, int nth) {
+    std::vector<uint8_t> flag(N, 0);
+    nsg_mark(G, ep, flag);
+    std::vector<uint32_t> un;
+    for (uint32_t i = 0; i < N; ++i)
+        if (!flag[i]) un.push_back(i);
+    if (un.empty()) return 0;
+    Adj g(N);
+    for (uint32_t i = 0; i < N; ++i)
+        for (uint64_t k : G[i]) g[i].push_back(key_id(k));
+    const std::vector<uint32_t> head = nsg_head(g, ep, L, N);
+    std::vector<uint32_t> hs(head);
+    std::sort(hs.begin(), hs.end());
+    std::vector<std::vector<uint64_t>> F(un.size());
+#pragma omp parallel num_threads(nth)
+    {
+        std::vector<uint32_t> stamp(N, 0), ent;
+        uint32_t epoch = 0;
+#pragma omp for schedule(dynamic, 4)
+        for (int64_t i = 0; i < (int64_t)un.size(); ++i) {
+            nsg_entries(head, hs, L, N, seed, un[i], ent);
+            nsg_collect(g, base, d, N, base + (size_t)un[i] * d, ent, L, C, stamp, epoch, F[i]);
+        }
+    }
+    uint32_t edges = 0;
+    for (size_t i = 0; i < un.size(); ++i) {
+        const uint32_t u = un[i];
+        if (flag[u]) continue;
+        uint32_t root = N;
+        float dd = 0.f;
+        for (uint64_t k : F[i])
+            if (flag[key_id(k)]) {
+                root = key_id(k);
+                dd = key_dist(k);
+                break;
+            }
+        if (root == N) {
+            const uint64_t s = stream_of(seed, TAG_NSGR, u);
+            for (uint64_t t = 0; t < (1ull << 26) && root == N; ++t) {
+                const uint32_t x = draw(s, t) % N;
+                if (flag[x]) root = x;
+            }
+            if (root == N) root = ep;
+            dd = dist_canon(base + (size_t)root * d, base + (size_t)u * d, d);
+        }
+        G[root].push_back(make_key(dd, u));
+        ++edges;
+        nsg_mark(G, u, flag);
+    }
+    return edges;
+}
+
+Adj adj_from(const uint32_t *ids, const uint32_t *deg, uint32_t stride, uint32_t N) {
+    Adj g(N);
+    for (uint32_t i = 0; i < N; ++i) {
+        const uint32_t c = std::min(deg[i], stride);
+        g[i].assign(ids + (size_t)i * stride, ids + (size_t)i * stride + c);
+    }
+    return g;
+}
+
+int write_graph(const std::vector<std::vector<uint64_t>> &G, uint32_t *out_ids, float *out_dist, uint32_t *out_deg,
+                uint32_t wcap, uint32_t *width) {
+    uint32_t w = 1;
+    for (const auto &r : G) w = std::max(w, (uint32_t)r.size());
+    if (width) *width = w;
+    if (w > wcap) return -2;
+    for (size_t i = 0; i < G.size(); ++i) {
+        out_deg[i] = (uint32_t)G[i].size();
+        for (uint32_t j = 0; j < wcap; ++j) {
+            const bool ok = j < G[i].size();
+            out_ids[i * wcap + j] = ok ? key_id(G[i][j]) : 0xFFFFFFFFu;
+            if (out_dist) out_dist[i * wcap + j] = ok ? key_dist(G[i][j]) : -1.0f;
+        }
+    }
+    return 0;
+}
+}  // namespace
+
+extern "C" {
+
+int wvo_nsg(const float *base, uint32_t N, uint32_t d, const uint32_t *knn, const uint32_t *kdeg, uint32_t kstride,
+            uint32_t L, uint32_t R, uint32_t C, uint32_t ep, uint64_t seed, uint32_t *out_ids, float *out_dist,
+            uint32_t *out_deg, uint32_t wcap, uint32_t *width, uint32_t *conn_edges, int nthreads) {
+    if (!base || !knn || !kdeg || !out_ids || !out_deg || N < 2 || d == 0) return -1;
+    if (L == 0 || L >= N || R == 0 || C == 0 || ep >= N || kstride == 0) return -1;
+    const int nth = nthreads_or_default(nthreads);
+    const Adj g = adj_from(knn, kdeg, kstride, N);
+    auto G = nsg_link(g, base, N, d, ep, L, R, C, seed, nth);
+    const uint32_t e = nsg_conn(G, base, N, d, ep, L, C, seed, nth);
+    if (conn_edges) *conn_edges = e;
+    return write_graph(G, out_ids, out_dist, out_deg, wcap, width);
+}
+
+int wvo_nsg_conn(const float *base, uint32_t N, uint32_t d, const uint32_t *ids, const uint32_t *deg,
+                 uint32_t stride, uint32_t L, uint32_t C, uint32_t ep, uint64_t seed, uint32_t *out_ids,
+                 float *out_dist, uint32_t *out_deg, uint32_t wcap, uint32_t *width, uint32_t *conn_edges,
+                 int nthreads) {
+    if (!base || !ids || !deg || !out_ids || !out_deg || N < 2 || L == 0 || L >= N || C == 0 || ep >= N) return -1;
+    std::vector<std::vector<uint64_t>> G(N);
+    for (uint32_t i = 0; i < N; ++i)
+        for (uint32_t j = 0; j < std::min(deg[i], stride); ++j) {
+            const uint32_t v = ids[(size_t)i * stride + j];
+            if (v < N) G[i].push_back(make_key(dist_canon(base + (size_t)i * d, base + (size_t)v * d, d), v));
+        }
+    const uint32_t e = nsg_conn(G, base, N, d, ep, L, C, seed, nthreads_or_default(nthreads));
+    if (conn_edges) *conn_edges = e;
+    return write_graph(G, out_ids, out_dist, out_deg, wcap, width);
+}
+
+}  // extern "C"

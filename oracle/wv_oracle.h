@@ -90,6 +90,21 @@ int wvo_nndescent_efanna(const float *base, uint32_t N, uint32_t d,
                          uint32_t *out_ids, float *out_dist, uint32_t *out_deg,
                          wvo_nnd_iter_stats *iter_stats, int nthreads);
 
+/* NSG refine (src/component_refine.cpp:393-540 with ComponentCandidateNSG,
+ * ComponentPruneNSG and ComponentConnNSGDFS) of the kNN graph knn
+ * (N*kstride ids, kdeg[N]) from navigating node ep: L = "L_refine",
+ * R = "R_refine", C = "C_refine".  Writes N*wcap ids/dist and deg[N];
+ * returns -2 with *width set when the widest row exceeds wcap. */
+int wvo_nsg(const float *base, uint32_t N, uint32_t d, const uint32_t *knn, const uint32_t *kdeg,
+            uint32_t kstride, uint32_t L, uint32_t R, uint32_t C, uint32_t ep, uint64_t seed,
+            uint32_t *out_ids, float *out_dist, uint32_t *out_deg, uint32_t wcap, uint32_t *width,
+            uint32_t *conn_edges, int nthreads);
+/* The connectivity pass alone, on graph ids/deg. */
+int wvo_nsg_conn(const float *base, uint32_t N, uint32_t d, const uint32_t *ids, const uint32_t *deg,
+                 uint32_t stride, uint32_t L, uint32_t C, uint32_t ep, uint64_t seed, uint32_t *out_ids,
+                 float *out_dist, uint32_t *out_deg, uint32_t wcap, uint32_t *width, uint32_t *conn_edges,
+                 int nthreads);
+
 /* Greedy search --------------------------------------------------------- */
 /* SearchEntryRand + RouteGreedy (src/component_search_entry.cpp:9-30,
  * src/component_route.cpp:9-53).  graph is N*deg_max ids, deg[N] valid

@@ -31,7 +31,7 @@ EXPORTS = [
     "wv_index_search_dev", "wv_nndescent_build", "wv_search_greedy", "wv_exact_knn",
     "wv_index_search_ext", "wv_index_search_ext_dev", "wv_index_autotune", "wv_search_graph_create",
     "wv_search_graph_launch", "wv_search_graph_destroy", "wv_mlp_grad", "wv_mlp_train",
-    "wv_mlp_predict", "wv_index_build_efanna",
+    "wv_mlp_predict", "wv_index_build_efanna", "wv_index_build_nsg",
 ]
 
 
@@ -52,6 +52,20 @@ class BuildStats(C.Structure):
                 ("join_ms", C.c_double), ("merge_ms", C.c_double), ("finalize_ms", C.c_double),
                 ("pairs", C.c_uint64), ("proposals", C.c_uint64), ("rows_gathered", C.c_uint64),
                 ("iters_run", C.c_uint32), ("cand_slots", C.c_uint32)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+class NsgParams(C.Structure):
+    _fields_ = [("L", C.c_uint32), ("R", C.c_uint32), ("C", C.c_uint32), ("ep", C.c_uint32),
+                ("seed", C.c_uint64)]
+
+
+class NsgStats(C.Structure):
+    _fields_ = [("total_ms", C.c_double), ("search_ms", C.c_double), ("prune_ms", C.c_double),
+                ("insert_ms", C.c_double), ("conn_ms", C.c_double), ("evals", C.c_uint64),
+                ("conn_edges", C.c_uint32), ("width", C.c_uint32)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}
@@ -145,6 +159,7 @@ def load() -> C.CDLL:
         "wv_index_set_search_config": ([vp, vp], i32),
         "wv_index_build_nndescent": ([vp, vp, vp, vp, vp], i32),
         "wv_index_build_efanna": ([vp, vp, vp, vp, u32, vp], i32),
+        "wv_index_build_nsg": ([vp, vp, vp], i32),
         "wv_index_search": ([vp, vp, u32, vp, u64, u32, u32, vp, vp, u32, vp, vp, vp, vp, vp], i32),
         "wv_index_search_dev": ([vp, vp, u32, vp, u64, u32, u32, vp, vp, u32, vp, vp, vp, vp, vp], i32),
         "wv_nndescent_build": ([vp, vp, u32, u32, vp, vp, vp, vp, vp, vp], i32),
@@ -363,6 +378,13 @@ class Index:
         stride = 0 if ii is None else ii.shape[1]
         self.ctx.check(self.ctx.lib.wv_index_build_efanna(self.h, C.byref(p), _p(ii), _p(ic), stride,
                                                           C.byref(st)))
+        return st.as_dict()
+
+    def build_nsg(self, L, R, C_, ep, seed=0):
+        """NSG refine of the index's kNN graph from navigating node ep."""
+        p = NsgParams(L, R, C_, ep, seed)
+        st = NsgStats()
+        self.ctx.check(self.ctx.lib.wv_index_build_nsg(self.h, C.byref(p), C.byref(st)))
         return st.as_dict()
 
     def search(self, queries, L, k, seed=0, entries=None, L_per_q=None, k_per_q=None,

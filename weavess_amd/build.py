@@ -16,7 +16,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libweavess_hip.so")
-SOURCES = ["wv_capi.hip", "wv_search.hip", "wv_exact.hip", "wv_nnd.hip", "wv_mlp.hip"]
+SOURCES = ["wv_capi.hip", "wv_search.hip", "wv_exact.hip", "wv_nnd.hip", "wv_mlp.hip", "wv_nsg.hip"]
 
 HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"

@@ -111,7 +111,7 @@ struct SearchPlan {
 // Pools from this size up get the 256-key merge buffer by default.
 constexpr uint32_t MERGE_BUFFER_MIN_L = 2560;
 
-int plan_search(wv_index *ix, uint32_t Lmax, SearchPlan &pl) {
+int plan_search(wv_index *ix, uint32_t Lmax, SearchPlan &pl, uint32_t C = 0) {
     pl.Lmax = Lmax;
     pl.P = std::max<uint32_t>(64, (Lmax + 63) / 64 * 64);
     // The visited table forgets under pressure (costing re-evaluations only);
@@ -131,7 +131,7 @@ int plan_search(wv_index *ix, uint32_t Lmax, SearchPlan &pl) {
     pl.B = mb == 0 ? (Lmax >= MERGE_BUFFER_MIN_L ? 256u : 0u) : (mb == 1 ? 0u : mb);
     uint32_t wpb = ix->cfg.waves_per_block ? ix->cfg.waves_per_block : 1;
     if (wpb != 1 && wpb != 2 && wpb != 4) return fail(ix->ctx, WV_EINVAL, "waves_per_block must be 1, 2 or 4");
-    size_t lds = wv::search_wave_lds(pl.P, pl.H, pl.B);
+    size_t lds = wv::search_wave_lds(pl.P, pl.H, pl.B, C);
     while (wpb > 1 && lds * wpb > 160 * 1024) wpb >>= 1;
     if (lds > 160 * 1024) return fail(ix->ctx, WV_ENOTSUP, "pool/visited table need %zu B of LDS", lds);
     pl.wpb = wpb;
@@ -146,6 +146,8 @@ struct SearchExtDev {
     float *trace = nullptr;
     uint32_t *trace_len = nullptr, *settle = nullptr;
     uint32_t cap = 0;
+    uint64_t *full = nullptr;  // candidate collection (Q x C keys), or null
+    uint32_t C = 0;
 };
 
 // Launch arguments of one search over device buffers (no allocation).
@@ -155,7 +157,7 @@ int prepare_search(wv_index *ix, const float *d_q, uint32_t Q, const uint32_t *d
                    const SearchExtDev &ext, wv::SearchArgs &a, SearchPlan &pl, int &cpl) {
     wv_ctx *c = ix->ctx;
     if (!ix->graph) return fail(c, WV_EINVAL, "index has no graph (build or set one first)");
-    int rc = plan_search(ix, Lmax, pl);
+    int rc = plan_search(ix, Lmax, pl, ext.C);
     if (rc) return rc;
     cpl = chunks_per_lane(ix->dp / 4);
     if (!cpl) return fail(c, WV_ENOTSUP, "dimension %u above 1024", ix->d);
@@ -182,7 +184,9 @@ int prepare_search(wv_index *ix, const float *d_q, uint32_t Q, const uint32_t *d
     a.P = pl.P;
     a.H = pl.H;
     a.B = pl.B;
-    a.wave_lds = (uint32_t)wv::search_wave_lds(pl.P, pl.H, pl.B);
+    a.wave_lds = (uint32_t)wv::search_wave_lds(pl.P, pl.H, pl.B, ext.C);
+    a.full = ext.full;
+    a.C = ext.full ? ext.C : 0;
     a.model = ext.model;
     a.trace = ext.trace;
     a.trace_len = ext.trace_len;
@@ -381,6 +385,50 @@ IndexView index_view(wv_index *ix) {
     return v;
 }
 int index_set_error(wv_index *ix, int code, const char *msg) { return fail(ix->ctx, code, "%s", msg); }
+
+int search_collect(wv_index *ix, const uint32_t *graph, const uint32_t *deg, uint32_t deg_max, const float *d_q,
+                   uint32_t Q, const uint32_t *d_entries, uint32_t L, uint32_t C, uint64_t *d_full,
+                   uint32_t *d_evals) {
+    wv_ctx *c = ix->ctx;
+    if (Q == 0) return WV_OK;
+    uint32_t *g_save = ix->graph, *d_save = ix->deg, m_save = ix->deg_max;
+    ix->graph = const_cast<uint32_t *>(graph);
+    ix->deg = const_cast<uint32_t *>(deg);
+    ix->deg_max = deg_max;
+    struct Restore {
+        wv_index *ix;
+        uint32_t *g, *d, m;
+        ~Restore() {
+            ix->graph = g;
+            ix->deg = d;
+            ix->deg_max = m;
+        }
+    } restore{ix, g_save, d_save, m_save};
+    DevBuf ids;
+    WV_HIP(c, ids.alloc((size_t)Q * 4));
+    SearchExtDev x;
+    x.full = d_full;
+    x.C = C;
+    wv::SearchArgs a;
+    SearchPlan pl;
+    int cpl = 0;
+    int rc = prepare_search(ix, d_q, Q, d_entries, 0, L, 1, nullptr, nullptr, L, 1, ids.as<uint32_t>(), nullptr,
+                            d_evals, nullptr, x, a, pl, cpl);
+    if (rc) return rc;
+    WV_HIP(c, wv::launch_greedy_search(a, cpl, (int)pl.groups, (int)pl.wpb, c->stream));
+    WV_HIP(c, hipStreamSynchronize(c->stream));
+    return WV_OK;
+}
+
+int index_upload_graph(wv_index *ix, const uint32_t *ids, const float *dist, const uint32_t *deg, uint32_t W) {
+    wv_ctx *c = ix->ctx;
+    int rc = index_alloc_graph(ix, W);
+    if (rc) return rc;
+    WV_HIP(c, hipMemcpy(ix->graph, ids, (size_t)ix->N * W * 4, hipMemcpyHostToDevice));
+    WV_HIP(c, hipMemcpy(ix->gdist, dist, (size_t)ix->N * W * 4, hipMemcpyHostToDevice));
+    WV_HIP(c, hipMemcpy(ix->deg, deg, (size_t)ix->N * 4, hipMemcpyHostToDevice));
+    return WV_OK;
+}
 }  // namespace wv
 
 extern "C" {
@@ -986,6 +1034,23 @@ int wv_index_build_efanna(wv_index *ix, const wv_nnd_params *p, const uint32_t *
     if (init_ids && (init_stride == 0 || init_stride > p->L))
         return fail(c, WV_EINVAL, "need 1 <= init_stride <= L (init_stride=%u L=%u)", init_stride, p->L);
     return run_nnd(ix, p, init_ids, init_cnt, stats, true, init_stride);
+}
+
+int wv_index_build_nsg(wv_index *ix, const wv_nsg_params *p, wv_nsg_stats *stats) {
+    if (!ix) return WV_EINVAL;
+    wv_ctx *c = ix->ctx;
+    std::lock_guard<std::mutex> g(c->mu);
+    if (!p) return fail(c, WV_EINVAL, "params are null");
+    if (!ix->graph) return fail(c, WV_EINVAL, "index has no graph (build or set the kNN graph first)");
+    if (p->L == 0 || p->L >= ix->N) return fail(c, WV_EINVAL, "need 1 <= L < N (L=%u N=%u)", p->L, ix->N);
+    if (p->L > 1024 || p->C == 0 || p->C > 1024)
+        return fail(c, WV_ENOTSUP, "need L <= 1024 and 1 <= C <= 1024 (L=%u C=%u)", p->L, p->C);
+    if (p->R == 0 || p->R > 256) return fail(c, WV_ENOTSUP, "need 1 <= R <= 256 (R=%u)", p->R);
+    if (p->ep >= ix->N) return fail(c, WV_EINVAL, "ep %u out of range", p->ep);
+    if (ix->dp > 1024) return fail(c, WV_ENOTSUP, "dimension above 1024");
+    WV_HIP(c, hipSetDevice(c->device));
+    // the kNN graph stays alive until the result replaces it
+    return wv::nsg_build(ix, p, stats, ix->graph, ix->deg, ix->deg_max);
 }
 
 int wv_nndescent_build(wv_ctx *c, const float *base, uint32_t N, uint32_t d,

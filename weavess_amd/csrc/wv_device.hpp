@@ -26,10 +26,17 @@ constexpr uint64_t TAG_EFANNA = 0x45464E41ull;  // "EFNA": EFANNA's first-round 
 // non-negative finite distances; ties between equal distances go to the
 // smaller id.
 __host__ __device__ inline uint64_t make_key(float d, uint32_t id) {
-    return ((uint64_t)__float_as_uint(d) << 32) | id;
+    uint32_t b;
+    __builtin_memcpy(&b, &d, 4);
+    return ((uint64_t)b << 32) | id;
 }
 __host__ __device__ inline uint32_t key_id(uint64_t k) { return (uint32_t)k; }
-__host__ __device__ inline float key_dist(uint64_t k) { return __uint_as_float((uint32_t)(k >> 32)); }
+__host__ __device__ inline float key_dist(uint64_t k) {
+    const uint32_t b = (uint32_t)(k >> 32);
+    float f;
+    __builtin_memcpy(&f, &b, 4);
+    return f;
+}
 
 // ---------------------------------------------------------------- hashing
 // splitmix64 finaliser (Steele, Lea, Flood 2014) and murmur3's fmix32.

@@ -35,9 +35,12 @@ struct SearchArgs {
     uint32_t *trace_len;      // Q
     uint32_t *settle;         // Q
     uint32_t trace_cap;
+    uint64_t *full;           // Q x C: the C smallest distinct keys evaluated (NSG candidates), or null
+    uint32_t C;
 };
 
-size_t search_wave_lds(uint32_t P, uint32_t H, uint32_t B);
+// LDS bytes per wavefront; C > 0 adds the collected candidates and a block.
+size_t search_wave_lds(uint32_t P, uint32_t H, uint32_t B, uint32_t C = 0);
 // One wavefront per query (grid of ceil(Q / waves_per_block) workgroups).
 hipError_t launch_greedy_search(const SearchArgs &a, int chunks_per_lane, int row_groups,
                                 int waves_per_block, hipStream_t s);

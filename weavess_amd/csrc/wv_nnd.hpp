@@ -28,4 +28,19 @@ int nndescent_build(wv_index *ix, const wv_nnd_params *p, const uint32_t *init_i
                     const uint32_t *init_cnt, wv_build_stats *stats, uint32_t *graph_out,
                     float *gdist_out, uint32_t *deg_out, bool efanna = false, uint32_t init_stride = 0);
 
+// Implemented in wv_capi.hip.  search_collect: greedy searches of Q
+// queries (device rows, dp floats each) over the device graph (graph, deg,
+// deg_max) from explicit entries (Q x L), writing each query's C smallest
+// distinct evaluated keys (Q x C, KEY_EMPTY-padded) and evaluation counts.
+int search_collect(wv_index *ix, const uint32_t *graph, const uint32_t *deg, uint32_t deg_max, const float *d_q,
+                   uint32_t Q, const uint32_t *d_entries, uint32_t L, uint32_t C, uint64_t *d_full,
+                   uint32_t *d_evals);
+// Replaces the index's graph with host arrays (N x W ids / dists, deg[N]).
+int index_upload_graph(wv_index *ix, const uint32_t *ids, const float *dist, const uint32_t *deg, uint32_t W);
+
+// Implemented in wv_nsg.hip: the NSG refine of the kNN graph (knn, kdeg,
+// kstride on the device); the result replaces the index's graph.
+int nsg_build(wv_index *ix, const wv_nsg_params *p, wv_nsg_stats *stats, const uint32_t *knn, const uint32_t *kdeg,
+              uint32_t kstride);
+
 }  // namespace wv

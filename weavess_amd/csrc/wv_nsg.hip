@@ -1,0 +1,690 @@
+// wv_nsg.hip -- NSG refine of a kNN graph on the GPU (ComponentRefineNSG,
+// src/component_refine.cpp:393-540, with ComponentCandidateNSG,
+// ComponentPruneNSG and ComponentConnNSGDFS).
+//
+//   1. candidates: per node, a greedy search for the node's own vector over
+//      the kNN graph from ep's neighbours plus random ids, pool L, keeping
+//      the C nearest distinct nodes it evaluated (the search kernel's
+//      collection mode);
+//   2. prune: those plus the node's kNN neighbours, sorted, first C; the
+//      occlusion rule keeps up to R;
+//   3. reverse insertion as one batch: every node's list grows by the nodes
+//      that kept it, and a list above R is re-pruned;
+//   4. connectivity: nodes unreachable from ep are linked to a reached node
+//      near them (host traversal, GPU searches).
+//
+// Distances are the build-order FP32 distance everywhere, so every decision
+// matches the CPU restatement bit for bit.
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <vector>
+
+#include "wv_device.hpp"
+#include "wv_internal.hpp"
+#include "wv_nnd.hpp"
+#include "wv_pool.hpp"
+
+namespace wv {
+namespace {
+
+constexpr uint64_t TAG_NSGC = 0x4E534743ull;  // "NSGC": candidate-search entries
+constexpr uint64_t TAG_NSGR = 0x4E534752ull;  // "NSGR": connectivity fallback root
+constexpr uint32_t NSG_TCAP = 1024;           // candidates of one reverse-insert re-prune
+constexpr int NWPB = 4;                       // wavefronts per block
+
+#define NSG_HIP(call)                                                                        \
+    do {                                                                                     \
+        hipError_t e_ = (call);                                                              \
+        if (e_ != hipSuccess) {                                                              \
+            char m_[256];                                                                    \
+            snprintf(m_, sizeof m_, "%s: %s (%s:%d)", #call, hipGetErrorString(e_), __FILE__, \
+                     __LINE__);                                                              \
+            return index_set_error(ix, e_ == hipErrorOutOfMemory ? WV_ENOMEM : WV_EHIP, m_); \
+        }                                                                                    \
+    } while (0)
+
+struct NsgArgs {
+    const float *base;
+    uint32_t N, dp, nch;
+    const uint32_t *knn;   // N x kstride kNN graph
+    const uint32_t *kdeg;
+    uint32_t kstride;
+    uint32_t R, C;
+    const uint64_t *full;  // batch x C collected keys
+    uint32_t n0, nb;       // batch: nodes n0 .. n0+nb-1
+    uint32_t *pr_ids;      // N x R pruned lists
+    float *pr_dist;
+    uint32_t *pr_cnt;
+    const uint32_t *inoff, *insrc;  // reverse edges (CSR)
+    const float *indist;
+    uint32_t *out_ids;     // N x R after the reverse insertion
+    float *out_dist;
+    uint32_t *out_cnt;
+    uint32_t lds_wave;
+};
+
+template <int CPL>
+constexpr int rows_g() { return CPL >= 16 ? 1 : CPL >= 8 ? 2 : 4; }
+
+template <int CPL>
+__device__ inline void load_row(const float *base, uint32_t dp, uint32_t nch, uint32_t id, float4 (&qv)[CPL]) {
+    const float4 *qp = reinterpret_cast<const float4 *>(base + (size_t)id * dp);
+    const int lane = lane_id();
+#pragma unroll
+    for (int c = 0; c < CPL; ++c) {
+        const int ch = (lane & 15) + 16 * c;
+        qv[c] = ch < (int)nch ? qp[ch] : make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+}
+
+// Occlusion (component_prune.cpp:40-58) over the sorted distinct keys
+// cand[0..m) of one node (key distance = distance to the node), entries with
+// alive[i] == 0 skipped: the first alive key is kept, and every kept key
+// clears the later alive keys it is strictly closer to than the node is; at
+// most R kept, written to res.  Returns the number kept.
+template <int CPL>
+__device__ int occlude(const float *base, uint32_t dp, uint32_t nch, const uint64_t *cand, int m, uint8_t *alive,
+                       int R, uint32_t *cid, uint32_t *cpos, uint64_t *ckey, uint64_t *res) {
+    const int lane = lane_id();
+    int cnt = 0, from = 0;
+    while (cnt < R) {
+        int nxt = m;
+        for (int t0 = from & ~(WAVE - 1); t0 < m; t0 += WAVE) {
+            const int p = t0 + lane;
+            const uint64_t b = __ballot(p >= from && p < m && alive[p]);
+            if (b) {
+                nxt = t0 + __ffsll((unsigned long long)b) - 1;
+                break;
+            }
+        }
+        if (nxt >= m) break;
+        const uint64_t ki = cand[nxt];
+        if (lane == 0) res[cnt] = ki;
+        ++cnt;
+        from = nxt + 1;
+        if (cnt >= R) break;
+        float4 qv[CPL];
+        load_row<CPL>(base, dp, nch, key_id(ki), qv);
+        for (int t0 = from & ~(WAVE - 1); t0 < m; t0 += WAVE) {
+            const int p = t0 + lane;
+            const bool ok = p >= from && p < m && alive[p];
+            const uint64_t b = __ballot(ok);
+            if (!b) continue;
+            const int mm = __popcll(b);
+            if (ok) {
+                cid[lanes_below(b)] = key_id(cand[p]);
+                cpos[lanes_below(b)] = (uint32_t)p;
+            }
+            wave_sync();
+            rows_dist<CPL, rows_g<CPL>()>(base, dp, nch, qv, cid, mm, ckey);
+            wave_sync();
+            if (lane < mm) {
+                const uint32_t pp = cpos[lane];
+                if (key_dist(ckey[lane]) < key_dist(cand[pp])) alive[pp] = 0;
+            }
+            wave_sync();
+        }
+    }
+    wave_sync();
+    return cnt;
+}
+
+// Entries of the candidate searches of nodes[q] (q-th query; nodes null:
+// node n0 + q): the h head ids (ep's list), then the ids of GenRandom(L) on
+// the node's "NSGC" stream that are not in the head (sorted copy hs), in
+// order, up to L.
+__global__ void __launch_bounds__(256) nsg_entries_kernel(const uint32_t *nodes, uint32_t n0, uint32_t Q,
+                                                          const uint32_t *head, const uint32_t *hs, uint32_t h,
+                                                          uint32_t L, uint32_t N, uint64_t seed, uint32_t lds_wave,
+                                                          uint32_t *out) {
+    extern __shared__ __align__(16) uint8_t smem[];
+    const int wid = threadIdx.x >> 6, lane = lane_id();
+    const uint32_t q = blockIdx.x * (blockDim.x >> 6) + wid;
+    if (q >= Q) return;
+    const uint32_t n = nodes ? nodes[q] : n0 + q;
+    uint64_t *a = reinterpret_cast<uint64_t *>(smem + (size_t)wid * lds_wave);
+    uint64_t *tmp = a + ((L + 63) & ~63u);
+    gen_random_lds(stream_of(seed, TAG_NSGC, n), L, N, a, tmp);
+    uint32_t *o = out + (size_t)q * L;
+    for (uint32_t i = lane; i < h; i += WAVE) o[i] = head[i];
+    uint32_t w = h;
+    for (uint32_t c0 = 0; c0 < L && w < L; c0 += WAVE) {
+        const uint32_t i = c0 + lane;
+        const uint32_t x = i < L ? (uint32_t)a[i] : ID_NONE;
+        bool keep = i < L;
+        if (keep) {  // not in the head
+            uint32_t lo = 0, hi = h;
+            while (lo < hi) {
+                const uint32_t mid = (lo + hi) >> 1;
+                if (hs[mid] < x) lo = mid + 1;
+                else hi = mid;
+            }
+            keep = !(lo < h && hs[lo] == x);
+        }
+        const uint64_t bm = __ballot(keep);
+        const uint32_t pos = w + lanes_below(bm);
+        if (keep && pos < L) o[pos] = x;
+        w += (uint32_t)__popcll(bm);
+    }
+}
+
+// PruneNSG (component_prune.cpp:20-66) for nodes n0 .. n0+nb-1: the
+// collected keys plus the node's kNN neighbours (distinct, sorted, first C),
+// self skipped, occlusion to R.
+template <int CPL>
+__global__ void __launch_bounds__(256) nsg_prune_kernel(NsgArgs a) {
+    extern __shared__ __align__(16) uint8_t smem[];
+    const int wid = threadIdx.x >> 6, lane = lane_id();
+    const uint32_t q = blockIdx.x * (blockDim.x >> 6) + wid;
+    if (q >= a.nb) return;
+    const uint32_t n = a.n0 + q;
+    uint8_t *my = smem + (size_t)wid * a.lds_wave;
+    uint64_t *cand = reinterpret_cast<uint64_t *>(my);  // C
+    uint64_t *ckey = cand + a.C;                         // 64
+    uint64_t *tmp = ckey + WAVE;                         // 64
+    uint64_t *res = tmp + WAVE;                          // R
+    uint32_t *cid = reinterpret_cast<uint32_t *>(res + a.R);  // 64
+    uint32_t *cpos = cid + WAVE;                              // 64
+    uint8_t *alive = reinterpret_cast<uint8_t *>(cpos + WAVE);  // C
+    const int C = (int)a.C;
+    int len = 0;
+    for (int t0 = 0; t0 < C; t0 += WAVE) {
+        const int i = t0 + lane;
+        const uint64_t k = i < C ? a.full[(size_t)q * a.C + i] : KEY_EMPTY;
+        if (i < C) cand[i] = k;
+        len += __popcll(__ballot(k != KEY_EMPTY));
+    }
+    wave_sync();
+    float4 qv[CPL];
+    load_row<CPL>(a.base, a.dp, a.nch, n, qv);
+    const uint32_t dg = min(a.kdeg[n], a.kstride);
+    for (uint32_t c0 = 0; c0 < dg; c0 += WAVE) {
+        const uint32_t id = c0 + lane < dg ? a.knn[(size_t)n * a.kstride + c0 + lane] : ID_NONE;
+        const bool ok = id < a.N;
+        const uint64_t b = __ballot(ok);
+        const int m = __popcll(b);
+        if (ok) cid[lanes_below(b)] = id;
+        wave_sync();
+        if (m == 0) continue;
+        rows_dist<CPL, rows_g<CPL>()>(a.base, a.dp, a.nch, qv, cid, m, ckey);
+        wave_sync();
+        len = merge_distinct(cand, len, C, ckey, m, tmp);
+    }
+    for (int i = lane; i < len; i += WAVE) alive[i] = key_id(cand[i]) != n;
+    wave_sync();
+    const int cnt = occlude<CPL>(a.base, a.dp, a.nch, cand, len, alive, (int)a.R, cid, cpos, ckey, res);
+    for (int i = lane; i < cnt; i += WAVE) {
+        a.pr_ids[(size_t)n * a.R + i] = key_id(res[i]);
+        a.pr_dist[(size_t)n * a.R + i] = key_dist(res[i]);
+    }
+    if (lane == 0) a.pr_cnt[n] = (uint32_t)cnt;
+}
+
+__global__ void nsg_rev_count_kernel(const uint32_t *pr_ids, const uint32_t *pr_cnt, uint32_t N, uint32_t R,
+                                     uint32_t *incnt) {
+    for (uint32_t n = blockIdx.x * blockDim.x + threadIdx.x; n < N; n += gridDim.x * blockDim.x)
+        for (uint32_t j = 0; j < pr_cnt[n]; ++j) atomicAdd(&incnt[pr_ids[(size_t)n * R + j]], 1u);
+}
+
+__global__ void nsg_rev_scatter_kernel(const uint32_t *pr_ids, const float *pr_dist, const uint32_t *pr_cnt,
+                                       uint32_t N, uint32_t R, const uint32_t *inoff, uint32_t *incur,
+                                       uint32_t *insrc, float *indist) {
+    for (uint32_t n = blockIdx.x * blockDim.x + threadIdx.x; n < N; n += gridDim.x * blockDim.x)
+        for (uint32_t j = 0; j < pr_cnt[n]; ++j) {
+            const uint32_t t = pr_ids[(size_t)n * R + j];
+            const uint32_t pos = inoff[t] + atomicAdd(&incur[t], 1u);
+            insrc[pos] = n;
+            indist[pos] = pr_dist[(size_t)n * R + j];
+        }
+}
+
+// InterInsert (component_refine.cpp:474-540) as one batch: node t's list
+// grows by every s whose pruned list holds t and that t's list does not
+// hold, in ascending s; above R, own + incoming are sorted, cut to the
+// NSG_TCAP nearest and re-pruned by occlusion.
+template <int CPL>
+__global__ void __launch_bounds__(256) nsg_insert_kernel(NsgArgs a) {
+    extern __shared__ __align__(16) uint8_t smem[];
+    const int wid = threadIdx.x >> 6, lane = lane_id();
+    const uint32_t t = blockIdx.x * (blockDim.x >> 6) + wid;
+    if (t >= a.N) return;
+    uint8_t *my = smem + (size_t)wid * a.lds_wave;
+    uint64_t *cand = reinterpret_cast<uint64_t *>(my);  // NSG_TCAP
+    uint64_t *ckey = cand + NSG_TCAP;                    // 64
+    uint64_t *tmp = ckey + WAVE;                         // 64
+    uint64_t *res = tmp + WAVE;                          // R
+    uint64_t *own = res + a.R;                           // R
+    uint32_t *cid = reinterpret_cast<uint32_t *>(own + a.R);  // 64
+    uint32_t *cpos = cid + WAVE;                              // 64
+    uint32_t *srt = cpos + WAVE;                              // pow2 >= 2R
+    uint8_t *alive = reinterpret_cast<uint8_t *>(srt + a.C);  // NSG_TCAP (a.C holds the sort width here)
+    const int R = (int)a.R;
+    const int no = (int)a.pr_cnt[t];
+    for (int i = lane; i < no; i += WAVE)
+        own[i] = make_key(a.pr_dist[(size_t)t * a.R + i], a.pr_ids[(size_t)t * a.R + i]);
+    wave_sync();
+    const uint32_t s0 = a.inoff[t], s1 = a.inoff[t + 1];
+    // incoming edges not already in the list
+    int nnew = 0;
+    for (uint32_t c0 = s0; c0 < s1; c0 += WAVE) {
+        const uint32_t i = c0 + lane;
+        bool fresh = false;
+        if (i < s1) {
+            const uint64_t k = make_key(a.indist[i], a.insrc[i]);
+            const int p = lower_bound_e(own, no, k);
+            fresh = !(p < no && own[p] == k);
+        }
+        nnew += __popcll(__ballot(fresh));
+    }
+    if (no + nnew <= R) {
+        // own list, then the new sources in ascending id order
+        const uint32_t w = a.C;  // power of two >= 2R, >= the segment here
+        for (uint32_t i = lane; i < w; i += WAVE) {
+            uint32_t v = ID_NONE;
+            if (s0 + i < s1) {
+                const uint64_t k = make_key(a.indist[s0 + i], a.insrc[s0 + i]);
+                const int p = lower_bound_e(own, no, k);
+                if (!(p < no && own[p] == k)) v = a.insrc[s0 + i];
+            }
+            srt[i] = v;
+        }
+        wave_sync();
+        lds_sort32(srt, (int)w);
+        for (int i = lane; i < no; i += WAVE) {
+            a.out_ids[(size_t)t * a.R + i] = key_id(own[i]);
+            a.out_dist[(size_t)t * a.R + i] = key_dist(own[i]);
+        }
+        for (int i = lane; i < nnew; i += WAVE) {
+            const uint32_t s = srt[i];
+            // the distance of edge s -> t: find it in the segment
+            float dd = 0.f;
+            for (uint32_t j = s0; j < s1; ++j)
+                if (a.insrc[j] == s) {
+                    dd = a.indist[j];
+                    break;
+                }
+            a.out_ids[(size_t)t * a.R + no + i] = s;
+            a.out_dist[(size_t)t * a.R + no + i] = dd;
+        }
+        if (lane == 0) a.out_cnt[t] = (uint32_t)(no + nnew);
+        return;
+    }
+    // re-prune: the NSG_TCAP smallest distinct keys of own + incoming
+    int len = 0;
+    for (int i = lane; i < no; i += WAVE) cand[i] = own[i];
+    len = min(no, (int)NSG_TCAP);
+    wave_sync();
+    for (uint32_t c0 = s0; c0 < s1; c0 += WAVE) {
+        const uint32_t i = c0 + lane;
+        if (i < s1) ckey[lane] = make_key(a.indist[i], a.insrc[i]);
+        wave_sync();
+        len = merge_distinct(cand, len, (int)NSG_TCAP, ckey, (int)min((uint32_t)WAVE, s1 - c0), tmp);
+    }
+    for (int i = lane; i < len; i += WAVE) alive[i] = 1;
+    wave_sync();
+    const int cnt = occlude<CPL>(a.base, a.dp, a.nch, cand, len, alive, R, cid, cpos, ckey, res);
+    for (int i = lane; i < cnt; i += WAVE) {
+        a.out_ids[(size_t)t * a.R + i] = key_id(res[i]);
+        a.out_dist[(size_t)t * a.R + i] = key_dist(res[i]);
+    }
+    if (lane == 0) a.out_cnt[t] = (uint32_t)cnt;
+}
+
+// The build-order distance key of rows u and v (one wavefront).
+template <int CPL>
+__global__ void pair_key_kernel(const float *base, uint32_t dp, uint32_t nch, uint32_t u, uint32_t v, uint64_t *out) {
+    __shared__ uint32_t cid[WAVE];
+    __shared__ uint64_t ck[WAVE];
+    float4 qv[CPL];
+    load_row<CPL>(base, dp, nch, u, qv);
+    if (lane_id() == 0) cid[0] = v;
+    wave_sync();
+    rows_dist<CPL, rows_g<CPL>()>(base, dp, nch, qv, cid, 1, ck);
+    wave_sync();
+    if (lane_id() == 0) out[0] = ck[0];
+}
+
+hipError_t pair_key(int cpl, const float *base, uint32_t dp, uint32_t nch, uint32_t u, uint32_t v, uint64_t *out,
+                    hipStream_t s) {
+    switch (cpl) {
+    case 1: hipLaunchKernelGGL(pair_key_kernel<1>, dim3(1), dim3(WAVE), 0, s, base, dp, nch, u, v, out); break;
+    case 2: hipLaunchKernelGGL(pair_key_kernel<2>, dim3(1), dim3(WAVE), 0, s, base, dp, nch, u, v, out); break;
+    case 4: hipLaunchKernelGGL(pair_key_kernel<4>, dim3(1), dim3(WAVE), 0, s, base, dp, nch, u, v, out); break;
+    case 8: hipLaunchKernelGGL(pair_key_kernel<8>, dim3(1), dim3(WAVE), 0, s, base, dp, nch, u, v, out); break;
+    default: hipLaunchKernelGGL(pair_key_kernel<16>, dim3(1), dim3(WAVE), 0, s, base, dp, nch, u, v, out); break;
+    }
+    return hipGetLastError();
+}
+
+__global__ void gather_rows_kernel(const float *base, uint32_t dp, const uint32_t *ids, uint32_t n, float *out) {
+    const size_t total = (size_t)n * dp;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < total; i += (size_t)gridDim.x * blockDim.x)
+        out[i] = base[(size_t)ids[i / dp] * dp + i % dp];
+}
+
+template <class K>
+hipError_t set_lds_k(K k, size_t bytes) {
+    if (bytes <= 65536) return hipSuccess;
+    return hipFuncSetAttribute(reinterpret_cast<const void *>(k), hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)bytes);
+}
+
+struct DBuf {
+    std::vector<void *> ps;
+    ~DBuf() {
+        for (void *p : ps) (void)hipFree(p);
+    }
+    template <class T>
+    hipError_t get(T **p, size_t n) {
+        void *q = nullptr;
+        hipError_t e = hipMalloc(&q, std::max<size_t>(n, 1) * sizeof(T));
+        if (e == hipSuccess) {
+            ps.push_back(q);
+            *p = static_cast<T *>(q);
+        }
+        return e;
+    }
+};
+
+uint32_t pow2_ge(uint32_t v, uint32_t lo) {
+    uint32_t p = lo;
+    while (p < v) p <<= 1;
+    return p;
+}
+
+template <int CPL>
+hipError_t launch_prune(const NsgArgs &a, hipStream_t s) {
+    const size_t lds = (size_t)NWPB * a.lds_wave;
+    hipError_t e = set_lds_k(nsg_prune_kernel<CPL>, lds);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(nsg_prune_kernel<CPL>, dim3((a.nb + NWPB - 1) / NWPB), dim3(64 * NWPB), lds, s, a);
+    return hipGetLastError();
+}
+template <int CPL>
+hipError_t launch_insert(const NsgArgs &a, hipStream_t s) {
+    const size_t lds = (size_t)NWPB * a.lds_wave;
+    hipError_t e = set_lds_k(nsg_insert_kernel<CPL>, lds);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(nsg_insert_kernel<CPL>, dim3((a.N + NWPB - 1) / NWPB), dim3(64 * NWPB), lds, s, a);
+    return hipGetLastError();
+}
+
+hipError_t dispatch_prune(int cpl, const NsgArgs &a, hipStream_t s) {
+    switch (cpl) {
+    case 1: return launch_prune<1>(a, s);
+    case 2: return launch_prune<2>(a, s);
+    case 4: return launch_prune<4>(a, s);
+    case 8: return launch_prune<8>(a, s);
+    default: return launch_prune<16>(a, s);
+    }
+}
+hipError_t dispatch_insert(int cpl, const NsgArgs &a, hipStream_t s) {
+    switch (cpl) {
+    case 1: return launch_insert<1>(a, s);
+    case 2: return launch_insert<2>(a, s);
+    case 4: return launch_insert<4>(a, s);
+    case 8: return launch_insert<8>(a, s);
+    default: return launch_insert<16>(a, s);
+    }
+}
+
+// ep's first min(L, degree) distinct in-range ids.
+std::vector<uint32_t> head_of(const uint32_t *row, uint32_t deg, uint32_t L, uint32_t N) {
+    std::vector<uint32_t> h;
+    for (uint32_t j = 0; j < deg && h.size() < L; ++j)
+        if (row[j] < N && std::find(h.begin(), h.end(), row[j]) == h.end()) h.push_back(row[j]);
+    return h;
+}
+
+double ms_since(std::chrono::high_resolution_clock::time_point t0) {
+    return std::chrono::duration<double, std::milli>(std::chrono::high_resolution_clock::now() - t0).count();
+}
+
+}  // namespace
+
+int nsg_build(wv_index *ix, const wv_nsg_params *p, wv_nsg_stats *stats, const uint32_t *knn, const uint32_t *kdeg,
+              uint32_t kstride) {
+    // knn / kdeg: the index's kNN graph on the device (kstride ids a row)
+    const IndexView v = index_view(ix);
+    const hipStream_t s = v.stream;
+    const uint32_t N = v.N, L = p->L, R = p->R, C = p->C, ep = p->ep;
+    const uint32_t nch = v.dp / 4;
+    const int cpl = nch <= 16 ? 1 : nch <= 32 ? 2 : nch <= 64 ? 4 : nch <= 128 ? 8 : 16;
+    const auto t_all = std::chrono::high_resolution_clock::now();
+    DBuf B;
+    NsgArgs a{};
+    a.base = v.base;
+    a.N = N;
+    a.dp = v.dp;
+    a.nch = nch;
+    a.knn = knn;
+    a.kdeg = kdeg;
+    a.kstride = kstride;
+    a.R = R;
+    a.C = C;
+    NSG_HIP(B.get(&a.pr_ids, (size_t)N * R));
+    NSG_HIP(B.get(&a.pr_dist, (size_t)N * R));
+    NSG_HIP(B.get(&a.pr_cnt, N));
+
+    // entries head: ep's kNN list
+    std::vector<uint32_t> eprow(kstride), epdeg(1);
+    NSG_HIP(hipMemcpy(eprow.data(), knn + (size_t)ep * kstride, (size_t)kstride * 4, hipMemcpyDeviceToHost));
+    NSG_HIP(hipMemcpy(epdeg.data(), kdeg + ep, 4, hipMemcpyDeviceToHost));
+    std::vector<uint32_t> head = head_of(eprow.data(), std::min(epdeg[0], kstride), L, N), hs;
+    hs = head;
+    std::sort(hs.begin(), hs.end());
+    uint32_t *d_head = nullptr, *d_hs = nullptr;
+    NSG_HIP(B.get(&d_head, head.size()));
+    NSG_HIP(B.get(&d_hs, head.size()));
+    if (!head.empty()) {
+        NSG_HIP(hipMemcpy(d_head, head.data(), head.size() * 4, hipMemcpyHostToDevice));
+        NSG_HIP(hipMemcpy(d_hs, hs.data(), hs.size() * 4, hipMemcpyHostToDevice));
+    }
+    const uint32_t ent_lds = (uint32_t)((((L + 63) & ~63u) + 64) * 8);
+    NSG_HIP(set_lds_k(nsg_entries_kernel, (size_t)NWPB * ent_lds));
+
+    // 1-2. candidates and prune, in node batches
+    const uint32_t BN = std::min<uint32_t>(N, 1u << 18);
+    uint32_t *d_ent = nullptr, *d_evals = nullptr;
+    uint64_t *d_full = nullptr;
+    NSG_HIP(B.get(&d_ent, (size_t)BN * L));
+    NSG_HIP(B.get(&d_full, (size_t)BN * C));
+    NSG_HIP(B.get(&d_evals, BN));
+    a.full = d_full;
+    a.lds_wave = (uint32_t)((((size_t)C * 8 + 64 * 8 * 2 + (size_t)R * 8 + 64 * 4 * 2 + C) + 15) & ~(size_t)15);
+    uint64_t evals = 0;
+    double search_ms = 0, prune_ms = 0;
+    std::vector<uint32_t> hev(BN);
+    for (uint32_t n0 = 0; n0 < N; n0 += BN) {
+        const uint32_t nb = std::min(BN, N - n0);
+        auto t0 = std::chrono::high_resolution_clock::now();
+        hipLaunchKernelGGL(nsg_entries_kernel, dim3((nb + NWPB - 1) / NWPB), dim3(64 * NWPB), NWPB * ent_lds, s,
+                           nullptr, n0, nb, d_head, d_hs, (uint32_t)head.size(), L, N, p->seed, ent_lds, d_ent);
+        NSG_HIP(hipGetLastError());
+        int rc = search_collect(ix, knn, kdeg, kstride, v.base + (size_t)n0 * v.dp, nb, d_ent, L, C, d_full,
+                                d_evals);
+        if (rc) return rc;
+        NSG_HIP(hipMemcpyAsync(hev.data(), d_evals, (size_t)nb * 4, hipMemcpyDeviceToHost, s));
+        NSG_HIP(hipStreamSynchronize(s));
+        for (uint32_t i = 0; i < nb; ++i) evals += hev[i];
+        search_ms += ms_since(t0);
+        t0 = std::chrono::high_resolution_clock::now();
+        a.n0 = n0;
+        a.nb = nb;
+        NSG_HIP(dispatch_prune(cpl, a, s));
+        NSG_HIP(hipStreamSynchronize(s));
+        prune_ms += ms_since(t0);
+    }
+
+    // 3. reverse insertion
+    auto t0 = std::chrono::high_resolution_clock::now();
+    uint32_t *incnt = nullptr, *inoff = nullptr, *incur = nullptr, *insrc = nullptr;
+    float *indist = nullptr;
+    NSG_HIP(B.get(&incnt, (size_t)N + 1));
+    NSG_HIP(B.get(&inoff, (size_t)N + 1));
+    NSG_HIP(B.get(&incur, N));
+    NSG_HIP(B.get(&insrc, (size_t)N * R));
+    NSG_HIP(B.get(&indist, (size_t)N * R));
+    NSG_HIP(hipMemsetAsync(incnt, 0, ((size_t)N + 1) * 4, s));
+    NSG_HIP(hipMemsetAsync(incur, 0, (size_t)N * 4, s));
+    hipLaunchKernelGGL(nsg_rev_count_kernel, dim3(1024), dim3(256), 0, s, a.pr_ids, a.pr_cnt, N, R, incnt);
+    NSG_HIP(hipGetLastError());
+    size_t scan_bytes = 0;
+    NSG_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, scan_bytes, incnt, inoff, N + 1, s));
+    uint8_t *scan_tmp = nullptr;
+    NSG_HIP(B.get(&scan_tmp, scan_bytes));
+    NSG_HIP(hipcub::DeviceScan::ExclusiveSum(scan_tmp, scan_bytes, incnt, inoff, N + 1, s));
+    hipLaunchKernelGGL(nsg_rev_scatter_kernel, dim3(1024), dim3(256), 0, s, a.pr_ids, a.pr_dist, a.pr_cnt, N, R,
+                       inoff, incur, insrc, indist);
+    NSG_HIP(hipGetLastError());
+    a.inoff = inoff;
+    a.insrc = insrc;
+    a.indist = indist;
+    NSG_HIP(B.get(&a.out_ids, (size_t)N * R));
+    NSG_HIP(B.get(&a.out_dist, (size_t)N * R));
+    NSG_HIP(B.get(&a.out_cnt, N));
+    const uint32_t sw = pow2_ge(2 * R, 64);
+    NsgArgs ai = a;
+    ai.C = sw;  // the insert kernel's sort width
+    ai.lds_wave = (uint32_t)((((size_t)NSG_TCAP * 8 + 64 * 8 * 2 + (size_t)R * 16 + 64 * 4 * 2 + (size_t)sw * 4 +
+                               NSG_TCAP) + 15) & ~(size_t)15);
+    NSG_HIP(dispatch_insert(cpl, ai, s));
+    NSG_HIP(hipStreamSynchronize(s));
+    const double insert_ms = ms_since(t0);
+
+    // 4. connectivity (component_conn.cpp:14-98)
+    t0 = std::chrono::high_resolution_clock::now();
+    std::vector<uint32_t> gid((size_t)N * R), gcnt(N);
+    std::vector<float> gdd((size_t)N * R);
+    NSG_HIP(hipMemcpy(gid.data(), a.out_ids, gid.size() * 4, hipMemcpyDeviceToHost));
+    NSG_HIP(hipMemcpy(gdd.data(), a.out_dist, gdd.size() * 4, hipMemcpyDeviceToHost));
+    NSG_HIP(hipMemcpy(gcnt.data(), a.out_cnt, (size_t)N * 4, hipMemcpyDeviceToHost));
+    std::vector<std::vector<std::pair<uint32_t, float>>> extra(N);
+    std::vector<uint8_t> flag(N, 0);
+    std::vector<uint32_t> st;
+    auto mark = [&](uint32_t root) {
+        st.assign(1, root);
+        flag[root] = 1;
+        while (!st.empty()) {
+            const uint32_t u = st.back();
+            st.pop_back();
+            for (uint32_t j = 0; j < gcnt[u]; ++j) {
+                const uint32_t w = gid[(size_t)u * R + j];
+                if (!flag[w]) {
+                    flag[w] = 1;
+                    st.push_back(w);
+                }
+            }
+            for (const auto &e : extra[u])
+                if (!flag[e.first]) {
+                    flag[e.first] = 1;
+                    st.push_back(e.first);
+                }
+        }
+    };
+    mark(ep);
+    std::vector<uint32_t> un;
+    for (uint32_t i = 0; i < N; ++i)
+        if (!flag[i]) un.push_back(i);
+    uint32_t conn_edges = 0;
+    if (!un.empty()) {
+        const uint32_t U = (uint32_t)un.size();
+        std::vector<uint32_t> h2 = head_of(gid.data() + (size_t)ep * R, gcnt[ep], L, N), hs2(h2);
+        std::sort(hs2.begin(), hs2.end());
+        uint32_t *d_h2 = nullptr, *d_hs2 = nullptr, *d_un = nullptr, *d_ent2 = nullptr, *d_ev2 = nullptr;
+        uint64_t *d_full2 = nullptr;
+        float *d_q2 = nullptr;
+        NSG_HIP(B.get(&d_h2, h2.size()));
+        NSG_HIP(B.get(&d_hs2, h2.size()));
+        NSG_HIP(B.get(&d_un, U));
+        NSG_HIP(B.get(&d_ent2, (size_t)U * L));
+        NSG_HIP(B.get(&d_full2, (size_t)U * C));
+        NSG_HIP(B.get(&d_ev2, U));
+        NSG_HIP(B.get(&d_q2, (size_t)U * v.dp));
+        if (!h2.empty()) {
+            NSG_HIP(hipMemcpy(d_h2, h2.data(), h2.size() * 4, hipMemcpyHostToDevice));
+            NSG_HIP(hipMemcpy(d_hs2, hs2.data(), hs2.size() * 4, hipMemcpyHostToDevice));
+        }
+        NSG_HIP(hipMemcpy(d_un, un.data(), (size_t)U * 4, hipMemcpyHostToDevice));
+        hipLaunchKernelGGL(gather_rows_kernel, dim3(1024), dim3(256), 0, s, v.base, v.dp, d_un, U, d_q2);
+        NSG_HIP(hipGetLastError());
+        hipLaunchKernelGGL(nsg_entries_kernel, dim3((U + NWPB - 1) / NWPB), dim3(64 * NWPB), NWPB * ent_lds, s, d_un,
+                           0u, U, d_h2, d_hs2, (uint32_t)h2.size(), L, N, p->seed, ent_lds, d_ent2);
+        NSG_HIP(hipGetLastError());
+        // search the pruned graph as it stands before any link
+        int rc = search_collect(ix, a.out_ids, a.out_cnt, R, d_q2, U, d_ent2, L, C, d_full2, d_ev2);
+        if (rc) return rc;
+        std::vector<uint64_t> F((size_t)U * C);
+        NSG_HIP(hipMemcpy(F.data(), d_full2, F.size() * 8, hipMemcpyDeviceToHost));
+        for (uint32_t i = 0; i < U; ++i) {
+            const uint32_t u = un[i];
+            if (flag[u]) continue;
+            uint32_t root = N;
+            float dd = 0.f;
+            for (uint32_t j = 0; j < C; ++j) {
+                const uint64_t k = F[(size_t)i * C + j];
+                if (k == KEY_EMPTY) break;
+                if (flag[key_id(k)]) {
+                    root = key_id(k);
+                    dd = key_dist(k);
+                    break;
+                }
+            }
+            if (root == N) {
+                const uint64_t sx = stream_of(p->seed, TAG_NSGR, u);
+                for (uint64_t tt = 0; tt < (1ull << 26) && root == N; ++tt) {
+                    const uint32_t x = draw(sx, tt) % N;
+                    if (flag[x]) root = x;
+                }
+                if (root == N) root = ep;
+                // the edge's distance, the build-order way on the device
+                uint64_t *d_k = nullptr;
+                NSG_HIP(B.get(&d_k, 1));
+                NSG_HIP(pair_key(cpl, v.base, v.dp, nch, root, u, d_k, s));
+                uint64_t kk = 0;
+                NSG_HIP(hipMemcpy(&kk, d_k, 8, hipMemcpyDeviceToHost));
+                dd = key_dist(kk);
+            }
+            extra[root].push_back({u, dd});
+            ++conn_edges;
+            mark(u);
+        }
+    }
+    // result graph: pruned + inserted lists, then the links
+    uint32_t W = 1;
+    for (uint32_t i = 0; i < N; ++i) W = std::max<uint32_t>(W, gcnt[i] + (uint32_t)extra[i].size());
+    std::vector<uint32_t> fid((size_t)N * W, ID_NONE), fdeg(N);
+    std::vector<float> fdist((size_t)N * W, -1.0f);
+    for (uint32_t i = 0; i < N; ++i) {
+        uint32_t c = 0;
+        for (uint32_t j = 0; j < gcnt[i]; ++j, ++c) {
+            fid[(size_t)i * W + c] = gid[(size_t)i * R + j];
+            fdist[(size_t)i * W + c] = gdd[(size_t)i * R + j];
+        }
+        for (const auto &e : extra[i]) {
+            fid[(size_t)i * W + c] = e.first;
+            fdist[(size_t)i * W + c] = e.second;
+            ++c;
+        }
+        fdeg[i] = c;
+    }
+    const double conn_ms = ms_since(t0);
+    int rc = index_upload_graph(ix, fid.data(), fdist.data(), fdeg.data(), W);
+    if (rc) return rc;
+    if (stats) {
+        stats->total_ms = ms_since(t_all);
+        stats->search_ms = search_ms;
+        stats->prune_ms = prune_ms;
+        stats->insert_ms = insert_ms;
+        stats->conn_ms = conn_ms;
+        stats->evals = evals;
+        stats->conn_edges = conn_edges;
+        stats->width = W;
+    }
+    return WV_OK;
+}
+
+}  // namespace wv

@@ -1,0 +1,151 @@
+// wv_pool.hpp -- sorted key arrays in LDS shared by the greedy search and
+// the NSG refine: entry keys with an "unexpanded" flag, binary searches, the
+// in-place merge of a sorted block, and GenRandom into LDS.
+#pragma once
+
+#include "wv_device.hpp"
+
+namespace wv {
+
+// Pool and buffer entries are keys with bit 31 (the top bit of the id word;
+// ids are below 2^31) set while the entry is unexpanded.  Comparisons use the
+// key with that bit cleared, so the (dist, id) order is unchanged, and a merge
+// moves the flag with its key.
+constexpr uint64_t FBIT = 1ull << 31;
+__device__ inline uint64_t canon(uint64_t e) { return e & ~FBIT; }
+__device__ inline uint32_t entry_id(uint64_t e) { return (uint32_t)e & 0x7FFFFFFFu; }
+
+// Number of entries of the sorted a[0..n) whose key is < key (<= key).
+__device__ inline int lower_bound_e(const uint64_t *a, int n, uint64_t key) {
+    int lo = 0, hi = n;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (canon(a[mid]) < key) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+__device__ inline int upper_bound_e(const uint64_t *a, int n, uint64_t key) {
+    int lo = 0, hi = n;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (canon(a[mid]) <= key) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// Merge c sorted entries src[0..c) into the sorted array arr[0..len) of
+// capacity cap, in place: an entry of arr moves right by the number of
+// smaller src keys, src[j] lands at j + the number of arr keys <= it (so
+// equal keys never collide), and entries pushed to slots >= cap are dropped.
+// With `mark`, the src entries are stored unexpanded (bit 31 set); without,
+// as they are.  Entries are moved in 64-slot chunks from the tail backwards
+// (all reads of a chunk precede its writes), so slots before the first
+// insertion point are never touched.  1 <= c <= 4 * WAVE.  Returns the final
+// slot of src[0] (>= cap when it was dropped).
+__device__ inline int merge_into(uint64_t *arr, int len, int cap, const uint64_t *src, int c, bool mark) {
+    const int lane = lane_id();
+    int cpos[4];
+    uint64_t ck[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const int j = lane + r * WAVE;
+        cpos[r] = cap;
+        ck[r] = 0;
+        if (j < c) {
+            ck[r] = mark ? (src[j] | FBIT) : src[j];
+            cpos[r] = j + upper_bound_e(arr, len, canon(ck[r]));
+        }
+    }
+    const int pos0 = __shfl(cpos[0], 0);  // lane 0 holds src[0]: j = 0
+    for (int t0 = (len - 1) & ~(WAVE - 1); t0 >= (pos0 & ~(WAVE - 1)); t0 -= WAVE) {
+        const int p = t0 + lane;
+        const bool mv = p >= pos0 && p < len;
+        uint64_t pk = 0;
+        int np = cap;
+        if (mv) {
+            pk = arr[p];
+            np = p + lower_bound_e(src, c, canon(pk));
+        }
+        wave_sync();
+        if (mv && np < cap) arr[np] = pk;
+        wave_sync();
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+        if (cpos[r] < cap) arr[cpos[r]] = ck[r];
+    wave_sync();
+    return pos0;
+}
+
+// GenRandom (include/weavess/util.h:13-27) of size L on counter stream s
+// (SearchEntryRand's entries use the query's stream): L draws mod (N-L), sorted, made strictly increasing
+// (a'[i] = i + max_{j<=i}(a[j]-j), a prefix-max scan), rotated by one more
+// draw mod N.  The draws are sorted as (value, index) keys, 64 at a time,
+// merged into a[0..L) (index i < 2^31, so bit 31 stays clear); on return
+// a[i] holds entry i's id.  tmp: 64 keys of scratch.
+__device__ inline void gen_random_lds(uint64_t s, uint32_t L, uint32_t N, uint64_t *a, uint64_t *tmp) {
+    const int lane = lane_id();
+    const uint32_t mod = N - L;
+    int len = 0;
+    for (uint32_t c0 = 0; c0 < L; c0 += WAVE) {
+        const uint32_t i = c0 + lane;
+        const uint64_t key = wave_sort64(i < L ? ((uint64_t)(draw(s, i) % mod) << 32) | i : KEY_EMPTY);
+        const int cnt = (int)min((uint32_t)WAVE, L - c0);
+        tmp[lane] = key;
+        wave_sync();
+        merge_into(a, len, (int)L, tmp, cnt, false);
+        len += cnt;
+    }
+    const int seg = (int)((L + WAVE - 1) / WAVE);
+    const int s0 = lane * seg, s1 = min(s0 + seg, (int)L);
+    int64_t mx = INT64_MIN;
+    for (int i = s0; i < s1; ++i) {
+        int64_t v = (int64_t)(a[i] >> 32) - i;
+        if (v > mx) mx = v;
+    }
+    int64_t incl = wave_scan_max64(mx);
+    uint32_t lo = (uint32_t)__shfl_up((int)(uint32_t)incl, 1);
+    uint32_t hi = (uint32_t)__shfl_up((int)(uint32_t)((uint64_t)incl >> 32), 1);
+    int64_t run = lane == 0 ? INT64_MIN : (int64_t)(((uint64_t)hi << 32) | lo);
+    const uint64_t off = draw(s, L) % N;
+    for (int i = s0; i < s1; ++i) {
+        int64_t v = (int64_t)(a[i] >> 32) - i;
+        if (v > run) run = v;
+        a[i] = ((uint64_t)(run + i) + off) % N;
+    }
+    wave_sync();
+}
+
+// Merge the distinct keys blk[0..m) (m <= 64, any order, no flag bit) into
+// the sorted distinct array full[0..len) of capacity cap, dropping keys
+// already there and keys that cannot enter a full array.  tmp: 64 keys of
+// scratch.  Returns the new length.
+__device__ inline int merge_distinct(uint64_t *full, int len, int cap, const uint64_t *blk, int m, uint64_t *tmp) {
+    const int lane = lane_id();
+    uint64_t k = lane < m ? blk[lane] : KEY_EMPTY;
+    if (k != KEY_EMPTY) {
+        if (len == cap && k >= full[cap - 1]) {
+            k = KEY_EMPTY;
+        } else {
+            const int p = lower_bound_e(full, len, k);
+            if (p < len && full[p] == k) k = KEY_EMPTY;
+        }
+    }
+    const uint64_t lm = __ballot(k != KEY_EMPTY);
+    if (lm == 0) return len;
+    const int c = __popcll(lm);
+    if (k != KEY_EMPTY) tmp[lanes_below(lm)] = k;
+    wave_sync();
+    int w = 2;
+    while (w < c) w <<= 1;
+    k = wave_sort64_n(lane < c ? tmp[lane] : KEY_EMPTY, w);
+    wave_sync();
+    if (lane < c) tmp[lane] = k;
+    wave_sync();
+    merge_into(full, len, cap, tmp, c, false);
+    return min(len + c, cap);
+}
+
+}  // namespace wv

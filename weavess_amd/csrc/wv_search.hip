@@ -38,6 +38,7 @@
 #include "wv_device.hpp"
 #include "wv_internal.hpp"
 #include "wv_mlp.hpp"
+#include "wv_pool.hpp"
 
 namespace wv {
 
@@ -61,78 +62,6 @@ __device__ inline int visit_first(uint32_t *vis, uint32_t hmask, uint32_t id, in
     }
     atomicExch(&vis[h0], id);
     return 2;
-}
-
-// Pool and buffer entries are keys with bit 31 (the top bit of the id word;
-// ids are below 2^31) set while the entry is unexpanded.  Comparisons use the
-// key with that bit cleared, so the (dist, id) order is unchanged, and a merge
-// moves the flag with its key.
-constexpr uint64_t FBIT = 1ull << 31;
-__device__ inline uint64_t canon(uint64_t e) { return e & ~FBIT; }
-__device__ inline uint32_t entry_id(uint64_t e) { return (uint32_t)e & 0x7FFFFFFFu; }
-
-// Number of entries of the sorted a[0..n) whose key is < key (<= key).
-__device__ inline int lower_bound_e(const uint64_t *a, int n, uint64_t key) {
-    int lo = 0, hi = n;
-    while (lo < hi) {
-        const int mid = (lo + hi) >> 1;
-        if (canon(a[mid]) < key) lo = mid + 1;
-        else hi = mid;
-    }
-    return lo;
-}
-__device__ inline int upper_bound_e(const uint64_t *a, int n, uint64_t key) {
-    int lo = 0, hi = n;
-    while (lo < hi) {
-        const int mid = (lo + hi) >> 1;
-        if (canon(a[mid]) <= key) lo = mid + 1;
-        else hi = mid;
-    }
-    return lo;
-}
-
-// Merge c sorted entries src[0..c) into the sorted array arr[0..len) of
-// capacity cap, in place: an entry of arr moves right by the number of
-// smaller src keys, src[j] lands at j + the number of arr keys <= it (so
-// equal keys never collide), and entries pushed to slots >= cap are dropped.
-// With `mark`, the src entries are stored unexpanded (bit 31 set); without,
-// as they are.  Entries are moved in 64-slot chunks from the tail backwards
-// (all reads of a chunk precede its writes), so slots before the first
-// insertion point are never touched.  1 <= c <= 4 * WAVE.  Returns the final
-// slot of src[0] (>= cap when it was dropped).
-__device__ inline int merge_into(uint64_t *arr, int len, int cap, const uint64_t *src, int c, bool mark) {
-    const int lane = lane_id();
-    int cpos[4];
-    uint64_t ck[4];
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-        const int j = lane + r * WAVE;
-        cpos[r] = cap;
-        ck[r] = 0;
-        if (j < c) {
-            ck[r] = mark ? (src[j] | FBIT) : src[j];
-            cpos[r] = j + upper_bound_e(arr, len, canon(ck[r]));
-        }
-    }
-    const int pos0 = __shfl(cpos[0], 0);  // lane 0 holds src[0]: j = 0
-    for (int t0 = (len - 1) & ~(WAVE - 1); t0 >= (pos0 & ~(WAVE - 1)); t0 -= WAVE) {
-        const int p = t0 + lane;
-        const bool mv = p >= pos0 && p < len;
-        uint64_t pk = 0;
-        int np = cap;
-        if (mv) {
-            pk = arr[p];
-            np = p + lower_bound_e(src, c, canon(pk));
-        }
-        wave_sync();
-        if (mv && np < cap) arr[np] = pk;
-        wave_sync();
-    }
-#pragma unroll
-    for (int r = 0; r < 4; ++r)
-        if (cpos[r] < cap) arr[cpos[r]] = ck[r];
-    wave_sync();
-    return pos0;
 }
 
 // First slot >= from of a[0..n) holding an unexpanded entry, or n.
@@ -163,47 +92,7 @@ __device__ inline uint64_t kth_union(const uint64_t *a, int na, const uint64_t *
     return x > y ? x : y;
 }
 
-// SearchEntryRand's GenRandom (include/weavess/util.h:13-27) on the query's
-// counter stream: L draws mod (N-L), sorted, made strictly increasing
-// (a'[i] = i + max_{j<=i}(a[j]-j), a prefix-max scan), rotated by one more
-// draw mod N.  The draws are sorted as (value, index) keys, 64 at a time,
-// merged into a[0..L) (index i < 2^31, so bit 31 stays clear); on return
-// a[i] holds entry i's id.  tmp: 64 keys of scratch.
-__device__ inline void gen_entries(uint64_t seed, uint32_t q, uint32_t L, uint32_t N, uint64_t *a, uint64_t *tmp) {
-    const int lane = lane_id();
-    const uint64_t s = stream_of(seed, TAG_ENTRY, q);
-    const uint32_t mod = N - L;
-    int len = 0;
-    for (uint32_t c0 = 0; c0 < L; c0 += WAVE) {
-        const uint32_t i = c0 + lane;
-        const uint64_t key = wave_sort64(i < L ? ((uint64_t)(draw(s, i) % mod) << 32) | i : KEY_EMPTY);
-        const int cnt = (int)min((uint32_t)WAVE, L - c0);
-        tmp[lane] = key;
-        wave_sync();
-        merge_into(a, len, (int)L, tmp, cnt, false);
-        len += cnt;
-    }
-    const int seg = (int)((L + WAVE - 1) / WAVE);
-    const int s0 = lane * seg, s1 = min(s0 + seg, (int)L);
-    int64_t mx = INT64_MIN;
-    for (int i = s0; i < s1; ++i) {
-        int64_t v = (int64_t)(a[i] >> 32) - i;
-        if (v > mx) mx = v;
-    }
-    int64_t incl = wave_scan_max64(mx);
-    uint32_t lo = (uint32_t)__shfl_up((int)(uint32_t)incl, 1);
-    uint32_t hi = (uint32_t)__shfl_up((int)(uint32_t)((uint64_t)incl >> 32), 1);
-    int64_t run = lane == 0 ? INT64_MIN : (int64_t)(((uint64_t)hi << 32) | lo);
-    const uint64_t off = draw(s, L) % N;
-    for (int i = s0; i < s1; ++i) {
-        int64_t v = (int64_t)(a[i] >> 32) - i;
-        if (v > run) run = v;
-        a[i] = ((uint64_t)(run + i) + off) % N;
-    }
-    wave_sync();
-}
-
-template <int CPL, int G>
+template <int CPL, int G, bool COLLECT>
 __global__ void __launch_bounds__(256) greedy_search_kernel(SearchArgs a) {
     extern __shared__ __align__(16) uint8_t smem[];
     const int wid = (int)(threadIdx.x >> 6);
@@ -217,6 +106,9 @@ __global__ void __launch_bounds__(256) greedy_search_kernel(SearchArgs a) {
     uint32_t *cid = reinterpret_cast<uint32_t *>(ckey + WAVE);  // 64
     uint32_t *vis = cid + WAVE;                                 // H (a power of two >= 64)
     uint64_t *buf = reinterpret_cast<uint64_t *>(vis + a.H);    // B
+    uint64_t *full = buf + a.B;                                 // C (collect mode)
+    uint64_t *fblk = full + a.C;                                // 64 (collect mode)
+    int flen = 0;
 
     const uint32_t N = a.N, nch = a.nch, dp = a.dp;
     // per-query budgets were checked on the host when the launch was planned;
@@ -247,7 +139,7 @@ __global__ void __launch_bounds__(256) greedy_search_kernel(SearchArgs a) {
         }
         wave_sync();
     } else {
-        gen_entries(a.seed, q, L, N, pool, ckey);
+        gen_random_lds(stream_of(a.seed, TAG_ENTRY, q), L, N, pool, ckey);
     }
     {
         int len = 0;
@@ -257,6 +149,7 @@ __global__ void __launch_bounds__(256) greedy_search_kernel(SearchArgs a) {
             wave_sync();
             rows_dist<CPL, G>(a.base, dp, nch, qv, cid, cnt, ckey);
             wave_sync();
+            if (COLLECT) flen = merge_distinct(full, flen, (int)a.C, ckey, cnt, fblk);
             const uint64_t key = wave_sort64(lane < cnt ? ckey[lane] : KEY_EMPTY);
             wave_sync();
             ckey[lane] = key;
@@ -343,6 +236,7 @@ __global__ void __launch_bounds__(256) greedy_search_kernel(SearchArgs a) {
             // way, so stop probing deeply
             if (evals - L > a.H) probes = VIS_PROBES_FULL;
             wave_sync();
+            if (COLLECT) flen = merge_distinct(full, flen, (int)a.C, ckey, m, fblk);
             uint64_t key = lane < m ? ckey[lane] : KEY_EMPTY;
             if (key < worst) {
                 // already in pool or buffer: a duplicate.  Both binary searches
@@ -401,6 +295,8 @@ __global__ void __launch_bounds__(256) greedy_search_kernel(SearchArgs a) {
         }
     }
     if (nb) merge_into(pool, Li, Li, buf, nb, false);
+    if (COLLECT)
+        for (uint32_t i = lane; i < a.C; i += WAVE) a.full[(size_t)q * a.C + i] = (int)i < flen ? full[i] : KEY_EMPTY;
 
     for (uint32_t i = lane; i < a.k_stride; i += WAVE) {
         const bool ok = i < kq;
@@ -415,42 +311,55 @@ __global__ void __launch_bounds__(256) greedy_search_kernel(SearchArgs a) {
     }
 }
 
-template <int CPL, int G>
+template <int CPL, int G, bool COLLECT>
 hipError_t launch_t(const SearchArgs &a, int waves_per_block, hipStream_t s) {
     const size_t lds = (size_t)a.wave_lds * waves_per_block;
     const uint32_t blocks = (a.Q + waves_per_block - 1) / waves_per_block;
     if (lds > 65536) {
-        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&greedy_search_kernel<CPL, G>),
+        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&greedy_search_kernel<CPL, G, COLLECT>),
                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (e != hipSuccess) return e;
     }
-    hipLaunchKernelGGL((greedy_search_kernel<CPL, G>), dim3(blocks), dim3(WAVE * waves_per_block),
+    hipLaunchKernelGGL((greedy_search_kernel<CPL, G, COLLECT>), dim3(blocks), dim3(WAVE * waves_per_block),
                        lds, s, a);
     return hipGetLastError();
 }
 
 }  // namespace
 
-size_t search_wave_lds(uint32_t P, uint32_t H, uint32_t B) {
+size_t search_wave_lds(uint32_t P, uint32_t H, uint32_t B, uint32_t C) {
     size_t b = (size_t)P * 8 + WAVE * 8 + WAVE * 4 + (size_t)H * 4 + (size_t)B * 8;
+    if (C) b += (size_t)C * 8 + WAVE * 8;
     return (b + 15) & ~(size_t)15;
 }
 
 hipError_t launch_greedy_search(const SearchArgs &a, int chunks_per_lane, int row_groups,
                                 int waves_per_block, hipStream_t s) {
+    if (a.full) {  // candidate collection: one gather shape per row length
+        switch (chunks_per_lane) {
+        case 1: return launch_t<1, 8, true>(a, waves_per_block, s);
+        case 2: return launch_t<2, 4, true>(a, waves_per_block, s);
+        case 4: return launch_t<4, 4, true>(a, waves_per_block, s);
+        case 8: return launch_t<8, 2, true>(a, waves_per_block, s);
+        case 16: return launch_t<16, 1, true>(a, waves_per_block, s);
+        default: return hipErrorInvalidValue;
+        }
+    }
     switch (chunks_per_lane) {
     case 1:
-        return row_groups >= 16 ? launch_t<1, 16>(a, waves_per_block, s) : launch_t<1, 8>(a, waves_per_block, s);
+        return row_groups >= 16 ? launch_t<1, 16, false>(a, waves_per_block, s)
+                                : launch_t<1, 8, false>(a, waves_per_block, s);
     case 2:
-        return row_groups >= 16 ? launch_t<2, 16>(a, waves_per_block, s)
-             : row_groups >= 8 ? launch_t<2, 8>(a, waves_per_block, s)
-                               : launch_t<2, 4>(a, waves_per_block, s);
+        return row_groups >= 16 ? launch_t<2, 16, false>(a, waves_per_block, s)
+             : row_groups >= 8 ? launch_t<2, 8, false>(a, waves_per_block, s)
+                               : launch_t<2, 4, false>(a, waves_per_block, s);
     case 4:
-        return row_groups >= 8 ? launch_t<4, 8>(a, waves_per_block, s) : launch_t<4, 4>(a, waves_per_block, s);
+        return row_groups >= 8 ? launch_t<4, 8, false>(a, waves_per_block, s)
+                               : launch_t<4, 4, false>(a, waves_per_block, s);
     case 8:
-        return launch_t<8, 2>(a, waves_per_block, s);
+        return launch_t<8, 2, false>(a, waves_per_block, s);
     case 16:
-        return launch_t<16, 1>(a, waves_per_block, s);
+        return launch_t<16, 1, false>(a, waves_per_block, s);
     default:
         return hipErrorInvalidValue;
     }
