@@ -144,3 +144,32 @@ def test_pipeline_efanna_build_matches_oracle(ctx, tmp_path, small_int):
     res, *_ = O.search_greedy(ri, rdeg, base, q, 40, 10, seed=6)
     assert float(re.search(r"10 NN accuracy: ([0-9.]+)", out).group(1)) == pytest.approx(
         O.recall(res, gt, 10), abs=1e-6)
+
+
+def test_pipeline_nsg_build_and_search(ctx, tmp_path, small_int):
+    """weavess_main nsg: NN-Descent, the NSG refine from the centroid entry
+    point, the NSG file layout; equal to the oracle's NSG of the oracle's
+    kNN graph, and its centroid-entry search gives the oracle's recall."""
+    from weavess_amd.entries import centroid_entries, centroid_ep
+    base, q = small_int
+    pre = str(tmp_path / "c1")
+    io.write_fvecs(pre + "_base.fvecs", base)
+    io.write_fvecs(pre + "_query.fvecs", q)
+    gt, _ = O.exact_knn(base, q, 10)
+    io.write_ivecs(pre + "_groundtruth.ivecs", gt)
+    env = dict(WEAVESS_K="24", WEAVESS_L="32", WEAVESS_ITER="6", WEAVESS_S="10", WEAVESS_R="32",
+               WEAVESS_SEED="4", WEAVESS_L_REFINE="40", WEAVESS_R_REFINE="16", WEAVESS_C_REFINE="200")
+    out = _run(["nsg", pre, "build"], env)
+    assert "__REFINE : NSG (HIP)__" in out
+    gids, gdeg, ep = io.read_graph(pre + ".nsg.graph", fmt="nsg")
+    assert ep == centroid_ep(ctx, base)
+    kid, _, kdeg = O.nndescent(base, 24, 32, 10, 32, 6, seed=4, cand_slots=128)
+    oi, _, odeg, _ = O.nsg(base, kid, kdeg, 40, 16, 200, ep, seed=4)
+    assert np.array_equal(gdeg, odeg)
+    for i in range(len(base)):
+        assert gids[i, :gdeg[i]].tolist() == oi[i, :odeg[i]].tolist(), i
+    out = _run(["nsg", pre, "search", "20"], env)
+    ent = centroid_entries(ep, oi, odeg, len(q), 20, len(base), seed=4)
+    res, *_ = O.search_greedy(oi, odeg, base, q, 20, 10, entries=ent)
+    assert float(re.search(r"10 NN accuracy: ([0-9.]+)", out).group(1)) == pytest.approx(
+        O.recall(res, gt, 10), abs=1e-6)

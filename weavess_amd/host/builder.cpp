@@ -48,6 +48,9 @@ IndexBuilder *IndexBuilder::refine(TYPE type, bool) {
         a.reset(new ComponentRefineNNDescentHIP(index_, WV_DTYPE_BF16));
     } else if (type == REFINE_NN_DESCENT_HIP_FP8) {
         a.reset(new ComponentRefineNNDescentHIP(index_, WV_DTYPE_FP8));
+    } else if (type == REFINE_NSG) {
+        std::cout << "__REFINE : NSG (HIP)__" << std::endl;  // src/builder.cpp:125-127
+        a.reset(new ComponentRefineNSGHIP(index_));
     } else if (type == REFINE_EFANNA || type == REFINE_EFANNA_HIP_BF16 || type == REFINE_EFANNA_HIP_FP8) {
         std::cout << "__REFINE : EFANNA (HIP)__" << std::endl;  // src/builder.cpp:137-139
         a.reset(new ComponentRefineNNDescentHIP(

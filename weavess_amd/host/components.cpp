@@ -190,6 +190,45 @@ void ComponentRefineNNDescentHIP::RefineInner() {
     index->markLoadGraphChanged();  // the device graph is now the built one
 }
 
+void ComponentRefineNSGHIP::RefineInner() {
+    auto &P = index->getParam();
+    wv_nsg_params p{};
+    p.R = P.get<unsigned>("R_refine");  // SetConfigs, component_refine.cpp:430-436
+    p.L = P.get<unsigned>("L_refine");
+    p.C = P.get<unsigned>("C_refine");
+    p.seed = index->seed();
+    if (!index->ep_set_) {
+        std::cout << "__ENTRY : Centroid__" << std::endl;
+        ComponentRefineEntryCentroidHIP(index).EntryInner();
+    }
+    p.ep = index->ep_;
+    const unsigned N = index->getBaseLen();
+    auto &fg = index->getFinalGraph();
+    if (fg.size() != N) throw std::invalid_argument("NSG refines a kNN graph: run init + refine first");
+    unsigned w = 1;
+    for (const auto &r : fg) w = std::max<unsigned>(w, (unsigned)r.size());
+    std::vector<uint32_t> ids((size_t)N * w, 0xFFFFFFFFu), deg(N);
+    for (unsigned i = 0; i < N; ++i) {
+        deg[i] = (uint32_t)fg[i].size();
+        for (unsigned j = 0; j < deg[i]; ++j) ids[(size_t)i * w + j] = fg[i][j].id;
+    }
+    wv_index *ix = index->deviceIndex();
+    index->check(wv_index_set_graph(ix, ids.data(), deg.data(), w));
+    std::cout << "__CANDIDATE : GREEDY(NSG)__ __PRUNE : NSG__ __CONN : DFS__" << std::endl;
+    index->check(wv_index_build_nsg(ix, &p, &stats_));
+    const unsigned W = stats_.width;
+    std::vector<uint32_t> gid((size_t)N * W), gdeg(N);
+    std::vector<float> gd((size_t)N * W);
+    index->check(wv_index_get_graph(ix, gid.data(), gd.data(), gdeg.data(), nullptr));
+    fg.assign(N, {});
+    for (unsigned i = 0; i < N; ++i) {
+        fg[i].reserve(gdeg[i]);
+        for (unsigned j = 0; j < gdeg[i]; ++j) fg[i].emplace_back(gid[(size_t)i * W + j], gd[(size_t)i * W + j]);
+    }
+    index->getLoadGraph().clear();  // search takes the new FinalGraph
+    index->markLoadGraphChanged();
+}
+
 // ------------------------------------------------------------------ search
 void ComponentSearchEntryRand::SearchEntryInner(unsigned query, std::vector<Index::Neighbor> &pool) {
     const unsigned L = index->getParam().get<unsigned>("L_search");

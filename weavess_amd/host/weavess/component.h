@@ -71,6 +71,20 @@ private:
     wv_build_stats stats_{};
 };
 
+// ComponentRefineNSG (src/component_refine.cpp:393-436) on the GPU: reads
+// "L_refine", "R_refine", "C_refine"; finds ep_ with
+// ComponentRefineEntryCentroidHIP unless it is set; refines FinalGraph (the
+// kNN graph) into the NSG graph, connectivity links included.
+class ComponentRefineNSGHIP : public ComponentRefine {
+public:
+    explicit ComponentRefineNSGHIP(Index *index) : ComponentRefine(index) {}
+    void RefineInner() override;
+    const wv_nsg_stats &stats() const { return stats_; }
+
+private:
+    wv_nsg_stats stats_{};
+};
+
 class ComponentSearchEntry : public Component {
 public:
     explicit ComponentSearchEntry(Index *index) : Component(index) {}

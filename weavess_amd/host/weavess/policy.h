@@ -21,6 +21,7 @@ enum TYPE {
     REFINE_EFANNA,          // EFANNA's NN-Descent on the GPU, FP32 join
     REFINE_EFANNA_HIP_BF16,
     REFINE_EFANNA_HIP_FP8,
+    REFINE_NSG,             // NSG refine of the kNN graph on the GPU
 
     SEARCH_ENTRY_RAND,      // L random entries per query (GenRandom)
     SEARCH_ENTRY_CENTROID,  // the entry point's neighbours, then distinct random ids

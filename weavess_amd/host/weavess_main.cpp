@@ -1,14 +1,15 @@
-// weavess_main.cpp -- KGraph and EFANNA drivers of the pipeline
-// (test/main.cpp:6-34, :190-217 and :480-546 of the reference, with the HIP
-// stages).
+// weavess_main.cpp -- KGraph, EFANNA and NSG drivers of the pipeline
+// (test/main.cpp:6-34, :66-95, :190-217 and :480-546 of the reference,
+// with the HIP stages).
 //
-//   weavess_main kgraph|efanna <prefix> build|search|info [L_search]
+//   weavess_main kgraph|efanna|nsg <prefix> build|search|info [L_search]
 //
 // <prefix>_base.fvecs, <prefix>_query.fvecs and <prefix>_groundtruth.ivecs
-// are read; the graph file is <prefix>.kgraph.graph or <prefix>.efanna.graph
-// (WEAVESS_GRAPH overrides).  K, L, ITER, S, R come from the per-dataset
-// tables below (keyed by the prefix's file name) and can be overridden with
-// WEAVESS_K, _L, _ITER, _S, _R; WEAVESS_SEED, WEAVESS_DTYPE (f32|bf16|fp8)
+// are read; the graph file is <prefix>.kgraph.graph, <prefix>.efanna.graph
+// or <prefix>.nsg.graph (WEAVESS_GRAPH overrides).  K, L, ITER, S, R (and
+// NSG's L_refine, R_refine, C) come from the per-dataset tables below (keyed
+// by the prefix's file name) and can be overridden with WEAVESS_K, _L,
+// _ITER, _S, _R, _L_REFINE, _R_REFINE, _C_REFINE; WEAVESS_SEED, WEAVESS_DTYPE (f32|bf16|fp8)
 // and WEAVESS_EXACT (exact-kNN graph for "info") are read too.
 // WEAVESS_ENTRY=centroid|subcentroid searches from the centroid entry point
 // or the entry set eps_ instead of random entries.  WEAVESS_FORMAT=nsg|ssg
@@ -17,7 +18,9 @@
 // finds the centroid entry point and writes it.  Without L_search, search
 // tunes L to WEAVESS_ACC_SET (default 0.95) recall.  EFANNA builds start
 // from InitRandom's graph (the reference's KD-tree initialisation is not
-// built) and are written in the plain layout.
+// built) and are written in the plain layout.  NSG builds run NN-Descent,
+// then the NSG refine, and are written in the NSG layout; their search
+// starts from the centroid entry point stored in the file.
 #include <cstdlib>
 #include <cstring>
 #include <iostream>
@@ -57,6 +60,21 @@ const std::map<std::string, KG> &table_efanna() {
     return t;
 }
 
+// NSG's K, L, Iter, S, R (its NN-Descent) and L_refine, R_refine, C per
+// dataset (include/weavess/exp_data.h:104-140).
+struct NS {
+    KG kg;
+    unsigned L_refine, R_refine, C;
+};
+const std::map<std::string, NS> &table_nsg() {
+    static const std::map<std::string, NS> t = {
+        {"siftsmall", {{25, 50, 6, 10, 100}, 100, 50, 500}}, {"sift1M", {{100, 120, 12, 25, 300}, 150, 30, 400}},
+        {"gist", {{400, 430, 12, 10, 200}, 500, 20, 400}},   {"glove-100", {{400, 420, 12, 20, 300}, 150, 90, 600}},
+        {"c1", {{50, 70, 10, 10, 100}, 40, 32, 200}},        {"c2", {{64, 64, 12, 16, 64}, 60, 32, 300}},
+    };
+    return t;
+}
+
 unsigned env_u(const char *name, unsigned dflt) {
     const char *v = std::getenv(name);
     return v && *v ? (unsigned)std::strtoul(v, nullptr, 10) : dflt;
@@ -66,15 +84,17 @@ unsigned env_u(const char *name, unsigned dflt) {
 
 int main(int argc, char **argv) {
     const std::string alg = argc > 1 ? argv[1] : "";
-    if (argc < 4 || argc > 5 || (alg != "kgraph" && alg != "efanna")) {
-        std::cout << "./weavess_main kgraph|efanna dataset_prefix build|search|info [L_search]" << std::endl;
+    if (argc < 4 || argc > 5 || (alg != "kgraph" && alg != "efanna" && alg != "nsg")) {
+        std::cout << "./weavess_main kgraph|efanna|nsg dataset_prefix build|search|info [L_search]" << std::endl;
         return 1;
     }
-    const bool efanna = alg == "efanna";
+    const bool efanna = alg == "efanna", nsg = alg == "nsg";
     const std::string prefix(argv[2]), exc(argv[3]);
     std::string name = prefix.substr(prefix.find_last_of('/') + 1);
     const auto &tab = efanna ? table_efanna() : table();
     KG kg = tab.count(name) ? tab.at(name) : tab.at("c1");
+    NS ns = table_nsg().count(name) ? table_nsg().at(name) : table_nsg().at("c1");
+    if (nsg) kg = ns.kg;
     weavess::Parameters P;
     P.set<unsigned>("K", env_u("WEAVESS_K", kg.K));
     P.set<unsigned>("L", env_u("WEAVESS_L", kg.L));
@@ -85,8 +105,11 @@ int main(int argc, char **argv) {
     if (const char *a = std::getenv("WEAVESS_ACC_SET")) P.set<std::string>("acc_set", a);
     else P.set<float>("acc_set", 0.95f);
     if (const char *e = std::getenv("WEAVESS_EXACT")) P.set<std::string>("exact_knng_path", e);
+    P.set<unsigned>("L_refine", env_u("WEAVESS_L_REFINE", ns.L_refine));
+    P.set<unsigned>("R_refine", env_u("WEAVESS_R_REFINE", ns.R_refine));
+    P.set<unsigned>("C_refine", env_u("WEAVESS_C_REFINE", ns.C));
     const char *gf = std::getenv("WEAVESS_GRAPH");
-    const std::string graph = gf ? gf : prefix + (efanna ? ".efanna.graph" : ".kgraph.graph");
+    const std::string graph = gf ? gf : prefix + (efanna ? ".efanna.graph" : nsg ? ".nsg.graph" : ".kgraph.graph");
     const std::string base = prefix + "_base.fvecs", query = prefix + "_query.fvecs",
                       ground = prefix + "_groundtruth.ivecs";
     const std::string dtype = std::getenv("WEAVESS_DTYPE") ? std::getenv("WEAVESS_DTYPE") : "f32";
@@ -97,11 +120,11 @@ int main(int argc, char **argv) {
                : (dtype == "bf16"  ? weavess::REFINE_NN_DESCENT_HIP_BF16
                   : dtype == "fp8" ? weavess::REFINE_NN_DESCENT_HIP_FP8
                                    : weavess::REFINE_NN_DESCENT_HIP);
-    const std::string ent = std::getenv("WEAVESS_ENTRY") ? std::getenv("WEAVESS_ENTRY") : "rand";
+    const std::string ent = std::getenv("WEAVESS_ENTRY") ? std::getenv("WEAVESS_ENTRY") : nsg ? "centroid" : "rand";
     const weavess::TYPE entry = ent == "centroid"      ? weavess::SEARCH_ENTRY_CENTROID
                                 : ent == "subcentroid" ? weavess::SEARCH_ENTRY_SUB_CENTROID
                                                        : weavess::SEARCH_ENTRY_RAND;
-    const std::string fmt = std::getenv("WEAVESS_FORMAT") ? std::getenv("WEAVESS_FORMAT") : "kgraph";
+    const std::string fmt = std::getenv("WEAVESS_FORMAT") ? std::getenv("WEAVESS_FORMAT") : nsg ? "nsg" : "kgraph";
     const weavess::TYPE gtype = fmt == "nsg" ? weavess::INDEX_NSG : fmt == "ssg" ? weavess::INDEX_SSG
                                                                                  : weavess::INDEX_KGRAPH;
     try {
@@ -111,7 +134,12 @@ int main(int argc, char **argv) {
             builder.load(base.c_str(), query.c_str(), ground.c_str(), P)->init(weavess::INIT_RANDOM);
             std::cout << "Init cost: " << builder.GetBuildTime().count() << std::endl;
             builder.refine(refine);
-            if (gtype == weavess::INDEX_NSG) weavess::ComponentRefineEntryCentroidHIP(builder.index()).EntryInner();
+            if (nsg) {  // test/main.cpp:76-81
+                std::cout << "Init cost: " << builder.GetBuildTime().count() << std::endl;
+                builder.refine(weavess::REFINE_NSG);
+            } else if (gtype == weavess::INDEX_NSG) {
+                weavess::ComponentRefineEntryCentroidHIP(builder.index()).EntryInner();
+            }
             builder.save_graph(gtype, graph.c_str());
             std::cout << "Build cost: " << builder.GetBuildTime().count() << std::endl;
         } else if (exc == "search") {
