@@ -223,6 +223,26 @@ typedef struct {
  * hold at most R ids plus the links. */
 int wv_index_build_nsg(wv_index *idx, const wv_nsg_params *p, wv_nsg_stats *stats);
 
+/* One pass of ComponentRefineVAMANA (src/component_refine.cpp:794-958): the
+ * reference runs two, alpha 1 then 2 (:871), from INIT_RAND's random graph. */
+typedef struct {
+    uint32_t L;      /* "L_refine": pool of each node's candidate search        */
+    uint32_t R;      /* "R_refine": list length                                  */
+    uint32_t C;      /* candidates the prune considers, 1..1024                   */
+    uint32_t ep;     /* navigating node                                           */
+    float alpha;     /* occlusion factor, >= 1 (on squared distances)             */
+    uint32_t reserved;
+    uint64_t seed;
+} wv_vamana_params;
+
+/* Vamana pass over the index's graph: NSG's candidate search from ep, then
+ * Vamana's prune (PruneVAMANA, component_prune.cpp:245-297): candidates by
+ * distance, p kept unless a kept r has alpha * d(r, p) < d(node, p), at most
+ * R, then topped up to R with the nearest dropped ones; reverse edges are
+ * inserted with the same prune (InterInsert, :874-958, as one batch).  No
+ * connectivity pass.  The result replaces the index's graph. */
+int wv_index_build_vamana(wv_index *idx, const wv_vamana_params *p, wv_nsg_stats *stats);
+
 /* Greedy search of Q queries (host arrays).  entries: Q*L ids or NULL
  * (seeded GenRandom per query).  L_per_q/k_per_q: NULL or Q values each
  * (then `entries` must be NULL).  Requires k <= L, and L < N for random

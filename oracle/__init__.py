@@ -90,6 +90,10 @@ def lib():
         L.wvo_nsg_conn.argtypes = [_f32p, C.c_uint32, C.c_uint32, _u32p, _u32p, C.c_uint32, C.c_uint32,
                                    C.c_uint32, C.c_uint32, C.c_uint64, _u32p, _f32p, _u32p, C.c_uint32,
                                    C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.c_int]
+        L.wvo_vamana.restype = C.c_int
+        L.wvo_vamana.argtypes = [_f32p, C.c_uint32, C.c_uint32, _u32p, _u32p, C.c_uint32, C.c_uint32,
+                                 C.c_uint32, C.c_uint32, C.c_float, C.c_uint32, C.c_uint64, _u32p, _f32p,
+                                 _u32p, C.c_uint32, C.POINTER(C.c_uint32), C.c_int]
         L.wvo_search_greedy.restype = C.c_int
         L.wvo_search_greedy.argtypes = [_u32p, _u32p, C.c_uint32, _f32p, C.c_uint32, C.c_uint32,
                                         _f32p, C.c_uint32, C.c_void_p, C.c_uint64,
@@ -274,6 +278,28 @@ def nsg(base, knn_ids, knn_deg, L, R, C_, ep, seed=0):
     N, d = base.shape
     return _graph_call(lib().wvo_nsg, N, R + 8, base, N, d, knn_ids, knn_deg, knn_ids.shape[1], L, R, C_, ep,
                        seed)
+
+
+def vamana(base, ids, deg, L, R, C_, alpha, ep, seed=0):
+    """One Vamana pass over graph ids/deg; returns (ids, dist, deg)."""
+    base = _c(base, np.float32)
+    ids = _c(ids, np.uint32)
+    deg = _c(deg, np.uint32)
+    N, d = base.shape
+    wcap = R
+    while True:
+        oi = np.zeros((N, wcap), np.uint32)
+        od = np.zeros((N, wcap), np.float32)
+        odeg = np.zeros(N, np.uint32)
+        w = C.c_uint32(0)
+        rc = lib().wvo_vamana(base, N, d, ids, deg, ids.shape[1], L, R, C_, alpha, ep, seed, oi, od, odeg, wcap,
+                              C.byref(w), 0)
+        if rc == -2:
+            wcap = w.value
+            continue
+        if rc != 0:
+            raise ValueError("vamana: bad arguments")
+        return oi[:, :w.value], od[:, :w.value], odeg
 
 
 def nsg_conn(base, ids, deg, L, C_, ep, seed=0):

@@ -1002,21 +1002,36 @@ void nsg_collect(const Adj &g, const float *base, uint32_t d, uint32_t N, const 
     out.swap(all);
 }
 
-/* The occlusion rule of PruneNSG (component_prune.cpp:40-58) over sorted
- * distinct keys c[0..m) of node `self`: self skipped; each next key is kept
- * unless a kept one is strictly closer to it than the node is; at most R.
- * Written as: every kept key removes the later keys it occludes. */
+/* How a refiner turns candidates into a list: NSG (component_prune.cpp:
+ * 20-66) adds the node's kNN neighbours and occludes with alpha = 1;
+ * Vamana (component_prune.cpp:245-297) uses the candidates alone, occludes
+ * with its alpha and fills up to R with the nearest occluded ones. */
+struct Rule {
+    bool add_knn, fill, conn;
+    float alpha;
+};
+
+/* The occlusion rule over sorted distinct keys c[0..m) of node `self`: self
+ * skipped; each next key is kept unless a kept r has alpha * d(r, key) <
+ * d(node, key); at most R.  Written as: every kept key removes the later
+ * keys it occludes.  With `fill`, a list shorter than R is topped up with
+ * the nearest removed keys, in order. */
 void nsg_occlude(const float *base, uint32_t d, const uint64_t *c, uint32_t m, uint32_t self, uint32_t R,
-                 std::vector<uint64_t> &res) {
+                 std::vector<uint64_t> &res, float alpha = 1.0f, bool fill = false) {
     res.clear();
     std::vector<uint8_t> alive(m, 1);
     for (uint32_t i = 0; i < m && res.size() < R; ++i) {
         if (!alive[i] || key_id(c[i]) == self) continue;
         res.push_back(c[i]);
+        alive[i] = 2;
         const float *ri = base + (size_t)key_id(c[i]) * d;
         for (uint32_t j = i + 1; j < m; ++j)
-            if (alive[j] && dist_canon(ri, base + (size_t)key_id(c[j]) * d, d) < key_dist(c[j])) alive[j] = 0;
+            if (alive[j] == 1 && alpha * dist_canon(ri, base + (size_t)key_id(c[j]) * d, d) < key_dist(c[j]))
+                alive[j] = 0;
     }
+    if (fill)
+        for (uint32_t i = 0; i < m && res.size() < R; ++i)
+            if (alive[i] == 0 && key_id(c[i]) != self) res.push_back(c[i]);
 }
 
 /* Link (component_refine.cpp:438-472) without the connectivity pass:
@@ -1025,7 +1040,8 @@ void nsg_occlude(const float *base, uint32_t d, const uint64_t *c, uint32_t m, u
  * ones already there); a list above R is sorted, cut to its NSG_TCAP
  * nearest and re-pruned. */
 std::vector<std::vector<uint64_t>> nsg_link(const Adj &g, const float *base, uint32_t N, uint32_t d, uint32_t ep,
-                                            uint32_t L, uint32_t R, uint32_t C, uint64_t seed, int nth) {
+                                            uint32_t L, uint32_t R, uint32_t C, uint64_t seed, int nth,
+                                            const Rule &rule) {
     const std::vector<uint32_t> head = nsg_head(g, ep, L, N);
     std::vector<uint32_t> hs(head);
     std::sort(hs.begin(), hs.end());
@@ -1041,12 +1057,13 @@ std::vector<std::vector<uint64_t>> nsg_link(const Adj &g, const float *base, uin
             const float *qv = base + (size_t)n * d;
             nsg_entries(head, hs, L, N, seed, n, ent);
             nsg_collect(g, base, d, N, qv, ent, L, C, stamp, epoch, F);
-            for (uint32_t id : g[n])  /* PruneNSG adds the node's own neighbours (:26-34) */
-                if (id < N) F.push_back(make_key(dist_canon(qv, base + (size_t)id * d, d), id));
+            if (rule.add_knn)  /* PruneNSG adds the node's own neighbours (:26-34) */
+                for (uint32_t id : g[n])
+                    if (id < N) F.push_back(make_key(dist_canon(qv, base + (size_t)id * d, d), id));
             std::sort(F.begin(), F.end());
             F.erase(std::unique(F.begin(), F.end()), F.end());
             if (F.size() > C) F.resize(C);
-            nsg_occlude(base, d, F.data(), (uint32_t)F.size(), n, R, pr[n]);
+            nsg_occlude(base, d, F.data(), (uint32_t)F.size(), n, R, pr[n], rule.alpha, rule.fill);
         }
     }
     std::vector<std::vector<uint64_t>> inc(N);
@@ -1068,7 +1085,7 @@ std::vector<std::vector<uint64_t>> nsg_link(const Adj &g, const float *base, uin
         } else {
             std::sort(temp.begin(), temp.end());
             if (temp.size() > NSG_TCAP) temp.resize(NSG_TCAP);
-            nsg_occlude(base, d, temp.data(), (uint32_t)temp.size(), t, R, out[t]);
+            nsg_occlude(base, d, temp.data(), (uint32_t)temp.size(), t, R, out[t], rule.alpha, rule.fill);
         }
     }
     return out;
@@ -1185,9 +1202,20 @@ int wvo_nsg(const float *base, uint32_t N, uint32_t d, const uint32_t *knn, cons
     if (L == 0 || L >= N || R == 0 || C == 0 || ep >= N || kstride == 0) return -1;
     const int nth = nthreads_or_default(nthreads);
     const Adj g = adj_from(knn, kdeg, kstride, N);
-    auto G = nsg_link(g, base, N, d, ep, L, R, C, seed, nth);
+    auto G = nsg_link(g, base, N, d, ep, L, R, C, seed, nth, Rule{true, false, true, 1.0f});
     const uint32_t e = nsg_conn(G, base, N, d, ep, L, C, seed, nth);
     if (conn_edges) *conn_edges = e;
+    return write_graph(G, out_ids, out_dist, out_deg, wcap, width);
+}
+
+int wvo_vamana(const float *base, uint32_t N, uint32_t d, const uint32_t *ids, const uint32_t *deg, uint32_t stride,
+               uint32_t L, uint32_t R, uint32_t C, float alpha, uint32_t ep, uint64_t seed, uint32_t *out_ids,
+               float *out_dist, uint32_t *out_deg, uint32_t wcap, uint32_t *width, int nthreads) {
+    if (!base || !ids || !deg || !out_ids || !out_deg || N < 2 || d == 0) return -1;
+    if (L == 0 || L >= N || R == 0 || C == 0 || ep >= N || stride == 0 || !(alpha >= 1.0f)) return -1;
+    const Adj g = adj_from(ids, deg, stride, N);
+    auto G = nsg_link(g, base, N, d, ep, L, R, C, seed, nthreads_or_default(nthreads),
+                      Rule{false, true, false, alpha});
     return write_graph(G, out_ids, out_dist, out_deg, wcap, width);
 }
 

@@ -99,6 +99,14 @@ int wvo_nsg(const float *base, uint32_t N, uint32_t d, const uint32_t *knn, cons
             uint32_t kstride, uint32_t L, uint32_t R, uint32_t C, uint32_t ep, uint64_t seed,
             uint32_t *out_ids, float *out_dist, uint32_t *out_deg, uint32_t wcap, uint32_t *width,
             uint32_t *conn_edges, int nthreads);
+/* One pass of ComponentRefineVAMANA (src/component_refine.cpp:794-871) on
+ * graph ids/deg: the NSG candidate search, Vamana's prune (alpha, filled
+ * to R with the nearest occluded candidates, component_prune.cpp:245-297),
+ * batch reverse insertion with the same prune; no connectivity pass. */
+int wvo_vamana(const float *base, uint32_t N, uint32_t d, const uint32_t *ids, const uint32_t *deg,
+               uint32_t stride, uint32_t L, uint32_t R, uint32_t C, float alpha, uint32_t ep, uint64_t seed,
+               uint32_t *out_ids, float *out_dist, uint32_t *out_deg, uint32_t wcap, uint32_t *width,
+               int nthreads);
 /* The connectivity pass alone, on graph ids/deg. */
 int wvo_nsg_conn(const float *base, uint32_t N, uint32_t d, const uint32_t *ids, const uint32_t *deg,
                  uint32_t stride, uint32_t L, uint32_t C, uint32_t ep, uint64_t seed, uint32_t *out_ids,

@@ -41,6 +41,7 @@ def test_struct_layouts_match_header():
     assert ctypes.sizeof(_capi.SearchConfig) == 16
     assert ctypes.sizeof(_capi.NsgParams) == 24
     assert ctypes.sizeof(_capi.NsgStats) == 56
+    assert ctypes.sizeof(_capi.VamanaParams) == 32
 
 
 def test_null_context_is_rejected():

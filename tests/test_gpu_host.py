@@ -173,3 +173,33 @@ def test_pipeline_nsg_build_and_search(ctx, tmp_path, small_int):
     res, *_ = O.search_greedy(oi, odeg, base, q, 20, 10, entries=ent)
     assert float(re.search(r"10 NN accuracy: ([0-9.]+)", out).group(1)) == pytest.approx(
         O.recall(res, gt, 10), abs=1e-6)
+
+
+def test_pipeline_vamana_build_and_search(ctx, tmp_path, small_int):
+    """weavess_main vamana: INIT_RAND, two Vamana passes (alpha 1, 2), the
+    Vamana layout with ep; equal to two oracle passes from the oracle's
+    random graph, and its centroid search gives the oracle's recall."""
+    from weavess_amd.entries import centroid_entries, centroid_ep
+    base, q = small_int
+    pre = str(tmp_path / "c1")
+    io.write_fvecs(pre + "_base.fvecs", base)
+    io.write_fvecs(pre + "_query.fvecs", q)
+    gt, _ = O.exact_knn(base, q, 10)
+    io.write_ivecs(pre + "_groundtruth.ivecs", gt)
+    env = dict(WEAVESS_SEED="5", WEAVESS_L_REFINE="40", WEAVESS_R_REFINE="24", WEAVESS_L="24",
+               WEAVESS_C_REFINE="300")
+    out = _run(["vamana", pre, "build"], env)
+    assert out.count("__REFINE : VAMANA (HIP)__") == 2
+    gids, gdeg, ep = io.read_graph(pre + ".vamana.graph", fmt="nsg")
+    assert ep == centroid_ep(ctx, base)
+    rid, _, rcnt = O.init_random(base, 24, seed=5)
+    g1, _, d1 = O.vamana(base, rid, rcnt, 40, 24, 300, 1.0, ep, seed=5)
+    g2, _, d2 = O.vamana(base, g1, d1, 40, 24, 300, 2.0, ep, seed=5)
+    assert np.array_equal(gdeg, d2)
+    for i in range(len(base)):
+        assert gids[i, :gdeg[i]].tolist() == g2[i, :d2[i]].tolist(), i
+    out = _run(["vamana", pre, "search", "60"], env)
+    ent = centroid_entries(ep, g2, d2, len(q), 60, len(base), seed=5)
+    res, *_ = O.search_greedy(g2, d2, base, q, 60, 10, entries=ent)
+    assert float(re.search(r"10 NN accuracy: ([0-9.]+)", out).group(1)) == pytest.approx(
+        O.recall(res, gt, 10), abs=1e-6)

@@ -31,7 +31,7 @@ EXPORTS = [
     "wv_index_search_dev", "wv_nndescent_build", "wv_search_greedy", "wv_exact_knn",
     "wv_index_search_ext", "wv_index_search_ext_dev", "wv_index_autotune", "wv_search_graph_create",
     "wv_search_graph_launch", "wv_search_graph_destroy", "wv_mlp_grad", "wv_mlp_train",
-    "wv_mlp_predict", "wv_index_build_efanna", "wv_index_build_nsg",
+    "wv_mlp_predict", "wv_index_build_efanna", "wv_index_build_nsg", "wv_index_build_vamana",
 ]
 
 
@@ -60,6 +60,11 @@ class BuildStats(C.Structure):
 class NsgParams(C.Structure):
     _fields_ = [("L", C.c_uint32), ("R", C.c_uint32), ("C", C.c_uint32), ("ep", C.c_uint32),
                 ("seed", C.c_uint64)]
+
+
+class VamanaParams(C.Structure):
+    _fields_ = [("L", C.c_uint32), ("R", C.c_uint32), ("C", C.c_uint32), ("ep", C.c_uint32),
+                ("alpha", C.c_float), ("reserved", C.c_uint32), ("seed", C.c_uint64)]
 
 
 class NsgStats(C.Structure):
@@ -160,6 +165,7 @@ def load() -> C.CDLL:
         "wv_index_build_nndescent": ([vp, vp, vp, vp, vp], i32),
         "wv_index_build_efanna": ([vp, vp, vp, vp, u32, vp], i32),
         "wv_index_build_nsg": ([vp, vp, vp], i32),
+        "wv_index_build_vamana": ([vp, vp, vp], i32),
         "wv_index_search": ([vp, vp, u32, vp, u64, u32, u32, vp, vp, u32, vp, vp, vp, vp, vp], i32),
         "wv_index_search_dev": ([vp, vp, u32, vp, u64, u32, u32, vp, vp, u32, vp, vp, vp, vp, vp], i32),
         "wv_nndescent_build": ([vp, vp, u32, u32, vp, vp, vp, vp, vp, vp], i32),
@@ -385,6 +391,13 @@ class Index:
         p = NsgParams(L, R, C_, ep, seed)
         st = NsgStats()
         self.ctx.check(self.ctx.lib.wv_index_build_nsg(self.h, C.byref(p), C.byref(st)))
+        return st.as_dict()
+
+    def build_vamana(self, L, R, C_, ep, alpha, seed=0):
+        """One Vamana pass over the index's graph from navigating node ep."""
+        p = VamanaParams(L, R, C_, ep, alpha, 0, seed)
+        st = NsgStats()
+        self.ctx.check(self.ctx.lib.wv_index_build_vamana(self.h, C.byref(p), C.byref(st)))
         return st.as_dict()
 
     def search(self, queries, L, k, seed=0, entries=None, L_per_q=None, k_per_q=None,

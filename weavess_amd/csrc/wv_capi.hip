@@ -1050,7 +1050,25 @@ int wv_index_build_nsg(wv_index *ix, const wv_nsg_params *p, wv_nsg_stats *stats
     if (ix->dp > 1024) return fail(c, WV_ENOTSUP, "dimension above 1024");
     WV_HIP(c, hipSetDevice(c->device));
     // the kNN graph stays alive until the result replaces it
-    return wv::nsg_build(ix, p, stats, ix->graph, ix->deg, ix->deg_max);
+    return wv::nsg_build(ix, p, stats, ix->graph, ix->deg, ix->deg_max, wv::RefineRule{true, false, true, 1.0f});
+}
+
+int wv_index_build_vamana(wv_index *ix, const wv_vamana_params *p, wv_nsg_stats *stats) {
+    if (!ix) return WV_EINVAL;
+    wv_ctx *c = ix->ctx;
+    std::lock_guard<std::mutex> g(c->mu);
+    if (!p) return fail(c, WV_EINVAL, "params are null");
+    if (!ix->graph) return fail(c, WV_EINVAL, "index has no graph (build or set the initial graph first)");
+    if (p->L == 0 || p->L >= ix->N) return fail(c, WV_EINVAL, "need 1 <= L < N (L=%u N=%u)", p->L, ix->N);
+    if (p->L > 1024 || p->C == 0 || p->C > 1024)
+        return fail(c, WV_ENOTSUP, "need L <= 1024 and 1 <= C <= 1024 (L=%u C=%u)", p->L, p->C);
+    if (p->R == 0 || p->R > 256) return fail(c, WV_ENOTSUP, "need 1 <= R <= 256 (R=%u)", p->R);
+    if (p->ep >= ix->N) return fail(c, WV_EINVAL, "ep %u out of range", p->ep);
+    if (!(p->alpha >= 1.0f) || !(p->alpha < 1e30f)) return fail(c, WV_EINVAL, "alpha must be finite and >= 1");
+    if (ix->dp > 1024) return fail(c, WV_ENOTSUP, "dimension above 1024");
+    WV_HIP(c, hipSetDevice(c->device));
+    wv_nsg_params q{p->L, p->R, p->C, p->ep, p->seed};
+    return wv::nsg_build(ix, &q, stats, ix->graph, ix->deg, ix->deg_max, wv::RefineRule{false, true, false, p->alpha});
 }
 
 int wv_nndescent_build(wv_ctx *c, const float *base, uint32_t N, uint32_t d,

@@ -38,9 +38,17 @@ int search_collect(wv_index *ix, const uint32_t *graph, const uint32_t *deg, uin
 // Replaces the index's graph with host arrays (N x W ids / dists, deg[N]).
 int index_upload_graph(wv_index *ix, const uint32_t *ids, const float *dist, const uint32_t *deg, uint32_t W);
 
-// Implemented in wv_nsg.hip: the NSG refine of the kNN graph (knn, kdeg,
-// kstride on the device); the result replaces the index's graph.
+// How a refiner turns candidates into lists: NSG adds the node's kNN
+// neighbours, occludes with alpha 1 and links unreachable nodes; Vamana
+// occludes with its alpha and fills lists up to R.
+struct RefineRule {
+    bool add_knn, fill, conn;
+    float alpha;
+};
+// Implemented in wv_nsg.hip: the NSG (or one Vamana pass) refine of the
+// graph knn, kdeg, kstride on the device; the result replaces the index's
+// graph.
 int nsg_build(wv_index *ix, const wv_nsg_params *p, wv_nsg_stats *stats, const uint32_t *knn, const uint32_t *kdeg,
-              uint32_t kstride);
+              uint32_t kstride, const RefineRule &rule);
 
 }  // namespace wv

@@ -64,6 +64,9 @@ struct NsgArgs {
     float *out_dist;
     uint32_t *out_cnt;
     uint32_t lds_wave;
+    float alpha;           // occlusion factor (NSG: 1)
+    uint32_t add_knn;      // candidates include the node's kNN neighbours (NSG)
+    uint32_t fill;         // top lists up with the nearest occluded candidates (Vamana)
 };
 
 template <int CPL>
@@ -80,21 +83,26 @@ __device__ inline void load_row(const float *base, uint32_t dp, uint32_t nch, ui
     }
 }
 
-// Occlusion (component_prune.cpp:40-58) over the sorted distinct keys
-// cand[0..m) of one node (key distance = distance to the node), entries with
-// alive[i] == 0 skipped: the first alive key is kept, and every kept key
-// clears the later alive keys it is strictly closer to than the node is; at
-// most R kept, written to res.  Returns the number kept.
+// Candidate states of the occlusion pass.
+constexpr uint8_t ST_DROP = 0, ST_LIVE = 1, ST_KEPT = 2, ST_SELF = 3;
+
+// Occlusion (component_prune.cpp:40-58, and Vamana's :245-297) over the
+// sorted distinct keys cand[0..m) of one node (key distance = distance to
+// the node), with st[i] the candidates' states (ST_LIVE or ST_SELF): the
+// first live key is kept, and every kept key drops the later live keys p
+// with alpha * d(kept, p) < d(node, p); at most R kept, written to res.
+// With `fill`, a list shorter than R is topped up with the nearest dropped
+// keys, in order.  Returns the list length.
 template <int CPL>
-__device__ int occlude(const float *base, uint32_t dp, uint32_t nch, const uint64_t *cand, int m, uint8_t *alive,
-                       int R, uint32_t *cid, uint32_t *cpos, uint64_t *ckey, uint64_t *res) {
+__device__ int occlude(const float *base, uint32_t dp, uint32_t nch, const uint64_t *cand, int m, uint8_t *st,
+                       int R, float alpha, bool fill, uint32_t *cid, uint32_t *cpos, uint64_t *ckey, uint64_t *res) {
     const int lane = lane_id();
     int cnt = 0, from = 0;
     while (cnt < R) {
         int nxt = m;
         for (int t0 = from & ~(WAVE - 1); t0 < m; t0 += WAVE) {
             const int p = t0 + lane;
-            const uint64_t b = __ballot(p >= from && p < m && alive[p]);
+            const uint64_t b = __ballot(p >= from && p < m && st[p] == ST_LIVE);
             if (b) {
                 nxt = t0 + __ffsll((unsigned long long)b) - 1;
                 break;
@@ -102,7 +110,10 @@ __device__ int occlude(const float *base, uint32_t dp, uint32_t nch, const uint6
         }
         if (nxt >= m) break;
         const uint64_t ki = cand[nxt];
-        if (lane == 0) res[cnt] = ki;
+        if (lane == 0) {
+            res[cnt] = ki;
+            st[nxt] = ST_KEPT;
+        }
         ++cnt;
         from = nxt + 1;
         if (cnt >= R) break;
@@ -110,7 +121,7 @@ __device__ int occlude(const float *base, uint32_t dp, uint32_t nch, const uint6
         load_row<CPL>(base, dp, nch, key_id(ki), qv);
         for (int t0 = from & ~(WAVE - 1); t0 < m; t0 += WAVE) {
             const int p = t0 + lane;
-            const bool ok = p >= from && p < m && alive[p];
+            const bool ok = p >= from && p < m && st[p] == ST_LIVE;
             const uint64_t b = __ballot(ok);
             if (!b) continue;
             const int mm = __popcll(b);
@@ -123,11 +134,21 @@ __device__ int occlude(const float *base, uint32_t dp, uint32_t nch, const uint6
             wave_sync();
             if (lane < mm) {
                 const uint32_t pp = cpos[lane];
-                if (key_dist(ckey[lane]) < key_dist(cand[pp])) alive[pp] = 0;
+                if (alpha * key_dist(ckey[lane]) < key_dist(cand[pp])) st[pp] = ST_DROP;
             }
             wave_sync();
         }
     }
+    wave_sync();
+    if (fill)
+        for (int t0 = 0; t0 < m && cnt < R; t0 += WAVE) {
+            const int p = t0 + lane;
+            const bool ok = p < m && st[p] == ST_DROP;
+            const uint64_t b = __ballot(ok);
+            const int pos = cnt + (int)lanes_below(b);
+            if (ok && pos < R) res[pos] = cand[p];
+            cnt = min(R, cnt + __popcll(b));
+        }
     wave_sync();
     return cnt;
 }
@@ -172,8 +193,8 @@ __global__ void __launch_bounds__(256) nsg_entries_kernel(const uint32_t *nodes,
 }
 
 // PruneNSG (component_prune.cpp:20-66) for nodes n0 .. n0+nb-1: the
-// collected keys plus the node's kNN neighbours (distinct, sorted, first C),
-// self skipped, occlusion to R.
+// collected keys plus (NSG) the node's kNN neighbours, distinct, sorted,
+// first C; self skipped; occlusion to R (Vamana: with its alpha and fill).
 template <int CPL>
 __global__ void __launch_bounds__(256) nsg_prune_kernel(NsgArgs a) {
     extern __shared__ __align__(16) uint8_t smem[];
@@ -200,7 +221,7 @@ __global__ void __launch_bounds__(256) nsg_prune_kernel(NsgArgs a) {
     wave_sync();
     float4 qv[CPL];
     load_row<CPL>(a.base, a.dp, a.nch, n, qv);
-    const uint32_t dg = min(a.kdeg[n], a.kstride);
+    const uint32_t dg = a.add_knn ? min(a.kdeg[n], a.kstride) : 0u;
     for (uint32_t c0 = 0; c0 < dg; c0 += WAVE) {
         const uint32_t id = c0 + lane < dg ? a.knn[(size_t)n * a.kstride + c0 + lane] : ID_NONE;
         const bool ok = id < a.N;
@@ -213,9 +234,10 @@ __global__ void __launch_bounds__(256) nsg_prune_kernel(NsgArgs a) {
         wave_sync();
         len = merge_distinct(cand, len, C, ckey, m, tmp);
     }
-    for (int i = lane; i < len; i += WAVE) alive[i] = key_id(cand[i]) != n;
+    for (int i = lane; i < len; i += WAVE) alive[i] = key_id(cand[i]) != n ? ST_LIVE : ST_SELF;
     wave_sync();
-    const int cnt = occlude<CPL>(a.base, a.dp, a.nch, cand, len, alive, (int)a.R, cid, cpos, ckey, res);
+    const int cnt = occlude<CPL>(a.base, a.dp, a.nch, cand, len, alive, (int)a.R, a.alpha, a.fill != 0, cid, cpos,
+                                 ckey, res);
     for (int i = lane; i < cnt; i += WAVE) {
         a.pr_ids[(size_t)n * a.R + i] = key_id(res[i]);
         a.pr_dist[(size_t)n * a.R + i] = key_dist(res[i]);
@@ -256,16 +278,21 @@ __global__ void __launch_bounds__(256) nsg_insert_kernel(NsgArgs a) {
     uint64_t *ckey = cand + NSG_TCAP;                    // 64
     uint64_t *tmp = ckey + WAVE;                         // 64
     uint64_t *res = tmp + WAVE;                          // R
-    uint64_t *own = res + a.R;                           // R
-    uint32_t *cid = reinterpret_cast<uint32_t *>(own + a.R);  // 64
+    const uint32_t ow = a.C >> 1;                        // power of two >= R
+    uint64_t *own = res + a.R;                           // ow
+    uint32_t *cid = reinterpret_cast<uint32_t *>(own + ow);   // 64
     uint32_t *cpos = cid + WAVE;                              // 64
     uint32_t *srt = cpos + WAVE;                              // pow2 >= 2R
     uint8_t *alive = reinterpret_cast<uint8_t *>(srt + a.C);  // NSG_TCAP (a.C holds the sort width here)
     const int R = (int)a.R;
     const int no = (int)a.pr_cnt[t];
-    for (int i = lane; i < no; i += WAVE)
-        own[i] = make_key(a.pr_dist[(size_t)t * a.R + i], a.pr_ids[(size_t)t * a.R + i]);
+    // the node's list as sorted keys (a Vamana list, topped up with dropped
+    // candidates, is not in key order): duplicate tests and the re-prune's
+    // merge need the order
+    for (uint32_t i = lane; i < ow; i += WAVE)
+        own[i] = (int)i < no ? make_key(a.pr_dist[(size_t)t * a.R + i], a.pr_ids[(size_t)t * a.R + i]) : KEY_EMPTY;
     wave_sync();
+    lds_sort64(own, (int)ow);
     const uint32_t s0 = a.inoff[t], s1 = a.inoff[t + 1];
     // incoming edges not already in the list
     int nnew = 0;
@@ -293,9 +320,9 @@ __global__ void __launch_bounds__(256) nsg_insert_kernel(NsgArgs a) {
         }
         wave_sync();
         lds_sort32(srt, (int)w);
-        for (int i = lane; i < no; i += WAVE) {
-            a.out_ids[(size_t)t * a.R + i] = key_id(own[i]);
-            a.out_dist[(size_t)t * a.R + i] = key_dist(own[i]);
+        for (int i = lane; i < no; i += WAVE) {  // the list in its own order
+            a.out_ids[(size_t)t * a.R + i] = a.pr_ids[(size_t)t * a.R + i];
+            a.out_dist[(size_t)t * a.R + i] = a.pr_dist[(size_t)t * a.R + i];
         }
         for (int i = lane; i < nnew; i += WAVE) {
             const uint32_t s = srt[i];
@@ -323,9 +350,10 @@ __global__ void __launch_bounds__(256) nsg_insert_kernel(NsgArgs a) {
         wave_sync();
         len = merge_distinct(cand, len, (int)NSG_TCAP, ckey, (int)min((uint32_t)WAVE, s1 - c0), tmp);
     }
-    for (int i = lane; i < len; i += WAVE) alive[i] = 1;
+    for (int i = lane; i < len; i += WAVE) alive[i] = ST_LIVE;
     wave_sync();
-    const int cnt = occlude<CPL>(a.base, a.dp, a.nch, cand, len, alive, R, cid, cpos, ckey, res);
+    const int cnt = occlude<CPL>(a.base, a.dp, a.nch, cand, len, alive, R, a.alpha, a.fill != 0, cid, cpos, ckey,
+                                 res);
     for (int i = lane; i < cnt; i += WAVE) {
         a.out_ids[(size_t)t * a.R + i] = key_id(res[i]);
         a.out_dist[(size_t)t * a.R + i] = key_dist(res[i]);
@@ -446,8 +474,8 @@ double ms_since(std::chrono::high_resolution_clock::time_point t0) {
 }  // namespace
 
 int nsg_build(wv_index *ix, const wv_nsg_params *p, wv_nsg_stats *stats, const uint32_t *knn, const uint32_t *kdeg,
-              uint32_t kstride) {
-    // knn / kdeg: the index's kNN graph on the device (kstride ids a row)
+              uint32_t kstride, const RefineRule &rule) {
+    // knn / kdeg: the graph being refined, on the device (kstride ids a row)
     const IndexView v = index_view(ix);
     const hipStream_t s = v.stream;
     const uint32_t N = v.N, L = p->L, R = p->R, C = p->C, ep = p->ep;
@@ -465,6 +493,9 @@ int nsg_build(wv_index *ix, const wv_nsg_params *p, wv_nsg_stats *stats, const u
     a.kstride = kstride;
     a.R = R;
     a.C = C;
+    a.alpha = rule.alpha;
+    a.add_knn = rule.add_knn ? 1u : 0u;
+    a.fill = rule.fill ? 1u : 0u;
     NSG_HIP(B.get(&a.pr_ids, (size_t)N * R));
     NSG_HIP(B.get(&a.pr_dist, (size_t)N * R));
     NSG_HIP(B.get(&a.pr_cnt, N));
@@ -546,11 +577,11 @@ int nsg_build(wv_index *ix, const wv_nsg_params *p, wv_nsg_stats *stats, const u
     NSG_HIP(B.get(&a.out_ids, (size_t)N * R));
     NSG_HIP(B.get(&a.out_dist, (size_t)N * R));
     NSG_HIP(B.get(&a.out_cnt, N));
-    const uint32_t sw = pow2_ge(2 * R, 64);
+    const uint32_t sw = pow2_ge(2 * R, 64);  // sort width of new sources; own keys sort in sw / 2 >= R
     NsgArgs ai = a;
-    ai.C = sw;  // the insert kernel's sort width
-    ai.lds_wave = (uint32_t)((((size_t)NSG_TCAP * 8 + 64 * 8 * 2 + (size_t)R * 16 + 64 * 4 * 2 + (size_t)sw * 4 +
-                               NSG_TCAP) + 15) & ~(size_t)15);
+    ai.C = sw;
+    ai.lds_wave = (uint32_t)((((size_t)NSG_TCAP * 8 + 64 * 8 * 2 + (size_t)R * 8 + (size_t)(sw / 2) * 8 +
+                               64 * 4 * 2 + (size_t)sw * 4 + NSG_TCAP) + 15) & ~(size_t)15);
     NSG_HIP(dispatch_insert(cpl, ai, s));
     NSG_HIP(hipStreamSynchronize(s));
     const double insert_ms = ms_since(t0);
@@ -585,10 +616,12 @@ int nsg_build(wv_index *ix, const wv_nsg_params *p, wv_nsg_stats *stats, const u
                 }
         }
     };
-    mark(ep);
     std::vector<uint32_t> un;
-    for (uint32_t i = 0; i < N; ++i)
-        if (!flag[i]) un.push_back(i);
+    if (rule.conn) {
+        mark(ep);
+        for (uint32_t i = 0; i < N; ++i)
+            if (!flag[i]) un.push_back(i);
+    }
     uint32_t conn_edges = 0;
     if (!un.empty()) {
         const uint32_t U = (uint32_t)un.size();

@@ -31,6 +31,9 @@ IndexBuilder *IndexBuilder::init(TYPE type, bool) {
     if (type == INIT_RANDOM || type == INIT_RANDOM_HIP) {
         std::cout << "__INIT : RANDOM (HIP)__" << std::endl;
         a.reset(new ComponentInitRandomHIP(index_));
+    } else if (type == INIT_RAND) {
+        std::cout << "__INIT : RAND (HIP)__" << std::endl;  // src/builder.cpp:51-53
+        a.reset(new ComponentInitRandomHIP(index_, true));
     } else {
         throw std::invalid_argument("__INIT : WRONG TYPE__");  // src/builder.cpp:91-93
     }
@@ -48,6 +51,9 @@ IndexBuilder *IndexBuilder::refine(TYPE type, bool) {
         a.reset(new ComponentRefineNNDescentHIP(index_, WV_DTYPE_BF16));
     } else if (type == REFINE_NN_DESCENT_HIP_FP8) {
         a.reset(new ComponentRefineNNDescentHIP(index_, WV_DTYPE_FP8));
+    } else if (type == REFINE_VAMANA) {
+        std::cout << "__REFINE : VAMANA (HIP)__" << std::endl;  // src/builder.cpp:134-136
+        a.reset(new ComponentRefineVamanaHIP(index_));
     } else if (type == REFINE_NSG) {
         std::cout << "__REFINE : NSG (HIP)__" << std::endl;  // src/builder.cpp:125-127
         a.reset(new ComponentRefineNSGHIP(index_));

@@ -108,8 +108,9 @@ void ComponentLoad::LoadInner(const char *data_file, const char *query_file, con
 
 // ------------------------------------------------------------------ init / refine
 void ComponentInitRandomHIP::InitInner() {
-    const unsigned S = index->getParam().get<unsigned>("S");  // component_init.cpp:47-49
-    index->setInitEdgesNum(S);
+    // component_init.cpp:47-49 ("S"), or ComponentInitRand's "L" (:1027-1029)
+    const unsigned S = index->getParam().get<unsigned>(rand_init_ ? "L" : "S");
+    if (!rand_init_) index->setInitEdgesNum(S);
     wv_nnd_params p{};
     p.K = S;
     p.L = S;
@@ -190,6 +191,36 @@ void ComponentRefineNNDescentHIP::RefineInner() {
     index->markLoadGraphChanged();  // the device graph is now the built one
 }
 
+// Upload FinalGraph as the device graph.
+static void upload_final_graph(Index *index) {
+    const unsigned N = index->getBaseLen();
+    auto &fg = index->getFinalGraph();
+    unsigned w = 1;
+    for (const auto &r : fg) w = std::max<unsigned>(w, (unsigned)r.size());
+    std::vector<uint32_t> ids((size_t)N * w, 0xFFFFFFFFu), deg(N);
+    for (unsigned i = 0; i < N; ++i) {
+        deg[i] = (uint32_t)fg[i].size();
+        for (unsigned j = 0; j < deg[i]; ++j) ids[(size_t)i * w + j] = fg[i][j].id;
+    }
+    index->check(wv_index_set_graph(index->deviceIndex(), ids.data(), deg.data(), w));
+}
+
+// Device graph (W ids a row) back into FinalGraph; search takes it next.
+static void download_final_graph(Index *index, unsigned W) {
+    const unsigned N = index->getBaseLen();
+    std::vector<uint32_t> gid((size_t)N * W), gdeg(N);
+    std::vector<float> gd((size_t)N * W);
+    index->check(wv_index_get_graph(index->deviceIndex(), gid.data(), gd.data(), gdeg.data(), nullptr));
+    auto &fg = index->getFinalGraph();
+    fg.assign(N, {});
+    for (unsigned i = 0; i < N; ++i) {
+        fg[i].reserve(gdeg[i]);
+        for (unsigned j = 0; j < gdeg[i]; ++j) fg[i].emplace_back(gid[(size_t)i * W + j], gd[(size_t)i * W + j]);
+    }
+    index->getLoadGraph().clear();
+    index->markLoadGraphChanged();
+}
+
 void ComponentRefineNSGHIP::RefineInner() {
     auto &P = index->getParam();
     wv_nsg_params p{};
@@ -202,31 +233,34 @@ void ComponentRefineNSGHIP::RefineInner() {
         ComponentRefineEntryCentroidHIP(index).EntryInner();
     }
     p.ep = index->ep_;
-    const unsigned N = index->getBaseLen();
-    auto &fg = index->getFinalGraph();
-    if (fg.size() != N) throw std::invalid_argument("NSG refines a kNN graph: run init + refine first");
-    unsigned w = 1;
-    for (const auto &r : fg) w = std::max<unsigned>(w, (unsigned)r.size());
-    std::vector<uint32_t> ids((size_t)N * w, 0xFFFFFFFFu), deg(N);
-    for (unsigned i = 0; i < N; ++i) {
-        deg[i] = (uint32_t)fg[i].size();
-        for (unsigned j = 0; j < deg[i]; ++j) ids[(size_t)i * w + j] = fg[i][j].id;
-    }
-    wv_index *ix = index->deviceIndex();
-    index->check(wv_index_set_graph(ix, ids.data(), deg.data(), w));
+    if (index->getFinalGraph().size() != index->getBaseLen())
+        throw std::invalid_argument("NSG refines a kNN graph: run init + refine first");
+    upload_final_graph(index);
     std::cout << "__CANDIDATE : GREEDY(NSG)__ __PRUNE : NSG__ __CONN : DFS__" << std::endl;
-    index->check(wv_index_build_nsg(ix, &p, &stats_));
-    const unsigned W = stats_.width;
-    std::vector<uint32_t> gid((size_t)N * W), gdeg(N);
-    std::vector<float> gd((size_t)N * W);
-    index->check(wv_index_get_graph(ix, gid.data(), gd.data(), gdeg.data(), nullptr));
-    fg.assign(N, {});
-    for (unsigned i = 0; i < N; ++i) {
-        fg[i].reserve(gdeg[i]);
-        for (unsigned j = 0; j < gdeg[i]; ++j) fg[i].emplace_back(gid[(size_t)i * W + j], gd[(size_t)i * W + j]);
+    index->check(wv_index_build_nsg(index->deviceIndex(), &p, &stats_));
+    download_final_graph(index, stats_.width);
+}
+
+void ComponentRefineVamanaHIP::RefineInner() {
+    auto &P = index->getParam();
+    wv_vamana_params p{};
+    p.R = P.get<unsigned>("R_refine");  // SetConfigs, component_refine.cpp:831-833
+    p.L = P.get<unsigned>("L_refine");
+    p.C = P.has("C_refine") ? P.get<unsigned>("C_refine") : 1024;
+    p.alpha = index->alpha;
+    p.seed = index->seed();
+    if (!index->ep_set_) {
+        std::cout << "__ENTRY : Centroid__" << std::endl;
+        ComponentRefineEntryCentroidHIP(index).EntryInner();
     }
-    index->getLoadGraph().clear();  // search takes the new FinalGraph
-    index->markLoadGraphChanged();
+    p.ep = index->ep_;
+    if (index->getFinalGraph().size() != index->getBaseLen())
+        throw std::invalid_argument("Vamana refines a graph: run an init stage first");
+    upload_final_graph(index);
+    std::cout << "alpha " << p.alpha << "\n__CANDIDATE : NSG__ __PRUNE : VAMANA__" << std::endl;
+    index->check(wv_index_build_vamana(index->deviceIndex(), &p, &stats_));
+    download_final_graph(index, stats_.width);
+    index->alpha = 2;  // the next pass (component_refine.cpp:871)
 }
 
 // ------------------------------------------------------------------ search

@@ -38,10 +38,17 @@ public:
 
 // Random S-NN initialisation on the GPU (src/component_init.cpp:11-45):
 // FinalGraph[i] = S random distinct ids != i, ascending (dist, id).
+// InitRandom (src/component_init.cpp:11-68) on the GPU: "S" random
+// neighbours a node (GenRandom, self skipped), sorted, into FinalGraph.
+// With rand_init, ComponentInitRand (:979-1020): the count is "L".
 class ComponentInitRandomHIP : public ComponentInit {
 public:
-    explicit ComponentInitRandomHIP(Index *index) : ComponentInit(index) {}
+    explicit ComponentInitRandomHIP(Index *index, bool rand_init = false)
+        : ComponentInit(index), rand_init_(rand_init) {}
     void InitInner() override;
+
+private:
+    bool rand_init_;
 };
 
 class ComponentRefine : public Component {
@@ -78,6 +85,20 @@ private:
 class ComponentRefineNSGHIP : public ComponentRefine {
 public:
     explicit ComponentRefineNSGHIP(Index *index) : ComponentRefine(index) {}
+    void RefineInner() override;
+    const wv_nsg_stats &stats() const { return stats_; }
+
+private:
+    wv_nsg_stats stats_{};
+};
+
+// ComponentRefineVAMANA (src/component_refine.cpp:794-958) on the GPU, one
+// pass: reads "L_refine", "R_refine" (and "C_refine" when set, else 1024
+// candidates); ep_ from ComponentRefineEntryCentroidHIP unless set; the
+// Index's alpha (1 on the first pass; the pass sets it to 2, :871).
+class ComponentRefineVamanaHIP : public ComponentRefine {
+public:
+    explicit ComponentRefineVamanaHIP(Index *index) : ComponentRefine(index) {}
     void RefineInner() override;
     const wv_nsg_stats &stats() const { return stats_; }
 

@@ -86,6 +86,8 @@ public:
     bool ep_set_ = false;
     // entry set eps_ of include/weavess/index.h, read from an SSG graph file
     std::vector<unsigned> eps_;
+    // Vamana's occlusion factor (include/weavess/index.h:215): 1, then 2
+    float alpha = 1;
 
     // counters (src/component_search_entry.cpp:25, src/component_route.cpp:24,34)
     void addDistCount(uint64_t n = 1) { dist_count_ += n; }

@@ -13,6 +13,7 @@ enum TYPE {
 
     INIT_RANDOM,            // ComponentInitRandom (host, reference semantics)
     INIT_RANDOM_HIP,        // random S-NN initialisation on the GPU
+    INIT_RAND,              // ComponentInitRand: "L" random neighbours a node, on the GPU
 
     REFINE_NN_DESCENT,      // accepted: runs the HIP NN-Descent (the only refine here)
     REFINE_NN_DESCENT_HIP,  // NN-Descent on the GPU, FP32 join
@@ -22,6 +23,7 @@ enum TYPE {
     REFINE_EFANNA_HIP_BF16,
     REFINE_EFANNA_HIP_FP8,
     REFINE_NSG,             // NSG refine of the kNN graph on the GPU
+    REFINE_VAMANA,          // one Vamana pass on the GPU (alpha 1, then 2 on the next call)
 
     SEARCH_ENTRY_RAND,      // L random entries per query (GenRandom)
     SEARCH_ENTRY_CENTROID,  // the entry point's neighbours, then distinct random ids

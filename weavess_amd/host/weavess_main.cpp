@@ -1,8 +1,8 @@
-// weavess_main.cpp -- KGraph, EFANNA and NSG drivers of the pipeline
-// (test/main.cpp:6-34, :66-95, :190-217 and :480-546 of the reference,
-// with the HIP stages).
+// weavess_main.cpp -- KGraph, EFANNA, NSG and Vamana drivers of the
+// pipeline (test/main.cpp:6-34, :66-95, :159-188, :190-217 and :480-546 of
+// the reference, with the HIP stages).
 //
-//   weavess_main kgraph|efanna|nsg <prefix> build|search|info [L_search]
+//   weavess_main kgraph|efanna|nsg|vamana <prefix> build|search|info [L_search]
 //
 // <prefix>_base.fvecs, <prefix>_query.fvecs and <prefix>_groundtruth.ivecs
 // are read; the graph file is <prefix>.kgraph.graph, <prefix>.efanna.graph
@@ -19,7 +19,8 @@
 // tunes L to WEAVESS_ACC_SET (default 0.95) recall.  EFANNA builds start
 // from InitRandom's graph (the reference's KD-tree initialisation is not
 // built) and are written in the plain layout.  NSG builds run NN-Descent,
-// then the NSG refine, and are written in the NSG layout; their search
+// then the NSG refine; Vamana builds run INIT_RAND and two Vamana passes
+// (alpha 1, then 2).  Both are written in the NSG layout, and their search
 // starts from the centroid entry point stored in the file.
 #include <cstdlib>
 #include <cstring>
@@ -75,6 +76,16 @@ const std::map<std::string, NS> &table_nsg() {
     return t;
 }
 
+// Vamana's L_refine and R_refine per dataset; its INIT_RAND draws R
+// neighbours a node ("L" = R) (include/weavess/exp_data.h:266-313).
+const std::map<std::string, std::pair<unsigned, unsigned>> &table_vamana() {
+    static const std::map<std::string, std::pair<unsigned, unsigned>> t = {
+        {"siftsmall", {30, 20}}, {"sift1M", {70, 50}}, {"gist", {60, 50}}, {"glove-100", {120, 110}},
+        {"c1", {40, 24}},        {"c2", {70, 50}},
+    };
+    return t;
+}
+
 unsigned env_u(const char *name, unsigned dflt) {
     const char *v = std::getenv(name);
     return v && *v ? (unsigned)std::strtoul(v, nullptr, 10) : dflt;
@@ -84,17 +95,24 @@ unsigned env_u(const char *name, unsigned dflt) {
 
 int main(int argc, char **argv) {
     const std::string alg = argc > 1 ? argv[1] : "";
-    if (argc < 4 || argc > 5 || (alg != "kgraph" && alg != "efanna" && alg != "nsg")) {
-        std::cout << "./weavess_main kgraph|efanna|nsg dataset_prefix build|search|info [L_search]" << std::endl;
+    if (argc < 4 || argc > 5 || (alg != "kgraph" && alg != "efanna" && alg != "nsg" && alg != "vamana")) {
+        std::cout << "./weavess_main kgraph|efanna|nsg|vamana dataset_prefix build|search|info [L_search]"
+                  << std::endl;
         return 1;
     }
-    const bool efanna = alg == "efanna", nsg = alg == "nsg";
+    const bool efanna = alg == "efanna", nsg = alg == "nsg", vamana = alg == "vamana";
     const std::string prefix(argv[2]), exc(argv[3]);
     std::string name = prefix.substr(prefix.find_last_of('/') + 1);
     const auto &tab = efanna ? table_efanna() : table();
     KG kg = tab.count(name) ? tab.at(name) : tab.at("c1");
     NS ns = table_nsg().count(name) ? table_nsg().at(name) : table_nsg().at("c1");
     if (nsg) kg = ns.kg;
+    if (vamana) {
+        const auto vm = table_vamana().count(name) ? table_vamana().at(name) : table_vamana().at("c1");
+        ns.L_refine = vm.first;
+        ns.R_refine = vm.second;
+        kg.L = vm.second;  // INIT_RAND's neighbours a node
+    }
     weavess::Parameters P;
     P.set<unsigned>("K", env_u("WEAVESS_K", kg.K));
     P.set<unsigned>("L", env_u("WEAVESS_L", kg.L));
@@ -109,7 +127,11 @@ int main(int argc, char **argv) {
     P.set<unsigned>("R_refine", env_u("WEAVESS_R_REFINE", ns.R_refine));
     P.set<unsigned>("C_refine", env_u("WEAVESS_C_REFINE", ns.C));
     const char *gf = std::getenv("WEAVESS_GRAPH");
-    const std::string graph = gf ? gf : prefix + (efanna ? ".efanna.graph" : nsg ? ".nsg.graph" : ".kgraph.graph");
+    const std::string graph = gf ? gf
+                                 : prefix + (efanna   ? ".efanna.graph"
+                                             : nsg    ? ".nsg.graph"
+                                             : vamana ? ".vamana.graph"
+                                                      : ".kgraph.graph");
     const std::string base = prefix + "_base.fvecs", query = prefix + "_query.fvecs",
                       ground = prefix + "_groundtruth.ivecs";
     const std::string dtype = std::getenv("WEAVESS_DTYPE") ? std::getenv("WEAVESS_DTYPE") : "f32";
@@ -120,20 +142,26 @@ int main(int argc, char **argv) {
                : (dtype == "bf16"  ? weavess::REFINE_NN_DESCENT_HIP_BF16
                   : dtype == "fp8" ? weavess::REFINE_NN_DESCENT_HIP_FP8
                                    : weavess::REFINE_NN_DESCENT_HIP);
-    const std::string ent = std::getenv("WEAVESS_ENTRY") ? std::getenv("WEAVESS_ENTRY") : nsg ? "centroid" : "rand";
+    const std::string ent =
+        std::getenv("WEAVESS_ENTRY") ? std::getenv("WEAVESS_ENTRY") : (nsg || vamana) ? "centroid" : "rand";
     const weavess::TYPE entry = ent == "centroid"      ? weavess::SEARCH_ENTRY_CENTROID
                                 : ent == "subcentroid" ? weavess::SEARCH_ENTRY_SUB_CENTROID
                                                        : weavess::SEARCH_ENTRY_RAND;
-    const std::string fmt = std::getenv("WEAVESS_FORMAT") ? std::getenv("WEAVESS_FORMAT") : nsg ? "nsg" : "kgraph";
+    const std::string fmt =
+        std::getenv("WEAVESS_FORMAT") ? std::getenv("WEAVESS_FORMAT") : (nsg || vamana) ? "nsg" : "kgraph";
     const weavess::TYPE gtype = fmt == "nsg" ? weavess::INDEX_NSG : fmt == "ssg" ? weavess::INDEX_SSG
                                                                                  : weavess::INDEX_KGRAPH;
     try {
         weavess::IndexBuilder builder(8);
         if (exc == "build") {
             if (gtype == weavess::INDEX_SSG) throw std::invalid_argument("SSG graphs are not built here");
-            builder.load(base.c_str(), query.c_str(), ground.c_str(), P)->init(weavess::INIT_RANDOM);
+            builder.load(base.c_str(), query.c_str(), ground.c_str(), P)
+                ->init(vamana ? weavess::INIT_RAND : weavess::INIT_RANDOM);
             std::cout << "Init cost: " << builder.GetBuildTime().count() << std::endl;
-            builder.refine(refine);
+            if (vamana)  // test/main.cpp:168-174
+                builder.refine(weavess::REFINE_VAMANA)->refine(weavess::REFINE_VAMANA);
+            else
+                builder.refine(refine);
             if (nsg) {  // test/main.cpp:76-81
                 std::cout << "Init cost: " << builder.GetBuildTime().count() << std::endl;
                 builder.refine(weavess::REFINE_NSG);
