@@ -243,6 +243,16 @@ typedef struct {
  * connectivity pass.  The result replaces the index's graph. */
 int wv_index_build_vamana(wv_index *idx, const wv_vamana_params *p, wv_nsg_stats *stats);
 
+/* DPG refine of the index's graph, which must be its kNN graph
+ * (ComponentRefineDPG, src/component_refine.cpp:728-790): each node keeps up
+ * to `range` (the reference's "K" / 2) of its first 2 * range neighbours,
+ * those occluded by the fewest nearer ones (ComponentPruneDPG,
+ * component_prune.cpp:124-186); then every edge is mirrored and each list
+ * sorted (ComponentConnReverse, component_conn.cpp:228-268).  Rows are not
+ * bounded; stats->width reports the widest.  The result replaces the
+ * index's graph. */
+int wv_index_build_dpg(wv_index *idx, uint32_t range, wv_nsg_stats *stats);
+
 /* Greedy search of Q queries (host arrays).  entries: Q*L ids or NULL
  * (seeded GenRandom per query).  L_per_q/k_per_q: NULL or Q values each
  * (then `entries` must be NULL).  Requires k <= L, and L < N for random

@@ -90,6 +90,9 @@ def lib():
         L.wvo_nsg_conn.argtypes = [_f32p, C.c_uint32, C.c_uint32, _u32p, _u32p, C.c_uint32, C.c_uint32,
                                    C.c_uint32, C.c_uint32, C.c_uint64, _u32p, _f32p, _u32p, C.c_uint32,
                                    C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.c_int]
+        L.wvo_dpg.restype = C.c_int
+        L.wvo_dpg.argtypes = [_f32p, C.c_uint32, C.c_uint32, _u32p, _u32p, C.c_uint32, C.c_uint32, _u32p,
+                              _f32p, _u32p, C.c_uint32, C.POINTER(C.c_uint32), C.c_int]
         L.wvo_vamana.restype = C.c_int
         L.wvo_vamana.argtypes = [_f32p, C.c_uint32, C.c_uint32, _u32p, _u32p, C.c_uint32, C.c_uint32,
                                  C.c_uint32, C.c_uint32, C.c_float, C.c_uint32, C.c_uint64, _u32p, _f32p,
@@ -299,6 +302,28 @@ def vamana(base, ids, deg, L, R, C_, alpha, ep, seed=0):
             continue
         if rc != 0:
             raise ValueError("vamana: bad arguments")
+        return oi[:, :w.value], od[:, :w.value], odeg
+
+
+def dpg(base, ids, deg, range_):
+    """DPG's prune of graph ids/deg to `range_` a node, then the reverse
+    union; returns (ids, dist, deg)."""
+    base = _c(base, np.float32)
+    ids = _c(ids, np.uint32)
+    deg = _c(deg, np.uint32)
+    N, d = base.shape
+    wcap = 2 * range_
+    while True:
+        oi = np.zeros((N, wcap), np.uint32)
+        od = np.zeros((N, wcap), np.float32)
+        odeg = np.zeros(N, np.uint32)
+        w = C.c_uint32(0)
+        rc = lib().wvo_dpg(base, N, d, ids, deg, ids.shape[1], range_, oi, od, odeg, wcap, C.byref(w), 0)
+        if rc == -2:
+            wcap = w.value
+            continue
+        if rc != 0:
+            raise ValueError("dpg: bad arguments")
         return oi[:, :w.value], od[:, :w.value], odeg
 
 

@@ -1219,6 +1219,59 @@ int wvo_vamana(const float *base, uint32_t N, uint32_t d, const uint32_t *ids, c
     return write_graph(G, out_ids, out_dist, out_deg, wcap, width);
 }
 
+int wvo_dpg(const float *base, uint32_t N, uint32_t d, const uint32_t *ids, const uint32_t *deg, uint32_t stride,
+            uint32_t range, uint32_t *out_ids, float *out_dist, uint32_t *out_deg, uint32_t wcap, uint32_t *width,
+            int nthreads) {
+    /* ComponentRefineDPG (src/component_refine.cpp:728-790): per node, its
+     * list by (distance, id), the first len = min(size, 2 range); hit[j] =
+     * the earlier entries i with d(p_i, p_j) < d(node, p_j); cut = the
+     * (range+1)-th smallest hit; keep, in order, the entries with hit <= cut,
+     * at most range (all of them when len <= range, where the reference reads
+     * past its array).  Then ComponentConnReverse (component_conn.cpp:
+     * 228-268): each list takes every reverse edge, sorted, repeated ids
+     * dropped. */
+    if (!base || !ids || !deg || !out_ids || !out_deg || N < 2 || d == 0 || range == 0 || stride == 0) return -1;
+    const int nth = nthreads_or_default(nthreads);
+    std::vector<std::vector<uint64_t>> pr(N);
+#pragma omp parallel for num_threads(nth) schedule(dynamic, 64)
+    for (int64_t nn = 0; nn < (int64_t)N; ++nn) {
+        const uint32_t n = (uint32_t)nn;
+        const float *x = base + (size_t)n * d;
+        std::vector<uint64_t> p;
+        for (uint32_t j = 0; j < std::min(deg[n], stride); ++j) {
+            const uint32_t v = ids[(size_t)n * stride + j];
+            if (v < N && v != n) p.push_back(make_key(dist_canon(x, base + (size_t)v * d, d), v));
+        }
+        std::sort(p.begin(), p.end());
+        p.erase(std::unique(p.begin(), p.end()), p.end());
+        const uint32_t len = std::min<uint32_t>((uint32_t)p.size(), 2 * range);
+        if (len <= range) {
+            pr[n].assign(p.begin(), p.begin() + len);
+            continue;
+        }
+        std::vector<uint32_t> hit(len, 0);
+        for (uint32_t i = 0; i + 1 < len; ++i)
+            for (uint32_t j = i + 1; j < len; ++j)
+                if (dist_canon(base + (size_t)key_id(p[i]) * d, base + (size_t)key_id(p[j]) * d, d) <
+                    key_dist(p[j]))
+                    ++hit[j];
+        std::vector<uint32_t> sh(hit);
+        std::sort(sh.begin(), sh.end());
+        const uint32_t cut = sh[range];
+        for (uint32_t j = 0; j < len && pr[n].size() < range; ++j)
+            if (hit[j] <= cut) pr[n].push_back(p[j]);
+    }
+    std::vector<std::vector<uint64_t>> G(pr);
+    for (uint32_t n = 0; n < N; ++n)
+        for (uint64_t k : pr[n]) G[key_id(k)].push_back(make_key(key_dist(k), n));
+#pragma omp parallel for num_threads(nth) schedule(dynamic, 64)
+    for (int64_t t = 0; t < (int64_t)N; ++t) {
+        std::sort(G[t].begin(), G[t].end());
+        G[t].erase(std::unique(G[t].begin(), G[t].end()), G[t].end());
+    }
+    return write_graph(G, out_ids, out_dist, out_deg, wcap, width);
+}
+
 int wvo_nsg_conn(const float *base, uint32_t N, uint32_t d, const uint32_t *ids, const uint32_t *deg,
                  uint32_t stride, uint32_t L, uint32_t C, uint32_t ep, uint64_t seed, uint32_t *out_ids,
                  float *out_dist, uint32_t *out_deg, uint32_t wcap, uint32_t *width, uint32_t *conn_edges,

@@ -107,6 +107,12 @@ int wvo_vamana(const float *base, uint32_t N, uint32_t d, const uint32_t *ids, c
                uint32_t stride, uint32_t L, uint32_t R, uint32_t C, float alpha, uint32_t ep, uint64_t seed,
                uint32_t *out_ids, float *out_dist, uint32_t *out_deg, uint32_t wcap, uint32_t *width,
                int nthreads);
+/* ComponentRefineDPG + ComponentConnReverse (src/component_refine.cpp:
+ * 728-790, component_conn.cpp:228-268) on graph ids/deg with list length
+ * range ("K" / 2 in the reference). */
+int wvo_dpg(const float *base, uint32_t N, uint32_t d, const uint32_t *ids, const uint32_t *deg, uint32_t stride,
+            uint32_t range, uint32_t *out_ids, float *out_dist, uint32_t *out_deg, uint32_t wcap, uint32_t *width,
+            int nthreads);
 /* The connectivity pass alone, on graph ids/deg. */
 int wvo_nsg_conn(const float *base, uint32_t N, uint32_t d, const uint32_t *ids, const uint32_t *deg,
                  uint32_t stride, uint32_t L, uint32_t C, uint32_t ep, uint64_t seed, uint32_t *out_ids,

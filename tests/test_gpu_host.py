@@ -203,3 +203,29 @@ def test_pipeline_vamana_build_and_search(ctx, tmp_path, small_int):
     res, *_ = O.search_greedy(g2, d2, base, q, 60, 10, entries=ent)
     assert float(re.search(r"10 NN accuracy: ([0-9.]+)", out).group(1)) == pytest.approx(
         O.recall(res, gt, 10), abs=1e-6)
+
+
+def test_pipeline_dpg_build_and_search(ctx, tmp_path, small_int):
+    """weavess_main dpg: NN-Descent, then DPG (K / 2 a node, mirrored);
+    equal to the oracle's DPG of the oracle's kNN graph, and the search
+    from random entries gives the oracle's recall."""
+    base, q = small_int
+    pre = str(tmp_path / "c1")
+    io.write_fvecs(pre + "_base.fvecs", base)
+    io.write_fvecs(pre + "_query.fvecs", q)
+    gt, _ = O.exact_knn(base, q, 10)
+    io.write_ivecs(pre + "_groundtruth.ivecs", gt)
+    env = dict(WEAVESS_K="24", WEAVESS_L="32", WEAVESS_ITER="6", WEAVESS_S="10", WEAVESS_R="32",
+               WEAVESS_SEED="7")
+    out = _run(["dpg", pre, "build"], env)
+    assert "__REFINE : DPG (HIP)__" in out
+    gids, gdeg = io.read_graph(pre + ".dpg.graph")
+    kid, _, kdeg = O.nndescent(base, 24, 32, 10, 32, 6, seed=7, cand_slots=128)
+    oi, _, odeg = O.dpg(base, kid, kdeg, 12)
+    assert np.array_equal(gdeg, odeg)
+    for i in range(len(base)):
+        assert gids[i, :gdeg[i]].tolist() == oi[i, :odeg[i]].tolist(), i
+    out = _run(["dpg", pre, "search", "30"], env)
+    res, *_ = O.search_greedy(oi, odeg, base, q, 30, 10, seed=7)
+    assert float(re.search(r"10 NN accuracy: ([0-9.]+)", out).group(1)) == pytest.approx(
+        O.recall(res, gt, 10), abs=1e-6)

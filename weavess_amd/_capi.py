@@ -32,6 +32,7 @@ EXPORTS = [
     "wv_index_search_ext", "wv_index_search_ext_dev", "wv_index_autotune", "wv_search_graph_create",
     "wv_search_graph_launch", "wv_search_graph_destroy", "wv_mlp_grad", "wv_mlp_train",
     "wv_mlp_predict", "wv_index_build_efanna", "wv_index_build_nsg", "wv_index_build_vamana",
+    "wv_index_build_dpg",
 ]
 
 
@@ -166,6 +167,7 @@ def load() -> C.CDLL:
         "wv_index_build_efanna": ([vp, vp, vp, vp, u32, vp], i32),
         "wv_index_build_nsg": ([vp, vp, vp], i32),
         "wv_index_build_vamana": ([vp, vp, vp], i32),
+        "wv_index_build_dpg": ([vp, u32, vp], i32),
         "wv_index_search": ([vp, vp, u32, vp, u64, u32, u32, vp, vp, u32, vp, vp, vp, vp, vp], i32),
         "wv_index_search_dev": ([vp, vp, u32, vp, u64, u32, u32, vp, vp, u32, vp, vp, vp, vp, vp], i32),
         "wv_nndescent_build": ([vp, vp, u32, u32, vp, vp, vp, vp, vp, vp], i32),
@@ -398,6 +400,12 @@ class Index:
         p = VamanaParams(L, R, C_, ep, alpha, 0, seed)
         st = NsgStats()
         self.ctx.check(self.ctx.lib.wv_index_build_vamana(self.h, C.byref(p), C.byref(st)))
+        return st.as_dict()
+
+    def build_dpg(self, range_):
+        """DPG refine of the index's kNN graph (`range_` a node, then mirrored)."""
+        st = NsgStats()
+        self.ctx.check(self.ctx.lib.wv_index_build_dpg(self.h, range_, C.byref(st)))
         return st.as_dict()
 
     def search(self, queries, L, k, seed=0, entries=None, L_per_q=None, k_per_q=None,

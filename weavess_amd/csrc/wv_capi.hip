@@ -420,6 +420,15 @@ int search_collect(wv_index *ix, const uint32_t *graph, const uint32_t *deg, uin
     return WV_OK;
 }
 
+int index_alloc_graph_dev(wv_index *ix, uint32_t W, uint32_t **ids, float **dist, uint32_t **deg) {
+    int rc = index_alloc_graph(ix, W);
+    if (rc) return rc;
+    *ids = ix->graph;
+    *dist = ix->gdist;
+    *deg = ix->deg;
+    return WV_OK;
+}
+
 int index_upload_graph(wv_index *ix, const uint32_t *ids, const float *dist, const uint32_t *deg, uint32_t W) {
     wv_ctx *c = ix->ctx;
     int rc = index_alloc_graph(ix, W);
@@ -1051,6 +1060,17 @@ int wv_index_build_nsg(wv_index *ix, const wv_nsg_params *p, wv_nsg_stats *stats
     WV_HIP(c, hipSetDevice(c->device));
     // the kNN graph stays alive until the result replaces it
     return wv::nsg_build(ix, p, stats, ix->graph, ix->deg, ix->deg_max, wv::RefineRule{true, false, true, 1.0f});
+}
+
+int wv_index_build_dpg(wv_index *ix, uint32_t range, wv_nsg_stats *stats) {
+    if (!ix) return WV_EINVAL;
+    wv_ctx *c = ix->ctx;
+    std::lock_guard<std::mutex> g(c->mu);
+    if (!ix->graph) return fail(c, WV_EINVAL, "index has no graph (build or set the kNN graph first)");
+    if (range == 0 || range > 256) return fail(c, WV_ENOTSUP, "need 1 <= range <= 256 (range=%u)", range);
+    if (ix->dp > 1024) return fail(c, WV_ENOTSUP, "dimension above 1024");
+    WV_HIP(c, hipSetDevice(c->device));
+    return wv::dpg_build(ix, range, stats, ix->graph, ix->deg, ix->deg_max);
 }
 
 int wv_index_build_vamana(wv_index *ix, const wv_vamana_params *p, wv_nsg_stats *stats) {

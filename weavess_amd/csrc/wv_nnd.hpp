@@ -35,6 +35,8 @@ int nndescent_build(wv_index *ix, const wv_nnd_params *p, const uint32_t *init_i
 int search_collect(wv_index *ix, const uint32_t *graph, const uint32_t *deg, uint32_t deg_max, const float *d_q,
                    uint32_t Q, const uint32_t *d_entries, uint32_t L, uint32_t C, uint64_t *d_full,
                    uint32_t *d_evals);
+// Reallocates the index's graph for width W and returns its device buffers.
+int index_alloc_graph_dev(wv_index *ix, uint32_t W, uint32_t **ids, float **dist, uint32_t **deg);
 // Replaces the index's graph with host arrays (N x W ids / dists, deg[N]).
 int index_upload_graph(wv_index *ix, const uint32_t *ids, const float *dist, const uint32_t *deg, uint32_t W);
 
@@ -50,5 +52,10 @@ struct RefineRule {
 // graph.
 int nsg_build(wv_index *ix, const wv_nsg_params *p, wv_nsg_stats *stats, const uint32_t *knn, const uint32_t *kdeg,
               uint32_t kstride, const RefineRule &rule);
+
+// Implemented in wv_nsg.hip: DPG's prune of the graph knn to `range` a
+// node, then the reverse union; the result replaces the index's graph.
+int dpg_build(wv_index *ix, uint32_t range, wv_nsg_stats *stats, const uint32_t *knn, const uint32_t *kdeg,
+              uint32_t kstride);
 
 }  // namespace wv

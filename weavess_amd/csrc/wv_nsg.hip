@@ -471,6 +471,138 @@ double ms_since(std::chrono::high_resolution_clock::time_point t0) {
     return std::chrono::duration<double, std::milli>(std::chrono::high_resolution_clock::now() - t0).count();
 }
 
+// DPG's prune (component_prune.cpp:124-186) of node n's list: the first
+// len = min(size, 2 range) distinct keys (self dropped); hit[j] counts the
+// earlier entries i with d(p_i, p_j) < d(n, p_j); cut is the (range+1)-th
+// smallest hit; the entries with hit <= cut are kept in order, at most
+// range (all of them when len <= range).
+template <int CPL>
+__global__ void __launch_bounds__(256) dpg_prune_kernel(NsgArgs a) {
+    extern __shared__ __align__(16) uint8_t smem[];
+    const int wid = threadIdx.x >> 6, lane = lane_id();
+    const uint32_t n = blockIdx.x * (blockDim.x >> 6) + wid;
+    if (n >= a.N) return;
+    const int R = (int)a.R, cap = 2 * R;
+    const uint32_t sp = a.C;  // power of two >= 2R: sort width of the hits
+    uint8_t *my = smem + (size_t)wid * a.lds_wave;
+    uint64_t *p = reinterpret_cast<uint64_t *>(my);  // 2R
+    uint64_t *ckey = p + cap;                         // 64
+    uint64_t *tmp = ckey + WAVE;                      // 64
+    uint32_t *cid = reinterpret_cast<uint32_t *>(tmp + WAVE);  // 64
+    uint32_t *hit = cid + WAVE;                                // 2R
+    uint32_t *srt = hit + cap;                                 // sp
+    float4 qv[CPL];
+    load_row<CPL>(a.base, a.dp, a.nch, n, qv);
+    int len = 0;
+    const uint32_t dg = min(a.kdeg[n], a.kstride);
+    for (uint32_t c0 = 0; c0 < dg; c0 += WAVE) {
+        const uint32_t id = c0 + lane < dg ? a.knn[(size_t)n * a.kstride + c0 + lane] : ID_NONE;
+        const bool ok = id < a.N && id != n;
+        const uint64_t b = __ballot(ok);
+        const int m = __popcll(b);
+        if (ok) cid[lanes_below(b)] = id;
+        wave_sync();
+        if (m == 0) continue;
+        rows_dist<CPL, rows_g<CPL>()>(a.base, a.dp, a.nch, qv, cid, m, ckey);
+        wave_sync();
+        len = merge_distinct(p, len, cap, ckey, m, tmp);
+    }
+    int cnt = 0;
+    if (len <= R) {
+        for (int i = lane; i < len; i += WAVE) {
+            a.pr_ids[(size_t)n * a.R + i] = key_id(p[i]);
+            a.pr_dist[(size_t)n * a.R + i] = key_dist(p[i]);
+        }
+        cnt = len;
+    } else {
+        for (int i = lane; i < len; i += WAVE) hit[i] = 0;
+        wave_sync();
+        for (int i = 0; i + 1 < len; ++i) {
+            float4 pv[CPL];
+            load_row<CPL>(a.base, a.dp, a.nch, key_id(p[i]), pv);
+            for (int t0 = i + 1; t0 < len; t0 += WAVE) {
+                const int m = min(WAVE, len - t0);
+                if (lane < m) cid[lane] = key_id(p[t0 + lane]);
+                wave_sync();
+                rows_dist<CPL, rows_g<CPL>()>(a.base, a.dp, a.nch, pv, cid, m, ckey);
+                wave_sync();
+                if (lane < m && key_dist(ckey[lane]) < key_dist(p[t0 + lane])) hit[t0 + lane] += 1;
+                wave_sync();
+            }
+        }
+        for (uint32_t i = lane; i < sp; i += WAVE) srt[i] = (int)i < len ? hit[i] : 0xFFFFFFFFu;
+        wave_sync();
+        lds_sort32(srt, (int)sp);
+        const uint32_t cut = srt[R];
+        for (int t0 = 0; t0 < len && cnt < R; t0 += WAVE) {
+            const int j = t0 + lane;
+            const bool keep = j < len && hit[j] <= cut;
+            const uint64_t b = __ballot(keep);
+            const int pos = cnt + (int)lanes_below(b);
+            if (keep && pos < R) {
+                a.pr_ids[(size_t)n * a.R + pos] = key_id(p[j]);
+                a.pr_dist[(size_t)n * a.R + pos] = key_dist(p[j]);
+            }
+            cnt = min(R, cnt + __popcll(b));
+        }
+    }
+    if (lane == 0) a.pr_cnt[n] = (uint32_t)cnt;
+}
+
+// ConnReverse (component_conn.cpp:228-268): node t's list plus every
+// reverse edge, as keys in t's CSR segment; own entries first, then the
+// reverse ones at a per-node cursor.
+__global__ void dpg_union_fill_kernel(const uint32_t *pr_ids, const float *pr_dist, const uint32_t *pr_cnt, uint32_t N,
+                                      uint32_t R, const uint32_t *off, uint32_t *cur, uint64_t *keys) {
+    for (uint32_t n = blockIdx.x * blockDim.x + threadIdx.x; n < N; n += gridDim.x * blockDim.x) {
+        const uint32_t c = pr_cnt[n];
+        for (uint32_t j = 0; j < c; ++j) {
+            const uint32_t t = pr_ids[(size_t)n * R + j];
+            const float dd = pr_dist[(size_t)n * R + j];
+            keys[off[n] + j] = make_key(dd, t);
+            keys[off[t] + pr_cnt[t] + atomicAdd(&cur[t], 1u)] = make_key(dd, n);
+        }
+    }
+}
+
+// Distinct keys of each sorted segment: count (write == 0) or write the
+// padded row of width W.
+__global__ void dpg_union_rows_kernel(const uint64_t *keys, const uint32_t *off, uint32_t N, uint32_t *ucnt,
+                                      uint32_t W, uint32_t *gid, float *gdist, uint32_t *gdeg, int write) {
+    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < N; t += gridDim.x * blockDim.x) {
+        uint32_t u = 0;
+        uint64_t prev = KEY_EMPTY;
+        for (uint32_t i = off[t]; i < off[t + 1]; ++i) {
+            const uint64_t k = keys[i];
+            if (k == prev) continue;
+            prev = k;
+            if (write) {
+                gid[(size_t)t * W + u] = key_id(k);
+                gdist[(size_t)t * W + u] = key_dist(k);
+            }
+            ++u;
+        }
+        if (write) {
+            for (uint32_t i = u; i < W; ++i) {
+                gid[(size_t)t * W + i] = ID_NONE;
+                gdist[(size_t)t * W + i] = -1.0f;
+            }
+            gdeg[t] = u;
+        } else {
+            ucnt[t] = u;
+        }
+    }
+}
+
+template <int CPL>
+hipError_t launch_dpg(const NsgArgs &a, hipStream_t s) {
+    const size_t lds = (size_t)NWPB * a.lds_wave;
+    hipError_t e = set_lds_k(dpg_prune_kernel<CPL>, lds);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(dpg_prune_kernel<CPL>, dim3((a.N + NWPB - 1) / NWPB), dim3(64 * NWPB), lds, s, a);
+    return hipGetLastError();
+}
+
 }  // namespace
 
 int nsg_build(wv_index *ix, const wv_nsg_params *p, wv_nsg_stats *stats, const uint32_t *knn, const uint32_t *kdeg,
@@ -481,6 +613,7 @@ int nsg_build(wv_index *ix, const wv_nsg_params *p, wv_nsg_stats *stats, const u
     const uint32_t N = v.N, L = p->L, R = p->R, C = p->C, ep = p->ep;
     const uint32_t nch = v.dp / 4;
     const int cpl = nch <= 16 ? 1 : nch <= 32 ? 2 : nch <= 64 ? 4 : nch <= 128 ? 8 : 16;
+    NSG_HIP(hipStreamSynchronize(s));
     const auto t_all = std::chrono::high_resolution_clock::now();
     DBuf B;
     NsgArgs a{};
@@ -677,6 +810,7 @@ int nsg_build(wv_index *ix, const wv_nsg_params *p, wv_nsg_stats *stats, const u
                 uint64_t *d_k = nullptr;
                 NSG_HIP(B.get(&d_k, 1));
                 NSG_HIP(pair_key(cpl, v.base, v.dp, nch, root, u, d_k, s));
+                NSG_HIP(hipStreamSynchronize(s));  // the context's stream does not wait for hipMemcpy
                 uint64_t kk = 0;
                 NSG_HIP(hipMemcpy(&kk, d_k, 8, hipMemcpyDeviceToHost));
                 dd = key_dist(kk);
@@ -715,6 +849,99 @@ int nsg_build(wv_index *ix, const wv_nsg_params *p, wv_nsg_stats *stats, const u
         stats->conn_ms = conn_ms;
         stats->evals = evals;
         stats->conn_edges = conn_edges;
+        stats->width = W;
+    }
+    return WV_OK;
+}
+
+int dpg_build(wv_index *ix, uint32_t range, wv_nsg_stats *stats, const uint32_t *knn, const uint32_t *kdeg,
+              uint32_t kstride) {
+    const IndexView v = index_view(ix);
+    const hipStream_t s = v.stream;
+    const uint32_t N = v.N, R = range, nch = v.dp / 4;
+    const int cpl = nch <= 16 ? 1 : nch <= 32 ? 2 : nch <= 64 ? 4 : nch <= 128 ? 8 : 16;
+    NSG_HIP(hipStreamSynchronize(s));
+    const auto t_all = std::chrono::high_resolution_clock::now();
+    DBuf B;
+    NsgArgs a{};
+    a.base = v.base;
+    a.N = N;
+    a.dp = v.dp;
+    a.nch = nch;
+    a.knn = knn;
+    a.kdeg = kdeg;
+    a.kstride = kstride;
+    a.R = R;
+    a.C = pow2_ge(2 * R, 64);  // hit sort width
+    a.lds_wave = (uint32_t)((((size_t)2 * R * 8 + 64 * 8 * 2 + 64 * 4 + (size_t)2 * R * 4 + (size_t)a.C * 4) + 15) &
+                            ~(size_t)15);
+    NSG_HIP(B.get(&a.pr_ids, (size_t)N * R));
+    NSG_HIP(B.get(&a.pr_dist, (size_t)N * R));
+    NSG_HIP(B.get(&a.pr_cnt, N));
+    switch (cpl) {
+    case 1: NSG_HIP(launch_dpg<1>(a, s)); break;
+    case 2: NSG_HIP(launch_dpg<2>(a, s)); break;
+    case 4: NSG_HIP(launch_dpg<4>(a, s)); break;
+    case 8: NSG_HIP(launch_dpg<8>(a, s)); break;
+    default: NSG_HIP(launch_dpg<16>(a, s)); break;
+    }
+    NSG_HIP(hipStreamSynchronize(s));
+    const double prune_ms = ms_since(t_all);
+    const auto t1 = std::chrono::high_resolution_clock::now();
+    // union CSR: own count + in-degree per node
+    uint32_t *incnt = nullptr, *tot = nullptr, *off = nullptr, *cur = nullptr, *ucnt = nullptr;
+    NSG_HIP(B.get(&incnt, (size_t)N + 1));
+    NSG_HIP(B.get(&tot, (size_t)N + 1));
+    NSG_HIP(B.get(&off, (size_t)N + 1));
+    NSG_HIP(B.get(&cur, N));
+    NSG_HIP(B.get(&ucnt, N));
+    NSG_HIP(hipMemsetAsync(incnt, 0, ((size_t)N + 1) * 4, s));
+    NSG_HIP(hipMemsetAsync(tot, 0, ((size_t)N + 1) * 4, s));
+    NSG_HIP(hipMemsetAsync(cur, 0, (size_t)N * 4, s));
+    hipLaunchKernelGGL(nsg_rev_count_kernel, dim3(1024), dim3(256), 0, s, a.pr_ids, a.pr_cnt, N, R, incnt);
+    NSG_HIP(hipGetLastError());
+    std::vector<uint32_t> h_in(N + 1), h_own(N);
+    NSG_HIP(hipMemcpyAsync(h_in.data(), incnt, (size_t)N * 4, hipMemcpyDeviceToHost, s));
+    NSG_HIP(hipMemcpyAsync(h_own.data(), a.pr_cnt, (size_t)N * 4, hipMemcpyDeviceToHost, s));
+    NSG_HIP(hipStreamSynchronize(s));
+    std::vector<uint32_t> h_off(N + 1, 0);
+    for (uint32_t i = 0; i < N; ++i) h_off[i + 1] = h_off[i] + h_own[i] + h_in[i];
+    const uint32_t total = h_off[N];
+    NSG_HIP(hipMemcpyAsync(off, h_off.data(), ((size_t)N + 1) * 4, hipMemcpyHostToDevice, s));
+    uint64_t *keys = nullptr, *sorted = nullptr;
+    NSG_HIP(B.get(&keys, total));
+    NSG_HIP(B.get(&sorted, total));
+    hipLaunchKernelGGL(dpg_union_fill_kernel, dim3(1024), dim3(256), 0, s, a.pr_ids, a.pr_dist, a.pr_cnt, N, R, off,
+                       cur, keys);
+    NSG_HIP(hipGetLastError());
+    size_t bytes = 0;
+    NSG_HIP(hipcub::DeviceSegmentedRadixSort::SortKeys(nullptr, bytes, keys, sorted, (int)total, (int)N, off, off + 1,
+                                                      0, 64, s));
+    uint8_t *tmpb = nullptr;
+    NSG_HIP(B.get(&tmpb, bytes));
+    NSG_HIP(hipcub::DeviceSegmentedRadixSort::SortKeys(tmpb, bytes, keys, sorted, (int)total, (int)N, off, off + 1, 0,
+                                                      64, s));
+    hipLaunchKernelGGL(dpg_union_rows_kernel, dim3(1024), dim3(256), 0, s, sorted, off, N, ucnt, 0u, nullptr, nullptr,
+                       nullptr, 0);
+    NSG_HIP(hipGetLastError());
+    NSG_HIP(hipStreamSynchronize(s));  // the context's stream does not wait for hipMemcpy
+    std::vector<uint32_t> h_u(N);
+    NSG_HIP(hipMemcpy(h_u.data(), ucnt, (size_t)N * 4, hipMemcpyDeviceToHost));
+    uint32_t W = 1;
+    for (uint32_t x : h_u) W = std::max(W, x);
+    uint32_t *gid = nullptr, *gdeg = nullptr;
+    float *gdist = nullptr;
+    int rc = index_alloc_graph_dev(ix, W, &gid, &gdist, &gdeg);  // frees the kNN graph
+    if (rc) return rc;
+    hipLaunchKernelGGL(dpg_union_rows_kernel, dim3(1024), dim3(256), 0, s, sorted, off, N, ucnt, W, gid, gdist, gdeg,
+                       1);
+    NSG_HIP(hipGetLastError());
+    NSG_HIP(hipStreamSynchronize(s));
+    if (stats) {
+        *stats = wv_nsg_stats{};
+        stats->total_ms = ms_since(t_all);
+        stats->prune_ms = prune_ms;
+        stats->insert_ms = ms_since(t1);
         stats->width = W;
     }
     return WV_OK;

@@ -51,6 +51,9 @@ IndexBuilder *IndexBuilder::refine(TYPE type, bool) {
         a.reset(new ComponentRefineNNDescentHIP(index_, WV_DTYPE_BF16));
     } else if (type == REFINE_NN_DESCENT_HIP_FP8) {
         a.reset(new ComponentRefineNNDescentHIP(index_, WV_DTYPE_FP8));
+    } else if (type == REFINE_DPG) {
+        std::cout << "__REFINE : DPG (HIP)__" << std::endl;  // src/builder.cpp:131-133
+        a.reset(new ComponentRefineDPGHIP(index_));
     } else if (type == REFINE_VAMANA) {
         std::cout << "__REFINE : VAMANA (HIP)__" << std::endl;  // src/builder.cpp:134-136
         a.reset(new ComponentRefineVamanaHIP(index_));

@@ -241,6 +241,16 @@ void ComponentRefineNSGHIP::RefineInner() {
     download_final_graph(index, stats_.width);
 }
 
+void ComponentRefineDPGHIP::RefineInner() {
+    const unsigned range = index->getParam().get<unsigned>("K") / 2;  // SetConfigs, component_refine.cpp:756-764
+    if (index->getFinalGraph().size() != index->getBaseLen())
+        throw std::invalid_argument("DPG refines a kNN graph: run init + refine first");
+    upload_final_graph(index);
+    std::cout << "__PRUNE : DPG__ __CONN : REVERSE__" << std::endl;
+    index->check(wv_index_build_dpg(index->deviceIndex(), range, &stats_));
+    download_final_graph(index, stats_.width);
+}
+
 void ComponentRefineVamanaHIP::RefineInner() {
     auto &P = index->getParam();
     wv_vamana_params p{};

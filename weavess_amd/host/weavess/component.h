@@ -106,6 +106,18 @@ private:
     wv_nsg_stats stats_{};
 };
 
+// ComponentRefineDPG (src/component_refine.cpp:728-790) on the GPU: keeps
+// "K" / 2 a node (L_dpg, :763) by occlusion counts, then mirrors every edge
+// (ComponentConnReverse).
+class ComponentRefineDPGHIP : public ComponentRefine {
+public:
+    explicit ComponentRefineDPGHIP(Index *index) : ComponentRefine(index) {}
+    void RefineInner() override;
+
+private:
+    wv_nsg_stats stats_{};
+};
+
 class ComponentSearchEntry : public Component {
 public:
     explicit ComponentSearchEntry(Index *index) : Component(index) {}

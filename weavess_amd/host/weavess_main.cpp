@@ -1,8 +1,8 @@
-// weavess_main.cpp -- KGraph, EFANNA, NSG and Vamana drivers of the
-// pipeline (test/main.cpp:6-34, :66-95, :159-188, :190-217 and :480-546 of
-// the reference, with the HIP stages).
+// weavess_main.cpp -- KGraph, EFANNA, NSG, Vamana and DPG drivers of the
+// pipeline (test/main.cpp:6-34, :66-95, :128-157, :159-188, :190-217 and
+// :480-546 of the reference, with the HIP stages).
 //
-//   weavess_main kgraph|efanna|nsg|vamana <prefix> build|search|info [L_search]
+//   weavess_main kgraph|efanna|nsg|vamana|dpg <prefix> build|search|info [L_search]
 //
 // <prefix>_base.fvecs, <prefix>_query.fvecs and <prefix>_groundtruth.ivecs
 // are read; the graph file is <prefix>.kgraph.graph, <prefix>.efanna.graph
@@ -21,7 +21,8 @@
 // built) and are written in the plain layout.  NSG builds run NN-Descent,
 // then the NSG refine; Vamana builds run INIT_RAND and two Vamana passes
 // (alpha 1, then 2).  Both are written in the NSG layout, and their search
-// starts from the centroid entry point stored in the file.
+// starts from the centroid entry point stored in the file.  DPG builds run
+// NN-Descent, then the DPG refine, and search from random entries.
 #include <cstdlib>
 #include <cstring>
 #include <iostream>
@@ -86,6 +87,17 @@ const std::map<std::string, std::pair<unsigned, unsigned>> &table_vamana() {
     return t;
 }
 
+// DPG's NN-Descent K, L, ITER, S, R per dataset (include/weavess/exp_data.h:
+// 215-264); the DPG refine keeps K / 2 a node.
+const std::map<std::string, KG> &table_dpg() {
+    static const std::map<std::string, KG> t = {
+        {"siftsmall", {25, 50, 6, 10, 100}}, {"sift1M", {100, 100, 12, 25, 300}},
+        {"gist", {100, 100, 12, 20, 100}},   {"glove-100", {100, 130, 12, 20, 100}},
+        {"c1", {50, 70, 10, 10, 100}},       {"c2", {64, 64, 12, 16, 64}},
+    };
+    return t;
+}
+
 unsigned env_u(const char *name, unsigned dflt) {
     const char *v = std::getenv(name);
     return v && *v ? (unsigned)std::strtoul(v, nullptr, 10) : dflt;
@@ -95,18 +107,20 @@ unsigned env_u(const char *name, unsigned dflt) {
 
 int main(int argc, char **argv) {
     const std::string alg = argc > 1 ? argv[1] : "";
-    if (argc < 4 || argc > 5 || (alg != "kgraph" && alg != "efanna" && alg != "nsg" && alg != "vamana")) {
-        std::cout << "./weavess_main kgraph|efanna|nsg|vamana dataset_prefix build|search|info [L_search]"
+    if (argc < 4 || argc > 5 ||
+        (alg != "kgraph" && alg != "efanna" && alg != "nsg" && alg != "vamana" && alg != "dpg")) {
+        std::cout << "./weavess_main kgraph|efanna|nsg|vamana|dpg dataset_prefix build|search|info [L_search]"
                   << std::endl;
         return 1;
     }
-    const bool efanna = alg == "efanna", nsg = alg == "nsg", vamana = alg == "vamana";
+    const bool efanna = alg == "efanna", nsg = alg == "nsg", vamana = alg == "vamana", dpg = alg == "dpg";
     const std::string prefix(argv[2]), exc(argv[3]);
     std::string name = prefix.substr(prefix.find_last_of('/') + 1);
     const auto &tab = efanna ? table_efanna() : table();
     KG kg = tab.count(name) ? tab.at(name) : tab.at("c1");
     NS ns = table_nsg().count(name) ? table_nsg().at(name) : table_nsg().at("c1");
     if (nsg) kg = ns.kg;
+    if (dpg) kg = table_dpg().count(name) ? table_dpg().at(name) : table_dpg().at("c1");
     if (vamana) {
         const auto vm = table_vamana().count(name) ? table_vamana().at(name) : table_vamana().at("c1");
         ns.L_refine = vm.first;
@@ -131,6 +145,7 @@ int main(int argc, char **argv) {
                                  : prefix + (efanna   ? ".efanna.graph"
                                              : nsg    ? ".nsg.graph"
                                              : vamana ? ".vamana.graph"
+                                             : dpg    ? ".dpg.graph"
                                                       : ".kgraph.graph");
     const std::string base = prefix + "_base.fvecs", query = prefix + "_query.fvecs",
                       ground = prefix + "_groundtruth.ivecs";
@@ -162,9 +177,9 @@ int main(int argc, char **argv) {
                 builder.refine(weavess::REFINE_VAMANA)->refine(weavess::REFINE_VAMANA);
             else
                 builder.refine(refine);
-            if (nsg) {  // test/main.cpp:76-81
+            if (nsg || dpg) {  // test/main.cpp:76-81, :137-142
                 std::cout << "Init cost: " << builder.GetBuildTime().count() << std::endl;
-                builder.refine(weavess::REFINE_NSG);
+                builder.refine(nsg ? weavess::REFINE_NSG : weavess::REFINE_DPG);
             } else if (gtype == weavess::INDEX_NSG) {
                 weavess::ComponentRefineEntryCentroidHIP(builder.index()).EntryInner();
             }
