@@ -243,6 +243,28 @@ typedef struct {
  * connectivity pass.  The result replaces the index's graph. */
 int wv_index_build_vamana(wv_index *idx, const wv_vamana_params *p, wv_nsg_stats *stats);
 
+/* SSG refine parameters (ComponentRefineSSG::SetConfigs,
+ * src/component_refine.cpp:584-591). */
+typedef struct {
+    uint32_t L;      /* "L_refine": two-hop candidates a node, 1..2048             */
+    uint32_t R;      /* "R_refine": list length, 1..256                             */
+    float angle;     /* "A": minimum angle between kept edges, degrees (60)         */
+    uint32_t n_try;  /* "n_try": roots of the connectivity pass (the entry set eps_) */
+    uint64_t seed;   /* the roots                                                   */
+} wv_ssg_params;
+
+/* SSG refine of the index's graph, which must be its kNN graph
+ * (ComponentRefineSSG::RefineInner, src/component_refine.cpp:550-582):
+ * candidates are a node's neighbours' neighbours, up to L, plus its own
+ * list (ComponentCandidatePropagation2, component_candidate.cpp:89-108); a
+ * candidate is kept unless it makes an angle below `angle` with a kept one
+ * (ComponentPruneSSG, component_prune.cpp:71-122); reverse edges are
+ * inserted with the same rule (:647-727, as one batch); from n_try random
+ * roots (written to eps) every node is linked in
+ * (ComponentConnSSGDFS, component_conn.cpp:169-226).  The result replaces
+ * the index's graph. */
+int wv_index_build_ssg(wv_index *idx, const wv_ssg_params *p, uint32_t *eps, wv_nsg_stats *stats);
+
 /* DPG refine of the index's graph, which must be its kNN graph
  * (ComponentRefineDPG, src/component_refine.cpp:728-790): each node keeps up
  * to `range` (the reference's "K" / 2) of its first 2 * range neighbours,

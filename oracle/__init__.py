@@ -90,6 +90,12 @@ def lib():
         L.wvo_nsg_conn.argtypes = [_f32p, C.c_uint32, C.c_uint32, _u32p, _u32p, C.c_uint32, C.c_uint32,
                                    C.c_uint32, C.c_uint32, C.c_uint64, _u32p, _f32p, _u32p, C.c_uint32,
                                    C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.c_int]
+        L.wvo_ssg_threshold.restype = C.c_float
+        L.wvo_ssg_threshold.argtypes = [C.c_float]
+        L.wvo_ssg.restype = C.c_int
+        L.wvo_ssg.argtypes = [_f32p, C.c_uint32, C.c_uint32, _u32p, _u32p, C.c_uint32, C.c_uint32, C.c_uint32,
+                              C.c_float, C.c_uint32, C.c_uint64, _u32p, _f32p, _u32p, C.c_uint32,
+                              C.POINTER(C.c_uint32), _u32p, C.POINTER(C.c_uint32), C.c_int]
         L.wvo_dpg.restype = C.c_int
         L.wvo_dpg.argtypes = [_f32p, C.c_uint32, C.c_uint32, _u32p, _u32p, C.c_uint32, C.c_uint32, _u32p,
                               _f32p, _u32p, C.c_uint32, C.POINTER(C.c_uint32), C.c_int]
@@ -303,6 +309,33 @@ def vamana(base, ids, deg, L, R, C_, alpha, ep, seed=0):
         if rc != 0:
             raise ValueError("vamana: bad arguments")
         return oi[:, :w.value], od[:, :w.value], odeg
+
+
+def ssg(base, knn_ids, knn_deg, L, R, angle=60.0, n_try=10, seed=0):
+    """SSG refine; returns (ids, dist, deg, eps, conn_edges)."""
+    base = _c(base, np.float32)
+    knn_ids = _c(knn_ids, np.uint32)
+    knn_deg = _c(knn_deg, np.uint32)
+    N, d = base.shape
+    wcap = R + 4
+    eps = np.zeros(n_try, np.uint32)
+    while True:
+        oi = np.zeros((N, wcap), np.uint32)
+        od = np.zeros((N, wcap), np.float32)
+        odeg = np.zeros(N, np.uint32)
+        w, e = C.c_uint32(0), C.c_uint32(0)
+        rc = lib().wvo_ssg(base, N, d, knn_ids, knn_deg, knn_ids.shape[1], L, R, angle, n_try, seed, oi, od,
+                           odeg, wcap, C.byref(w), eps, C.byref(e), 0)
+        if rc == -2:
+            wcap = w.value
+            continue
+        if rc != 0:
+            raise ValueError("ssg: bad arguments")
+        return oi[:, :w.value], od[:, :w.value], odeg, eps, e.value
+
+
+def ssg_threshold(angle):
+    return float(lib().wvo_ssg_threshold(angle))
 
 
 def dpg(base, ids, deg, range_):

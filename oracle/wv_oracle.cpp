@@ -1272,6 +1272,141 @@ int wvo_dpg(const float *base, uint32_t N, uint32_t d, const uint32_t *ids, cons
     return write_graph(G, out_ids, out_dist, out_deg, wcap, width);
 }
 
+namespace {
+constexpr uint64_t TAG_SSGE = 0x53534745ull;  /* "SSGE": SSG's n_try roots */
+
+/* PruneSSG's angle rule (component_prune.cpp:71-122) over sorted distinct
+ * keys c[0..m) of node `self` (key distance = squared distance to the node):
+ * each next key p is kept unless a kept r has cos(p, r) > thr, with
+ * cos = (d(p) + d(r) - d(r, p)) / 2 / sqrt(d(p) d(r)) in FP32 (law of
+ * cosines on squared distances); at most R. */
+void ssg_occlude(const float *base, uint32_t d, const uint64_t *c, uint32_t m, uint32_t self, uint32_t R, float thr,
+                 std::vector<uint64_t> &res) {
+    res.clear();
+    std::vector<uint8_t> alive(m, 1);
+    for (uint32_t i = 0; i < m && res.size() < R; ++i) {
+        if (!alive[i] || key_id(c[i]) == self) continue;
+        res.push_back(c[i]);
+        const float *ri = base + (size_t)key_id(c[i]) * d;
+        const float dr = key_dist(c[i]);
+        for (uint32_t j = i + 1; j < m; ++j) {
+            if (!alive[j]) continue;
+            const float dp = key_dist(c[j]);
+            const float djk = dist_canon(ri, base + (size_t)key_id(c[j]) * d, d);
+            const float cs = ((dp + dr) - djk) / 2.0f / std::sqrt(dp * dr);
+            if (cs > thr) alive[j] = 0;
+        }
+    }
+}
+}  // namespace
+
+float wvo_ssg_threshold(float angle) {
+    /* component_refine.cpp:638-639, component_prune.cpp:89-90: cos(A / 180 * pi), A in FP32, pi and
+     * the cosine in FP64, rounded to FP32 */
+    const double kPi = std::acos(-1.0);
+    return (float)std::cos(angle / 180 * kPi);
+}
+
+int wvo_ssg(const float *base, uint32_t N, uint32_t d, const uint32_t *knn, const uint32_t *kdeg, uint32_t kstride,
+            uint32_t L, uint32_t R, float angle, uint32_t n_try, uint64_t seed, uint32_t *out_ids, float *out_dist,
+            uint32_t *out_deg, uint32_t wcap, uint32_t *width, uint32_t *eps, uint32_t *conn_edges, int nthreads) {
+    /* ComponentRefineSSG (src/component_refine.cpp:550-727). */
+    if (!base || !knn || !kdeg || !out_ids || !out_deg || N < 2 || d == 0 || L == 0 || R == 0) return -1;
+    if (n_try == 0 || n_try >= N || kstride == 0) return -1;
+    const int nth = nthreads_or_default(nthreads);
+    const float thr = wvo_ssg_threshold(angle);
+    const Adj g = adj_from(knn, kdeg, kstride, N);
+    std::vector<std::vector<uint64_t>> pr(N);
+#pragma omp parallel for num_threads(nth) schedule(dynamic, 64)
+    for (int64_t nn = 0; nn < (int64_t)N; ++nn) {
+        const uint32_t n = (uint32_t)nn;
+        const float *x = base + (size_t)n * d;
+        /* Propagation2 (component_candidate.cpp:89-108): the neighbours'
+         * neighbours in list order, each once and never the node, until L */
+        std::vector<uint32_t> seen{n};
+        std::vector<uint64_t> c;
+        for (uint32_t nid : g[n]) {
+            if (c.size() >= L) break;
+            if (nid >= N) continue;
+            for (uint32_t v : g[nid]) {
+                if (v >= N || std::find(seen.begin(), seen.end(), v) != seen.end()) continue;
+                seen.push_back(v);
+                c.push_back(make_key(dist_canon(x, base + (size_t)v * d, d), v));
+                if (c.size() >= L) break;
+            }
+        }
+        for (uint32_t v : g[n])  /* PruneSSG adds the node's own list (:74-81) */
+            if (v < N && v != n) c.push_back(make_key(dist_canon(x, base + (size_t)v * d, d), v));
+        std::sort(c.begin(), c.end());
+        c.erase(std::unique(c.begin(), c.end()), c.end());
+        ssg_occlude(base, d, c.data(), (uint32_t)c.size(), n, R, thr, pr[n]);
+    }
+    /* InterInsert with the angle rule (:647-727), as one batch */
+    std::vector<std::vector<uint64_t>> inc(N);
+    for (uint32_t n = 0; n < N; ++n)
+        for (uint64_t k : pr[n]) inc[key_id(k)].push_back(make_key(key_dist(k), n));
+    std::vector<std::vector<uint64_t>> G(N);
+#pragma omp parallel for num_threads(nth) schedule(dynamic, 64)
+    for (int64_t tt = 0; tt < (int64_t)N; ++tt) {
+        const uint32_t t = (uint32_t)tt;
+        std::vector<uint64_t> temp(pr[t]);
+        for (uint64_t k : inc[t]) {
+            bool dup = false;
+            for (uint64_t o : pr[t])
+                if (key_id(o) == key_id(k)) { dup = true; break; }
+            if (!dup) temp.push_back(k);
+        }
+        if (temp.size() <= R) {
+            G[t].swap(temp);
+        } else {
+            std::sort(temp.begin(), temp.end());
+            if (temp.size() > NSG_TCAP) temp.resize(NSG_TCAP);
+            ssg_occlude(base, d, temp.data(), (uint32_t)temp.size(), t, R, thr, G[t]);
+        }
+    }
+    /* ComponentConnSSGDFS (component_conn.cpp:169-226): roots eps = GenRandom
+     * (n_try) on the "SSGE" stream, taken in order; from each, a BFS; when
+     * it stops short, the smallest unreached node u gets an edge from the
+     * smallest reached node with fewer than R neighbours (if any), u counts
+     * as reached, and the BFS goes on from u. */
+    std::vector<uint32_t> roots(n_try);
+    gen_random(stream_of(seed, TAG_SSGE, 0), roots.data(), n_try, N);
+    uint32_t edges = 0;
+    for (uint32_t r = 0; r < n_try; ++r) {
+        std::vector<uint8_t> flag(N, 0);
+        std::vector<uint32_t> q{roots[r]};
+        flag[roots[r]] = 1;
+        size_t head = 0;
+        uint32_t low = 0;  /* every id below `low` is reached */
+        for (;;) {
+            while (head < q.size()) {
+                const uint32_t u = q[head++];
+                for (uint64_t k : G[u]) {
+                    const uint32_t v = key_id(k);
+                    if (!flag[v]) {
+                        flag[v] = 1;
+                        q.push_back(v);
+                    }
+                }
+            }
+            while (low < N && flag[low]) ++low;
+            if (low == N) break;
+            const uint32_t u = low;
+            for (uint32_t j = 0; j < N; ++j)
+                if (flag[j] && G[j].size() < R) {
+                    G[j].push_back(make_key(dist_canon(base + (size_t)j * d, base + (size_t)u * d, d), u));
+                    ++edges;
+                    break;
+                }
+            flag[u] = 1;
+            q.push_back(u);
+        }
+    }
+    if (eps) std::memcpy(eps, roots.data(), n_try * 4);
+    if (conn_edges) *conn_edges = edges;
+    return write_graph(G, out_ids, out_dist, out_deg, wcap, width);
+}
+
 int wvo_nsg_conn(const float *base, uint32_t N, uint32_t d, const uint32_t *ids, const uint32_t *deg,
                  uint32_t stride, uint32_t L, uint32_t C, uint32_t ep, uint64_t seed, uint32_t *out_ids,
                  float *out_dist, uint32_t *out_deg, uint32_t wcap, uint32_t *width, uint32_t *conn_edges,

@@ -113,6 +113,14 @@ int wvo_vamana(const float *base, uint32_t N, uint32_t d, const uint32_t *ids, c
 int wvo_dpg(const float *base, uint32_t N, uint32_t d, const uint32_t *ids, const uint32_t *deg, uint32_t stride,
             uint32_t range, uint32_t *out_ids, float *out_dist, uint32_t *out_deg, uint32_t wcap, uint32_t *width,
             int nthreads);
+/* ComponentRefineSSG (src/component_refine.cpp:550-727): Propagation2
+ * candidates (L = "L_refine"), PruneSSG's angle rule (angle "A" degrees),
+ * batch InterInsert, ComponentConnSSGDFS from n_try roots (written to eps).
+ * wvo_ssg_threshold: the cosine threshold used. */
+float wvo_ssg_threshold(float angle);
+int wvo_ssg(const float *base, uint32_t N, uint32_t d, const uint32_t *knn, const uint32_t *kdeg, uint32_t kstride,
+            uint32_t L, uint32_t R, float angle, uint32_t n_try, uint64_t seed, uint32_t *out_ids, float *out_dist,
+            uint32_t *out_deg, uint32_t wcap, uint32_t *width, uint32_t *eps, uint32_t *conn_edges, int nthreads);
 /* The connectivity pass alone, on graph ids/deg. */
 int wvo_nsg_conn(const float *base, uint32_t N, uint32_t d, const uint32_t *ids, const uint32_t *deg,
                  uint32_t stride, uint32_t L, uint32_t C, uint32_t ep, uint64_t seed, uint32_t *out_ids,

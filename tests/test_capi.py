@@ -42,6 +42,7 @@ def test_struct_layouts_match_header():
     assert ctypes.sizeof(_capi.NsgParams) == 24
     assert ctypes.sizeof(_capi.NsgStats) == 56
     assert ctypes.sizeof(_capi.VamanaParams) == 32
+    assert ctypes.sizeof(_capi.SsgParams) == 24
 
 
 def test_null_context_is_rejected():

@@ -229,3 +229,31 @@ def test_pipeline_dpg_build_and_search(ctx, tmp_path, small_int):
     res, *_ = O.search_greedy(oi, odeg, base, q, 30, 10, seed=7)
     assert float(re.search(r"10 NN accuracy: ([0-9.]+)", out).group(1)) == pytest.approx(
         O.recall(res, gt, 10), abs=1e-6)
+
+
+def test_pipeline_ssg_build_and_search(ctx, tmp_path, small_int):
+    """weavess_main ssg: NN-Descent, then SSG (roots eps_ in the SSG
+    layout); equal to the oracle's SSG of the oracle's kNN graph, and the
+    sub-centroid search from eps_ gives the oracle's recall."""
+    from weavess_amd.entries import subcentroid_entries
+    base, q = small_int
+    pre = str(tmp_path / "c1")
+    io.write_fvecs(pre + "_base.fvecs", base)
+    io.write_fvecs(pre + "_query.fvecs", q)
+    gt, _ = O.exact_knn(base, q, 10)
+    io.write_ivecs(pre + "_groundtruth.ivecs", gt)
+    env = dict(WEAVESS_K="24", WEAVESS_L="32", WEAVESS_ITER="6", WEAVESS_S="10", WEAVESS_R="32",
+               WEAVESS_SEED="8", WEAVESS_L_REFINE="50", WEAVESS_R_REFINE="20")
+    out = _run(["ssg", pre, "build"], env)
+    assert "__REFINE : SSG (HIP)__" in out
+    gids, gdeg, eps = io.read_graph(pre + ".ssg.graph", fmt="ssg")
+    kid, _, kdeg = O.nndescent(base, 24, 32, 10, 32, 6, seed=8, cand_slots=128)
+    oi, _, odeg, oeps, _ = O.ssg(base, kid, kdeg, 50, 20, 60.0, 10, seed=8)
+    assert eps.tolist() == oeps.tolist() and np.array_equal(gdeg, odeg)
+    for i in range(len(base)):
+        assert gids[i, :gdeg[i]].tolist() == oi[i, :odeg[i]].tolist(), i
+    out = _run(["ssg", pre, "search", "30"], env)
+    ent = subcentroid_entries(oeps, len(q), 30, len(base), seed=8)
+    res, *_ = O.search_greedy(oi, odeg, base, q, 30, 10, entries=ent)
+    assert float(re.search(r"10 NN accuracy: ([0-9.]+)", out).group(1)) == pytest.approx(
+        O.recall(res, gt, 10), abs=1e-6)

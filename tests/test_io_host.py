@@ -122,7 +122,7 @@ def test_host_driver_reads_graph_headers(tmp_path):
     assert r.returncode == 2 and "truncated graph file" in r.stderr
     r = subprocess.run([exe, "kgraph", pre, "build"], capture_output=True, text=True,
                        env={**os.environ, "WEAVESS_FORMAT": "ssg"})
-    assert r.returncode == 2 and "SSG graphs are not built here" in r.stderr
+    assert r.returncode == 2 and "SSG files are written by ssg builds" in r.stderr
 
 
 def test_subcentroid_entries_rules():

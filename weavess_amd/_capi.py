@@ -32,7 +32,7 @@ EXPORTS = [
     "wv_index_search_ext", "wv_index_search_ext_dev", "wv_index_autotune", "wv_search_graph_create",
     "wv_search_graph_launch", "wv_search_graph_destroy", "wv_mlp_grad", "wv_mlp_train",
     "wv_mlp_predict", "wv_index_build_efanna", "wv_index_build_nsg", "wv_index_build_vamana",
-    "wv_index_build_dpg",
+    "wv_index_build_dpg", "wv_index_build_ssg",
 ]
 
 
@@ -66,6 +66,11 @@ class NsgParams(C.Structure):
 class VamanaParams(C.Structure):
     _fields_ = [("L", C.c_uint32), ("R", C.c_uint32), ("C", C.c_uint32), ("ep", C.c_uint32),
                 ("alpha", C.c_float), ("reserved", C.c_uint32), ("seed", C.c_uint64)]
+
+
+class SsgParams(C.Structure):
+    _fields_ = [("L", C.c_uint32), ("R", C.c_uint32), ("angle", C.c_float), ("n_try", C.c_uint32),
+                ("seed", C.c_uint64)]
 
 
 class NsgStats(C.Structure):
@@ -168,6 +173,7 @@ def load() -> C.CDLL:
         "wv_index_build_nsg": ([vp, vp, vp], i32),
         "wv_index_build_vamana": ([vp, vp, vp], i32),
         "wv_index_build_dpg": ([vp, u32, vp], i32),
+        "wv_index_build_ssg": ([vp, vp, vp, vp], i32),
         "wv_index_search": ([vp, vp, u32, vp, u64, u32, u32, vp, vp, u32, vp, vp, vp, vp, vp], i32),
         "wv_index_search_dev": ([vp, vp, u32, vp, u64, u32, u32, vp, vp, u32, vp, vp, vp, vp, vp], i32),
         "wv_nndescent_build": ([vp, vp, u32, u32, vp, vp, vp, vp, vp, vp], i32),
@@ -401,6 +407,14 @@ class Index:
         st = NsgStats()
         self.ctx.check(self.ctx.lib.wv_index_build_vamana(self.h, C.byref(p), C.byref(st)))
         return st.as_dict()
+
+    def build_ssg(self, L, R, angle=60.0, n_try=10, seed=0):
+        """SSG refine of the index's kNN graph; returns (stats, eps)."""
+        p = SsgParams(L, R, angle, n_try, seed)
+        st = NsgStats()
+        eps = np.zeros(n_try, np.uint32)
+        self.ctx.check(self.ctx.lib.wv_index_build_ssg(self.h, C.byref(p), _p(eps), C.byref(st)))
+        return st.as_dict(), eps
 
     def build_dpg(self, range_):
         """DPG refine of the index's kNN graph (`range_` a node, then mirrored)."""

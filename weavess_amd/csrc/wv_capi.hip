@@ -1062,6 +1062,21 @@ int wv_index_build_nsg(wv_index *ix, const wv_nsg_params *p, wv_nsg_stats *stats
     return wv::nsg_build(ix, p, stats, ix->graph, ix->deg, ix->deg_max, wv::RefineRule{true, false, true, 1.0f});
 }
 
+int wv_index_build_ssg(wv_index *ix, const wv_ssg_params *p, uint32_t *eps, wv_nsg_stats *stats) {
+    if (!ix) return WV_EINVAL;
+    wv_ctx *c = ix->ctx;
+    std::lock_guard<std::mutex> g(c->mu);
+    if (!p) return fail(c, WV_EINVAL, "params are null");
+    if (!ix->graph) return fail(c, WV_EINVAL, "index has no graph (build or set the kNN graph first)");
+    if (p->L == 0 || p->L > 2048) return fail(c, WV_ENOTSUP, "need 1 <= L <= 2048 (L=%u)", p->L);
+    if (p->R == 0 || p->R > 256) return fail(c, WV_ENOTSUP, "need 1 <= R <= 256 (R=%u)", p->R);
+    if (p->n_try == 0 || p->n_try >= ix->N) return fail(c, WV_EINVAL, "need 1 <= n_try < N (n_try=%u)", p->n_try);
+    if (!(p->angle > 0.0f) || !(p->angle < 180.0f)) return fail(c, WV_EINVAL, "angle must be in (0, 180) degrees");
+    if (ix->dp > 1024) return fail(c, WV_ENOTSUP, "dimension above 1024");
+    WV_HIP(c, hipSetDevice(c->device));
+    return wv::ssg_build(ix, p, eps, stats, ix->graph, ix->deg, ix->deg_max);
+}
+
 int wv_index_build_dpg(wv_index *ix, uint32_t range, wv_nsg_stats *stats) {
     if (!ix) return WV_EINVAL;
     wv_ctx *c = ix->ctx;

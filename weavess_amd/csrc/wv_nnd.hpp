@@ -53,6 +53,10 @@ struct RefineRule {
 int nsg_build(wv_index *ix, const wv_nsg_params *p, wv_nsg_stats *stats, const uint32_t *knn, const uint32_t *kdeg,
               uint32_t kstride, const RefineRule &rule);
 
+// Implemented in wv_nsg.hip: the SSG refine of the graph knn (n_try roots
+// written to eps_out); the result replaces the index's graph.
+int ssg_build(wv_index *ix, const wv_ssg_params *p, uint32_t *eps_out, wv_nsg_stats *stats, const uint32_t *knn,
+              const uint32_t *kdeg, uint32_t kstride);
 // Implemented in wv_nsg.hip: DPG's prune of the graph knn to `range` a
 // node, then the reverse union; the result replaces the index's graph.
 int dpg_build(wv_index *ix, uint32_t range, wv_nsg_stats *stats, const uint32_t *knn, const uint32_t *kdeg,

@@ -19,7 +19,9 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cmath>
 #include <cstring>
+#include <set>
 #include <vector>
 
 #include "wv_device.hpp"
@@ -32,6 +34,7 @@ namespace {
 
 constexpr uint64_t TAG_NSGC = 0x4E534743ull;  // "NSGC": candidate-search entries
 constexpr uint64_t TAG_NSGR = 0x4E534752ull;  // "NSGR": connectivity fallback root
+constexpr uint64_t TAG_SSGE = 0x53534745ull;  // "SSGE": SSG's n_try roots
 constexpr uint32_t NSG_TCAP = 1024;           // candidates of one reverse-insert re-prune
 constexpr int NWPB = 4;                       // wavefronts per block
 
@@ -67,6 +70,9 @@ struct NsgArgs {
     float alpha;           // occlusion factor (NSG: 1)
     uint32_t add_knn;      // candidates include the node's kNN neighbours (NSG)
     uint32_t fill;         // top lists up with the nearest occluded candidates (Vamana)
+    uint32_t angle;        // SSG's angle rule instead of occlusion
+    float thr;             // its cosine threshold
+    uint32_t L, HS, CAPC;  // SSG candidates: two-hop count, visited slots, candidate capacity
 };
 
 template <int CPL>
@@ -93,9 +99,13 @@ constexpr uint8_t ST_DROP = 0, ST_LIVE = 1, ST_KEPT = 2, ST_SELF = 3;
 // with alpha * d(kept, p) < d(node, p); at most R kept, written to res.
 // With `fill`, a list shorter than R is topped up with the nearest dropped
 // keys, in order.  Returns the list length.
+// With `angle` (PruneSSG, component_prune.cpp:84-105) the test is instead
+// cos(p, kept) > thr, the cosine from the law of cosines on squared
+// distances: (d(p) + d(kept) - d(kept, p)) / 2 / sqrt(d(p) d(kept)).
 template <int CPL>
 __device__ int occlude(const float *base, uint32_t dp, uint32_t nch, const uint64_t *cand, int m, uint8_t *st,
-                       int R, float alpha, bool fill, uint32_t *cid, uint32_t *cpos, uint64_t *ckey, uint64_t *res) {
+                       int R, float alpha, bool fill, uint32_t *cid, uint32_t *cpos, uint64_t *ckey, uint64_t *res,
+                       bool angle = false, float thr = 0.f) {
     const int lane = lane_id();
     int cnt = 0, from = 0;
     while (cnt < R) {
@@ -134,7 +144,15 @@ __device__ int occlude(const float *base, uint32_t dp, uint32_t nch, const uint6
             wave_sync();
             if (lane < mm) {
                 const uint32_t pp = cpos[lane];
-                if (alpha * key_dist(ckey[lane]) < key_dist(cand[pp])) st[pp] = ST_DROP;
+                const float dpn = key_dist(cand[pp]), djk = key_dist(ckey[lane]);
+                bool drop;
+                if (angle) {
+                    const float dr = key_dist(ki);
+                    drop = ((dpn + dr) - djk) / 2.0f / sqrtf(dpn * dr) > thr;
+                } else {
+                    drop = alpha * djk < dpn;
+                }
+                if (drop) st[pp] = ST_DROP;
             }
             wave_sync();
         }
@@ -353,7 +371,7 @@ __global__ void __launch_bounds__(256) nsg_insert_kernel(NsgArgs a) {
     for (int i = lane; i < len; i += WAVE) alive[i] = ST_LIVE;
     wave_sync();
     const int cnt = occlude<CPL>(a.base, a.dp, a.nch, cand, len, alive, R, a.alpha, a.fill != 0, cid, cpos, ckey,
-                                 res);
+                                 res, a.angle != 0, a.thr);
     for (int i = lane; i < cnt; i += WAVE) {
         a.out_ids[(size_t)t * a.R + i] = key_id(res[i]);
         a.out_dist[(size_t)t * a.R + i] = key_dist(res[i]);
@@ -469,6 +487,141 @@ std::vector<uint32_t> head_of(const uint32_t *row, uint32_t deg, uint32_t L, uin
 
 double ms_since(std::chrono::high_resolution_clock::time_point t0) {
     return std::chrono::duration<double, std::milli>(std::chrono::high_resolution_clock::now() - t0).count();
+}
+
+// SSG's candidates and prune for node n (Propagation2,
+// component_candidate.cpp:89-108, then PruneSSG, component_prune.cpp:71-122):
+// the neighbours' neighbours in list order, each once and never n, until
+// L; then n's own list; distinct, sorted; the angle rule keeps up to R.
+// The visited ids sit in an exact LDS hash (at most L + 1 ids in 2(L + 1)+
+// slots, so it never fills).
+template <int CPL>
+__global__ void __launch_bounds__(256) ssg_prune_kernel(NsgArgs a) {
+    extern __shared__ __align__(16) uint8_t smem[];
+    const int wid = threadIdx.x >> 6, lane = lane_id();
+    const uint32_t n = blockIdx.x * (blockDim.x >> 6) + wid;
+    if (n >= a.N) return;
+    uint8_t *my = smem + (size_t)wid * a.lds_wave;
+    uint64_t *cand = reinterpret_cast<uint64_t *>(my);  // CAPC
+    uint64_t *ckey = cand + a.CAPC;                      // 64
+    uint64_t *tmp = ckey + WAVE;                         // 64
+    uint64_t *res = tmp + WAVE;                          // R
+    uint32_t *cid = reinterpret_cast<uint32_t *>(res + a.R);  // 64
+    uint32_t *cpos = cid + WAVE;                              // 64
+    uint32_t *vis = cpos + WAVE;                              // HS
+    uint8_t *st = reinterpret_cast<uint8_t *>(vis + a.HS);    // CAPC
+    const uint32_t hmask = a.HS - 1;
+    for (uint32_t i = lane; i < a.HS; i += WAVE) vis[i] = ID_NONE;
+    wave_sync();
+    if (lane == 0) {
+        uint32_t h = fmix32(n) & hmask;
+        vis[h] = n;
+    }
+    wave_sync();
+    float4 qv[CPL];
+    load_row<CPL>(a.base, a.dp, a.nch, n, qv);
+    int len = 0;
+    uint32_t taken = 0;
+    const uint32_t dn = min(a.kdeg[n], a.kstride);
+    for (uint32_t i = 0; i < dn && taken < a.L; ++i) {
+        const uint32_t nid = a.knn[(size_t)n * a.kstride + i];
+        if (nid >= a.N) continue;
+        const uint32_t d2 = min(a.kdeg[nid], a.kstride);
+        for (uint32_t c0 = 0; c0 < d2 && taken < a.L; c0 += WAVE) {
+            const uint32_t v = c0 + lane < d2 ? a.knn[(size_t)nid * a.kstride + c0 + lane] : ID_NONE;
+            bool fresh = v < a.N;
+            if (fresh) {  // not yet visited (lookup)
+                uint32_t h = fmix32(v) & hmask;
+                for (;;) {
+                    const uint32_t e = vis[h];
+                    if (e == v) {
+                        fresh = false;
+                        break;
+                    }
+                    if (e == ID_NONE) break;
+                    h = (h + 1) & hmask;
+                }
+            }
+            const uint64_t b = __ballot(fresh);
+            const uint32_t room = a.L - taken;
+            const bool take = fresh && lanes_below(b) < room;
+            const uint64_t tb = __ballot(take);
+            const int m = __popcll(tb);
+            if (take) {
+                cid[lanes_below(tb)] = v;
+                uint32_t h = fmix32(v) & hmask;
+                while (atomicCAS(&vis[h], ID_NONE, v) != ID_NONE) h = (h + 1) & hmask;
+            }
+            wave_sync();
+            if (m == 0) continue;
+            rows_dist<CPL, rows_g<CPL>()>(a.base, a.dp, a.nch, qv, cid, m, ckey);
+            wave_sync();
+            len = merge_distinct(cand, len, (int)a.CAPC, ckey, m, tmp);
+            taken += (uint32_t)m;
+        }
+    }
+    for (uint32_t c0 = 0; c0 < dn; c0 += WAVE) {  // the node's own list
+        const uint32_t id = c0 + lane < dn ? a.knn[(size_t)n * a.kstride + c0 + lane] : ID_NONE;
+        const bool ok = id < a.N && id != n;
+        const uint64_t b = __ballot(ok);
+        const int m = __popcll(b);
+        if (ok) cid[lanes_below(b)] = id;
+        wave_sync();
+        if (m == 0) continue;
+        rows_dist<CPL, rows_g<CPL>()>(a.base, a.dp, a.nch, qv, cid, m, ckey);
+        wave_sync();
+        len = merge_distinct(cand, len, (int)a.CAPC, ckey, m, tmp);
+    }
+    for (int i = lane; i < len; i += WAVE) st[i] = ST_LIVE;
+    wave_sync();
+    const int cnt = occlude<CPL>(a.base, a.dp, a.nch, cand, len, st, (int)a.R, 1.0f, false, cid, cpos, ckey, res,
+                                 true, a.thr);
+    for (int i = lane; i < cnt; i += WAVE) {
+        a.pr_ids[(size_t)n * a.R + i] = key_id(res[i]);
+        a.pr_dist[(size_t)n * a.R + i] = key_dist(res[i]);
+    }
+    if (lane == 0) a.pr_cnt[n] = (uint32_t)cnt;
+}
+
+// Distance keys of node pairs (u[i], v[i]): one wavefront a pair.
+template <int CPL>
+__global__ void __launch_bounds__(256) pair_keys_kernel(const float *base, uint32_t dp, uint32_t nch,
+                                                        const uint32_t *u, const uint32_t *v, uint32_t P,
+                                                        uint64_t *out) {
+    __shared__ uint32_t cid[4][WAVE];
+    __shared__ uint64_t ck[4][WAVE];
+    const int wid = threadIdx.x >> 6;
+    const uint32_t i = blockIdx.x * (blockDim.x >> 6) + wid;
+    if (i >= P) return;
+    float4 qv[CPL];
+    load_row<CPL>(base, dp, nch, u[i], qv);
+    if (lane_id() == 0) cid[wid][0] = v[i];
+    wave_sync();
+    rows_dist<CPL, rows_g<CPL>()>(base, dp, nch, qv, cid[wid], 1, ck[wid]);
+    wave_sync();
+    if (lane_id() == 0) out[i] = ck[wid][0];
+}
+
+template <int CPL>
+hipError_t launch_ssg(const NsgArgs &a, hipStream_t s) {
+    const size_t lds = (size_t)NWPB * a.lds_wave;
+    hipError_t e = set_lds_k(ssg_prune_kernel<CPL>, lds);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(ssg_prune_kernel<CPL>, dim3((a.N + NWPB - 1) / NWPB), dim3(64 * NWPB), lds, s, a);
+    return hipGetLastError();
+}
+
+hipError_t pair_keys(int cpl, const float *base, uint32_t dp, uint32_t nch, const uint32_t *u, const uint32_t *v,
+                     uint32_t P, uint64_t *out, hipStream_t s) {
+    const dim3 g((P + 3) / 4), b(256);
+    switch (cpl) {
+    case 1: hipLaunchKernelGGL(pair_keys_kernel<1>, g, b, 0, s, base, dp, nch, u, v, P, out); break;
+    case 2: hipLaunchKernelGGL(pair_keys_kernel<2>, g, b, 0, s, base, dp, nch, u, v, P, out); break;
+    case 4: hipLaunchKernelGGL(pair_keys_kernel<4>, g, b, 0, s, base, dp, nch, u, v, P, out); break;
+    case 8: hipLaunchKernelGGL(pair_keys_kernel<8>, g, b, 0, s, base, dp, nch, u, v, P, out); break;
+    default: hipLaunchKernelGGL(pair_keys_kernel<16>, g, b, 0, s, base, dp, nch, u, v, P, out); break;
+    }
+    return hipGetLastError();
 }
 
 // DPG's prune (component_prune.cpp:124-186) of node n's list: the first
@@ -603,6 +756,58 @@ hipError_t launch_dpg(const NsgArgs &a, hipStream_t s) {
     return hipGetLastError();
 }
 
+
+// Reverse insertion of the pruned lists in `a` (pr_*) into a.out_* with the
+// rule in `a` (NSG, Vamana or SSG re-prune); see nsg_insert_kernel.
+int reverse_insert(wv_index *ix, NsgArgs &a, int cpl, DBuf &B, hipStream_t s) {
+    const uint32_t N = a.N, R = a.R;
+    uint32_t *incnt = nullptr, *inoff = nullptr, *incur = nullptr, *insrc = nullptr;
+    float *indist = nullptr;
+    NSG_HIP(B.get(&incnt, (size_t)N + 1));
+    NSG_HIP(B.get(&inoff, (size_t)N + 1));
+    NSG_HIP(B.get(&incur, N));
+    NSG_HIP(B.get(&insrc, (size_t)N * R));
+    NSG_HIP(B.get(&indist, (size_t)N * R));
+    NSG_HIP(hipMemsetAsync(incnt, 0, ((size_t)N + 1) * 4, s));
+    NSG_HIP(hipMemsetAsync(incur, 0, (size_t)N * 4, s));
+    hipLaunchKernelGGL(nsg_rev_count_kernel, dim3(1024), dim3(256), 0, s, a.pr_ids, a.pr_cnt, N, R, incnt);
+    NSG_HIP(hipGetLastError());
+    size_t scan_bytes = 0;
+    NSG_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, scan_bytes, incnt, inoff, N + 1, s));
+    uint8_t *scan_tmp = nullptr;
+    NSG_HIP(B.get(&scan_tmp, scan_bytes));
+    NSG_HIP(hipcub::DeviceScan::ExclusiveSum(scan_tmp, scan_bytes, incnt, inoff, N + 1, s));
+    hipLaunchKernelGGL(nsg_rev_scatter_kernel, dim3(1024), dim3(256), 0, s, a.pr_ids, a.pr_dist, a.pr_cnt, N, R,
+                       inoff, incur, insrc, indist);
+    NSG_HIP(hipGetLastError());
+    a.inoff = inoff;
+    a.insrc = insrc;
+    a.indist = indist;
+    NSG_HIP(B.get(&a.out_ids, (size_t)N * R));
+    NSG_HIP(B.get(&a.out_dist, (size_t)N * R));
+    NSG_HIP(B.get(&a.out_cnt, N));
+    const uint32_t sw = pow2_ge(2 * R, 64);  // sort width of new sources; own keys sort in sw / 2 >= R
+    NsgArgs ai = a;
+    ai.C = sw;
+    ai.lds_wave = (uint32_t)((((size_t)NSG_TCAP * 8 + 64 * 8 * 2 + (size_t)R * 8 + (size_t)(sw / 2) * 8 +
+                               64 * 4 * 2 + (size_t)sw * 4 + NSG_TCAP) + 15) & ~(size_t)15);
+    NSG_HIP(dispatch_insert(cpl, ai, s));
+    NSG_HIP(hipStreamSynchronize(s));
+    return WV_OK;
+}
+
+// GenRandom (include/weavess/util.h:13-27) of `size` distinct ids below N on
+// counter stream st, as the device and the CPU restatement draw it.
+void gen_random_host(uint64_t st, uint32_t size, uint32_t N, uint32_t *out) {
+    std::vector<uint32_t> v(size);
+    for (uint32_t i = 0; i < size; ++i) v[i] = draw(st, i) % (N - size);
+    std::sort(v.begin(), v.end());
+    for (uint32_t i = 1; i < size; ++i)
+        if (v[i] <= v[i - 1]) v[i] = v[i - 1] + 1;
+    const uint32_t off = draw(st, size) % N;
+    for (uint32_t i = 0; i < size; ++i) out[i] = (uint32_t)(((uint64_t)v[i] + off) % N);
+}
+
 }  // namespace
 
 int nsg_build(wv_index *ix, const wv_nsg_params *p, wv_nsg_stats *stats, const uint32_t *knn, const uint32_t *kdeg,
@@ -685,38 +890,10 @@ int nsg_build(wv_index *ix, const wv_nsg_params *p, wv_nsg_stats *stats, const u
 
     // 3. reverse insertion
     auto t0 = std::chrono::high_resolution_clock::now();
-    uint32_t *incnt = nullptr, *inoff = nullptr, *incur = nullptr, *insrc = nullptr;
-    float *indist = nullptr;
-    NSG_HIP(B.get(&incnt, (size_t)N + 1));
-    NSG_HIP(B.get(&inoff, (size_t)N + 1));
-    NSG_HIP(B.get(&incur, N));
-    NSG_HIP(B.get(&insrc, (size_t)N * R));
-    NSG_HIP(B.get(&indist, (size_t)N * R));
-    NSG_HIP(hipMemsetAsync(incnt, 0, ((size_t)N + 1) * 4, s));
-    NSG_HIP(hipMemsetAsync(incur, 0, (size_t)N * 4, s));
-    hipLaunchKernelGGL(nsg_rev_count_kernel, dim3(1024), dim3(256), 0, s, a.pr_ids, a.pr_cnt, N, R, incnt);
-    NSG_HIP(hipGetLastError());
-    size_t scan_bytes = 0;
-    NSG_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, scan_bytes, incnt, inoff, N + 1, s));
-    uint8_t *scan_tmp = nullptr;
-    NSG_HIP(B.get(&scan_tmp, scan_bytes));
-    NSG_HIP(hipcub::DeviceScan::ExclusiveSum(scan_tmp, scan_bytes, incnt, inoff, N + 1, s));
-    hipLaunchKernelGGL(nsg_rev_scatter_kernel, dim3(1024), dim3(256), 0, s, a.pr_ids, a.pr_dist, a.pr_cnt, N, R,
-                       inoff, incur, insrc, indist);
-    NSG_HIP(hipGetLastError());
-    a.inoff = inoff;
-    a.insrc = insrc;
-    a.indist = indist;
-    NSG_HIP(B.get(&a.out_ids, (size_t)N * R));
-    NSG_HIP(B.get(&a.out_dist, (size_t)N * R));
-    NSG_HIP(B.get(&a.out_cnt, N));
-    const uint32_t sw = pow2_ge(2 * R, 64);  // sort width of new sources; own keys sort in sw / 2 >= R
-    NsgArgs ai = a;
-    ai.C = sw;
-    ai.lds_wave = (uint32_t)((((size_t)NSG_TCAP * 8 + 64 * 8 * 2 + (size_t)R * 8 + (size_t)(sw / 2) * 8 +
-                               64 * 4 * 2 + (size_t)sw * 4 + NSG_TCAP) + 15) & ~(size_t)15);
-    NSG_HIP(dispatch_insert(cpl, ai, s));
-    NSG_HIP(hipStreamSynchronize(s));
+    {
+        const int rc = reverse_insert(ix, a, cpl, B, s);
+        if (rc) return rc;
+    }
     const double insert_ms = ms_since(t0);
 
     // 4. connectivity (component_conn.cpp:14-98)
@@ -849,6 +1026,149 @@ int nsg_build(wv_index *ix, const wv_nsg_params *p, wv_nsg_stats *stats, const u
         stats->conn_ms = conn_ms;
         stats->evals = evals;
         stats->conn_edges = conn_edges;
+        stats->width = W;
+    }
+    return WV_OK;
+}
+
+int ssg_build(wv_index *ix, const wv_ssg_params *p, uint32_t *eps_out, wv_nsg_stats *stats, const uint32_t *knn,
+              const uint32_t *kdeg, uint32_t kstride) {
+    const IndexView v = index_view(ix);
+    const hipStream_t s = v.stream;
+    const uint32_t N = v.N, R = p->R, nch = v.dp / 4;
+    const int cpl = nch <= 16 ? 1 : nch <= 32 ? 2 : nch <= 64 ? 4 : nch <= 128 ? 8 : 16;
+    NSG_HIP(hipStreamSynchronize(s));
+    const auto t_all = std::chrono::high_resolution_clock::now();
+    DBuf B;
+    NsgArgs a{};
+    a.base = v.base;
+    a.N = N;
+    a.dp = v.dp;
+    a.nch = nch;
+    a.knn = knn;
+    a.kdeg = kdeg;
+    a.kstride = kstride;
+    a.R = R;
+    a.alpha = 1.0f;
+    a.angle = 1;
+    // component_refine.cpp:638-639: cos(A / 180 * pi), A in FP32, pi and the
+    // cosine in FP64, rounded to FP32
+    a.thr = (float)std::cos(p->angle / 180 * std::acos(-1.0));
+    a.L = p->L;
+    a.HS = pow2_ge(2 * (p->L + 1), 64);
+    a.CAPC = p->L + kstride;
+    a.lds_wave = (uint32_t)((((size_t)a.CAPC * 8 + 64 * 8 * 2 + (size_t)R * 8 + 64 * 4 * 2 + (size_t)a.HS * 4 +
+                              a.CAPC) + 15) & ~(size_t)15);
+    if ((size_t)NWPB * a.lds_wave > 160 * 1024)
+        return index_set_error(ix, WV_ENOTSUP, "L_refine + kNN degree too large for LDS");
+    NSG_HIP(B.get(&a.pr_ids, (size_t)N * R));
+    NSG_HIP(B.get(&a.pr_dist, (size_t)N * R));
+    NSG_HIP(B.get(&a.pr_cnt, N));
+    switch (cpl) {
+    case 1: NSG_HIP(launch_ssg<1>(a, s)); break;
+    case 2: NSG_HIP(launch_ssg<2>(a, s)); break;
+    case 4: NSG_HIP(launch_ssg<4>(a, s)); break;
+    case 8: NSG_HIP(launch_ssg<8>(a, s)); break;
+    default: NSG_HIP(launch_ssg<16>(a, s)); break;
+    }
+    NSG_HIP(hipStreamSynchronize(s));
+    const double prune_ms = ms_since(t_all);
+    auto t0 = std::chrono::high_resolution_clock::now();
+    int rc = reverse_insert(ix, a, cpl, B, s);
+    if (rc) return rc;
+    const double insert_ms = ms_since(t0);
+
+    // ComponentConnSSGDFS (component_conn.cpp:169-226)
+    t0 = std::chrono::high_resolution_clock::now();
+    std::vector<uint32_t> gid((size_t)N * R), gcnt(N);
+    std::vector<float> gdd((size_t)N * R);
+    NSG_HIP(hipMemcpy(gid.data(), a.out_ids, gid.size() * 4, hipMemcpyDeviceToHost));
+    NSG_HIP(hipMemcpy(gdd.data(), a.out_dist, gdd.size() * 4, hipMemcpyDeviceToHost));
+    NSG_HIP(hipMemcpy(gcnt.data(), a.out_cnt, (size_t)N * 4, hipMemcpyDeviceToHost));
+    std::vector<std::vector<uint32_t>> extra(N);
+    std::vector<uint32_t> lu, lv;  // links lu[i] -> lv[i]
+    std::vector<uint32_t> roots(p->n_try);
+    gen_random_host(stream_of(p->seed, TAG_SSGE, 0), p->n_try, N, roots.data());
+    std::vector<uint8_t> flag(N);
+    std::vector<uint32_t> q;
+    for (uint32_t r = 0; r < p->n_try; ++r) {
+        std::fill(flag.begin(), flag.end(), 0);
+        std::set<uint32_t> spare;  // reached nodes with fewer than R neighbours
+        auto reach = [&](uint32_t x) {
+            flag[x] = 1;
+            q.push_back(x);
+            if (gcnt[x] + extra[x].size() < R) spare.insert(x);
+        };
+        q.clear();
+        reach(roots[r]);
+        size_t head = 0;
+        uint32_t low = 0;  // every id below `low` is reached
+        for (;;) {
+            while (head < q.size()) {
+                const uint32_t u = q[head++];
+                for (uint32_t j = 0; j < gcnt[u]; ++j) {
+                    const uint32_t w = gid[(size_t)u * R + j];
+                    if (!flag[w]) reach(w);
+                }
+                for (uint32_t w : extra[u])
+                    if (!flag[w]) reach(w);
+            }
+            while (low < N && flag[low]) ++low;
+            if (low == N) break;
+            const uint32_t u = low;
+            if (!spare.empty()) {
+                const uint32_t j = *spare.begin();
+                extra[j].push_back(u);
+                lu.push_back(j);
+                lv.push_back(u);
+                if (gcnt[j] + extra[j].size() >= R) spare.erase(spare.begin());
+            }
+            reach(u);
+        }
+    }
+    // the links' distances, the build-order way on the device
+    std::vector<uint64_t> lk(lu.size());
+    if (!lu.empty()) {
+        uint32_t *d_u = nullptr, *d_v = nullptr;
+        uint64_t *d_k = nullptr;
+        NSG_HIP(B.get(&d_u, lu.size()));
+        NSG_HIP(B.get(&d_v, lv.size()));
+        NSG_HIP(B.get(&d_k, lu.size()));
+        NSG_HIP(hipMemcpy(d_u, lu.data(), lu.size() * 4, hipMemcpyHostToDevice));
+        NSG_HIP(hipMemcpy(d_v, lv.data(), lv.size() * 4, hipMemcpyHostToDevice));
+        NSG_HIP(pair_keys(cpl, v.base, v.dp, nch, d_u, d_v, (uint32_t)lu.size(), d_k, s));
+        NSG_HIP(hipStreamSynchronize(s));
+        NSG_HIP(hipMemcpy(lk.data(), d_k, lk.size() * 8, hipMemcpyDeviceToHost));
+    }
+    std::vector<std::vector<float>> extra_d(N);
+    for (size_t i = 0; i < lu.size(); ++i) extra_d[lu[i]].push_back(key_dist(lk[i]));
+    uint32_t W = 1;
+    for (uint32_t i = 0; i < N; ++i) W = std::max<uint32_t>(W, gcnt[i] + (uint32_t)extra[i].size());
+    std::vector<uint32_t> fid((size_t)N * W, ID_NONE), fdeg(N);
+    std::vector<float> fdist((size_t)N * W, -1.0f);
+    for (uint32_t i = 0; i < N; ++i) {
+        uint32_t c = 0;
+        for (uint32_t j = 0; j < gcnt[i]; ++j, ++c) {
+            fid[(size_t)i * W + c] = gid[(size_t)i * R + j];
+            fdist[(size_t)i * W + c] = gdd[(size_t)i * R + j];
+        }
+        for (size_t j = 0; j < extra[i].size(); ++j, ++c) {
+            fid[(size_t)i * W + c] = extra[i][j];
+            fdist[(size_t)i * W + c] = extra_d[i][j];
+        }
+        fdeg[i] = c;
+    }
+    const double conn_ms = ms_since(t0);
+    rc = index_upload_graph(ix, fid.data(), fdist.data(), fdeg.data(), W);
+    if (rc) return rc;
+    if (eps_out) std::memcpy(eps_out, roots.data(), (size_t)p->n_try * 4);
+    if (stats) {
+        *stats = wv_nsg_stats{};
+        stats->total_ms = ms_since(t_all);
+        stats->prune_ms = prune_ms;
+        stats->insert_ms = insert_ms;
+        stats->conn_ms = conn_ms;
+        stats->conn_edges = (uint32_t)lu.size();
         stats->width = W;
     }
     return WV_OK;

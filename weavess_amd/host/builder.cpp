@@ -51,6 +51,9 @@ IndexBuilder *IndexBuilder::refine(TYPE type, bool) {
         a.reset(new ComponentRefineNNDescentHIP(index_, WV_DTYPE_BF16));
     } else if (type == REFINE_NN_DESCENT_HIP_FP8) {
         a.reset(new ComponentRefineNNDescentHIP(index_, WV_DTYPE_FP8));
+    } else if (type == REFINE_SSG) {
+        std::cout << "__REFINE : SSG (HIP)__" << std::endl;  // src/builder.cpp:128-130
+        a.reset(new ComponentRefineSSGHIP(index_));
     } else if (type == REFINE_DPG) {
         std::cout << "__REFINE : DPG (HIP)__" << std::endl;  // src/builder.cpp:131-133
         a.reset(new ComponentRefineDPGHIP(index_));

@@ -241,6 +241,23 @@ void ComponentRefineNSGHIP::RefineInner() {
     download_final_graph(index, stats_.width);
 }
 
+void ComponentRefineSSGHIP::RefineInner() {
+    auto &P = index->getParam();
+    wv_ssg_params p{};
+    p.R = P.get<unsigned>("R_refine");
+    p.L = P.get<unsigned>("L_refine");
+    p.angle = P.get<float>("A");
+    p.n_try = P.get<unsigned>("n_try");
+    p.seed = index->seed();
+    if (index->getFinalGraph().size() != index->getBaseLen())
+        throw std::invalid_argument("SSG refines a kNN graph: run init + refine first");
+    upload_final_graph(index);
+    std::cout << "__CANDIDATE : PROPAGATION 2__ __PRUNE : NSSG__ __CONN : DFS_Expand__" << std::endl;
+    index->eps_.assign(p.n_try, 0);
+    index->check(wv_index_build_ssg(index->deviceIndex(), &p, index->eps_.data(), &stats_));
+    download_final_graph(index, stats_.width);
+}
+
 void ComponentRefineDPGHIP::RefineInner() {
     const unsigned range = index->getParam().get<unsigned>("K") / 2;  // SetConfigs, component_refine.cpp:756-764
     if (index->getFinalGraph().size() != index->getBaseLen())

@@ -118,6 +118,18 @@ private:
     wv_nsg_stats stats_{};
 };
 
+// ComponentRefineSSG (src/component_refine.cpp:550-727) on the GPU: reads
+// "L_refine", "R_refine", "A", "n_try" (SetConfigs, :584-591); writes the
+// SSG graph to FinalGraph and its connectivity roots to eps_.
+class ComponentRefineSSGHIP : public ComponentRefine {
+public:
+    explicit ComponentRefineSSGHIP(Index *index) : ComponentRefine(index) {}
+    void RefineInner() override;
+
+private:
+    wv_nsg_stats stats_{};
+};
+
 class ComponentSearchEntry : public Component {
 public:
     explicit ComponentSearchEntry(Index *index) : Component(index) {}

@@ -25,6 +25,7 @@ enum TYPE {
     REFINE_NSG,             // NSG refine of the kNN graph on the GPU
     REFINE_VAMANA,          // one Vamana pass on the GPU (alpha 1, then 2 on the next call)
     REFINE_DPG,             // DPG refine of the kNN graph on the GPU
+    REFINE_SSG,             // SSG refine of the kNN graph on the GPU (sets eps_)
 
     SEARCH_ENTRY_RAND,      // L random entries per query (GenRandom)
     SEARCH_ENTRY_CENTROID,  // the entry point's neighbours, then distinct random ids

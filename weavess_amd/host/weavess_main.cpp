@@ -1,8 +1,8 @@
-// weavess_main.cpp -- KGraph, EFANNA, NSG, Vamana and DPG drivers of the
-// pipeline (test/main.cpp:6-34, :66-95, :128-157, :159-188, :190-217 and
-// :480-546 of the reference, with the HIP stages).
+// weavess_main.cpp -- KGraph, EFANNA, NSG, SSG, Vamana and DPG drivers of
+// the pipeline (test/main.cpp:6-34, :66-95, :97-126, :128-157, :159-188,
+// :190-217 and :480-546 of the reference, with the HIP stages).
 //
-//   weavess_main kgraph|efanna|nsg|vamana|dpg <prefix> build|search|info [L_search]
+//   weavess_main kgraph|efanna|nsg|ssg|vamana|dpg <prefix> build|search|info [L_search]
 //
 // <prefix>_base.fvecs, <prefix>_query.fvecs and <prefix>_groundtruth.ivecs
 // are read; the graph file is <prefix>.kgraph.graph, <prefix>.efanna.graph
@@ -22,7 +22,10 @@
 // then the NSG refine; Vamana builds run INIT_RAND and two Vamana passes
 // (alpha 1, then 2).  Both are written in the NSG layout, and their search
 // starts from the centroid entry point stored in the file.  DPG builds run
-// NN-Descent, then the DPG refine, and search from random entries.
+// NN-Descent, then the DPG refine, and search from random entries.  SSG
+// builds run NN-Descent, then the SSG refine, and are written in the SSG
+// layout with their roots eps_, which their search starts from (WEAVESS_A,
+// WEAVESS_N_TRY override A = 60 and n_try = 10).
 #include <cstdlib>
 #include <cstring>
 #include <iostream>
@@ -87,6 +90,17 @@ const std::map<std::string, std::pair<unsigned, unsigned>> &table_vamana() {
     return t;
 }
 
+// SSG's NN-Descent K, L, ITER, S, R and L_refine, R_refine per dataset
+// (include/weavess/exp_data.h:159-214; A = 60, n_try = 10 for all).
+const std::map<std::string, NS> &table_ssg() {
+    static const std::map<std::string, NS> t = {
+        {"siftsmall", {{25, 50, 6, 10, 100}, 100, 50, 0}},    {"sift1M", {{400, 420, 12, 20, 100}, 50, 20, 0}},
+        {"gist", {{300, 330, 12, 20, 200}, 200, 40, 0}},      {"glove-100", {{300, 320, 12, 10, 200}, 150, 30, 0}},
+        {"c1", {{50, 70, 10, 10, 100}, 50, 20, 0}},           {"c2", {{64, 64, 12, 16, 64}, 100, 32, 0}},
+    };
+    return t;
+}
+
 // DPG's NN-Descent K, L, ITER, S, R per dataset (include/weavess/exp_data.h:
 // 215-264); the DPG refine keeps K / 2 a node.
 const std::map<std::string, KG> &table_dpg() {
@@ -108,12 +122,13 @@ unsigned env_u(const char *name, unsigned dflt) {
 int main(int argc, char **argv) {
     const std::string alg = argc > 1 ? argv[1] : "";
     if (argc < 4 || argc > 5 ||
-        (alg != "kgraph" && alg != "efanna" && alg != "nsg" && alg != "vamana" && alg != "dpg")) {
-        std::cout << "./weavess_main kgraph|efanna|nsg|vamana|dpg dataset_prefix build|search|info [L_search]"
+        (alg != "kgraph" && alg != "efanna" && alg != "nsg" && alg != "vamana" && alg != "dpg" && alg != "ssg")) {
+        std::cout << "./weavess_main kgraph|efanna|nsg|vamana|dpg|ssg dataset_prefix build|search|info [L_search]"
                   << std::endl;
         return 1;
     }
-    const bool efanna = alg == "efanna", nsg = alg == "nsg", vamana = alg == "vamana", dpg = alg == "dpg";
+    const bool efanna = alg == "efanna", nsg = alg == "nsg", vamana = alg == "vamana", dpg = alg == "dpg",
+               ssg = alg == "ssg";
     const std::string prefix(argv[2]), exc(argv[3]);
     std::string name = prefix.substr(prefix.find_last_of('/') + 1);
     const auto &tab = efanna ? table_efanna() : table();
@@ -121,6 +136,10 @@ int main(int argc, char **argv) {
     NS ns = table_nsg().count(name) ? table_nsg().at(name) : table_nsg().at("c1");
     if (nsg) kg = ns.kg;
     if (dpg) kg = table_dpg().count(name) ? table_dpg().at(name) : table_dpg().at("c1");
+    if (ssg) {
+        ns = table_ssg().count(name) ? table_ssg().at(name) : table_ssg().at("c1");
+        kg = ns.kg;
+    }
     if (vamana) {
         const auto vm = table_vamana().count(name) ? table_vamana().at(name) : table_vamana().at("c1");
         ns.L_refine = vm.first;
@@ -140,12 +159,15 @@ int main(int argc, char **argv) {
     P.set<unsigned>("L_refine", env_u("WEAVESS_L_REFINE", ns.L_refine));
     P.set<unsigned>("R_refine", env_u("WEAVESS_R_REFINE", ns.R_refine));
     P.set<unsigned>("C_refine", env_u("WEAVESS_C_REFINE", ns.C));
+    P.set<float>("A", (float)env_u("WEAVESS_A", 60));  // include/weavess/exp_data.h:211-212
+    P.set<unsigned>("n_try", env_u("WEAVESS_N_TRY", 10));
     const char *gf = std::getenv("WEAVESS_GRAPH");
     const std::string graph = gf ? gf
                                  : prefix + (efanna   ? ".efanna.graph"
                                              : nsg    ? ".nsg.graph"
                                              : vamana ? ".vamana.graph"
                                              : dpg    ? ".dpg.graph"
+                                             : ssg    ? ".ssg.graph"
                                                       : ".kgraph.graph");
     const std::string base = prefix + "_base.fvecs", query = prefix + "_query.fvecs",
                       ground = prefix + "_groundtruth.ivecs";
@@ -158,18 +180,24 @@ int main(int argc, char **argv) {
                   : dtype == "fp8" ? weavess::REFINE_NN_DESCENT_HIP_FP8
                                    : weavess::REFINE_NN_DESCENT_HIP);
     const std::string ent =
-        std::getenv("WEAVESS_ENTRY") ? std::getenv("WEAVESS_ENTRY") : (nsg || vamana) ? "centroid" : "rand";
+        std::getenv("WEAVESS_ENTRY") ? std::getenv("WEAVESS_ENTRY")
+        : (nsg || vamana)            ? "centroid"
+        : ssg                        ? "subcentroid"
+                                     : "rand";
     const weavess::TYPE entry = ent == "centroid"      ? weavess::SEARCH_ENTRY_CENTROID
                                 : ent == "subcentroid" ? weavess::SEARCH_ENTRY_SUB_CENTROID
                                                        : weavess::SEARCH_ENTRY_RAND;
     const std::string fmt =
-        std::getenv("WEAVESS_FORMAT") ? std::getenv("WEAVESS_FORMAT") : (nsg || vamana) ? "nsg" : "kgraph";
+        std::getenv("WEAVESS_FORMAT") ? std::getenv("WEAVESS_FORMAT")
+        : (nsg || vamana)             ? "nsg"
+        : ssg                         ? "ssg"
+                                      : "kgraph";
     const weavess::TYPE gtype = fmt == "nsg" ? weavess::INDEX_NSG : fmt == "ssg" ? weavess::INDEX_SSG
                                                                                  : weavess::INDEX_KGRAPH;
     try {
         weavess::IndexBuilder builder(8);
         if (exc == "build") {
-            if (gtype == weavess::INDEX_SSG) throw std::invalid_argument("SSG graphs are not built here");
+            if (gtype == weavess::INDEX_SSG && !ssg) throw std::invalid_argument("SSG files are written by ssg builds");
             builder.load(base.c_str(), query.c_str(), ground.c_str(), P)
                 ->init(vamana ? weavess::INIT_RAND : weavess::INIT_RANDOM);
             std::cout << "Init cost: " << builder.GetBuildTime().count() << std::endl;
@@ -177,9 +205,9 @@ int main(int argc, char **argv) {
                 builder.refine(weavess::REFINE_VAMANA)->refine(weavess::REFINE_VAMANA);
             else
                 builder.refine(refine);
-            if (nsg || dpg) {  // test/main.cpp:76-81, :137-142
+            if (nsg || dpg || ssg) {  // test/main.cpp:76-81, :106-112, :137-142
                 std::cout << "Init cost: " << builder.GetBuildTime().count() << std::endl;
-                builder.refine(nsg ? weavess::REFINE_NSG : weavess::REFINE_DPG);
+                builder.refine(nsg ? weavess::REFINE_NSG : dpg ? weavess::REFINE_DPG : weavess::REFINE_SSG);
             } else if (gtype == weavess::INDEX_NSG) {
                 weavess::ComponentRefineEntryCentroidHIP(builder.index()).EntryInner();
             }
