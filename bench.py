@@ -281,12 +281,124 @@ def run_c4(args, ctx, Index, datagen):
     ix.close()
 
 
+def run_refine(args, ctx, Index, datagen):
+    """Refiners downstream of NN-Descent on the 1M x 128 mixture of the
+    NN-Descent leg: NSG, SSG, DPG and Vamana (two passes from a random
+    graph), each searched from its own entries; EFANNA's NN-Descent against
+    KGraph's; the CPU restatement of NSG timed on a 20k-node sample."""
+    from weavess_amd.entries import centroid_ep, centroid_entries, subcentroid_entries
+    w = NND_LEG
+    base, q = getattr(datagen, w["gen"])(n_query=10_000)
+    N, d = base.shape
+    ix = Index(ctx, base)
+    gt, _, _ = ctx.exact_knn(base, q, 10)
+    ctl = 10_000
+    gtg, _, _ = ctx.exact_knn(base, base[:ctl], 10, self_offset=0)
+    kst = ix.build_nndescent(w["K"], w["L"], w["S"], w["R"], w["iters"], seed=1, dtype="bf16",
+                             cand_slots=w["cand_slots"])
+    kid, _, kdeg = ix.get_graph()
+    ep = centroid_ep(ctx, base)
+    log(f"refine: kNN graph {kst['total_ms']:.1f} ms, ep {ep}")
+    row_bytes = ((d + 3) // 4) * 16
+
+    def sweep(g, dg, entries, Ls):
+        ix.set_graph(g, dg)
+        out = []
+        for L in Ls:
+            if entries == "rand":
+                ids, _, ev, _, ss = ix.search(q, L, 10, seed=0)
+            else:
+                ent = entries(g, dg, L)
+                ids, _, ev, _, ss = ix.search(q, L, 10, entries=ent)
+            out.append(dict(L=L, recall=round(recall_at_k(ids, gt, 10), 4),
+                            qps=round(len(q) / (ss["kernel_ms"] / 1e3)), dist_evals=round(float(ev.mean()))))
+        return out
+
+    def rounded(st):
+        return {k: (round(v, 1) if isinstance(v, float) else int(v)) for k, v in st.items()}
+
+    cent = lambda g, dg, L: centroid_entries(ep, g, dg, len(q), L, N)
+    res = {"knn_graph": dict(build_s=round(kst["total_ms"] / 1e3, 4), search=sweep(kid, kdeg, "rand", [80, 160]))}
+    # NSG (L_refine 150, R 30, C 400: the reference's sift1M setting)
+    ix.set_graph(kid, kdeg)
+    st = ix.build_nsg(150, 30, 400, ep, seed=1)
+    g, _, dg = ix.get_graph()
+    st = rounded(st)
+    st["candidate_search_gather_GBps"] = round(st["evals"] * row_bytes / (st["search_ms"] / 1e3) / 1e9, 1)
+    res["nsg"] = dict(params="L_refine=150 R=30 C=400", stats=st, deg_mean=round(float(dg.mean()), 2),
+                      search=sweep(g, dg, cent, [40, 80, 160]))
+    log(f"refine nsg {st}")
+    # SSG (L_refine 100, R 32, A 60, n_try 10)
+    ix.set_graph(kid, kdeg)
+    st, eps = ix.build_ssg(100, 32, 60.0, 10, seed=1)
+    g, _, dg = ix.get_graph()
+    res["ssg"] = dict(params="L_refine=100 R=32 A=60 n_try=10", stats=rounded(st), deg_mean=round(float(dg.mean()), 2),
+                      search=sweep(g, dg, lambda g_, d_, L: subcentroid_entries(eps, len(q), L, N), [40, 80, 160]))
+    log(f"refine ssg {res['ssg']['stats']}")
+    # DPG (K / 2 = 32 a node, mirrored)
+    ix.set_graph(kid, kdeg)
+    st = ix.build_dpg(w["K"] // 2)
+    g, _, dg = ix.get_graph()
+    res["dpg"] = dict(params=f"range={w['K'] // 2}", stats=rounded(st), deg_mean=round(float(dg.mean()), 2),
+                      search=sweep(g, dg, "rand", [40, 80, 160]))
+    log(f"refine dpg {res['dpg']['stats']}")
+    # Vamana: INIT_RAND (32 random neighbours), passes alpha 1 and 2 (L_refine 70, R 32)
+    t0 = time.time()
+    ix.build_nndescent(32, 32, 32, 0, 0, seed=1)
+    sts = [rounded(ix.build_vamana(70, 32, 1024, ep, a, seed=1)) for a in (1.0, 2.0)]
+    g, _, dg = ix.get_graph()
+    res["vamana"] = dict(params="INIT_RAND 32, L_refine=70 R=32 C=1024, alpha 1 then 2", passes=sts,
+                         wall_s=round(time.time() - t0, 3), deg_mean=round(float(dg.mean()), 2),
+                         search=sweep(g, dg, cent, [40, 80, 160]))
+    log(f"refine vamana {sts}")
+    # EFANNA vs KGraph NN-Descent (BF16, same K, L, S, R, slots)
+    ef = {}
+    for name, fn in (("kgraph", ix.build_nndescent), ("efanna", ix.build_efanna)):
+        for it in (10, 11, 12):
+            st = fn(w["K"], w["L"], w["S"], w["R"], it, seed=1, dtype="bf16", cand_slots=w["cand_slots"])
+            gg, _, gd = ix.get_graph()
+            ef[f"{name}_{it}"] = dict(build_s=round(st["total_ms"] / 1e3, 4),
+                                      graph_recall=round(graph_recall(gg[:ctl], gd[:ctl], gtg), 4))
+    res["efanna_vs_kgraph"] = ef
+    log(f"refine efanna {ef}")
+    ix.close()
+    cpu = None
+    if not args.no_cpu:
+        import oracle as O
+        from weavess_amd.devmem import host_cores
+        sub = base[:20_000]
+        ix2 = Index(ctx, sub)
+        ix2.build_nndescent(w["K"], w["L"], w["S"], w["R"], w["iters"], seed=1, dtype="bf16",
+                            cand_slots=w["cand_slots"])
+        k2, _, d2 = ix2.get_graph()
+        ep2 = centroid_ep(ctx, sub)
+        st2 = ix2.build_nsg(150, 30, 400, ep2, seed=1)
+        g2, _, gd2 = ix2.get_graph()
+        ix2.close()
+        t1 = time.time()
+        oi, _, odeg, _ = O.nsg(sub, k2, d2, 150, 30, 400, ep2, seed=1)
+        cpu_s = time.time() - t1
+        same = bool(np.array_equal(gd2, odeg) and all(g2[i, :gd2[i]].tolist() == oi[i, :odeg[i]].tolist()
+                                                      for i in range(len(sub))))
+        cpu = dict(value=round(cpu_s, 3), unit="s (NSG refine of 20,000 nodes)", cores=host_cores(), kind="port",
+                   sample="first 20,000 points, their BF16 NN-Descent graph, L_refine=150 R=30 C=400",
+                   gpu_s=round(st2["total_ms"] / 1e3, 4), graph_bit_exact_vs_gpu=same)
+        log(f"refine cpu baseline {cpu}")
+    out = {"metric": "refiners of the NN-Descent graph: build seconds and search recall/QPS per graph",
+           "value": res["nsg"]["stats"]["total_ms"] / 1e3, "unit": "s (NSG refine, 1M x 128)", "n_gpus": 1,
+           "higher_is_better": False, "data": f"synthetic, seeded ({w['gen']}), {N}x{d}, 10000 queries",
+           "config": {"workload": "refine: 1M x 128 Zipf(1.1) GMM; BF16 NN-Descent K=64 L=96 S=16 R=96 iters=12; "
+                                  "refiners on the GPU; greedy search k=10"},
+           "results": res, "cpu_baseline": cpu}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="c5", choices=sorted(WORKLOADS) + ["c3", "c4"])
+    ap.add_argument("--workload", default="c5", choices=sorted(WORKLOADS) + ["c3", "c4", "refine"])
     ap.add_argument("--target", type=float, default=0.95)
     ap.add_argument("--no-cpu", action="store_true", help="skip the host-core baseline")
     ap.add_argument("--no-nnd-leg", action="store_true", help="skip the 1M x 128 NN-Descent leg")
@@ -300,8 +412,8 @@ def main():
     from weavess_amd.devmem import DeviceArray, host_cores, pad_rows, synchronize
 
     ctx = Context(0)
-    if args.workload in ("c3", "c4"):
-        (run_c3 if args.workload == "c3" else run_c4)(args, ctx, Index, datagen)
+    if args.workload in ("c3", "c4", "refine"):
+        {"c3": run_c3, "c4": run_c4, "refine": run_refine}[args.workload](args, ctx, Index, datagen)
         ctx.close()
         return
     w = WORKLOADS[args.workload]

@@ -21,7 +21,6 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
-#include <set>
 #include <vector>
 
 #include "wv_device.hpp"
@@ -1090,14 +1089,14 @@ int ssg_build(wv_index *ix, const wv_ssg_params *p, uint32_t *eps_out, wv_nsg_st
     std::vector<uint32_t> roots(p->n_try);
     gen_random_host(stream_of(p->seed, TAG_SSGE, 0), p->n_try, N, roots.data());
     std::vector<uint8_t> flag(N);
-    std::vector<uint32_t> q;
+    std::vector<uint32_t> q, dgr(gcnt);  // dgr: current degree (links included)
     for (uint32_t r = 0; r < p->n_try; ++r) {
         std::fill(flag.begin(), flag.end(), 0);
-        std::set<uint32_t> spare;  // reached nodes with fewer than R neighbours
+        uint32_t best = N;  // the smallest reached node with fewer than R neighbours
         auto reach = [&](uint32_t x) {
             flag[x] = 1;
             q.push_back(x);
-            if (gcnt[x] + extra[x].size() < R) spare.insert(x);
+            if (dgr[x] < R && x < best) best = x;
         };
         q.clear();
         reach(roots[r]);
@@ -1116,12 +1115,16 @@ int ssg_build(wv_index *ix, const wv_ssg_params *p, uint32_t *eps_out, wv_nsg_st
             while (low < N && flag[low]) ++low;
             if (low == N) break;
             const uint32_t u = low;
-            if (!spare.empty()) {
-                const uint32_t j = *spare.begin();
+            if (best < N) {
+                const uint32_t j = best;
                 extra[j].push_back(u);
                 lu.push_back(j);
                 lv.push_back(u);
-                if (gcnt[j] + extra[j].size() >= R) spare.erase(spare.begin());
+                if (++dgr[j] >= R) {  // the next reached node with room: every one is above j
+                    uint32_t b = j + 1;
+                    while (b < N && !(flag[b] && dgr[b] < R)) ++b;
+                    best = b;
+                }
             }
             reach(u);
         }
