@@ -170,6 +170,74 @@ __device__ int occlude(const float *base, uint32_t dp, uint32_t nch, const uint6
     return cnt;
 }
 
+// The occlusion rule of `occlude` evaluated block by block: candidates in
+// blocks of 64; at a block's start every key kept so far passes once over
+// the block's live candidates, then the block's candidates are taken in
+// order, each kept one passing over the rest of the block.  A candidate is
+// kept or dropped by the same keys as in `occlude`, so the lists are equal;
+// the work stops at the block where the R-th key is kept, which pays when
+// the rule drops few candidates (Vamana's alpha > 1).
+template <int CPL>
+__device__ int occlude_blocked(const float *base, uint32_t dp, uint32_t nch, const uint64_t *cand, int m,
+                               uint8_t *st, int R, float alpha, bool fill, uint32_t *cid, uint32_t *cpos,
+                               uint64_t *ckey, uint64_t *res) {
+    const int lane = lane_id();
+    int cnt = 0;
+    // drop the block's live candidates that kept key k occludes
+    auto pass = [&](uint64_t k, int from, int bend) {
+        float4 qv[CPL];
+        load_row<CPL>(base, dp, nch, key_id(k), qv);
+        const int p = from + lane;
+        const bool ok = p < bend && st[p] == ST_LIVE;
+        const uint64_t b = __ballot(ok);
+        if (!b) return;
+        const int mm = __popcll(b);
+        if (ok) {
+            cid[lanes_below(b)] = key_id(cand[p]);
+            cpos[lanes_below(b)] = (uint32_t)p;
+        }
+        wave_sync();
+        rows_dist<CPL, rows_g<CPL>()>(base, dp, nch, qv, cid, mm, ckey);
+        wave_sync();
+        if (lane < mm) {
+            const uint32_t pp = cpos[lane];
+            if (alpha * key_dist(ckey[lane]) < key_dist(cand[pp])) st[pp] = ST_DROP;
+        }
+        wave_sync();
+    };
+    for (int b0 = 0; b0 < m && cnt < R; b0 += WAVE) {
+        const int bend = min(m, b0 + WAVE);
+        for (int r = 0; r < cnt; ++r) pass(res[r], b0, bend);
+        int from = b0;
+        while (cnt < R) {
+            const uint64_t b = __ballot(from + lane < bend && st[from + lane] == ST_LIVE);
+            if (!b) break;
+            const int nxt = from + __ffsll((unsigned long long)b) - 1;
+            const uint64_t ki = cand[nxt];
+            if (lane == 0) {
+                res[cnt] = ki;
+                st[nxt] = ST_KEPT;
+            }
+            wave_sync();
+            ++cnt;
+            from = nxt + 1;
+            if (cnt < R && from < bend) pass(ki, from, bend);
+        }
+    }
+    wave_sync();
+    if (fill)
+        for (int t0 = 0; t0 < m && cnt < R; t0 += WAVE) {
+            const int p = t0 + lane;
+            const bool ok = p < m && st[p] == ST_DROP;
+            const uint64_t b = __ballot(ok);
+            const int pos = cnt + (int)lanes_below(b);
+            if (ok && pos < R) res[pos] = cand[p];
+            cnt = min(R, cnt + __popcll(b));
+        }
+    wave_sync();
+    return cnt;
+}
+
 // Entries of the candidate searches of nodes[q] (q-th query; nodes null:
 // node n0 + q): the h head ids (ep's list), then the ids of GenRandom(L) on
 // the node's "NSGC" stream that are not in the head (sorted copy hs), in
@@ -253,8 +321,11 @@ __global__ void __launch_bounds__(256) nsg_prune_kernel(NsgArgs a) {
     }
     for (int i = lane; i < len; i += WAVE) alive[i] = key_id(cand[i]) != n ? ST_LIVE : ST_SELF;
     wave_sync();
-    const int cnt = occlude<CPL>(a.base, a.dp, a.nch, cand, len, alive, (int)a.R, a.alpha, a.fill != 0, cid, cpos,
-                                 ckey, res);
+    const int cnt = a.alpha > 1.0f
+                        ? occlude_blocked<CPL>(a.base, a.dp, a.nch, cand, len, alive, (int)a.R, a.alpha, a.fill != 0,
+                                               cid, cpos, ckey, res)
+                        : occlude<CPL>(a.base, a.dp, a.nch, cand, len, alive, (int)a.R, a.alpha, a.fill != 0, cid,
+                                       cpos, ckey, res);
     for (int i = lane; i < cnt; i += WAVE) {
         a.pr_ids[(size_t)n * a.R + i] = key_id(res[i]);
         a.pr_dist[(size_t)n * a.R + i] = key_dist(res[i]);
@@ -369,8 +440,11 @@ __global__ void __launch_bounds__(256) nsg_insert_kernel(NsgArgs a) {
     }
     for (int i = lane; i < len; i += WAVE) alive[i] = ST_LIVE;
     wave_sync();
-    const int cnt = occlude<CPL>(a.base, a.dp, a.nch, cand, len, alive, R, a.alpha, a.fill != 0, cid, cpos, ckey,
-                                 res, a.angle != 0, a.thr);
+    const int cnt = (a.alpha > 1.0f && !a.angle)
+                        ? occlude_blocked<CPL>(a.base, a.dp, a.nch, cand, len, alive, R, a.alpha, a.fill != 0, cid,
+                                               cpos, ckey, res)
+                        : occlude<CPL>(a.base, a.dp, a.nch, cand, len, alive, R, a.alpha, a.fill != 0, cid, cpos,
+                                       ckey, res, a.angle != 0, a.thr);
     for (int i = lane; i < cnt; i += WAVE) {
         a.out_ids[(size_t)t * a.R + i] = key_id(res[i]);
         a.out_dist[(size_t)t * a.R + i] = key_dist(res[i]);
