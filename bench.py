@@ -611,6 +611,35 @@ def main():
                                "gpu_evals_over_reference_evals": round(float(gpu_ev.sum()) / float(cev.sum()), 4)}
         log(f"cpu baseline {cpu_qps:.0f} QPS on {cores} threads ({sample} queries); ids equal: "
             f"{out['cpu_baseline']['ids_bit_exact_vs_gpu']}")
+        # one thread, as the reference's KGraph search loop runs (src/builder.cpp:445-454)
+        s1 = min(sample, 32)
+        c0 = time.perf_counter()
+        oracle.search_greedy(graph, gdeg, base, queries[:s1], L_plain, k, seed=7, nthreads=1)
+        out["cpu_baseline"]["single_thread_qps"] = round(s1 / (time.perf_counter() - c0), 2)
+        out["cpu_baseline"]["single_thread_sample"] = f"{s1} workload queries"
+        # NN-Descent build on the host cores at the 10k x 128 configuration
+        # (InitRandom + NN-Descent + finalize, the reference's "Build cost")
+        w1 = WORKLOADS["c1"]
+        b1, _ = getattr(datagen, w1["gen"])(**w1["gen_kw"])
+        ctl1 = min(2000, len(b1))
+        gt1, _ = oracle.exact_knn(b1, b1[:ctl1], 10, self_offset=0, nthreads=cores)
+        ix1 = Index(ctx, b1)
+        c0 = time.perf_counter()
+        st1 = ix1.build_nndescent(w1["K"], w1["L"], w1["S"], w1["R"], w1["iters"], seed=1, dtype="f32")
+        g1, _, d1 = ix1.get_graph()
+        ix1.close()
+        c0 = time.perf_counter()
+        o1, _, od1 = oracle.nndescent(b1, w1["K"], w1["L"], w1["S"], w1["R"], w1["iters"], seed=1,
+                                      cand_slots=st1["cand_slots"], nthreads=cores)
+        cpu_nnd_s = time.perf_counter() - c0
+        out["cpu_baseline"]["nndescent_10k"] = {
+            "workload": f"10k x 128 integer GMM, K={w1['K']} L={w1['L']} S={w1['S']} R={w1['R']} iters={w1['iters']}",
+            "cpu_s": round(cpu_nnd_s, 3), "gpu_s": round(st1["total_ms"] / 1e3, 4), "cores": cores,
+            "graph_recall_cpu": round(graph_recall(o1[:ctl1], od1[:ctl1], gt1), 4),
+            "graph_recall_gpu": round(graph_recall(g1[:ctl1], d1[:ctl1], gt1), 4),
+            "graphs_bit_exact": bool(np.array_equal(o1, g1[:, :w1["K"]]))}
+        log(f"cpu baseline single thread {out['cpu_baseline']['single_thread_qps']} QPS; nndescent 10k "
+            f"{out['cpu_baseline']['nndescent_10k']}")
     # roofline of the served method's launch: algorithmic bytes = rows
     # gathered for the reference's evaluation count (the kernel's count less
     # the re-evaluations of its forgetful visited table, measured against the
