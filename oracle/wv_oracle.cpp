@@ -21,6 +21,21 @@
  *                               include/weavess/index.h:105-110                       and repeats dropped (numpy); 2S=N-1 with exact
  *                                                                                     insertion gives the exact kNN graph in one
  *                                                                                     round (the samples are joined); graph recall
+ * wvo_nsg                       src/component_refine.cpp:393-540,                     every node a candidate gives the MRNG selection
+ *                               component_candidate.cpp:10-85,                        (numpy brute force) plus reverse edges; a degree
+ *                               component_prune.cpp:20-66,                            cap re-prunes as stated; every node reachable
+ *                               component_conn.cpp:10-140
+ * wvo_nsg_conn                  component_conn.cpp:14-98                              two disjoint cliques get exactly one link, from the
+ *                                                                                     nearest reached node
+ * wvo_vamana                    src/component_refine.cpp:794-958,                     huge alpha gives the exact R-NN graph; every node
+ *                               component_prune.cpp:245-297                           a candidate gives numpy RobustPrune + fill
+ * wvo_ssg                       src/component_refine.cpp:550-727,                     threshold = cos(A); a hand-built star keeps rays
+ *                               component_candidate.cpp:89-108,                       60 degrees apart; two cliques get the links the
+ *                               component_prune.cpp:71-122,                           rule names; every node reachable from every root
+ *                               component_conn.cpp:169-226
+ * wvo_dpg                       src/component_refine.cpp:728-790,                     points on a line (closed-form occlusion counts);
+ *                               component_prune.cpp:124-186,                          rows = numpy selection + reverse edges, sorted
+ *                               component_conn.cpp:228-268
  * wvo_search_greedy             src/component_search_entry.cpp:9-30,                  complete graph / all-node entry set / L>=N give
  *                               src/component_route.cpp:9-53,                         exact kNN; hand-worked 8-node KAT for the
  *                               include/weavess/index.h:152-180                       k-pointer restart and duplicate rejection
