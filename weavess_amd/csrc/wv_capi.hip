@@ -1057,6 +1057,8 @@ int wv_index_build_nsg(wv_index *ix, const wv_nsg_params *p, wv_nsg_stats *stats
     if (p->R == 0 || p->R > 256) return fail(c, WV_ENOTSUP, "need 1 <= R <= 256 (R=%u)", p->R);
     if (p->ep >= ix->N) return fail(c, WV_EINVAL, "ep %u out of range", p->ep);
     if (ix->dp > 1024) return fail(c, WV_ENOTSUP, "dimension above 1024");
+    if ((uint64_t)ix->N * p->R >= (1ull << 31))
+        return fail(c, WV_ENOTSUP, "N x R above 2^31 edges (the edge offsets are 32-bit)");
     WV_HIP(c, hipSetDevice(c->device));
     // the kNN graph stays alive until the result replaces it
     return wv::nsg_build(ix, p, stats, ix->graph, ix->deg, ix->deg_max, wv::RefineRule{true, false, true, 1.0f});
@@ -1073,6 +1075,8 @@ int wv_index_build_ssg(wv_index *ix, const wv_ssg_params *p, uint32_t *eps, wv_n
     if (p->n_try == 0 || p->n_try >= ix->N) return fail(c, WV_EINVAL, "need 1 <= n_try < N (n_try=%u)", p->n_try);
     if (!(p->angle > 0.0f) || !(p->angle < 180.0f)) return fail(c, WV_EINVAL, "angle must be in (0, 180) degrees");
     if (ix->dp > 1024) return fail(c, WV_ENOTSUP, "dimension above 1024");
+    if ((uint64_t)ix->N * p->R >= (1ull << 31))
+        return fail(c, WV_ENOTSUP, "N x R above 2^31 edges (the edge offsets are 32-bit)");
     WV_HIP(c, hipSetDevice(c->device));
     return wv::ssg_build(ix, p, eps, stats, ix->graph, ix->deg, ix->deg_max);
 }
@@ -1084,6 +1088,8 @@ int wv_index_build_dpg(wv_index *ix, uint32_t range, wv_nsg_stats *stats) {
     if (!ix->graph) return fail(c, WV_EINVAL, "index has no graph (build or set the kNN graph first)");
     if (range == 0 || range > 256) return fail(c, WV_ENOTSUP, "need 1 <= range <= 256 (range=%u)", range);
     if (ix->dp > 1024) return fail(c, WV_ENOTSUP, "dimension above 1024");
+    if ((uint64_t)ix->N * range >= (1ull << 31))
+        return fail(c, WV_ENOTSUP, "N x range above 2^31 edges (the edge offsets are 32-bit)");
     WV_HIP(c, hipSetDevice(c->device));
     return wv::dpg_build(ix, range, stats, ix->graph, ix->deg, ix->deg_max);
 }
@@ -1101,6 +1107,8 @@ int wv_index_build_vamana(wv_index *ix, const wv_vamana_params *p, wv_nsg_stats 
     if (p->ep >= ix->N) return fail(c, WV_EINVAL, "ep %u out of range", p->ep);
     if (!(p->alpha >= 1.0f) || !(p->alpha < 1e30f)) return fail(c, WV_EINVAL, "alpha must be finite and >= 1");
     if (ix->dp > 1024) return fail(c, WV_ENOTSUP, "dimension above 1024");
+    if ((uint64_t)ix->N * p->R >= (1ull << 31))
+        return fail(c, WV_ENOTSUP, "N x R above 2^31 edges (the edge offsets are 32-bit)");
     WV_HIP(c, hipSetDevice(c->device));
     wv_nsg_params q{p->L, p->R, p->C, p->ep, p->seed};
     return wv::nsg_build(ix, &q, stats, ix->graph, ix->deg, ix->deg_max, wv::RefineRule{false, true, false, p->alpha});

@@ -1302,7 +1302,13 @@ int dpg_build(wv_index *ix, uint32_t range, wv_nsg_stats *stats, const uint32_t 
     NSG_HIP(hipMemcpyAsync(h_own.data(), a.pr_cnt, (size_t)N * 4, hipMemcpyDeviceToHost, s));
     NSG_HIP(hipStreamSynchronize(s));
     std::vector<uint32_t> h_off(N + 1, 0);
-    for (uint32_t i = 0; i < N; ++i) h_off[i + 1] = h_off[i] + h_own[i] + h_in[i];
+    uint64_t tot64 = 0;
+    for (uint32_t i = 0; i < N; ++i) {
+        tot64 += (uint64_t)h_own[i] + h_in[i];
+        if (tot64 >= (1ull << 31))
+            return index_set_error(ix, WV_ENOTSUP, "DPG union above 2^31 edges (the edge offsets are 32-bit)");
+        h_off[i + 1] = (uint32_t)tot64;
+    }
     const uint32_t total = h_off[N];
     NSG_HIP(hipMemcpyAsync(off, h_off.data(), ((size_t)N + 1) * 4, hipMemcpyHostToDevice, s));
     uint64_t *keys = nullptr, *sorted = nullptr;
