@@ -55,7 +55,6 @@ __device__ inline unsigned long long *ctr_slot(unsigned long long *ctr) {
     return ctr + (size_t)(blockIdx.x & (CTR_SLOTS - 1)) * 4;
 }
 constexpr float FP8_MAX = 448.0f;  // largest finite e4m3 magnitude
-constexpr int TILE_ST_MAX = 528;   // LDS bytes per staged tile row at most (512 + 16 pad)
 
 struct NndDev {
     const float *base;     // N x dp FP32 rows (zero padded)
@@ -602,13 +601,20 @@ __global__ void __launch_bounds__(256) nnd_rev_select_kernel(NndDev a, uint32_t 
 }
 
 // ------------------------------------------------------------------ join
+// Per element type: bytes per element, elements per K chunk (one chunk's
+// tile row is at most 512 bytes), elements per MFMA step, and the lane's
+// operand vector of one step (KSTEP / 4 elements).
 template <int DT> struct JoinT;
-template <> struct JoinT<DT_F32> { static constexpr int ES = 4, KC = 128, KSTEP = 16; };
-template <> struct JoinT<DT_BF16> { static constexpr int ES = 2, KC = 256, KSTEP = 32; };
-template <> struct JoinT<DT_FP8> { static constexpr int ES = 1, KC = 512, KSTEP = 32; };
+template <> struct JoinT<DT_F32> { static constexpr int ES = 4, KC = 128, KSTEP = 16; using V = uint4; };
+template <> struct JoinT<DT_BF16> { static constexpr int ES = 2, KC = 256, KSTEP = 32; using V = uint4; };
+template <> struct JoinT<DT_FP8> { static constexpr int ES = 1, KC = 512, KSTEP = 32; using V = uint2; };
+
+__device__ inline void vzero(uint4 &v) { v = make_uint4(0, 0, 0, 0); }
+__device__ inline void vzero(uint2 &v) { v = make_uint2(0, 0); }
 
 // Stage join rows list[r0 .. r0+16) (zero rows past `count`), elements
 // [k0, k0 + klen) with klen a multiple of KSTEP, into a 16 x tile_st tile.
+// Rows longer than one K chunk only.
 template <int DT>
 __device__ inline void stage_tile(const NndDev &a, uint8_t *tile, const uint32_t *list, int r0,
                                   int count, uint32_t k0, uint32_t klen) {
@@ -628,50 +634,8 @@ __device__ inline void stage_tile(const NndDev &a, uint8_t *tile, const uint32_t
     }
 }
 
-// The same tile in two steps for one-chunk rows (klen <= KC): issue the
-// global loads into registers (at most 8 16-byte vectors per lane), store
-// them to LDS later.  Lets the next column tile load while the current one's
-// proposals are probed.  map[it] packs the row (low 8 bits) and the byte
-// column (<< 8) of this lane's vector it, or TILE_NONE; it depends only on
-// the lane and the row width, so it is computed once per wavefront.
-constexpr int TILE_REGS = 8;
-constexpr uint32_t TILE_NONE = 0xFFFFFFFFu;
-template <int DT>
-__device__ inline void tile_map(uint32_t klen, uint32_t kdim, uint32_t map[TILE_REGS]) {
-    constexpr int ES = JoinT<DT>::ES;
-    const int lane = lane_id();
-    const int vpr = (int)(klen * ES / 16), epv = 16 / ES;
-#pragma unroll
-    for (int it = 0; it < TILE_REGS; ++it) {
-        const int v = lane + it * WAVE;
-        const int r = v / vpr, cv = v - r * vpr;
-        map[it] = v < 16 * vpr ? (uint32_t)r | ((uint32_t)cv * 16u << 8) | ((uint32_t)(cv * epv) < kdim ? 0u : 0x80u)
-                               : TILE_NONE;
-    }
-}
-// bit 7 of a map entry: the vector lies past the row's data (zero fill)
-__device__ inline void load_tile_regs(const NndDev &a, const uint32_t *list, int r0, int count,
-                                      const uint32_t map[TILE_REGS], uint4 regs[TILE_REGS]) {
-#pragma unroll
-    for (int it = 0; it < TILE_REGS; ++it) {
-        regs[it] = make_uint4(0, 0, 0, 0);
-        const uint32_t m = map[it];
-        if (m == TILE_NONE || (m & 0x80u)) continue;
-        const int r = (int)(m & 0x7Fu);
-        if (r0 + r < count)
-            regs[it] = *reinterpret_cast<const uint4 *>(a.xq + (size_t)list[r0 + r] * a.xq_stride + (m >> 8));
-    }
-}
-__device__ inline void store_tile_regs(uint8_t *tile, uint32_t st, const uint32_t map[TILE_REGS],
-                                       const uint4 regs[TILE_REGS]) {
-#pragma unroll
-    for (int it = 0; it < TILE_REGS; ++it) {
-        const uint32_t m = map[it];
-        if (m == TILE_NONE) continue;
-        *reinterpret_cast<uint4 *>(tile + (m & 0x7Fu) * st + (m >> 8)) = regs[it];
-    }
-}
-
+// The MFMA steps of one K chunk over staged tiles: lane (r = lane & 15,
+// g = lane >> 4) feeds row r's elements [KSTEP t + KSTEP/4 g, + KSTEP/4).
 template <int DT>
 __device__ inline floatx4 tile_mfma(const uint8_t *A, const uint8_t *B, uint32_t st, uint32_t klen, floatx4 acc) {
     const int lane = lane_id(), r = lane & 15, g = lane >> 4;
@@ -700,6 +664,57 @@ __device__ inline floatx4 tile_mfma(const uint8_t *A, const uint8_t *B, uint32_t
     return acc;
 }
 
+// Rows of one K chunk go to the MFMA straight from the join rows, without
+// LDS: in that operand layout each lane reads row r's own bytes, nothing a
+// neighbour loaded.  ops[t] (t < T) is lane (r, g)'s step-t vector of rows
+// list[r0 + r]; rows at or past `count` and elements at or past kdim read
+// as zero.
+template <int DT, int TS>
+__device__ inline void load_ops(const NndDev &a, const uint32_t *list, int r0, int count, int T,
+                                typename JoinT<DT>::V ops[TS]) {
+    using V = typename JoinT<DT>::V;
+    constexpr int KSTEP = JoinT<DT>::KSTEP, ES = JoinT<DT>::ES;
+    const int lane = lane_id(), r = lane & 15, g = lane >> 4;
+    const bool live = r0 + r < count;
+    const uint8_t *row = a.xq + (size_t)(live ? list[r0 + r] : 0u) * a.xq_stride;
+#pragma unroll
+    for (int t = 0; t < TS; ++t) {
+        const uint32_t e0 = (uint32_t)(KSTEP * t + (KSTEP / 4) * g);
+        vzero(ops[t]);
+        if (live && t < T && e0 < a.kdim) ops[t] = *reinterpret_cast<const V *>(row + (size_t)e0 * ES);
+    }
+}
+
+// The MFMA steps of tile_mfma, over register operands; same order.
+template <int DT, int TS>
+__device__ inline floatx4 ops_mfma(const typename JoinT<DT>::V A[TS], const typename JoinT<DT>::V B[TS], int T) {
+    floatx4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int t = 0; t < TS; ++t) {
+        if (t >= T) continue;
+        if (DT == DT_F32) {
+            float4 x, y;
+            __builtin_memcpy(&x, &A[t], 16);
+            __builtin_memcpy(&y, &B[t], 16);
+            acc = __builtin_amdgcn_mfma_f32_16x16x4f32(x.x, y.x, acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_16x16x4f32(x.y, y.y, acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_16x16x4f32(x.z, y.z, acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_16x16x4f32(x.w, y.w, acc, 0, 0, 0);
+        } else if (DT == DT_BF16) {
+            bf16x8 x, y;
+            __builtin_memcpy(&x, &A[t], 16);
+            __builtin_memcpy(&y, &B[t], 16);
+            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(x, y, acc, 0, 0, 0);
+        } else {
+            long x, y;
+            __builtin_memcpy(&x, &A[t], 8);
+            __builtin_memcpy(&y, &B[t], 8);
+            acc = __builtin_amdgcn_mfma_f32_16x16x32_fp8_fp8(x, y, acc, 0, 0, 0);
+        }
+    }
+    return acc;
+}
+
 // Is `id` in target t's pool?  Probing of t's pool-id hash from slot `i`,
 // whose entry `e` was already read.
 __device__ inline bool pooled_from(const NndDev &a, uint32_t t, uint32_t id, uint32_t i, uint32_t e) {
@@ -713,24 +728,110 @@ __device__ inline bool pooled_from(const NndDev &a, uint32_t t, uint32_t id, uin
     }
 }
 
+// Proposals that pass the target's threshold wait in a per-wavefront LDS
+// ring and are resolved 64 at a time, one per lane: the pool-id hash probe,
+// then atomicMin on the candidate slot.  Only a few percent pass after the
+// first rounds, so the wavefront probes full rows of proposals instead of
+// eight sparse ones per tile.  An entry holds the distance's bits and the
+// list positions of the target (low 16 bits) and of the proposed id.
+constexpr uint32_t JQ_CAP = 256;  // >= 63 waiting + 128 pushed per pair index
+constexpr uint32_t JQ_BYTES = JQ_CAP * 8;
+struct JoinQ {
+    uint2 *q;
+    uint32_t head, n;  // wavefront-uniform
+};
+
+__device__ inline void jq_push(JoinQ &Q, bool v, float d, uint32_t ti, uint32_t si) {
+    const uint64_t b = __ballot(v);
+    if (v) {
+        const uint32_t below = __builtin_amdgcn_mbcnt_hi((uint32_t)(b >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)b, 0u));
+        Q.q[(Q.head + Q.n + below) & (JQ_CAP - 1)] = make_uint2(__float_as_uint(d), ti | (si << 16));
+    }
+    Q.n += (uint32_t)__popcll(b);
+}
+
+// Resolve the ring's first min(n, 64) entries; returns this lane's count
+// of proposals that reached a candidate slot.
+__device__ inline uint32_t jq_flush(const NndDev &a, JoinQ &Q, const uint32_t *col) {
+    wave_sync();
+    const uint32_t lane = (uint32_t)lane_id();
+    const uint32_t m = min(Q.n, (uint32_t)WAVE);
+    uint32_t hit = 0;
+    if (lane < m) {
+        const uint2 e = Q.q[(Q.head + lane) & (JQ_CAP - 1)];
+        const uint32_t t = col[e.y & 0xFFFFu], id = col[e.y >> 16];
+        const uint32_t h = fmix32(id) & (a.HP - 1);
+        if (!pooled_from(a, t, id, h, a.phash[(size_t)t * a.HP + h])) {
+            hit = 1;
+            atomicMin(reinterpret_cast<unsigned long long *>(a.cand + (size_t)t * a.C + cand_slot(t, id, a.C - 1)),
+                      (unsigned long long)make_key(__uint_as_float(e.x), id));
+        }
+    }
+    Q.head = (Q.head + m) & (JQ_CAP - 1);
+    Q.n -= m;
+    wave_sync();
+    return hit;
+}
+
+// Proposals of tile pair (rt, ct) from its dot products: lane holds rows
+// 16rt+4g+i, column 16ct+cl.  Each valid pair proposes both directions;
+// a proposal at or below the target's worst (LDS) enters the ring.
+template <int DT>
+__device__ inline void tile_proposals(const NndDev &a, JoinQ &Q, const uint32_t *col, const uint64_t *cthr,
+                                      const float *cnorm, const float *cscale, uint32_t nn, uint32_t nc, int rt,
+                                      int ct, floatx4 acc, uint32_t &pairs, uint32_t &props) {
+    const int lane = lane_id(), g = lane >> 4, cl = lane & 15;
+    const uint32_t b = (uint32_t)(ct * 16 + cl);
+    const bool bok = b < nc;
+    const uint32_t ib = bok ? col[b] : 0u;
+    const uint64_t tb = bok ? cthr[b] : 0u;
+    const float nb = bok ? cnorm[b] : 0.f;
+    const float sb = (DT == DT_FP8 && bok) ? cscale[b] : 1.f;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const uint32_t ar = (uint32_t)(rt * 16 + 4 * g + i);
+        // new x new: i < j once
+        bool ok = bok && ar < nn && !(b < nn && !(ar < b));
+        const uint32_t ia = ok ? col[ar] : 0u;
+        ok = ok && ia != ib;
+        bool v0 = false, v1 = false;
+        float d = 0.f;
+        if (ok) {
+            float dot = acc[i];
+            if (DT == DT_FP8) dot = dot * (cscale[ar] * sb);
+            d = (cnorm[ar] + nb) - 2.0f * dot;
+            d = fmaxf(d, 0.0f);
+            ++pairs;
+            v0 = make_key(d, ib) <= cthr[ar];  // ib into ia's pool
+            v1 = make_key(d, ia) <= tb;        // ia into ib's pool
+        }
+        jq_push(Q, v0, d, ar, b);
+        jq_push(Q, v1, d, b, ar);
+        while (Q.n >= (uint32_t)WAVE) props += jq_flush(a, Q, col);
+    }
+}
+
 // join(), src/component_refine.cpp:127-143 / nhood::join,
 // include/weavess/index.h:137-149, one wavefront per node.
-template <int DT>
-__global__ void __launch_bounds__(256) nnd_join_kernel(NndDev a) {
+// TS > 0: rows fit one K chunk of at most TS MFMA steps; operands are
+// loaded to registers, RT row tiles share each column sweep, and the next
+// column tile loads while the current one's proposals are made.
+// TS == 0: longer rows, staged through LDS chunk by chunk.
+template <int DT, int TS>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) nnd_join_kernel(NndDev a) {
     extern __shared__ __align__(16) uint8_t smem[];
     constexpr int KC = JoinT<DT>::KC, KSTEP = JoinT<DT>::KSTEP;
     const int wid = threadIdx.x >> 6, lane = lane_id();
     const uint32_t n = blockIdx.x * (blockDim.x >> 6) + wid;
     if (n >= a.N) return;
     const uint32_t ncap = a.ncap;  // |new| <= S+R (2S in EFANNA's first round), |old| <= 2R
-    uint8_t *my = smem + (size_t)wid * (a.lds_join_list + 2 * 16 * a.tile_st);
+    uint8_t *my = smem + (size_t)wid * (a.lds_join_list + JQ_BYTES + (TS == 0 ? 2 * 16 * a.tile_st : 0u));
     uint32_t *nw = reinterpret_cast<uint32_t *>(my);              // nw_pow2
     uint32_t *col = nw + a.nw_pow2;                               // ncap
     uint64_t *cthr = reinterpret_cast<uint64_t *>(col + ((ncap + 1) & ~1u));
     float *cnorm = reinterpret_cast<float *>(cthr + ncap);
     float *cscale = cnorm + ncap;
-    uint8_t *tA = my + a.lds_join_list;
-    uint8_t *tB = tA + 16 * a.tile_st;
+    JoinQ Q{reinterpret_cast<uint2 *>(my + a.lds_join_list), 0u, 0u};
 
     const uint32_t fnc = a.fnew_cnt[n], rnc = a.rsel_new_cnt[n];
     const uint32_t foc = a.fold_cnt[n], roc = a.rsel_old_cnt[n];
@@ -756,88 +857,53 @@ __global__ void __launch_bounds__(256) nnd_join_kernel(NndDev a) {
     wave_sync();
 
     const uint32_t kdim = a.kdim;
-    const uint32_t nkc = (kdim + KC - 1) / KC;
     const int nrt = (int)((nn + 15) / 16), nct = (int)((nc + 15) / 16);
-    const int g = lane >> 4, cl = lane & 15;
     uint32_t pairs = 0, props = 0;
-    const uint32_t klen1 = ((kdim + KSTEP - 1) / KSTEP) * KSTEP;  // one chunk when nkc == 1
-    uint32_t map[TILE_REGS];
-    tile_map<DT>(klen1, kdim, map);
-    uint4 nxt[TILE_REGS];
-    for (int rt = 0; rt < nrt; ++rt) {
-        if (nkc == 1) {
-            load_tile_regs(a, col, rt * 16, (int)nn, map, nxt);  // row tile rt
-            store_tile_regs(tA, a.tile_st, map, nxt);
-            load_tile_regs(a, col, rt * 16, (int)nc, map, nxt);  // column tile rt
+    if constexpr (TS > 0) {
+        using V = typename JoinT<DT>::V;
+        constexpr int RT = TS * (int)sizeof(V) <= 64 ? 2 : 1;
+        const int T = (int)((kdim + KSTEP - 1) / KSTEP);
+        for (int rt0 = 0; rt0 < nrt; rt0 += RT) {
+            V A[RT][TS], B[TS];
+#pragma unroll
+            for (int q = 0; q < RT; ++q) load_ops<DT, TS>(a, col, (rt0 + q) * 16, (int)nn, T, A[q]);
+            load_ops<DT, TS>(a, col, rt0 * 16, (int)nc, T, B);
+            for (int ct = rt0; ct < nct; ++ct) {
+                floatx4 acc[RT];
+#pragma unroll
+                for (int q = 0; q < RT; ++q)
+                    if (rt0 + q < nrt && ct >= rt0 + q) acc[q] = ops_mfma<DT, TS>(A[q], B, T);
+                // the next column tile loads while these proposals are made
+                if (ct + 1 < nct) load_ops<DT, TS>(a, col, (ct + 1) * 16, (int)nc, T, B);
+#pragma unroll
+                for (int q = 0; q < RT; ++q)
+                    if (rt0 + q < nrt && ct >= rt0 + q)
+                        tile_proposals<DT>(a, Q, col, cthr, cnorm, cscale, nn, nc, rt0 + q, ct, acc[q], pairs,
+                                           props);
+            }
         }
-        for (int ct = rt; ct < nct; ++ct) {
-            floatx4 acc = {0.f, 0.f, 0.f, 0.f};
-            if (nkc == 1) {
-                store_tile_regs(tB, a.tile_st, map, nxt);
-                wave_sync();
-                acc = tile_mfma<DT>(tA, tB, a.tile_st, klen1, acc);
-                wave_sync();
-                // next column tile in flight during this tile's proposals
-                if (ct + 1 < nct) load_tile_regs(a, col, (ct + 1) * 16, (int)nc, map, nxt);
-            }
-            for (uint32_t kc = 0; nkc > 1 && kc < nkc; ++kc) {
-                const uint32_t k0 = kc * KC;
-                uint32_t klen = min((uint32_t)KC, kdim - k0);
-                klen = ((klen + KSTEP - 1) / KSTEP) * KSTEP;
-                stage_tile<DT>(a, tA, col, rt * 16, (int)nn, k0, klen);
-                stage_tile<DT>(a, tB, col, ct * 16, (int)nc, k0, klen);
-                wave_sync();
-                acc = tile_mfma<DT>(tA, tB, a.tile_st, klen, acc);
-                wave_sync();
-            }
-            // epilogue: lane holds rows 16rt+4g+i, column 16ct+cl.  Up to 8
-            // proposals (both directions of 4 pairs); those at or below the
-            // target's worst (LDS) probe the target's pool-id hash all at
-            // once, and the survivors compete for their slot by atomicMin.
-            const uint32_t b = (uint32_t)(ct * 16 + cl);
-            uint32_t pt[8], pid[8], ph[8], pe[8];
-            uint64_t pk[8];
-            bool pv[8];
-#pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                const uint32_t ar = (uint32_t)(rt * 16 + 4 * g + i);
-                pv[2 * i] = pv[2 * i + 1] = false;
-                pt[2 * i] = pt[2 * i + 1] = pid[2 * i] = pid[2 * i + 1] = 0;
-                pk[2 * i] = pk[2 * i + 1] = 0;
-                if (ar >= nn || b >= nc) continue;
-                if (b < nn && !(ar < b)) continue;  // new x new: i < j once
-                const uint32_t ia = col[ar], ib = col[b];
-                if (ia == ib) continue;
-                float dot = acc[i];
-                if (DT == DT_FP8) dot = dot * (cscale[ar] * cscale[b]);
-                float d = (cnorm[ar] + cnorm[b]) - 2.0f * dot;
-                d = fmaxf(d, 0.0f);
-                ++pairs;
-                pt[2 * i] = ia;
-                pid[2 * i] = ib;
-                pk[2 * i] = make_key(d, ib);
-                pv[2 * i] = pk[2 * i] <= cthr[ar];
-                pt[2 * i + 1] = ib;
-                pid[2 * i + 1] = ia;
-                pk[2 * i + 1] = make_key(d, ia);
-                pv[2 * i + 1] = pk[2 * i + 1] <= cthr[b];
-            }
-            const uint32_t hmask = a.HP - 1;
-#pragma unroll
-            for (int j = 0; j < 8; ++j) {
-                ph[j] = fmix32(pid[j]) & hmask;
-                pe[j] = pv[j] ? a.phash[(size_t)pt[j] * a.HP + ph[j]] : ID_NONE;
-            }
-#pragma unroll
-            for (int j = 0; j < 8; ++j) {
-                if (!pv[j] || pooled_from(a, pt[j], pid[j], ph[j], pe[j])) continue;
-                ++props;
-                atomicMin(reinterpret_cast<unsigned long long *>(a.cand + (size_t)pt[j] * a.C +
-                                                                 cand_slot(pt[j], pid[j], a.C - 1)),
-                          (unsigned long long)pk[j]);
+    } else {
+        uint8_t *tA = my + a.lds_join_list + JQ_BYTES;
+        uint8_t *tB = tA + 16 * a.tile_st;
+        const uint32_t nkc = (kdim + KC - 1) / KC;
+        for (int rt = 0; rt < nrt; ++rt) {
+            for (int ct = rt; ct < nct; ++ct) {
+                floatx4 acc = {0.f, 0.f, 0.f, 0.f};
+                for (uint32_t kc = 0; kc < nkc; ++kc) {
+                    const uint32_t k0 = kc * KC;
+                    uint32_t klen = min((uint32_t)KC, kdim - k0);
+                    klen = ((klen + KSTEP - 1) / KSTEP) * KSTEP;
+                    stage_tile<DT>(a, tA, col, rt * 16, (int)nn, k0, klen);
+                    stage_tile<DT>(a, tB, col, ct * 16, (int)nc, k0, klen);
+                    wave_sync();
+                    acc = tile_mfma<DT>(tA, tB, a.tile_st, klen, acc);
+                    wave_sync();
+                }
+                tile_proposals<DT>(a, Q, col, cthr, cnorm, cscale, nn, nc, rt, ct, acc, pairs, props);
             }
         }
     }
+    while (Q.n > 0) props += jq_flush(a, Q, col);
     pairs = wave_sum_u32(pairs);
     props = wave_sum_u32(props);
     if (lane == 0) {
@@ -1075,7 +1141,20 @@ struct Bufs {
 };
 
 constexpr int WPB = 4;       // wavefronts (nodes) per workgroup
-constexpr int JOIN_WPB = 2;  // the join holds ~22 KB of LDS per wavefront
+constexpr int JOIN_WPB = 2;  // the join holds up to ~22 KB of LDS per wavefront
+
+// The join kernel for this element type: register operands of at most ts
+// MFMA steps (rows of one K chunk), or the LDS-staged form (ts == 0).
+using JoinFn = void (*)(NndDev);
+template <int DT>
+JoinFn join_kernel(int ts) {
+    if (ts == 4) return nnd_join_kernel<DT, 4>;
+    if (ts == 8) return nnd_join_kernel<DT, 8>;
+    if constexpr (DT == DT_FP8) {
+        if (ts == 16) return nnd_join_kernel<DT, 16>;
+    }
+    return nnd_join_kernel<DT, 0>;
+}
 
 hipError_t scan_u32(const uint32_t *in, uint32_t n, uint32_t *out, uint32_t *bsum, hipStream_t s) {
     const uint32_t nb = (n + SCAN_B - 1) / SCAN_B;
@@ -1209,6 +1288,7 @@ int nndescent_build(wv_index *ix, const wv_nnd_params *p, const uint32_t *init_i
     // list area: nw ids, col ids, cthr keys, cnorm, and FP8 row scales
     a.lds_join_list = (uint32_t)(((a.nw_pow2 * 4 + ((ncap + 1) & ~1u) * 4 + ncap * (dt == DT_FP8 ? 16 : 12)) + 15) &
                                  ~15u);
+    int join_ts = 0;
     {
         // tile rows as wide as one K chunk of this row length: BF16 / FP8 rows
         // of d <= 256 / 512 need a fraction of the 512-byte maximum
@@ -1216,8 +1296,14 @@ int nndescent_build(wv_index *ix, const wv_nnd_params *p, const uint32_t *init_i
         const uint32_t kc = 512 / es, kstep = dt == DT_F32 ? 16 : 32;
         const uint32_t klen = a.kdim <= kc ? (a.kdim + kstep - 1) / kstep * kstep : kc;
         a.tile_st = klen * es + 16;
+        // steps of one chunk: 16 (F32) or 32 elements each
+        const uint32_t steps = (a.kdim + kstep - 1) / kstep;
+        join_ts = a.kdim > kc ? 0 : steps <= 4 ? 4 : steps <= 8 ? 8 : 16;
     }
-    const size_t join_lds = (size_t)JOIN_WPB * (a.lds_join_list + 2 * 16 * a.tile_st);
+    // list area, proposal ring, and the two staged tiles of the chunked form;
+    // the ring's 16-bit list positions hold any ncap these 160 KB allow
+    const size_t join_lds =
+        (size_t)JOIN_WPB * (a.lds_join_list + JQ_BYTES + (join_ts == 0 ? 2 * 16 * a.tile_st : 0u));
     if (join_lds > 160 * 1024)
         return index_set_error(ix, WV_ENOTSUP, "S/R too large for the join's LDS lists");
     const uint32_t up2 = pow2_at_least(L + C, 64), hp2 = pow2_at_least(2 * L, 64);
@@ -1287,9 +1373,9 @@ int nndescent_build(wv_index *ix, const wv_nnd_params *p, const uint32_t *init_i
     float ms_init = 0.f;
     NND_HIP(hipEventElapsedTime(&ms_init, ev[0], ev[1]));
 
-    NND_HIP(set_lds(nnd_join_kernel<DT_F32>, join_lds));
-    NND_HIP(set_lds(nnd_join_kernel<DT_BF16>, join_lds));
-    NND_HIP(set_lds(nnd_join_kernel<DT_FP8>, join_lds));
+    const JoinFn join_fn = dt == DT_F32 ? join_kernel<DT_F32>(join_ts)
+                         : dt == DT_BF16 ? join_kernel<DT_BF16>(join_ts) : join_kernel<DT_FP8>(join_ts);
+    NND_HIP(set_lds(join_fn, join_lds));
     NND_HIP(set_lds(nnd_merge_kernel, merge_lds));
     NND_HIP(set_lds(nnd_rev_select_kernel, sel_lds));
 
@@ -1303,10 +1389,9 @@ int nndescent_build(wv_index *ix, const wv_nnd_params *p, const uint32_t *init_i
             }
             NND_HIP(hipEventRecord(ev[2], s));
             const uint32_t jb = (N + JOIN_WPB - 1) / JOIN_WPB;
-            if (dt == DT_F32) hipLaunchKernelGGL(nnd_join_kernel<DT_F32>, dim3(jb), dim3(64 * JOIN_WPB), join_lds, s, a);
-            else if (dt == DT_BF16) hipLaunchKernelGGL(nnd_join_kernel<DT_BF16>, dim3(jb), dim3(64 * JOIN_WPB), join_lds, s, a);
-            else hipLaunchKernelGGL(nnd_join_kernel<DT_FP8>, dim3(jb), dim3(64 * JOIN_WPB), join_lds, s, a);
-            NND_HIP(hipGetLastError());
+            void *jargs[] = {&a};
+            NND_HIP(hipLaunchKernel(reinterpret_cast<const void *>(join_fn), dim3(jb), dim3(64 * JOIN_WPB), jargs,
+                                    join_lds, s));
             NND_HIP(hipEventRecord(ev[3], s));
             hipLaunchKernelGGL(nnd_merge_kernel, dim3(blocks), dim3(64 * WPB), merge_lds, s, a, up2, hp2,
                                (int)(it + 1 < p->iters));
