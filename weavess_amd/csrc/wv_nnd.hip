@@ -715,16 +715,25 @@ __device__ inline floatx4 ops_mfma(const typename JoinT<DT>::V A[TS], const type
     return acc;
 }
 
-// Is `id` in target t's pool?  Probing of t's pool-id hash from slot `i`,
-// whose entry `e` was already read.
-__device__ inline bool pooled_from(const NndDev &a, uint32_t t, uint32_t id, uint32_t i, uint32_t e) {
-    const uint32_t *h = a.phash + (size_t)t * a.HP;
+// Is `id` in target t's pool?  Linear probing of t's pool-id hash from slot
+// h, read as aligned groups of 4 slots (HP is a multiple of 64, so groups
+// never straddle the wrap): slots h..(h|3) of the first group, then whole
+// groups, in probe order, until `id` or an empty slot.  Most lookups miss
+// and end within the first group, in one load instead of a chain.
+__device__ inline bool pooled(const NndDev &a, uint32_t t, uint32_t id, uint32_t h) {
+    const uint32_t *tab = a.phash + (size_t)t * a.HP;
     const uint32_t mask = a.HP - 1;
+    uint32_t g0 = h & ~3u, from = 0xFu << (h & 3u);
     for (;;) {
-        if (e == id) return true;
-        if (e == ID_NONE) return false;
-        i = (i + 1) & mask;
-        e = h[i];
+        const uint4 g = *reinterpret_cast<const uint4 *>(tab + g0);
+        const uint32_t hit = (g.x == id ? 1u : 0u) | (g.y == id ? 2u : 0u) | (g.z == id ? 4u : 0u) |
+                             (g.w == id ? 8u : 0u);
+        const uint32_t emp = (g.x == ID_NONE ? 1u : 0u) | (g.y == ID_NONE ? 2u : 0u) |
+                             (g.z == ID_NONE ? 4u : 0u) | (g.w == ID_NONE ? 8u : 0u);
+        const uint32_t stop = (hit | emp) & from;
+        if (stop) return (hit & (stop & (0u - stop))) != 0;  // the first stop in probe order
+        g0 = (g0 + 4) & mask;
+        from = 0xFu;
     }
 }
 
@@ -734,43 +743,89 @@ __device__ inline bool pooled_from(const NndDev &a, uint32_t t, uint32_t id, uin
 // first rounds, so the wavefront probes full rows of proposals instead of
 // eight sparse ones per tile.  An entry holds the distance's bits and the
 // list positions of the target (low 16 bits) and of the proposed id.
-constexpr uint32_t JQ_CAP = 256;  // >= 63 waiting + 128 pushed per pair index
+// A flush is split in two: `issue` loads each lane's first hash group and
+// `finish` (at the next flush, or at the end) probes and proposes, so the
+// load's latency overlaps the tiles in between.  The ring holds the block
+// in flight ([head, head + pn)) and the waiting entries behind it.
+constexpr uint32_t JQ_CAP = 256;  // >= 64 in flight + 63 waiting + 128 pushed per pair index
 constexpr uint32_t JQ_BYTES = JQ_CAP * 8;
 struct JoinQ {
     uint2 *q;
-    uint32_t head, n;  // wavefront-uniform
+    uint32_t head, pn, n;  // wavefront-uniform
+    uint4 g;               // this lane's first hash group of the block in flight
 };
 
 __device__ inline void jq_push(JoinQ &Q, bool v, float d, uint32_t ti, uint32_t si) {
     const uint64_t b = __ballot(v);
     if (v) {
         const uint32_t below = __builtin_amdgcn_mbcnt_hi((uint32_t)(b >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)b, 0u));
-        Q.q[(Q.head + Q.n + below) & (JQ_CAP - 1)] = make_uint2(__float_as_uint(d), ti | (si << 16));
+        Q.q[(Q.head + Q.pn + Q.n + below) & (JQ_CAP - 1)] = make_uint2(__float_as_uint(d), ti | (si << 16));
     }
     Q.n += (uint32_t)__popcll(b);
 }
 
-// Resolve the ring's first min(n, 64) entries; returns this lane's count
-// of proposals that reached a candidate slot.
-__device__ inline uint32_t jq_flush(const NndDev &a, JoinQ &Q, const uint32_t *col) {
+// An entry's target, proposed id and first hash slot.
+__device__ inline void jq_entry(const NndDev &a, const JoinQ &Q, const uint32_t *col, uint32_t lane, uint32_t &t,
+                                uint32_t &id, uint32_t &h, uint32_t &dbits) {
+    const uint2 e = Q.q[(Q.head + lane) & (JQ_CAP - 1)];
+    t = col[e.y & 0xFFFFu];
+    id = col[e.y >> 16];
+    h = fmix32(id) & (a.HP - 1);
+    dbits = e.x;
+}
+
+// Start resolving the first min(n, 64) waiting entries.
+__device__ inline void jq_issue(const NndDev &a, JoinQ &Q, const uint32_t *col) {
     wave_sync();
     const uint32_t lane = (uint32_t)lane_id();
     const uint32_t m = min(Q.n, (uint32_t)WAVE);
-    uint32_t hit = 0;
     if (lane < m) {
-        const uint2 e = Q.q[(Q.head + lane) & (JQ_CAP - 1)];
-        const uint32_t t = col[e.y & 0xFFFFu], id = col[e.y >> 16];
-        const uint32_t h = fmix32(id) & (a.HP - 1);
-        if (!pooled_from(a, t, id, h, a.phash[(size_t)t * a.HP + h])) {
-            hit = 1;
+        uint32_t t, id, h, dbits;
+        jq_entry(a, Q, col, lane, t, id, h, dbits);
+        Q.g = *reinterpret_cast<const uint4 *>(a.phash + (size_t)t * a.HP + (h & ~3u));
+    }
+    Q.pn = m;
+    Q.n -= m;
+}
+
+// Probe and propose the block in flight; returns this lane's count of
+// proposals that reached a candidate slot.  Same probe order as `pooled`,
+// whose first group load is the one `issue` made.
+__device__ inline uint32_t jq_finish(const NndDev &a, JoinQ &Q, const uint32_t *col) {
+    wave_sync();
+    const uint32_t lane = (uint32_t)lane_id();
+    uint32_t hitc = 0;
+    if (lane < Q.pn) {
+        uint32_t t, id, h, dbits;
+        jq_entry(a, Q, col, lane, t, id, h, dbits);
+        const uint4 g = Q.g;
+        const uint32_t hit = (g.x == id ? 1u : 0u) | (g.y == id ? 2u : 0u) | (g.z == id ? 4u : 0u) |
+                             (g.w == id ? 8u : 0u);
+        const uint32_t emp = (g.x == ID_NONE ? 1u : 0u) | (g.y == ID_NONE ? 2u : 0u) |
+                             (g.z == ID_NONE ? 4u : 0u) | (g.w == ID_NONE ? 8u : 0u);
+        const uint32_t stop = (hit | emp) & (0xFu << (h & 3u));
+        const bool in_pool = stop ? (hit & (stop & (0u - stop))) != 0
+                                  : pooled(a, t, id, ((h & ~3u) + 4) & (a.HP - 1));
+        if (!in_pool) {
+            hitc = 1;
             atomicMin(reinterpret_cast<unsigned long long *>(a.cand + (size_t)t * a.C + cand_slot(t, id, a.C - 1)),
-                      (unsigned long long)make_key(__uint_as_float(e.x), id));
+                      (unsigned long long)make_key(__uint_as_float(dbits), id));
         }
     }
-    Q.head = (Q.head + m) & (JQ_CAP - 1);
-    Q.n -= m;
+    Q.head = (Q.head + Q.pn) & (JQ_CAP - 1);
+    Q.pn = 0;
     wave_sync();
-    return hit;
+    return hitc;
+}
+
+// Keep at most 63 entries waiting: finish the block in flight, start the next.
+__device__ inline uint32_t jq_drain(const NndDev &a, JoinQ &Q, const uint32_t *col) {
+    uint32_t hits = 0;
+    while (Q.n >= (uint32_t)WAVE) {
+        if (Q.pn) hits += jq_finish(a, Q, col);
+        jq_issue(a, Q, col);
+    }
+    return hits;
 }
 
 // Proposals of tile pair (rt, ct) from its dot products: lane holds rows
@@ -807,7 +862,7 @@ __device__ inline void tile_proposals(const NndDev &a, JoinQ &Q, const uint32_t 
         }
         jq_push(Q, v0, d, ar, b);
         jq_push(Q, v1, d, b, ar);
-        while (Q.n >= (uint32_t)WAVE) props += jq_flush(a, Q, col);
+        props += jq_drain(a, Q, col);
     }
 }
 
@@ -831,7 +886,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) n
     uint64_t *cthr = reinterpret_cast<uint64_t *>(col + ((ncap + 1) & ~1u));
     float *cnorm = reinterpret_cast<float *>(cthr + ncap);
     float *cscale = cnorm + ncap;
-    JoinQ Q{reinterpret_cast<uint2 *>(my + a.lds_join_list), 0u, 0u};
+    JoinQ Q{reinterpret_cast<uint2 *>(my + a.lds_join_list), 0u, 0u, 0u, make_uint4(0, 0, 0, 0)};
 
     const uint32_t fnc = a.fnew_cnt[n], rnc = a.rsel_new_cnt[n];
     const uint32_t foc = a.fold_cnt[n], roc = a.rsel_old_cnt[n];
@@ -903,7 +958,11 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) n
             }
         }
     }
-    while (Q.n > 0) props += jq_flush(a, Q, col);
+    if (Q.pn) props += jq_finish(a, Q, col);
+    while (Q.n > 0) {
+        jq_issue(a, Q, col);
+        props += jq_finish(a, Q, col);
+    }
     pairs = wave_sum_u32(pairs);
     props = wave_sum_u32(props);
     if (lane == 0) {
@@ -946,8 +1005,10 @@ __host__ __device__ inline size_t merge_wave_lds(uint32_t up2, uint32_t hp2, uin
 // when `rehash`, rebuild the node's pool-id hash for the next join.  All of
 // the node's global loads (C slots, L pool keys and flags; C, L <= 1024, so
 // at most 16 per lane each) are issued before any of them is used.
-constexpr int MERGE_REGS = 16;
+// MR: key registers per lane, >= C / 64 and >= L / 64 (at most 16).
+template <int MR>
 __global__ void __launch_bounds__(256) nnd_merge_kernel(NndDev a, uint32_t up2, uint32_t hp2, int rehash) {
+    constexpr int MERGE_REGS = MR;
     extern __shared__ __align__(16) uint8_t smem[];
     const int wid = threadIdx.x >> 6, lane = lane_id();
     const uint32_t n = blockIdx.x * (blockDim.x >> 6) + wid;
@@ -995,7 +1056,7 @@ __global__ void __launch_bounds__(256) nnd_merge_kernel(NndDev a, uint32_t up2, 
     uint32_t nw = 0;
 #pragma unroll
     for (int r = 0; r < MERGE_REGS; ++r) {
-        if ((uint32_t)r >= cper) break;
+        if ((uint32_t)r >= cper) continue;  // uniform; `break` would keep the loop rolled
         const uint64_t k = ck[r];
         bool take = false;
         if (k != KEY_EMPTY) {
@@ -1376,7 +1437,12 @@ int nndescent_build(wv_index *ix, const wv_nnd_params *p, const uint32_t *init_i
     const JoinFn join_fn = dt == DT_F32 ? join_kernel<DT_F32>(join_ts)
                          : dt == DT_BF16 ? join_kernel<DT_BF16>(join_ts) : join_kernel<DT_FP8>(join_ts);
     NND_HIP(set_lds(join_fn, join_lds));
-    NND_HIP(set_lds(nnd_merge_kernel, merge_lds));
+    // merge: key registers per lane for max(C, L) keys
+    const uint32_t mr_need = (std::max(C, L) + WAVE - 1) / WAVE;
+    void (*merge_fn)(NndDev, uint32_t, uint32_t, int) =
+        mr_need <= 2 ? nnd_merge_kernel<2> : mr_need <= 4 ? nnd_merge_kernel<4>
+                                           : mr_need <= 8 ? nnd_merge_kernel<8> : nnd_merge_kernel<16>;
+    NND_HIP(set_lds(merge_fn, merge_lds));
     NND_HIP(set_lds(nnd_rev_select_kernel, sel_lds));
 
     for (uint32_t it = 0; it < p->iters; ++it) {
@@ -1393,9 +1459,11 @@ int nndescent_build(wv_index *ix, const wv_nnd_params *p, const uint32_t *init_i
             NND_HIP(hipLaunchKernel(reinterpret_cast<const void *>(join_fn), dim3(jb), dim3(64 * JOIN_WPB), jargs,
                                     join_lds, s));
             NND_HIP(hipEventRecord(ev[3], s));
-            hipLaunchKernelGGL(nnd_merge_kernel, dim3(blocks), dim3(64 * WPB), merge_lds, s, a, up2, hp2,
-                               (int)(it + 1 < p->iters));
-            NND_HIP(hipGetLastError());
+            uint32_t up2_ = up2, hp2_ = hp2;
+            int rehash = (int)(it + 1 < p->iters);
+            void *margs[] = {&a, &up2_, &hp2_, &rehash};
+            NND_HIP(hipLaunchKernel(reinterpret_cast<const void *>(merge_fn), dim3(blocks), dim3(64 * WPB), margs,
+                                    merge_lds, s));
             NND_HIP(hipEventRecord(ev[4], s));
         } else {
             hipLaunchKernelGGL(nnd_m0_kernel, dim3(blocks), dim3(64 * WPB), 0, s, a);
