@@ -74,7 +74,8 @@ struct NndDev {
     float *worst;
     uint64_t *fnew, *fold;
     uint32_t *fnew_cnt, *fold_cnt;
-    uint32_t *rcnt_new, *rcnt_old, *rcur_new, *rcur_old, *roff_new, *roff_old, *rv_new, *rv_old;
+    uint32_t *rcnt_new, *rcnt_old, *roff_new, *roff_old, *rv_new, *rv_old;
+    uint32_t *rpos_new, *rpos_old;  // per fnew / fold entry: its slot in the neighbour's reverse list, or ID_NONE
     uint32_t *rsel_new, *rsel_old, *rsel_new_cnt, *rsel_old_cnt;
     uint64_t *cand;
     uint32_t *phash;          // N x HP open-addressing table of the pool's ids
@@ -444,7 +445,8 @@ __global__ void __launch_bounds__(256) nnd_efanna_seed_kernel(NndDev a) {
 // ------------------------------------------------------------------ sample
 // update() :176-209 forward part: first M pool entries, flagged ones to the
 // new list (flag cleared), the rest to the old list, in pool order; count a
-// reverse proposal at the neighbour when dist > neighbour's worst.
+// reverse proposal at the neighbour when dist > neighbour's worst, and keep
+// the count's old value as the proposal's slot in the neighbour's list.
 __global__ void __launch_bounds__(256) nnd_sample_kernel(NndDev a) {
     const int wid = threadIdx.x >> 6, lane = lane_id();
     const uint32_t n = blockIdx.x * (blockDim.x >> 6) + wid;
@@ -459,11 +461,19 @@ __global__ void __launch_bounds__(256) nnd_sample_kernel(NndDev a) {
         const uint64_t key = act ? pr[p] : KEY_EMPTY;
         const bool f = act && fl[p];
         const uint64_t bn = __ballot(f), bo = __ballot(act && !f);
-        if (f) a.fnew[(size_t)n * a.fst + nf + lanes_below(bn)] = key;
-        if (act && !f) a.fold[(size_t)n * a.L + no + lanes_below(bo)] = key;
         if (act) {
             const uint32_t id = key_id(key);
-            if (key_dist(key) > a.worst[id]) atomicAdd(f ? &a.rcnt_new[id] : &a.rcnt_old[id], 1u);
+            const uint32_t pos =
+                key_dist(key) > a.worst[id] ? atomicAdd(f ? &a.rcnt_new[id] : &a.rcnt_old[id], 1u) : ID_NONE;
+            if (f) {
+                const size_t j = (size_t)n * a.fst + nf + lanes_below(bn);
+                a.fnew[j] = key;
+                a.rpos_new[j] = pos;
+            } else {
+                const size_t j = (size_t)n * a.L + no + lanes_below(bo);
+                a.fold[j] = key;
+                a.rpos_old[j] = pos;
+            }
             fl[p] = 0;
         }
         nf += (uint32_t)__popcll(bn);
@@ -475,6 +485,8 @@ __global__ void __launch_bounds__(256) nnd_sample_kernel(NndDev a) {
     }
 }
 
+// Place each reverse proposal at its slot in the neighbour's list, the
+// neighbour's offset plus the slot the sample kernel's count handed out.
 __global__ void __launch_bounds__(256) nnd_rev_scatter_kernel(NndDev a) {
     const int wid = threadIdx.x >> 6, lane = lane_id();
     const uint32_t n = blockIdx.x * (blockDim.x >> 6) + wid;
@@ -482,16 +494,12 @@ __global__ void __launch_bounds__(256) nnd_rev_scatter_kernel(NndDev a) {
     const uint32_t nf = a.fnew_cnt[n], no = a.fold_cnt[n];
     for (uint32_t i = lane; i < nf + no; i += WAVE) {
         const bool isnew = i < nf;
-        const uint64_t key = isnew ? a.fnew[(size_t)n * a.fst + i] : a.fold[(size_t)n * a.L + (i - nf)];
-        const uint32_t id = key_id(key);
-        if (!(key_dist(key) > a.worst[id])) continue;
-        if (isnew) {
-            const uint32_t pos = atomicAdd(&a.rcur_new[id], 1u);
-            a.rv_new[a.roff_new[id] + pos] = n;
-        } else {
-            const uint32_t pos = atomicAdd(&a.rcur_old[id], 1u);
-            a.rv_old[a.roff_old[id] + pos] = n;
-        }
+        const size_t j = isnew ? (size_t)n * a.fst + i : (size_t)n * a.L + (i - nf);
+        const uint32_t pos = isnew ? a.rpos_new[j] : a.rpos_old[j];
+        if (pos == ID_NONE) continue;
+        const uint32_t id = key_id(isnew ? a.fnew[j] : a.fold[j]);
+        if (isnew) a.rv_new[a.roff_new[id] + pos] = n;
+        else a.rv_old[a.roff_old[id] + pos] = n;
     }
 }
 
@@ -525,6 +533,15 @@ __global__ void __launch_bounds__(256) nnd_rev_select_kernel(NndDev a, uint32_t 
         const uint64_t sx = rev_stream(a.seed, which ? TAG_REVOLD : TAG_REVNEW, a.iter, x);
         const uint32_t R = a.R;
         uint32_t kept;
+        if (c <= R && c <= (uint32_t)WAVE) {
+            // all kept, one per lane: sorted in registers
+            int w = 2;
+            while ((uint32_t)w < c) w <<= 1;
+            const uint64_t v = wave_sort64_n((uint32_t)lane < c ? (uint64_t)seg[lane] : (uint64_t)ID_NONE, w);
+            if ((uint32_t)lane < c) (which ? a.rsel_old : a.rsel_new)[(size_t)x * R + lane] = (uint32_t)v;
+            if (lane == 0) (which ? a.rsel_old_cnt : a.rsel_new_cnt)[x] = c;
+            continue;
+        }
         if (c <= R) {
             for (uint32_t i = lane; i < rp2; i += WAVE) ids[i] = i < c ? seg[i] : ID_NONE;
             kept = c;
@@ -592,7 +609,9 @@ __global__ void __launch_bounds__(256) nnd_rev_select_kernel(NndDev a, uint32_t 
             kept = R;
         }
         wave_sync();
-        lds_sort32(ids, (int)rp2);
+        int sw = 2;  // slots from `kept` up hold ID_NONE: sort the power of two covering it
+        while ((uint32_t)sw < kept) sw <<= 1;
+        lds_sort32(ids, sw);
         uint32_t *dst = (which ? a.rsel_old : a.rsel_new) + (size_t)x * R;
         for (uint32_t i = lane; i < kept; i += WAVE) dst[i] = ids[i];
         if (lane == 0) (which ? a.rsel_old_cnt : a.rsel_new_cnt)[x] = kept;
@@ -1322,8 +1341,8 @@ int nndescent_build(wv_index *ix, const wv_nnd_params *p, const uint32_t *init_i
     NND_HIP(B.get(&a.fold_cnt, N));
     NND_HIP(B.get(&a.rcnt_new, N));
     NND_HIP(B.get(&a.rcnt_old, N));
-    NND_HIP(B.get(&a.rcur_new, N));
-    NND_HIP(B.get(&a.rcur_old, N));
+    NND_HIP(B.get(&a.rpos_new, (size_t)N * a.fst));
+    NND_HIP(B.get(&a.rpos_old, (size_t)N * L));
     NND_HIP(B.get(&a.roff_new, (size_t)N + 1));
     NND_HIP(B.get(&a.roff_old, (size_t)N + 1));
     NND_HIP(B.get(&a.rv_new, (size_t)N * L));
@@ -1473,8 +1492,6 @@ int nndescent_build(wv_index *ix, const wv_nnd_params *p, const uint32_t *init_i
             NND_HIP(hipEventRecord(ev[5], s));
             hipLaunchKernelGGL(fill_u32_kernel, dim3(1024), dim3(256), 0, s, a.rcnt_new, 0u, (size_t)N);
             hipLaunchKernelGGL(fill_u32_kernel, dim3(1024), dim3(256), 0, s, a.rcnt_old, 0u, (size_t)N);
-            hipLaunchKernelGGL(fill_u32_kernel, dim3(1024), dim3(256), 0, s, a.rcur_new, 0u, (size_t)N);
-            hipLaunchKernelGGL(fill_u32_kernel, dim3(1024), dim3(256), 0, s, a.rcur_old, 0u, (size_t)N);
             hipLaunchKernelGGL(nnd_sample_kernel, dim3(blocks), dim3(64 * WPB), 0, s, a);
             NND_HIP(hipGetLastError());
             NND_HIP(scan_u32(a.rcnt_new, N, a.roff_new, bsum, s));
