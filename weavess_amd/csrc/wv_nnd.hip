@@ -85,7 +85,6 @@ struct NndDev {
     float *out_dist;
     uint32_t *out_deg;
     uint32_t lds_join_list;   // bytes of list area in the join's per-wave LDS
-    uint32_t tile_st;         // LDS bytes per staged tile row: the chunk's bytes + 16 pad
     uint32_t nw_pow2;         // power of two >= the longest new list (S + R; 2S in EFANNA's first round)
     uint32_t fst;             // ids per fnew row: S, or 2S for EFANNA's first-round samples
     uint32_t ncap;            // longest new + old list: S + 3R, or at least 2S for EFANNA
@@ -620,76 +619,35 @@ __global__ void __launch_bounds__(256) nnd_rev_select_kernel(NndDev a, uint32_t 
 }
 
 // ------------------------------------------------------------------ join
-// Per element type: bytes per element, elements per K chunk (one chunk's
-// tile row is at most 512 bytes), elements per MFMA step, and the lane's
-// operand vector of one step (KSTEP / 4 elements).
+// Per element type: bytes per element, elements per MFMA step, the lane's
+// operand vector of one step (KSTEP / 4 elements), and the steps per
+// streamed piece of rows longer than 512 bytes (the longest whose operands
+// stay in registers across a column sweep).
 template <int DT> struct JoinT;
-template <> struct JoinT<DT_F32> { static constexpr int ES = 4, KC = 128, KSTEP = 16; using V = uint4; };
-template <> struct JoinT<DT_BF16> { static constexpr int ES = 2, KC = 256, KSTEP = 32; using V = uint4; };
-template <> struct JoinT<DT_FP8> { static constexpr int ES = 1, KC = 512, KSTEP = 32; using V = uint2; };
+template <> struct JoinT<DT_F32> {
+    static constexpr int ES = 4, KSTEP = 16, PIECE = 4;
+    using V = uint4;
+};
+template <> struct JoinT<DT_BF16> {
+    static constexpr int ES = 2, KSTEP = 32, PIECE = 4;
+    using V = uint4;
+};
+template <> struct JoinT<DT_FP8> {
+    static constexpr int ES = 1, KSTEP = 32, PIECE = 8;
+    using V = uint2;
+};
 
 __device__ inline void vzero(uint4 &v) { v = make_uint4(0, 0, 0, 0); }
 __device__ inline void vzero(uint2 &v) { v = make_uint2(0, 0); }
 
-// Stage join rows list[r0 .. r0+16) (zero rows past `count`), elements
-// [k0, k0 + klen) with klen a multiple of KSTEP, into a 16 x tile_st tile.
-// Rows longer than one K chunk only.
-template <int DT>
-__device__ inline void stage_tile(const NndDev &a, uint8_t *tile, const uint32_t *list, int r0,
-                                  int count, uint32_t k0, uint32_t klen) {
-    constexpr int ES = JoinT<DT>::ES;
-    const int lane = lane_id();
-    const int vpr = (int)(klen * ES / 16);  // 16-byte vectors per row
-    const int epv = 16 / ES;                // elements per vector
-    for (int v = lane; v < 16 * vpr; v += WAVE) {
-        const int r = v / vpr, cv = v - r * vpr;
-        const uint32_t e0 = k0 + (uint32_t)(cv * epv);
-        uint4 val = make_uint4(0, 0, 0, 0);
-        if (r0 + r < count && e0 < a.kdim) {
-            const uint32_t id = list[r0 + r];
-            val = *reinterpret_cast<const uint4 *>(a.xq + (size_t)id * a.xq_stride + (size_t)e0 * ES);
-        }
-        *reinterpret_cast<uint4 *>(tile + r * a.tile_st + cv * 16) = val;
-    }
-}
-
-// The MFMA steps of one K chunk over staged tiles: lane (r = lane & 15,
-// g = lane >> 4) feeds row r's elements [KSTEP t + KSTEP/4 g, + KSTEP/4).
-template <int DT>
-__device__ inline floatx4 tile_mfma(const uint8_t *A, const uint8_t *B, uint32_t st, uint32_t klen, floatx4 acc) {
-    const int lane = lane_id(), r = lane & 15, g = lane >> 4;
-    if (DT == DT_F32) {
-        for (uint32_t t = 0; t < klen / 16; ++t) {
-            const float4 x = *reinterpret_cast<const float4 *>(A + r * st + (16 * t + 4 * g) * 4);
-            const float4 y = *reinterpret_cast<const float4 *>(B + r * st + (16 * t + 4 * g) * 4);
-            acc = __builtin_amdgcn_mfma_f32_16x16x4f32(x.x, y.x, acc, 0, 0, 0);
-            acc = __builtin_amdgcn_mfma_f32_16x16x4f32(x.y, y.y, acc, 0, 0, 0);
-            acc = __builtin_amdgcn_mfma_f32_16x16x4f32(x.z, y.z, acc, 0, 0, 0);
-            acc = __builtin_amdgcn_mfma_f32_16x16x4f32(x.w, y.w, acc, 0, 0, 0);
-        }
-    } else if (DT == DT_BF16) {
-        for (uint32_t t = 0; t < klen / 32; ++t) {
-            const bf16x8 x = *reinterpret_cast<const bf16x8 *>(A + r * st + (32 * t + 8 * g) * 2);
-            const bf16x8 y = *reinterpret_cast<const bf16x8 *>(B + r * st + (32 * t + 8 * g) * 2);
-            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(x, y, acc, 0, 0, 0);
-        }
-    } else {
-        for (uint32_t t = 0; t < klen / 32; ++t) {
-            const long x = *reinterpret_cast<const long *>(A + r * st + 32 * t + 8 * g);
-            const long y = *reinterpret_cast<const long *>(B + r * st + 32 * t + 8 * g);
-            acc = __builtin_amdgcn_mfma_f32_16x16x32_fp8_fp8(x, y, acc, 0, 0, 0);
-        }
-    }
-    return acc;
-}
-
-// Rows of one K chunk go to the MFMA straight from the join rows, without
-// LDS: in that operand layout each lane reads row r's own bytes, nothing a
-// neighbour loaded.  ops[t] (t < T) is lane (r, g)'s step-t vector of rows
+// Join rows go to the MFMA straight from HBM, without LDS: in the 16x16
+// operand layout lane (r = lane & 15, g = lane >> 4) feeds row r's elements
+// [KSTEP t + KSTEP/4 g, + KSTEP/4) at step t, bytes no other lane reads.
+// ops[j] is lane (r, g)'s vector of step t0 + j (j < TS, t0 + j < T) of row
 // list[r0 + r]; rows at or past `count` and elements at or past kdim read
 // as zero.
 template <int DT, int TS>
-__device__ inline void load_ops(const NndDev &a, const uint32_t *list, int r0, int count, int T,
+__device__ inline void load_ops(const NndDev &a, const uint32_t *list, int r0, int count, int T, int t0,
                                 typename JoinT<DT>::V ops[TS]) {
     using V = typename JoinT<DT>::V;
     constexpr int KSTEP = JoinT<DT>::KSTEP, ES = JoinT<DT>::ES;
@@ -697,20 +655,20 @@ __device__ inline void load_ops(const NndDev &a, const uint32_t *list, int r0, i
     const bool live = r0 + r < count;
     const uint8_t *row = a.xq + (size_t)(live ? list[r0 + r] : 0u) * a.xq_stride;
 #pragma unroll
-    for (int t = 0; t < TS; ++t) {
-        const uint32_t e0 = (uint32_t)(KSTEP * t + (KSTEP / 4) * g);
-        vzero(ops[t]);
-        if (live && t < T && e0 < a.kdim) ops[t] = *reinterpret_cast<const V *>(row + (size_t)e0 * ES);
+    for (int j = 0; j < TS; ++j) {
+        const uint32_t e0 = (uint32_t)(KSTEP * (t0 + j) + (KSTEP / 4) * g);
+        vzero(ops[j]);
+        if (live && t0 + j < T && e0 < a.kdim) ops[j] = *reinterpret_cast<const V *>(row + (size_t)e0 * ES);
     }
 }
 
-// The MFMA steps of tile_mfma, over register operands; same order.
+// MFMA steps j < n of register operands into acc, in step order.
 template <int DT, int TS>
-__device__ inline floatx4 ops_mfma(const typename JoinT<DT>::V A[TS], const typename JoinT<DT>::V B[TS], int T) {
-    floatx4 acc = {0.f, 0.f, 0.f, 0.f};
+__device__ inline floatx4 ops_mfma(const typename JoinT<DT>::V A[TS], const typename JoinT<DT>::V B[TS], int n,
+                                   floatx4 acc) {
 #pragma unroll
     for (int t = 0; t < TS; ++t) {
-        if (t >= T) continue;
+        if (t >= n) continue;
         if (DT == DT_F32) {
             float4 x, y;
             __builtin_memcpy(&x, &A[t], 16);
@@ -777,8 +735,7 @@ struct JoinQ {
 __device__ inline void jq_push(JoinQ &Q, bool v, float d, uint32_t ti, uint32_t si) {
     const uint64_t b = __ballot(v);
     if (v) {
-        const uint32_t below = __builtin_amdgcn_mbcnt_hi((uint32_t)(b >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)b, 0u));
-        Q.q[(Q.head + Q.pn + Q.n + below) & (JQ_CAP - 1)] = make_uint2(__float_as_uint(d), ti | (si << 16));
+        Q.q[(Q.head + Q.pn + Q.n + lanes_below(b)) & (JQ_CAP - 1)] = make_uint2(__float_as_uint(d), ti | (si << 16));
     }
     Q.n += (uint32_t)__popcll(b);
 }
@@ -890,16 +847,16 @@ __device__ inline void tile_proposals(const NndDev &a, JoinQ &Q, const uint32_t 
 // TS > 0: rows fit one K chunk of at most TS MFMA steps; operands are
 // loaded to registers, RT row tiles share each column sweep, and the next
 // column tile loads while the current one's proposals are made.
-// TS == 0: longer rows, staged through LDS chunk by chunk.
+// TS == 0: longer rows, streamed through registers a piece at a time.
 template <int DT, int TS>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) nnd_join_kernel(NndDev a) {
     extern __shared__ __align__(16) uint8_t smem[];
-    constexpr int KC = JoinT<DT>::KC, KSTEP = JoinT<DT>::KSTEP;
+    constexpr int KSTEP = JoinT<DT>::KSTEP;
     const int wid = threadIdx.x >> 6, lane = lane_id();
     const uint32_t n = blockIdx.x * (blockDim.x >> 6) + wid;
     if (n >= a.N) return;
     const uint32_t ncap = a.ncap;  // |new| <= S+R (2S in EFANNA's first round), |old| <= 2R
-    uint8_t *my = smem + (size_t)wid * (a.lds_join_list + JQ_BYTES + (TS == 0 ? 2 * 16 * a.tile_st : 0u));
+    uint8_t *my = smem + (size_t)wid * (a.lds_join_list + JQ_BYTES);
     uint32_t *nw = reinterpret_cast<uint32_t *>(my);              // nw_pow2
     uint32_t *col = nw + a.nw_pow2;                               // ncap
     uint64_t *cthr = reinterpret_cast<uint64_t *>(col + ((ncap + 1) & ~1u));
@@ -940,15 +897,16 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) n
         for (int rt0 = 0; rt0 < nrt; rt0 += RT) {
             V A[RT][TS], B[TS];
 #pragma unroll
-            for (int q = 0; q < RT; ++q) load_ops<DT, TS>(a, col, (rt0 + q) * 16, (int)nn, T, A[q]);
-            load_ops<DT, TS>(a, col, rt0 * 16, (int)nc, T, B);
+            for (int q = 0; q < RT; ++q) load_ops<DT, TS>(a, col, (rt0 + q) * 16, (int)nn, T, 0, A[q]);
+            load_ops<DT, TS>(a, col, rt0 * 16, (int)nc, T, 0, B);
             for (int ct = rt0; ct < nct; ++ct) {
                 floatx4 acc[RT];
 #pragma unroll
                 for (int q = 0; q < RT; ++q)
-                    if (rt0 + q < nrt && ct >= rt0 + q) acc[q] = ops_mfma<DT, TS>(A[q], B, T);
+                    if (rt0 + q < nrt && ct >= rt0 + q)
+                        acc[q] = ops_mfma<DT, TS>(A[q], B, T, floatx4{0.f, 0.f, 0.f, 0.f});
                 // the next column tile loads while these proposals are made
-                if (ct + 1 < nct) load_ops<DT, TS>(a, col, (ct + 1) * 16, (int)nc, T, B);
+                if (ct + 1 < nct) load_ops<DT, TS>(a, col, (ct + 1) * 16, (int)nc, T, 0, B);
 #pragma unroll
                 for (int q = 0; q < RT; ++q)
                     if (rt0 + q < nrt && ct >= rt0 + q)
@@ -957,21 +915,28 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) n
             }
         }
     } else {
-        uint8_t *tA = my + a.lds_join_list + JQ_BYTES;
-        uint8_t *tB = tA + 16 * a.tile_st;
-        const uint32_t nkc = (kdim + KC - 1) / KC;
+        // longer rows: per tile pair, pieces of PIECE steps streamed through
+        // registers, the next piece loading during the current one's MFMAs
+        using V = typename JoinT<DT>::V;
+        constexpr int PS = JoinT<DT>::PIECE;
+        const int T = (int)((kdim + KSTEP - 1) / KSTEP);
         for (int rt = 0; rt < nrt; ++rt) {
             for (int ct = rt; ct < nct; ++ct) {
                 floatx4 acc = {0.f, 0.f, 0.f, 0.f};
-                for (uint32_t kc = 0; kc < nkc; ++kc) {
-                    const uint32_t k0 = kc * KC;
-                    uint32_t klen = min((uint32_t)KC, kdim - k0);
-                    klen = ((klen + KSTEP - 1) / KSTEP) * KSTEP;
-                    stage_tile<DT>(a, tA, col, rt * 16, (int)nn, k0, klen);
-                    stage_tile<DT>(a, tB, col, ct * 16, (int)nc, k0, klen);
-                    wave_sync();
-                    acc = tile_mfma<DT>(tA, tB, a.tile_st, klen, acc);
-                    wave_sync();
+                V A[PS], B[PS], An[PS], Bn[PS];
+                load_ops<DT, PS>(a, col, rt * 16, (int)nn, T, 0, A);
+                load_ops<DT, PS>(a, col, ct * 16, (int)nc, T, 0, B);
+                for (int t0 = 0; t0 < T; t0 += PS) {
+                    if (t0 + PS < T) {
+                        load_ops<DT, PS>(a, col, rt * 16, (int)nn, T, t0 + PS, An);
+                        load_ops<DT, PS>(a, col, ct * 16, (int)nc, T, t0 + PS, Bn);
+                    }
+                    acc = ops_mfma<DT, PS>(A, B, min(PS, T - t0), acc);
+#pragma unroll
+                    for (int j = 0; j < PS; ++j) {
+                        A[j] = An[j];
+                        B[j] = Bn[j];
+                    }
                 }
                 tile_proposals<DT>(a, Q, col, cthr, cnorm, cscale, nn, nc, rt, ct, acc, pairs, props);
             }
@@ -1221,7 +1186,7 @@ struct Bufs {
 };
 
 constexpr int WPB = 4;       // wavefronts (nodes) per workgroup
-constexpr int JOIN_WPB = 2;  // the join holds up to ~22 KB of LDS per wavefront
+constexpr int JOIN_WPB = 2;  // the join holds its lists and proposal ring in LDS
 
 // The join kernel for this element type: register operands of at most ts
 // MFMA steps (rows of one K chunk), or the LDS-staged form (ts == 0).
@@ -1370,20 +1335,16 @@ int nndescent_build(wv_index *ix, const wv_nnd_params *p, const uint32_t *init_i
                                  ~15u);
     int join_ts = 0;
     {
-        // tile rows as wide as one K chunk of this row length: BF16 / FP8 rows
-        // of d <= 256 / 512 need a fraction of the 512-byte maximum
+        // MFMA steps of a row (16 FP32 or 32 BF16/FP8 elements each); rows of
+        // at most one 512-byte chunk keep their operands in registers
         const uint32_t es = dt == DT_F32 ? 4 : (dt == DT_BF16 ? 2 : 1);
         const uint32_t kc = 512 / es, kstep = dt == DT_F32 ? 16 : 32;
-        const uint32_t klen = a.kdim <= kc ? (a.kdim + kstep - 1) / kstep * kstep : kc;
-        a.tile_st = klen * es + 16;
-        // steps of one chunk: 16 (F32) or 32 elements each
         const uint32_t steps = (a.kdim + kstep - 1) / kstep;
         join_ts = a.kdim > kc ? 0 : steps <= 4 ? 4 : steps <= 8 ? 8 : 16;
     }
-    // list area, proposal ring, and the two staged tiles of the chunked form;
-    // the ring's 16-bit list positions hold any ncap these 160 KB allow
-    const size_t join_lds =
-        (size_t)JOIN_WPB * (a.lds_join_list + JQ_BYTES + (join_ts == 0 ? 2 * 16 * a.tile_st : 0u));
+    // list area and proposal ring; the ring's 16-bit list positions hold any
+    // ncap these 160 KB allow
+    const size_t join_lds = (size_t)JOIN_WPB * (a.lds_join_list + JQ_BYTES);
     if (join_lds > 160 * 1024)
         return index_set_error(ix, WV_ENOTSUP, "S/R too large for the join's LDS lists");
     const uint32_t up2 = pow2_at_least(L + C, 64), hp2 = pow2_at_least(2 * L, 64);
