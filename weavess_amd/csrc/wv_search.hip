@@ -218,12 +218,16 @@ __global__ void __launch_bounds__(256) greedy_search_kernel(SearchArgs a) {
         if (from_buf) kb = jb + 1;
         else kp = kk + 1;
         ++hops;
+        // the first 64 neighbour slots load together with the degree, not
+        // after it: a row holds deg_max slots, so those past the degree are
+        // in bounds, and they are masked off below
+        const uint32_t *adj = a.graph + (size_t)u * a.deg_max;
+        const uint32_t id0 = (uint32_t)lane < a.deg_max ? adj[lane] : ID_NONE;
         uint32_t du = a.deg[u];
         if (du > a.deg_max) du = a.deg_max;
-        const uint32_t *adj = a.graph + (size_t)u * a.deg_max;
         for (uint32_t m0 = 0; m0 < du; m0 += WAVE) {
             const uint32_t mi = m0 + lane;
-            const uint32_t id = mi < du ? adj[mi] : ID_NONE;
+            const uint32_t id = mi < du ? (m0 == 0 ? id0 : adj[mi]) : ID_NONE;
             const bool fresh = id < N && visit_first(vis, hmask, id, probes) != 0;
             const uint64_t bm = __ballot(fresh);
             const int m = __popcll(bm);
