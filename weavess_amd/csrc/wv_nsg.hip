@@ -675,6 +675,65 @@ __global__ void __launch_bounds__(256) pair_keys_kernel(const float *base, uint3
     if (lane_id() == 0) out[i] = ck[wid][0];
 }
 
+// One level of a breadth-first closure over the pruned graph (gid rows of
+// gcnt ids) plus link lists (CSR xoff / xid): one wavefront per frontier
+// node; each out-neighbour not yet reached is marked and appended to the
+// next frontier (one counter atomic per wavefront batch).
+__global__ void __launch_bounds__(256) ssg_bfs_kernel(const uint32_t *gid, const uint32_t *gcnt, uint32_t R,
+                                                      const uint32_t *xoff, const uint32_t *xid, uint32_t *flag,
+                                                      const uint32_t *fin, uint32_t nin, uint32_t *fout,
+                                                      uint32_t *nout) {
+    const int wid = threadIdx.x >> 6, lane = lane_id();
+    const uint32_t i = blockIdx.x * (blockDim.x >> 6) + wid;
+    if (i >= nin) return;
+    const uint32_t u = fin[i];
+    const uint32_t cu = gcnt[u], x0 = xoff[u], cx = xoff[u + 1] - x0;
+    for (uint32_t j0 = 0; j0 < cu + cx; j0 += WAVE) {
+        const uint32_t j = j0 + (uint32_t)lane;
+        uint32_t w = ID_NONE;
+        if (j < cu) w = gid[(size_t)u * R + j];
+        else if (j < cu + cx) w = xid[x0 + (j - cu)];
+        const bool fresh = w != ID_NONE && flag[w] == 0u && atomicExch(&flag[w], 1u) == 0u;
+        const uint64_t b = __ballot(fresh);
+        if (b == 0) continue;
+        uint32_t base = 0;
+        if (lane == 0) base = atomicAdd(nout, (uint32_t)__popcll(b));
+        base = (uint32_t)__shfl((int)base, 0);
+        if (fresh) fout[base + lanes_below(b)] = w;
+    }
+}
+
+// Labels of the nodes a closure left unreached: label[w] = the smallest
+// unreached id with a path to w (reached nodes hold N and are skipped; no
+// path from a reached node leads back to an unreached one).  One relaxation
+// pass over the unreached nodes' out-edges, a wavefront per node; *changed
+// is set when a label drops.  Repeated until nothing changes.
+__global__ void __launch_bounds__(256) ssg_label_kernel(const uint32_t *gid, const uint32_t *gcnt, uint32_t R,
+                                                        const uint32_t *xoff, const uint32_t *xid,
+                                                        const uint32_t *flag, uint32_t *label, uint32_t N,
+                                                        uint32_t *changed) {
+    const int wid = threadIdx.x >> 6, lane = lane_id();
+    const uint32_t u = blockIdx.x * (blockDim.x >> 6) + wid;
+    if (u >= N || flag[u]) return;
+    const uint32_t lu = label[u];
+    const uint32_t cu = gcnt[u], x0 = xoff[u], cx = xoff[u + 1] - x0;
+    bool dropped = false;
+    for (uint32_t j = (uint32_t)lane; j < cu + cx; j += WAVE) {
+        const uint32_t w = j < cu ? gid[(size_t)u * R + j] : xid[x0 + (j - cu)];
+        if (!flag[w] && lu < label[w]) {
+            atomicMin(&label[w], lu);
+            dropped = true;
+        }
+    }
+    if (__ballot(dropped) && lane == 0) atomicOr(changed, 1u);
+}
+
+// label[x] = flag[x] ? N : x
+__global__ void ssg_label_init_kernel(const uint32_t *flag, uint32_t *label, uint32_t N) {
+    const uint32_t x = blockIdx.x * blockDim.x + threadIdx.x;
+    if (x < N) label[x] = flag[x] ? N : x;
+}
+
 template <int CPL>
 hipError_t launch_ssg(const NsgArgs &a, hipStream_t s) {
     const size_t lds = (size_t)NWPB * a.lds_wave;
@@ -1151,7 +1210,18 @@ int ssg_build(wv_index *ix, const wv_ssg_params *p, uint32_t *eps_out, wv_nsg_st
     if (rc) return rc;
     const double insert_ms = ms_since(t0);
 
-    // ComponentConnSSGDFS (component_conn.cpp:169-226)
+    // ComponentConnSSGDFS (component_conn.cpp:169-226), per root:
+    //  1. the root's closure, on the device, a breadth-first level per launch
+    //     (what a root reaches does not depend on the order of the visit);
+    //  2. the reference then links the smallest unreached node `low` from
+    //     the smallest reached node with room, reaches low's closure, and
+    //     repeats.  A node x it leaves unreached is reached at the step of
+    //     label(x), the smallest unreached id with a path to x, and low takes
+    //     exactly the values v with label(v) == v: any smaller unreached
+    //     node reaching v, or anything reaching it, would have been linked
+    //     first and reached v.  Labels are propagated on the device;
+    //  3. the host walks the labels in ascending order, linking and reaching
+    //     as the reference does.
     t0 = std::chrono::high_resolution_clock::now();
     std::vector<uint32_t> gid((size_t)N * R), gcnt(N);
     std::vector<float> gdd((size_t)N * R);
@@ -1162,45 +1232,99 @@ int ssg_build(wv_index *ix, const wv_ssg_params *p, uint32_t *eps_out, wv_nsg_st
     std::vector<uint32_t> lu, lv;  // links lu[i] -> lv[i]
     std::vector<uint32_t> roots(p->n_try);
     gen_random_host(stream_of(p->seed, TAG_SSGE, 0), p->n_try, N, roots.data());
+    uint32_t *d_flag = nullptr, *d_f0 = nullptr, *d_f1 = nullptr, *d_n = nullptr, *d_xoff = nullptr,
+             *d_xid = nullptr, *d_label = nullptr;
+    NSG_HIP(B.get(&d_flag, N));
+    NSG_HIP(B.get(&d_f0, N));
+    NSG_HIP(B.get(&d_f1, N));
+    NSG_HIP(B.get(&d_n, 1));
+    NSG_HIP(B.get(&d_xoff, (size_t)N + 1));
+    NSG_HIP(B.get(&d_label, N));
+    std::vector<uint32_t> xoff((size_t)N + 1), xid, flag32(N), label(N), boff((size_t)N + 1), bnode;
+    size_t xcap = 0;
     std::vector<uint8_t> flag(N);
-    std::vector<uint32_t> q, dgr(gcnt);  // dgr: current degree (links included)
+    std::vector<uint32_t> dgr(gcnt);  // current degree (links included)
     for (uint32_t r = 0; r < p->n_try; ++r) {
-        std::fill(flag.begin(), flag.end(), 0);
+        // links so far, as CSR, for the device passes
+        xid.clear();
+        for (uint32_t i = 0; i < N; ++i) {
+            xoff[i] = (uint32_t)xid.size();
+            xid.insert(xid.end(), extra[i].begin(), extra[i].end());
+        }
+        xoff[N] = (uint32_t)xid.size();
+        if (xid.size() > xcap) {
+            xcap = std::max<size_t>(xid.size(), 2 * xcap);
+            NSG_HIP(B.get(&d_xid, xcap));
+        }
+        NSG_HIP(hipMemcpyAsync(d_xoff, xoff.data(), xoff.size() * 4, hipMemcpyHostToDevice, s));
+        if (!xid.empty()) NSG_HIP(hipMemcpyAsync(d_xid, xid.data(), xid.size() * 4, hipMemcpyHostToDevice, s));
+        // 1. the root's closure
+        NSG_HIP(hipMemsetAsync(d_flag, 0, (size_t)N * 4, s));
+        const uint32_t one = 1u, root = roots[r];
+        NSG_HIP(hipMemcpyAsync(d_flag + root, &one, 4, hipMemcpyHostToDevice, s));
+        NSG_HIP(hipMemcpyAsync(d_f0, &root, 4, hipMemcpyHostToDevice, s));
+        uint32_t nin = 1;
+        uint32_t *fin = d_f0, *fout = d_f1;
+        while (nin > 0) {
+            NSG_HIP(hipMemsetAsync(d_n, 0, 4, s));
+            hipLaunchKernelGGL(ssg_bfs_kernel, dim3((nin + 3) / 4), dim3(256), 0, s, a.out_ids, a.out_cnt, R, d_xoff,
+                               d_xid, d_flag, fin, nin, fout, d_n);
+            NSG_HIP(hipGetLastError());
+            NSG_HIP(hipMemcpyAsync(&nin, d_n, 4, hipMemcpyDeviceToHost, s));
+            NSG_HIP(hipStreamSynchronize(s));
+            std::swap(fin, fout);
+        }
+        // 2. labels of the unreached nodes (labels only drop, so this ends)
+        hipLaunchKernelGGL(ssg_label_init_kernel, dim3((N + 255) / 256), dim3(256), 0, s, d_flag, d_label, N);
+        NSG_HIP(hipGetLastError());
+        for (uint32_t changed = 1; changed;) {
+            NSG_HIP(hipMemsetAsync(d_n, 0, 4, s));
+            hipLaunchKernelGGL(ssg_label_kernel, dim3((N + 3) / 4), dim3(256), 0, s, a.out_ids, a.out_cnt, R, d_xoff,
+                               d_xid, d_flag, d_label, N, d_n);
+            NSG_HIP(hipGetLastError());
+            NSG_HIP(hipMemcpyAsync(&changed, d_n, 4, hipMemcpyDeviceToHost, s));
+            NSG_HIP(hipStreamSynchronize(s));
+        }
+        NSG_HIP(hipMemcpyAsync(flag32.data(), d_flag, (size_t)N * 4, hipMemcpyDeviceToHost, s));
+        NSG_HIP(hipMemcpyAsync(label.data(), d_label, (size_t)N * 4, hipMemcpyDeviceToHost, s));
+        NSG_HIP(hipStreamSynchronize(s));
+        // 3. unreached nodes bucketed by label, then the reference's walk
+        std::fill(boff.begin(), boff.end(), 0u);
         uint32_t best = N;  // the smallest reached node with fewer than R neighbours
-        auto reach = [&](uint32_t x) {
-            flag[x] = 1;
-            q.push_back(x);
-            if (dgr[x] < R && x < best) best = x;
-        };
-        q.clear();
-        reach(roots[r]);
-        size_t head = 0;
-        uint32_t low = 0;  // every id below `low` is reached
-        for (;;) {
-            while (head < q.size()) {
-                const uint32_t u = q[head++];
-                for (uint32_t j = 0; j < gcnt[u]; ++j) {
-                    const uint32_t w = gid[(size_t)u * R + j];
-                    if (!flag[w]) reach(w);
-                }
-                for (uint32_t w : extra[u])
-                    if (!flag[w]) reach(w);
+        for (uint32_t x = 0; x < N; ++x) {
+            flag[x] = flag32[x] != 0;
+            if (flag[x]) {
+                if (dgr[x] < R && best == N) best = x;
+            } else {
+                ++boff[label[x] + 1];
             }
-            while (low < N && flag[low]) ++low;
-            if (low == N) break;
-            const uint32_t u = low;
+        }
+        for (uint32_t v = 0; v < N; ++v) boff[v + 1] += boff[v];
+        bnode.resize(boff[N]);
+        {
+            std::vector<uint32_t> cur(boff.begin(), boff.end() - 1);
+            for (uint32_t x = 0; x < N; ++x)
+                if (!flag[x]) bnode[cur[label[x]]++] = x;
+        }
+        for (uint32_t v = 0; v < N; ++v) {
+            if (flag[v] || label[v] != v) continue;  // not a `low` of this root
             if (best < N) {
                 const uint32_t j = best;
-                extra[j].push_back(u);
+                extra[j].push_back(v);
                 lu.push_back(j);
-                lv.push_back(u);
+                lv.push_back(v);
                 if (++dgr[j] >= R) {  // the next reached node with room: every one is above j
                     uint32_t b = j + 1;
                     while (b < N && !(flag[b] && dgr[b] < R)) ++b;
                     best = b;
                 }
             }
-            reach(u);
+            // v's closure among the unreached: the nodes labelled v
+            for (uint32_t i = boff[v]; i < boff[v + 1]; ++i) {
+                const uint32_t x = bnode[i];
+                flag[x] = 1;
+                if (dgr[x] < R && x < best) best = x;
+            }
         }
     }
     // the links' distances, the build-order way on the device
