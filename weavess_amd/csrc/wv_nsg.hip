@@ -734,6 +734,32 @@ __global__ void ssg_label_init_kernel(const uint32_t *flag, uint32_t *label, uin
     if (x < N) label[x] = flag[x] ? N : x;
 }
 
+// Closure of `root` over rows gid (gcnt ids of R slots) plus link lists
+// (CSR xoff / xid): reached flags (u32) into d_flag, a breadth-first level
+// per launch.  d_f0 / d_f1 hold N ids each, d_n one counter.
+hipError_t closure_dev(const uint32_t *gid, const uint32_t *gcnt, uint32_t R, const uint32_t *d_xoff,
+                       const uint32_t *d_xid, uint32_t N, uint32_t root, uint32_t *d_flag, uint32_t *d_f0,
+                       uint32_t *d_f1, uint32_t *d_n, hipStream_t s) {
+    hipError_t e = hipMemsetAsync(d_flag, 0, (size_t)N * 4, s);
+    const uint32_t one = 1u;
+    if (e == hipSuccess) e = hipMemcpyAsync(d_flag + root, &one, 4, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(d_f0, &root, 4, hipMemcpyHostToDevice, s);
+    uint32_t nin = 1;
+    uint32_t *fin = d_f0, *fout = d_f1;
+    while (e == hipSuccess && nin > 0) {
+        e = hipMemsetAsync(d_n, 0, 4, s);
+        if (e != hipSuccess) break;
+        hipLaunchKernelGGL(ssg_bfs_kernel, dim3((nin + 3) / 4), dim3(256), 0, s, gid, gcnt, R, d_xoff, d_xid, d_flag,
+                           fin, nin, fout, d_n);
+        e = hipGetLastError();
+        if (e == hipSuccess) e = hipMemcpyAsync(&nin, d_n, 4, hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        std::swap(fin, fout);
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(s);  // `one` and `root` outlive their copies
+    return e;
+}
+
 template <int CPL>
 hipError_t launch_ssg(const NsgArgs &a, hipStream_t s) {
     const size_t lds = (size_t)NWPB * a.lds_wave;
@@ -1060,9 +1086,22 @@ int nsg_build(wv_index *ix, const wv_nsg_params *p, wv_nsg_stats *stats, const u
     };
     std::vector<uint32_t> un;
     if (rule.conn) {
-        mark(ep);
-        for (uint32_t i = 0; i < N; ++i)
+        // what ep reaches, on the device (the later links' closures are small
+        // and stay on the host)
+        uint32_t *d_flag = nullptr, *d_f0 = nullptr, *d_f1 = nullptr, *d_n = nullptr, *d_xoff = nullptr;
+        NSG_HIP(B.get(&d_flag, N));
+        NSG_HIP(B.get(&d_f0, N));
+        NSG_HIP(B.get(&d_f1, N));
+        NSG_HIP(B.get(&d_n, 1));
+        NSG_HIP(B.get(&d_xoff, (size_t)N + 1));
+        NSG_HIP(hipMemsetAsync(d_xoff, 0, ((size_t)N + 1) * 4, s));
+        NSG_HIP(closure_dev(a.out_ids, a.out_cnt, R, d_xoff, nullptr, N, ep, d_flag, d_f0, d_f1, d_n, s));
+        std::vector<uint32_t> flag32(N);
+        NSG_HIP(hipMemcpy(flag32.data(), d_flag, (size_t)N * 4, hipMemcpyDeviceToHost));
+        for (uint32_t i = 0; i < N; ++i) {
+            flag[i] = flag32[i] != 0;
             if (!flag[i]) un.push_back(i);
+        }
     }
     uint32_t conn_edges = 0;
     if (!un.empty()) {
@@ -1259,21 +1298,7 @@ int ssg_build(wv_index *ix, const wv_ssg_params *p, uint32_t *eps_out, wv_nsg_st
         NSG_HIP(hipMemcpyAsync(d_xoff, xoff.data(), xoff.size() * 4, hipMemcpyHostToDevice, s));
         if (!xid.empty()) NSG_HIP(hipMemcpyAsync(d_xid, xid.data(), xid.size() * 4, hipMemcpyHostToDevice, s));
         // 1. the root's closure
-        NSG_HIP(hipMemsetAsync(d_flag, 0, (size_t)N * 4, s));
-        const uint32_t one = 1u, root = roots[r];
-        NSG_HIP(hipMemcpyAsync(d_flag + root, &one, 4, hipMemcpyHostToDevice, s));
-        NSG_HIP(hipMemcpyAsync(d_f0, &root, 4, hipMemcpyHostToDevice, s));
-        uint32_t nin = 1;
-        uint32_t *fin = d_f0, *fout = d_f1;
-        while (nin > 0) {
-            NSG_HIP(hipMemsetAsync(d_n, 0, 4, s));
-            hipLaunchKernelGGL(ssg_bfs_kernel, dim3((nin + 3) / 4), dim3(256), 0, s, a.out_ids, a.out_cnt, R, d_xoff,
-                               d_xid, d_flag, fin, nin, fout, d_n);
-            NSG_HIP(hipGetLastError());
-            NSG_HIP(hipMemcpyAsync(&nin, d_n, 4, hipMemcpyDeviceToHost, s));
-            NSG_HIP(hipStreamSynchronize(s));
-            std::swap(fin, fout);
-        }
+        NSG_HIP(closure_dev(a.out_ids, a.out_cnt, R, d_xoff, d_xid, N, roots[r], d_flag, d_f0, d_f1, d_n, s));
         // 2. labels of the unreached nodes (labels only drop, so this ends)
         hipLaunchKernelGGL(ssg_label_init_kernel, dim3((N + 255) / 256), dim3(256), 0, s, d_flag, d_label, N);
         NSG_HIP(hipGetLastError());
