@@ -968,6 +968,70 @@ void gen_random_host(uint64_t st, uint32_t size, uint32_t N, uint32_t *out) {
 
 }  // namespace
 
+// The result graph's rows, a wavefront per node: the node's list slots
+// (ids / dist, cnt of R), then ID_NONE / -1 up to W.
+__global__ void __launch_bounds__(256) result_rows_kernel(const uint32_t *ids, const float *dist,
+                                                          const uint32_t *cnt, uint32_t N, uint32_t R, uint32_t W,
+                                                          uint32_t *gid, float *gdist) {
+    const uint32_t i = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    if (i >= N) return;
+    const uint32_t c = cnt[i];
+    for (uint32_t j = (uint32_t)lane_id(); j < W; j += WAVE) {
+        const bool in = j < c;
+        gid[(size_t)i * W + j] = in ? ids[(size_t)i * R + j] : ID_NONE;
+        gdist[(size_t)i * W + j] = in ? dist[(size_t)i * R + j] : -1.0f;
+    }
+}
+
+// Links after the lists: slot lslot[k] of node lnode[k] takes lid[k], ldist[k].
+__global__ void result_links_kernel(const uint32_t *lnode, const uint32_t *lslot, const uint32_t *lid,
+                                    const float *ldist, uint32_t nl, uint32_t W, uint32_t *gid, float *gdist) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nl) return;
+    const size_t e = (size_t)lnode[k] * W + lslot[k];
+    gid[e] = lid[k];
+    gdist[e] = ldist[k];
+}
+
+// Links of a connectivity step, in the order they follow each node's list.
+struct LinkSet {
+    std::vector<uint32_t> node, slot, id;
+    std::vector<float> dist;
+};
+
+// Store the refined graph in the index, assembled on the device from the
+// lists in a (out_ids / out_dist / out_cnt) plus the links; deg is each
+// node's final degree, W the widest.
+int store_graph(wv_index *ix, const NsgArgs &a, uint32_t N, uint32_t R, uint32_t W, const std::vector<uint32_t> &deg,
+                const LinkSet &links, DBuf &B, hipStream_t s) {
+    uint32_t *gi = nullptr, *gg = nullptr;
+    float *gd = nullptr;
+    int rc = index_alloc_graph_dev(ix, W, &gi, &gd, &gg);  // frees the kNN graph
+    if (rc) return rc;
+    hipLaunchKernelGGL(result_rows_kernel, dim3((N + 3) / 4), dim3(256), 0, s, a.out_ids, a.out_dist, a.out_cnt, N, R,
+                       W, gi, gd);
+    NSG_HIP(hipGetLastError());
+    const uint32_t nl = (uint32_t)links.node.size();
+    if (nl) {
+        uint32_t *d_node = nullptr, *d_slot = nullptr, *d_id = nullptr;
+        float *d_dist = nullptr;
+        NSG_HIP(B.get(&d_node, nl));
+        NSG_HIP(B.get(&d_slot, nl));
+        NSG_HIP(B.get(&d_id, nl));
+        NSG_HIP(B.get(&d_dist, nl));
+        NSG_HIP(hipMemcpyAsync(d_node, links.node.data(), (size_t)nl * 4, hipMemcpyHostToDevice, s));
+        NSG_HIP(hipMemcpyAsync(d_slot, links.slot.data(), (size_t)nl * 4, hipMemcpyHostToDevice, s));
+        NSG_HIP(hipMemcpyAsync(d_id, links.id.data(), (size_t)nl * 4, hipMemcpyHostToDevice, s));
+        NSG_HIP(hipMemcpyAsync(d_dist, links.dist.data(), (size_t)nl * 4, hipMemcpyHostToDevice, s));
+        hipLaunchKernelGGL(result_links_kernel, dim3((nl + 255) / 256), dim3(256), 0, s, d_node, d_slot, d_id, d_dist,
+                           nl, W, gi, gd);
+        NSG_HIP(hipGetLastError());
+    }
+    NSG_HIP(hipMemcpyAsync(gg, deg.data(), (size_t)N * 4, hipMemcpyHostToDevice, s));
+    NSG_HIP(hipStreamSynchronize(s));
+    return WV_OK;
+}
+
 int nsg_build(wv_index *ix, const wv_nsg_params *p, wv_nsg_stats *stats, const uint32_t *knn, const uint32_t *kdeg,
               uint32_t kstride, const RefineRule &rule) {
     // knn / kdeg: the graph being refined, on the device (kstride ids a row)
@@ -1054,12 +1118,11 @@ int nsg_build(wv_index *ix, const wv_nsg_params *p, wv_nsg_stats *stats, const u
     }
     const double insert_ms = ms_since(t0);
 
-    // 4. connectivity (component_conn.cpp:14-98)
+    // 4. connectivity (component_conn.cpp:14-98); the lists come to the host
+    // only when links may be needed (their reach is marked there)
     t0 = std::chrono::high_resolution_clock::now();
-    std::vector<uint32_t> gid((size_t)N * R), gcnt(N);
-    std::vector<float> gdd((size_t)N * R);
-    NSG_HIP(hipMemcpy(gid.data(), a.out_ids, gid.size() * 4, hipMemcpyDeviceToHost));
-    NSG_HIP(hipMemcpy(gdd.data(), a.out_dist, gdd.size() * 4, hipMemcpyDeviceToHost));
+    std::vector<uint32_t> gid(rule.conn ? (size_t)N * R : 0), gcnt(N);
+    if (rule.conn) NSG_HIP(hipMemcpy(gid.data(), a.out_ids, gid.size() * 4, hipMemcpyDeviceToHost));
     NSG_HIP(hipMemcpy(gcnt.data(), a.out_cnt, (size_t)N * 4, hipMemcpyDeviceToHost));
     std::vector<std::vector<std::pair<uint32_t, float>>> extra(N);
     std::vector<uint8_t> flag(N, 0);
@@ -1170,25 +1233,21 @@ int nsg_build(wv_index *ix, const wv_nsg_params *p, wv_nsg_stats *stats, const u
     }
     // result graph: pruned + inserted lists, then the links
     uint32_t W = 1;
-    for (uint32_t i = 0; i < N; ++i) W = std::max<uint32_t>(W, gcnt[i] + (uint32_t)extra[i].size());
-    std::vector<uint32_t> fid((size_t)N * W, ID_NONE), fdeg(N);
-    std::vector<float> fdist((size_t)N * W, -1.0f);
+    std::vector<uint32_t> fdeg(N);
+    LinkSet links;
     for (uint32_t i = 0; i < N; ++i) {
-        uint32_t c = 0;
-        for (uint32_t j = 0; j < gcnt[i]; ++j, ++c) {
-            fid[(size_t)i * W + c] = gid[(size_t)i * R + j];
-            fdist[(size_t)i * W + c] = gdd[(size_t)i * R + j];
+        fdeg[i] = gcnt[i] + (uint32_t)extra[i].size();
+        W = std::max<uint32_t>(W, fdeg[i]);
+        for (size_t j = 0; j < extra[i].size(); ++j) {
+            links.node.push_back(i);
+            links.slot.push_back(gcnt[i] + (uint32_t)j);
+            links.id.push_back(extra[i][j].first);
+            links.dist.push_back(extra[i][j].second);
         }
-        for (const auto &e : extra[i]) {
-            fid[(size_t)i * W + c] = e.first;
-            fdist[(size_t)i * W + c] = e.second;
-            ++c;
-        }
-        fdeg[i] = c;
     }
-    const double conn_ms = ms_since(t0);
-    int rc = index_upload_graph(ix, fid.data(), fdist.data(), fdeg.data(), W);
+    int rc = store_graph(ix, a, N, R, W, fdeg, links, B, s);
     if (rc) return rc;
+    const double conn_ms = ms_since(t0);
     if (stats) {
         stats->total_ms = ms_since(t_all);
         stats->search_ms = search_ms;
@@ -1262,10 +1321,7 @@ int ssg_build(wv_index *ix, const wv_ssg_params *p, uint32_t *eps_out, wv_nsg_st
     //  3. the host walks the labels in ascending order, linking and reaching
     //     as the reference does.
     t0 = std::chrono::high_resolution_clock::now();
-    std::vector<uint32_t> gid((size_t)N * R), gcnt(N);
-    std::vector<float> gdd((size_t)N * R);
-    NSG_HIP(hipMemcpy(gid.data(), a.out_ids, gid.size() * 4, hipMemcpyDeviceToHost));
-    NSG_HIP(hipMemcpy(gdd.data(), a.out_dist, gdd.size() * 4, hipMemcpyDeviceToHost));
+    std::vector<uint32_t> gcnt(N);
     NSG_HIP(hipMemcpy(gcnt.data(), a.out_cnt, (size_t)N * 4, hipMemcpyDeviceToHost));
     std::vector<std::vector<uint32_t>> extra(N);
     std::vector<uint32_t> lu, lv;  // links lu[i] -> lv[i]
@@ -1366,27 +1422,21 @@ int ssg_build(wv_index *ix, const wv_ssg_params *p, uint32_t *eps_out, wv_nsg_st
         NSG_HIP(hipStreamSynchronize(s));
         NSG_HIP(hipMemcpy(lk.data(), d_k, lk.size() * 8, hipMemcpyDeviceToHost));
     }
-    std::vector<std::vector<float>> extra_d(N);
-    for (size_t i = 0; i < lu.size(); ++i) extra_d[lu[i]].push_back(key_dist(lk[i]));
-    uint32_t W = 1;
-    for (uint32_t i = 0; i < N; ++i) W = std::max<uint32_t>(W, gcnt[i] + (uint32_t)extra[i].size());
-    std::vector<uint32_t> fid((size_t)N * W, ID_NONE), fdeg(N);
-    std::vector<float> fdist((size_t)N * W, -1.0f);
-    for (uint32_t i = 0; i < N; ++i) {
-        uint32_t c = 0;
-        for (uint32_t j = 0; j < gcnt[i]; ++j, ++c) {
-            fid[(size_t)i * W + c] = gid[(size_t)i * R + j];
-            fdist[(size_t)i * W + c] = gdd[(size_t)i * R + j];
-        }
-        for (size_t j = 0; j < extra[i].size(); ++j, ++c) {
-            fid[(size_t)i * W + c] = extra[i][j];
-            fdist[(size_t)i * W + c] = extra_d[i][j];
-        }
-        fdeg[i] = c;
+    // result graph: pruned + inserted lists, then each node's links in the
+    // order they were made
+    std::vector<uint32_t> fdeg(gcnt);
+    LinkSet links;
+    for (size_t i = 0; i < lu.size(); ++i) {
+        links.node.push_back(lu[i]);
+        links.slot.push_back(fdeg[lu[i]]++);
+        links.id.push_back(lv[i]);
+        links.dist.push_back(key_dist(lk[i]));
     }
-    const double conn_ms = ms_since(t0);
-    rc = index_upload_graph(ix, fid.data(), fdist.data(), fdeg.data(), W);
+    uint32_t W = 1;
+    for (uint32_t i = 0; i < N; ++i) W = std::max<uint32_t>(W, fdeg[i]);
+    rc = store_graph(ix, a, N, R, W, fdeg, links, B, s);
     if (rc) return rc;
+    const double conn_ms = ms_since(t0);
     if (eps_out) std::memcpy(eps_out, roots.data(), (size_t)p->n_try * 4);
     if (stats) {
         *stats = wv_nsg_stats{};
