@@ -391,19 +391,25 @@ int search_collect(wv_index *ix, const uint32_t *graph, const uint32_t *deg, uin
                    uint32_t *d_evals) {
     wv_ctx *c = ix->ctx;
     if (Q == 0) return WV_OK;
-    uint32_t *g_save = ix->graph, *d_save = ix->deg, m_save = ix->deg_max;
+    uint32_t *g_save = ix->graph, *d_save = ix->deg, m_save = ix->deg_max, h_save = ix->cfg.visited_slots;
     ix->graph = const_cast<uint32_t *>(graph);
     ix->deg = const_cast<uint32_t *>(deg);
     ix->deg_max = deg_max;
+    // The collected candidates do not depend on the visited table; at
+    // L <= 256 a 1024-slot table leaves more queries in flight than the
+    // 2048 slots that keep plain search's evaluation counts exact (NSG at
+    // L=150 on 1M x 128: 0.45 s of searching instead of 0.65 s).
+    if (h_save == 0 && L <= 256) ix->cfg.visited_slots = 1024;
     struct Restore {
         wv_index *ix;
-        uint32_t *g, *d, m;
+        uint32_t *g, *d, m, h;
         ~Restore() {
             ix->graph = g;
             ix->deg = d;
             ix->deg_max = m;
+            ix->cfg.visited_slots = h;
         }
-    } restore{ix, g_save, d_save, m_save};
+    } restore{ix, g_save, d_save, m_save, h_save};
     DevBuf ids;
     WV_HIP(c, ids.alloc((size_t)Q * 4));
     SearchExtDev x;
