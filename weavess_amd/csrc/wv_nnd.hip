@@ -22,9 +22,10 @@
 // Iteration it (0-based; it = 0's join is empty, component_refine.cpp:90
 // with include/weavess/index.h:98-103):
 //   join   one wavefront per node: new x new (i < j) and new x old Gram
-//          blocks on MFMA from LDS-staged 16-row tiles, d = |a|^2 + |b|^2 -
-//          2 a.b with precomputed squared norms; proposals at or below the
-//          target's threshold go to its candidate slots;
+//          blocks of 16-row tiles on MFMA, operands loaded straight into
+//          registers, d = |a|^2 + |b|^2 - 2 a.b with precomputed squared
+//          norms; proposals at or below the target's threshold queue in
+//          LDS and go to its candidate slots 64 at a time;
 //   merge  one wavefront per node: pool U slot winners (duplicates by id
 //          dropped, pool entry wins), bitonic sort, first L kept, winners
 //          flagged new, slots cleared, new M;
@@ -1189,7 +1190,8 @@ constexpr int WPB = 4;       // wavefronts (nodes) per workgroup
 constexpr int JOIN_WPB = 2;  // the join holds its lists and proposal ring in LDS
 
 // The join kernel for this element type: register operands of at most ts
-// MFMA steps (rows of one K chunk), or the LDS-staged form (ts == 0).
+// MFMA steps (rows of at most 512 bytes), or rows streamed through
+// registers a piece at a time (ts == 0).
 using JoinFn = void (*)(NndDev);
 template <int DT>
 JoinFn join_kernel(int ts) {
