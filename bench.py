@@ -35,6 +35,7 @@ Prints ONE JSON line on stdout; progress goes to stderr.
 L=R=64 graph) or the 10k x 128 integer mixture (K=50, L=70, S=10, R=100).
 --workload c3: NN-Descent precision on 1M x 960 (FP32 / BF16 / FP8 joins):
 graph recall and distance error against FP64.
+--workload nnd: the NN-Descent leg alone.
 --workload c4: one batched launch over 200,000 queries with per-query
 L ~ U{16..400} and k in {1, 10, 100} on the 2M x 96 graph.
 """
@@ -221,6 +222,15 @@ def run_c3(args, ctx, Index, datagen):
     print(json.dumps(out), flush=True)
 
 
+def run_nnd(args, ctx, Index, datagen):
+    """The NN-Descent leg alone (FP32 and BF16 builds of the 1M x 128 mixture)."""
+    leg = nndescent_leg(ctx, Index, datagen)
+    out = {"metric": "NN-Descent build s @ 0.95 graph recall", "value": leg["bf16"]["build_s"], "unit": "s (BF16 join)",
+           "n_gpus": 1, "higher_is_better": False, "dtype": "bf16", "data": "synthetic, seeded (gmm_c2)",
+           "config": {"workload": "nnd: " + NND_LEG["desc"]}, "nndescent": leg}
+    print(json.dumps(out), flush=True)
+
+
 def run_c4(args, ctx, Index, datagen):
     """Batching: one launch over 200,000 queries with per-query L ~ U{16..400}
     and k in {1, 10, 100} (L_eff = max(L, k)) on the 2M x 96 graph."""
@@ -398,7 +408,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="c5", choices=sorted(WORKLOADS) + ["c3", "c4", "refine"])
+    ap.add_argument("--workload", default="c5", choices=sorted(WORKLOADS) + ["c3", "c4", "refine", "nnd"])
     ap.add_argument("--target", type=float, default=0.95)
     ap.add_argument("--no-cpu", action="store_true", help="skip the host-core baseline")
     ap.add_argument("--no-nnd-leg", action="store_true", help="skip the 1M x 128 NN-Descent leg")
@@ -412,8 +422,8 @@ def main():
     from weavess_amd.devmem import DeviceArray, host_cores, pad_rows, synchronize
 
     ctx = Context(0)
-    if args.workload in ("c3", "c4", "refine"):
-        {"c3": run_c3, "c4": run_c4, "refine": run_refine}[args.workload](args, ctx, Index, datagen)
+    if args.workload in ("c3", "c4", "refine", "nnd"):
+        {"c3": run_c3, "c4": run_c4, "refine": run_refine, "nnd": run_nnd}[args.workload](args, ctx, Index, datagen)
         ctx.close()
         return
     w = WORKLOADS[args.workload]

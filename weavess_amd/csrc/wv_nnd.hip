@@ -8,7 +8,9 @@
 //   pool[n][0..L)   ascending (dist,id) keys of its neighbour pool, unused
 //                   slots KEY_EMPTY; pflag = "new" flags; pcnt = size
 //   M[n]            sampling depth of update() (component_refine.cpp:167-175)
-//   thr[n]          worst key of the pool (join acceptance threshold)
+//   rec[n]          what a join reads of a listed node, in one 16-byte
+//                   record: thr, the worst key of the pool (acceptance
+//                   threshold), the join row's squared norm and its FP8 scale
 //   worst[n]        its distance (reverse-sample test, :191/:202)
 //   fnew/fold       forward new/old samples (keys) of the last update
 //   rv_*            reverse proposals in CSR form, rsel_* the kept ones
@@ -57,21 +59,27 @@ __device__ inline unsigned long long *ctr_slot(unsigned long long *ctr) {
 }
 constexpr float FP8_MAX = 448.0f;  // largest finite e4m3 magnitude
 
+// A node's join-time scalars, fetched with one 16-byte load per list entry
+// (separate arrays cost a cache line each per random read).
+struct alignas(16) NodeRec {
+    uint64_t thr;  // worst key of the pool
+    float norm;    // squared norm of the join row
+    float scale;   // FP8 row scale (1 for FP32 / BF16)
+};
+
 struct NndDev {
     const float *base;     // N x dp FP32 rows (zero padded)
     uint32_t dp;
     const uint8_t *xq;     // join rows in join precision
     uint32_t xq_stride;    // bytes per join row
     uint32_t kdim;         // elements per join row (FP32: dp; BF16/FP8: multiple of 32)
-    const float *norm;     // squared norms of the join rows
-    const float *scale;    // FP8 per-row scale, else null
     uint32_t N, L, S, R, K, C, rerank;
     uint32_t iter;
     uint64_t seed;
     uint64_t *pool;
     uint8_t *pflag;
     uint32_t *pcnt, *M;
-    uint64_t *thr;
+    NodeRec *rec;
     float *worst;
     uint64_t *fnew, *fold;
     uint32_t *fnew_cnt, *fold_cnt;
@@ -179,7 +187,7 @@ __global__ void scan_final_kernel(const uint32_t *in, uint32_t n, const uint32_t
 // Squared norms of the join rows; for BF16/FP8 also the quantised rows.
 // 16 lanes per row, chunk c of lane j = j + 16c.
 template <int DT>
-__global__ void nnd_prep_kernel(NndDev a, uint8_t *xq_out, float *norm_out, float *scale_out) {
+__global__ void nnd_prep_kernel(NndDev a, uint8_t *xq_out) {
     const uint32_t row = (blockIdx.x * blockDim.x + threadIdx.x) >> 4;
     const int sub = threadIdx.x & 15;
     const bool live = row < a.N;
@@ -225,8 +233,8 @@ __global__ void nnd_prep_kernel(NndDev a, uint8_t *xq_out, float *norm_out, floa
     }
     acc = fold16(acc);
     if (live && sub == 0) {
-        norm_out[r] = (DT == DT_FP8) ? acc * sc * sc : acc;
-        if (DT == DT_FP8) scale_out[r] = sc;
+        a.rec[r].norm = (DT == DT_FP8) ? acc * sc * sc : acc;
+        a.rec[r].scale = sc;
     }
 }
 
@@ -315,7 +323,7 @@ __global__ void __launch_bounds__(256) nnd_init_kernel(NndDev a, const uint32_t 
     if (lane == 0) {
         a.pcnt[n] = (uint32_t)m;
         a.M[n] = S;
-        a.thr[n] = m > 0 ? last : KEY_EMPTY;
+        a.rec[n].thr = m > 0 ? last : KEY_EMPTY;
         a.worst[n] = m > 0 ? key_dist(last) : -1.0f;
     }
 }
@@ -394,7 +402,7 @@ __global__ void __launch_bounds__(256) nnd_init_rows_kernel(NndDev a, const uint
         const uint64_t last = m > 0 ? keys[m - 1] : KEY_EMPTY;
         a.pcnt[n] = m;
         a.M[n] = a.S;
-        a.thr[n] = last;
+        a.rec[n].thr = last;
         a.worst[n] = m > 0 ? key_dist(last) : -1.0f;
     }
 }
@@ -882,9 +890,10 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) n
         else if (i - nn < foc) id = key_id(a.fold[(size_t)n * a.L + (i - nn)]);
         else id = a.rsel_old[(size_t)n * a.R + (i - nn - foc)];
         col[i] = id;
-        cthr[i] = a.thr[id];
-        cnorm[i] = a.norm[id];
-        if (DT == DT_FP8) cscale[i] = a.scale[id];  // only the FP8 list has this array
+        const uint4 e = *reinterpret_cast<const uint4 *>(a.rec + id);  // thr, norm, scale in one load
+        cthr[i] = (uint64_t)e.x | ((uint64_t)e.y << 32);
+        cnorm[i] = __uint_as_float(e.z);
+        if (DT == DT_FP8) cscale[i] = __uint_as_float(e.w);  // only the FP8 list has this array
     }
     wave_sync();
 
@@ -1105,7 +1114,7 @@ __global__ void __launch_bounds__(256) nnd_merge_kernel(NndDev a, uint32_t up2, 
     fresh = wave_sum_u32(fresh);
     if (lane == 0) {
         a.pcnt[n] = keep;
-        a.thr[n] = keep ? w[keep - 1] : KEY_EMPTY;
+        a.rec[n].thr = keep ? w[keep - 1] : KEY_EMPTY;
         a.worst[n] = keep ? key_dist(w[keep - 1]) : -1.0f;
         a.M[n] = m;
         atomicAdd(&ctr_slot(a.ctr)[3], (unsigned long long)fresh);
@@ -1278,9 +1287,8 @@ int nndescent_build(wv_index *ix, const wv_nnd_params *p, const uint32_t *init_i
     a.dp = v.dp;
     a.N = N; a.L = L; a.S = S; a.R = R; a.K = K; a.C = C; a.rerank = rer;
     a.seed = p->seed;
-    float *norm = nullptr, *scale = nullptr;
     uint8_t *xq = nullptr;
-    NND_HIP(B.get(&norm, N));
+    NND_HIP(B.get(&a.rec, N));
     if (dt == DT_F32) {
         a.xq = reinterpret_cast<const uint8_t *>(v.base);
         a.xq_stride = v.dp * 4;
@@ -1291,15 +1299,11 @@ int nndescent_build(wv_index *ix, const wv_nnd_params *p, const uint32_t *init_i
         NND_HIP(B.get(&xq, (size_t)N * a.xq_stride));
         NND_HIP(hipMemsetAsync(xq, 0, (size_t)N * a.xq_stride, s));
         a.xq = xq;
-        if (dt == DT_FP8) NND_HIP(B.get(&scale, N));
     }
-    a.norm = norm;
-    a.scale = scale;
     NND_HIP(B.get(&a.pool, (size_t)N * L));
     NND_HIP(B.get(&a.pflag, (size_t)N * L));
     NND_HIP(B.get(&a.pcnt, N));
     NND_HIP(B.get(&a.M, N));
-    NND_HIP(B.get(&a.thr, N));
     NND_HIP(B.get(&a.worst, N));
     a.fst = efanna ? 2 * S : S;
     NND_HIP(B.get(&a.fnew, (size_t)N * a.fst));
@@ -1374,11 +1378,11 @@ int nndescent_build(wv_index *ix, const wv_nnd_params *p, const uint32_t *init_i
     {
         const uint32_t pb = (N * 16 + 255) / 256;
         if (dt == DT_F32)
-            hipLaunchKernelGGL(nnd_prep_kernel<DT_F32>, dim3(pb), dim3(256), 0, s, a, xq, norm, scale);
+            hipLaunchKernelGGL(nnd_prep_kernel<DT_F32>, dim3(pb), dim3(256), 0, s, a, xq);
         else if (dt == DT_BF16)
-            hipLaunchKernelGGL(nnd_prep_kernel<DT_BF16>, dim3(pb), dim3(256), 0, s, a, xq, norm, scale);
+            hipLaunchKernelGGL(nnd_prep_kernel<DT_BF16>, dim3(pb), dim3(256), 0, s, a, xq);
         else
-            hipLaunchKernelGGL(nnd_prep_kernel<DT_FP8>, dim3(pb), dim3(256), 0, s, a, xq, norm, scale);
+            hipLaunchKernelGGL(nnd_prep_kernel<DT_FP8>, dim3(pb), dim3(256), 0, s, a, xq);
         NND_HIP(hipGetLastError());
     }
     uint32_t *d_init = nullptr, *d_icnt = nullptr;
