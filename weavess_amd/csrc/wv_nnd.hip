@@ -878,11 +878,25 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) n
     const uint32_t nn = fnc + rnc;
     const uint32_t no = min(foc + roc, 2 * a.R);
     if (nn == 0) return;
-    for (uint32_t i = lane; i < a.nw_pow2; i += WAVE)
-        nw[i] = i < fnc ? key_id(a.fnew[(size_t)n * a.fst + i])
-              : i < nn ? a.rsel_new[(size_t)n * a.R + (i - fnc)] : ID_NONE;
+    // the new ids, ascending (position order == id order): in registers
+    // when they fit one wavefront, else in LDS over the width in use
+    if (nn <= (uint32_t)WAVE) {
+        const uint32_t i = (uint32_t)lane;
+        const uint32_t v = i < fnc ? key_id(a.fnew[(size_t)n * a.fst + i])
+                         : i < nn ? a.rsel_new[(size_t)n * a.R + (i - fnc)] : ID_NONE;
+        int w = 2;
+        while ((uint32_t)w < nn) w <<= 1;
+        nw[i] = (uint32_t)wave_sort64_n((uint64_t)v, w);  // lanes from w up hold ID_NONE
+    } else {
+        int pw = 2 * WAVE;  // <= nw_pow2
+        while ((uint32_t)pw < nn) pw <<= 1;
+        for (uint32_t i = lane; i < (uint32_t)pw; i += WAVE)
+            nw[i] = i < fnc ? key_id(a.fnew[(size_t)n * a.fst + i])
+                  : i < nn ? a.rsel_new[(size_t)n * a.R + (i - fnc)] : ID_NONE;
+        wave_sync();
+        lds_sort32(nw, pw);
+    }
     wave_sync();
-    lds_sort32(nw, (int)a.nw_pow2);  // ascending ids: position order == id order
     const uint32_t nc = nn + no;
     for (uint32_t i = lane; i < nc; i += WAVE) {
         uint32_t id;
