@@ -148,4 +148,56 @@ __device__ inline int merge_distinct(uint64_t *full, int len, int cap, const uin
     return min(len + c, cap);
 }
 
+// Collection staged in blocks of 64: keys of `blk` (m <= 64) that can still
+// enter the sorted distinct top-cap list `full` wait in `stg` (128 keys, sn
+// of them); every 64 staged keys are merged at once.  A key at or above a
+// full list's last entry is dropped on arrival: that entry only falls.  The
+// final list is the same as merging every block as it comes (the cap
+// smallest distinct keys seen), for fewer passes over `full`.
+__device__ inline int merge_staged(uint64_t *full, int len, int cap, const uint64_t *stg, int m, uint64_t *tmp) {
+    const int lane = lane_id();
+    // sorted, then repeated keys (one id evaluated twice) dropped
+    uint64_t k = wave_sort64(lane < m ? stg[lane] : KEY_EMPTY);
+    const uint64_t prev = shfl64(k, lane > 0 ? lane - 1 : 0);
+    if (lane > 0 && prev == k) k = KEY_EMPTY;
+    if (k != KEY_EMPTY) {
+        if (len == cap && k >= full[cap - 1]) {
+            k = KEY_EMPTY;
+        } else {
+            const int p = lower_bound_e(full, len, k);
+            if (p < len && full[p] == k) k = KEY_EMPTY;
+        }
+    }
+    const uint64_t lm = __ballot(k != KEY_EMPTY);
+    if (lm == 0) return len;
+    const int c = __popcll(lm);
+    wave_sync();
+    if (k != KEY_EMPTY) tmp[lanes_below(lm)] = k;  // still ascending
+    wave_sync();
+    merge_into(full, len, cap, tmp, c, false);
+    return min(len + c, cap);
+}
+__device__ inline void collect_stage(uint64_t *full, int &len, int cap, const uint64_t *blk, int m, uint64_t *stg,
+                                     int &sn, uint64_t *tmp) {
+    const int lane = lane_id();
+    uint64_t k = lane < m ? blk[lane] : KEY_EMPTY;
+    if (k != KEY_EMPTY && len == cap && k >= full[cap - 1]) k = KEY_EMPTY;
+    const uint64_t b = __ballot(k != KEY_EMPTY);
+    if (k != KEY_EMPTY) stg[sn + (int)lanes_below(b)] = k;
+    sn += __popcll(b);
+    if (sn >= WAVE) {
+        wave_sync();
+        len = merge_staged(full, len, cap, stg, WAVE, tmp);
+        const uint64_t r = lane + WAVE < sn ? stg[lane + WAVE] : KEY_EMPTY;
+        wave_sync();
+        if (lane + WAVE < sn) stg[lane] = r;
+        sn -= WAVE;
+        wave_sync();
+    }
+}
+__device__ inline int collect_flush(uint64_t *full, int len, int cap, const uint64_t *stg, int sn, uint64_t *tmp) {
+    wave_sync();
+    return sn > 0 ? merge_staged(full, len, cap, stg, sn, tmp) : len;
+}
+
 }  // namespace wv

@@ -108,7 +108,8 @@ __global__ void __launch_bounds__(256) greedy_search_kernel(SearchArgs a) {
     uint64_t *buf = reinterpret_cast<uint64_t *>(vis + a.H);    // B
     uint64_t *full = buf + a.B;                                 // C (collect mode)
     uint64_t *fblk = full + a.C;                                // 64 (collect mode)
-    int flen = 0;
+    uint64_t *fstg = fblk + WAVE;                               // 128 (collect mode)
+    int flen = 0, fsn = 0;
 
     const uint32_t N = a.N, nch = a.nch, dp = a.dp;
     // per-query budgets were checked on the host when the launch was planned;
@@ -149,7 +150,7 @@ __global__ void __launch_bounds__(256) greedy_search_kernel(SearchArgs a) {
             wave_sync();
             rows_dist<CPL, G>(a.base, dp, nch, qv, cid, cnt, ckey);
             wave_sync();
-            if (COLLECT) flen = merge_distinct(full, flen, (int)a.C, ckey, cnt, fblk);
+            if (COLLECT) collect_stage(full, flen, (int)a.C, ckey, cnt, fstg, fsn, fblk);
             const uint64_t key = wave_sort64(lane < cnt ? ckey[lane] : KEY_EMPTY);
             wave_sync();
             ckey[lane] = key;
@@ -240,7 +241,7 @@ __global__ void __launch_bounds__(256) greedy_search_kernel(SearchArgs a) {
             // way, so stop probing deeply
             if (evals - L > a.H) probes = VIS_PROBES_FULL;
             wave_sync();
-            if (COLLECT) flen = merge_distinct(full, flen, (int)a.C, ckey, m, fblk);
+            if (COLLECT) collect_stage(full, flen, (int)a.C, ckey, m, fstg, fsn, fblk);
             uint64_t key = lane < m ? ckey[lane] : KEY_EMPTY;
             if (key < worst) {
                 // already in pool or buffer: a duplicate.  Both binary searches
@@ -299,8 +300,11 @@ __global__ void __launch_bounds__(256) greedy_search_kernel(SearchArgs a) {
         }
     }
     if (nb) merge_into(pool, Li, Li, buf, nb, false);
-    if (COLLECT)
+    if (COLLECT) {
+        flen = collect_flush(full, flen, (int)a.C, fstg, fsn, fblk);
+        wave_sync();
         for (uint32_t i = lane; i < a.C; i += WAVE) a.full[(size_t)q * a.C + i] = (int)i < flen ? full[i] : KEY_EMPTY;
+    }
 
     for (uint32_t i = lane; i < a.k_stride; i += WAVE) {
         const bool ok = i < kq;
@@ -333,7 +337,7 @@ hipError_t launch_t(const SearchArgs &a, int waves_per_block, hipStream_t s) {
 
 size_t search_wave_lds(uint32_t P, uint32_t H, uint32_t B, uint32_t C) {
     size_t b = (size_t)P * 8 + WAVE * 8 + WAVE * 4 + (size_t)H * 4 + (size_t)B * 8;
-    if (C) b += (size_t)C * 8 + WAVE * 8;
+    if (C) b += (size_t)C * 8 + WAVE * 8 + 2 * WAVE * 8;
     return (b + 15) & ~(size_t)15;
 }
 
