@@ -851,6 +851,11 @@ __device__ inline void tile_proposals(const NndDev &a, JoinQ &Q, const uint32_t 
     }
 }
 
+// Wavefronts per SIMD the join is compiled for: 4 (at most 128 VGPRs),
+// except FP32 rows of 5-8 MFMA steps, whose 32-VGPR tiles spill under that
+// cap; at 3 they do not, and that measured 2 % faster.
+template <int DT, int TS> struct JoinWaves { static constexpr int value = (DT == DT_F32 && TS == 8) ? 3 : 4; };
+
 // join(), src/component_refine.cpp:127-143 / nhood::join,
 // include/weavess/index.h:137-149, one wavefront per node.
 // TS > 0: rows fit one K chunk of at most TS MFMA steps; operands are
@@ -858,7 +863,8 @@ __device__ inline void tile_proposals(const NndDev &a, JoinQ &Q, const uint32_t 
 // column tile loads while the current one's proposals are made.
 // TS == 0: longer rows, streamed through registers a piece at a time.
 template <int DT, int TS>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) nnd_join_kernel(NndDev a) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(JoinWaves<DT, TS>::value)))
+nnd_join_kernel(NndDev a) {
     extern __shared__ __align__(16) uint8_t smem[];
     constexpr int KSTEP = JoinT<DT>::KSTEP;
     const int wid = threadIdx.x >> 6, lane = lane_id();
