@@ -175,9 +175,9 @@ __device__ int occlude(const float *base, uint32_t dp, uint32_t nch, const uint6
 // the block's live candidates, then the block's candidates are taken in
 // order, each kept one passing over the rest of the block.  A candidate is
 // kept or dropped by the same keys as in `occlude`, so the lists are equal;
-// the work stops at the block where the R-th key is kept.  That pays most
-// when the rule drops few candidates (Vamana's alpha > 1), and still at
-// alpha = 1 (NSG, Vamana's first pass); SSG's angle rule keeps `occlude`.
+// the work stops at the block where the R-th key is kept.  That pays when
+// the rule drops few candidates (Vamana's alpha > 1) or when the R keys
+// come early (the reverse re-prunes); SSG's angle rule keeps `occlude`.
 template <int CPL>
 __device__ int occlude_blocked(const float *base, uint32_t dp, uint32_t nch, const uint64_t *cand, int m,
                                uint8_t *st, int R, float alpha, bool fill, uint32_t *cid, uint32_t *cpos,
@@ -322,8 +322,13 @@ __global__ void __launch_bounds__(256) nsg_prune_kernel(NsgArgs a) {
     }
     for (int i = lane; i < len; i += WAVE) alive[i] = key_id(cand[i]) != n ? ST_LIVE : ST_SELF;
     wave_sync();
-    const int cnt = occlude_blocked<CPL>(a.base, a.dp, a.nch, cand, len, alive, (int)a.R, a.alpha, a.fill != 0, cid,
-                                         cpos, ckey, res);
+    // blocked when the rule drops few (alpha > 1); at alpha = 1 over a
+    // random initial graph's candidates the push form measured faster
+    const int cnt = a.alpha > 1.0f
+                        ? occlude_blocked<CPL>(a.base, a.dp, a.nch, cand, len, alive, (int)a.R, a.alpha, a.fill != 0,
+                                               cid, cpos, ckey, res)
+                        : occlude<CPL>(a.base, a.dp, a.nch, cand, len, alive, (int)a.R, a.alpha, a.fill != 0, cid,
+                                       cpos, ckey, res);
     for (int i = lane; i < cnt; i += WAVE) {
         a.pr_ids[(size_t)n * a.R + i] = key_id(res[i]);
         a.pr_dist[(size_t)n * a.R + i] = key_dist(res[i]);
