@@ -76,6 +76,7 @@ struct NsgArgs {
 
 template <int CPL>
 constexpr int rows_g() { return CPL >= 16 ? 1 : CPL >= 8 ? 2 : 4; }
+constexpr int DPG_NQ = 4;  // DPG's prune: earlier entries per pass over the later ones
 
 template <int CPL>
 __device__ inline void load_row(const float *base, uint32_t dp, uint32_t nch, uint32_t id, float4 (&qv)[CPL]) {
@@ -790,6 +791,42 @@ hipError_t pair_keys(int cpl, const float *base, uint32_t dp, uint32_t nch, cons
 // earlier entries i with d(p_i, p_j) < d(n, p_j); cut is the (range+1)-th
 // smallest hit; the entries with hit <= cut are kept in order, at most
 // range (all of them when len <= range).
+// rows_dist for NQ query rows at once: each gathered row serves all of
+// them.  dout[q * 64 + r] is row r's distance to query q, computed with
+// rows_dist's operations in rows_dist's order.
+template <int CPL, int G, int NQ>
+__device__ inline void rows_dist_multi(const float *__restrict__ base, uint32_t dp, uint32_t nch,
+                                       const float4 (&qv)[NQ][CPL], const uint32_t *ids, int m, float *dout) {
+    const int lane = lane_id(), sub = lane & 15, grp = lane >> 4;
+    for (int r0 = 0; r0 < m; r0 += 4 * G) {
+        float4 v[G][CPL];
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+            const int row = r0 + 4 * g + grp;
+            const uint32_t id = row < m ? ids[row] : 0u;
+            const float4 *p = reinterpret_cast<const float4 *>(base + (size_t)id * dp);
+#pragma unroll
+            for (int c = 0; c < CPL; ++c) {
+                const int ch = sub + 16 * c;
+                v[g][c] = (row < m && ch < (int)nch) ? p[ch] : make_float4(0.f, 0.f, 0.f, 0.f);
+            }
+        }
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+            const int row = r0 + 4 * g + grp;
+#pragma unroll
+            for (int q = 0; q < NQ; ++q) {
+                float acc = 0.f;
+#pragma unroll
+                for (int c = 0; c < CPL; ++c)
+                    if (sub + 16 * c < (int)nch) acc = acc + chunk_sq(qv[q][c], v[g][c]);
+                acc = fold16(acc);
+                if (sub == 0 && row < m) dout[q * WAVE + row] = acc;
+            }
+        }
+    }
+}
+
 template <int CPL>
 __global__ void __launch_bounds__(256) dpg_prune_kernel(NsgArgs a) {
     extern __shared__ __align__(16) uint8_t smem[];
@@ -831,16 +868,29 @@ __global__ void __launch_bounds__(256) dpg_prune_kernel(NsgArgs a) {
     } else {
         for (int i = lane; i < len; i += WAVE) hit[i] = 0;
         wave_sync();
-        for (int i = 0; i + 1 < len; ++i) {
-            float4 pv[CPL];
-            load_row<CPL>(a.base, a.dp, a.nch, key_id(p[i]), pv);
-            for (int t0 = i + 1; t0 < len; t0 += WAVE) {
+        // entries i .. i + DPG_NQ - 1 against the later ones at once: each
+        // later row is gathered once for the group (ckey and tmp, 1 KB,
+        // hold the group's distances)
+        float *dout = reinterpret_cast<float *>(ckey);
+        for (int i0 = 0; i0 + 1 < len; i0 += DPG_NQ) {
+            float4 pv[DPG_NQ][CPL];
+#pragma unroll
+            for (int k = 0; k < DPG_NQ; ++k) load_row<CPL>(a.base, a.dp, a.nch, key_id(p[min(i0 + k, len - 1)]), pv[k]);
+            for (int t0 = i0 + 1; t0 < len; t0 += WAVE) {
                 const int m = min(WAVE, len - t0);
                 if (lane < m) cid[lane] = key_id(p[t0 + lane]);
                 wave_sync();
-                rows_dist<CPL, rows_g<CPL>()>(a.base, a.dp, a.nch, pv, cid, m, ckey);
+                rows_dist_multi<CPL, rows_g<CPL>(), DPG_NQ>(a.base, a.dp, a.nch, pv, cid, m, dout);
                 wave_sync();
-                if (lane < m && key_dist(ckey[lane]) < key_dist(p[t0 + lane])) hit[t0 + lane] += 1;
+                if (lane < m) {
+                    const int j = t0 + lane;
+                    const float dn = key_dist(p[j]);
+                    uint32_t h = 0;
+#pragma unroll
+                    for (int k = 0; k < DPG_NQ; ++k)  // pairs i < j, i <= len - 2
+                        if (i0 + k < j && i0 + k + 1 < len && dout[k * WAVE + lane] < dn) ++h;
+                    hit[j] += h;
+                }
                 wave_sync();
             }
         }
