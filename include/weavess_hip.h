@@ -205,6 +205,8 @@ typedef struct {
 } wv_nsg_params;
 
 typedef struct {
+    /* stage times; conn_ms covers the connectivity links and storing the
+     * result rows in the index */
     double total_ms, search_ms, prune_ms, insert_ms, conn_ms;
     uint64_t evals;       /* distance evaluations of the candidate searches */
     uint32_t conn_edges;  /* edges added to reach every node from ep        */
