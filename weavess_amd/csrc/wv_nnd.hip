@@ -1216,7 +1216,7 @@ struct Bufs {
 };
 
 constexpr int WPB = 4;       // wavefronts (nodes) per workgroup
-constexpr int JOIN_WPB = 2;  // the join holds its lists and proposal ring in LDS
+constexpr int JOIN_WPB = 1;  // one node a workgroup: a finished wavefront frees its slot at once
 
 // The join kernel for this element type: register operands of at most ts
 // MFMA steps (rows of at most 512 bytes), or rows streamed through
