@@ -35,7 +35,7 @@ constexpr uint64_t TAG_NSGC = 0x4E534743ull;  // "NSGC": candidate-search entrie
 constexpr uint64_t TAG_NSGR = 0x4E534752ull;  // "NSGR": connectivity fallback root
 constexpr uint64_t TAG_SSGE = 0x53534745ull;  // "SSGE": SSG's n_try roots
 constexpr uint32_t NSG_TCAP = 1024;           // candidates of one reverse-insert re-prune
-constexpr int NWPB = 4;                       // wavefronts per block
+constexpr int NWPB = 1;                       // wavefronts per block: one node each, a finished one frees its slot at once
 
 #define NSG_HIP(call)                                                                        \
     do {                                                                                     \
