@@ -929,30 +929,40 @@ __global__ void dpg_union_fill_kernel(const uint32_t *pr_ids, const float *pr_di
     }
 }
 
-// Distinct keys of each sorted segment: count (write == 0) or write the
-// padded row of width W.
-__global__ void dpg_union_rows_kernel(const uint64_t *keys, const uint32_t *off, uint32_t N, uint32_t *ucnt,
-                                      uint32_t W, uint32_t *gid, float *gdist, uint32_t *gdeg, int write) {
-    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < N; t += gridDim.x * blockDim.x) {
+// Distinct keys of each sorted segment, one wavefront a node: count
+// (write == 0) or write the padded row of width W.  Lanes take 64 keys at a
+// time; a key is kept when it differs from the key before it in the segment
+// (the first against KEY_EMPTY), so each kept key's column is the number of
+// kept keys before it, and the row and its padding are stored coalesced.
+__global__ void __launch_bounds__(256) dpg_union_rows_kernel(const uint64_t *keys, const uint32_t *off, uint32_t N,
+                                                             uint32_t *ucnt, uint32_t W, uint32_t *gid, float *gdist,
+                                                             uint32_t *gdeg, int write) {
+    const uint32_t lane = threadIdx.x & (WAVE - 1);
+    const uint32_t nw = (gridDim.x * blockDim.x) / WAVE;
+    for (uint32_t t = (blockIdx.x * blockDim.x + threadIdx.x) / WAVE; t < N; t += nw) {
+        const uint32_t b = off[t], e = off[t + 1];
         uint32_t u = 0;
-        uint64_t prev = KEY_EMPTY;
-        for (uint32_t i = off[t]; i < off[t + 1]; ++i) {
-            const uint64_t k = keys[i];
-            if (k == prev) continue;
-            prev = k;
-            if (write) {
-                gid[(size_t)t * W + u] = key_id(k);
-                gdist[(size_t)t * W + u] = key_dist(k);
+        for (uint32_t i0 = b; i0 < e; i0 += WAVE) {
+            const uint32_t i = i0 + lane;
+            const bool in = i < e;
+            const uint64_t k = in ? keys[i] : KEY_EMPTY;
+            const uint64_t kp = in && i > b ? keys[i - 1] : KEY_EMPTY;
+            const bool keep = in && k != kp;
+            const uint64_t bm = __ballot(keep);
+            if (write && keep) {
+                const size_t c = (size_t)t * W + u + lanes_below(bm);
+                gid[c] = key_id(k);
+                gdist[c] = key_dist(k);
             }
-            ++u;
+            u += (uint32_t)__popcll(bm);
         }
         if (write) {
-            for (uint32_t i = u; i < W; ++i) {
+            for (uint32_t i = u + lane; i < W; i += WAVE) {
                 gid[(size_t)t * W + i] = ID_NONE;
                 gdist[(size_t)t * W + i] = -1.0f;
             }
-            gdeg[t] = u;
-        } else {
+            if (lane == 0) gdeg[t] = u;
+        } else if (lane == 0) {
             ucnt[t] = u;
         }
     }
@@ -1576,7 +1586,7 @@ int dpg_build(wv_index *ix, uint32_t range, wv_nsg_stats *stats, const uint32_t 
     NSG_HIP(B.get(&tmpb, bytes));
     NSG_HIP(hipcub::DeviceSegmentedRadixSort::SortKeys(tmpb, bytes, keys, sorted, (int)total, (int)N, off, off + 1, 0,
                                                       64, s));
-    hipLaunchKernelGGL(dpg_union_rows_kernel, dim3(1024), dim3(256), 0, s, sorted, off, N, ucnt, 0u, nullptr, nullptr,
+    hipLaunchKernelGGL(dpg_union_rows_kernel, dim3(8192), dim3(256), 0, s, sorted, off, N, ucnt, 0u, nullptr, nullptr,
                        nullptr, 0);
     NSG_HIP(hipGetLastError());
     NSG_HIP(hipStreamSynchronize(s));  // the context's stream does not wait for hipMemcpy
@@ -1588,7 +1598,7 @@ int dpg_build(wv_index *ix, uint32_t range, wv_nsg_stats *stats, const uint32_t 
     float *gdist = nullptr;
     int rc = index_alloc_graph_dev(ix, W, &gid, &gdist, &gdeg);  // frees the kNN graph
     if (rc) return rc;
-    hipLaunchKernelGGL(dpg_union_rows_kernel, dim3(1024), dim3(256), 0, s, sorted, off, N, ucnt, W, gid, gdist, gdeg,
+    hipLaunchKernelGGL(dpg_union_rows_kernel, dim3(8192), dim3(256), 0, s, sorted, off, N, ucnt, W, gid, gdist, gdeg,
                        1);
     NSG_HIP(hipGetLastError());
     NSG_HIP(hipStreamSynchronize(s));
