@@ -62,3 +62,37 @@ def test_dpg_rejects_bad_params(ctx, small_float):
         with pytest.raises(WvError):
             ix.build_dpg(bad)
     ix.close()
+
+
+def _hub_graph(N, rng_, seed):
+    """Node 0 sits at the origin and every other node lists it among its
+    2*range neighbours (the rest random): it is nearly always the nearest
+    entry, kept with no hits, so node 0's mirrored row holds most of the
+    nodes."""
+    rng = np.random.default_rng(seed)
+    base = rng.normal(size=(N, 8)).astype(np.float32)
+    base[0] = 0
+    ids = np.empty((N, 2 * rng_), np.uint32)
+    ids[0] = rng.choice(np.arange(1, N), 2 * rng_, replace=False)
+    for n in range(1, N):
+        others = np.concatenate([np.arange(1, n), np.arange(n + 1, N)])
+        ids[n, 0] = 0
+        ids[n, 1:] = rng.choice(others, 2 * rng_ - 1, replace=False)
+    return base, ids, np.full(N, 2 * rng_, np.uint32)
+
+
+def test_dpg_hub_row_spans_chunks(ctx):
+    """A mirrored row of several hundred keys: the distinct pass runs over
+    many 64-key chunks, each key compared with the one before it across the
+    chunk edges, and the row is padded to the width."""
+    from weavess_amd import Index
+    base, ids, deg = _hub_graph(700, 4, seed=7)
+    o = O.dpg(base, ids, deg, 4)
+    assert o[0].shape[1] > 3 * 64
+    ix = Index(ctx, base)
+    ix.set_graph(ids, deg)
+    st = ix.build_dpg(4)
+    g = ix.get_graph()
+    ix.close()
+    assert st["width"] == o[0].shape[1]
+    _same(g, o)
