@@ -1215,7 +1215,7 @@ struct Bufs {
     }
 };
 
-constexpr int WPB = 4;       // wavefronts (nodes) per workgroup
+constexpr int WPB = 1;       // wavefronts (nodes) per workgroup: one, as in the join
 constexpr int JOIN_WPB = 1;  // one node a workgroup: a finished wavefront frees its slot at once
 
 // The join kernel for this element type: register operands of at most ts
