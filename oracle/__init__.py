@@ -25,7 +25,8 @@ _f64p = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
 class NndParams(C.Structure):
     _fields_ = [("K", C.c_uint32), ("L", C.c_uint32), ("S", C.c_uint32),
                 ("R", C.c_uint32), ("iters", C.c_uint32),
-                ("cand_slots", C.c_uint32), ("seed", C.c_uint64)]
+                ("cand_slots", C.c_uint32), ("sweep", C.c_uint32),
+                ("reserved", C.c_uint32), ("seed", C.c_uint64)]
 
 
 class TermModel(C.Structure):
@@ -78,6 +79,12 @@ def lib():
         L.wvo_nndescent.argtypes = [_f32p, C.c_uint32, C.c_uint32, C.POINTER(NndParams),
                                     C.c_void_p, C.c_void_p, C.c_void_p,
                                     _u32p, _f32p, _u32p, C.c_void_p, C.c_int]
+        L.wvo_nndescent_state.restype = C.c_int
+        L.wvo_nndescent_state.argtypes = [_f32p, C.c_uint32, C.c_uint32, C.POINTER(NndParams),
+                                          C.c_void_p, C.c_void_p, C.c_void_p,
+                                          C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                          C.c_void_p, C.c_void_p, C.c_uint32,
+                                          C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.c_int]
         L.wvo_nndescent_efanna.restype = C.c_int
         L.wvo_nndescent_efanna.argtypes = [_f32p, C.c_uint32, C.c_uint32, C.POINTER(NndParams),
                                            C.c_void_p, C.c_void_p, C.c_uint32,
@@ -214,10 +221,15 @@ def init_random(base, S: int, seed: int, nthreads: int = 0):
 
 
 def nndescent(base, K, L, S, R, iters, seed=0, cand_slots=0, init=None,
-              nthreads: int = 0, with_stats: bool = False):
+              nthreads: int = 0, with_stats: bool = False, sweep: bool = False):
+    """InitRandom + NN-Descent + PruneNaive.  cand_slots=0: the reference's
+    exact bounded insertion; > 0: the slot table the device uses.  sweep:
+    the reference's single-thread reverse test (a neighbour visited earlier
+    in the sweep reads a heap leaf as its worst) instead of the two-phase
+    one."""
     base = _c(base, np.float32)
     N, d = base.shape
-    p = NndParams(K, L, S, R, iters, cand_slots, seed)
+    p = NndParams(K, L, S, R, iters, cand_slots, int(sweep), 0, seed)
     ids = np.zeros((N, K), np.uint32)
     dist = np.zeros((N, K), np.float32)
     deg = np.zeros(N, np.uint32)
@@ -237,13 +249,55 @@ def nndescent(base, K, L, S, R, iters, seed=0, cand_slots=0, init=None,
     return ids, dist, deg
 
 
+def nndescent_state(base, K, L, S, R, iters, seed=0, cand_slots=0, init=None,
+                    nthreads: int = 1, new_cap=None, old_cap=None, sweep: bool = False):
+    """State after ``iters`` (join, update) rounds, before the final sort:
+    dict of pool ids/dist/flag (N x L, ``pool_cnt`` valid), ``M``, the sample
+    lists ``nn_new``/``nn_old`` (Python lists per node) and per-round
+    (pairs, inserts).  Single-threaded by default so that exact insertion is
+    deterministic."""
+    base = _c(base, np.float32)
+    N, d = base.shape
+    p = NndParams(K, L, S, R, iters, cand_slots, int(sweep), 0, seed)
+    new_cap = new_cap or max(2 * S, S + R)
+    old_cap = old_cap or max(L, 2 * R)
+    pid = np.zeros((N, L), np.uint32)
+    pdist = np.zeros((N, L), np.float32)
+    pflag = np.zeros((N, L), np.uint8)
+    pcnt = np.zeros(N, np.uint32)
+    M = np.zeros(N, np.uint32)
+    nn_new = np.zeros((N, new_cap), np.uint32)
+    nn_old = np.zeros((N, old_cap), np.uint32)
+    ncnt = np.zeros(N, np.uint32)
+    ocnt = np.zeros(N, np.uint32)
+    st = (NndIterStats * max(iters, 1))()
+    if init is not None:
+        i_ids, i_dist, i_cnt = (_c(init[0], np.uint32), _c(init[1], np.float32),
+                                _c(init[2], np.uint32))
+        args = (_ptr(i_ids), _ptr(i_dist), _ptr(i_cnt))
+    else:
+        args = (None, None, None)
+    rc = lib().wvo_nndescent_state(base, N, d, C.byref(p), *args, _ptr(pid), _ptr(pdist),
+                                   _ptr(pflag), _ptr(pcnt), _ptr(M), _ptr(nn_new), _ptr(ncnt),
+                                   new_cap, _ptr(nn_old), _ptr(ocnt), old_cap,
+                                   C.cast(st, C.c_void_p), nthreads)
+    if rc != 0:
+        raise ValueError("nndescent_state: bad arguments (%d)" % rc)
+    return {
+        "pool_ids": pid, "pool_dist": pdist, "pool_flag": pflag, "pool_cnt": pcnt, "M": M,
+        "nn_new": [list(nn_new[i, :ncnt[i]]) for i in range(N)],
+        "nn_old": [list(nn_old[i, :ocnt[i]]) for i in range(N)],
+        "stats": [(s.pairs, s.inserts) for s in st[:iters]],
+    }
+
+
 def nndescent_efanna(base, K, L, S, R, iters, seed=0, cand_slots=0, init=None,
                      nthreads: int = 0, with_stats: bool = False):
     """EFANNA's refine.  init: None (InitRandom's graph) or (ids N x stride,
     cnt[N] or None) of any initial graph with stride <= L."""
     base = _c(base, np.float32)
     N, d = base.shape
-    p = NndParams(K, L, S, R, iters, cand_slots, seed)
+    p = NndParams(K, L, S, R, iters, cand_slots, 0, 0, seed)
     ids = np.zeros((N, K), np.uint32)
     dist = np.zeros((N, K), np.float32)
     deg = np.zeros(N, np.uint32)

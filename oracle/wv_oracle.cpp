@@ -14,9 +14,13 @@
  * wvo_gen_random                include/weavess/util.h:13-27                          oracle/_ref build of util.h fed the same draws,
  *                                                                                     invariants (distinct, in range, count)
  * wvo_init_random               src/component_init.cpp:11-45                          invariants (no self, distinct, sorted, dist)
- * wvo_nndescent                 src/component_refine.cpp:12-239,                      S>=N-1 gives the exact kNN graph; a 5-point
- *                               include/weavess/index.h:86-150,                       hand-worked case; invariants; graph recall
- *                               src/component_prune.cpp:8-18                          vs FP64 exact kNN
+ * wvo_nndescent                 src/component_refine.cpp:12-239,                      a 6-point hand-worked case: pools, flags, M,
+ *                               include/weavess/index.h:86-150,                       nn_new/nn_old and pair counts after 1-3 rounds,
+ *                               src/component_prune.cpp:8-18                          and the reference's single-thread sweep (D5)
+ *                                                                                     (test_oracle_nnd_kat.py); S>=N-1 gives the
+ *                                                                                     exact kNN graph; invariants; graph recall
+ *                                                                                     vs FP64 exact kNN
+ * wvo_nndescent_state           the same rounds, state before the final sort          the hand-worked case above
  * wvo_nndescent_efanna          src/component_refine.cpp:959-1183,                    ITER=0 returns the initial rows sorted, self
  *                               include/weavess/index.h:105-110                       and repeats dropped (numpy); 2S=N-1 with exact
  *                                                                                     insertion gives the exact kNN graph in one
@@ -277,6 +281,7 @@ struct EntLess {
 
 struct NndState {
     uint32_t N, d, L, S, R;
+    bool sweep = false;
     const float *base;
     std::vector<std::vector<Ent>> pool;      /* max-heap during join, sorted otherwise */
     std::vector<std::mutex> locks;
@@ -408,7 +413,8 @@ uint64_t rev_key(uint64_t seed, uint64_t tag, uint32_t iter, uint32_t x, uint32_
  * before any reverse test) and with the reverse cap of D3. */
 void nnd_update(NndState &st, int nth, uint32_t iter, uint64_t seed) {
     const uint32_t N = st.N, S = st.S, R = st.R;
-    std::vector<float> worst(N);
+    std::vector<float> worst(N), leaf(N);
+    const bool sweep = st.sweep;
     /* :157-175 sort, truncate to L, new M */
 #pragma omp parallel for num_threads(nth) schedule(static)
     for (int64_t n = 0; n < (int64_t)N; ++n) {
@@ -423,6 +429,13 @@ void nnd_update(NndState &st, int nth, uint32_t iter, uint64_t seed) {
         }
         st.M[n] = l;
         worst[n] = p.empty() ? -1.0f : key_dist(p.back().key);
+        if (sweep) {
+            /* what pool.back() reads once n's own sampling re-heaped it
+             * (src/component_refine.cpp:212) */
+            std::vector<Ent> h(p.begin(), p.end());
+            std::make_heap(h.begin(), h.end(), EntLess());
+            leaf[n] = h.empty() ? -1.0f : key_dist(h.back().key);
+        }
         st.nn_new[n].clear();
         st.nn_old[n].clear();
     }
@@ -434,7 +447,7 @@ void nnd_update(NndState &st, int nth, uint32_t iter, uint64_t seed) {
         auto &p = st.pool[n];
         for (uint32_t l = 0; l < st.M[n]; ++l) {
             uint32_t id = key_id(p[l].key);
-            bool rev = key_dist(p[l].key) > worst[id];
+            bool rev = key_dist(p[l].key) > ((sweep && id < (uint32_t)n) ? leaf[id] : worst[id]);
             if (p[l].flag) {
                 st.nn_new[n].push_back(id);
                 if (rev) cnt_new[id].fetch_add(1, std::memory_order_relaxed);
@@ -457,7 +470,7 @@ void nnd_update(NndState &st, int nth, uint32_t iter, uint64_t seed) {
         auto &p = st.pool[n];
         for (uint32_t l = 0; l < st.M[n]; ++l) {
             uint32_t id = key_id(p[l].key);
-            if (!(key_dist(p[l].key) > worst[id])) continue;
+            if (!(key_dist(p[l].key) > ((sweep && id < (uint32_t)n) ? leaf[id] : worst[id]))) continue;
             if (p[l].flag)
                 rv_new[off_new[id] + cnt_new[id].fetch_add(1, std::memory_order_relaxed)] = (uint32_t)n;
             else
@@ -488,11 +501,10 @@ void nnd_update(NndState &st, int nth, uint32_t iter, uint64_t seed) {
     }
 }
 
-/* ITER rounds of join() + update(), then RefineInner's tail (:22-74) and
- * PruneNaive: sort, keep the first K.  first_join: round 0 joins the new
- * lists already set (EFANNA's samples); else round 0 has nothing to join. */
-void nnd_rounds(NndState &st, const wvo_nnd_params *p, int nth, bool first_join, uint32_t *out_ids,
-                float *out_dist, uint32_t *out_deg, wvo_nnd_iter_stats *iter_stats) {
+/* ITER rounds of join() + update().  first_join: round 0 joins the new lists
+ * already set (EFANNA's samples); else round 0 has nothing to join. */
+void nnd_iterate(NndState &st, const wvo_nnd_params *p, int nth, bool first_join,
+                 wvo_nnd_iter_stats *iter_stats) {
     for (uint32_t it = 0; it < p->iters; ++it) {
         uint64_t pairs = 0, acc = 0;
         if (it > 0 || first_join) nnd_join(st, nth, pairs, acc);
@@ -502,6 +514,11 @@ void nnd_rounds(NndState &st, const wvo_nnd_params *p, int nth, bool first_join,
             iter_stats[it].inserts = acc;
         }
     }
+}
+
+/* RefineInner's tail (:22-74) and PruneNaive: sort, keep the first K. */
+void nnd_finalize(NndState &st, const wvo_nnd_params *p, int nth, uint32_t *out_ids, float *out_dist,
+                  uint32_t *out_deg) {
 #pragma omp parallel for num_threads(nth) schedule(static)
     for (int64_t i = 0; i < (int64_t)st.N; ++i) {
         auto &pl = st.pool[i];
@@ -520,19 +537,16 @@ void nnd_rounds(NndState &st, const wvo_nnd_params *p, int nth, bool first_join,
     }
 }
 
-}  // namespace
-
-int wvo_nndescent(const float *base, uint32_t N, uint32_t d, const wvo_nnd_params *p,
-                  const uint32_t *init_ids, const float *init_dist, const uint32_t *init_cnt,
-                  uint32_t *out_ids, float *out_dist, uint32_t *out_deg,
-                  wvo_nnd_iter_stats *iter_stats, int nthreads) {
-    if (!base || !p || !out_ids || !out_dist || !out_deg) return -1;
-    if (N < 2 || d == 0 || p->S == 0 || p->S >= N || p->L == 0 || p->K == 0 || p->K > p->L) return -1;
-    if (p->S > p->L) return -1;
-    if (p->cand_slots && (p->cand_slots & (p->cand_slots - 1))) return -1;
-    const int nth = nthreads_or_default(nthreads);
+/* KGraph's init() (src/component_refine.cpp:87-104): pools seeded from the
+ * initial graph (or InitRandom's), all flagged new, M = S.  Returns false on
+ * bad arguments. */
+bool nnd_setup(NndState &st, const float *base, uint32_t N, uint32_t d, const wvo_nnd_params *p,
+               const uint32_t *init_ids, const float *init_dist, const uint32_t *init_cnt, int nth) {
+    if (!base || !p) return false;
+    if (N < 2 || d == 0 || p->S == 0 || p->S >= N || p->L == 0 || p->K == 0 || p->K > p->L) return false;
+    if (p->S > p->L) return false;
+    if (p->cand_slots && (p->cand_slots & (p->cand_slots - 1))) return false;
     const uint32_t S = p->S;
-
     std::vector<uint32_t> ids0, cnt0;
     std::vector<float> dist0;
     if (!init_ids) {
@@ -544,17 +558,14 @@ int wvo_nndescent(const float *base, uint32_t N, uint32_t d, const wvo_nnd_param
         init_dist = dist0.data();
         init_cnt = cnt0.data();
     }
-
-    NndState st(N);
     st.N = N; st.d = d; st.L = p->L; st.S = S; st.R = p->R; st.base = base;
+    st.sweep = p->sweep != 0;
     st.C = p->cand_slots;
     if (st.C) {
         st.slots.assign((size_t)N * st.C, KEY_EMPTY);
         st.thr.assign(N, KEY_EMPTY);
         st.pool_ids.resize(N);
     }
-    /* init(), src/component_refine.cpp:87-104: pools seeded from the initial
-     * graph, all flagged new, M = S. */
     for (uint32_t i = 0; i < N; ++i) {
         auto &pl = st.pool[i];
         pl.reserve(p->L);
@@ -564,9 +575,56 @@ int wvo_nndescent(const float *base, uint32_t N, uint32_t d, const wvo_nnd_param
         std::sort(pl.begin(), pl.end(), EntLess());
         st.M[i] = S;
     }
+    return true;
+}
+
+}  // namespace
+
+int wvo_nndescent(const float *base, uint32_t N, uint32_t d, const wvo_nnd_params *p,
+                  const uint32_t *init_ids, const float *init_dist, const uint32_t *init_cnt,
+                  uint32_t *out_ids, float *out_dist, uint32_t *out_deg,
+                  wvo_nnd_iter_stats *iter_stats, int nthreads) {
+    if (!out_ids || !out_dist || !out_deg) return -1;
+    const int nth = nthreads_or_default(nthreads);
+    NndState st(N);
+    if (!nnd_setup(st, base, N, d, p, init_ids, init_dist, init_cnt, nth)) return -1;
     /* NNDescent(), :106-125.  Iteration 0's join sees nn_new = 2S copies of
      * id 0 and no nn_old (index.h:98-103), so it evaluates nothing. */
-    nnd_rounds(st, p, nth, false, out_ids, out_dist, out_deg, iter_stats);
+    nnd_iterate(st, p, nth, false, iter_stats);
+    nnd_finalize(st, p, nth, out_ids, out_dist, out_deg);
+    return 0;
+}
+
+int wvo_nndescent_state(const float *base, uint32_t N, uint32_t d, const wvo_nnd_params *p,
+                        const uint32_t *init_ids, const float *init_dist, const uint32_t *init_cnt,
+                        uint32_t *pool_ids, float *pool_dist, uint8_t *pool_flag, uint32_t *pool_cnt,
+                        uint32_t *M, uint32_t *nn_new, uint32_t *new_cnt, uint32_t new_cap,
+                        uint32_t *nn_old, uint32_t *old_cnt, uint32_t old_cap,
+                        wvo_nnd_iter_stats *iter_stats, int nthreads) {
+    if (!pool_ids || !pool_dist || !pool_flag || !pool_cnt || !M || !nn_new || !new_cnt || !nn_old || !old_cnt)
+        return -1;
+    const int nth = nthreads_or_default(nthreads);
+    NndState st(N);
+    if (!nnd_setup(st, base, N, d, p, init_ids, init_dist, init_cnt, nth)) return -1;
+    nnd_iterate(st, p, nth, false, iter_stats);
+    for (uint32_t i = 0; i < N; ++i) {
+        const auto &pl = st.pool[i];
+        if (pl.size() > p->L || st.nn_new[i].size() > new_cap || st.nn_old[i].size() > old_cap) return -2;
+        pool_cnt[i] = (uint32_t)pl.size();
+        for (uint32_t j = 0; j < p->L; ++j) {
+            const bool ok = j < pl.size();
+            pool_ids[(size_t)i * p->L + j] = ok ? key_id(pl[j].key) : 0xFFFFFFFFu;
+            pool_dist[(size_t)i * p->L + j] = ok ? key_dist(pl[j].key) : -1.0f;
+            pool_flag[(size_t)i * p->L + j] = ok ? pl[j].flag : 0;
+        }
+        M[i] = st.M[i];
+        new_cnt[i] = (uint32_t)st.nn_new[i].size();
+        old_cnt[i] = (uint32_t)st.nn_old[i].size();
+        for (uint32_t j = 0; j < new_cap; ++j)
+            nn_new[(size_t)i * new_cap + j] = j < st.nn_new[i].size() ? st.nn_new[i][j] : 0xFFFFFFFFu;
+        for (uint32_t j = 0; j < old_cap; ++j)
+            nn_old[(size_t)i * old_cap + j] = j < st.nn_old[i].size() ? st.nn_old[i][j] : 0xFFFFFFFFu;
+    }
     return 0;
 }
 
@@ -625,7 +683,8 @@ int wvo_nndescent_efanna(const float *base, uint32_t N, uint32_t d, const wvo_nn
         st.nn_old[i].clear();
     }
     /* NNDescent(), :1059-1078: the first join runs over those samples. */
-    nnd_rounds(st, p, nth, true, out_ids, out_dist, out_deg, iter_stats);
+    nnd_iterate(st, p, nth, true, iter_stats);
+    nnd_finalize(st, p, nth, out_ids, out_dist, out_deg);
     return 0;
 }
 

@@ -18,6 +18,13 @@
  *       (src/component_refine.cpp:190-208), which makes the choice
  *       independent of thread order.
  *   D4  "build order" distance: see wvo_dist_canon.
+ *   D5  update()'s reverse test reads each neighbour's sorted worst.  In the
+ *       reference the loop that samples node n ends with make_heap of n's
+ *       pool (src/component_refine.cpp:212), so a node visited later reads
+ *       a heap leaf from n's pool.back() (:185, :196), not n's worst; with
+ *       OpenMP threads which neighbours were visited first depends on the
+ *       schedule.  wvo_nnd_params.sweep = 1 restates the single-thread
+ *       order (neighbours with a smaller id read the leaf).
  *
  * Parity status of every exported function is listed in the table at the top
  * of wv_oracle.cpp.
@@ -53,6 +60,12 @@ typedef struct {
     uint32_t K, L, S, R, iters;
     uint32_t cand_slots;   /* 0: exact bounded insertion (reference);      */
                            /* >0: power-of-two lossy slot table per node   */
+    uint32_t sweep;        /* 0: every reverse test reads the neighbour's  */
+                           /* sorted worst (two-phase, D5); 1: the         */
+                           /* reference's single-thread sweep, in which a  */
+                           /* neighbour already visited was re-heaped and  */
+                           /* its pool.back() is a heap leaf               */
+    uint32_t reserved;
     uint64_t seed;
 } wvo_nnd_params;
 
@@ -77,6 +90,23 @@ int wvo_nndescent(const float *base, uint32_t N, uint32_t d,
                   const uint32_t *init_cnt,
                   uint32_t *out_ids, float *out_dist, uint32_t *out_deg,
                   wvo_nnd_iter_stats *iter_stats, int nthreads);
+
+/* The state of wvo_nndescent after p->iters (join, update) rounds, before
+ * the final sort and PruneNaive: each node's pool (N*L ids, dist and "new"
+ * flags, pool_cnt[N] valid, ascending), M[N] (the sample window of the last
+ * update, src/component_refine.cpp:166-174), and the sample lists that the
+ * next join would read: nn_new (N*new_cap, new_cnt[N]) and nn_old
+ * (N*old_cap, old_cnt[N]), forward samples in pool order followed by the
+ * reverse samples in ascending id.  Returns -2 when a list exceeds its cap. */
+int wvo_nndescent_state(const float *base, uint32_t N, uint32_t d,
+                        const wvo_nnd_params *p,
+                        const uint32_t *init_ids, const float *init_dist,
+                        const uint32_t *init_cnt,
+                        uint32_t *pool_ids, float *pool_dist, uint8_t *pool_flag,
+                        uint32_t *pool_cnt, uint32_t *M,
+                        uint32_t *nn_new, uint32_t *new_cnt, uint32_t new_cap,
+                        uint32_t *nn_old, uint32_t *old_cnt, uint32_t old_cap,
+                        wvo_nnd_iter_stats *iter_stats, int nthreads);
 
 /* ComponentRefineEFANNA + PruneNaive (src/component_refine.cpp:959-1183):
  * pools from an initial graph (init_ids rows of init_stride <= L ids,
