@@ -129,56 +129,12 @@ def nndescent_leg(ctx, Index, datagen):
     return out
 
 
-def bf16_round(x):
-    """float32 -> bfloat16 -> float32, round to nearest even (the device's
-    conversion in the join's row preparation)."""
-    b = np.ascontiguousarray(x, np.float32).view(np.uint32).astype(np.uint64)
-    b = (b + 0x7FFF + ((b >> 16) & 1)) & 0xFFFF0000
-    return b.astype(np.uint32).view(np.float32)
-
-
-def e4m3_round(x):
-    """Round to OCP FP8 E4M3 (3 mantissa bits, exponent bias 7, subnormal
-    step 2^-9), nearest even; |x| <= 448 already (the join clamps)."""
-    x = np.asarray(x, np.float64)
-    ax = np.abs(x)
-    e = np.floor(np.log2(np.maximum(ax, 2.0 ** -6)))
-    step = 2.0 ** (e - 3)
-    return np.sign(x) * np.round(ax / step) * step
-
-
 def join_distance_error(base, ids, deg, dtype, sample, seed=0):
-    """Max / mean relative error of the join's distance formula
-    |a|^2 + |b|^2 - 2 a.b, in FP32 arithmetic on quantised rows (BF16, or FP8
-    with per-row absmax/448 scales; FP32 rows for f32), against FP64 on the
-    original rows,
-    over the graph edges of `sample` random nodes (the near pairs, where the
-    formula cancels most)."""
-    g = np.random.default_rng(seed)
-    nodes = g.choice(len(base), sample, replace=False)
-    a_idx = np.repeat(nodes, [int(deg[n]) for n in nodes])
-    b_idx = np.concatenate([ids[n, :deg[n]] for n in nodes]).astype(np.int64)
-    A, B = base[a_idx].astype(np.float64), base[b_idx].astype(np.float64)
-    exact = ((A - B) ** 2).sum(1)
-
-    def quant(X):
-        if dtype == "bf16":
-            return bf16_round(X).astype(np.float64)
-        if dtype == "fp8":
-            sc = np.abs(X).max(1, keepdims=True) / 448.0
-            sc[sc == 0] = 1.0
-            return e4m3_round(np.clip(X / sc, -448, 448)) * sc
-        return X.astype(np.float64)
-
-    # products of BF16 / FP8 values are exact in FP32; sums are FP32, as on MFMA
-    qa, qb = quant(base[a_idx]).astype(np.float32), quant(base[b_idx]).astype(np.float32)
-    na = (qa * qa).sum(1, dtype=np.float32)
-    nb = (qb * qb).sum(1, dtype=np.float32)
-    dot = (qa * qb).sum(1, dtype=np.float32)
-    approx = np.maximum((na + nb) - np.float32(2.0) * dot, np.float32(0.0)).astype(np.float64)
-    ok = exact > 0
-    rel = np.abs(approx[ok] - exact[ok]) / exact[ok]
-    return float(rel.max()), float(rel.mean()), int(ok.sum())
+    """Max / mean relative error of the join's distance formula on quantised
+    rows against FP64, over the graph edges of `sample` random nodes."""
+    from weavess_amd.precision import join_formula_rel_err
+    nodes = np.random.default_rng(seed).choice(len(base), sample, replace=False)
+    return join_formula_rel_err(base, ids, deg, dtype, nodes)
 
 
 def run_c3(args, ctx, Index, datagen):
