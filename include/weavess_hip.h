@@ -294,6 +294,24 @@ int wv_index_search_dev(wv_index *idx, const float *d_queries, uint32_t Q,
                         uint32_t *d_out_ids, float *d_out_dist, uint32_t *d_evals,
                         uint32_t *d_hops, wv_search_stats *stats);
 
+/* Greedy search that allocates nothing, for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  No counterpart in the reference.  It
+ * is wv_index_search_dev with seeded random entries and one L and k for
+ * every query (no explicit entries, no per-query budgets), and returns the
+ * same ids, distances, counts and statistics.  The per-query evaluation and
+ * hop counts behind `stats`, when d_evals/d_hops are NULL, and the totals
+ * live in a file-scope device array of the library, about 2 MiB, which each
+ * GPU the library runs on holds until the process ends; Q above
+ * WV_NOALLOC_MAX_Q is rejected with WV_EINVAL.  Every call on one GPU
+ * shares that GPU's array: the library serialises such calls in one
+ * process, and kernels from two calls that overlapped would race on it. */
+#define WV_NOALLOC_MAX_Q (1u << 18)
+int wv_index_search_dev_noalloc(wv_index *idx, const float *d_queries, uint32_t Q, uint64_t seed,
+                                uint32_t L, uint32_t k, uint32_t k_stride, uint32_t *d_out_ids,
+                                float *d_out_dist, uint32_t *d_evals, uint32_t *d_hops,
+                                wv_search_stats *stats);
+
 /* Greedy search with learned termination and/or per-hop traces. */
 int wv_index_search_ext(wv_index *idx, const float *queries, uint32_t Q, const uint32_t *entries,
                         uint64_t seed, uint32_t L, uint32_t k, const uint32_t *L_per_q,

@@ -32,8 +32,9 @@ EXPORTS = [
     "wv_index_search_ext", "wv_index_search_ext_dev", "wv_index_autotune", "wv_search_graph_create",
     "wv_search_graph_launch", "wv_search_graph_destroy", "wv_mlp_grad", "wv_mlp_train",
     "wv_mlp_predict", "wv_index_build_efanna", "wv_index_build_nsg", "wv_index_build_vamana",
-    "wv_index_build_dpg", "wv_index_build_ssg",
+    "wv_index_build_dpg", "wv_index_build_ssg", "wv_index_search_dev_noalloc",
 ]
+WV_NOALLOC_MAX_Q = 1 << 18
 
 
 class WvError(RuntimeError):
@@ -176,6 +177,7 @@ def load() -> C.CDLL:
         "wv_index_build_ssg": ([vp, vp, vp, vp], i32),
         "wv_index_search": ([vp, vp, u32, vp, u64, u32, u32, vp, vp, u32, vp, vp, vp, vp, vp], i32),
         "wv_index_search_dev": ([vp, vp, u32, vp, u64, u32, u32, vp, vp, u32, vp, vp, vp, vp, vp], i32),
+        "wv_index_search_dev_noalloc": ([vp, vp, u32, u64, u32, u32, u32, vp, vp, vp, vp, vp], i32),
         "wv_nndescent_build": ([vp, vp, u32, u32, vp, vp, vp, vp, vp, vp], i32),
         "wv_search_greedy": ([vp, vp, vp, u32, vp, u32, vp, u64, u32, u32, vp, vp, u32, vp, vp, vp,
                               vp, vp], i32),
@@ -525,4 +527,17 @@ class Index:
             self.h, vp(d_queries), Q, vp(d_entries), seed, L, k, vp(d_L), vp(d_k),
             k_stride if k_stride is not None else k, vp(d_out_ids), vp(d_out_dist),
             vp(d_evals), vp(d_hops), C.byref(st) if want_stats else None))
+        return st.as_dict()
+
+    def search_dev_noalloc(self, d_queries: int, Q: int, L: int, k: int, d_out_ids: int,
+                           d_out_dist: int = 0, seed: int = 0, k_stride: Optional[int] = None,
+                           d_evals: int = 0, d_hops: int = 0, want_stats: bool = True):
+        """wv_index_search_dev_noalloc: random entries, one L and k, no device
+        allocation inside the call (its scratch is a fixed array of the
+        library, WV_NOALLOC_MAX_Q queries a call)."""
+        st = SearchStats()
+        vp = lambda x: C.c_void_p(x) if x else None
+        self.ctx.check(self.ctx.lib.wv_index_search_dev_noalloc(
+            self.h, vp(d_queries), Q, seed, L, k, k_stride if k_stride is not None else k,
+            vp(d_out_ids), vp(d_out_dist), vp(d_evals), vp(d_hops), C.byref(st) if want_stats else None))
         return st.as_dict()

@@ -24,6 +24,8 @@ struct wv_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    void *scratch = nullptr;        // this device's copy of `scratch` (no-allocation search)
+    hipError_t scratch_err = hipSuccess;
     std::mutex mu;
     std::string err;
     int cu_count = 0;
@@ -278,6 +280,62 @@ const float *model_field(const void *dev_model, size_t offset) {
 
 }  // namespace
 
+// ------------------------------------------------------------ no allocation
+// Scratch of wv_index_search_dev_noalloc: the per-query evaluation and hop
+// counts a caller does not take, then the two totals of the statistics.  It
+// is file-scope device memory: each GPU the library runs on holds its own
+// copy, at its own address, until the process ends.  Host code never passes
+// the symbol itself to a kernel or a copy (on the host it only names the
+// array); it uses the address hipGetSymbolAddress returned for the device.
+constexpr size_t NOALLOC_COUNTS = 2 * (size_t)WV_NOALLOC_MAX_Q * 4;
+static __device__ __align__(256) uint8_t scratch[NOALLOC_COUNTS + 256];
+
+namespace {
+
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+void *g_scratch[MAX_DEVICES];          // per device: the address of `scratch`
+std::mutex g_scratch_call_mu[MAX_DEVICES];  // calls on one GPU share its array
+
+// Once per device, with that device current (wv_ctx_create).
+hipError_t scratch_address(int device, void **out) {
+    std::lock_guard<std::mutex> g(g_scratch_mu);
+    *out = nullptr;
+    if (device < 0 || device >= MAX_DEVICES) return hipErrorInvalidDevice;
+    if (!g_scratch[device]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        g_scratch[device] = p;
+    }
+    *out = g_scratch[device];
+    return hipSuccess;
+}
+
+__global__ void totals_clear_kernel(unsigned long long *tot) {
+    if (threadIdx.x < 2) tot[threadIdx.x] = 0;
+}
+
+// tot[0] += sum of ev[0..Q), tot[1] += sum of hp[0..Q)
+__global__ void __launch_bounds__(256) totals_kernel(const uint32_t *ev, const uint32_t *hp, uint32_t Q,
+                                                     unsigned long long *tot) {
+    uint64_t se = 0, sh = 0;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < Q; i += gridDim.x * blockDim.x) {
+        se += ev[i];
+        sh += hp[i];
+    }
+    for (int o = wv::WAVE / 2; o > 0; o >>= 1) {
+        se += wv::shfl_xor64(se, o);
+        sh += wv::shfl_xor64(sh, o);
+    }
+    if ((threadIdx.x & (wv::WAVE - 1)) == 0) {
+        atomicAdd(&tot[0], (unsigned long long)se);
+        atomicAdd(&tot[1], (unsigned long long)sh);
+    }
+}
+
+}  // namespace
+
 extern "C" {
 
 int wv_version(void) { return 1; }
@@ -298,6 +356,10 @@ int wv_ctx_create(int device, wv_ctx **out) {
     }
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->cu_count = prop.multiProcessorCount;
+    // the no-allocation search's scratch: its address on this device, found
+    // once per device while it is current (a failure is reported by that
+    // entry point only)
+    c->scratch_err = scratch_address(device, &c->scratch);
     *out = c;
     return WV_OK;
 }
@@ -570,6 +632,66 @@ int wv_index_search_dev(wv_index *ix, const float *d_queries, uint32_t Q, const 
     }
     return search_core(ix, d_queries, Q, d_entries, seed, L, k, d_L_per_q, d_k_per_q, Lmax,
                        k_stride, d_out_ids, d_out_dist, d_evals, d_hops, stats);
+}
+
+int wv_index_search_dev_noalloc(wv_index *ix, const float *d_queries, uint32_t Q, uint64_t seed, uint32_t L,
+                                uint32_t k, uint32_t k_stride, uint32_t *d_out_ids, float *d_out_dist,
+                                uint32_t *d_evals, uint32_t *d_hops, wv_search_stats *stats) {
+    if (!ix) return WV_EINVAL;
+    wv_ctx *c = ix->ctx;
+    std::lock_guard<std::mutex> g(c->mu);
+    if (!d_queries || !d_out_ids) return fail(c, WV_EINVAL, "queries/out_ids are null");
+    if (Q > WV_NOALLOC_MAX_Q)
+        return fail(c, WV_EINVAL, "no-allocation search: %u queries, above its maximum of %u a call", Q,
+                    (unsigned)WV_NOALLOC_MAX_Q);
+    uint32_t Lmax = 0;
+    int rc = check_search_args(c, ix->N, Q, nullptr, L, k, nullptr, nullptr, k_stride, Lmax);
+    if (rc) return rc;
+    if (!ix->graph) return fail(c, WV_EINVAL, "index has no graph (build or set one first)");
+    if (!c->scratch)
+        return fail(c, WV_EHIP, "no-allocation search: scratch address unavailable on device %d (%s)", c->device,
+                    hipGetErrorString(c->scratch_err));
+    WV_HIP(c, hipSetDevice(c->device));
+    if (Q == 0) {
+        if (stats) *stats = wv_search_stats{0.0, 0, 0};
+        return WV_OK;
+    }
+    std::lock_guard<std::mutex> g2(g_scratch_call_mu[c->device]);
+    uint32_t *s_ev = static_cast<uint32_t *>(c->scratch);
+    uint32_t *s_hp = s_ev + WV_NOALLOC_MAX_Q;
+    auto *tot = reinterpret_cast<unsigned long long *>(static_cast<uint8_t *>(c->scratch) + NOALLOC_COUNTS);
+    uint32_t *ev = d_evals ? d_evals : (stats ? s_ev : nullptr);
+    uint32_t *hp = d_hops ? d_hops : (stats ? s_hp : nullptr);
+    wv::SearchArgs a;
+    SearchPlan pl;
+    int cpl = 0;
+    rc = prepare_search(ix, d_queries, Q, nullptr, seed, L, k, nullptr, nullptr, Lmax, k_stride, d_out_ids,
+                        d_out_dist, ev, hp, SearchExtDev(), a, pl, cpl);
+    if (rc) return rc;
+    // the totals start at zero on every call (the array keeps the last call's)
+    if (stats) {
+        hipLaunchKernelGGL(totals_clear_kernel, dim3(1), dim3(64), 0, c->stream, tot);
+        WV_HIP(c, hipGetLastError());
+    }
+    WV_HIP(c, hipEventRecord(c->ev0, c->stream));
+    WV_HIP(c, wv::launch_greedy_search(a, cpl, (int)pl.groups, (int)pl.wpb, c->stream));
+    WV_HIP(c, hipEventRecord(c->ev1, c->stream));
+    unsigned long long h_tot[2] = {0, 0};
+    if (stats) {
+        const uint32_t blocks = std::min<uint32_t>(1024, (Q + 255) / 256);
+        hipLaunchKernelGGL(totals_kernel, dim3(blocks), dim3(256), 0, c->stream, ev, hp, Q, tot);
+        WV_HIP(c, hipGetLastError());
+        WV_HIP(c, hipMemcpyAsync(h_tot, tot, sizeof h_tot, hipMemcpyDeviceToHost, c->stream));
+    }
+    WV_HIP(c, hipStreamSynchronize(c->stream));
+    if (stats) {
+        float ms = 0.f;
+        WV_HIP(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+        stats->kernel_ms = ms;
+        stats->dist_evals = h_tot[0];
+        stats->hops = h_tot[1];
+    }
+    return WV_OK;
 }
 
 int wv_index_search_ext(wv_index *ix, const float *queries, uint32_t Q, const uint32_t *entries,
