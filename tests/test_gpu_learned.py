@@ -51,6 +51,23 @@ def test_traces_match_oracle(ctx, small_float, small_float_graph, L, mb):
     ix.close()
 
 
+@pytest.mark.parametrize("L,mb", [(48, 0), (48, 64), (600, 0), (600, 256)])
+def test_settle_without_traces_matches_oracle(ctx, small_float, small_float_graph, L, mb):
+    """The settle hop as an output of its own (no traces, no model), with
+    and without the merge buffer."""
+    base, q = small_float
+    g, _, deg = small_float_graph
+    ix = Index(ctx, base)
+    ix.set_graph(g, deg)
+    ix.set_search_config(merge_buffer=mb)
+    ids, _, ev, hp, st, trace, tlen, settle = ix.search_ext(q, L, 10, seed=5, want_settle=True)
+    rids, _, rev, rhp, rtrace, rtlen, rsettle = O.search_greedy_ext(g, deg, base, q, L, 10, seed=5, trace_cap=4)
+    assert trace is None
+    assert np.array_equal(ids, rids) and np.array_equal(settle, rsettle)
+    assert np.all(settle <= hp) and settle.max() > 0
+    ix.close()
+
+
 def test_mlp_grad_matches_fp64_oracle(ctx):
     g = np.random.default_rng(11)
     X = g.normal(size=(1000, 8)).astype(np.float32)

@@ -172,6 +172,8 @@ __global__ void __launch_bounds__(256) greedy_search_kernel(SearchArgs a) {
     uint64_t worst = canon(pool[Li - 1]);  // the L-th key of pool + buffer
     // learned termination / traces: state features (identical on every lane)
     const bool watch = a.model != nullptr || a.trace != nullptr;
+    // the settle hop is also an output of its own (without traces or a model)
+    const bool track_settle = watch || a.settle != nullptr;
     const int kidx = (int)(kq < L ? kq : L) - 1;
     const float entry_d = watch ? key_dist(pool[0]) : 0.f;
     // k-th distance 1..4 expansions ago
@@ -294,7 +296,7 @@ __global__ void __launch_bounds__(256) greedy_search_kernel(SearchArgs a) {
                 const int bp = merge_into(buf, nb, B, ckey, c, true);
                 nb += c;
                 if (bp < kb) kb = bp;
-                if (watch && bp + lower_bound_e(pool, Li, ckey[0]) <= kidx) settle = hops;
+                if (track_settle && bp + lower_bound_e(pool, Li, ckey[0]) <= kidx) settle = hops;
                 worst = kth_union(pool, Li, buf, nb, Li - 1);
             }
         }
