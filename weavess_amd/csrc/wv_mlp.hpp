@@ -57,4 +57,53 @@ __device__ inline float term_logit(const wv_term_model *m, const float f[MLP_IN]
     return __shfl(part, 0) + m->b2;
 }
 
+// The network kept in registers for the fused search step: lane j < 32
+// holds hidden unit j's weights (w1 and w2 rounded to BF16 once), every lane
+// the standardisation constants.  term_logit_regs computes exactly
+// term_logit's sums in the same order.
+struct TermRegs {
+    float w1[MLP_IN];
+    float b1, w2;
+    float mean[MLP_IN], inv_std[MLP_IN];
+    float b2, threshold;
+    uint32_t min_hops;
+};
+
+__device__ inline void term_load(const wv_term_model *m, TermRegs &r) {
+    const int lane = lane_id();
+    const int j = lane < MLP_HID ? lane : 0;
+#pragma unroll
+    for (int i = 0; i < MLP_IN; ++i) {
+        r.w1[i] = bf16_round(m->w1[j][i]);
+        r.mean[i] = m->mean[i];
+        r.inv_std[i] = m->inv_std[i];
+    }
+    r.b1 = m->b1[j];
+    r.w2 = bf16_round(m->w2[j]);
+    r.b2 = m->b2;
+    r.threshold = m->logit_threshold;
+    r.min_hops = m->min_hops;
+}
+
+__device__ inline float term_logit_regs(const TermRegs &r, const float f[MLP_IN]) {
+    const int lane = lane_id();
+    float part = 0.0f;
+    if (lane < MLP_HID) {
+        float acc = r.b1;
+#pragma unroll
+        for (int i = 0; i < MLP_IN; ++i) {
+            const float x = bf16_round((f[i] - r.mean[i]) * r.inv_std[i]);
+            acc = acc + r.w1[i] * x;
+        }
+        const float h = acc > 0.0f ? acc : 0.0f;
+        part = r.w2 * bf16_round(h);
+    }
+    part = part + __shfl_xor(part, 16);
+    part = part + __shfl_xor(part, 8);
+    part = part + __shfl_xor(part, 4);
+    part = part + __shfl_xor(part, 2);
+    part = part + __shfl_xor(part, 1);
+    return __shfl(part, 0) + r.b2;
+}
+
 }  // namespace wv

@@ -92,7 +92,10 @@ __device__ inline uint64_t kth_union(const uint64_t *a, int na, const uint64_t *
     return x > y ? x : y;
 }
 
-template <int CPL, int G, bool COLLECT>
+// WATCH: the variant for the learned stop and per-hop traces (the state
+// features after every expansion, the network in registers); plain launches
+// use the variant without that code.
+template <int CPL, int G, bool COLLECT, bool WATCH>
 __global__ void __launch_bounds__(256) greedy_search_kernel(SearchArgs a) {
     extern __shared__ __align__(16) uint8_t smem[];
     const int wid = (int)(threadIdx.x >> 6);
@@ -171,7 +174,10 @@ __global__ void __launch_bounds__(256) greedy_search_kernel(SearchArgs a) {
     int nb = 0, kb = 0;        // buffer length; buffer slots before kb are expanded
     uint64_t worst = canon(pool[Li - 1]);  // the L-th key of pool + buffer
     // learned termination / traces: state features (identical on every lane)
-    const bool watch = a.model != nullptr || a.trace != nullptr;
+    constexpr bool watch = WATCH;
+    TermRegs net;
+    const bool use_model = WATCH && a.model != nullptr;
+    if (use_model) term_load(a.model, net);
     // the settle hop is also an output of its own (without traces or a model)
     const bool track_settle = watch || a.settle != nullptr;
     const int kidx = (int)(kq < L ? kq : L) - 1;
@@ -208,8 +214,7 @@ __global__ void __launch_bounds__(256) greedy_search_kernel(SearchArgs a) {
                 t[0] = make_float4(f[0], f[1], f[2], f[3]);
                 t[1] = make_float4(f[4], f[5], f[6], f[7]);
             }
-            if (!done && a.model && hops >= a.model->min_hops && term_logit(a.model, f) > a.model->logit_threshold)
-                break;
+            if (!done && use_model && hops >= net.min_hops && term_logit_regs(net, f) > net.threshold) break;
         }
         if (done) break;
         const uint32_t u = key_id(cur);  // cur is canonical
@@ -321,18 +326,24 @@ __global__ void __launch_bounds__(256) greedy_search_kernel(SearchArgs a) {
     }
 }
 
-template <int CPL, int G, bool COLLECT>
-hipError_t launch_t(const SearchArgs &a, int waves_per_block, hipStream_t s) {
+template <int CPL, int G, bool COLLECT, bool WATCH>
+hipError_t launch_w(const SearchArgs &a, int waves_per_block, hipStream_t s) {
     const size_t lds = (size_t)a.wave_lds * waves_per_block;
     const uint32_t blocks = (a.Q + waves_per_block - 1) / waves_per_block;
     if (lds > 65536) {
-        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&greedy_search_kernel<CPL, G, COLLECT>),
-                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        hipError_t e = hipFuncSetAttribute(
+            reinterpret_cast<const void *>(&greedy_search_kernel<CPL, G, COLLECT, WATCH>),
+            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (e != hipSuccess) return e;
     }
-    hipLaunchKernelGGL((greedy_search_kernel<CPL, G, COLLECT>), dim3(blocks), dim3(WAVE * waves_per_block),
+    hipLaunchKernelGGL((greedy_search_kernel<CPL, G, COLLECT, WATCH>), dim3(blocks), dim3(WAVE * waves_per_block),
                        lds, s, a);
     return hipGetLastError();
+}
+template <int CPL, int G, bool COLLECT>
+hipError_t launch_t(const SearchArgs &a, int waves_per_block, hipStream_t s) {
+    if (!COLLECT && (a.model || a.trace)) return launch_w<CPL, G, false, true>(a, waves_per_block, s);
+    return launch_w<CPL, G, COLLECT, false>(a, waves_per_block, s);
 }
 
 }  // namespace
