@@ -182,6 +182,22 @@ int wv_index_set_search_config(wv_index *idx, const wv_search_config *cfg);
 int wv_index_build_nndescent(wv_index *idx, const wv_nnd_params *p, const uint32_t *init_ids,
                              const uint32_t *init_cnt, wv_build_stats *stats);
 
+/* wv_index_build_nndescent with the per-round graph-recall monitor of the
+ * reference's debug path (ComponentRefineNNDescent::NNDescent with
+ * index->debug, src/component_refine.cpp:106-118; eval_recall :258-275):
+ * after round r (its join and update), per_round[r] = mean over the n_ctrl
+ * control nodes of |pool ∩ truth row| / truth_k.  ctrl: n_ctrl node ids
+ * (< N); truth: n_ctrl x truth_k exact neighbour ids, the caller's choice
+ * (the reference takes 100 GenRandom nodes and their exact top 100
+ * including the node itself, generate_control_set :241-256; wv_exact_knn
+ * gives them).  per_round has p->iters entries.  1 <= n_ctrl <= 2^20,
+ * 1 <= truth_k <= 1024.  The monitor's kernels run inside the build's
+ * timed span. */
+int wv_index_build_nndescent_monitor(wv_index *idx, const wv_nnd_params *p, const uint32_t *init_ids,
+                                     const uint32_t *init_cnt, const uint32_t *ctrl, uint32_t n_ctrl,
+                                     const uint32_t *truth, uint32_t truth_k, double *per_round,
+                                     wv_build_stats *stats);
+
 /* EFANNA's NN-Descent (ComponentRefineEFANNA, src/component_refine.cpp:
  * 959-1183) on the resident base; the result becomes the index's graph.
  * init_ids: optional N*init_stride ids of an initial graph (host; rows of

@@ -99,3 +99,29 @@ def test_nndescent_rejects_bad_params(ctx, small_float):
         p.update(bad)
         with pytest.raises(WvError):
             ctx.nndescent_build(base, p["K"], p["L"], p["S"], p["R"], 3)
+
+
+def test_monitor_per_round_matches_oracle(ctx, small_int):
+    """The per-round graph recall of the debug monitor (component_refine.cpp:
+    106-118, 241-275) against the oracle's pools after each round, on
+    integer data where the builds are bit-exact; the monitored build's graph
+    equals the plain build's."""
+    from weavess_amd import Index
+    base, _ = small_int
+    K, L, S, R, it, slots = 24, 32, 10, 32, 6, 64
+    ctrl = np.random.default_rng(4).choice(len(base), 100, replace=False).astype(np.uint32)
+    truth, _ = O.exact_knn(base, base[ctrl], 100)  # top 100 including the node itself, as the reference
+    ix = Index(ctx, base)
+    st, per = ix.build_nndescent_monitor(K, L, S, R, it, ctrl, truth, seed=3, cand_slots=slots)
+    g, _, gdeg = ix.get_graph()
+    ix.close()
+    ids, _, deg, _ = ctx.nndescent_build(base, K, L, S, R, it, seed=3, cand_slots=slots)
+    assert np.array_equal(g, ids) and np.array_equal(gdeg, deg)
+    want = []
+    for r in range(1, it + 1):
+        s = O.nndescent_state(base, K, L, S, R, r, seed=3, cand_slots=slots, nthreads=0)
+        hits = [len(set(s["pool_ids"][c, :s["pool_cnt"][c]].tolist()) & set(truth[i].tolist()))
+                for i, c in enumerate(ctrl)]
+        want.append(sum(h / 100 for h in hits) / len(ctrl))
+    np.testing.assert_allclose(per, want, rtol=0, atol=1e-12)
+    assert per[-1] > per[0]

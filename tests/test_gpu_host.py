@@ -44,6 +44,27 @@ def test_pipeline_build_search_matches_c_abi(ctx, tmp_path, small_int):
     assert float(m.group(2)) >= 0.9
 
 
+def test_pipeline_debug_monitor(ctx, tmp_path, small_int):
+    """WEAVESS_DEBUG=1 passes refine's debug flag: the build prints the graph
+    recall of 100 control nodes after every round (the reference's
+    "NN-Descent iter" / "Graph Quality" lines) and builds the same graph."""
+    base, q = small_int
+    pre = str(tmp_path / "dbg")
+    io.write_fvecs(pre + "_base.fvecs", base)
+    io.write_fvecs(pre + "_query.fvecs", q)
+    gt, _ = O.exact_knn(base, q, 10)
+    io.write_ivecs(pre + "_groundtruth.ivecs", gt)
+    env = dict(WEAVESS_K="24", WEAVESS_L="32", WEAVESS_ITER="6", WEAVESS_S="10", WEAVESS_R="32",
+               WEAVESS_SEED="3", WEAVESS_DEBUG="1")
+    out = _run(["kgraph", pre, "build"], env)
+    q_lines = [float(v) for v in re.findall(r"Graph Quality : ([0-9.e+-]+)", out)]
+    assert len(q_lines) == 6 and re.findall(r"NN-Descent iter: (\d+)", out) == [str(i) for i in range(6)]
+    assert 0.0 < q_lines[0] < q_lines[-1] <= 0.32  # pools hold at most L = 32 of the top 100
+    gids, gdeg = io.read_graph(pre + ".kgraph.graph")
+    ids, _, deg, _ = ctx.nndescent_build(base, 24, 32, 10, 32, 6, seed=3)
+    assert np.array_equal(gdeg, deg) and np.array_equal(gids[:, :24], ids)
+
+
 def test_pipeline_centroid_entry(ctx, tmp_path, small_int):
     """WEAVESS_ENTRY=centroid: ep is the exact nearest base point to the
     centroid, and search from ep's neighbours plus the counter-stream fill

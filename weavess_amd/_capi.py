@@ -33,6 +33,7 @@ EXPORTS = [
     "wv_search_graph_launch", "wv_search_graph_destroy", "wv_mlp_grad", "wv_mlp_train",
     "wv_mlp_predict", "wv_index_build_efanna", "wv_index_build_nsg", "wv_index_build_vamana",
     "wv_index_build_dpg", "wv_index_build_ssg", "wv_index_search_dev_noalloc",
+    "wv_index_build_nndescent_monitor",
 ]
 WV_NOALLOC_MAX_Q = 1 << 18
 
@@ -171,6 +172,7 @@ def load() -> C.CDLL:
         "wv_index_set_search_config": ([vp, vp], i32),
         "wv_index_build_nndescent": ([vp, vp, vp, vp, vp], i32),
         "wv_index_build_efanna": ([vp, vp, vp, vp, u32, vp], i32),
+        "wv_index_build_nndescent_monitor": ([vp, vp, vp, vp, vp, u32, vp, u32, vp, vp], i32),
         "wv_index_build_nsg": ([vp, vp, vp], i32),
         "wv_index_build_vamana": ([vp, vp, vp], i32),
         "wv_index_build_dpg": ([vp, u32, vp], i32),
@@ -384,6 +386,22 @@ class Index:
         self.ctx.check(self.ctx.lib.wv_index_build_nndescent(self.h, C.byref(p), _p(ii), _p(ic),
                                                              C.byref(st)))
         return st.as_dict()
+
+    def build_nndescent_monitor(self, K, L, S, R, iters, ctrl, truth, seed=0, dtype="f32", rerank_mult=0,
+                                cand_slots=0, init_ids=None, init_cnt=None):
+        """NN-Descent with the per-round graph recall of the control nodes
+        `ctrl` against their exact neighbour rows `truth`; returns
+        (stats, per_round recall)."""
+        p = NndParams(K, L, S, R, iters, DTYPES[dtype], rerank_mult, cand_slots, seed)
+        st = BuildStats()
+        ctrl = _c(ctrl, np.uint32)
+        truth = _c(truth, np.uint32)
+        per = np.zeros(max(iters, 1), np.float64)
+        ii, ic = _c(init_ids, np.uint32), _c(init_cnt, np.uint32)
+        self.ctx.check(self.ctx.lib.wv_index_build_nndescent_monitor(
+            self.h, C.byref(p), _p(ii), _p(ic), _p(ctrl), len(ctrl), _p(truth), truth.shape[1], _p(per),
+            C.byref(st)))
+        return st.as_dict(), per[:iters]
 
     def build_efanna(self, K, L, S, R, iters, seed=0, dtype="f32", rerank_mult=0,
                      cand_slots=0, init_ids=None, init_cnt=None):

@@ -1133,12 +1133,13 @@ int check_nnd_params(wv_ctx *c, const wv_index *ix, const wv_nnd_params *p) {
 }
 
 int run_nnd(wv_index *ix, const wv_nnd_params *p, const uint32_t *init_ids, const uint32_t *init_cnt,
-            wv_build_stats *stats, bool efanna, uint32_t init_stride) {
+            wv_build_stats *stats, bool efanna, uint32_t init_stride, const wv::NndMonitor *monitor = nullptr) {
     wv_ctx *c = ix->ctx;
     WV_HIP(c, hipSetDevice(c->device));
     int rc = wv::index_alloc_graph(ix, p->K);
     if (rc) return rc;
-    rc = wv::nndescent_build(ix, p, init_ids, init_cnt, stats, ix->graph, ix->gdist, ix->deg, efanna, init_stride);
+    rc = wv::nndescent_build(ix, p, init_ids, init_cnt, stats, ix->graph, ix->gdist, ix->deg, efanna, init_stride,
+                             monitor);
     if (rc) {
         (void)hipFree(ix->graph);
         (void)hipFree(ix->gdist);
@@ -1160,6 +1161,43 @@ int wv_index_build_nndescent(wv_index *ix, const wv_nnd_params *p, const uint32_
     int rc = check_nnd_params(c, ix, p);
     if (rc) return rc;
     return run_nnd(ix, p, init_ids, init_cnt, stats, false, 0);
+}
+
+int wv_index_build_nndescent_monitor(wv_index *ix, const wv_nnd_params *p, const uint32_t *init_ids,
+                                     const uint32_t *init_cnt, const uint32_t *ctrl, uint32_t n_ctrl,
+                                     const uint32_t *truth, uint32_t truth_k, double *per_round,
+                                     wv_build_stats *stats) {
+    if (!ix) return WV_EINVAL;
+    wv_ctx *c = ix->ctx;
+    std::lock_guard<std::mutex> g(c->mu);
+    int rc = check_nnd_params(c, ix, p);
+    if (rc) return rc;
+    if (!ctrl || !truth || !per_round) return fail(c, WV_EINVAL, "ctrl/truth/per_round are null");
+    if (n_ctrl == 0 || n_ctrl > (1u << 20) || truth_k == 0 || truth_k > 1024)
+        return fail(c, WV_EINVAL, "need 1 <= n_ctrl <= 2^20 and 1 <= truth_k <= 1024 (n_ctrl=%u truth_k=%u)",
+                    n_ctrl, truth_k);
+    for (uint32_t i = 0; i < n_ctrl; ++i)
+        if (ctrl[i] >= ix->N) return fail(c, WV_EINVAL, "control node %u is %u, not below N=%u", i, ctrl[i], ix->N);
+    WV_HIP(c, hipSetDevice(c->device));
+    const uint32_t rounds = std::max<uint32_t>(p->iters, 1);
+    DevBuf dc, dt, dh;
+    WV_HIP(c, dc.alloc((size_t)n_ctrl * 4));
+    WV_HIP(c, dt.alloc((size_t)n_ctrl * truth_k * 4));
+    WV_HIP(c, dh.alloc((size_t)rounds * n_ctrl * 4));
+    WV_HIP(c, hipMemcpyAsync(dc.p, ctrl, (size_t)n_ctrl * 4, hipMemcpyHostToDevice, c->stream));
+    WV_HIP(c, hipMemcpyAsync(dt.p, truth, (size_t)n_ctrl * truth_k * 4, hipMemcpyHostToDevice, c->stream));
+    WV_HIP(c, hipMemsetAsync(dh.p, 0, (size_t)rounds * n_ctrl * 4, c->stream));
+    wv::NndMonitor m{dc.as<uint32_t>(), dt.as<uint32_t>(), n_ctrl, truth_k, dh.as<uint32_t>()};
+    rc = run_nnd(ix, p, init_ids, init_cnt, stats, false, 0, &m);
+    if (rc) return rc;
+    std::vector<uint32_t> hits((size_t)rounds * n_ctrl);
+    WV_HIP(c, hipMemcpy(hits.data(), dh.p, hits.size() * 4, hipMemcpyDeviceToHost));
+    for (uint32_t r = 0; r < p->iters; ++r) {
+        double sum = 0.0;
+        for (uint32_t i = 0; i < n_ctrl; ++i) sum += (double)hits[(size_t)r * n_ctrl + i] / truth_k;
+        per_round[r] = sum / n_ctrl;
+    }
+    return WV_OK;
 }
 
 int wv_index_build_efanna(wv_index *ix, const wv_nnd_params *p, const uint32_t *init_ids,

@@ -336,6 +336,34 @@ __global__ void __launch_bounds__(256) nnd_m0_kernel(NndDev a) {
     if (lane_id() == 0) a.M[n] = m;
 }
 
+// Per-round monitor (ComponentRefineNNDescent::eval_recall,
+// src/component_refine.cpp:258-275): one wavefront per control node counts
+// the pool's ids that appear in the node's truth row (held in LDS).
+__global__ void __launch_bounds__(256) nnd_monitor_kernel(NndDev a, NndMonitor m, uint32_t round) {
+    extern __shared__ __align__(16) uint8_t smem[];
+    const int wid = threadIdx.x >> 6, lane = lane_id();
+    const uint32_t i = blockIdx.x * (blockDim.x >> 6) + wid;
+    if (i >= m.n_ctrl) return;
+    uint32_t *tr = reinterpret_cast<uint32_t *>(smem) + (size_t)wid * m.truth_k;
+    for (uint32_t j = lane; j < m.truth_k; j += WAVE) tr[j] = m.truth[(size_t)i * m.truth_k + j];
+    wave_sync();
+    const uint32_t c = m.ctrl[i];
+    const uint64_t *pr = a.pool + (size_t)c * a.L;
+    uint32_t hit = 0;
+    for (uint32_t j = lane; j < a.L; j += WAVE) {
+        const uint64_t key = pr[j];
+        if (key == KEY_EMPTY) continue;
+        const uint32_t id = key_id(key);
+        for (uint32_t t = 0; t < m.truth_k; ++t)
+            if (tr[t] == id) {
+                ++hit;
+                break;
+            }
+    }
+    hit = wave_sum_u32(hit);
+    if (lane == 0) m.hits[(size_t)round * m.n_ctrl + i] = hit;
+}
+
 // ComponentRefineEFANNA::init (src/component_refine.cpp:1036-1058): the pool
 // is the initial graph's row (up to `stride` <= L ids, cnt[n] valid) sorted
 // by (dist, id) with self, out-of-range and repeated ids dropped, first L
@@ -1286,7 +1314,8 @@ hipError_t launch_final(const NndDev &a, uint32_t rp2, uint32_t blocks, hipStrea
 
 int nndescent_build(wv_index *ix, const wv_nnd_params *p, const uint32_t *init_ids,
                     const uint32_t *init_cnt, wv_build_stats *stats, uint32_t *graph_out,
-                    float *gdist_out, uint32_t *deg_out, bool efanna, uint32_t init_stride) {
+                    float *gdist_out, uint32_t *deg_out, bool efanna, uint32_t init_stride,
+                    const NndMonitor *monitor) {
     const IndexView v = index_view(ix);
     const hipStream_t s = v.stream;
     const uint32_t N = v.N;
@@ -1473,6 +1502,11 @@ int nndescent_build(wv_index *ix, const wv_nnd_params *p, const uint32_t *init_i
             NND_HIP(hipEventRecord(ev[4], s));
         } else {
             hipLaunchKernelGGL(nnd_m0_kernel, dim3(blocks), dim3(64 * WPB), 0, s, a);
+            NND_HIP(hipGetLastError());
+        }
+        if (monitor) {  // after the round's merge: the pools update() would sort (outside the timed stages)
+            const uint32_t mb = (monitor->n_ctrl + 3) / 4;
+            hipLaunchKernelGGL(nnd_monitor_kernel, dim3(mb), dim3(256), 4 * monitor->truth_k * 4, s, a, *monitor, it);
             NND_HIP(hipGetLastError());
         }
         if (it + 1 < p->iters) {

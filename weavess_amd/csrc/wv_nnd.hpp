@@ -19,6 +19,15 @@ IndexView index_view(wv_index *ix);
 int index_alloc_graph(wv_index *ix, uint32_t deg_max);
 int index_set_error(wv_index *ix, int code, const char *msg);
 
+// Per-round graph recall of control nodes (the reference's debug monitor):
+// after round r, hits[r * n_ctrl + i] = |pool(ctrl[i]) ∩ truth row i|.
+struct NndMonitor {
+    const uint32_t *ctrl;   // n_ctrl control node ids (device)
+    const uint32_t *truth;  // n_ctrl x truth_k exact neighbour ids (device)
+    uint32_t n_ctrl, truth_k;
+    uint32_t *hits;         // iters x n_ctrl (device)
+};
+
 // Implemented in wv_nnd.hip.  Writes the finished K-NN graph into the
 // index's graph buffers (allocated through index_alloc_graph).  efanna:
 // EFANNA's refine, whose pools come from init_ids rows of init_stride ids
@@ -26,7 +35,8 @@ int index_set_error(wv_index *ix, int code, const char *msg);
 // 2S random samples per node; else KGraph's, with init_ids rows of S ids.
 int nndescent_build(wv_index *ix, const wv_nnd_params *p, const uint32_t *init_ids,
                     const uint32_t *init_cnt, wv_build_stats *stats, uint32_t *graph_out,
-                    float *gdist_out, uint32_t *deg_out, bool efanna = false, uint32_t init_stride = 0);
+                    float *gdist_out, uint32_t *deg_out, bool efanna = false, uint32_t init_stride = 0,
+                    const NndMonitor *monitor = nullptr);
 
 // Implemented in wv_capi.hip.  search_collect: greedy searches of Q
 // queries (device rows, dp floats each) over the device graph (graph, deg,

@@ -42,8 +42,9 @@ IndexBuilder *IndexBuilder::init(TYPE type, bool) {
     return this;
 }
 
-IndexBuilder *IndexBuilder::refine(TYPE type, bool) {
+IndexBuilder *IndexBuilder::refine(TYPE type, bool debug) {
     std::unique_ptr<ComponentRefine> a;
+    index_->debug = debug;
     if (type == REFINE_NN_DESCENT || type == REFINE_NN_DESCENT_HIP) {
         std::cout << "__REFINE : KGRAPH (HIP)__" << std::endl;
         a.reset(new ComponentRefineNNDescentHIP(index_, WV_DTYPE_F32));

@@ -19,6 +19,8 @@ inline uint64_t fmix64(uint64_t z) {  // splitmix64 finaliser
 }
 constexpr uint64_t GOLDEN = 0x9E3779B97F4A7C15ull;
 constexpr uint64_t TAG_ENTRY = 0x454E5452ull;
+constexpr uint64_t TAG_CONTROL = 0x4354524Cull;     // "CTRL": the debug monitor's control nodes
+constexpr unsigned CONTROL_NUM = 100;               // include/weavess/index.h:13
 constexpr uint64_t TAG_CENTROID = 0x43454E54ull;     // "CENT": centroid-entry fill
 constexpr uint64_t TAG_SUBCENTROID = 0x53554243ull;  // "SUBC": sub-centroid-entry fill
 uint64_t stream_of(uint64_t seed, uint64_t tag, uint64_t idx) {
@@ -174,12 +176,38 @@ void ComponentRefineNNDescentHIP::RefineInner() {
         }
     }
     wv_index *ix = index->deviceIndex();
-    if (efanna_)
+    if (efanna_) {
         index->check(wv_index_build_efanna(ix, &p, init.empty() ? nullptr : init.data(),
                                            cnt.empty() ? nullptr : cnt.data(), stride, &stats_));
-    else
+    } else if (index->debug) {
+        // NNDescent()'s debug path (component_refine.cpp:106-118): CONTROL_NUM
+        // GenRandom control nodes, their exact top CONTROL_NUM including
+        // themselves (generate_control_set, :241-256), the recall of each
+        // round printed as eval_recall does (:258-275)
+        const unsigned nc = std::min<unsigned>(CONTROL_NUM, N - 1);
+        std::vector<unsigned> ctrl(nc);
+        GenRandomStream(p.seed, TAG_CONTROL, 0, ctrl.data(), nc, N);
+        const unsigned tk = std::min<unsigned>(CONTROL_NUM, N);
+        std::vector<float> rows((size_t)nc * index->getBaseDim());
+        for (unsigned i = 0; i < nc; ++i)
+            std::copy(index->getBaseData() + (size_t)ctrl[i] * index->getBaseDim(),
+                      index->getBaseData() + (size_t)(ctrl[i] + 1) * index->getBaseDim(),
+                      rows.begin() + (size_t)i * index->getBaseDim());
+        std::vector<uint32_t> truth((size_t)nc * tk);
+        std::vector<double> td((size_t)nc * tk), per(p.iters);
+        index->check(wv_exact_knn(index->ctx(), index->getBaseData(), N, rows.data(), nc, index->getBaseDim(), tk,
+                                  -1, truth.data(), td.data(), nullptr));
+        index->check(wv_index_build_nndescent_monitor(ix, &p, init.empty() ? nullptr : init.data(),
+                                                      cnt.empty() ? nullptr : cnt.data(), ctrl.data(), nc,
+                                                      truth.data(), tk, per.data(), &stats_));
+        for (unsigned it = 0; it < p.iters; ++it) {
+            std::cout << "NN-Descent iter: " << it << std::endl;
+            std::cout << "Graph Quality : " << per[it] << std::endl;
+        }
+    } else {
         index->check(wv_index_build_nndescent(ix, &p, init.empty() ? nullptr : init.data(),
                                               cnt.empty() ? nullptr : cnt.data(), &stats_));
+    }
     std::vector<uint32_t> ids((size_t)N * p.K), deg(N);
     std::vector<float> dist((size_t)N * p.K);
     index->check(wv_index_get_graph(ix, ids.data(), dist.data(), deg.data(), nullptr));

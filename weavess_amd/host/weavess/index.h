@@ -88,6 +88,9 @@ public:
     std::vector<unsigned> eps_;
     // Vamana's occlusion factor (include/weavess/index.h:215): 1, then 2
     float alpha = 1;
+    // the reference's debug flag (include/weavess/index.h:1734): NN-Descent
+    // then prints the graph recall of CONTROL_NUM control nodes every round
+    bool debug = false;
 
     // counters (src/component_search_entry.cpp:25, src/component_route.cpp:24,34)
     void addDistCount(uint64_t n = 1) { dist_count_ += n; }

@@ -203,8 +203,8 @@ int main(int argc, char **argv) {
             std::cout << "Init cost: " << builder.GetBuildTime().count() << std::endl;
             if (vamana)  // test/main.cpp:168-174
                 builder.refine(weavess::REFINE_VAMANA)->refine(weavess::REFINE_VAMANA);
-            else
-                builder.refine(refine);
+            else  // WEAVESS_DEBUG=1: the per-round graph-recall printout of the refine's debug path
+                builder.refine(refine, std::getenv("WEAVESS_DEBUG") && std::string(std::getenv("WEAVESS_DEBUG")) == "1");
             if (nsg || dpg || ssg) {  // test/main.cpp:76-81, :106-112, :137-142
                 std::cout << "Init cost: " << builder.GetBuildTime().count() << std::endl;
                 builder.refine(nsg ? weavess::REFINE_NSG : dpg ? weavess::REFINE_DPG : weavess::REFINE_SSG);
