@@ -101,10 +101,16 @@ typedef struct {
     uint32_t waves_per_block;  /* 1, 2 or 4 queries per workgroup (0 -> auto)        */
     uint32_t row_groups;       /* gather tile: 4*row_groups rows in flight (0 -> auto) */
     uint32_t visited_slots;    /* LDS visited table size, power of two >= 64
-                                  (0 -> 8L rounded up to a power of two, clamped
-                                  to [1024, 2048])                                   */
+                                  (0 -> up to L = 256: 8L rounded up to a power
+                                  of two, clamped to [1024, 2048]; up to 512:
+                                  1024; above: 256)                                  */
     uint32_t merge_buffer;     /* keys buffered before a pool merge: 0 auto (256 from
-                                  L = 1536 up), 1 none, or 64/128/192/256           */
+                                  L = 2560 up), 1 none, or 64/128/192/256           */
+    uint32_t schedule;         /* 0 or 1: one wavefront per query (the hardware
+                                  dispatcher hands out workgroups); 2: persistent
+                                  wavefronts, about as many as the GPU holds,
+                                  taking queries from a device work queue, the
+                                  largest per-query L first                          */
 } wv_search_config;
 
 /* Learned early termination: a 2-layer MLP (8 -> 32 ReLU -> 1) over
@@ -345,7 +351,8 @@ int wv_index_search_ext_dev(wv_index *idx, const float *d_queries, uint32_t Q,
  * for wv_index_search_dev; random entries from `seed`, one L and k, and the
  * model of `ext` when given) under candidate tunings, by coordinate descent
  * from the default plan over visited_slots {256..4096}, row_groups {4,8,16},
- * merge_buffer {none,128,256} and waves_per_block {1,2,4}, each candidate
+ * merge_buffer {none,128,256}, waves_per_block {1,2,4} and schedule
+ * {per query, persistent}, each candidate
  * timed as the best of 3 launches after a warm-up and taken only when 2 %
  * faster than the current choice.  The fastest tuning is
  * kept in the index (as wv_index_set_search_config would) and written to

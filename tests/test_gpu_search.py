@@ -183,23 +183,94 @@ def test_search_merge_buffer_integer_ties(ctx, small_int, small_int_graph, L):
     assert np.array_equal(got[2], ref[2]) and np.array_equal(got[3], ref[3])
 
 
-@pytest.mark.skip(reason="persistent work-queue search with an HBM visited bitmap is switched off: "
-                         "its first GPU run faulted and the cause was not found by reading the code")
-def test_search_persistent_queue_many_queries(ctx, small_float, small_float_graph):
-    """More queries than resident wavefronts: every query is taken exactly once
-    from the device work queue, with the exact bitmap behind a tiny table."""
+def _many_queries(base, n, seed):
+    rng = np.random.default_rng(seed)
+    return (base[rng.integers(0, len(base), n)] + rng.normal(0, 0.05, (n, base.shape[1]))).astype(np.float32)
+
+
+@pytest.mark.parametrize("wpb", [1, 2, 4])
+def test_search_persistent_queue_many_queries(ctx, small_float, small_float_graph, wpb):
+    """The persistent schedule (wavefronts taking queries from the device work
+    queue): 40,000 queries, several per wavefront.  Every query is served
+    once, with the grid schedule's ids, distances, evaluations and hops, and
+    the oracle's ids, distances and hops."""
     from weavess_amd import Index
-    base, q = small_float
+    base, _ = small_float
     g, _, deg = small_float_graph
-    rng = np.random.default_rng(9)
-    qq = (base[rng.integers(0, len(base), 3000)] + rng.normal(0, 0.05, (3000, base.shape[1]))).astype(np.float32)
-    ref = O.search_greedy(g, deg, base, qq, 96, 10, seed=4)
+    qq = _many_queries(base, 40_000, 9)
     ix = Index(ctx, base)
     ix.set_graph(g, deg)
-    ix.set_search_config(waves_per_cu=1, visited_slots=128, visited_bitmap=2)
+    ix.set_search_config(waves_per_block=wpb, schedule=1)
+    grid = ix.search(qq, 96, 10, seed=4)
+    ix.set_search_config(waves_per_block=wpb, schedule=2)
     got = ix.search(qq, 96, 10, seed=4)
-    assert np.array_equal(got[0], ref[0]) and np.array_equal(got[1], ref[1])
-    assert np.array_equal(got[2], ref[2]) and np.array_equal(got[3], ref[3])
+    ix.close()
+    for i in range(4):
+        assert np.array_equal(got[i], grid[i]), i
+    ref = O.search_greedy(g, deg, base, qq[:3000], 96, 10, seed=4)
+    assert np.array_equal(got[0][:3000], ref[0]) and np.array_equal(got[1][:3000], ref[1])
+    assert np.array_equal(got[3][:3000], ref[3])
+
+
+def test_search_persistent_budgets_and_model(ctx, small_float, small_float_graph):
+    """Per-query L and k (slots served largest L first) and the learned stop
+    under the persistent schedule equal the grid schedule and the oracle."""
+    from weavess_amd import Index
+    from weavess_amd._capi import TermModel
+    base, _ = small_float
+    g, _, deg = small_float_graph
+    qq = _many_queries(base, 20_000, 3)
+    rng = np.random.default_rng(5)
+    Lq = rng.integers(16, 300, len(qq)).astype(np.uint32)
+    kq = np.minimum(rng.choice([1, 10, 100], len(qq)), Lq).astype(np.uint32)
+    m = TermModel.init(3)
+    m.b2 = -1.0
+    m.logit_threshold = 0.0
+    ix = Index(ctx, base)
+    ix.set_graph(g, deg)
+    out = {}
+    for sched in (1, 2):
+        ix.set_search_config(schedule=sched)
+        out[sched] = (ix.search(qq, 0, 0, seed=6, L_per_q=Lq, k_per_q=kq, k_stride=100),
+                      ix.search_ext(qq, 150, 10, seed=6, model=m))
+    ix.close()
+    for i in range(4):
+        assert np.array_equal(out[2][0][i], out[1][0][i]), i
+        assert np.array_equal(out[2][1][i], out[1][1][i]), i
+    n = 2000
+    ref = O.search_greedy(g, deg, base, qq[:n], 0, 0, seed=6, L_per_q=Lq[:n], k_per_q=kq[:n], k_stride=100)
+    assert np.array_equal(out[2][0][0][:n], ref[0]) and np.array_equal(out[2][0][3][:n], ref[3])
+
+
+def test_search_persistent_graph_replays_and_noalloc(ctx, small_float, small_float_graph):
+    """The persistent schedule inside a captured launch (the counter is zeroed
+    by a kernel of the launch, so every replay serves every query) and in the
+    no-allocation entry point (its counter in the library's array)."""
+    from weavess_amd import Index
+    from weavess_amd.devmem import DeviceArray, pad_rows
+    base, _ = small_float
+    g, _, deg = small_float_graph
+    qq = _many_queries(base, 30_000, 11)
+    Q = len(qq)
+    ix = Index(ctx, base)
+    ix.set_graph(g, deg)
+    ix.set_search_config(schedule=1)
+    ref = ix.search(qq, 64, 10, seed=2)
+    ix.set_search_config(schedule=2)
+    dq = DeviceArray.from_host(pad_rows(qq))
+    d_ids, d_ev = DeviceArray((Q, 10), np.uint32), DeviceArray((Q,), np.uint32)
+    sg = ix.search_graph(dq.ptr, Q, 64, 10, d_ids.ptr, seed=2, d_evals=d_ev.ptr)
+    for rep in range(3):
+        ix.search(qq[:100], 32, 5, seed=1)  # eager work between replays
+        sg.launch()
+        assert np.array_equal(d_ids.to_host(), ref[0]) and np.array_equal(d_ev.to_host(), ref[2]), rep
+    sg.close()
+    for n in (Q, 5000):
+        st = ix.search_dev_noalloc(dq.ptr, n, 64, 10, d_ids.ptr, seed=2, d_evals=d_ev.ptr)
+        assert np.array_equal(d_ids.to_host()[:n], ref[0][:n]) and np.array_equal(d_ev.to_host()[:n], ref[2][:n])
+        assert st["dist_evals"] == int(ref[2][:n].sum()) and st["hops"] == int(ref[3][:n].sum())
+    for a_ in (dq, d_ids, d_ev):
+        a_.free()
     ix.close()
 
 

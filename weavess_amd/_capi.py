@@ -98,7 +98,7 @@ class Graph(C.Structure):
 
 class SearchConfig(C.Structure):
     _fields_ = [("waves_per_block", C.c_uint32), ("row_groups", C.c_uint32),
-                ("visited_slots", C.c_uint32), ("merge_buffer", C.c_uint32)]
+                ("visited_slots", C.c_uint32), ("merge_buffer", C.c_uint32), ("schedule", C.c_uint32)]
 
 
 class TermModel(C.Structure):
@@ -374,8 +374,10 @@ class Index:
         self.ctx.check(self.ctx.lib.wv_index_get_graph(self.h, _p(ids), _p(dist), _p(deg), None))
         return ids, dist, deg
 
-    def set_search_config(self, waves_per_block=0, row_groups=0, visited_slots=0, merge_buffer=0):
-        cfg = SearchConfig(waves_per_block, row_groups, visited_slots, merge_buffer)
+    def set_search_config(self, waves_per_block=0, row_groups=0, visited_slots=0, merge_buffer=0, schedule=0):
+        """Search tuning; schedule 2 serves queries from persistent wavefronts
+        through a device work queue (0/1: one wavefront per query)."""
+        cfg = SearchConfig(waves_per_block, row_groups, visited_slots, merge_buffer, schedule)
         self.ctx.check(self.ctx.lib.wv_index_set_search_config(self.h, C.byref(cfg)))
 
     def build_nndescent(self, K, L, S, R, iters, seed=0, dtype="f32", rerank_mult=0,

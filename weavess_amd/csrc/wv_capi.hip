@@ -39,15 +39,11 @@ struct wv_index {
     float *gdist = nullptr;     // N x deg_max (NaN when unknown)
     uint32_t *deg = nullptr;    // N
     uint32_t deg_max = 0;
-    wv_search_config cfg = {0, 0, 0, 0};
+    wv_search_config cfg = {0, 0, 0, 0, 0};
+    uint32_t *queue = nullptr;  // work-queue counter of the persistent schedule (allocated on first use)
 };
 
-struct wv_search_graph {
-    wv_index *ix = nullptr;
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
-    void *model = nullptr;  // device copy of the stop model, or null
-};
+struct wv_search_graph;
 
 namespace {
 
@@ -108,6 +104,7 @@ int chunks_per_lane(uint32_t nch) {
 
 struct SearchPlan {
     uint32_t P, H, B, wpb, groups, Lmax;
+    bool persistent;
 };
 
 // Pools from this size up get the 256-key merge buffer by default.
@@ -137,6 +134,8 @@ int plan_search(wv_index *ix, uint32_t Lmax, SearchPlan &pl, uint32_t C = 0) {
     while (wpb > 1 && lds * wpb > 160 * 1024) wpb >>= 1;
     if (lds > 160 * 1024) return fail(ix->ctx, WV_ENOTSUP, "pool/visited table need %zu B of LDS", lds);
     pl.wpb = wpb;
+    if (ix->cfg.schedule > 2) return fail(ix->ctx, WV_EINVAL, "schedule must be 0, 1 or 2");
+    pl.persistent = ix->cfg.schedule == 2 && C == 0;
     // gather tile: 4 row groups keep VGPRs (and so residency) low for d <= 256
     pl.groups = ix->cfg.row_groups ? ix->cfg.row_groups : (chunks_per_lane(ix->dp / 4) <= 4 ? 4u : 8u);
     return WV_OK;
@@ -194,6 +193,29 @@ int prepare_search(wv_index *ix, const float *d_q, uint32_t Q, const uint32_t *d
     a.trace_len = ext.trace_len;
     a.settle = ext.settle;
     a.trace_cap = ext.cap;
+    a.queue = nullptr;
+    a.order = nullptr;
+    return WV_OK;
+}
+
+// The persistent schedule's work queue (the index's counter) and, given the
+// per-query budgets, the order of its slots: largest L first (ties by query),
+// so that the longest searches start first.  order_buf holds the order.
+int attach_queue(wv_index *ix, const SearchPlan &pl, const uint32_t *h_L, uint32_t Q, DevBuf &order_buf,
+                 wv::SearchArgs &a) {
+    if (!pl.persistent) return WV_OK;
+    wv_ctx *c = ix->ctx;
+    if (!ix->queue) WV_HIP(c, hipMalloc(&ix->queue, 256));
+    a.queue = ix->queue;
+    if (h_L && Q > 1) {
+        std::vector<uint32_t> ord(Q);
+        std::iota(ord.begin(), ord.end(), 0u);
+        std::stable_sort(ord.begin(), ord.end(), [h_L](uint32_t x, uint32_t y) { return h_L[x] > h_L[y]; });
+        WV_HIP(c, order_buf.alloc((size_t)Q * 4));
+        WV_HIP(c, hipMemcpyAsync(order_buf.p, ord.data(), (size_t)Q * 4, hipMemcpyHostToDevice, c->stream));
+        WV_HIP(c, hipStreamSynchronize(c->stream));
+        a.order = order_buf.as<uint32_t>();
+    }
     return WV_OK;
 }
 
@@ -201,7 +223,7 @@ int search_core(wv_index *ix, const float *d_q, uint32_t Q, const uint32_t *d_en
                 uint64_t seed, uint32_t L, uint32_t k, const uint32_t *d_L, const uint32_t *d_k,
                 uint32_t Lmax, uint32_t k_stride, uint32_t *d_ids, float *d_dist,
                 uint32_t *d_evals, uint32_t *d_hops, wv_search_stats *stats,
-                const SearchExtDev &ext = SearchExtDev()) {
+                const SearchExtDev &ext = SearchExtDev(), const uint32_t *h_L = nullptr) {
     wv_ctx *c = ix->ctx;
     if (!ix->graph) return fail(c, WV_EINVAL, "index has no graph (build or set one first)");
     if (Q == 0) {
@@ -217,14 +239,16 @@ int search_core(wv_index *ix, const float *d_q, uint32_t Q, const uint32_t *d_en
         WV_HIP(c, hop_tmp.alloc((size_t)Q * 4));
         d_hops = hop_tmp.as<uint32_t>();
     }
-    wv::SearchArgs a;
+    wv::SearchArgs a{};
     SearchPlan pl;
     int cpl = 0;
     int rc = prepare_search(ix, d_q, Q, d_entries, seed, L, k, d_L, d_k, Lmax, k_stride, d_ids, d_dist, d_evals,
                             d_hops, ext, a, pl, cpl);
     if (rc) return rc;
+    DevBuf order;
+    if ((rc = attach_queue(ix, pl, h_L, Q, order, a))) return rc;
     WV_HIP(c, hipEventRecord(c->ev0, c->stream));
-    WV_HIP(c, wv::launch_greedy_search(a, cpl, (int)pl.groups, (int)pl.wpb, c->stream));
+    WV_HIP(c, wv::launch_greedy_search(a, cpl, (int)pl.groups, (int)pl.wpb, c->stream, (uint32_t)c->cu_count));
     WV_HIP(c, hipEventRecord(c->ev1, c->stream));
     WV_HIP(c, hipStreamSynchronize(c->stream));
     if (stats) {
@@ -279,6 +303,14 @@ const float *model_field(const void *dev_model, size_t offset) {
 }
 
 }  // namespace
+
+struct wv_search_graph {
+    wv_index *ix = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    void *model = nullptr;  // device copy of the stop model, or null
+    DevBuf order;           // persistent schedule: the slots' query order, or empty
+};
 
 // ------------------------------------------------------------ no allocation
 // Scratch of wv_index_search_dev_noalloc: the per-query evaluation and hop
@@ -409,6 +441,7 @@ int wv_index_destroy(wv_index *ix) {
     if (!ix) return WV_EINVAL;
     std::lock_guard<std::mutex> g(ix->ctx->mu);
     (void)hipSetDevice(ix->ctx->device);
+    if (ix->queue) (void)hipFree(ix->queue);
     if (ix->base) (void)hipFree(ix->base);
     if (ix->graph) (void)hipFree(ix->graph);
     if (ix->gdist) (void)hipFree(ix->gdist);
@@ -479,7 +512,7 @@ int search_collect(wv_index *ix, const uint32_t *graph, const uint32_t *deg, uin
     SearchExtDev x;
     x.full = d_full;
     x.C = C;
-    wv::SearchArgs a;
+    wv::SearchArgs a{};
     SearchPlan pl;
     int cpl = 0;
     int rc = prepare_search(ix, d_q, Q, d_entries, 0, L, 1, nullptr, nullptr, L, 1, ids.as<uint32_t>(), nullptr,
@@ -550,6 +583,7 @@ int wv_index_set_search_config(wv_index *ix, const wv_search_config *cfg) {
         return fail(ix->ctx, WV_EINVAL, "waves_per_block must be 0, 1, 2 or 4");
     if (cfg->merge_buffer > 1 && (cfg->merge_buffer % 64 || cfg->merge_buffer > 256))
         return fail(ix->ctx, WV_EINVAL, "merge_buffer must be 0, 1, 64, 128, 192 or 256");
+    if (cfg->schedule > 2) return fail(ix->ctx, WV_EINVAL, "schedule must be 0, 1 or 2");
     ix->cfg = *cfg;
     return WV_OK;
 }
@@ -592,7 +626,7 @@ int wv_index_search(wv_index *ix, const float *queries, uint32_t Q, const uint32
     if (hops) WV_HIP(c, dh.alloc((size_t)Q * 4));
     rc = search_core(ix, dq.as<float>(), Q, de.as<uint32_t>(), seed, L, k, dl.as<uint32_t>(),
                      dk.as<uint32_t>(), Lmax, k_stride, di.as<uint32_t>(), dd.as<float>(),
-                     dev.as<uint32_t>(), dh.as<uint32_t>(), stats);
+                     dev.as<uint32_t>(), dh.as<uint32_t>(), stats, SearchExtDev(), L_per_q);
     if (rc) return rc;
     WV_HIP(c, hipMemcpy(out_ids, di.p, (size_t)Q * k_stride * 4, hipMemcpyDeviceToHost));
     if (out_dist) WV_HIP(c, hipMemcpy(out_dist, dd.p, (size_t)Q * k_stride * 4, hipMemcpyDeviceToHost));
@@ -631,7 +665,8 @@ int wv_index_search_dev(wv_index *ix, const float *d_queries, uint32_t Q, const 
         if ((rc = check_entries(c, he.data(), he.size(), ix->N))) return rc;
     }
     return search_core(ix, d_queries, Q, d_entries, seed, L, k, d_L_per_q, d_k_per_q, Lmax,
-                       k_stride, d_out_ids, d_out_dist, d_evals, d_hops, stats);
+                       k_stride, d_out_ids, d_out_dist, d_evals, d_hops, stats, SearchExtDev(),
+                       d_L_per_q ? hl.data() : nullptr);
 }
 
 int wv_index_search_dev_noalloc(wv_index *ix, const float *d_queries, uint32_t Q, uint64_t seed, uint32_t L,
@@ -662,19 +697,22 @@ int wv_index_search_dev_noalloc(wv_index *ix, const float *d_queries, uint32_t Q
     auto *tot = reinterpret_cast<unsigned long long *>(static_cast<uint8_t *>(c->scratch) + NOALLOC_COUNTS);
     uint32_t *ev = d_evals ? d_evals : (stats ? s_ev : nullptr);
     uint32_t *hp = d_hops ? d_hops : (stats ? s_hp : nullptr);
-    wv::SearchArgs a;
+    wv::SearchArgs a{};
     SearchPlan pl;
     int cpl = 0;
     rc = prepare_search(ix, d_queries, Q, nullptr, seed, L, k, nullptr, nullptr, Lmax, k_stride, d_out_ids,
                         d_out_dist, ev, hp, SearchExtDev(), a, pl, cpl);
     if (rc) return rc;
+    // the persistent schedule's work-queue counter is in the array too (the
+    // launch zeroes it)
+    if (pl.persistent) a.queue = reinterpret_cast<uint32_t *>(tot + 2);
     // the totals start at zero on every call (the array keeps the last call's)
     if (stats) {
         hipLaunchKernelGGL(totals_clear_kernel, dim3(1), dim3(64), 0, c->stream, tot);
         WV_HIP(c, hipGetLastError());
     }
     WV_HIP(c, hipEventRecord(c->ev0, c->stream));
-    WV_HIP(c, wv::launch_greedy_search(a, cpl, (int)pl.groups, (int)pl.wpb, c->stream));
+    WV_HIP(c, wv::launch_greedy_search(a, cpl, (int)pl.groups, (int)pl.wpb, c->stream, (uint32_t)c->cu_count));
     WV_HIP(c, hipEventRecord(c->ev1, c->stream));
     unsigned long long h_tot[2] = {0, 0};
     if (stats) {
@@ -752,7 +790,7 @@ int wv_index_search_ext(wv_index *ix, const float *queries, uint32_t Q, const ui
     }
     rc = search_core(ix, dq.as<float>(), Q, de.as<uint32_t>(), seed, L, k, dl.as<uint32_t>(),
                      dk.as<uint32_t>(), Lmax, k_stride, di.as<uint32_t>(), dd.as<float>(),
-                     dev.as<uint32_t>(), dh.as<uint32_t>(), stats, x);
+                     dev.as<uint32_t>(), dh.as<uint32_t>(), stats, x, L_per_q);
     if (rc) return rc;
     WV_HIP(c, hipMemcpy(out_ids, di.p, (size_t)Q * k_stride * 4, hipMemcpyDeviceToHost));
     if (out_dist) WV_HIP(c, hipMemcpy(out_dist, dd.p, (size_t)Q * k_stride * 4, hipMemcpyDeviceToHost));
@@ -812,7 +850,7 @@ int wv_index_search_ext_dev(wv_index *ix, const float *d_queries, uint32_t Q, co
         x.settle = ext->settle_hop;
     }
     return search_core(ix, d_queries, Q, d_entries, seed, L, k, d_L_per_q, d_k_per_q, Lmax, k_stride,
-                       d_out_ids, d_out_dist, d_evals, d_hops, stats, x);
+                       d_out_ids, d_out_dist, d_evals, d_hops, stats, x, d_L_per_q ? hl.data() : nullptr);
 }
 
 int wv_index_autotune(wv_index *ix, const float *d_queries, uint32_t Q, uint64_t seed, uint32_t L, uint32_t k,
@@ -848,14 +886,14 @@ int wv_index_autotune(wv_index *ix, const float *d_queries, uint32_t Q, uint64_t
         }
         return WV_OK;
     };
-    wv_search_config cur = {0, 0, 0, 0};
+    wv_search_config cur = {0, 0, 0, 0, 0};
     double cur_ms = 0;
     if ((rc = time_cfg(cur, cur_ms))) {
         ix->cfg = saved;
         return rc;
     }
     static const uint32_t hs[] = {256, 512, 1024, 2048, 4096}, rgs[] = {4, 8, 16}, mbs[] = {1, 128, 256},
-                          wps[] = {1, 2, 4};
+                          wps[] = {1, 2, 4}, scs[] = {1, 2};
     struct Axis {
         uint32_t wv_search_config::*field;
         const uint32_t *vals;
@@ -864,7 +902,8 @@ int wv_index_autotune(wv_index *ix, const float *d_queries, uint32_t Q, uint64_t
     const Axis axes[] = {{&wv_search_config::visited_slots, hs, 5},
                          {&wv_search_config::row_groups, rgs, 3},
                          {&wv_search_config::merge_buffer, mbs, 3},
-                         {&wv_search_config::waves_per_block, wps, 3}};
+                         {&wv_search_config::waves_per_block, wps, 3},
+                         {&wv_search_config::schedule, scs, 2}};
     for (const Axis &ax : axes) {
         const wv_search_config base = cur;
         for (int i = 0; i < ax.n; ++i) {
@@ -922,22 +961,25 @@ int wv_search_graph_create(wv_index *ix, const float *d_queries, uint32_t Q, uin
         WV_HIP(c, hipMemcpy(sg->model, ext->model, sizeof(wv_term_model), hipMemcpyHostToDevice));
         x.model = static_cast<const wv_term_model *>(sg->model);
     }
-    wv::SearchArgs a;
+    wv::SearchArgs a{};
     SearchPlan pl;
     int cpl = 0;
     rc = prepare_search(ix, d_queries, Q, nullptr, seed, L, k, d_L_per_q, d_k_per_q, Lmax, k_stride, d_out_ids,
                         d_out_dist, d_evals, d_hops, x, a, pl, cpl);
+    if (!rc) rc = attach_queue(ix, pl, d_L_per_q ? hl.data() : nullptr, Q, sg->order, a);
     if (rc) {
         if (sg->model) (void)hipFree(sg->model);
         return rc;
     }
     // one eager launch sets the kernel's LDS attribute outside the capture and
-    // fills the outputs once
-    hipError_t e = wv::launch_greedy_search(a, cpl, (int)pl.groups, (int)pl.wpb, c->stream);
+    // fills the outputs once (the persistent schedule's counter is zeroed by
+    // a kernel inside the launch, so every replay starts the queue afresh)
+    const uint32_t cus = (uint32_t)c->cu_count;
+    hipError_t e = wv::launch_greedy_search(a, cpl, (int)pl.groups, (int)pl.wpb, c->stream, cus);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     if (e == hipSuccess) e = hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal);
     if (e == hipSuccess) {
-        hipError_t le = wv::launch_greedy_search(a, cpl, (int)pl.groups, (int)pl.wpb, c->stream);
+        hipError_t le = wv::launch_greedy_search(a, cpl, (int)pl.groups, (int)pl.wpb, c->stream, cus);
         e = hipStreamEndCapture(c->stream, &sg->graph);
         if (e == hipSuccess) e = le;
     }

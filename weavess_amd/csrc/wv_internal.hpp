@@ -37,13 +37,17 @@ struct SearchArgs {
     uint32_t trace_cap;
     uint64_t *full;           // Q x C: the C smallest distinct keys evaluated (NSG candidates), or null
     uint32_t C;
+    uint32_t *queue;          // persistent schedule: the work-queue counter (zeroed by the launch), or null
+    const uint32_t *order;    // persistent schedule: query of each slot (Q), or null (slot i = query i)
 };
 
 // LDS bytes per wavefront; C > 0 adds the collected candidates and a block.
 size_t search_wave_lds(uint32_t P, uint32_t H, uint32_t B, uint32_t C = 0);
-// One wavefront per query (grid of ceil(Q / waves_per_block) workgroups).
+// One wavefront per query (grid of ceil(Q / waves_per_block) workgroups),
+// or, with a.queue, persistent wavefronts taking queries from a device work
+// queue (about cu_count x the workgroups a CU holds).
 hipError_t launch_greedy_search(const SearchArgs &a, int chunks_per_lane, int row_groups,
-                                int waves_per_block, hipStream_t s);
+                                int waves_per_block, hipStream_t s, uint32_t cu_count = 256);
 
 // ------------------------------------------------------------ MLP (wv_mlp.hip)
 hipError_t mlp_grad_launch(const float *params, const float *mean, const float *inv_std, const float *X,

@@ -40,6 +40,8 @@
 #include "wv_mlp.hpp"
 #include "wv_pool.hpp"
 
+#include <algorithm>
+
 namespace wv {
 
 namespace {
@@ -95,15 +97,11 @@ __device__ inline uint64_t kth_union(const uint64_t *a, int na, const uint64_t *
 // WATCH: the variant for the learned stop and per-hop traces (the state
 // features after every expansion, the network in registers); plain launches
 // use the variant without that code.
+// One query, by one wavefront, in its LDS region `my` (every slot it reads
+// is written first, so a region can serve query after query).
 template <int CPL, int G, bool COLLECT, bool WATCH>
-__global__ void __launch_bounds__(256) greedy_search_kernel(SearchArgs a) {
-    extern __shared__ __align__(16) uint8_t smem[];
-    const int wid = (int)(threadIdx.x >> 6);
+__device__ inline void search_query(const SearchArgs &a, uint8_t *my, const uint32_t q) {
     const int lane = lane_id();
-    const uint32_t q = blockIdx.x * (blockDim.x >> 6) + wid;
-    if (q >= a.Q) return;
-
-    uint8_t *my = smem + (size_t)wid * a.wave_lds;
     uint64_t *pool = reinterpret_cast<uint64_t *>(my);         // P keys (P >= L, multiple of 64)
     uint64_t *ckey = pool + a.P;                                // 64
     uint32_t *cid = reinterpret_cast<uint32_t *>(ckey + WAVE);  // 64
@@ -326,6 +324,40 @@ __global__ void __launch_bounds__(256) greedy_search_kernel(SearchArgs a) {
     }
 }
 
+// One wavefront per query: query = its global wavefront index.
+template <int CPL, int G, bool COLLECT, bool WATCH>
+__global__ void __launch_bounds__(256) greedy_search_kernel(SearchArgs a) {
+    extern __shared__ __align__(16) uint8_t smem[];
+    const int wid = (int)(threadIdx.x >> 6);
+    const uint32_t q = blockIdx.x * (blockDim.x >> 6) + wid;
+    if (q >= a.Q) return;
+    search_query<CPL, G, COLLECT, WATCH>(a, smem + (size_t)wid * a.wave_lds, q);
+}
+
+// Persistent form: a grid of about as many wavefronts as the GPU holds at
+// once, each taking query slots from the device work queue `a.queue` (a
+// counter the host zeroes before the launch) until the slots run out.  Lane
+// 0 claims a slot with one atomicAdd and broadcasts it; a slot at or past Q
+// ends the wavefront, and the counter only grows, so every wavefront ends
+// once the Q slots are claimed.  Slot i serves query order[i] (longest
+// budgets first) or query i.
+template <int CPL, int G, bool WATCH>
+__global__ void __launch_bounds__(256) greedy_search_persistent_kernel(SearchArgs a) {
+    extern __shared__ __align__(16) uint8_t smem[];
+    const int wid = (int)(threadIdx.x >> 6);
+    const int lane = lane_id();
+    uint8_t *my = smem + (size_t)wid * a.wave_lds;
+    for (;;) {
+        uint32_t slot = 0;
+        if (lane == 0) slot = atomicAdd(a.queue, 1u);
+        slot = (uint32_t)__shfl((int)slot, 0);
+        if (slot >= a.Q) break;
+        const uint32_t q = a.order ? a.order[slot] : slot;
+        search_query<CPL, G, false, WATCH>(a, my, q);
+        wave_sync();  // the next query rewrites this region
+    }
+}
+
 template <int CPL, int G, bool COLLECT, bool WATCH>
 hipError_t launch_w(const SearchArgs &a, int waves_per_block, hipStream_t s) {
     const size_t lds = (size_t)a.wave_lds * waves_per_block;
@@ -340,9 +372,39 @@ hipError_t launch_w(const SearchArgs &a, int waves_per_block, hipStream_t s) {
                        lds, s, a);
     return hipGetLastError();
 }
+__global__ void queue_reset_kernel(uint32_t *queue) {
+    if (threadIdx.x == 0) *queue = 0u;
+}
+
+template <int CPL, int G, bool WATCH>
+hipError_t launch_p(const SearchArgs &a, int waves_per_block, uint32_t resident_blocks, hipStream_t s) {
+    const size_t lds = (size_t)a.wave_lds * waves_per_block;
+    const void *fn = reinterpret_cast<const void *>(&greedy_search_persistent_kernel<CPL, G, WATCH>);
+    if (lds > 65536) {
+        hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return e;
+    }
+    // as many workgroups as are resident at once (fewer when Q is small)
+    int per_cu = 0;
+    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, WAVE * waves_per_block, lds);
+    if (e != hipSuccess) return e;
+    const uint32_t want = (a.Q + waves_per_block - 1) / waves_per_block;
+    uint32_t blocks = (uint32_t)std::max(per_cu, 1) * resident_blocks;
+    if (blocks > want) blocks = want;
+    hipLaunchKernelGGL(queue_reset_kernel, dim3(1), dim3(64), 0, s, a.queue);
+    hipLaunchKernelGGL((greedy_search_persistent_kernel<CPL, G, WATCH>), dim3(blocks), dim3(WAVE * waves_per_block),
+                       lds, s, a);
+    return hipGetLastError();
+}
+
 template <int CPL, int G, bool COLLECT>
-hipError_t launch_t(const SearchArgs &a, int waves_per_block, hipStream_t s) {
-    if (!COLLECT && (a.model || a.trace)) return launch_w<CPL, G, false, true>(a, waves_per_block, s);
+hipError_t launch_t(const SearchArgs &a, int waves_per_block, uint32_t cu_count, hipStream_t s) {
+    const bool watch = a.model || a.trace;
+    if (!COLLECT && a.queue) {
+        return watch ? launch_p<CPL, G, true>(a, waves_per_block, cu_count, s)
+                     : launch_p<CPL, G, false>(a, waves_per_block, cu_count, s);
+    }
+    if (!COLLECT && watch) return launch_w<CPL, G, false, true>(a, waves_per_block, s);
     return launch_w<CPL, G, COLLECT, false>(a, waves_per_block, s);
 }
 
@@ -355,32 +417,32 @@ size_t search_wave_lds(uint32_t P, uint32_t H, uint32_t B, uint32_t C) {
 }
 
 hipError_t launch_greedy_search(const SearchArgs &a, int chunks_per_lane, int row_groups,
-                                int waves_per_block, hipStream_t s) {
+                                int waves_per_block, hipStream_t s, uint32_t cu_count) {
     if (a.full) {  // candidate collection: one gather shape per row length
         switch (chunks_per_lane) {
-        case 1: return launch_t<1, 8, true>(a, waves_per_block, s);
-        case 2: return launch_t<2, 4, true>(a, waves_per_block, s);
-        case 4: return launch_t<4, 4, true>(a, waves_per_block, s);
-        case 8: return launch_t<8, 2, true>(a, waves_per_block, s);
-        case 16: return launch_t<16, 1, true>(a, waves_per_block, s);
+        case 1: return launch_t<1, 8, true>(a, waves_per_block, cu_count, s);
+        case 2: return launch_t<2, 4, true>(a, waves_per_block, cu_count, s);
+        case 4: return launch_t<4, 4, true>(a, waves_per_block, cu_count, s);
+        case 8: return launch_t<8, 2, true>(a, waves_per_block, cu_count, s);
+        case 16: return launch_t<16, 1, true>(a, waves_per_block, cu_count, s);
         default: return hipErrorInvalidValue;
         }
     }
     switch (chunks_per_lane) {
     case 1:
-        return row_groups >= 16 ? launch_t<1, 16, false>(a, waves_per_block, s)
-                                : launch_t<1, 8, false>(a, waves_per_block, s);
+        return row_groups >= 16 ? launch_t<1, 16, false>(a, waves_per_block, cu_count, s)
+                                : launch_t<1, 8, false>(a, waves_per_block, cu_count, s);
     case 2:
-        return row_groups >= 16 ? launch_t<2, 16, false>(a, waves_per_block, s)
-             : row_groups >= 8 ? launch_t<2, 8, false>(a, waves_per_block, s)
-                               : launch_t<2, 4, false>(a, waves_per_block, s);
+        return row_groups >= 16 ? launch_t<2, 16, false>(a, waves_per_block, cu_count, s)
+             : row_groups >= 8 ? launch_t<2, 8, false>(a, waves_per_block, cu_count, s)
+                               : launch_t<2, 4, false>(a, waves_per_block, cu_count, s);
     case 4:
-        return row_groups >= 8 ? launch_t<4, 8, false>(a, waves_per_block, s)
-                               : launch_t<4, 4, false>(a, waves_per_block, s);
+        return row_groups >= 8 ? launch_t<4, 8, false>(a, waves_per_block, cu_count, s)
+                               : launch_t<4, 4, false>(a, waves_per_block, cu_count, s);
     case 8:
-        return launch_t<8, 2, false>(a, waves_per_block, s);
+        return launch_t<8, 2, false>(a, waves_per_block, cu_count, s);
     case 16:
-        return launch_t<16, 1, false>(a, waves_per_block, s);
+        return launch_t<16, 1, false>(a, waves_per_block, cu_count, s);
     default:
         return hipErrorInvalidValue;
     }
