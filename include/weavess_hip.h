@@ -111,6 +111,11 @@ typedef struct {
                                   wavefronts, about as many as the GPU holds,
                                   taking queries from a device work queue, the
                                   largest per-query L first                          */
+    uint32_t waves_per_query;  /* 0 or 1: one wavefront per query; 2 or 4: that many
+                                  wavefronts of one workgroup share each query
+                                  (the neighbours' visited tests and distances
+                                  split between them); plain search only, the
+                                  learned stop and traces run one wavefront      */
 } wv_search_config;
 
 /* Learned early termination: a 2-layer MLP (8 -> 32 ReLU -> 1) over
@@ -351,8 +356,8 @@ int wv_index_search_ext_dev(wv_index *idx, const float *d_queries, uint32_t Q,
  * for wv_index_search_dev; random entries from `seed`, one L and k, and the
  * model of `ext` when given) under candidate tunings, by coordinate descent
  * from the default plan over visited_slots {256..4096}, row_groups {4,8,16},
- * merge_buffer {none,128,256}, waves_per_block {1,2,4} and schedule
- * {per query, persistent}, each candidate
+ * merge_buffer {none,128,256}, waves_per_block {1,2,4}, schedule
+ * {per query, persistent} and waves_per_query {1,2,4}, each candidate
  * timed as the best of 3 launches after a warm-up and taken only when 2 %
  * faster than the current choice.  The fastest tuning is
  * kept in the index (as wv_index_set_search_config would) and written to

@@ -38,7 +38,7 @@ def test_struct_layouts_match_header():
     assert ctypes.sizeof(_capi.BuildStats) == 80
     assert ctypes.sizeof(_capi.SearchStats) == 24
     assert ctypes.sizeof(_capi.Graph) == 24
-    assert ctypes.sizeof(_capi.SearchConfig) == 20
+    assert ctypes.sizeof(_capi.SearchConfig) == 24
     assert ctypes.sizeof(_capi.NsgParams) == 24
     assert ctypes.sizeof(_capi.NsgStats) == 56
     assert ctypes.sizeof(_capi.VamanaParams) == 32

@@ -274,6 +274,54 @@ def test_search_persistent_graph_replays_and_noalloc(ctx, small_float, small_flo
     ix.close()
 
 
+@pytest.mark.parametrize("wpq", [2, 4])
+@pytest.mark.parametrize("L,mb", [(48, 1), (300, 128), (1500, 256)])
+def test_search_waves_per_query_parity(ctx, small_float, small_float_graph, wpq, L, mb):
+    """Several wavefronts per query (the neighbours' visited tests and
+    distances split between them, wave 0 merging block by block) return the
+    oracle's ids, distances, evaluations and hops (a table large enough to
+    keep the counts exact), with random and with explicit entries."""
+    from weavess_amd import Index
+    base, q = small_float
+    g, _, deg = small_float_graph
+    ref = O.search_greedy(g, deg, base, q, L, 10, seed=8)
+    ix = Index(ctx, base)
+    ix.set_graph(g, deg)
+    ix.set_search_config(merge_buffer=mb, visited_slots=8192, waves_per_query=wpq)
+    got = ix.search(q, L, 10, seed=8)
+    for i in range(4):
+        assert np.array_equal(got[i], ref[i]), i
+    ent = np.stack([np.random.default_rng(r).choice(len(base), L, replace=False) for r in range(len(q))])
+    ent = ent.astype(np.uint32)
+    ref2 = O.search_greedy(g, deg, base, q, L, L, entries=ent)
+    got2 = ix.search(q, L, L, entries=ent)
+    ix.close()
+    for i in range(4):
+        assert np.array_equal(got2[i], ref2[i]), i
+
+
+@pytest.mark.parametrize("wpq", [2, 4])
+def test_search_waves_per_query_wide_rows_and_degrees(ctx, wpq):
+    """d = 960 rows (16 chunks a lane) and a degree-128 graph (more than one
+    wavefront's worth of neighbours a hop), per-query L and k."""
+    from weavess_amd import Index
+    rng = np.random.default_rng(3)
+    base = np.abs(rng.normal(size=(3000, 960))).astype(np.float32)
+    q = np.abs(rng.normal(size=(120, 960))).astype(np.float32)
+    g, _ = O.exact_knn(base, base, 128, self_offset=0)
+    deg = np.full(len(base), 128, np.uint32)
+    Lq = rng.integers(16, 400, len(q)).astype(np.uint32)
+    kq = np.minimum(rng.choice([1, 10, 64], len(q)), Lq).astype(np.uint32)
+    ref = O.search_greedy(g, deg, base, q, 0, 0, seed=2, L_per_q=Lq, k_per_q=kq, k_stride=64)
+    ix = Index(ctx, base)
+    ix.set_graph(g, deg)
+    ix.set_search_config(visited_slots=16384, waves_per_query=wpq)
+    got = ix.search(q, 0, 0, seed=2, L_per_q=Lq, k_per_q=kq, k_stride=64)
+    ix.close()
+    for i in range(4):
+        assert np.array_equal(got[i], ref[i]), i
+
+
 def test_search_rejects_bad_arguments(ctx, small_float, small_float_graph):
     from weavess_amd import WvError
     base, q = small_float

@@ -98,7 +98,8 @@ class Graph(C.Structure):
 
 class SearchConfig(C.Structure):
     _fields_ = [("waves_per_block", C.c_uint32), ("row_groups", C.c_uint32),
-                ("visited_slots", C.c_uint32), ("merge_buffer", C.c_uint32), ("schedule", C.c_uint32)]
+                ("visited_slots", C.c_uint32), ("merge_buffer", C.c_uint32), ("schedule", C.c_uint32),
+                ("waves_per_query", C.c_uint32)]
 
 
 class TermModel(C.Structure):
@@ -374,10 +375,12 @@ class Index:
         self.ctx.check(self.ctx.lib.wv_index_get_graph(self.h, _p(ids), _p(dist), _p(deg), None))
         return ids, dist, deg
 
-    def set_search_config(self, waves_per_block=0, row_groups=0, visited_slots=0, merge_buffer=0, schedule=0):
+    def set_search_config(self, waves_per_block=0, row_groups=0, visited_slots=0, merge_buffer=0, schedule=0,
+                          waves_per_query=0):
         """Search tuning; schedule 2 serves queries from persistent wavefronts
-        through a device work queue (0/1: one wavefront per query)."""
-        cfg = SearchConfig(waves_per_block, row_groups, visited_slots, merge_buffer, schedule)
+        through a device work queue (0/1: one wavefront per query);
+        waves_per_query 2 or 4 lets that many wavefronts share each query."""
+        cfg = SearchConfig(waves_per_block, row_groups, visited_slots, merge_buffer, schedule, waves_per_query)
         self.ctx.check(self.ctx.lib.wv_index_set_search_config(self.h, C.byref(cfg)))
 
     def build_nndescent(self, K, L, S, R, iters, seed=0, dtype="f32", rerank_mult=0,

@@ -39,7 +39,7 @@ struct wv_index {
     float *gdist = nullptr;     // N x deg_max (NaN when unknown)
     uint32_t *deg = nullptr;    // N
     uint32_t deg_max = 0;
-    wv_search_config cfg = {0, 0, 0, 0, 0};
+    wv_search_config cfg = {0, 0, 0, 0, 0, 0};
     uint32_t *queue = nullptr;  // work-queue counter of the persistent schedule (allocated on first use)
 };
 
@@ -105,6 +105,7 @@ int chunks_per_lane(uint32_t nch) {
 struct SearchPlan {
     uint32_t P, H, B, wpb, groups, Lmax;
     bool persistent;
+    uint32_t wpq;  // wavefronts per query (1, 2 or 4)
 };
 
 // Pools from this size up get the 256-key merge buffer by default.
@@ -135,7 +136,13 @@ int plan_search(wv_index *ix, uint32_t Lmax, SearchPlan &pl, uint32_t C = 0) {
     if (lds > 160 * 1024) return fail(ix->ctx, WV_ENOTSUP, "pool/visited table need %zu B of LDS", lds);
     pl.wpb = wpb;
     if (ix->cfg.schedule > 2) return fail(ix->ctx, WV_EINVAL, "schedule must be 0, 1 or 2");
-    pl.persistent = ix->cfg.schedule == 2 && C == 0;
+    pl.wpq = ix->cfg.waves_per_query ? ix->cfg.waves_per_query : 1;
+    if (pl.wpq != 1 && pl.wpq != 2 && pl.wpq != 4) return fail(ix->ctx, WV_EINVAL, "waves_per_query must be 1, 2 or 4");
+    if (C) pl.wpq = 1;
+    if (pl.wpq > 1 && wv::search_coop_lds(pl.P, pl.H, pl.B, pl.wpq) > 160 * 1024)
+        return fail(ix->ctx, WV_ENOTSUP, "pool/visited table need %zu B of LDS",
+                    wv::search_coop_lds(pl.P, pl.H, pl.B, pl.wpq));
+    pl.persistent = ix->cfg.schedule == 2 && C == 0 && pl.wpq == 1;
     // gather tile: 4 row groups keep VGPRs (and so residency) low for d <= 256
     pl.groups = ix->cfg.row_groups ? ix->cfg.row_groups : (chunks_per_lane(ix->dp / 4) <= 4 ? 4u : 8u);
     return WV_OK;
@@ -219,6 +226,14 @@ int attach_queue(wv_index *ix, const SearchPlan &pl, const uint32_t *h_L, uint32
     return WV_OK;
 }
 
+// The plan's launch: W wavefronts a query when the plan asks for them and
+// the search is plain, else one wavefront a query (grid or work queue).
+hipError_t launch_plan(const wv::SearchArgs &a, const SearchPlan &pl, int cpl, wv_ctx *c) {
+    if (pl.wpq > 1 && !a.model && !a.trace && !a.full && !a.queue)
+        return wv::launch_greedy_search_coop(a, cpl, (int)pl.wpq, c->stream);
+    return wv::launch_greedy_search(a, cpl, (int)pl.groups, (int)pl.wpb, c->stream, (uint32_t)c->cu_count);
+}
+
 int search_core(wv_index *ix, const float *d_q, uint32_t Q, const uint32_t *d_entries,
                 uint64_t seed, uint32_t L, uint32_t k, const uint32_t *d_L, const uint32_t *d_k,
                 uint32_t Lmax, uint32_t k_stride, uint32_t *d_ids, float *d_dist,
@@ -248,7 +263,7 @@ int search_core(wv_index *ix, const float *d_q, uint32_t Q, const uint32_t *d_en
     DevBuf order;
     if ((rc = attach_queue(ix, pl, h_L, Q, order, a))) return rc;
     WV_HIP(c, hipEventRecord(c->ev0, c->stream));
-    WV_HIP(c, wv::launch_greedy_search(a, cpl, (int)pl.groups, (int)pl.wpb, c->stream, (uint32_t)c->cu_count));
+    WV_HIP(c, launch_plan(a, pl, cpl, c));
     WV_HIP(c, hipEventRecord(c->ev1, c->stream));
     WV_HIP(c, hipStreamSynchronize(c->stream));
     if (stats) {
@@ -584,6 +599,8 @@ int wv_index_set_search_config(wv_index *ix, const wv_search_config *cfg) {
     if (cfg->merge_buffer > 1 && (cfg->merge_buffer % 64 || cfg->merge_buffer > 256))
         return fail(ix->ctx, WV_EINVAL, "merge_buffer must be 0, 1, 64, 128, 192 or 256");
     if (cfg->schedule > 2) return fail(ix->ctx, WV_EINVAL, "schedule must be 0, 1 or 2");
+    if (cfg->waves_per_query > 4 || cfg->waves_per_query == 3)
+        return fail(ix->ctx, WV_EINVAL, "waves_per_query must be 0, 1, 2 or 4");
     ix->cfg = *cfg;
     return WV_OK;
 }
@@ -712,7 +729,7 @@ int wv_index_search_dev_noalloc(wv_index *ix, const float *d_queries, uint32_t Q
         WV_HIP(c, hipGetLastError());
     }
     WV_HIP(c, hipEventRecord(c->ev0, c->stream));
-    WV_HIP(c, wv::launch_greedy_search(a, cpl, (int)pl.groups, (int)pl.wpb, c->stream, (uint32_t)c->cu_count));
+    WV_HIP(c, launch_plan(a, pl, cpl, c));
     WV_HIP(c, hipEventRecord(c->ev1, c->stream));
     unsigned long long h_tot[2] = {0, 0};
     if (stats) {
@@ -886,14 +903,14 @@ int wv_index_autotune(wv_index *ix, const float *d_queries, uint32_t Q, uint64_t
         }
         return WV_OK;
     };
-    wv_search_config cur = {0, 0, 0, 0, 0};
+    wv_search_config cur = {0, 0, 0, 0, 0, 0};
     double cur_ms = 0;
     if ((rc = time_cfg(cur, cur_ms))) {
         ix->cfg = saved;
         return rc;
     }
     static const uint32_t hs[] = {256, 512, 1024, 2048, 4096}, rgs[] = {4, 8, 16}, mbs[] = {1, 128, 256},
-                          wps[] = {1, 2, 4}, scs[] = {1, 2};
+                          wps[] = {1, 2, 4}, scs[] = {1, 2}, wqs[] = {1, 2, 4};
     struct Axis {
         uint32_t wv_search_config::*field;
         const uint32_t *vals;
@@ -903,7 +920,8 @@ int wv_index_autotune(wv_index *ix, const float *d_queries, uint32_t Q, uint64_t
                          {&wv_search_config::row_groups, rgs, 3},
                          {&wv_search_config::merge_buffer, mbs, 3},
                          {&wv_search_config::waves_per_block, wps, 3},
-                         {&wv_search_config::schedule, scs, 2}};
+                         {&wv_search_config::schedule, scs, 2},
+                         {&wv_search_config::waves_per_query, wqs, 3}};
     for (const Axis &ax : axes) {
         const wv_search_config base = cur;
         for (int i = 0; i < ax.n; ++i) {
@@ -974,12 +992,11 @@ int wv_search_graph_create(wv_index *ix, const float *d_queries, uint32_t Q, uin
     // one eager launch sets the kernel's LDS attribute outside the capture and
     // fills the outputs once (the persistent schedule's counter is zeroed by
     // a kernel inside the launch, so every replay starts the queue afresh)
-    const uint32_t cus = (uint32_t)c->cu_count;
-    hipError_t e = wv::launch_greedy_search(a, cpl, (int)pl.groups, (int)pl.wpb, c->stream, cus);
+    hipError_t e = launch_plan(a, pl, cpl, c);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     if (e == hipSuccess) e = hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal);
     if (e == hipSuccess) {
-        hipError_t le = wv::launch_greedy_search(a, cpl, (int)pl.groups, (int)pl.wpb, c->stream, cus);
+        hipError_t le = launch_plan(a, pl, cpl, c);
         e = hipStreamEndCapture(c->stream, &sg->graph);
         if (e == hipSuccess) e = le;
     }

@@ -43,6 +43,11 @@ struct SearchArgs {
 
 // LDS bytes per wavefront; C > 0 adds the collected candidates and a block.
 size_t search_wave_lds(uint32_t P, uint32_t H, uint32_t B, uint32_t C = 0);
+// LDS bytes of one query served by W cooperating wavefronts.
+size_t search_coop_lds(uint32_t P, uint32_t H, uint32_t B, uint32_t W);
+// W (2 or 4) wavefronts of one workgroup per query (plain search only: no
+// model, traces, collection or work queue).
+hipError_t launch_greedy_search_coop(const SearchArgs &a, int chunks_per_lane, int waves_per_query, hipStream_t s);
 // One wavefront per query (grid of ceil(Q / waves_per_block) workgroups),
 // or, with a.queue, persistent wavefronts taking queries from a device work
 // queue (about cu_count x the workgroups a CU holds).

@@ -372,6 +372,234 @@ hipError_t launch_w(const SearchArgs &a, int waves_per_block, hipStream_t s) {
                        lds, s, a);
     return hipGetLastError();
 }
+// ---------------------------------------------------------------- cooperative
+// W wavefronts of one workgroup serve one query.  Wave 0 owns the pool, the
+// merge buffer and the expansion order; every wave takes a share of the
+// expanded node's neighbours (wave w the w-th 64 of each W x 64), tests them
+// in the shared visited table and computes their distances into its own
+// candidate block; wave 0 then filters, sorts and merges block after block,
+// in neighbour order, exactly as one wavefront merges its 64-neighbour
+// chunks, so the pool and the expansion order are the same.  The entries'
+// distances are split the same way.  Workgroup barriers separate the
+// phases; every wave passes the same barriers because the loop bounds and
+// the stop decision are read from LDS after a barrier.
+template <int CPL, int G, int W>
+__global__ void __launch_bounds__(256) greedy_search_coop_kernel(SearchArgs a) {
+    extern __shared__ __align__(16) uint8_t smem[];
+    const int wid = (int)(threadIdx.x >> 6);
+    const int lane = lane_id();
+    const uint32_t q = blockIdx.x;
+    if (q >= a.Q) return;  // the whole workgroup
+    uint64_t *pool = reinterpret_cast<uint64_t *>(smem);          // P
+    uint64_t *buf = pool + a.P;                                    // B
+    uint64_t *ckeys = buf + a.B;                                   // W x 64
+    uint32_t *cids = reinterpret_cast<uint32_t *>(ckeys + W * WAVE);  // W x 64
+    uint32_t *vis = cids + W * WAVE;                               // H
+    // [0], [1] the node to expand or ID_NONE (alternating by hop, so wave 0
+    // never overwrites a value another wave has yet to read), [2] evals,
+    // [3..3+W) the waves' candidate counts
+    uint32_t *ctl = vis + a.H;
+    uint64_t *ckey = ckeys + wid * WAVE;
+    uint32_t *cid = cids + wid * WAVE;
+
+    const uint32_t N = a.N, nch = a.nch, dp = a.dp;
+    const uint32_t lcap = min(a.P, a.entries ? N : (N > 1 ? N - 1 : 1u));
+    const uint32_t L = min(max(a.L_per_q ? a.L_per_q[q] : a.L, 1u), lcap);
+    const uint32_t kq = min(max(a.k_per_q ? a.k_per_q[q] : a.k, 1u), min(L, a.k_stride));
+    float4 qv[CPL];
+    {
+        const float4 *qp = reinterpret_cast<const float4 *>(a.queries + (size_t)q * dp);
+#pragma unroll
+        for (int c = 0; c < CPL; ++c) {
+            const int ch = (lane & 15) + 16 * c;
+            qv[c] = ch < (int)nch ? qp[ch] : make_float4(0.f, 0.f, 0.f, 0.f);
+        }
+    }
+    // ---- entries: ids into the pool's storage (wave 0), then each round
+    // every wave scores one 64-id chunk and sorts it in its block, and wave 0
+    // merges the round's chunks in order (a merge writes only below the
+    // merged length; later rounds' ids sit above it)
+    if (wid == 0) {
+        if (a.entries) {
+            for (uint32_t i = lane; i < L; i += WAVE) {
+                const uint32_t e = a.entries[(size_t)q * L + i];
+                pool[i] = e < N ? e : N - 1;
+            }
+            wave_sync();
+        } else {
+            gen_random_lds(stream_of(a.seed, TAG_ENTRY, q), L, N, pool, ckey);
+        }
+    }
+    __syncthreads();
+    int len = 0;
+    for (uint32_t r0 = 0; r0 < L; r0 += W * WAVE) {
+        const uint32_t c0 = r0 + (uint32_t)wid * WAVE;
+        if (c0 < L) {
+            const int cnt = (int)min((uint32_t)WAVE, L - c0);
+            if (lane < cnt) cid[lane] = (uint32_t)pool[c0 + lane];
+            wave_sync();
+            rows_dist<CPL, G>(a.base, dp, nch, qv, cid, cnt, ckey);
+            wave_sync();
+            const uint64_t key = wave_sort64(lane < cnt ? ckey[lane] : KEY_EMPTY);
+            wave_sync();
+            ckey[lane] = key;
+            wave_sync();
+        }
+        __syncthreads();
+        if (wid == 0) {
+            for (int w = 0; w < W; ++w) {
+                const uint32_t cw = r0 + (uint32_t)w * WAVE;
+                if (cw >= L) break;
+                const int cnt = (int)min((uint32_t)WAVE, L - cw);
+                merge_into(pool, len, (int)L, ckeys + w * WAVE, cnt, true);
+                len += cnt;
+            }
+        }
+        __syncthreads();
+    }
+    if (wid == 0)
+        for (uint32_t i = L + lane; i < a.P; i += WAVE) pool[i] = KEY_EMPTY;
+    const uint32_t hmask = a.H - 1;
+    for (uint32_t i = threadIdx.x; i < a.H; i += blockDim.x) vis[i] = ID_NONE;
+    if (threadIdx.x == 0) ctl[2] = L;
+    __syncthreads();
+
+    uint32_t evals = L, hops = 0;  // wave 0's
+    uint32_t step = 0;             // every wave's count of selection steps
+    const int Li = (int)L;
+    const int B = (int)a.B;
+    int kp = 0, nb = 0, kb = 0;
+    uint64_t worst = canon(pool[Li - 1]);
+    for (;;) {
+        if (wid == 0) {
+            const int kk = first_flag(pool, kp, Li);
+            const int jb = nb ? first_flag(buf, kb, nb) : 0;
+            const uint64_t ka = kk < Li ? canon(pool[kk]) : KEY_EMPTY;
+            const uint64_t kbk = jb < nb ? canon(buf[jb]) : KEY_EMPTY;
+            const bool from_buf = kbk < ka;
+            const uint64_t cur = from_buf ? kbk : ka;
+            const bool done = cur == KEY_EMPTY || cur > worst;
+            wave_sync();
+            if (!done) {
+                if (lane == 0) {
+                    if (from_buf) buf[jb] = cur;
+                    else pool[kk] = cur;
+                }
+                if (from_buf) kb = jb + 1;
+                else kp = kk + 1;
+                ++hops;
+            }
+            if (lane == 0) ctl[step & 1] = done ? ID_NONE : key_id(cur);
+        }
+        __syncthreads();
+        const uint32_t u = ctl[step & 1];
+        ++step;
+        if (u == ID_NONE) break;
+        // probes as one wavefront would choose them from the evaluations so far
+        const int probes = ctl[2] - L > a.H ? VIS_PROBES_FULL : VIS_PROBES;
+        const uint32_t *adj = a.graph + (size_t)u * a.deg_max;
+        uint32_t du = a.deg[u];
+        if (du > a.deg_max) du = a.deg_max;
+        for (uint32_t m0 = 0; m0 < du; m0 += W * WAVE) {
+            const uint32_t mi = m0 + (uint32_t)wid * WAVE + lane;
+            const uint32_t id = mi < du ? adj[mi] : ID_NONE;
+            const bool fresh = id < N && visit_first(vis, hmask, id, probes) != 0;
+            const uint64_t bm = __ballot(fresh);
+            const int m = __popcll(bm);
+            if (fresh) cid[lanes_below(bm)] = id;
+            wave_sync();
+            if (m > 0) rows_dist<CPL, G>(a.base, dp, nch, qv, cid, m, ckey);
+            if (lane == 0) ctl[3 + wid] = (uint32_t)m;
+            __syncthreads();
+            if (wid == 0) {
+                for (int w = 0; w < W; ++w) {
+                    const int mw = (int)ctl[3 + w];
+                    if (mw == 0) continue;
+                    uint64_t *blk = ckeys + w * WAVE;
+                    evals += (uint32_t)mw;
+                    uint64_t key = lane < mw ? blk[lane] : KEY_EMPTY;
+                    if (key < worst) {
+                        int plo = 0, phi = Li, blo = 0, bhi = nb;
+                        while (plo < phi || blo < bhi) {
+                            if (plo < phi) {
+                                const int mid = (plo + phi) >> 1;
+                                if (canon(pool[mid]) < key) plo = mid + 1;
+                                else phi = mid;
+                            }
+                            if (blo < bhi) {
+                                const int mid = (blo + bhi) >> 1;
+                                if (canon(buf[mid]) < key) blo = mid + 1;
+                                else bhi = mid;
+                            }
+                        }
+                        if (canon(pool[plo]) == key || (blo < nb && canon(buf[blo]) == key)) key = KEY_EMPTY;
+                    } else {
+                        key = KEY_EMPTY;
+                    }
+                    const uint64_t lm = __ballot(key != KEY_EMPTY);
+                    if (lm == 0) continue;
+                    const int c = __popcll(lm);
+                    wave_sync();
+                    if (key != KEY_EMPTY) blk[lanes_below(lm)] = key;
+                    wave_sync();
+                    int wd = 2;
+                    while (wd < c) wd <<= 1;
+                    key = wave_sort64_n(lane < c ? blk[lane] : KEY_EMPTY, wd);
+                    wave_sync();
+                    if (lane < c) blk[lane] = key;
+                    wave_sync();
+                    if (B == 0) {
+                        const int pos0 = merge_into(pool, Li, Li, blk, c, true);
+                        if (pos0 < kp) kp = pos0;
+                        worst = canon(pool[Li - 1]);
+                    } else {
+                        if (nb + c > B) {
+                            const int pos0 = merge_into(pool, Li, Li, buf, nb, false);
+                            if (pos0 < kp) kp = pos0;
+                            nb = 0;
+                            kb = 0;
+                        }
+                        const int bp = merge_into(buf, nb, B, blk, c, true);
+                        nb += c;
+                        if (bp < kb) kb = bp;
+                        worst = kth_union(pool, Li, buf, nb, Li - 1);
+                    }
+                }
+                if (lane == 0) ctl[2] = evals;
+            }
+            __syncthreads();  // the blocks and counts are rewritten next
+        }
+    }
+    if (wid != 0) return;
+    if (nb) merge_into(pool, Li, Li, buf, nb, false);
+    for (uint32_t i = lane; i < a.k_stride; i += WAVE) {
+        const bool ok = i < kq;
+        a.out_ids[(size_t)q * a.k_stride + i] = ok ? entry_id(pool[i]) : ID_NONE;
+        if (a.out_dist) a.out_dist[(size_t)q * a.k_stride + i] = ok ? key_dist(pool[i]) : -1.0f;
+    }
+    if (lane == 0) {
+        if (a.evals) a.evals[q] = evals;
+        if (a.hops) a.hops[q] = hops;
+        if (a.settle) a.settle[q] = 0;
+    }
+}
+
+template <int CPL, int G>
+hipError_t launch_c(const SearchArgs &a, int wpq, hipStream_t s) {
+    const size_t lds = search_coop_lds(a.P, a.H, a.B, (uint32_t)wpq);
+    const void *fn = wpq == 4 ? reinterpret_cast<const void *>(&greedy_search_coop_kernel<CPL, G, 4>)
+                              : reinterpret_cast<const void *>(&greedy_search_coop_kernel<CPL, G, 2>);
+    if (lds > 65536) {
+        hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return e;
+    }
+    if (wpq == 4)
+        hipLaunchKernelGGL((greedy_search_coop_kernel<CPL, G, 4>), dim3(a.Q), dim3(4 * WAVE), lds, s, a);
+    else
+        hipLaunchKernelGGL((greedy_search_coop_kernel<CPL, G, 2>), dim3(a.Q), dim3(2 * WAVE), lds, s, a);
+    return hipGetLastError();
+}
+
 __global__ void queue_reset_kernel(uint32_t *queue) {
     if (threadIdx.x == 0) *queue = 0u;
 }
@@ -414,6 +642,24 @@ size_t search_wave_lds(uint32_t P, uint32_t H, uint32_t B, uint32_t C) {
     size_t b = (size_t)P * 8 + WAVE * 8 + WAVE * 4 + (size_t)H * 4 + (size_t)B * 8;
     if (C) b += (size_t)C * 8 + WAVE * 8 + 2 * WAVE * 8;
     return (b + 15) & ~(size_t)15;
+}
+
+size_t search_coop_lds(uint32_t P, uint32_t H, uint32_t B, uint32_t W) {
+    const size_t b = (size_t)P * 8 + (size_t)B * 8 + (size_t)W * WAVE * 12 + (size_t)H * 4 + 64;
+    return (b + 15) & ~(size_t)15;
+}
+
+hipError_t launch_greedy_search_coop(const SearchArgs &a, int chunks_per_lane, int waves_per_query, hipStream_t s) {
+    if (a.full || a.model || a.trace || a.queue || (waves_per_query != 2 && waves_per_query != 4))
+        return hipErrorInvalidValue;
+    switch (chunks_per_lane) {
+    case 1: return launch_c<1, 8>(a, waves_per_query, s);
+    case 2: return launch_c<2, 4>(a, waves_per_query, s);
+    case 4: return launch_c<4, 4>(a, waves_per_query, s);
+    case 8: return launch_c<8, 2>(a, waves_per_query, s);
+    case 16: return launch_c<16, 1>(a, waves_per_query, s);
+    default: return hipErrorInvalidValue;
+    }
 }
 
 hipError_t launch_greedy_search(const SearchArgs &a, int chunks_per_lane, int row_groups,
