@@ -11,33 +11,45 @@ Default workload "c5" (the configuration the metric is quoted on):
   * Graph: NN-Descent on the GPU, K=48 (L=128, S=32, R=128, 20 rounds,
     BF16 join with FP32 rerank).
   * Plain: the smallest L whose recall@10 (exact FP64 kNN, first 20,000
-    workload queries) reaches 0.95.
-  * Learned: the 8 -> 32 -> 1 stop network, trained on the GPU on at most
-    500,000 per-expansion traces of the training queries at pool cap
-    1.5 x L_plain, threshold calibrated on the calibration queries for
-    recall 0.95 + 0.003; searched with the same cap.
+    workload queries) reaches the target, by sweep and bisection; at 0.95
+    (the value) and at 0.99.
+  * Learned, at each target: the 8 -> 32 -> 1 stop network, trained on the
+    GPU on at most 500,000 per-expansion traces of the training queries at
+    pool cap 1.3 x L_plain, threshold calibrated on the calibration queries
+    for target + 0.003, searched at that cap.  Beside it, the ceiling of any
+    stop rule: plain search at L_plain stopped at each query's settle hop.
   * NN-Descent leg: the 1M x 128 Zipf mixture (K=64, S=16, 12 rounds) with
-    L = 64, R = 128 and 128 candidate slots per node (the fastest settings
-    swept that reach graph recall 0.95), built in FP32 and in BF16 (FP32
-    rerank); graph recall on 10,000 control nodes against exact FP64 kNN.
+    L = 64, R = 128 and 128 candidate slots per node, built in FP32 and in
+    BF16 (FP32 rerank); graph recall on 10,000 control nodes against exact
+    FP64 kNN; the join's roofline; graph recall with 256 and 1,024 slots;
+    the 1M x 128 configuration's fixed-L search (L 20-160) on the FP32 graph.
+  * Batching leg: per-query L ~ U{16..400}, k in {1, 10, 100} over the
+    200,000 queries in one autotuned launch on the 2M x 96 graph.
+  * CPU baseline (host cores, the oracle): plain search on a bounded sample
+    of the workload queries at L_plain (ids compared with the GPU's), one
+    thread too; NN-Descent at 10k x 128 with the reference's exact bounded
+    insertion over 3 seeds, beside the GPU's graph recall; the 10k x 128
+    search at L=100.
 
 A "step" is one search pass over the 200,000 workload queries, with base,
 graph, queries, model and outputs resident in HBM.  Each method's search
 tuning is picked by the library's autotuner on 8,192 workload queries before
-its timed steps (results never depend on it).  Exactly --steps steps of each
-method are timed after --warmup untimed ones, bracketed by device
-synchronisation.  `value` is the QPS of the faster method among those that
-reach the target recall on the sample (plain fixed-L search is the fallback
-the learned stop is judged against); both are reported.
-Prints ONE JSON line on stdout; progress goes to stderr.
+its timed steps (results never depend on it).  Exactly --steps steps of the
+0.95 methods are timed after --warmup untimed ones (at most 2 at 0.99),
+bracketed by device synchronisation.  `value` is the QPS of the faster
+method among those that reach the target recall on the sample (plain
+fixed-L search is the fallback the learned stop is judged against); both
+are reported.  Prints ONE JSON line on stdout; progress goes to stderr.
 
 --workload c2 / c1: plain fixed-L search only, on the 1M x 128 (K=64,
 L=R=64 graph) or the 10k x 128 integer mixture (K=50, L=70, S=10, R=100).
 --workload c3: NN-Descent precision on 1M x 960 (FP32 / BF16 / FP8 joins):
 graph recall and distance error against FP64.
 --workload nnd: the NN-Descent leg alone.
---workload c4: one batched launch over 200,000 queries with per-query
-L ~ U{16..400} and k in {1, 10, 100} on the 2M x 96 graph.
+--workload c4: the batching configuration alone.
+--workload graphs: the L recall 0.95 / 0.99 need on the bench's NN-Descent
+graph, a denser NN-Descent build and the exact K=48 kNN graph.
+--workload refine: the NSG / SSG / DPG / Vamana refiners and EFANNA.
 """
 from __future__ import annotations
 
@@ -137,12 +149,32 @@ def join_roofline(st, N, d, kdim, L, dt, traffic_per_launch=None):
     return out
 
 
+def c2_search(ctx, ix, base, q, g, dg):
+    """The 1M x 128 configuration's search: fixed L in {20, 40, 80, 160},
+    k = 10, seeded random entries, on the FP32 NN-Descent graph (K=64 S=16
+    12 rounds); recall@10 on all 10,000 queries against FP64 exact kNN, ids
+    and distances compared with the oracle on the first 500."""
+    import oracle  # the checker
+    gt, _, _ = ctx.exact_knn(base, q, 10)
+    rows = []
+    for L in (20, 40, 80, 160):
+        ids, dist, ev, hp, st = ix.search(q, L, 10, seed=7)
+        rid, rdist, _, _ = oracle.search_greedy(g, dg, base, q[:500], L, 10, seed=7)
+        rows.append(dict(L=L, recall_at_10=round(recall_at_k(ids, gt, 10), 4),
+                         qps=round(len(q) / (st["kernel_ms"] / 1e3)), dist_evals_per_query=round(float(ev.mean()), 1),
+                         ids_bit_exact_vs_oracle_500=bool(np.array_equal(ids[:500], rid) and
+                                                          np.array_equal(dist[:500], rdist))))
+    log(f"c2 search: {rows}")
+    return {"workload": "1M x 128 Zipf GMM, 10,000 queries, FP32 NN-Descent graph K=64 (L=64 S=16 R=128 "
+                        "iters=12), fixed-L greedy search k=10 from seeded random entries", "sweep": rows}
+
+
 def nndescent_leg(ctx, Index, datagen):
     """FP32 and BF16 NN-Descent on the 1M x 128 mixture with the leg's
     settings, with the join's roofline, and the BF16 build's graph recall at
     larger candidate tables (towards the reference's exact insertion)."""
     w = NND_LEG
-    base, _ = getattr(datagen, w["gen"])(n_query=1)
+    base, cq = getattr(datagen, w["gen"])(n_query=10_000)
     N, d = base.shape
     ix = Index(ctx, base)
     ctl = min(w["control"], len(base))
@@ -169,6 +201,8 @@ def nndescent_leg(ctx, Index, datagen):
                        finalize_ms=round(st["finalize_ms"], 1), pairs=int(st["pairs"]),
                        roofline=join_roofline(st, N, d, kdim[dt], w["L"], dt, traffic.get(dt)))
         log(f"nndescent leg {dt}: {st['total_ms']:.1f} ms, graph recall {gr:.4f}, join {out[dt]['roofline']}")
+        if dt == "f32":
+            out["c2_search"] = c2_search(ctx, ix, base, cq, g, dg)
     out["bf16_speedup_vs_f32"] = round(out["f32"]["build_s"] / out["bf16"]["build_s"], 3)
     out["bf16_join_speedup_vs_f32"] = round(out["f32"]["join_ms"] / out["bf16"]["join_ms"], 3)
     slots = {}
@@ -478,6 +512,48 @@ def run_refine(args, ctx, Index, datagen):
                                   "refiners on the GPU; greedy search k=10"},
            "results": res, "cpu_baseline": cpu}
     print(json.dumps(out), flush=True)
+
+
+def batched_leg(ctx, ix, base, queries, dq, datagen, args):
+    """The batching configuration on the same graph: one launch over the
+    200,000 queries with per-query L ~ U{16..400} and k in {1, 10, 100}
+    (L_eff = max(L, k)), tuned by the autotuner (gather tile, visited table,
+    buffer, workgroup packing, schedule: grid or persistent work queue,
+    wavefronts per query); recall@k per k on 4,000 queries."""
+    from weavess_amd.devmem import DeviceArray, synchronize
+    Q = len(queries)
+    Lq, kq = datagen.c4_query_budgets(Q)
+    dL, dk = DeviceArray.from_host(Lq), DeviceArray.from_host(kq)
+    d_ids = DeviceArray((Q, 100), np.uint32)
+    t1 = time.time()
+    tuning, _ = ix.autotune_dev(dq.ptr, 8192, int(Lq.max()), 100, seed=7)
+    log(f"batched: autotuned {tuning} ({time.time() - t1:.1f} s)")
+
+    def run():
+        return ix.search_dev(dq.ptr, Q, 0, 0, d_ids.ptr, seed=7, d_L=dL.ptr, d_k=dk.ptr, k_stride=100)
+    run()
+    synchronize()
+    steps = min(args.steps, 3)
+    kern = []
+    t1 = time.perf_counter()
+    for _ in range(steps):
+        kern.append(run()["kernel_ms"])
+    synchronize()
+    ms = (time.perf_counter() - t1) * 1e3 / steps
+    ns = 4000
+    gt, _, _ = ctx.exact_knn(base, queries[:ns], 100)
+    ids = d_ids.to_host()[:ns]
+    rec = {}
+    for kk in (1, 10, 100):
+        m = kq[:ns] == kk
+        rec[f"k={kk}"] = dict(queries=int(m.sum()), recall=round(recall_at_k(ids[m], gt[m], kk), 4))
+    for a_ in (dL, dk, d_ids):
+        a_.free()
+    out = dict(workload="2M x 96, 200,000 queries, per-query L ~ U{16..400}, k in {1,10,100}, one launch",
+               qps=round(Q / (ms / 1e3), 1), ms_per_step=round(ms, 3), kernel_ms=round(float(np.mean(kern)), 3),
+               steps=steps, tuning=tuning, recall_by_k=rec)
+    log(f"batched: {out}")
+    return out
 
 
 def learned_leg(ctx, ix, allq, n_work, w, L_plain, target, k, run_learned, timed, steps, warmup, gt, ns, Q,
@@ -795,8 +871,26 @@ def main():
             "graph_recall_gpu": [round(x, 5) for x in gpu_rec],
             "max_abs_gap_gpu_vs_exact_pt": round(100 * max(abs(a_ - b_) for a_, b_ in zip(gpu_rec, cpu_rec)), 3),
             "gpu_equals_oracle_slot_mode_bit_exact": bool(np.array_equal(so, g1_first[:, :w1["K"]]))}
+        # the 10k configuration's search: k = 10, L = 100, random entries
+        b1q = getattr(datagen, w1["gen"])(**w1["gen_kw"])[1]
+        gq1, _, _ = ctx.exact_knn(b1, b1q, 10)
+        ix1 = Index(ctx, b1)
+        ix1.build_nndescent(*prm, seed=1, dtype="f32")
+        g1b, _, d1b = ix1.get_graph()
+        s_ids, s_dist, _, _, s_st = ix1.search(b1q, 100, 10, seed=7)
+        ix1.close()
+        c0 = time.perf_counter()
+        r_ids, r_dist, _, _ = oracle.search_greedy(g1b, d1b, b1, b1q, 100, 10, seed=7, nthreads=cores)
+        c1_cpu_s = time.perf_counter() - c0
+        out["cpu_baseline"]["c1_search"] = {
+            "workload": "10k x 128, 1,000 queries, k=10, L=100, random entries, on the GPU's C1 graph (seed 1)",
+            "gpu_qps": round(len(b1q) / (s_st["kernel_ms"] / 1e3)), "cpu_qps": round(len(b1q) / c1_cpu_s),
+            "cores": cores, "recall_at_10": round(recall_at_k(s_ids, gq1, 10), 4),
+            "ids_bit_exact_vs_oracle": bool(np.array_equal(s_ids, r_ids) and np.array_equal(s_dist, r_dist))}
         log(f"cpu baseline single thread {out['cpu_baseline']['single_thread_qps']} QPS; nndescent 10k "
             f"{out['cpu_baseline']['nndescent_10k']}")
+    if w["learned"]:  # the 2M x 96 graph also serves the batching configuration
+        out["batched_c4"] = batched_leg(ctx, ix, base, queries, dq, datagen, args)
     # roofline of the served method's launch: algorithmic bytes = rows
     # gathered for the reference's evaluation count (the kernel's count less
     # the re-evaluations of its forgetful visited table, measured against the
