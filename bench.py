@@ -619,6 +619,9 @@ def main():
     ap.add_argument("--target", type=float, default=0.95)
     ap.add_argument("--no-cpu", action="store_true", help="skip the host-core baseline")
     ap.add_argument("--no-nnd-leg", action="store_true", help="skip the 1M x 128 NN-Descent leg")
+    ap.add_argument("--fixed-L", type=int, default=0,
+                    help="profiling run: plain search at this L only (no L sweep, learned stop, second "
+                         "target or batching leg)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -634,7 +637,9 @@ def main():
          "graphs": run_graphs}[args.workload](args, ctx, Index, datagen)
         ctx.close()
         return
-    w = WORKLOADS[args.workload]
+    w = dict(WORKLOADS[args.workload])
+    if args.fixed_L:
+        w.update(sweep=[args.fixed_L], learned=False)
     nnd_leg = None
     if not args.no_nnd_leg and args.workload == "c5":
         nnd_leg = nndescent_leg(ctx, Index, datagen)
@@ -730,7 +735,7 @@ def main():
     for ti, tgt in enumerate(targets):
         main_t = ti == 0
         steps, warm = (args.steps, args.warmup) if main_t else (min(args.steps, 2), min(args.warmup, 1))
-        L_t, reached = l_for(tgt)
+        L_t, reached = (args.fixed_L, measure(args.fixed_L)["recall"] >= tgt) if args.fixed_L else l_for(tgt)
         op = measure(L_t)
         log(f"plain operating point for {tgt}: L={L_t}, recall {op['recall']:.4f}" +
             ("" if reached else f" (target not reached at L <= {L_MAX})"))

@@ -243,8 +243,10 @@ __device__ inline void rows_dist(const float *__restrict__ base, uint32_t dp, ui
     const int lane = lane_id(), sub = lane & 15, grp = lane >> 4;
     for (int r0 = 0; r0 < m; r0 += 4 * G) {
         float4 v[G][CPL];
+        // groups wholly past m (the same for the whole wavefront) are skipped
 #pragma unroll
         for (int g = 0; g < G; ++g) {
+            if (r0 + 4 * g >= m) break;
             const int row = r0 + 4 * g + grp;
             const uint32_t id = row < m ? ids[row] : 0u;
             const float4 *p = reinterpret_cast<const float4 *>(base + (size_t)id * dp);
@@ -256,6 +258,7 @@ __device__ inline void rows_dist(const float *__restrict__ base, uint32_t dp, ui
         }
 #pragma unroll
         for (int g = 0; g < G; ++g) {
+            if (r0 + 4 * g >= m) break;
             const int row = r0 + 4 * g + grp;
             float acc = 0.f;
 #pragma unroll

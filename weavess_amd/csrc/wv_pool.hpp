@@ -35,6 +35,39 @@ __device__ inline int upper_bound_e(const uint64_t *a, int n, uint64_t key) {
     return lo;
 }
 
+// One round of an 8-ary search for the number of entries < key in a sorted
+// range: the answer lies in [lo, hi] (entries before lo are < key, from hi
+// on >= key) and hi - lo >= 16.  The 7 slots lo + j*s (s = (hi-lo)/8, j =
+// 1..7) are read at once -- independent LDS reads instead of a chain of
+// dependent ones -- and the range narrows to the part between the last slot
+// below key and the first one not below it.
+__device__ inline void narrow8_e(const uint64_t *a, int &lo, int &hi, uint64_t key) {
+    const int s = (hi - lo) >> 3;
+    uint64_t x[7];
+#pragma unroll
+    for (int j = 0; j < 7; ++j) x[j] = canon(a[lo + (j + 1) * s]);
+    int t = 0;
+#pragma unroll
+    for (int j = 0; j < 7; ++j) t += x[j] < key ? 1 : 0;
+    const int nlo = t > 0 ? lo + t * s + 1 : lo;
+    hi = t < 7 ? lo + (t + 1) * s : hi;
+    lo = nlo;
+}
+
+// Number of entries of the sorted a[0..n) whose key is < key: 8-ary rounds
+// while the range holds 16 or more slots, then binary steps.  The result is
+// lower_bound_e's; only the reads are fewer rounds deep.
+__device__ inline int lower_bound8_e(const uint64_t *a, int n, uint64_t key) {
+    int lo = 0, hi = n;
+    while (hi - lo >= 16) narrow8_e(a, lo, hi, key);
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (canon(a[mid]) < key) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
 // Merge c sorted entries src[0..c) into the sorted array arr[0..len) of
 // capacity cap, in place: an entry of arr moves right by the number of
 // smaller src keys, src[j] lands at j + the number of arr keys <= it (so
@@ -55,7 +88,7 @@ __device__ inline int merge_into(uint64_t *arr, int len, int cap, const uint64_t
         ck[r] = 0;
         if (j < c) {
             ck[r] = mark ? (src[j] | FBIT) : src[j];
-            cpos[r] = j + upper_bound_e(arr, len, canon(ck[r]));
+            cpos[r] = j + lower_bound8_e(arr, len, canon(ck[r]) + 1);  // arr keys <= it
         }
     }
     const int pos0 = __shfl(cpos[0], 0);  // lane 0 holds src[0]: j = 0

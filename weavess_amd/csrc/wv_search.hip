@@ -249,10 +249,15 @@ __device__ inline void search_query(const SearchArgs &a, uint8_t *my, const uint
             if (COLLECT) collect_stage(full, flen, (int)a.C, ckey, m, fstg, fsn, fblk);
             uint64_t key = lane < m ? ckey[lane] : KEY_EMPTY;
             if (key < worst) {
-                // already in pool or buffer: a duplicate.  Both binary searches
-                // advance in one loop so their LDS reads overlap; worst <=
+                // already in pool or buffer: a duplicate.  Both searches
+                // advance in one loop so their LDS reads overlap, 8-ary while
+                // a range holds 16 or more slots, then binary; worst <=
                 // pool[L-1], so the pool slot read is in range.
                 int plo = 0, phi = Li, blo = 0, bhi = nb;
+                while (phi - plo >= 16 || bhi - blo >= 16) {
+                    if (phi - plo >= 16) narrow8_e(pool, plo, phi, key);
+                    if (bhi - blo >= 16) narrow8_e(buf, blo, bhi, key);
+                }
                 while (plo < phi || blo < bhi) {
                     if (plo < phi) {
                         const int mid = (plo + phi) >> 1;
