@@ -89,6 +89,33 @@ def test_search_other_dims(ctx, d):
     assert np.array_equal(got[0], ref[0]) and np.array_equal(got[1], ref[1])
 
 
+@pytest.mark.parametrize("d", [68, 72, 96])
+@pytest.mark.parametrize("rg", [4, 8, 16])
+def test_search_eight_lane_rows(ctx, d, rg):
+    """Rows of 17-24 chunks (d = 65..96) are gathered 8 lanes a row, 3 chunks
+    a lane, each lane's two accumulators summed as the fold's first step:
+    ids, distances, evaluations and hops equal the oracle's for every gather
+    tile, on the grid and persistent schedules, with 2 wavefronts per query
+    and with the merge buffer.  d = 68 and 72 leave the last chunk slots of
+    some lanes empty."""
+    from weavess_amd import Index
+    rng = np.random.default_rng(d + rg)
+    base = rng.normal(size=(1500, d)).astype(np.float32)
+    q = rng.normal(size=(60, d)).astype(np.float32)
+    g, _, deg = O.nndescent(base, 24, 32, 8, 32, 5, seed=1)
+    ref = O.search_greedy(g, deg, base, q, 200, 10, seed=4)
+    ix = Index(ctx, base)
+    ix.set_graph(g, deg)
+    for extra in ({}, {"schedule": 2}, {"waves_per_query": 2}, {"merge_buffer": 64}):
+        # a table ten times the base never forgets, so the counts are exact
+        ix.set_search_config(row_groups=rg, visited_slots=16384, **extra)
+        got = ix.search(q, 200, 10, seed=4)
+        assert np.array_equal(got[0], ref[0]) and np.array_equal(got[1], ref[1]), extra
+        assert np.array_equal(got[2], ref[2]), ("evals", extra)
+        assert np.array_equal(got[3], ref[3]), ("hops", extra)
+    ix.close()
+
+
 def test_search_complete_graph_exact(ctx, small_float):
     base, q = small_float
     base = base[:200]

@@ -93,8 +93,12 @@ hipError_t upload_padded(float *dst, const float *src, uint32_t rows, uint32_t d
                             hipMemcpyHostToDevice, s);
 }
 
+// Gather shape of the search kernels for rows of nch 4-element chunks: the
+// chunks a lane holds, 16 lanes a row (1, 2, 4, 8, 16), or 8 lanes a row
+// for 17-24 chunks (3).
 int chunks_per_lane(uint32_t nch) {
     if (nch <= 16) return 1;
+    if (nch <= 24) return 3;
     if (nch <= 32) return 2;
     if (nch <= 64) return 4;
     if (nch <= 128) return 8;

@@ -232,39 +232,77 @@ __device__ inline float fold16(float acc) {
     return acc;
 }
 
-// keys_out[r] = (dist(query, base[ids[r]]), ids[r]) for r < m.  16 lanes per
-// row; lane j of a row group owns accumulator j of the build order, i.e.
-// chunks j, j+16, ...; G row groups of 4 are gathered before any is reduced
-// so that 4*G rows of loads are in flight.
+// Lanes that share one row in rows_dist: 16, lane j owning accumulator j of
+// the build order (chunks j, j+16, ...); or, for rows of 17-24 chunks
+// (CPL 3), 8, lane j owning accumulators j and j+8 (chunks j, j+8, j+16
+// alternate between them), whose sum is the first step of the fold.  The
+// second form reads 24 chunks of a row in 3 loads per lane instead of 32
+// slots in 2 (half of the second masked off), with the same sums in the
+// same order.
+template <int CPL> constexpr int row_lanes() { return CPL == 3 ? 8 : 16; }
+
+// The query's chunks in the lane order rows_dist expects.
+template <int CPL>
+__device__ inline void load_query(const float *q, uint32_t nch, float4 (&qv)[CPL]) {
+    constexpr int LPR = row_lanes<CPL>();
+    const int sub = lane_id() % LPR;
+    const float4 *qp = reinterpret_cast<const float4 *>(q);
+#pragma unroll
+    for (int c = 0; c < CPL; ++c) {
+        const int ch = sub + LPR * c;
+        qv[c] = ch < (int)nch ? qp[ch] : make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+}
+
+// keys_out[r] = (dist(query, base[ids[r]]), ids[r]) for r < m.  row_lanes
+// lanes per row; G row groups of 64 / row_lanes rows are gathered before
+// any is reduced, so that all their loads are in flight.
 template <int CPL, int G>
 __device__ inline void rows_dist(const float *__restrict__ base, uint32_t dp, uint32_t nch,
                                  const float4 (&qv)[CPL], const uint32_t *ids, int m,
                                  uint64_t *keys_out) {
-    const int lane = lane_id(), sub = lane & 15, grp = lane >> 4;
-    for (int r0 = 0; r0 < m; r0 += 4 * G) {
+    constexpr int LPR = row_lanes<CPL>(), RPG = WAVE / LPR;
+    const int lane = lane_id(), sub = lane % LPR, grp = lane / LPR;
+    for (int r0 = 0; r0 < m; r0 += RPG * G) {
         float4 v[G][CPL];
         // groups wholly past m (the same for the whole wavefront) are skipped
 #pragma unroll
         for (int g = 0; g < G; ++g) {
-            if (r0 + 4 * g >= m) break;
-            const int row = r0 + 4 * g + grp;
+            if (r0 + RPG * g >= m) break;
+            const int row = r0 + RPG * g + grp;
             const uint32_t id = row < m ? ids[row] : 0u;
             const float4 *p = reinterpret_cast<const float4 *>(base + (size_t)id * dp);
 #pragma unroll
             for (int c = 0; c < CPL; ++c) {
-                const int ch = sub + 16 * c;
+                const int ch = sub + LPR * c;
                 v[g][c] = (row < m && ch < (int)nch) ? p[ch] : make_float4(0.f, 0.f, 0.f, 0.f);
             }
         }
 #pragma unroll
         for (int g = 0; g < G; ++g) {
-            if (r0 + 4 * g >= m) break;
-            const int row = r0 + 4 * g + grp;
-            float acc = 0.f;
+            if (r0 + RPG * g >= m) break;
+            const int row = r0 + RPG * g + grp;
+            float acc;
+            if constexpr (LPR == 16) {
+                acc = 0.f;
 #pragma unroll
-            for (int c = 0; c < CPL; ++c)
-                if (sub + 16 * c < (int)nch) acc = acc + chunk_sq(qv[c], v[g][c]);
-            acc = fold16(acc);
+                for (int c = 0; c < CPL; ++c)
+                    if (sub + 16 * c < (int)nch) acc = acc + chunk_sq(qv[c], v[g][c]);
+                acc = fold16(acc);
+            } else {
+                float a0 = 0.f, a1 = 0.f;  // accumulators sub and sub + 8
+#pragma unroll
+                for (int c = 0; c < CPL; ++c) {
+                    if (sub + 8 * c < (int)nch) {
+                        if (c & 1) a1 = a1 + chunk_sq(qv[c], v[g][c]);
+                        else a0 = a0 + chunk_sq(qv[c], v[g][c]);
+                    }
+                }
+                acc = a0 + a1;
+                acc = acc + __shfl_xor(acc, 4);
+                acc = acc + __shfl_xor(acc, 2);
+                acc = acc + __shfl_xor(acc, 1);
+            }
             if (sub == 0 && row < m) keys_out[row] = make_key(acc, ids[row]);
         }
     }

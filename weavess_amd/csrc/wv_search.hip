@@ -120,14 +120,7 @@ __device__ inline void search_query(const SearchArgs &a, uint8_t *my, const uint
     const uint32_t kq = min(max(a.k_per_q ? a.k_per_q[q] : a.k, 1u), min(L, a.k_stride));
 
     float4 qv[CPL];
-    {
-        const float4 *qp = reinterpret_cast<const float4 *>(a.queries + (size_t)q * dp);
-#pragma unroll
-        for (int c = 0; c < CPL; ++c) {
-            const int ch = (lane & 15) + 16 * c;
-            qv[c] = ch < (int)nch ? qp[ch] : make_float4(0.f, 0.f, 0.f, 0.f);
-        }
-    }
+    load_query<CPL>(a.queries + (size_t)q * dp, nch, qv);
 
     // ---- entries (SearchEntryRand): not marked visited, all flagged.  The
     // pool's storage first holds the L entry ids (one per 8-byte slot), then
@@ -412,14 +405,7 @@ __global__ void __launch_bounds__(256) greedy_search_coop_kernel(SearchArgs a) {
     const uint32_t L = min(max(a.L_per_q ? a.L_per_q[q] : a.L, 1u), lcap);
     const uint32_t kq = min(max(a.k_per_q ? a.k_per_q[q] : a.k, 1u), min(L, a.k_stride));
     float4 qv[CPL];
-    {
-        const float4 *qp = reinterpret_cast<const float4 *>(a.queries + (size_t)q * dp);
-#pragma unroll
-        for (int c = 0; c < CPL; ++c) {
-            const int ch = (lane & 15) + 16 * c;
-            qv[c] = ch < (int)nch ? qp[ch] : make_float4(0.f, 0.f, 0.f, 0.f);
-        }
-    }
+    load_query<CPL>(a.queries + (size_t)q * dp, nch, qv);
     // ---- entries: ids into the pool's storage (wave 0), then each round
     // every wave scores one 64-id chunk and sorts it in its block, and wave 0
     // merges the round's chunks in order (a merge writes only below the
@@ -660,6 +646,7 @@ hipError_t launch_greedy_search_coop(const SearchArgs &a, int chunks_per_lane, i
     switch (chunks_per_lane) {
     case 1: return launch_c<1, 8>(a, waves_per_query, s);
     case 2: return launch_c<2, 4>(a, waves_per_query, s);
+    case 3: return launch_c<3, 2>(a, waves_per_query, s);
     case 4: return launch_c<4, 4>(a, waves_per_query, s);
     case 8: return launch_c<8, 2>(a, waves_per_query, s);
     case 16: return launch_c<16, 1>(a, waves_per_query, s);
@@ -673,6 +660,7 @@ hipError_t launch_greedy_search(const SearchArgs &a, int chunks_per_lane, int ro
         switch (chunks_per_lane) {
         case 1: return launch_t<1, 8, true>(a, waves_per_block, cu_count, s);
         case 2: return launch_t<2, 4, true>(a, waves_per_block, cu_count, s);
+        case 3: return launch_t<3, 2, true>(a, waves_per_block, cu_count, s);
         case 4: return launch_t<4, 4, true>(a, waves_per_block, cu_count, s);
         case 8: return launch_t<8, 2, true>(a, waves_per_block, cu_count, s);
         case 16: return launch_t<16, 1, true>(a, waves_per_block, cu_count, s);
@@ -687,6 +675,10 @@ hipError_t launch_greedy_search(const SearchArgs &a, int chunks_per_lane, int ro
         return row_groups >= 16 ? launch_t<2, 16, false>(a, waves_per_block, cu_count, s)
              : row_groups >= 8 ? launch_t<2, 8, false>(a, waves_per_block, cu_count, s)
                                : launch_t<2, 4, false>(a, waves_per_block, cu_count, s);
+    case 3:  // 8 lanes a row: G groups of 8 rows, the same rows per batch as 2 * G groups of 4
+        return row_groups >= 16 ? launch_t<3, 8, false>(a, waves_per_block, cu_count, s)
+             : row_groups >= 8 ? launch_t<3, 4, false>(a, waves_per_block, cu_count, s)
+                               : launch_t<3, 2, false>(a, waves_per_block, cu_count, s);
     case 4:
         return row_groups >= 8 ? launch_t<4, 8, false>(a, waves_per_block, cu_count, s)
                                : launch_t<4, 4, false>(a, waves_per_block, cu_count, s);
