@@ -79,28 +79,43 @@ __device__ inline int lower_bound8_e(const uint64_t *a, int n, uint64_t key) {
 // slot of src[0] (>= cap when it was dropped).
 __device__ inline int merge_into(uint64_t *arr, int len, int cap, const uint64_t *src, int c, bool mark) {
     const int lane = lane_id();
-    int cpos[4];
+    int cpos[4], ub[4];
     uint64_t ck[4];
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
         const int j = lane + r * WAVE;
         cpos[r] = cap;
+        ub[r] = 0x7FFFFFFF;
         ck[r] = 0;
         if (j < c) {
             ck[r] = mark ? (src[j] | FBIT) : src[j];
-            cpos[r] = j + lower_bound8_e(arr, len, canon(ck[r]) + 1);  // arr keys <= it
+            ub[r] = lower_bound8_e(arr, len, canon(ck[r]) + 1);  // arr keys <= src[j]
+            cpos[r] = j + ub[r];
         }
     }
+    // The src keys below arr[p] are those whose ub (non-decreasing in j) is
+    // <= p.  For a chunk [t0, t0 + 64): the ub below t0 all count (a ballot),
+    // and the few in the chunk are read from their lanes one by one.
     const int pos0 = __shfl(cpos[0], 0);  // lane 0 holds src[0]: j = 0
     for (int t0 = (len - 1) & ~(WAVE - 1); t0 >= (pos0 & ~(WAVE - 1)); t0 -= WAVE) {
         const int p = t0 + lane;
         const bool mv = p >= pos0 && p < len;
         uint64_t pk = 0;
-        int np = cap;
-        if (mv) {
-            pk = arr[p];
-            np = p + lower_bound_e(src, c, canon(pk));
+        if (mv) pk = arr[p];
+        int base = 0, top = 0;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            if (r * WAVE >= c) break;
+            base += (int)__popcll(__ballot(ub[r] < t0));
+            top += (int)__popcll(__ballot(ub[r] < t0 + WAVE));
         }
+        int below = base;
+        for (int jj = base; jj < top; ++jj) {
+            const int rr = jj / WAVE;
+            const int u = rr == 0 ? ub[0] : rr == 1 ? ub[1] : rr == 2 ? ub[2] : ub[3];
+            below += __builtin_amdgcn_readlane(u, jj % WAVE) <= p ? 1 : 0;
+        }
+        const int np = p + below;
         wave_sync();
         if (mv && np < cap) arr[np] = pk;
         wave_sync();
@@ -110,6 +125,76 @@ __device__ inline int merge_into(uint64_t *arr, int len, int cap, const uint64_t
         if (cpos[r] < cap) arr[cpos[r]] = ck[r];
     wave_sync();
     return pos0;
+}
+
+// merge_into(buf, nb, B, src, c, true) for 1 <= c <= 64 sorted, distinct
+// keys that are not in buf (lane t < c holds src[t] in `skey`, without the
+// flag bit), nb + c <= B <= 256, followed by the new L-th key of pool +
+// buffer (kth_union(pool, Li, buf, nb + c, Li - 1)) into `worst`.  Both come
+// from registers and ballots instead of chains of dependent LDS searches:
+// every lane reads its buffer entries (l, l + 64, ...); a loop over the
+// src keys counts, per entry, the src keys below it (its shift right) and,
+// per src key, the entries below it (a ballot); all keys are then written to
+// their final slots at once.  New buffer entry j is among the L smallest of
+// pool + buffer iff it is below pool[L-1-j] (true for a prefix of j), so
+// the number i taken from the buffer is a ballot count and the L-th key is
+// the larger of buffer entry i-1 and pool entry L-1-i.  Returns the final
+// slot of src[0].
+__device__ inline int buffer_insert(uint64_t *buf, int nb, uint64_t skey, int c, const uint64_t *pool, int Li,
+                                    uint64_t &worst) {
+    const int lane = lane_id();
+    constexpr int R = 4;  // entries a lane holds: B <= 256
+    uint64_t b[R];
+    int np[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int j = lane + r * WAVE;
+        b[r] = j < nb ? buf[j] : KEY_EMPTY;
+        np[r] = j;
+    }
+    int spos = 0;
+    for (int t = 0; t < c; ++t) {
+        const uint64_t s = readlane64(skey, t);
+        int below = 0;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            if (r * WAVE >= nb) break;
+            const bool in = lane + r * WAVE < nb;
+            const uint64_t cb = canon(b[r]);
+            if (in && s < cb) ++np[r];
+            below += (int)__popcll(__ballot(in && cb < s));
+        }
+        if (lane == t) spos = t + below;
+    }
+    wave_sync();
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+        if (lane + r * WAVE < nb) buf[np[r]] = b[r];
+    if (lane < c) buf[spos] = skey | FBIT;
+    wave_sync();
+    const int nn = nb + c, hi = nn < Li ? nn : Li;
+    uint64_t bn[R], pt[R];
+    int taken = 0;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int j = lane + r * WAVE;
+        bn[r] = j < nn ? canon(buf[j]) : KEY_EMPTY;
+        pt[r] = j < Li ? canon(pool[Li - 1 - j]) : 0;
+    }
+#pragma unroll
+    for (int r = 0; r < R; ++r) taken += (int)__popcll(__ballot(lane + r * WAVE < hi && bn[r] < pt[r]));
+    uint64_t x = 0, y = 0;
+    if (taken > 0) {
+        const int i = taken - 1, rr = i / WAVE;
+        x = readlane64(rr == 0 ? bn[0] : rr == 1 ? bn[1] : rr == 2 ? bn[2] : bn[3], i % WAVE);
+    }
+    if (taken < Li) {
+        const int rr = taken / WAVE;
+        y = rr < R ? readlane64(rr == 0 ? pt[0] : rr == 1 ? pt[1] : rr == 2 ? pt[2] : pt[3], taken % WAVE)
+                   : canon(pool[Li - 1 - taken]);
+    }
+    worst = x > y ? x : y;
+    return __shfl(spos, 0);
 }
 
 // GenRandom (include/weavess/util.h:13-27) of size L on counter stream s

@@ -294,11 +294,10 @@ __device__ inline void search_query(const SearchArgs &a, uint8_t *my, const uint
                     nb = 0;
                     kb = 0;
                 }
-                const int bp = merge_into(buf, nb, B, ckey, c, true);
+                const int bp = buffer_insert(buf, nb, key, c, pool, Li, worst);
                 nb += c;
                 if (bp < kb) kb = bp;
                 if (track_settle && bp + lower_bound_e(pool, Li, ckey[0]) <= kidx) settle = hops;
-                worst = kth_union(pool, Li, buf, nb, Li - 1);
             }
         }
     }
@@ -550,10 +549,9 @@ __global__ void __launch_bounds__(256) greedy_search_coop_kernel(SearchArgs a) {
                             nb = 0;
                             kb = 0;
                         }
-                        const int bp = merge_into(buf, nb, B, blk, c, true);
+                        const int bp = buffer_insert(buf, nb, key, c, pool, Li, worst);
                         nb += c;
                         if (bp < kb) kb = bp;
-                        worst = kth_union(pool, Li, buf, nb, Li - 1);
                     }
                 }
                 if (lane == 0) ctl[2] = evals;
