@@ -37,10 +37,10 @@ __device__ inline int upper_bound_e(const uint64_t *a, int n, uint64_t key) {
 
 // One round of an 8-ary search for the number of entries < key in a sorted
 // range: the answer lies in [lo, hi] (entries before lo are < key, from hi
-// on >= key) and hi - lo >= 16.  The 7 slots lo + j*s (s = (hi-lo)/8, j =
+// on >= key) and hi - lo >= 8.  The 7 slots lo + j*s (s = (hi-lo)/8, j =
 // 1..7) are read at once -- independent LDS reads instead of a chain of
 // dependent ones -- and the range narrows to the part between the last slot
-// below key and the first one not below it.
+// below key and the first one not below it (at most s + 6 slots).
 __device__ inline void narrow8_e(const uint64_t *a, int &lo, int &hi, uint64_t key) {
     const int s = (hi - lo) >> 3;
     uint64_t x[7];
@@ -54,18 +54,25 @@ __device__ inline void narrow8_e(const uint64_t *a, int &lo, int &hi, uint64_t k
     lo = nlo;
 }
 
+// The last round for a range of at most 7 slots: all of them read at once;
+// returns lo plus the number below key.
+__device__ inline int finish8_e(const uint64_t *a, int lo, int hi, uint64_t key) {
+    uint64_t x[7];
+#pragma unroll
+    for (int j = 0; j < 7; ++j) x[j] = lo + j < hi ? canon(a[lo + j]) : KEY_EMPTY;
+    int t = 0;
+#pragma unroll
+    for (int j = 0; j < 7; ++j) t += x[j] < key ? 1 : 0;
+    return lo + t;
+}
+
 // Number of entries of the sorted a[0..n) whose key is < key: 8-ary rounds
-// while the range holds 16 or more slots, then binary steps.  The result is
+// down to at most 7 slots, then one round over those.  The result is
 // lower_bound_e's; only the reads are fewer rounds deep.
 __device__ inline int lower_bound8_e(const uint64_t *a, int n, uint64_t key) {
     int lo = 0, hi = n;
-    while (hi - lo >= 16) narrow8_e(a, lo, hi, key);
-    while (lo < hi) {
-        const int mid = (lo + hi) >> 1;
-        if (canon(a[mid]) < key) lo = mid + 1;
-        else hi = mid;
-    }
-    return lo;
+    while (hi - lo >= 8) narrow8_e(a, lo, hi, key);
+    return finish8_e(a, lo, hi, key);
 }
 
 // Merge c sorted entries src[0..c) into the sorted array arr[0..len) of

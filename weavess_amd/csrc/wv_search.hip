@@ -66,14 +66,22 @@ __device__ inline int visit_first(uint32_t *vis, uint32_t hmask, uint32_t id, in
     return 2;
 }
 
-// First slot >= from of a[0..n) holding an unexpanded entry, or n.
-__device__ inline int first_flag(const uint64_t *a, int from, int n) {
+// First slot >= from of a[0..n) holding an unexpanded entry, or n; `key`
+// gets that entry's key without the flag (KEY_EMPTY when there is none),
+// taken from the lane that read it.
+__device__ inline int first_flag(const uint64_t *a, int from, int n, uint64_t &key) {
     const int lane = lane_id();
     for (int t0 = from & ~(WAVE - 1); t0 < n; t0 += WAVE) {
         const int p = t0 + lane;
-        const uint64_t b = __ballot(p >= from && p < n && (a[p] & FBIT));
-        if (b) return t0 + __ffsll((unsigned long long)b) - 1;
+        const uint64_t e = p >= from && p < n ? a[p] : 0;
+        const uint64_t b = __ballot((e & FBIT) != 0);
+        if (b) {
+            const int i = __ffsll((unsigned long long)b) - 1;
+            key = canon(readlane64(e, i));
+            return t0 + i;
+        }
     }
+    key = KEY_EMPTY;
     return n;
 }
 
@@ -179,10 +187,9 @@ __device__ inline void search_query(const SearchArgs &a, uint8_t *my, const uint
     int probes = VIS_PROBES;
     for (;;) {
         // U's first unexpanded key: the smaller of the pool's and the buffer's
-        const int kk = first_flag(pool, kp, Li);
-        const int jb = nb ? first_flag(buf, kb, nb) : 0;
-        const uint64_t ka = kk < Li ? canon(pool[kk]) : KEY_EMPTY;
-        const uint64_t kbk = jb < nb ? canon(buf[jb]) : KEY_EMPTY;
+        uint64_t ka, kbk = KEY_EMPTY;
+        const int kk = first_flag(pool, kp, Li, ka);
+        const int jb = nb ? first_flag(buf, kb, nb, kbk) : 0;
         const bool from_buf = kbk < ka;
         const uint64_t cur = from_buf ? kbk : ka;
         // cur is in U iff it is not above U's L-th key (`worst` is kept exact)
@@ -462,10 +469,9 @@ __global__ void __launch_bounds__(256) greedy_search_coop_kernel(SearchArgs a) {
     uint64_t worst = canon(pool[Li - 1]);
     for (;;) {
         if (wid == 0) {
-            const int kk = first_flag(pool, kp, Li);
-            const int jb = nb ? first_flag(buf, kb, nb) : 0;
-            const uint64_t ka = kk < Li ? canon(pool[kk]) : KEY_EMPTY;
-            const uint64_t kbk = jb < nb ? canon(buf[jb]) : KEY_EMPTY;
+            uint64_t ka, kbk = KEY_EMPTY;
+            const int kk = first_flag(pool, kp, Li, ka);
+            const int jb = nb ? first_flag(buf, kb, nb, kbk) : 0;
             const bool from_buf = kbk < ka;
             const uint64_t cur = from_buf ? kbk : ka;
             const bool done = cur == KEY_EMPTY || cur > worst;
