@@ -102,6 +102,33 @@ __device__ inline uint64_t kth_union(const uint64_t *a, int na, const uint64_t *
     return x > y ? x : y;
 }
 
+// The survivors (lanes whose `key` is not KEY_EMPTY, mask lm, c of them,
+// distinct keys) in ascending order into blk[0..c), and lane t < c's `key`
+// becomes the t-th (lanes from c up get KEY_EMPTY).  Up to 16 survivors
+// count their ranks from one another's registers and are written straight
+// to their slots; more are compacted and sorted across the wavefront.
+__device__ inline uint64_t sort_survivors(uint64_t key, uint64_t lm, int c, uint64_t *blk) {
+    const int lane = lane_id();
+    if (c <= 16) {
+        int rank = 0;
+        for (uint64_t mm = lm; mm; mm &= mm - 1) rank += readlane64(key, __ffsll((unsigned long long)mm) - 1) < key;
+        wave_sync();
+        if (key != KEY_EMPTY) blk[rank] = key;
+        wave_sync();
+        return lane < c ? blk[lane] : KEY_EMPTY;
+    }
+    wave_sync();
+    if (key != KEY_EMPTY) blk[lanes_below(lm)] = key;
+    wave_sync();
+    int w = 2;
+    while (w < c) w <<= 1;
+    key = wave_sort64_n(lane < c ? blk[lane] : KEY_EMPTY, w);
+    wave_sync();
+    if (lane < c) blk[lane] = key;
+    wave_sync();
+    return lane < c ? key : KEY_EMPTY;
+}
+
 // WATCH: the variant for the learned stop and per-hop traces (the state
 // features after every expansion, the network in registers); plain launches
 // use the variant without that code.
@@ -280,15 +307,7 @@ __device__ inline void search_query(const SearchArgs &a, uint8_t *my, const uint
             const uint64_t lm = __ballot(key != KEY_EMPTY);
             if (lm == 0) continue;
             const int c = __popcll(lm);
-            wave_sync();
-            if (key != KEY_EMPTY) ckey[lanes_below(lm)] = key;
-            wave_sync();
-            int w = 2;
-            while (w < c) w <<= 1;
-            key = wave_sort64_n(lane < c ? ckey[lane] : KEY_EMPTY, w);
-            wave_sync();
-            if (lane < c) ckey[lane] = key;
-            wave_sync();
+            key = sort_survivors(key, lm, c, ckey);
             if (B == 0) {
                 const int pos0 = merge_into(pool, Li, Li, ckey, c, true);
                 if (pos0 < kp) kp = pos0;
@@ -535,15 +554,7 @@ __global__ void __launch_bounds__(256) greedy_search_coop_kernel(SearchArgs a) {
                     const uint64_t lm = __ballot(key != KEY_EMPTY);
                     if (lm == 0) continue;
                     const int c = __popcll(lm);
-                    wave_sync();
-                    if (key != KEY_EMPTY) blk[lanes_below(lm)] = key;
-                    wave_sync();
-                    int wd = 2;
-                    while (wd < c) wd <<= 1;
-                    key = wave_sort64_n(lane < c ? blk[lane] : KEY_EMPTY, wd);
-                    wave_sync();
-                    if (lane < c) blk[lane] = key;
-                    wave_sync();
+                    key = sort_survivors(key, lm, c, blk);
                     if (B == 0) {
                         const int pos0 = merge_into(pool, Li, Li, blk, c, true);
                         if (pos0 < kp) kp = pos0;
