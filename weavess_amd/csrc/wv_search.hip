@@ -102,6 +102,34 @@ __device__ inline uint64_t kth_union(const uint64_t *a, int na, const uint64_t *
     return x > y ? x : y;
 }
 
+// `key` if it enters U -- below the L-th key `worst` and in neither pool nor
+// buffer (a duplicate comes from a forgotten node evaluated again) -- else
+// KEY_EMPTY.  Both searches advance in one loop so their LDS reads overlap,
+// 8-ary while a range holds 16 or more slots, then binary; worst <=
+// pool[L-1], so the pool slot read is in range.
+__device__ inline uint64_t survivor(uint64_t key, const uint64_t *pool, int Li, const uint64_t *buf, int nb,
+                                    uint64_t worst) {
+    if (!(key < worst)) return KEY_EMPTY;
+    int plo = 0, phi = Li, blo = 0, bhi = nb;
+    while (phi - plo >= 16 || bhi - blo >= 16) {
+        if (phi - plo >= 16) narrow8_e(pool, plo, phi, key);
+        if (bhi - blo >= 16) narrow8_e(buf, blo, bhi, key);
+    }
+    while (plo < phi || blo < bhi) {
+        if (plo < phi) {
+            const int mid = (plo + phi) >> 1;
+            if (canon(pool[mid]) < key) plo = mid + 1;
+            else phi = mid;
+        }
+        if (blo < bhi) {
+            const int mid = (blo + bhi) >> 1;
+            if (canon(buf[mid]) < key) blo = mid + 1;
+            else bhi = mid;
+        }
+    }
+    return (canon(pool[plo]) == key || (blo < nb && canon(buf[blo]) == key)) ? KEY_EMPTY : key;
+}
+
 // The survivors (lanes whose `key` is not KEY_EMPTY, mask lm, c of them,
 // distinct keys) in ascending order into blk[0..c), and lane t < c's `key`
 // becomes the t-th (lanes from c up get KEY_EMPTY).  Up to 16 survivors
@@ -212,118 +240,196 @@ __device__ inline void search_query(const SearchArgs &a, uint8_t *my, const uint
     float r0 = watch ? key_dist(pool[kidx]) : 0.f, r1 = r0, r2 = r0, r3 = r0;  // bit 31 is not in the distance
     uint32_t settle = 0;  // last expansion that changed the top-k
     int probes = VIS_PROBES;
-    for (;;) {
-        // U's first unexpanded key: the smaller of the pool's and the buffer's
-        uint64_t ka, kbk = KEY_EMPTY;
-        const int kk = first_flag(pool, kp, Li, ka);
-        const int jb = nb ? first_flag(buf, kb, nb, kbk) : 0;
-        const bool from_buf = kbk < ka;
-        const uint64_t cur = from_buf ? kbk : ka;
-        // cur is in U iff it is not above U's L-th key (`worst` is kept exact)
-        const bool done = cur == KEY_EMPTY || cur > worst;
-        if (watch && hops > 0) {
-            // state after `hops` expansions
-            float f[MLP_IN];
-            const uint64_t kth = nb ? kth_union(pool, Li, buf, nb, kidx) : canon(pool[kidx]);
-            const uint64_t best = (nb && canon(buf[0]) < canon(pool[0])) ? canon(buf[0]) : canon(pool[0]);
-            term_features((float)hops, key_dist(kth), key_dist(best), entry_d, r3, (float)(hops - settle),
-                          (float)L, (float)kq, f);
-            r3 = r2;
-            r2 = r1;
-            r1 = r0;
-            r0 = key_dist(kth);
-            // one lane stores the row as two float4s: a lane-indexed pick
-            // would put f in scratch memory for the whole loop
-            if (a.trace && hops <= a.trace_cap && lane == 0) {
-                float4 *t = reinterpret_cast<float4 *>(a.trace + ((size_t)q * a.trace_cap + (hops - 1)) * MLP_IN);
-                t[0] = make_float4(f[0], f[1], f[2], f[3]);
-                t[1] = make_float4(f[4], f[5], f[6], f[7]);
+    if constexpr (!COLLECT && !WATCH) {
+        // The loop below with the next node chosen before the merge: once
+        // the last block of u's survivors is sorted, the next node is the
+        // smaller of U's first unexpanded key and the smallest survivor (both
+        // are in U after the merge, which keeps every other key's order).
+        // It is marked expanded and its degree and first neighbour slots are
+        // loaded before the block is merged, so those loads travel during
+        // the merge.  Expansions, evaluations and results are the loop's.
+        uint32_t u = 0, du = 0, id0 = ID_NONE;
+        auto start = [&](uint32_t node) {
+            // a row holds deg_max slots, so the first 64 load with the
+            // degree (those past it are in bounds and masked off)
+            u = node;
+            const uint32_t *adj = a.graph + (size_t)node * a.deg_max;
+            id0 = (uint32_t)lane < a.deg_max ? adj[lane] : ID_NONE;
+            du = a.deg[node];
+        };
+        bool live;
+        {
+            uint64_t cur;
+            const int kk = first_flag(pool, kp, Li, cur);  // the buffer is empty
+            live = !(cur == KEY_EMPTY || cur > worst);
+            if (live) {
+                wave_sync();
+                if (lane == 0) pool[kk] = cur;  // expanded: stored without bit 31
+                kp = kk + 1;
+                start(key_id(cur));
             }
-            if (!done && use_model && hops >= net.min_hops && term_logit_regs(net, f) > net.threshold) break;
         }
-        if (done) break;
-        const uint32_t u = key_id(cur);  // cur is canonical
-        wave_sync();
-        if (lane == 0) {  // expanded: store the key without bit 31
-            if (from_buf) buf[jb] = cur;
-            else pool[kk] = cur;
-        }
-        if (from_buf) kb = jb + 1;
-        else kp = kk + 1;
-        ++hops;
-        // the first 64 neighbour slots load together with the degree, not
-        // after it: a row holds deg_max slots, so those past the degree are
-        // in bounds, and they are masked off below
-        const uint32_t *adj = a.graph + (size_t)u * a.deg_max;
-        const uint32_t id0 = (uint32_t)lane < a.deg_max ? adj[lane] : ID_NONE;
-        uint32_t du = a.deg[u];
-        if (du > a.deg_max) du = a.deg_max;
-        for (uint32_t m0 = 0; m0 < du; m0 += WAVE) {
-            const uint32_t mi = m0 + lane;
-            const uint32_t id = mi < du ? (m0 == 0 ? id0 : adj[mi]) : ID_NONE;
-            const bool fresh = id < N && visit_first(vis, hmask, id, probes) != 0;
-            const uint64_t bm = __ballot(fresh);
-            const int m = __popcll(bm);
-            if (fresh) cid[lanes_below(bm)] = id;
-            wave_sync();
-            if (m == 0) continue;
-            rows_dist<CPL, G>(a.base, dp, nch, qv, cid, m, ckey);
-            evals += (uint32_t)m;
-            // more ids inserted than the table has slots: it forgets either
-            // way, so stop probing deeply
-            if (evals - L > a.H) probes = VIS_PROBES_FULL;
-            wave_sync();
-            if (COLLECT) collect_stage(full, flen, (int)a.C, ckey, m, fstg, fsn, fblk);
-            uint64_t key = lane < m ? ckey[lane] : KEY_EMPTY;
-            if (key < worst) {
-                // already in pool or buffer: a duplicate.  Both searches
-                // advance in one loop so their LDS reads overlap, 8-ary while
-                // a range holds 16 or more slots, then binary; worst <=
-                // pool[L-1], so the pool slot read is in range.
-                int plo = 0, phi = Li, blo = 0, bhi = nb;
-                while (phi - plo >= 16 || bhi - blo >= 16) {
-                    if (phi - plo >= 16) narrow8_e(pool, plo, phi, key);
-                    if (bhi - blo >= 16) narrow8_e(buf, blo, bhi, key);
+        while (live) {
+            ++hops;
+            if (du > a.deg_max) du = a.deg_max;
+            const uint32_t nblk = du > (uint32_t)WAVE ? (du + WAVE - 1) / WAVE : 1u;
+            const uint32_t *adj = a.graph + (size_t)u * a.deg_max;
+            const uint32_t first = id0;
+            for (uint32_t bi = 0; bi < nblk; ++bi) {
+                const uint32_t mi = bi * WAVE + lane;
+                const uint32_t id = mi < du ? (bi == 0 ? first : adj[mi]) : ID_NONE;
+                const bool fresh = id < N && visit_first(vis, hmask, id, probes) != 0;
+                const uint64_t bm = __ballot(fresh);
+                const int m = __popcll(bm);
+                if (fresh) cid[lanes_below(bm)] = id;
+                wave_sync();
+                int c = 0;
+                uint64_t key = KEY_EMPTY;
+                if (m > 0) {
+                    rows_dist<CPL, G>(a.base, dp, nch, qv, cid, m, ckey);
+                    evals += (uint32_t)m;
+                    if (evals - L > a.H) probes = VIS_PROBES_FULL;
+                    wave_sync();
+                    key = survivor(lane < m ? ckey[lane] : KEY_EMPTY, pool, Li, buf, nb, worst);
+                    const uint64_t lm = __ballot(key != KEY_EMPTY);
+                    c = __popcll(lm);
+                    if (c > 0) key = sort_survivors(key, lm, c, ckey);
                 }
-                while (plo < phi || blo < bhi) {
-                    if (plo < phi) {
-                        const int mid = (plo + phi) >> 1;
-                        if (canon(pool[mid]) < key) plo = mid + 1;
-                        else phi = mid;
-                    }
-                    if (blo < bhi) {
-                        const int mid = (blo + bhi) >> 1;
-                        if (canon(buf[mid]) < key) blo = mid + 1;
-                        else bhi = mid;
+                bool next_new = false;  // the next node is the smallest survivor
+                if (bi + 1 == nblk) {
+                    uint64_t ka, kbk = KEY_EMPTY;
+                    const int kk = first_flag(pool, kp, Li, ka);
+                    const int jb = nb ? first_flag(buf, kb, nb, kbk) : 0;
+                    const bool from_buf = kbk < ka;
+                    const uint64_t x = from_buf ? kbk : ka;
+                    const uint64_t y = c > 0 ? readlane64(key, 0) : KEY_EMPTY;
+                    next_new = y < x;
+                    const uint64_t cur = next_new ? y : x;
+                    // with survivors cur <= y < worst; without, worst stays
+                    live = !(cur == KEY_EMPTY || cur > worst);
+                    if (live) {
+                        if (!next_new) {
+                            wave_sync();
+                            if (lane == 0) {
+                                if (from_buf) buf[jb] = cur;
+                                else pool[kk] = cur;
+                            }
+                            if (from_buf) kb = jb + 1;
+                            else kp = kk + 1;
+                        }
+                        start(key_id(cur));
                     }
                 }
-                if (canon(pool[plo]) == key || (blo < nb && canon(buf[blo]) == key)) key = KEY_EMPTY;
-            } else {
-                key = KEY_EMPTY;
-            }
-            // the batch's keys are distinct (the visited table admits an id
-            // once per batch): compact the survivors to the front lanes and
-            // sort only the smallest power-of-two width holding them
-            const uint64_t lm = __ballot(key != KEY_EMPTY);
-            if (lm == 0) continue;
-            const int c = __popcll(lm);
-            key = sort_survivors(key, lm, c, ckey);
-            if (B == 0) {
-                const int pos0 = merge_into(pool, Li, Li, ckey, c, true);
-                if (pos0 < kp) kp = pos0;
-                if (pos0 <= kidx) settle = hops;  // a candidate entered the top-k
-                worst = canon(pool[Li - 1]);
-            } else {
-                if (nb + c > B) {  // full: merge the buffer into the pool
-                    const int pos0 = merge_into(pool, Li, Li, buf, nb, false);
+                if (c == 0) continue;
+                if (B == 0) {
+                    const int pos0 = merge_into(pool, Li, Li, ckey, c, true);
                     if (pos0 < kp) kp = pos0;
-                    nb = 0;
-                    kb = 0;
+                    if (pos0 <= kidx) settle = hops;
+                    if (next_new && lane == 0) pool[pos0] = ckey[0];  // pos0 < L: ckey[0] < worst
+                    worst = canon(pool[Li - 1]);
+                } else {
+                    if (nb + c > B) {
+                        const int pos0 = merge_into(pool, Li, Li, buf, nb, false);
+                        if (pos0 < kp) kp = pos0;
+                        nb = 0;
+                        kb = 0;
+                    }
+                    const int bp = buffer_insert(buf, nb, key, c, pool, Li, worst);
+                    nb += c;
+                    if (bp < kb) kb = bp;
+                    if (next_new && lane == 0) buf[bp] = ckey[0];
+                    if (track_settle && bp + lower_bound_e(pool, Li, ckey[0]) <= kidx) settle = hops;
                 }
-                const int bp = buffer_insert(buf, nb, key, c, pool, Li, worst);
-                nb += c;
-                if (bp < kb) kb = bp;
-                if (track_settle && bp + lower_bound_e(pool, Li, ckey[0]) <= kidx) settle = hops;
+                wave_sync();
+            }
+        }
+    } else {
+        for (;;) {
+            // U's first unexpanded key: the smaller of the pool's and the buffer's
+            uint64_t ka, kbk = KEY_EMPTY;
+            const int kk = first_flag(pool, kp, Li, ka);
+            const int jb = nb ? first_flag(buf, kb, nb, kbk) : 0;
+            const bool from_buf = kbk < ka;
+            const uint64_t cur = from_buf ? kbk : ka;
+            // cur is in U iff it is not above U's L-th key (`worst` is kept exact)
+            const bool done = cur == KEY_EMPTY || cur > worst;
+            if (watch && hops > 0) {
+                // state after `hops` expansions
+                float f[MLP_IN];
+                const uint64_t kth = nb ? kth_union(pool, Li, buf, nb, kidx) : canon(pool[kidx]);
+                const uint64_t best = (nb && canon(buf[0]) < canon(pool[0])) ? canon(buf[0]) : canon(pool[0]);
+                term_features((float)hops, key_dist(kth), key_dist(best), entry_d, r3, (float)(hops - settle),
+                              (float)L, (float)kq, f);
+                r3 = r2;
+                r2 = r1;
+                r1 = r0;
+                r0 = key_dist(kth);
+                // one lane stores the row as two float4s: a lane-indexed pick
+                // would put f in scratch memory for the whole loop
+                if (a.trace && hops <= a.trace_cap && lane == 0) {
+                    float4 *t = reinterpret_cast<float4 *>(a.trace + ((size_t)q * a.trace_cap + (hops - 1)) * MLP_IN);
+                    t[0] = make_float4(f[0], f[1], f[2], f[3]);
+                    t[1] = make_float4(f[4], f[5], f[6], f[7]);
+                }
+                if (!done && use_model && hops >= net.min_hops && term_logit_regs(net, f) > net.threshold) break;
+            }
+            if (done) break;
+            const uint32_t u = key_id(cur);  // cur is canonical
+            wave_sync();
+            if (lane == 0) {  // expanded: store the key without bit 31
+                if (from_buf) buf[jb] = cur;
+                else pool[kk] = cur;
+            }
+            if (from_buf) kb = jb + 1;
+            else kp = kk + 1;
+            ++hops;
+            // the first 64 neighbour slots load together with the degree, not
+            // after it: a row holds deg_max slots, so those past the degree are
+            // in bounds, and they are masked off below
+            const uint32_t *adj = a.graph + (size_t)u * a.deg_max;
+            const uint32_t id0 = (uint32_t)lane < a.deg_max ? adj[lane] : ID_NONE;
+            uint32_t du = a.deg[u];
+            if (du > a.deg_max) du = a.deg_max;
+            for (uint32_t m0 = 0; m0 < du; m0 += WAVE) {
+                const uint32_t mi = m0 + lane;
+                const uint32_t id = mi < du ? (m0 == 0 ? id0 : adj[mi]) : ID_NONE;
+                const bool fresh = id < N && visit_first(vis, hmask, id, probes) != 0;
+                const uint64_t bm = __ballot(fresh);
+                const int m = __popcll(bm);
+                if (fresh) cid[lanes_below(bm)] = id;
+                wave_sync();
+                if (m == 0) continue;
+                rows_dist<CPL, G>(a.base, dp, nch, qv, cid, m, ckey);
+                evals += (uint32_t)m;
+                // more ids inserted than the table has slots: it forgets either
+                // way, so stop probing deeply
+                if (evals - L > a.H) probes = VIS_PROBES_FULL;
+                wave_sync();
+                if (COLLECT) collect_stage(full, flen, (int)a.C, ckey, m, fstg, fsn, fblk);
+                // the batch's keys are distinct (the visited table admits an id
+                // once per batch)
+                uint64_t key = survivor(lane < m ? ckey[lane] : KEY_EMPTY, pool, Li, buf, nb, worst);
+                const uint64_t lm = __ballot(key != KEY_EMPTY);
+                if (lm == 0) continue;
+                const int c = __popcll(lm);
+                key = sort_survivors(key, lm, c, ckey);
+                if (B == 0) {
+                    const int pos0 = merge_into(pool, Li, Li, ckey, c, true);
+                    if (pos0 < kp) kp = pos0;
+                    if (pos0 <= kidx) settle = hops;  // a candidate entered the top-k
+                    worst = canon(pool[Li - 1]);
+                } else {
+                    if (nb + c > B) {  // full: merge the buffer into the pool
+                        const int pos0 = merge_into(pool, Li, Li, buf, nb, false);
+                        if (pos0 < kp) kp = pos0;
+                        nb = 0;
+                        kb = 0;
+                    }
+                    const int bp = buffer_insert(buf, nb, key, c, pool, Li, worst);
+                    nb += c;
+                    if (bp < kb) kb = bp;
+                    if (track_settle && bp + lower_bound_e(pool, Li, ckey[0]) <= kidx) settle = hops;
+                }
             }
         }
     }
@@ -347,9 +453,16 @@ __device__ inline void search_query(const SearchArgs &a, uint8_t *my, const uint
     }
 }
 
+// Wavefronts per SIMD the plain grid kernel is compiled for: 3 (at most 168
+// VGPRs, no spills) for the 64-row gather tile of rows of 17-24 chunks,
+// which otherwise takes a few more and drops to 2 where LDS would allow 3.
+// (The persistent form keeps more state live and would spill under it.)
+template <int CPL, int G, bool EXTRA> struct SearchWaves { static constexpr int value = (CPL == 3 && G == 8 && !EXTRA) ? 3 : 1; };
+
 // One wavefront per query: query = its global wavefront index.
 template <int CPL, int G, bool COLLECT, bool WATCH>
-__global__ void __launch_bounds__(256) greedy_search_kernel(SearchArgs a) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SearchWaves<CPL, G, COLLECT || WATCH>::value)))
+greedy_search_kernel(SearchArgs a) {
     extern __shared__ __align__(16) uint8_t smem[];
     const int wid = (int)(threadIdx.x >> 6);
     const uint32_t q = blockIdx.x * (blockDim.x >> 6) + wid;
