@@ -1116,6 +1116,74 @@ __global__ void __launch_bounds__(256) nnd_merge_kernel(NndDev a, uint32_t up2, 
         if (take) u[cnt + nw + lanes_below(bm)] = k;
         nw += (uint32_t)__popcll(bm);
     }
+    if (a.L <= (uint32_t)WAVE && nw <= (uint32_t)WAVE) {
+        // Pools of at most 64 with at most 64 winners, in registers: lane i
+        // holds pool key i (its flag in bit 31) and, after a sort across the
+        // wavefront, winner i (tagged new in bit 63, flagged).  Comparing
+        // each pool key with the winner of the mirrored lane leaves the 64
+        // smallest of both runs as a bitonic sequence, and six merge steps
+        // sort it: the new pool is its first L, the same keys in the same
+        // order as the merge by rank below, with the same flags.
+        constexpr uint64_t NEWB = 1ull << 63, FLB = 1ull << 31, CM = ~(NEWB | FLB);
+        wave_sync();
+        int pw = 2;
+        while ((uint32_t)pw < nw) pw <<= 1;
+        // lanes from pw up hold KEY_EMPTY before and after the sort
+        uint64_t wk = wave_sort64_n((uint32_t)lane < nw ? u[cnt + lane] : KEY_EMPTY, pw);
+        if (wk != KEY_EMPTY) wk |= NEWB | FLB;
+        const uint64_t pkey = (uint32_t)lane < cnt ? (pk[0] | (pfl[0] ? FLB : 0ull)) : KEY_EMPTY;
+        const uint64_t rw = shfl64(wk, WAVE - 1 - lane);
+        uint64_t x = (pkey & CM) < (rw & CM) ? pkey : rw;
+#pragma unroll
+        for (int j = WAVE / 2; j > 0; j >>= 1) {
+            const uint64_t o = shfl_xor64(x, j);
+            const bool below = (x & CM) < (o & CM);
+            x = ((lane & j) == 0) == below ? x : o;
+        }
+        const uint32_t keep = min(cnt + nw, a.L);
+        const bool in = (uint32_t)lane < keep;
+        const uint64_t k = in ? (x & CM) : KEY_EMPTY;
+        const bool f = in && (x & FLB) != 0;
+        if ((uint32_t)lane < a.L) {
+            a.pool[(size_t)n * a.L + lane] = k;
+            a.pflag[(size_t)n * a.L + lane] = f ? 1 : 0;
+        }
+        const uint32_t fresh = (uint32_t)__popcll(__ballot(in && (x & NEWB) != 0));
+        // compute_M on the new flags (one ballot: the pool holds <= 64)
+        uint32_t m = 0;
+        if (a.S > 0) {
+            const uint32_t maxl = min(a.M[n] + a.S, keep);
+            uint64_t b = __ballot((uint32_t)lane < maxl && f);
+            if ((uint32_t)__popcll(b) >= a.S) {
+                for (uint32_t i = 1; i < a.S; ++i) b &= b - 1;
+                m = (uint32_t)__ffsll((unsigned long long)b);  // position + 1
+            } else {
+                m = maxl;
+            }
+        }
+        const uint64_t last = keep ? readlane64(k, (int)keep - 1) : KEY_EMPTY;
+        if (lane == 0) {
+            a.pcnt[n] = keep;
+            a.rec[n].thr = last;
+            a.worst[n] = keep ? key_dist(last) : -1.0f;
+            a.M[n] = m;
+            atomicAdd(&ctr_slot(a.ctr)[3], (unsigned long long)fresh);
+        }
+        if (rehash) {
+            wave_sync();
+            for (uint32_t i = lane; i < hp2; i += WAVE) hk[i] = ID_NONE;
+            wave_sync();
+            if (in) {
+                const uint32_t id = key_id(k);
+                uint32_t h = fmix32(id) & hmask;
+                while (atomicCAS(&hk[h], ID_NONE, id) != ID_NONE) h = (h + 1) & hmask;
+            }
+            wave_sync();
+            uint32_t *dst = a.phash + (size_t)n * a.HP;
+            for (uint32_t i = lane; i < hp2; i += WAVE) dst[i] = hk[i];
+        }
+        return;
+    }
     // sort the winners only (the pool part is sorted), then merge the two runs
     // by rank: a key's slot is its index plus the other run's keys below it
     if (nw > 0) {
