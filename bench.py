@@ -149,6 +149,21 @@ def join_roofline(st, N, d, kdim, L, dt, traffic_per_launch=None):
     return out
 
 
+def lds_residency(L, tuning):
+    """Wavefronts a CU holds for a plain search launch of pool L as its LDS
+    allows (the library's plan: the pool of L rounded up to 64 keys, a
+    64-key block and 64 ids, the visited table and the merge buffer, in the
+    160 KiB of a CU), at most 16; the kernel's VGPRs can lower it further
+    for small pools."""
+    P = max(64, (L + 63) // 64 * 64)
+    H = tuning.get("visited_slots") or (min(2048, max(1024, 1 << (8 * L - 1).bit_length())) if L <= 256
+                                        else (1024 if L <= 512 else 256))
+    mb = tuning.get("merge_buffer", 0)
+    B = (256 if L >= 2560 else 0) if mb == 0 else (0 if mb == 1 else mb)
+    lds = (P * 8 + 64 * 8 + 64 * 4 + H * 4 + B * 8 + 15) // 16 * 16
+    return max(1, min(16, (160 * 1024) // lds))
+
+
 def c2_search(ctx, ix, base, q, g, dg):
     """The 1M x 128 configuration's search: fixed L in {20, 40, 80, 160},
     k = 10, seeded random entries, on the FP32 NN-Descent graph (K=64 S=16
@@ -921,6 +936,21 @@ def main():
                        "dist_evals_per_query": round(evals / Q, 1),
                        "reference_evals_per_query": round(ref_evals / Q, 1),
                        "hops_per_query": round(hops / Q, 1)}
+    # An achievable ceiling for the access pattern: the search's distance
+    # phase alone (random 64-row gathers of this base's rows, scored with the
+    # search's gather tile) at the residency the served launch's LDS allows,
+    # and at others for comparison.
+    res = lds_residency(L_plain, plain_tuning)
+    ceil = {}
+    for wpc in sorted({4, res, 12, 16}):
+        g_bps, _ = ctx.gather_rate(N, d, wpc, batches=256)
+        ceil[str(wpc)] = round(g_bps, 1)
+    out["roofline"]["gather_ceiling"] = {
+        "unit": "GB/s", "by_waves_per_cu": ceil, "search_waves_per_cu": res,
+        "frac_of_ceiling": round(achieved / ceil[str(res)], 4),
+        "note": "random rows of the base gathered and scored 64 at a time with the search's tile, "
+                "waves_per_cu wavefronts resident a CU (wv_gather_rate)"}
+    log(f"gather ceiling {ceil} GB/s; search at {res} wavefronts/CU: {achieved:.0f} GB/s")
     print(json.dumps(out), flush=True)
     for a in (dq, d_ids, d_dist, d_ev, d_hp):
         a.free()

@@ -413,6 +413,18 @@ int wv_exact_knn(wv_ctx *ctx, const float *base, uint32_t N, const float *querie
                  uint32_t d, uint32_t k, int64_t self_offset, uint32_t *out_ids,
                  double *out_dist, double *kernel_ms);
 
+/* ---------------------------------------------------------------- measurement */
+/* The rate this GPU sustains for the search's distance phase alone: random
+ * row gathers scored against a query with the search's 64-row gather tile.
+ * No counterpart in the reference; it gives the search's roofline an
+ * achievable ceiling for its access pattern beside the HBM peak.
+ * waves_per_cu wavefronts (1..16) are held resident on every CU (one a
+ * workgroup, residency fixed through LDS), each drawing `batches` x 64
+ * uniformly random ids of an N x d table (d <= 256, N >= 1) in device
+ * memory.  *gbps = rows x 4d bytes / kernel time; *kernel_ms the time. */
+int wv_gather_rate(wv_ctx *ctx, uint32_t N, uint32_t d, uint32_t waves_per_cu, uint32_t batches,
+                   double *gbps, double *kernel_ms);
+
 #ifdef __cplusplus
 }
 #endif

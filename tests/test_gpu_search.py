@@ -425,3 +425,20 @@ def test_search_graph_per_query_budgets_and_model(ctx, small_float, small_float_
     for a in (dq, dL, dk, d_ids, d_ev, m_ids):
         a.free()
     ix.close()
+
+
+def test_gather_rate(ctx):
+    """The gather-rate measurement runs the search's gather tile over random
+    rows at a fixed residency and reports a positive, finite rate; more
+    resident wavefronts do not make it slower by more than noise; shapes
+    outside its range are refused."""
+    from weavess_amd import WvError
+    lo, ms_lo = ctx.gather_rate(200_000, 96, 2, batches=64)
+    hi, ms_hi = ctx.gather_rate(200_000, 96, 8, batches=64)
+    assert 0 < lo < 1e5 and 0 < hi < 1e5 and ms_lo > 0 and ms_hi > 0
+    assert hi > 0.8 * lo
+    g128, _ = ctx.gather_rate(50_000, 128, 4, batches=16)
+    assert g128 > 0
+    for bad in ((1000, 300, 4), (1000, 96, 0), (1000, 96, 17), (0, 96, 4)):
+        with pytest.raises(WvError):
+            ctx.gather_rate(*bad, batches=4)

@@ -33,7 +33,7 @@ EXPORTS = [
     "wv_search_graph_launch", "wv_search_graph_destroy", "wv_mlp_grad", "wv_mlp_train",
     "wv_mlp_predict", "wv_index_build_efanna", "wv_index_build_nsg", "wv_index_build_vamana",
     "wv_index_build_dpg", "wv_index_build_ssg", "wv_index_search_dev_noalloc",
-    "wv_index_build_nndescent_monitor",
+    "wv_index_build_nndescent_monitor", "wv_gather_rate",
 ]
 WV_NOALLOC_MAX_Q = 1 << 18
 
@@ -185,6 +185,7 @@ def load() -> C.CDLL:
         "wv_search_greedy": ([vp, vp, vp, u32, vp, u32, vp, u64, u32, u32, vp, vp, u32, vp, vp, vp,
                               vp, vp], i32),
         "wv_exact_knn": ([vp, vp, u32, vp, u32, u32, u32, C.c_int64, vp, vp, vp], i32),
+        "wv_gather_rate": ([vp, u32, u32, u32, u32, C.POINTER(C.c_double), C.POINTER(C.c_double)], i32),
         "wv_index_search_ext": ([vp, vp, u32, vp, u64, u32, u32, vp, vp, u32, vp, vp, vp, vp, vp, vp], i32),
         "wv_index_search_ext_dev": ([vp, vp, u32, vp, u64, u32, u32, vp, vp, u32, vp, vp, vp, vp, vp, vp],
                                     i32),
@@ -250,6 +251,14 @@ class Context:
         self.check(self.lib.wv_exact_knn(self.h, _p(base), N, _p(queries), Q, d, k, self_offset,
                                          _p(ids), _p(dist), C.byref(ms)))
         return ids, dist, ms.value
+
+    def gather_rate(self, N, d, waves_per_cu, batches=256):
+        """Random row gathers of an N x d table scored with the search's
+        64-row gather tile, waves_per_cu wavefronts resident a CU; returns
+        (GB/s of row bytes, kernel ms)."""
+        gbps, ms = C.c_double(0.0), C.c_double(0.0)
+        self.check(self.lib.wv_gather_rate(self.h, N, d, waves_per_cu, batches, C.byref(gbps), C.byref(ms)))
+        return gbps.value, ms.value
 
     def mlp_grad(self, model: "TermModel", X, y, pos_weight=1.0):
         X = _c(X, np.float32)

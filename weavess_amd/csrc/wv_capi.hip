@@ -1377,6 +1377,39 @@ int wv_search_greedy(wv_ctx *c, const wv_graph *gr, const float *base, uint32_t 
     return rc;
 }
 
+int wv_gather_rate(wv_ctx *c, uint32_t N, uint32_t d, uint32_t waves_per_cu, uint32_t batches, double *gbps,
+                   double *kernel_ms) {
+    if (!c) return WV_EINVAL;
+    std::lock_guard<std::mutex> g(c->mu);
+    if (!gbps) return fail(c, WV_EINVAL, "null argument");
+    if (N == 0 || batches == 0) return fail(c, WV_EINVAL, "N and batches must be positive");
+    if (d == 0 || d > 256) return fail(c, WV_ENOTSUP, "gather rate supports 1 <= d <= 256");
+    if (waves_per_cu == 0 || waves_per_cu > 16) return fail(c, WV_EINVAL, "waves_per_cu must be 1..16");
+    if (c->cu_count <= 0) return fail(c, WV_EHIP, "compute unit count unknown");
+    WV_HIP(c, hipSetDevice(c->device));
+    const uint32_t dp = (d + 3) / 4 * 4;
+    const uint32_t blocks = (uint32_t)c->cu_count * waves_per_cu;
+    // LDS a workgroup, in whole KiB, so that exactly waves_per_cu fit the
+    // 160 KiB of a CU (at least the 768 bytes the kernel uses)
+    const uint32_t lds = std::max<uint32_t>(768u, (160u * 1024u / waves_per_cu) & ~1023u);
+    DevBuf db, dout;
+    WV_HIP(c, db.alloc((size_t)N * dp * 4));
+    WV_HIP(c, dout.alloc((size_t)blocks * 64 * 8));
+    WV_HIP(c, hipMemsetAsync(db.p, 0, (size_t)N * dp * 4, c->stream));
+    // one untimed launch, then the timed one
+    WV_HIP(c, wv::launch_gather_rate(db.as<float>(), N, dp, blocks, std::min<uint32_t>(batches, 4u), lds,
+                                     dout.as<uint64_t>(), c->stream));
+    WV_HIP(c, hipEventRecord(c->ev0, c->stream));
+    WV_HIP(c, wv::launch_gather_rate(db.as<float>(), N, dp, blocks, batches, lds, dout.as<uint64_t>(), c->stream));
+    WV_HIP(c, hipEventRecord(c->ev1, c->stream));
+    WV_HIP(c, hipStreamSynchronize(c->stream));
+    float ms = 0.f;
+    WV_HIP(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    if (kernel_ms) *kernel_ms = ms;
+    *gbps = (double)blocks * 64.0 * batches * 4.0 * d / (ms * 1e-3) / 1e9;
+    return WV_OK;
+}
+
 int wv_exact_knn(wv_ctx *c, const float *base, uint32_t N, const float *queries, uint32_t Q,
                  uint32_t d, uint32_t k, int64_t self_offset, uint32_t *out_ids, double *out_dist,
                  double *kernel_ms) {

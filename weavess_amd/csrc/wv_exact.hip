@@ -123,7 +123,58 @@ __global__ void __launch_bounds__(256) exact_knn_kernel(ExactArgs a) {
     }
 }
 
+// Random row gathers in the search's distance phase, alone: each
+// wavefront draws 64 uniformly random ids (a hash of wavefront, batch and
+// lane, mod N) into LDS and scores their rows against a fixed query with
+// the search's gather tile, `batches` times.  The dynamic LDS the launch
+// asks for fixes how many wavefronts a CU holds.
+template <int CPL, int G>
+__global__ void __launch_bounds__(64) gather_rate_kernel(const float *base, uint32_t N, uint32_t dp, uint32_t nch,
+                                                         uint32_t batches, uint64_t *out) {
+    extern __shared__ __align__(16) uint8_t smem[];
+    uint32_t *cid = reinterpret_cast<uint32_t *>(smem);
+    uint64_t *ckey = reinterpret_cast<uint64_t *>(smem + WAVE * 4);
+    const int lane = lane_id();
+    const uint32_t w = blockIdx.x;
+    float4 qv[CPL];
+    load_query<CPL>(base, nch, qv);
+    uint64_t acc = 0;
+    for (uint32_t b = 0; b < batches; ++b) {
+        cid[lane] = fmix32(w * 0x9E3779B1u + (b * WAVE + (uint32_t)lane) * 0x85EBCA6Bu) % N;
+        wave_sync();
+        rows_dist<CPL, G>(base, dp, nch, qv, cid, WAVE, ckey);
+        wave_sync();
+        acc ^= ckey[lane];
+        wave_sync();
+    }
+    out[(size_t)w * WAVE + lane] = acc;
+}
+
+template <int CPL, int G>
+hipError_t launch_gather_rate_t(const float *base, uint32_t N, uint32_t dp, uint32_t nch, uint32_t blocks,
+                                uint32_t batches, uint32_t lds, uint64_t *out, hipStream_t s) {
+    const void *fn = reinterpret_cast<const void *>(&gather_rate_kernel<CPL, G>);
+    if (lds > 65536) {
+        const hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL((gather_rate_kernel<CPL, G>), dim3(blocks), dim3(WAVE), lds, s, base, N, dp, nch, batches,
+                       out);
+    return hipGetLastError();
+}
+
 }  // namespace
+
+hipError_t launch_gather_rate(const float *base, uint32_t N, uint32_t dp, uint32_t blocks, uint32_t batches,
+                              uint32_t lds, uint64_t *out, hipStream_t s) {
+    const uint32_t nch = dp / 4;
+    // the search's 64-row gather tile for each row length
+    if (nch <= 16) return launch_gather_rate_t<1, 16>(base, N, dp, nch, blocks, batches, lds, out, s);
+    if (nch <= 24) return launch_gather_rate_t<3, 8>(base, N, dp, nch, blocks, batches, lds, out, s);
+    if (nch <= 32) return launch_gather_rate_t<2, 16>(base, N, dp, nch, blocks, batches, lds, out, s);
+    if (nch <= 64) return launch_gather_rate_t<4, 8>(base, N, dp, nch, blocks, batches, lds, out, s);
+    return hipErrorInvalidValue;
+}
 
 hipError_t launch_exact_knn(const ExactArgs &a, hipStream_t s) {
     if (a.k == 0 || a.k > EX_KP) return hipErrorInvalidValue;

@@ -76,4 +76,11 @@ struct ExactArgs {
 };
 hipError_t launch_exact_knn(const ExactArgs &a, hipStream_t s);
 
+// ------------------------------------------------------------ gather rate
+// `blocks` one-wavefront workgroups, each scoring `batches` x 64 random rows
+// of the N x dp table (dp <= 256) with the search's 64-row gather tile;
+// `lds` bytes of dynamic LDS a workgroup (>= 768) set the residency.
+hipError_t launch_gather_rate(const float *base, uint32_t N, uint32_t dp, uint32_t blocks, uint32_t batches,
+                              uint32_t lds, uint64_t *out, hipStream_t s);
+
 }  // namespace wv
