@@ -457,7 +457,11 @@ __device__ inline void search_query(const SearchArgs &a, uint8_t *my, const uint
 // VGPRs, no spills) for the 64-row gather tile of rows of 17-24 chunks,
 // which otherwise takes a few more and drops to 2 where LDS would allow 3.
 // (The persistent form keeps more state live and would spill under it.)
-template <int CPL, int G, bool EXTRA> struct SearchWaves { static constexpr int value = (CPL == 3 && G == 8 && !EXTRA) ? 3 : 1; };
+template <int CPL, int G, bool EXTRA> struct SearchWaves {
+    static constexpr int value = EXTRA ? 1
+                               : (CPL == 3 && G == 8) ? 3
+                               : ((CPL == 1 && G == 8) || (CPL == 2 && G == 4) || (CPL == 3 && G == 2)) ? 5 : 1;
+};
 
 // One wavefront per query: query = its global wavefront index.
 template <int CPL, int G, bool COLLECT, bool WATCH>
