@@ -490,12 +490,16 @@ __global__ void __launch_bounds__(256) nnd_sample_kernel(NndDev a) {
     const uint32_t Mn = a.M[n];
     const uint64_t *pr = a.pool + (size_t)n * a.L;
     uint8_t *fl = a.pflag + (size_t)n * a.L;
+    // the first 64 slots load with M, not after it (a row holds L slots;
+    // those at or past M are masked below)
+    const uint64_t key0 = (uint32_t)lane < a.L ? pr[lane] : KEY_EMPTY;
+    const uint8_t f0 = (uint32_t)lane < a.L ? fl[lane] : (uint8_t)0;
     uint32_t nf = 0, no = 0;
     for (uint32_t t0 = 0; t0 < Mn; t0 += WAVE) {
         const uint32_t p = t0 + lane;
         const bool act = p < Mn;
-        const uint64_t key = act ? pr[p] : KEY_EMPTY;
-        const bool f = act && fl[p];
+        const uint64_t key = act ? (t0 == 0 ? key0 : pr[p]) : KEY_EMPTY;
+        const bool f = act && (t0 == 0 ? f0 : fl[p]);
         const uint64_t bn = __ballot(f), bo = __ballot(act && !f);
         if (act) {
             const uint32_t id = key_id(key);
@@ -528,14 +532,24 @@ __global__ void __launch_bounds__(256) nnd_rev_scatter_kernel(NndDev a) {
     const uint32_t n = blockIdx.x * (blockDim.x >> 6) + wid;
     if (n >= a.N) return;
     const uint32_t nf = a.fnew_cnt[n], no = a.fold_cnt[n];
-    for (uint32_t i = lane; i < nf + no; i += WAVE) {
-        const bool isnew = i < nf;
-        const size_t j = isnew ? (size_t)n * a.fst + i : (size_t)n * a.L + (i - nf);
-        const uint32_t pos = isnew ? a.rpos_new[j] : a.rpos_old[j];
+    // the first 64 entries of both lists load with the counts, not after
+    // them (rows hold fst and L entries; those past a count are masked)
+    const size_t bn = (size_t)n * a.fst, bo = (size_t)n * a.L;
+    const uint32_t pn0 = (uint32_t)lane < a.fst ? a.rpos_new[bn + lane] : ID_NONE;
+    const uint64_t kn0 = (uint32_t)lane < a.fst ? a.fnew[bn + lane] : 0;
+    const uint32_t po0 = (uint32_t)lane < a.L ? a.rpos_old[bo + lane] : ID_NONE;
+    const uint64_t ko0 = (uint32_t)lane < a.L ? a.fold[bo + lane] : 0;
+    for (uint32_t i = lane; i < nf; i += WAVE) {
+        const uint32_t pos = i < (uint32_t)WAVE ? pn0 : a.rpos_new[bn + i];
         if (pos == ID_NONE) continue;
-        const uint32_t id = key_id(isnew ? a.fnew[j] : a.fold[j]);
-        if (isnew) a.rv_new[a.roff_new[id] + pos] = n;
-        else a.rv_old[a.roff_old[id] + pos] = n;
+        const uint32_t id = key_id(i < (uint32_t)WAVE ? kn0 : a.fnew[bn + i]);
+        a.rv_new[a.roff_new[id] + pos] = n;
+    }
+    for (uint32_t i = lane; i < no; i += WAVE) {
+        const uint32_t pos = i < (uint32_t)WAVE ? po0 : a.rpos_old[bo + i];
+        if (pos == ID_NONE) continue;
+        const uint32_t id = key_id(i < (uint32_t)WAVE ? ko0 : a.fold[bo + i]);
+        a.rv_old[a.roff_old[id] + pos] = n;
     }
 }
 
@@ -562,10 +576,15 @@ __global__ void __launch_bounds__(256) nnd_rev_select_kernel(NndDev a, uint32_t 
     uint32_t *ids = reinterpret_cast<uint32_t *>(my);
     uint32_t *hist = ids + rp2;
     uint32_t *cnt = hist + 256;
+    // both lists' offsets, then the first 64 ids of both segments, before
+    // either is processed
+    const uint32_t on0 = a.roff_new[x], on1 = a.roff_new[x + 1];
+    const uint32_t oo0 = a.roff_old[x], oo1 = a.roff_old[x + 1];
+    const uint32_t vfirst[2] = {(uint32_t)lane < on1 - on0 ? a.rv_new[on0 + lane] : ID_NONE,
+                                (uint32_t)lane < oo1 - oo0 ? a.rv_old[oo0 + lane] : ID_NONE};
     for (int which = 0; which < 2; ++which) {
-        const uint32_t *off = which ? a.roff_old : a.roff_new;
-        const uint32_t *seg = (which ? a.rv_old : a.rv_new) + off[x];
-        const uint32_t c = off[x + 1] - off[x];
+        const uint32_t *seg = which ? a.rv_old + oo0 : a.rv_new + on0;
+        const uint32_t c = which ? oo1 - oo0 : on1 - on0;
         const uint64_t sx = rev_stream(a.seed, which ? TAG_REVOLD : TAG_REVNEW, a.iter, x);
         const uint32_t R = a.R;
         uint32_t kept;
@@ -573,7 +592,7 @@ __global__ void __launch_bounds__(256) nnd_rev_select_kernel(NndDev a, uint32_t 
             // all kept, one per lane: sorted in registers
             int w = 2;
             while ((uint32_t)w < c) w <<= 1;
-            const uint64_t v = wave_sort64_n((uint32_t)lane < c ? (uint64_t)seg[lane] : (uint64_t)ID_NONE, w);
+            const uint64_t v = wave_sort64_n((uint32_t)lane < c ? (uint64_t)vfirst[which] : (uint64_t)ID_NONE, w);
             if ((uint32_t)lane < c) (which ? a.rsel_old : a.rsel_new)[(size_t)x * R + lane] = (uint32_t)v;
             if (lane == 0) (which ? a.rsel_old_cnt : a.rsel_new_cnt)[x] = c;
             continue;
@@ -1073,8 +1092,11 @@ __global__ void __launch_bounds__(256) nnd_merge_kernel(NndDev a, uint32_t up2, 
     for (int r = 0; r < MERGE_REGS; ++r) {
         const uint32_t i = lane + r * WAVE;
         ck[r] = (uint32_t)r < cper ? cs[i] : KEY_EMPTY;
-        pk[r] = i < cnt ? pr[i] : KEY_EMPTY;
-        pfl[r] = i < cnt ? pf[i] : (uint8_t)0;
+        // loaded with the count, not after it (a row holds L slots)
+        const uint64_t k = i < a.L ? pr[i] : KEY_EMPTY;
+        const uint8_t f = i < a.L ? pf[i] : (uint8_t)0;
+        pk[r] = i < cnt ? k : KEY_EMPTY;
+        pfl[r] = i < cnt ? f : (uint8_t)0;
     }
     for (uint32_t i = lane; i < hp2; i += WAVE) hk[i] = ID_NONE;
     wave_sync();
