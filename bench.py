@@ -23,6 +23,11 @@ Default workload "c5" (the configuration the metric is quoted on):
     BF16 (FP32 rerank); graph recall on 10,000 control nodes against exact
     FP64 kNN; the join's roofline; graph recall with 256 and 1,024 slots;
     the 1M x 128 configuration's fixed-L search (L 20-160) on the FP32 graph.
+  * Precision leg: the 1M x 960 non-negative mixture (K=64 L=64 S=16 R=64,
+    12 rounds) built with FP32, BF16 and FP8 joins (FP32 rerank of each
+    pool's top 2K): build seconds, graph recall, the join formula's and the
+    stored distances' error against FP64; and each dtype's build at the
+    fewest rounds that reach graph recall 0.95.
   * Batching leg: per-query L ~ U{16..400}, k in {1, 10, 100} over the
     200,000 queries in one autotuned launch on the 2M x 96 graph.
   * CPU baseline (host cores, the oracle): plain search on a bounded sample
@@ -239,21 +244,27 @@ def join_distance_error(base, ids, deg, dtype, sample, seed=0):
     return join_formula_rel_err(base, ids, deg, dtype, nodes)
 
 
-def run_c3(args, ctx, Index, datagen):
+C3_SETTINGS = dict(K=64, L=64, S=16, R=64, iters=12, rerank_mult=2)
+
+
+def c3_leg(ctx, Index, datagen):
     """Precision: NN-Descent on 1M x 960 with FP32, BF16 and FP8 joins (FP32
-    rerank of each pool's top 2K), graph recall and distance error vs FP64."""
+    rerank of each pool's top 2K), graph recall and distance error vs FP64,
+    at the workload's 12 rounds and at the fewest rounds that reach graph
+    recall 0.95 (each dtype's own count)."""
     t0 = time.time()
     base, _ = datagen.gmm_c3(n_query=1)
     N, d = base.shape
-    log(f"data {N}x{d} in {time.time() - t0:.1f}s")
+    log(f"c3 data {N}x{d} in {time.time() - t0:.1f}s")
     ix = Index(ctx, base)
     ctl = 10_000
     gt, _, gt_ms = ctx.exact_knn(base, base[:ctl], 10, self_offset=0)
-    log(f"exact FP64 10-NN of {ctl} control nodes: {gt_ms:.1f} ms")
-    K, L, S, R, iters = 64, 64, 16, 64, 12
+    log(f"c3 exact FP64 10-NN of {ctl} control nodes: {gt_ms:.1f} ms")
+    c = C3_SETTINGS
     per = {}
     for dt in ("f32", "bf16", "fp8"):
-        st = ix.build_nndescent(K, L, S, R, iters, seed=1, dtype=dt, rerank_mult=2)
+        st = ix.build_nndescent(c["K"], c["L"], c["S"], c["R"], c["iters"], seed=1, dtype=dt,
+                                rerank_mult=c["rerank_mult"])
         ids, dist, deg = ix.get_graph()
         gr = graph_recall(ids[:ctl], deg[:ctl], gt)
         jmax, jmean, npairs = join_distance_error(base, ids, deg, dt, 2000)
@@ -272,16 +283,42 @@ def run_c3(args, ctx, Index, datagen):
                        graph_distance_rel_err_mean=float(srel.mean()), error_pairs=npairs)
         log(f"c3 {dt}: {st['total_ms']:.1f} ms, graph recall {gr:.4f}, join distance rel err max {jmax:.3e} "
             f"mean {jmean:.3e}; graph distances (FP32 rerank) max {srel.max():.3e}")
+    # build seconds at graph recall 0.95: each dtype's fewest rounds reaching it
+    at95 = {}
+    for dt in ("f32", "bf16", "fp8"):
+        for it in range(2, c["iters"] + 1):
+            st = ix.build_nndescent(c["K"], c["L"], c["S"], c["R"], it, seed=1, dtype=dt,
+                                    rerank_mult=c["rerank_mult"])
+            ids, _, deg = ix.get_graph()
+            gr = graph_recall(ids[:ctl], deg[:ctl], gt)
+            if gr >= 0.95 or it == c["iters"]:
+                at95[dt] = dict(iters=it, build_s=round(st["total_ms"] / 1e3, 4), join_ms=round(st["join_ms"], 1),
+                                graph_recall=round(gr, 4), reached_0_95=gr >= 0.95)
+                break
+    log(f"c3 at graph recall 0.95: {at95}")
     ix.close()
+    del base
+    return {"workload": f"c3: {N} x {d} non-negative 500-cluster GMM; NN-Descent K={c['K']} L={c['L']} "
+                        f"S={c['S']} R={c['R']} iters={c['iters']}; FP32 rerank of each pool's top 2K",
+            "control_nodes": ctl,
+            "error_sample": "graph edges of 2,000 random nodes; join formula emulated on the host "
+                            "(BF16 RNE; FP8 E4M3 with per-row absmax/448 scale)",
+            "per_dtype": per,
+            "bf16_speedup_vs_f32": round(per["f32"]["build_s"] / per["bf16"]["build_s"], 3),
+            "fp8_speedup_vs_f32": round(per["f32"]["build_s"] / per["fp8"]["build_s"], 3),
+            "at_graph_recall_0_95": at95,
+            "bf16_speedup_vs_f32_at_0_95": round(at95["f32"]["build_s"] / at95["bf16"]["build_s"], 3)}
+
+
+def run_c3(args, ctx, Index, datagen):
+    """The precision configuration alone, as its own JSON line."""
+    leg = c3_leg(ctx, Index, datagen)
     out = {"metric": "NN-Descent precision: graph recall and distance error vs FP64 per join dtype",
-           "value": per["bf16"]["graph_recall"], "unit": "graph recall@10 (BF16 join)", "n_gpus": 1,
-           "higher_is_better": True, "data": f"synthetic, seeded (gmm_c3), {N}x{d}",
-           "config": {"workload": "c3: 1M x 960 non-negative 500-cluster GMM; NN-Descent K=64 L=64 S=16 R=64 "
-                                  "iters=12; FP32 rerank of each pool's top 2K",
-                      "control_nodes": ctl,
-                      "error_sample": "graph edges of 2,000 random nodes; join formula emulated on the host "
-                                      "(BF16 RNE; FP8 E4M3 with per-row absmax/448 scale)"},
-           "per_dtype": per}
+           "value": leg["per_dtype"]["bf16"]["graph_recall"], "unit": "graph recall@10 (BF16 join)", "n_gpus": 1,
+           "higher_is_better": True, "data": "synthetic, seeded (gmm_c3), 1000000x960",
+           "config": {"workload": leg["workload"], "control_nodes": leg["control_nodes"],
+                      "error_sample": leg["error_sample"]},
+           **{k: v for k, v in leg.items() if k not in ("workload", "control_nodes", "error_sample")}}
     print(json.dumps(out), flush=True)
 
 
@@ -634,9 +671,10 @@ def main():
     ap.add_argument("--target", type=float, default=0.95)
     ap.add_argument("--no-cpu", action="store_true", help="skip the host-core baseline")
     ap.add_argument("--no-nnd-leg", action="store_true", help="skip the 1M x 128 NN-Descent leg")
+    ap.add_argument("--no-c3-leg", action="store_true", help="skip the 1M x 960 precision leg")
     ap.add_argument("--fixed-L", type=int, default=0,
                     help="profiling run: plain search at this L only (no L sweep, learned stop, second "
-                         "target or batching leg)")
+                         "target, NN-Descent legs or batching leg)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -656,8 +694,11 @@ def main():
     if args.fixed_L:
         w.update(sweep=[args.fixed_L], learned=False)
     nnd_leg = None
-    if not args.no_nnd_leg and args.workload == "c5":
+    if not args.no_nnd_leg and args.workload == "c5" and not args.fixed_L:
         nnd_leg = nndescent_leg(ctx, Index, datagen)
+    c3 = None
+    if not args.no_c3_leg and args.workload == "c5" and not args.fixed_L:
+        c3 = c3_leg(ctx, Index, datagen)
 
     t0 = time.time()
     base, allq = getattr(datagen, w["gen"])(**w["gen_kw"])
@@ -815,6 +856,8 @@ def main():
         out["targets"] = per_target
     if nnd_leg is not None:
         out["nndescent"] = nnd_leg
+    if c3 is not None:
+        out["nndescent_c3"] = c3
 
     if not args.no_cpu:
         import oracle  # the CPU baseline leg: the oracle is timed here, never the product

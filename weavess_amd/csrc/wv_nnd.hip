@@ -718,10 +718,33 @@ __device__ inline void load_ops(const NndDev &a, const uint32_t *list, int r0, i
     }
 }
 
+// Branch-free form of load_ops for a whole row tile of TS steps (rows of at
+// most 512 bytes), so that waits on these loads can count them: rows at or
+// past `count` read row 0 and steps at or past T repeat step T - 1; their
+// products reach only outputs no proposal reads or steps no MFMA takes.
+// FP32 elements at or past kdim (padded widths that are not a multiple of
+// 16) read the row's first vector and are zeroed at the MFMA (ops_mfma_k);
+// BF16 and FP8 rows are padded to a multiple of 32 and have none.
+template <int DT, int TS>
+__device__ inline void load_tile(const NndDev &a, const uint32_t *list, int r0, int count, int T,
+                                 typename JoinT<DT>::V ops[TS]) {
+    using V = typename JoinT<DT>::V;
+    constexpr int KSTEP = JoinT<DT>::KSTEP, ES = JoinT<DT>::ES;
+    const int lane = lane_id(), r = lane & 15, g = lane >> 4;
+    const uint32_t li = list[r0 + r];  // LDS; read past the list, unused there
+    const uint8_t *row = a.xq + (size_t)(r0 + r < count ? li : 0u) * a.xq_stride;
+#pragma unroll
+    for (int j = 0; j < TS; ++j) {
+        const uint32_t e0 = (uint32_t)(KSTEP * min(j, T - 1) + (KSTEP / 4) * g);
+        ops[j] = *reinterpret_cast<const V *>(row + (size_t)(e0 < a.kdim ? e0 : 0u) * ES);
+    }
+}
+
 // MFMA steps j < n of register operands into acc, in step order.
+// ztail: this lane's step n - 1 is zero on both operands.
 template <int DT, int TS>
 __device__ inline floatx4 ops_mfma(const typename JoinT<DT>::V A[TS], const typename JoinT<DT>::V B[TS], int n,
-                                   floatx4 acc) {
+                                   floatx4 acc, bool ztail = false) {
 #pragma unroll
     for (int t = 0; t < TS; ++t) {
         if (t >= n) continue;
@@ -729,6 +752,7 @@ __device__ inline floatx4 ops_mfma(const typename JoinT<DT>::V A[TS], const type
             float4 x, y;
             __builtin_memcpy(&x, &A[t], 16);
             __builtin_memcpy(&y, &B[t], 16);
+            if (ztail && t == n - 1) x = y = make_float4(0.f, 0.f, 0.f, 0.f);
             acc = __builtin_amdgcn_mfma_f32_16x16x4f32(x.x, y.x, acc, 0, 0, 0);
             acc = __builtin_amdgcn_mfma_f32_16x16x4f32(x.y, y.y, acc, 0, 0, 0);
             acc = __builtin_amdgcn_mfma_f32_16x16x4f32(x.z, y.z, acc, 0, 0, 0);
@@ -746,6 +770,22 @@ __device__ inline floatx4 ops_mfma(const typename JoinT<DT>::V A[TS], const type
         }
     }
     return acc;
+}
+
+// ops_mfma for tiles from load_tile: FP32 step T - 1's elements at or past
+// kdim are zeroed on both operands (as load_ops leaves them), only when the
+// padded width is not a multiple of 16 (a wavefront-uniform test).
+template <int DT, int TS>
+__device__ inline floatx4 ops_mfma_k(const typename JoinT<DT>::V A[TS], const typename JoinT<DT>::V B[TS], int T,
+                                     uint32_t kdim, floatx4 acc) {
+    if constexpr (DT == DT_F32) {
+        if (kdim & (JoinT<DT>::KSTEP - 1)) {
+            const uint32_t e_last =
+                (uint32_t)(JoinT<DT>::KSTEP * (T - 1) + (JoinT<DT>::KSTEP / 4) * (lane_id() >> 4));
+            return ops_mfma<DT, TS>(A, B, T, acc, e_last >= kdim);
+        }
+    }
+    return ops_mfma<DT, TS>(A, B, T, acc);
 }
 
 // Is `id` in target t's pool?  Linear probing of t's pool-id hash from slot
@@ -780,7 +820,7 @@ __device__ inline bool pooled(const NndDev &a, uint32_t t, uint32_t id, uint32_t
 // `finish` (at the next flush, or at the end) probes and proposes, so the
 // load's latency overlaps the tiles in between.  The ring holds the block
 // in flight ([head, head + pn)) and the waiting entries behind it.
-constexpr uint32_t JQ_CAP = 256;  // >= 64 in flight + 63 waiting + 128 pushed per pair index
+constexpr uint32_t JQ_CAP = 256;  // >= 128 in flight or waiting before a row's pushes, + 128 pushed
 constexpr uint32_t JQ_BYTES = JQ_CAP * 8;
 struct JoinQ {
     uint2 *q;
@@ -860,9 +900,33 @@ __device__ inline uint32_t jq_drain(const NndDev &a, JoinQ &Q, const uint32_t *c
     return hits;
 }
 
+// At a node's end (fewer than 64 waiting): the waiting entries' first hash
+// groups load before the block in flight is finished, so both blocks
+// resolve after one wait instead of two in a row.
+__device__ inline uint32_t jq_final(const NndDev &a, JoinQ &Q, const uint32_t *col) {
+    wave_sync();
+    const uint32_t lane = (uint32_t)lane_id();
+    const uint32_t m = Q.n;
+    uint4 g2 = make_uint4(0, 0, 0, 0);
+    if (lane < m) {
+        const uint2 e = Q.q[(Q.head + Q.pn + lane) & (JQ_CAP - 1)];
+        const uint32_t t = col[e.y & 0xFFFFu], id = col[e.y >> 16];
+        const uint32_t h = fmix32(id) & (a.HP - 1);
+        g2 = *reinterpret_cast<const uint4 *>(a.phash + (size_t)t * a.HP + (h & ~3u));
+    }
+    uint32_t hits = 0;
+    if (Q.pn) hits += jq_finish(a, Q, col);
+    Q.g = g2;
+    Q.pn = m;
+    Q.n = 0;
+    if (m) hits += jq_finish(a, Q, col);
+    return hits;
+}
+
 // Proposals of tile pair (rt, ct) from its dot products: lane holds rows
 // 16rt+4g+i, column 16ct+cl.  Each valid pair proposes both directions;
-// a proposal at or below the target's worst (LDS) enters the ring.
+// a proposal at or below the target's worst (LDS) enters the ring.  The
+// ring is drained here only when it could not hold the next row's pushes.
 template <int DT>
 __device__ inline void tile_proposals(const NndDev &a, JoinQ &Q, const uint32_t *col, const uint64_t *cthr,
                                       const float *cnorm, const float *cscale, uint32_t nn, uint32_t nc, int rt,
@@ -874,7 +938,8 @@ __device__ inline void tile_proposals(const NndDev &a, JoinQ &Q, const uint32_t 
     const uint64_t tb = bok ? cthr[b] : 0u;
     const float nb = bok ? cnorm[b] : 0.f;
     const float sb = (DT == DT_FP8 && bok) ? cscale[b] : 1.f;
-#pragma unroll
+    // one copy of the row body (and of its rare drain), not four
+#pragma unroll 1
     for (int i = 0; i < 4; ++i) {
         const uint32_t ar = (uint32_t)(rt * 16 + 4 * g + i);
         // new x new: i < j once
@@ -884,7 +949,7 @@ __device__ inline void tile_proposals(const NndDev &a, JoinQ &Q, const uint32_t 
         bool v0 = false, v1 = false;
         float d = 0.f;
         if (ok) {
-            float dot = acc[i];
+            float dot = i == 0 ? acc[0] : i == 1 ? acc[1] : i == 2 ? acc[2] : acc[3];
             if (DT == DT_FP8) dot = dot * (cscale[ar] * sb);
             d = (cnorm[ar] + nb) - 2.0f * dot;
             d = fmaxf(d, 0.0f);
@@ -894,14 +959,14 @@ __device__ inline void tile_proposals(const NndDev &a, JoinQ &Q, const uint32_t 
         }
         jq_push(Q, v0, d, ar, b);
         jq_push(Q, v1, d, b, ar);
-        props += jq_drain(a, Q, col);
+        // room for the next 128 pushes (the caller drains at its own point)
+        if (Q.pn + Q.n > JQ_CAP - 2 * WAVE) props += jq_drain(a, Q, col);
     }
 }
 
 // Wavefronts per SIMD the join is compiled for: 4 (at most 128 VGPRs),
-// except FP32 rows of 5-8 MFMA steps, whose 32-VGPR tiles spill under that
-// cap; at 3 they do not, and that measured 2 % faster.
-template <int DT, int TS> struct JoinWaves { static constexpr int value = (DT == DT_F32 && TS == 8) ? 3 : 4; };
+// except FP8 rows of 9-16 MFMA steps, whose tiles spill under that cap.
+template <int DT, int TS> struct JoinWaves { static constexpr int value = (DT == DT_FP8 && TS == 16) ? 3 : 4; };
 
 // join(), src/component_refine.cpp:127-143 / nhood::join,
 // include/weavess/index.h:137-149, one wavefront per node.
@@ -928,41 +993,67 @@ nnd_join_kernel(NndDev a) {
 
     const uint32_t fnc = a.fnew_cnt[n], rnc = a.rsel_new_cnt[n];
     const uint32_t foc = a.fold_cnt[n], roc = a.rsel_old_cnt[n];
+    // the first 64 entries of the four lists load with the counts, not after
+    // them (rows hold fst, R, L and R entries; entries past a count are
+    // loaded but never used)
+    const size_t bf = (size_t)n * a.fst, br = (size_t)n * a.R, bo = (size_t)n * a.L;
+    const uint32_t fn0 = (uint32_t)lane < a.fst ? key_id(a.fnew[bf + lane]) : ID_NONE;
+    const uint32_t rn0 = (uint32_t)lane < a.R ? a.rsel_new[br + lane] : ID_NONE;
+    const uint32_t fo0 = (uint32_t)lane < a.L ? key_id(a.fold[bo + lane]) : ID_NONE;
+    const uint32_t ro0 = (uint32_t)lane < a.R ? a.rsel_old[br + lane] : ID_NONE;
     const uint32_t nn = fnc + rnc;
     const uint32_t no = min(foc + roc, 2 * a.R);
     if (nn == 0) return;
-    // the new ids, ascending (position order == id order): in registers
-    // when they fit one wavefront, else in LDS over the width in use
+    // new entry i: fnew's i-th, then rsel_new's (i - fnc)-th; sorted
+    // ascending (position order == id order) in registers when they fit
+    // one wavefront, else in LDS over the width in use
     if (nn <= (uint32_t)WAVE) {
         const uint32_t i = (uint32_t)lane;
-        const uint32_t v = i < fnc ? key_id(a.fnew[(size_t)n * a.fst + i])
-                         : i < nn ? a.rsel_new[(size_t)n * a.R + (i - fnc)] : ID_NONE;
+        const uint32_t rs = (uint32_t)__shfl((int)rn0, (int)((i - fnc) & (WAVE - 1)));
+        const uint32_t v = i < fnc ? fn0 : i < nn ? rs : ID_NONE;
         int w = 2;
         while ((uint32_t)w < nn) w <<= 1;
-        nw[i] = (uint32_t)wave_sort64_n((uint64_t)v, w);  // lanes from w up hold ID_NONE
+        const uint32_t sv = (uint32_t)wave_sort64_n((uint64_t)v, w);  // lanes from w up hold ID_NONE
+        if (i < nn) col[i] = sv;
     } else {
         int pw = 2 * WAVE;  // <= nw_pow2
         while ((uint32_t)pw < nn) pw <<= 1;
         for (uint32_t i = lane; i < (uint32_t)pw; i += WAVE)
-            nw[i] = i < fnc ? key_id(a.fnew[(size_t)n * a.fst + i])
-                  : i < nn ? a.rsel_new[(size_t)n * a.R + (i - fnc)] : ID_NONE;
+            nw[i] = i < fnc ? key_id(a.fnew[bf + i]) : i < nn ? a.rsel_new[br + (i - fnc)] : ID_NONE;
         wave_sync();
         lds_sort32(nw, pw);
+        for (uint32_t i = lane; i < nn; i += WAVE) col[i] = nw[i];
+    }
+    // old entry j (list position nn + j): fold's j-th, then rsel_old's (j - foc)-th
+    {
+        const uint32_t j = (uint32_t)lane;
+        const uint32_t rs = (uint32_t)__shfl((int)ro0, (int)((j - foc) & (WAVE - 1)));
+        if (j < no) col[nn + j] = j < foc ? fo0 : rs;
+        for (uint32_t jj = j + WAVE; jj < no; jj += WAVE)
+            col[nn + jj] = jj < foc ? key_id(a.fold[bo + jj]) : a.rsel_old[br + (jj - foc)];
     }
     wave_sync();
     const uint32_t nc = nn + no;
-    for (uint32_t i = lane; i < nc; i += WAVE) {
-        uint32_t id;
-        if (i < nn) id = nw[i];
-        else if (i - nn < foc) id = key_id(a.fold[(size_t)n * a.L + (i - nn)]);
-        else id = a.rsel_old[(size_t)n * a.R + (i - nn - foc)];
-        col[i] = id;
-        const uint4 e = *reinterpret_cast<const uint4 *>(a.rec + id);  // thr, norm, scale in one load
-        cthr[i] = (uint64_t)e.x | ((uint64_t)e.y << 32);
-        cnorm[i] = __uint_as_float(e.z);
-        if (DT == DT_FP8) cscale[i] = __uint_as_float(e.w);  // only the FP8 list has this array
-    }
-    wave_sync();
+    // each entry's threshold, norm and scale in one 16-byte record; the first
+    // 128 records stay in flight while the first row tiles are requested
+    // (unconditional, node 0's record for lanes past the list: a load under
+    // a branch is waited for inside it)
+    const uint4 rec0 = *reinterpret_cast<const uint4 *>(a.rec + ((uint32_t)lane < nc ? col[lane] : 0u));
+    const uint4 rec1 =
+        *reinterpret_cast<const uint4 *>(a.rec + ((uint32_t)lane + WAVE < nc ? col[lane + WAVE] : 0u));
+    auto put_records = [&]() {
+        auto put = [&](uint32_t i, uint4 e) {
+            cthr[i] = (uint64_t)e.x | ((uint64_t)e.y << 32);
+            cnorm[i] = __uint_as_float(e.z);
+            if (DT == DT_FP8) cscale[i] = __uint_as_float(e.w);  // only the FP8 list has this array
+        };
+        // rec1 is selected outside the branch, so its load is not moved into it
+        const uint4 e1 = (uint32_t)lane + WAVE < nc ? rec1 : make_uint4(0, 0, 0, 0);
+        if ((uint32_t)lane < nc) put(lane, rec0);
+        if ((uint32_t)lane + WAVE < nc) put(lane + WAVE, e1);
+        for (uint32_t i = lane + 2 * WAVE; i < nc; i += WAVE) put(i, *reinterpret_cast<const uint4 *>(a.rec + col[i]));
+        wave_sync();
+    };
 
     const uint32_t kdim = a.kdim;
     const int nrt = (int)((nn + 15) / 16), nct = (int)((nc + 15) / 16);
@@ -971,27 +1062,53 @@ nnd_join_kernel(NndDev a) {
         using V = typename JoinT<DT>::V;
         constexpr int RT = TS * (int)sizeof(V) <= 64 ? 2 : 1;
         const int T = (int)((kdim + KSTEP - 1) / KSTEP);
-        for (int rt0 = 0; rt0 < nrt; rt0 += RT) {
-            V A[RT][TS], B[TS];
+        // A sweep takes row tiles rt0 .. rt0+RT-1.  They are also column
+        // tiles rt0 .. rt0+RT-1, so A[q] is gathered once with every listed
+        // row (rows at or past nn give dot products no proposal reads), and
+        // the column tiles after them stream through B, the first one
+        // requested with A.  Tiles past the list load as zeros, without
+        // memory accesses.
+        V A[RT][TS], B[TS];
+        auto load_sweep = [&](int rt0) {
 #pragma unroll
-            for (int q = 0; q < RT; ++q) load_ops<DT, TS>(a, col, (rt0 + q) * 16, (int)nn, T, 0, A[q]);
-            load_ops<DT, TS>(a, col, rt0 * 16, (int)nc, T, 0, B);
+            for (int q = 0; q < RT; ++q) load_tile<DT, TS>(a, col, (rt0 + q) * 16, (int)nc, T, A[q]);
+            load_tile<DT, TS>(a, col, (rt0 + RT) * 16, (int)nc, T, B);
+        };
+        // Every load here is issued unconditionally (row 0 past the list), so
+        // a wait on one tile counts the loads behind it and does not wait for
+        // them.  The ring is drained right after a streamed tile's MFMAs,
+        // before the next tile is requested: a drain waits for its hash
+        // group loads, which would otherwise also wait for that tile.
+        const floatx4 z = {0.f, 0.f, 0.f, 0.f};
+        load_sweep(0);
+        put_records();
+        for (int rt0 = 0; rt0 < nrt; rt0 += RT) {
+            if (rt0 > 0) load_sweep(rt0);
             for (int ct = rt0; ct < nct; ++ct) {
+                const int j = ct - rt0;  // column tile ct is A[j] while j < RT, else B
                 floatx4 acc[RT];
 #pragma unroll
-                for (int q = 0; q < RT; ++q)
-                    if (rt0 + q < nrt && ct >= rt0 + q)
-                        acc[q] = ops_mfma<DT, TS>(A[q], B, T, floatx4{0.f, 0.f, 0.f, 0.f});
-                // the next column tile loads while these proposals are made
-                if (ct + 1 < nct) load_ops<DT, TS>(a, col, (ct + 1) * 16, (int)nc, T, 0, B);
+                for (int q = 0; q < RT; ++q) {
+                    if (rt0 + q < nrt && j >= q) {
+                        if (j >= RT) acc[q] = ops_mfma_k<DT, TS>(A[q], B, T, kdim, z);
+                        else if (RT > 1 && j == 1) acc[q] = ops_mfma_k<DT, TS>(A[q], A[RT - 1], T, kdim, z);
+                        else acc[q] = ops_mfma_k<DT, TS>(A[q], A[0], T, kdim, z);
+                    }
+                }
+                if (j >= RT) {
+                    props += jq_drain(a, Q, col);
+                    // the next column tile loads while these proposals are made
+                    load_tile<DT, TS>(a, col, (ct + 1) * 16, (int)nc, T, B);
+                }
 #pragma unroll
                 for (int q = 0; q < RT; ++q)
-                    if (rt0 + q < nrt && ct >= rt0 + q)
+                    if (rt0 + q < nrt && j >= q)
                         tile_proposals<DT>(a, Q, col, cthr, cnorm, cscale, nn, nc, rt0 + q, ct, acc[q], pairs,
                                            props);
             }
         }
     } else {
+        put_records();
         // longer rows: per tile pair, pieces of PIECE steps streamed through
         // registers, the next piece loading during the current one's MFMAs
         using V = typename JoinT<DT>::V;
@@ -1016,14 +1133,12 @@ nnd_join_kernel(NndDev a) {
                     }
                 }
                 tile_proposals<DT>(a, Q, col, cthr, cnorm, cscale, nn, nc, rt, ct, acc, pairs, props);
+                props += jq_drain(a, Q, col);
             }
         }
     }
-    if (Q.pn) props += jq_finish(a, Q, col);
-    while (Q.n > 0) {
-        jq_issue(a, Q, col);
-        props += jq_finish(a, Q, col);
-    }
+    props += jq_drain(a, Q, col);
+    props += jq_final(a, Q, col);
     pairs = wave_sum_u32(pairs);
     props = wave_sum_u32(props);
     if (lane == 0) {
