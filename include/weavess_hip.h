@@ -72,7 +72,8 @@ typedef struct {
     double total_ms;     /* InitRandom + NN-Descent + finalize, device time          */
     double init_ms, sample_ms, join_ms, merge_ms, finalize_ms;
     uint64_t pairs;      /* distance evaluations in all joins                        */
-    uint64_t proposals;  /* join results that passed the target's threshold          */
+    uint64_t proposals;  /* join results that passed the target's threshold and were
+                            not already in its pool (candidate-slot updates)         */
     uint64_t rows_gathered; /* sample rows loaded by the joins                       */
     uint32_t iters_run;
     uint32_t cand_slots;
