@@ -125,7 +125,7 @@ def graph_recall(ids, deg, gt, control=10):
 MFMA_PEAK_TFLOPS = {"f32": 157.3, "bf16": 2500.0, "fp8": 5000.0}
 
 
-def join_roofline(st, N, d, kdim, L, dt, traffic_per_launch=None):
+def join_roofline(st, N, d, kdim, L, dt, traffic_per_launch=None, misses_per_launch=None, line_ceiling=None):
     """Roofline of the local join over the build's joins.  Algorithmic work
     per (node, round): flops 2 d P with P the pairs evaluated; bytes
     (|new| + |old|) (kdim b + 4) for the listed rows (b = 4 / 2 / 1 bytes an
@@ -151,6 +151,16 @@ def join_roofline(st, N, d, kdim, L, dt, traffic_per_launch=None):
         out["traffic_over_alg_bytes"] = round(traffic_per_launch / (alg_bytes / rounds), 3)
     else:
         out["traffic"] = None
+    if misses_per_launch and line_ceiling:
+        # the join's records, probes, slot updates and row pieces are random
+        # lines: their rate against what the GPU sustains for random lines
+        got = misses_per_launch / (t / rounds) / 1e9
+        out["line_rate"] = {"unit": "G lines/s", "l2_miss_lines_per_launch": int(misses_per_launch),
+                            "achieved": round(got, 1), "ceiling": round(line_ceiling, 1),
+                            "frac": round(got / line_ceiling, 3),
+                            "note": "L2 misses of the join (profiles/nndescent_join_traffic.json) over its "
+                                    "live kernel time, against random 16-byte loads from a 1 GiB array "
+                                    "(wv_line_rate, measured in this run)"}
     return out
 
 
@@ -199,13 +209,16 @@ def nndescent_leg(ctx, Index, datagen):
     ix = Index(ctx, base)
     ctl = min(w["control"], len(base))
     gt, _, _ = ctx.exact_knn(base, base[:ctl], 10, self_offset=0)
-    traffic = {}
+    traffic, misses = {}, {}
     tp = os.path.join(ROOT, "profiles", "nndescent_join_traffic.json")
     if os.path.exists(tp):
         try:
-            traffic = json.load(open(tp)).get("hbm_bytes_per_launch", {})
+            tj = json.load(open(tp))
+            traffic, misses = tj.get("hbm_bytes_per_launch", {}), tj.get("l2_miss_lines_per_launch", {})
         except (ValueError, OSError):
-            traffic = {}
+            traffic, misses = {}, {}
+    line_ceiling, _ = ctx.line_rate(1 << 30, iters=64)
+    log(f"random-line ceiling: {line_ceiling:.1f} G loads/s (1 GiB array)")
     out = {"workload": w["desc"]}
     kdim = {"f32": (d + 3) // 4 * 4, "bf16": (d + 31) // 32 * 32}
     for dt in ("bf16", "f32"):
@@ -219,7 +232,8 @@ def nndescent_leg(ctx, Index, datagen):
                        reached_0_95=gr >= 0.95, init_ms=round(st["init_ms"], 1), sample_ms=round(st["sample_ms"], 1),
                        join_ms=round(st["join_ms"], 1), merge_ms=round(st["merge_ms"], 1),
                        finalize_ms=round(st["finalize_ms"], 1), pairs=int(st["pairs"]),
-                       roofline=join_roofline(st, N, d, kdim[dt], w["L"], dt, traffic.get(dt)))
+                       roofline=join_roofline(st, N, d, kdim[dt], w["L"], dt, traffic.get(dt), misses.get(dt),
+                                              line_ceiling))
         log(f"nndescent leg {dt}: {st['total_ms']:.1f} ms, graph recall {gr:.4f}, join {out[dt]['roofline']}")
         if dt == "f32":
             out["c2_search"] = c2_search(ctx, ix, base, cq, g, dg)

@@ -426,6 +426,15 @@ int wv_exact_knn(wv_ctx *ctx, const float *base, uint32_t N, const float *querie
 int wv_gather_rate(wv_ctx *ctx, uint32_t N, uint32_t d, uint32_t waves_per_cu, uint32_t batches,
                    double *gbps, double *kernel_ms);
 
+/* The rate this GPU sustains for loads of random lines: 16-byte loads at
+ * uniformly random (hashed) positions of a device array of `bytes` bytes
+ * (a multiple of 16, at least 1 MiB), 4 independent loads a thread per
+ * round, `iters` rounds, 8 wavefronts resident a SIMD.  No counterpart in
+ * the reference; it is the ceiling of the NN-Descent join, whose threshold
+ * records, pool-id probes and candidate-slot updates are such loads.
+ * *gaccess = loads / kernel time in G/s; *kernel_ms the time. */
+int wv_line_rate(wv_ctx *ctx, uint64_t bytes, uint32_t iters, double *gaccess, double *kernel_ms);
+
 #ifdef __cplusplus
 }
 #endif

@@ -442,3 +442,17 @@ def test_gather_rate(ctx):
     for bad in ((1000, 300, 4), (1000, 96, 0), (1000, 96, 17), (0, 96, 4)):
         with pytest.raises(WvError):
             ctx.gather_rate(*bad, batches=4)
+
+
+def test_line_rate(ctx):
+    """The random-line measurement reports a positive, finite load rate; an
+    array that fits the L2 caches is served at least as fast as one of 1 GiB;
+    sizes outside its range are refused."""
+    from weavess_amd import WvError
+    small, ms_s = ctx.line_rate(4 << 20, iters=8)
+    big, ms_b = ctx.line_rate(1 << 30, iters=8)
+    assert 0 < big < 1e4 and 0 < small < 1e4 and ms_s > 0 and ms_b > 0
+    assert small > 0.8 * big
+    for nbytes, iters in (((1 << 20) - 16, 4), ((1 << 20) + 8, 4), (1 << 20, 0)):
+        with pytest.raises(WvError):
+            ctx.line_rate(nbytes, iters=iters)

@@ -33,7 +33,7 @@ EXPORTS = [
     "wv_search_graph_launch", "wv_search_graph_destroy", "wv_mlp_grad", "wv_mlp_train",
     "wv_mlp_predict", "wv_index_build_efanna", "wv_index_build_nsg", "wv_index_build_vamana",
     "wv_index_build_dpg", "wv_index_build_ssg", "wv_index_search_dev_noalloc",
-    "wv_index_build_nndescent_monitor", "wv_gather_rate",
+    "wv_index_build_nndescent_monitor", "wv_gather_rate", "wv_line_rate",
 ]
 WV_NOALLOC_MAX_Q = 1 << 18
 
@@ -186,6 +186,7 @@ def load() -> C.CDLL:
                               vp, vp], i32),
         "wv_exact_knn": ([vp, vp, u32, vp, u32, u32, u32, C.c_int64, vp, vp, vp], i32),
         "wv_gather_rate": ([vp, u32, u32, u32, u32, C.POINTER(C.c_double), C.POINTER(C.c_double)], i32),
+        "wv_line_rate": ([vp, C.c_uint64, u32, C.POINTER(C.c_double), C.POINTER(C.c_double)], i32),
         "wv_index_search_ext": ([vp, vp, u32, vp, u64, u32, u32, vp, vp, u32, vp, vp, vp, vp, vp, vp], i32),
         "wv_index_search_ext_dev": ([vp, vp, u32, vp, u64, u32, u32, vp, vp, u32, vp, vp, vp, vp, vp, vp],
                                     i32),
@@ -259,6 +260,13 @@ class Context:
         gbps, ms = C.c_double(0.0), C.c_double(0.0)
         self.check(self.lib.wv_gather_rate(self.h, N, d, waves_per_cu, batches, C.byref(gbps), C.byref(ms)))
         return gbps.value, ms.value
+
+    def line_rate(self, nbytes, iters=64):
+        """16-byte loads at random positions of an nbytes device array, 8
+        wavefronts a SIMD; returns (G loads/s, kernel ms)."""
+        g, ms = C.c_double(0.0), C.c_double(0.0)
+        self.check(self.lib.wv_line_rate(self.h, nbytes, iters, C.byref(g), C.byref(ms)))
+        return g.value, ms.value
 
     def mlp_grad(self, model: "TermModel", X, y, pos_weight=1.0):
         X = _c(X, np.float32)

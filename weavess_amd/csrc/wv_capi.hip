@@ -1410,6 +1410,33 @@ int wv_gather_rate(wv_ctx *c, uint32_t N, uint32_t d, uint32_t waves_per_cu, uin
     return WV_OK;
 }
 
+int wv_line_rate(wv_ctx *c, uint64_t bytes, uint32_t iters, double *gaccess, double *kernel_ms) {
+    if (!c) return WV_EINVAL;
+    std::lock_guard<std::mutex> g(c->mu);
+    if (!gaccess) return fail(c, WV_EINVAL, "null argument");
+    if (bytes < ((uint64_t)1 << 20) || bytes % 16) return fail(c, WV_EINVAL, "bytes must be a multiple of 16, >= 1 MiB");
+    if (iters == 0) return fail(c, WV_EINVAL, "iters must be positive");
+    if (c->cu_count <= 0) return fail(c, WV_EHIP, "compute unit count unknown");
+    WV_HIP(c, hipSetDevice(c->device));
+    // 32 workgroups of 256 a CU: the 8 wavefronts a SIMD the hardware holds
+    const uint32_t blocks = (uint32_t)c->cu_count * 32;
+    DevBuf db, dout;
+    WV_HIP(c, db.alloc(bytes));
+    WV_HIP(c, dout.alloc((size_t)blocks * 256 * 4));
+    WV_HIP(c, hipMemsetAsync(db.p, 0, bytes, c->stream));
+    // one short untimed launch, then the timed one
+    WV_HIP(c, wv::launch_line_rate(db.p, bytes, blocks, std::min<uint32_t>(iters, 4u), dout.as<uint32_t>(), c->stream));
+    WV_HIP(c, hipEventRecord(c->ev0, c->stream));
+    WV_HIP(c, wv::launch_line_rate(db.p, bytes, blocks, iters, dout.as<uint32_t>(), c->stream));
+    WV_HIP(c, hipEventRecord(c->ev1, c->stream));
+    WV_HIP(c, hipStreamSynchronize(c->stream));
+    float ms = 0.f;
+    WV_HIP(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    if (kernel_ms) *kernel_ms = ms;
+    *gaccess = (double)blocks * 256.0 * iters * 4.0 / (ms * 1e-3) / 1e9;
+    return WV_OK;
+}
+
 int wv_exact_knn(wv_ctx *c, const float *base, uint32_t N, const float *queries, uint32_t Q,
                  uint32_t d, uint32_t k, int64_t self_offset, uint32_t *out_ids, double *out_dist,
                  double *kernel_ms) {

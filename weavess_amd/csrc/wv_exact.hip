@@ -176,6 +176,36 @@ hipError_t launch_gather_rate(const float *base, uint32_t N, uint32_t dp, uint32
     return hipErrorInvalidValue;
 }
 
+// Random 16-byte loads from an array of n16 16-byte vectors: each thread
+// makes `iters` rounds of 4 independent loads at hashed indices (mod n16,
+// so every access is inside the array).  What the GPU sustains for loads of
+// random lines, the access pattern of the NN-Descent join's records, hash
+// probes and slot updates.
+__global__ void __launch_bounds__(256) line_rate_kernel(const uint4 *__restrict__ a, uint64_t n16, uint32_t iters,
+                                                        uint32_t *out) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t h = fmix32(t * 0x9E3779B1u + 0x7F4A7C15u), acc = 0;
+    for (uint32_t i = 0; i < iters; ++i) {
+        uint4 v[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            h = fmix32(h + 0x9E3779B9u);
+            v[j] = a[(uint64_t)h % n16];
+        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc ^= v[j].x ^ v[j].w;
+    }
+    out[t] = acc;
+}
+
+hipError_t launch_line_rate(const void *a, uint64_t bytes, uint32_t blocks, uint32_t iters, uint32_t *out,
+                            hipStream_t s) {
+    if (bytes < 16) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(line_rate_kernel, dim3(blocks), dim3(256), 0, s, static_cast<const uint4 *>(a), bytes / 16,
+                       iters, out);
+    return hipGetLastError();
+}
+
 hipError_t launch_exact_knn(const ExactArgs &a, hipStream_t s) {
     if (a.k == 0 || a.k > EX_KP) return hipErrorInvalidValue;
     const size_t lds = sizeof(XKey) * EX_TQ * EX_SORT + sizeof(double) * EX_TQ * a.d;

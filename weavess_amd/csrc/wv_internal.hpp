@@ -82,5 +82,7 @@ hipError_t launch_exact_knn(const ExactArgs &a, hipStream_t s);
 // `lds` bytes of dynamic LDS a workgroup (>= 768) set the residency.
 hipError_t launch_gather_rate(const float *base, uint32_t N, uint32_t dp, uint32_t blocks, uint32_t batches,
                               uint32_t lds, uint64_t *out, hipStream_t s);
+hipError_t launch_line_rate(const void *a, uint64_t bytes, uint32_t blocks, uint32_t iters, uint32_t *out,
+                            hipStream_t s);
 
 }  // namespace wv
