@@ -366,6 +366,15 @@ int wv_index_search_ext_dev(wv_index *idx, const float *d_queries, uint32_t Q,
 int wv_index_autotune(wv_index *idx, const float *d_queries, uint32_t Q, uint64_t seed, uint32_t L,
                       uint32_t k, const wv_search_ext *ext, wv_search_config *best,
                       double *best_ms);
+/* The same autotuner timing the search of the Q sample queries under their
+ * own per-query budgets (d_L_per_q, d_k_per_q: Q values each on the device,
+ * checked as for wv_index_search_dev; outputs of stride k_stride), for
+ * launches that serve uneven L and k: the schedule and tile that win for
+ * one L can lose for a mix.  No counterpart in the reference (the batching
+ * extension). */
+int wv_index_autotune_budgets(wv_index *idx, const float *d_queries, uint32_t Q, uint64_t seed,
+                              const uint32_t *d_L_per_q, const uint32_t *d_k_per_q, uint32_t k_stride,
+                              const wv_search_ext *ext, wv_search_config *best, double *best_ms);
 
 /* hipGraph-captured search for serving loops: the launch of
  * wv_index_search_ext_dev (random entries, with the stop model of `ext` when

@@ -170,6 +170,32 @@ def test_search_autotune_changes_speed_only(ctx, small_float, small_float_graph,
     assert np.array_equal(got[3], ref[3]) and np.all(got[2] >= ref[2])
 
 
+def test_search_autotune_budgets_changes_speed_only(ctx, small_float, small_float_graph):
+    """The autotuner under per-query budgets keeps a valid tuning in the
+    index, and the uneven-budget search it serves still returns the oracle's
+    ids, distances and hops (evaluations only grow where a smaller visited
+    table forgets)."""
+    from weavess_amd import Index
+    from weavess_amd.devmem import DeviceArray, pad_rows
+    base, q = small_float
+    g, _, deg = small_float_graph
+    rng = np.random.default_rng(11)
+    Lq = rng.integers(16, 300, size=len(q)).astype(np.uint32)
+    kq = rng.choice(np.array([1, 10, 16], np.uint32), size=len(q))
+    ref = O.search_greedy(g, deg, base, q, 0, 0, seed=6, L_per_q=Lq, k_per_q=kq, k_stride=16)
+    ix = Index(ctx, base)
+    ix.set_graph(g, deg)
+    dq, dL, dk = DeviceArray.from_host(pad_rows(q)), DeviceArray.from_host(Lq), DeviceArray.from_host(kq)
+    cfg, ms = ix.autotune_budgets_dev(dq.ptr, len(q), dL.ptr, dk.ptr, 16, seed=6)
+    for a_ in (dq, dL, dk):
+        a_.free()
+    assert ms > 0 and cfg["schedule"] in (0, 1, 2) and cfg["waves_per_query"] in (0, 1, 2, 4)
+    got = ix.search(q, 0, 0, seed=6, L_per_q=Lq, k_per_q=kq, k_stride=16)
+    ix.close()
+    assert np.array_equal(got[0], ref[0]) and np.array_equal(got[1], ref[1])
+    assert np.array_equal(got[3], ref[3]) and np.all(got[2] >= ref[2])
+
+
 @pytest.mark.parametrize("L", [48, 130, 600, 1500])
 @pytest.mark.parametrize("mb", [1, 64, 256])
 def test_search_merge_buffer_parity(ctx, small_float, small_float_graph, L, mb):

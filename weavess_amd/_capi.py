@@ -33,7 +33,7 @@ EXPORTS = [
     "wv_search_graph_launch", "wv_search_graph_destroy", "wv_mlp_grad", "wv_mlp_train",
     "wv_mlp_predict", "wv_index_build_efanna", "wv_index_build_nsg", "wv_index_build_vamana",
     "wv_index_build_dpg", "wv_index_build_ssg", "wv_index_search_dev_noalloc",
-    "wv_index_build_nndescent_monitor", "wv_gather_rate", "wv_line_rate",
+    "wv_index_build_nndescent_monitor", "wv_gather_rate", "wv_line_rate", "wv_index_autotune_budgets",
 ]
 WV_NOALLOC_MAX_Q = 1 << 18
 
@@ -191,6 +191,7 @@ def load() -> C.CDLL:
         "wv_index_search_ext_dev": ([vp, vp, u32, vp, u64, u32, u32, vp, vp, u32, vp, vp, vp, vp, vp, vp],
                                     i32),
         "wv_index_autotune": ([vp, vp, u32, u64, u32, u32, vp, vp, C.POINTER(C.c_double)], i32),
+        "wv_index_autotune_budgets": ([vp, vp, u32, u64, vp, vp, u32, vp, vp, C.POINTER(C.c_double)], i32),
         "wv_search_graph_create": ([vp, vp, u32, u64, u32, u32, vp, vp, u32, vp, vp, vp, vp, vp,
                                     C.POINTER(vp)], i32),
         "wv_search_graph_launch": ([vp, i32], i32),
@@ -532,6 +533,18 @@ class Index:
         self.ctx.check(self.ctx.lib.wv_index_autotune(self.h, C.c_void_p(d_queries), Q, seed, L, k,
                                                       C.byref(ext) if ext is not None else None,
                                                       C.byref(cfg), C.byref(ms)))
+        return {f: getattr(cfg, f) for f, _ in cfg._fields_}, ms.value
+
+    def autotune_budgets_dev(self, d_queries: int, Q: int, d_L: int, d_k: int, k_stride: int, seed: int = 0,
+                             model: Optional["TermModel"] = None):
+        """autotune_dev under the sample queries' own per-query L and k
+        (device arrays of Q values each); the index keeps the winner."""
+        cfg = SearchConfig()
+        ms = C.c_double(0.0)
+        ext = SearchExt(C.addressof(model), None, None, None, 0) if model is not None else None
+        self.ctx.check(self.ctx.lib.wv_index_autotune_budgets(
+            self.h, C.c_void_p(d_queries), Q, seed, C.c_void_p(d_L), C.c_void_p(d_k), k_stride,
+            C.byref(ext) if ext is not None else None, C.byref(cfg), C.byref(ms)))
         return {f: getattr(cfg, f) for f, _ in cfg._fields_}, ms.value
 
     def search_graph(self, d_queries: int, Q: int, L: int, k: int, d_out_ids: int, seed: int = 0,

@@ -874,15 +874,17 @@ int wv_index_search_ext_dev(wv_index *ix, const float *d_queries, uint32_t Q, co
                        d_out_ids, d_out_dist, d_evals, d_hops, stats, x, d_L_per_q ? hl.data() : nullptr);
 }
 
-int wv_index_autotune(wv_index *ix, const float *d_queries, uint32_t Q, uint64_t seed, uint32_t L, uint32_t k,
-                      const wv_search_ext *ext, wv_search_config *best, double *best_ms) {
-    if (!ix) return WV_EINVAL;
+}  // extern "C"
+
+namespace {
+
+// The autotuner over one L and k, or over per-query budgets (d_L/d_k, with
+// their host copies hl/hk, already checked): coordinate descent from the
+// default plan, each candidate the best of 3 timed launches after a warm-up.
+int autotune_core(wv_index *ix, const float *d_queries, uint32_t Q, uint64_t seed, uint32_t L, uint32_t k,
+                  const uint32_t *d_L, const uint32_t *d_k, const uint32_t *hl, uint32_t k_stride, uint32_t Lmax,
+                  const wv_search_ext *ext, wv_search_config *best, double *best_ms) {
     wv_ctx *c = ix->ctx;
-    std::lock_guard<std::mutex> g(c->mu);
-    if (!d_queries || Q == 0) return fail(c, WV_EINVAL, "autotune needs sample queries");
-    uint32_t Lmax = 0;
-    int rc = check_search_args(c, ix->N, Q, nullptr, L, k, nullptr, nullptr, k, Lmax);
-    if (rc) return rc;
     WV_HIP(c, hipSetDevice(c->device));
     DevBuf dm, di;
     SearchExtDev x;
@@ -891,7 +893,7 @@ int wv_index_autotune(wv_index *ix, const float *d_queries, uint32_t Q, uint64_t
         WV_HIP(c, hipMemcpy(dm.p, ext->model, sizeof(wv_term_model), hipMemcpyHostToDevice));
         x.model = dm.as<wv_term_model>();
     }
-    WV_HIP(c, di.alloc((size_t)Q * k * 4));
+    WV_HIP(c, di.alloc((size_t)Q * k_stride * 4));
     const wv_search_config saved = ix->cfg;
     // best of 3 timed launches after a warm-up; planning errors reject the
     // candidate (rc EINVAL/ENOTSUP), launch errors end the tuning
@@ -900,8 +902,8 @@ int wv_index_autotune(wv_index *ix, const float *d_queries, uint32_t Q, uint64_t
         wv_search_stats st;
         ms = 1e30;
         for (int r = 0; r < 4; ++r) {
-            const int e = search_core(ix, d_queries, Q, nullptr, seed, L, k, nullptr, nullptr, Lmax, k,
-                                      di.as<uint32_t>(), nullptr, nullptr, nullptr, &st, x);
+            const int e = search_core(ix, d_queries, Q, nullptr, seed, L, k, d_L, d_k, Lmax, k_stride,
+                                      di.as<uint32_t>(), nullptr, nullptr, nullptr, &st, x, hl);
             if (e) return e;
             if (r > 0) ms = std::min(ms, st.kernel_ms);
         }
@@ -909,7 +911,8 @@ int wv_index_autotune(wv_index *ix, const float *d_queries, uint32_t Q, uint64_t
     };
     wv_search_config cur = {0, 0, 0, 0, 0, 0};
     double cur_ms = 0;
-    if ((rc = time_cfg(cur, cur_ms))) {
+    int rc = time_cfg(cur, cur_ms);
+    if (rc) {
         ix->cfg = saved;
         return rc;
     }
@@ -948,6 +951,41 @@ int wv_index_autotune(wv_index *ix, const float *d_queries, uint32_t Q, uint64_t
     if (best) *best = cur;
     if (best_ms) *best_ms = cur_ms;
     return WV_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int wv_index_autotune(wv_index *ix, const float *d_queries, uint32_t Q, uint64_t seed, uint32_t L, uint32_t k,
+                      const wv_search_ext *ext, wv_search_config *best, double *best_ms) {
+    if (!ix) return WV_EINVAL;
+    wv_ctx *c = ix->ctx;
+    std::lock_guard<std::mutex> g(c->mu);
+    if (!d_queries || Q == 0) return fail(c, WV_EINVAL, "autotune needs sample queries");
+    uint32_t Lmax = 0;
+    int rc = check_search_args(c, ix->N, Q, nullptr, L, k, nullptr, nullptr, k, Lmax);
+    if (rc) return rc;
+    return autotune_core(ix, d_queries, Q, seed, L, k, nullptr, nullptr, nullptr, k, Lmax, ext, best, best_ms);
+}
+
+int wv_index_autotune_budgets(wv_index *ix, const float *d_queries, uint32_t Q, uint64_t seed,
+                              const uint32_t *d_L_per_q, const uint32_t *d_k_per_q, uint32_t k_stride,
+                              const wv_search_ext *ext, wv_search_config *best, double *best_ms) {
+    if (!ix) return WV_EINVAL;
+    wv_ctx *c = ix->ctx;
+    std::lock_guard<std::mutex> g(c->mu);
+    if (!d_queries || Q == 0) return fail(c, WV_EINVAL, "autotune needs sample queries");
+    if (!d_L_per_q || !d_k_per_q) return fail(c, WV_EINVAL, "per-query L and k are required");
+    WV_HIP(c, hipSetDevice(c->device));
+    std::vector<uint32_t> hl(Q), hk(Q);
+    WV_HIP(c, hipMemcpy(hl.data(), d_L_per_q, (size_t)Q * 4, hipMemcpyDeviceToHost));
+    WV_HIP(c, hipMemcpy(hk.data(), d_k_per_q, (size_t)Q * 4, hipMemcpyDeviceToHost));
+    uint32_t Lmax = 0;
+    int rc = check_search_args(c, ix->N, Q, nullptr, 0, 0, hl.data(), hk.data(), k_stride, Lmax);
+    if (rc) return rc;
+    return autotune_core(ix, d_queries, Q, seed, 0, 0, d_L_per_q, d_k_per_q, hl.data(), k_stride, Lmax, ext, best,
+                         best_ms);
 }
 
 int wv_search_graph_create(wv_index *ix, const float *d_queries, uint32_t Q, uint64_t seed, uint32_t L,
