@@ -121,8 +121,9 @@ typedef struct {
 
 /* Learned early termination: a 2-layer MLP (8 -> 32 ReLU -> 1) over
  * per-hop search-state features, evaluated in BF16 (FP32 accumulation)
- * after every expansion.  Search stops when the logit exceeds
- * logit_threshold and at least min_hops expansions were made.  Features
+ * after every 4th expansion.  Search stops when the logit exceeds
+ * logit_threshold and at least min_hops expansions were made.  (Traces
+ * record the features after every expansion.)  Features
  * g(state), standardised as (g - mean) * inv_std:
  *   0 log2(1 + hops)          1 kth / entry     2 best / entry
  *   3 log2(entry)             4 (kth 4 hops ago - kth) / entry

@@ -896,7 +896,9 @@ int wvo_search_greedy_ext(const uint32_t *graph, const uint32_t *deg, uint32_t d
                     hist[0] = kth;
                     if (trace && hp <= cap)
                         for (int i = 0; i < 8; ++i) trace[((size_t)q * cap + (hp - 1)) * 8 + i] = f[i];
-                    if (model && kk < Lq && hp >= model->min_hops && logit_bf16(model, f) > model->logit_threshold)
+                    /* the rule is evaluated after every 4th expansion */
+                    if (model && hp % 4 == 0 && kk < Lq && hp >= model->min_hops &&
+                        logit_bf16(model, f) > model->logit_threshold)
                         break;
                 }
             }

@@ -195,8 +195,9 @@ int wvo_mlp_grad(const wvo_term_model *m, const float *X, const float *y, uint32
  * bf16 == 0 is plain FP64. */
 int wvo_mlp_logit(const wvo_term_model *m, const float *X, uint32_t n, int bf16, double *out);
 
-/* Greedy search with per-hop traces and/or the learned stop rule.  model,
- * trace (Q*cap*8), trace_len, settle may be NULL. */
+/* Greedy search with per-hop traces and/or the learned stop rule, the rule
+ * evaluated after every 4th expansion.  model, trace (Q*cap*8), trace_len,
+ * settle may be NULL. */
 int wvo_search_greedy_ext(const uint32_t *graph, const uint32_t *deg, uint32_t deg_max,
                           const float *base, uint32_t N, uint32_t d, const float *queries,
                           uint32_t Q, const uint32_t *entries, uint64_t seed, uint32_t L,

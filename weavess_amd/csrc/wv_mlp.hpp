@@ -8,6 +8,10 @@
 namespace wv {
 
 constexpr int MLP_IN = 8, MLP_HID = 32, MLP_PARAMS = MLP_HID * MLP_IN + MLP_HID + MLP_HID + 1;  // 321
+// The stop rule is evaluated after every MLP_STRIDE-th expansion (the
+// features' window: "k-th distance 4 expansions earlier" is then the last
+// evaluation's k-th distance).
+constexpr uint32_t MLP_STRIDE = 4;
 
 __device__ inline float bf16_round(float x) { return (float)(__bf16)x; }
 

@@ -238,9 +238,10 @@ __device__ inline void search_query(const SearchArgs &a, uint8_t *my, const uint
     const float entry_d = watch ? key_dist(pool[0]) : 0.f;
     // k-th distance 1..4 expansions ago
     float r0 = watch ? key_dist(pool[kidx]) : 0.f, r1 = r0, r2 = r0, r3 = r0;  // bit 31 is not in the distance
+    float kprev = r0;  // k-th distance at the last stop-rule evaluation (the entries' at first)
     uint32_t settle = 0;  // last expansion that changed the top-k
     int probes = VIS_PROBES;
-    if constexpr (!COLLECT && !WATCH) {
+    if constexpr (!COLLECT) {
         // The loop below with the next node chosen before the merge: once
         // the last block of u's survivors is sorted, the next node is the
         // smaller of U's first unexpanded key and the smallest survivor (both
@@ -342,8 +343,42 @@ __device__ inline void search_query(const SearchArgs &a, uint8_t *my, const uint
                 }
                 wave_sync();
             }
+            if constexpr (WATCH) {
+                // The state after `hops` expansions: features and the trace
+                // row after every expansion while tracing, the stop rule after
+                // every MLP_STRIDE-th.  The node already chosen for the next
+                // hop is expanded only if the rule does not stop here
+                // (choosing it changed no key's order, only its flag).
+                const bool tr = a.trace != nullptr && hops <= a.trace_cap;
+                const bool at = hops % MLP_STRIDE == 0;
+                if (tr || (use_model && at)) {
+                    float f[MLP_IN];
+                    const uint64_t kth = nb ? kth_union(pool, Li, buf, nb, kidx) : canon(pool[kidx]);
+                    const uint64_t best = (nb && canon(buf[0]) < canon(pool[0])) ? canon(buf[0]) : canon(pool[0]);
+                    // k-th distance MLP_STRIDE expansions ago: the per-hop
+                    // history while tracing, else the last evaluation's
+                    term_features((float)hops, key_dist(kth), key_dist(best), entry_d, tr ? r3 : kprev,
+                                  (float)(hops - settle), (float)L, (float)kq, f);
+                    if (tr) {
+                        r3 = r2;
+                        r2 = r1;
+                        r1 = r0;
+                        r0 = key_dist(kth);
+                        if (lane == 0) {
+                            float4 *t = reinterpret_cast<float4 *>(a.trace +
+                                                                   ((size_t)q * a.trace_cap + (hops - 1)) * MLP_IN);
+                            t[0] = make_float4(f[0], f[1], f[2], f[3]);
+                            t[1] = make_float4(f[4], f[5], f[6], f[7]);
+                        }
+                    }
+                    if (at) kprev = key_dist(kth);
+                    if (at && live && use_model && hops >= net.min_hops && term_logit_regs(net, f) > net.threshold)
+                        break;
+                }
+            }
         }
     } else {
+        static_assert(!WATCH, "the collecting mode runs without the stop rule or traces");
         for (;;) {
             // U's first unexpanded key: the smaller of the pool's and the buffer's
             uint64_t ka, kbk = KEY_EMPTY;
@@ -353,26 +388,6 @@ __device__ inline void search_query(const SearchArgs &a, uint8_t *my, const uint
             const uint64_t cur = from_buf ? kbk : ka;
             // cur is in U iff it is not above U's L-th key (`worst` is kept exact)
             const bool done = cur == KEY_EMPTY || cur > worst;
-            if (watch && hops > 0) {
-                // state after `hops` expansions
-                float f[MLP_IN];
-                const uint64_t kth = nb ? kth_union(pool, Li, buf, nb, kidx) : canon(pool[kidx]);
-                const uint64_t best = (nb && canon(buf[0]) < canon(pool[0])) ? canon(buf[0]) : canon(pool[0]);
-                term_features((float)hops, key_dist(kth), key_dist(best), entry_d, r3, (float)(hops - settle),
-                              (float)L, (float)kq, f);
-                r3 = r2;
-                r2 = r1;
-                r1 = r0;
-                r0 = key_dist(kth);
-                // one lane stores the row as two float4s: a lane-indexed pick
-                // would put f in scratch memory for the whole loop
-                if (a.trace && hops <= a.trace_cap && lane == 0) {
-                    float4 *t = reinterpret_cast<float4 *>(a.trace + ((size_t)q * a.trace_cap + (hops - 1)) * MLP_IN);
-                    t[0] = make_float4(f[0], f[1], f[2], f[3]);
-                    t[1] = make_float4(f[4], f[5], f[6], f[7]);
-                }
-                if (!done && use_model && hops >= net.min_hops && term_logit_regs(net, f) > net.threshold) break;
-            }
             if (done) break;
             const uint32_t u = key_id(cur);  // cur is canonical
             wave_sync();
