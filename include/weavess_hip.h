@@ -66,6 +66,10 @@ typedef struct {
     uint32_t cand_slots; /* per-node candidate table, power of two in [64, 1024]
                             (0 -> automatic)                                        */
     uint64_t seed;       /* random initialisation and reverse-sample choice          */
+    uint32_t order;      /* node processing order: 0 automatic (grouped from 32,768 nodes),
+                            1 by id, 2 grouped by nearest pivot row.  Speed only: the
+                            graph is the same in every order                         */
+    uint32_t reserved;   /* 0                                                        */
 } wv_nnd_params;
 
 typedef struct {

@@ -34,7 +34,7 @@ def test_every_declared_symbol_is_exported():
 
 def test_struct_layouts_match_header():
     # sizes of the POD structs as laid out by the C compiler for the header
-    assert ctypes.sizeof(_capi.NndParams) == 40
+    assert ctypes.sizeof(_capi.NndParams) == 48
     assert ctypes.sizeof(_capi.BuildStats) == 80
     assert ctypes.sizeof(_capi.SearchStats) == 24
     assert ctypes.sizeof(_capi.Graph) == 24

@@ -61,6 +61,42 @@ def test_nndescent_bit_exact_on_integer_data(ctx, small_int, slots):
     assert np.array_equal(gd, rd)
 
 
+@pytest.mark.parametrize("dtype", ["f32", "bf16"])
+def test_nndescent_grouped_order_bit_exact_on_integer_data(ctx, small_int, dtype):
+    """With the nodes processed grouped by nearest pivot (forced on at this
+    size) the build is still the oracle's, bit for bit: no per-node kernel's
+    outcome depends on which wavefront serves a node, or when (BF16 of
+    these small integers is exact too)."""
+    base, _ = small_int
+    K, L, S, R, it = 24, 32, 10, 32, 6
+    gi, gd, gdeg, st = ctx.nndescent_build(base, K, L, S, R, it, seed=3, cand_slots=64, dtype=dtype, order=2)
+    ri, rd, rdeg, rst = O.nndescent(base, K, L, S, R, it, seed=3, cand_slots=64, with_stats=True)
+    assert st["pairs"] == sum(p for p, _ in rst)
+    assert st["proposals"] == sum(a for _, a in rst)
+    assert np.array_equal(gdeg, rdeg) and np.array_equal(gi, ri) and np.array_equal(gd, rd)
+
+
+@pytest.mark.parametrize("dtype", ["f32", "bf16", "fp8"])
+def test_nndescent_order_changes_speed_only(ctx, dtype):
+    """60,000 continuous points (the Zipf mixture, where the automatic order
+    is on): the graph built with nodes grouped by nearest pivot equals the
+    one built in id order, in every join precision."""
+    from weavess_amd import Index, datagen
+    base, _ = datagen.gmm_c2(n_base=60_000, n_query=1)
+    ix = Index(ctx, base)
+    try:
+        out = []
+        for order in (1, 2):
+            st = ix.build_nndescent(32, 48, 12, 64, 8, seed=5, dtype=dtype, cand_slots=128, order=order)
+            out.append((ix.get_graph(), st["pairs"], st["proposals"]))
+        (g1, p1, q1), (g2, p2, q2) = out
+        assert p1 == p2 and q1 == q2
+        for a_, b_ in zip(g1, g2):
+            assert np.array_equal(a_, b_)
+    finally:
+        ix.close()
+
+
 def test_nndescent_full_init_is_exact(ctx, small_float):
     base, _ = small_float
     base = base[:60]

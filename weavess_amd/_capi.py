@@ -47,7 +47,8 @@ class WvError(RuntimeError):
 class NndParams(C.Structure):
     _fields_ = [("K", C.c_uint32), ("L", C.c_uint32), ("S", C.c_uint32), ("R", C.c_uint32),
                 ("iters", C.c_uint32), ("dtype", C.c_uint32), ("rerank_mult", C.c_uint32),
-                ("cand_slots", C.c_uint32), ("seed", C.c_uint64)]
+                ("cand_slots", C.c_uint32), ("seed", C.c_uint64), ("order", C.c_uint32),
+                ("reserved", C.c_uint32)]
 
 
 class BuildStats(C.Structure):
@@ -296,10 +297,10 @@ class Context:
         return z
 
     def nndescent_build(self, base, K, L, S, R, iters, seed=0, dtype="f32", rerank_mult=0,
-                        cand_slots=0, init_ids=None):
+                        cand_slots=0, init_ids=None, order=0):
         base = _c(base, np.float32)
         N, d = base.shape
-        p = NndParams(K, L, S, R, iters, DTYPES[dtype], rerank_mult, cand_slots, seed)
+        p = NndParams(K, L, S, R, iters, DTYPES[dtype], rerank_mult, cand_slots, seed, order)
         ids = np.zeros((N, K), np.uint32)
         dist = np.zeros((N, K), np.float32)
         deg = np.zeros(N, np.uint32)
@@ -402,8 +403,10 @@ class Index:
         self.ctx.check(self.ctx.lib.wv_index_set_search_config(self.h, C.byref(cfg)))
 
     def build_nndescent(self, K, L, S, R, iters, seed=0, dtype="f32", rerank_mult=0,
-                        cand_slots=0, init_ids=None, init_cnt=None):
-        p = NndParams(K, L, S, R, iters, DTYPES[dtype], rerank_mult, cand_slots, seed)
+                        cand_slots=0, init_ids=None, init_cnt=None, order=0):
+        """NN-Descent on the resident base; order: 0 automatic, 1 nodes by id,
+        2 grouped by nearest pivot (speed only, the graph is the same)."""
+        p = NndParams(K, L, S, R, iters, DTYPES[dtype], rerank_mult, cand_slots, seed, order)
         st = BuildStats()
         ii, ic = _c(init_ids, np.uint32), _c(init_cnt, np.uint32)
         self.ctx.check(self.ctx.lib.wv_index_build_nndescent(self.h, C.byref(p), _p(ii), _p(ic),
@@ -411,11 +414,11 @@ class Index:
         return st.as_dict()
 
     def build_nndescent_monitor(self, K, L, S, R, iters, ctrl, truth, seed=0, dtype="f32", rerank_mult=0,
-                                cand_slots=0, init_ids=None, init_cnt=None):
+                                cand_slots=0, init_ids=None, init_cnt=None, order=0):
         """NN-Descent with the per-round graph recall of the control nodes
         `ctrl` against their exact neighbour rows `truth`; returns
         (stats, per_round recall)."""
-        p = NndParams(K, L, S, R, iters, DTYPES[dtype], rerank_mult, cand_slots, seed)
+        p = NndParams(K, L, S, R, iters, DTYPES[dtype], rerank_mult, cand_slots, seed, order)
         st = BuildStats()
         ctrl = _c(ctrl, np.uint32)
         truth = _c(truth, np.uint32)
@@ -427,9 +430,9 @@ class Index:
         return st.as_dict(), per[:iters]
 
     def build_efanna(self, K, L, S, R, iters, seed=0, dtype="f32", rerank_mult=0,
-                     cand_slots=0, init_ids=None, init_cnt=None):
+                     cand_slots=0, init_ids=None, init_cnt=None, order=0):
         """EFANNA's refine; init_ids: None or an N x stride initial graph."""
-        p = NndParams(K, L, S, R, iters, DTYPES[dtype], rerank_mult, cand_slots, seed)
+        p = NndParams(K, L, S, R, iters, DTYPES[dtype], rerank_mult, cand_slots, seed, order)
         st = BuildStats()
         ii, ic = _c(init_ids, np.uint32), _c(init_cnt, np.uint32)
         stride = 0 if ii is None else ii.shape[1]

@@ -97,7 +97,19 @@ struct NndDev {
     uint32_t nw_pow2;         // power of two >= the longest new list (S + R; 2S in EFANNA's first round)
     uint32_t fst;             // ids per fnew row: S, or 2S for EFANNA's first-round samples
     uint32_t ncap;            // longest new + old list: S + 3R, or at least 2S for EFANNA
+    const uint32_t *order;    // processing order of the nodes (null: by id)
 };
+
+// The node the g-th wavefront serves in the kernels that gather other
+// nodes' data (init, join, finalize): the g-th of the processing order, or
+// g.  The order groups nodes by their nearest pivot row, so the wavefronts
+// resident at one time work on one neighbourhood and its rows, records,
+// pool-id tables and candidate slots stay in L2.  Their outcome is the same
+// whichever wavefront serves a node and when: the join's slot updates are
+// atomicMin, every other write is the node's own.  The kernels that work on
+// a node's own rows (sample, reverse scatter and selection, merge, hash)
+// keep id order, in which neighbouring wavefronts read neighbouring rows.
+__device__ inline uint32_t node_at(const NndDev &a, uint32_t g) { return a.order ? a.order[g] : g; }
 
 __device__ inline uint32_t wave_sum_u32(uint32_t v) {
 #pragma unroll
@@ -284,8 +296,9 @@ __global__ void __launch_bounds__(256) nnd_init_kernel(NndDev a, const uint32_t 
                                                        const uint32_t *init_cnt) {
     extern __shared__ __align__(16) uint8_t smem[];
     const int wid = threadIdx.x >> 6, lane = lane_id();
-    const uint32_t n = blockIdx.x * (blockDim.x >> 6) + wid;
-    if (n >= a.N) return;
+    const uint32_t g = blockIdx.x * (blockDim.x >> 6) + wid;
+    if (g >= a.N) return;
+    const uint32_t n = node_at(a, g);
     uint32_t *cid = reinterpret_cast<uint32_t *>(smem) + wid * 64;
     uint64_t *ckey = reinterpret_cast<uint64_t *>(smem + (blockDim.x >> 6) * 256) + wid * 64;
     const uint32_t S = a.S, N = a.N;
@@ -980,8 +993,9 @@ nnd_join_kernel(NndDev a) {
     extern __shared__ __align__(16) uint8_t smem[];
     constexpr int KSTEP = JoinT<DT>::KSTEP;
     const int wid = threadIdx.x >> 6, lane = lane_id();
-    const uint32_t n = blockIdx.x * (blockDim.x >> 6) + wid;
-    if (n >= a.N) return;
+    const uint32_t g = blockIdx.x * (blockDim.x >> 6) + wid;
+    if (g >= a.N) return;
+    const uint32_t n = node_at(a, g);
     const uint32_t ncap = a.ncap;  // |new| <= S+R (2S in EFANNA's first round), |old| <= 2R
     uint8_t *my = smem + (size_t)wid * (a.lds_join_list + JQ_BYTES);
     uint32_t *nw = reinterpret_cast<uint32_t *>(my);              // nw_pow2
@@ -1146,6 +1160,142 @@ nnd_join_kernel(NndDev a) {
         atomicAdd(&c[0], (unsigned long long)pairs);
         atomicAdd(&c[1], (unsigned long long)props);
         atomicAdd(&c[2], (unsigned long long)nc);
+    }
+}
+
+// ------------------------------------------------------------------ order
+// Processing order: nodes grouped by the nearest of M pivot rows (M a
+// multiple of 16, pivot j = node floor(j N / M)), by squared distance over
+// the first kd <= 128 FP32 elements of the base rows.  Only the kernels'
+// speed depends on it (see node_at).
+constexpr uint32_t ORDER_MIN_N = 32768, ORDER_MAX_PIVOTS = 256;
+
+__device__ inline uint32_t pivot_node(uint32_t j, uint32_t N, uint32_t M) {
+    return (uint32_t)((uint64_t)j * N / M);
+}
+
+__global__ void __launch_bounds__(256) nnd_pivot_norm_kernel(const float *base, uint32_t dp, uint32_t kd, uint32_t N,
+                                                             uint32_t M, float *pnorm) {
+    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= M) return;
+    const float *row = base + (size_t)pivot_node(j, N, M) * dp;
+    float s = 0.f;
+    for (uint32_t e = 0; e < kd; ++e) s += row[e] * row[e];
+    pnorm[j] = s;
+}
+
+// One wavefront per 16 nodes: |p|^2 - 2 x.p for 16 pivots at a time on
+// MFMA (16x16x4 f32, operands as in the join's FP32 tiles); each lane keeps
+// the smallest of its column across the pivot tiles, and one reduction over
+// the 16 lanes of a row group gives each node its nearest (ties: the lower
+// pivot).  Loads are issued without branches: steps past T repeat step T-1
+// (not used) and elements at or past kd read the row's first vector and are
+// zeroed at the MFMA.
+__device__ inline void pivot_tile_ops(const float *base, uint32_t dp, uint32_t kd, uint32_t row_node, int T,
+                                      uint4 ops[8]) {
+    const int g = lane_id() >> 4;
+    const float *row = base + (size_t)row_node * dp;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+        const uint32_t e0 = (uint32_t)(16 * min(j, T - 1) + 4 * g);
+        ops[j] = *reinterpret_cast<const uint4 *>(row + (e0 < kd ? e0 : 0u));
+    }
+}
+
+__global__ void __launch_bounds__(256) nnd_assign_kernel(const float *base, uint32_t dp, uint32_t kd, uint32_t N,
+                                                         uint32_t M, const float *pnorm, uint32_t *label) {
+    const int lane = lane_id(), r = lane & 15, g = lane >> 4;
+    const uint32_t n0 = (blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * 16;
+    if (n0 >= N) return;
+    const int T = (int)((kd + 15) / 16);  // steps of 16 elements, 4 MFMAs each
+    const bool ztail = (uint32_t)(16 * (T - 1) + 4 * g) >= kd;
+    uint4 A[8], B[8];
+    pivot_tile_ops(base, dp, kd, min(n0 + (uint32_t)r, N - 1), T, A);  // rows past N: labels not written
+    float bv[4];
+    uint32_t bi[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        bv[i] = INFINITY;
+        bi[i] = 0;
+    }
+    for (uint32_t t = 0; t < M / 16; ++t) {
+        const uint32_t col = t * 16 + (uint32_t)r;  // output: lane holds column r of rows 4g + i
+        pivot_tile_ops(base, dp, kd, pivot_node(col, N, M), T, B);
+        const floatx4 acc = ops_mfma<DT_F32, 8>(A, B, T, floatx4{0.f, 0.f, 0.f, 0.f}, ztail);
+        const float pn = pnorm[col];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const float v = pn - 2.0f * acc[i];
+            if (v < bv[i]) {  // columns ascend, so an equal value keeps the lower pivot
+                bv[i] = v;
+                bi[i] = col;
+            }
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+#pragma unroll
+        for (int o = 1; o < 16; o <<= 1) {  // within the 16 lanes of row group g
+            const float ov = __shfl_xor(bv[i], o);
+            const uint32_t ok = (uint32_t)__shfl_xor((int)bi[i], o);
+            if (ov < bv[i] || (ov == bv[i] && ok < bi[i])) {
+                bv[i] = ov;
+                bi[i] = ok;
+            }
+        }
+    }
+    if (r == 0) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const uint32_t n = n0 + 4 * (uint32_t)g + (uint32_t)i;
+            if (n < N) label[n] = bi[i];
+        }
+    }
+}
+
+// Counting sort of the nodes by label: per-label counts, their exclusive
+// scan, then each workgroup ranks its nodes in LDS and claims one range per
+// label.  The order within a label is whatever the atomics give.
+__global__ void __launch_bounds__(256) nnd_label_hist_kernel(const uint32_t *label, uint32_t N, uint32_t M,
+                                                             uint32_t *counts) {
+    __shared__ uint32_t h[ORDER_MAX_PIVOTS];
+    for (uint32_t i = threadIdx.x; i < M; i += blockDim.x) h[i] = 0;
+    __syncthreads();
+    for (uint32_t n = blockIdx.x * blockDim.x + threadIdx.x; n < N; n += gridDim.x * blockDim.x)
+        atomicAdd(&h[label[n]], 1u);
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < M; i += blockDim.x)
+        if (h[i]) atomicAdd(&counts[i], h[i]);
+}
+
+__global__ void nnd_label_scan_kernel(const uint32_t *counts, uint32_t M, uint32_t *offs) {
+    if (threadIdx.x == 0) {
+        uint32_t acc = 0;
+        for (uint32_t b = 0; b < M; ++b) {
+            offs[b] = acc;
+            acc += counts[b];
+        }
+    }
+}
+
+constexpr uint32_t ORDER_CHUNK = 4096;  // nodes a workgroup of the scatter ranks in LDS
+
+__global__ void __launch_bounds__(256) nnd_label_scatter_kernel(const uint32_t *label, uint32_t N, uint32_t M,
+                                                                uint32_t *offs, uint32_t *order) {
+    __shared__ uint32_t h[ORDER_MAX_PIVOTS], start[ORDER_MAX_PIVOTS];
+    const uint32_t c0 = blockIdx.x * ORDER_CHUNK, c1 = min(c0 + ORDER_CHUNK, N);
+    for (uint32_t i = threadIdx.x; i < M; i += blockDim.x) h[i] = 0;
+    __syncthreads();
+    for (uint32_t n = c0 + threadIdx.x; n < c1; n += blockDim.x) atomicAdd(&h[label[n]], 1u);
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < M; i += blockDim.x) {
+        start[i] = h[i] ? atomicAdd(&offs[i], h[i]) : 0u;
+        h[i] = 0;
+    }
+    __syncthreads();
+    for (uint32_t n = c0 + threadIdx.x; n < c1; n += blockDim.x) {
+        const uint32_t b_ = label[n];
+        order[start[b_] + atomicAdd(&h[b_], 1u)] = n;
     }
 }
 
@@ -1394,8 +1544,9 @@ template <int CPL>
 __global__ void __launch_bounds__(256) nnd_finalize_kernel(NndDev a, uint32_t rp2) {
     extern __shared__ __align__(16) uint8_t smem[];
     const int wid = threadIdx.x >> 6, lane = lane_id();
-    const uint32_t n = blockIdx.x * (blockDim.x >> 6) + wid;
-    if (n >= a.N) return;
+    const uint32_t g = blockIdx.x * (blockDim.x >> 6) + wid;
+    if (g >= a.N) return;
+    const uint32_t n = node_at(a, g);
     uint8_t *my = smem + (size_t)wid * (rp2 * 12 + 16);
     uint64_t *keys = reinterpret_cast<uint64_t *>(my);
     uint32_t *ids = reinterpret_cast<uint32_t *>(keys + rp2);
@@ -1638,6 +1789,32 @@ int nndescent_build(wv_index *ix, const wv_nnd_params *p, const uint32_t *init_i
         else
             hipLaunchKernelGGL(nnd_prep_kernel<DT_FP8>, dim3(pb), dim3(256), 0, s, a, xq);
         NND_HIP(hipGetLastError());
+    }
+    // the processing order (speed only; the graph is the same without it)
+    if (p->order == 2 || (p->order == 0 && N >= ORDER_MIN_N)) {
+        uint32_t M = ORDER_MAX_PIVOTS;
+        while (M > 16 && (size_t)M * 128 > N) M >>= 1;  // at least about 128 nodes a pivot
+        const uint32_t kd = std::min(v.dp, 128u);
+        uint32_t *label = nullptr, *counts = nullptr, *offs = nullptr, *order = nullptr;
+        float *pnorm = nullptr;
+        NND_HIP(B.get(&label, N));
+        NND_HIP(B.get(&order, N));
+        NND_HIP(B.get(&counts, M));
+        NND_HIP(B.get(&offs, M));
+        NND_HIP(B.get(&pnorm, M));
+        hipLaunchKernelGGL(nnd_pivot_norm_kernel, dim3((M + 255) / 256), dim3(256), 0, s, v.base, v.dp, kd, N, M,
+                           pnorm);
+        const uint32_t aw = (N + 15) / 16;  // wavefronts of 16 nodes
+        hipLaunchKernelGGL(nnd_assign_kernel, dim3((aw + 3) / 4), dim3(256), 0, s, v.base, v.dp, kd, N, M, pnorm,
+                           label);
+        hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(256), 0, s, counts, 0u, (size_t)M);
+        hipLaunchKernelGGL(nnd_label_hist_kernel, dim3(std::min<uint32_t>(256, (N + 255) / 256)), dim3(256), 0, s,
+                           label, N, M, counts);
+        hipLaunchKernelGGL(nnd_label_scan_kernel, dim3(1), dim3(64), 0, s, counts, M, offs);
+        hipLaunchKernelGGL(nnd_label_scatter_kernel, dim3((N + ORDER_CHUNK - 1) / ORDER_CHUNK), dim3(256), 0, s, label,
+                           N, M, offs, order);
+        NND_HIP(hipGetLastError());
+        a.order = order;
     }
     uint32_t *d_init = nullptr, *d_icnt = nullptr;
     const uint32_t ist = efanna && init_ids ? init_stride : S;  // ids per initial row
