@@ -11,7 +11,7 @@ Workflow (all GPU work through the C ABI):
    queries reaches the target (the search stops when the network's logit
    exceeds the threshold).
 4. Search with ``Index.search_ext(..., model=m)``: the BF16 inference step runs
-   inside the search loop after every expansion; plain fixed-L search remains
+   inside the search loop after every 4th expansion; plain fixed-L search remains
    available as the fallback and as the reference for the recall loss.
 """
 from __future__ import annotations
