@@ -15,13 +15,15 @@ Default workload "c5" (the configuration the metric is quoted on):
     (the value) and at 0.99.
   * Learned, at each target: the 8 -> 32 -> 1 stop network, trained on the
     GPU on at most 500,000 per-expansion traces of the training queries at
-    pool cap 1.3 x L_plain, threshold calibrated on the calibration queries
-    for target + 0.003, searched at that cap.  Beside it, the ceiling of any
+    pool cap 1.3 x L_plain (at most 4,096), threshold calibrated on the
+    calibration queries for target + min(0.003, (1 - target) / 10), searched
+    at that cap.  Beside it, the ceiling of any
     stop rule: plain search at L_plain stopped at each query's settle hop.
   * NN-Descent leg: the 1M x 128 Zipf mixture (K=64, S=16, 12 rounds) with
     L = 64, R = 128 and 128 candidate slots per node, built in FP32 and in
     BF16 (FP32 rerank); graph recall on 10,000 control nodes against exact
-    FP64 kNN; the join's roofline; graph recall with 256 and 1,024 slots;
+    FP64 kNN; the join's roofline; each dtype's build at the fewest rounds
+    that reach graph recall 0.95; graph recall with 256 and 1,024 slots;
     the 1M x 128 configuration's fixed-L search (L 20-160) on the FP32 graph.
   * Precision leg: the 1M x 960 non-negative mixture (K=64 L=64 S=16 R=64,
     12 rounds) built with FP32, BF16 and FP8 joins (FP32 rerank of each
@@ -201,8 +203,10 @@ def c2_search(ctx, ix, base, q, g, dg):
 
 def nndescent_leg(ctx, Index, datagen):
     """FP32 and BF16 NN-Descent on the 1M x 128 mixture with the leg's
-    settings, with the join's roofline, and the BF16 build's graph recall at
-    larger candidate tables (towards the reference's exact insertion)."""
+    settings, with the join's roofline; each dtype's build at the fewest
+    rounds (from 8) that reach graph recall 0.95; and the BF16 build's graph
+    recall at larger candidate tables (towards the reference's exact
+    insertion)."""
     w = NND_LEG
     base, cq = getattr(datagen, w["gen"])(n_query=10_000)
     N, d = base.shape
@@ -239,6 +243,21 @@ def nndescent_leg(ctx, Index, datagen):
             out["c2_search"] = c2_search(ctx, ix, base, cq, g, dg)
     out["bf16_speedup_vs_f32"] = round(out["f32"]["build_s"] / out["bf16"]["build_s"], 3)
     out["bf16_join_speedup_vs_f32"] = round(out["f32"]["join_ms"] / out["bf16"]["join_ms"], 3)
+    # build seconds at graph recall 0.95: each dtype's fewest rounds reaching it
+    at95 = {}
+    for dt in ("bf16", "f32"):
+        for it in range(8, w["iters"] + 1):
+            st = ix.build_nndescent(w["K"], w["L"], w["S"], w["R"], it, seed=1, dtype=dt,
+                                    cand_slots=w["cand_slots"])
+            g, _, dg = ix.get_graph()
+            gr = graph_recall(g[:ctl], dg[:ctl], gt)
+            if gr >= 0.95 or it == w["iters"]:
+                at95[dt] = dict(iters=it, build_s=round(st["total_ms"] / 1e3, 4), join_ms=round(st["join_ms"], 1),
+                                graph_recall=round(gr, 4), reached_0_95=gr >= 0.95)
+                break
+    out["at_graph_recall_0_95"] = at95
+    out["bf16_speedup_vs_f32_at_0_95"] = round(at95["f32"]["build_s"] / at95["bf16"]["build_s"], 3)
+    log(f"nndescent leg at graph recall 0.95: {at95}")
     slots = {}
     for c in (256, 1024):
         st = ix.build_nndescent(w["K"], w["L"], w["S"], w["R"], w["iters"], seed=1, dtype="bf16", cand_slots=c)
@@ -659,10 +678,12 @@ def learned_leg(ctx, ix, allq, n_work, w, L_plain, target, k, run_learned, timed
     Traces of plain search at the pool cap 1.3 x L_plain on the training
     queries (at most 500,000 per-expansion rows) train the 8 -> 32 -> 1
     network on the GPU; the threshold is calibrated on the calibration
-    queries for target + 0.003; the search is timed at that cap.  Beside it,
+    queries (target below); the search is timed at that cap.  Beside it,
     the ceiling of any stop rule at cap L_plain: a rule that stops every
     query exactly at its settle hop (the last expansion that changed its
-    top-k) returns plain's results after that many expansions."""
+    top-k) returns plain's results after that many expansions.  The
+    calibration target is the recall target plus 0.003, or plus a tenth of
+    1 - target where that is smaller."""
     from weavess_amd import learned
     tq = allq[n_work:n_work + w["n_train"]]
     cq = allq[n_work + w["n_train"]:n_work + w["n_train"] + w["n_cal"]]
@@ -683,7 +704,10 @@ def learned_leg(ctx, ix, allq, n_work, w, L_plain, target, k, run_learned, timed
         sel = np.sort(np.random.default_rng(0).choice(len(X), 500_000, replace=False))
         X, y = X[sel], y[sel]
     model, tst = learned.train(ctx, X, y, epochs=20, batch=4096, seed=1)
-    cal = learned.calibrate(ix, cq, gt_cal, model, cap, k, min(target + 0.003, 0.9999), seed=5,
+    # calibration margin: 0.003, or a tenth of the misses the target allows
+    # when that is smaller (at 0.99 the cap's own recall is about 0.991)
+    cal_target = target + min(0.003, 0.1 * (1.0 - target))
+    cal = learned.calibrate(ix, cq, gt_cal, model, cap, k, cal_target, seed=5,
                             thresholds=np.arange(-4.0, 12.01, 0.25), bisect=True)
     log(f"learned at {target}: {len(X)} traces, training {tst['kernel_ms']:.1f} ms device, loss "
         f"{tst['first_loss']:.4f} -> {tst['last_loss']:.4f}; calibration {cal}; {time.time() - t1:.1f} s")
