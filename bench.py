@@ -309,8 +309,9 @@ def c3_leg(ctx, Index, datagen):
         stored = np.concatenate([dist[n, :deg[n]] for n in sel]).astype(np.float64)
         ok = ex > 0
         srel = np.abs(stored[ok] - ex[ok]) / ex[ok]
+        kd = {"f32": (d + 3) // 4 * 4, "bf16": (d + 31) // 32 * 32, "fp8": (d + 31) // 32 * 32}[dt]
         per[dt] = dict(build_s=round(st["total_ms"] / 1e3, 4), join_ms=round(st["join_ms"], 1),
-                       graph_recall=round(gr, 4),
+                       graph_recall=round(gr, 4), roofline=join_roofline(st, N, d, kd, c["L"], dt),
                        join_distance_rel_err_max=jmax, join_distance_rel_err_mean=jmean,
                        graph_distance_rel_err_max=float(srel.max()),
                        graph_distance_rel_err_mean=float(srel.mean()), error_pairs=npairs)

@@ -21,7 +21,7 @@ def int_rows(n, d, hi, seed):
 
 @pytest.mark.parametrize("d, dtype", [
     (100, "f32"),   # 7 steps of 16, the last one partial (elements 96..99)
-    (200, "f32"),   # 13 steps: longer than 512 bytes, streamed in pieces of 4
+    (200, "f32"),   # 13 steps: longer than 512 bytes, streamed in k pieces of 2
     (136, "bf16"),  # 5 steps of 32 in the 8-step register tile
     (300, "bf16"),  # 10 steps: streamed
 ])
@@ -30,6 +30,23 @@ def test_nndescent_bit_exact_row_lengths(ctx, d, dtype):
     K, L, S, R, it = 16, 24, 8, 24, 5
     gi, gd, gdeg, st = ctx.nndescent_build(base, K, L, S, R, it, seed=4, dtype=dtype, cand_slots=64)
     ri, rd, rdeg, rst = O.nndescent(base, K, L, S, R, it, seed=4, cand_slots=64, with_stats=True)
+    assert st["pairs"] == sum(p for p, _ in rst)
+    assert st["proposals"] == sum(a for _, a in rst)
+    assert np.array_equal(gdeg, rdeg)
+    assert np.array_equal(gi, ri)
+    assert np.array_equal(gd, rd)
+
+
+@pytest.mark.parametrize("d, dtype", [(200, "f32"), (300, "bf16"), (960, "f32"), (960, "bf16")])
+def test_nndescent_bit_exact_long_lists(ctx, d, dtype):
+    """Streamed rows with new lists of up to 56 (four row tiles: two blocks
+    of row tiles) and up to 136 listed rows (nine column tiles: column
+    blocks after the first), including d = 960 as in the precision
+    configuration."""
+    base = int_rows(900, d, 15, seed=d + 1)
+    K, L, S, R, it = 24, 40, 16, 40, 5
+    gi, gd, gdeg, st = ctx.nndescent_build(base, K, L, S, R, it, seed=6, dtype=dtype, cand_slots=64)
+    ri, rd, rdeg, rst = O.nndescent(base, K, L, S, R, it, seed=6, cand_slots=64, with_stats=True)
     assert st["pairs"] == sum(p for p, _ in rst)
     assert st["proposals"] == sum(a for _, a in rst)
     assert np.array_equal(gdeg, rdeg)

@@ -689,52 +689,30 @@ __global__ void __launch_bounds__(256) nnd_rev_select_kernel(NndDev a, uint32_t 
 
 // ------------------------------------------------------------------ join
 // Per element type: bytes per element, elements per MFMA step, the lane's
-// operand vector of one step (KSTEP / 4 elements), and the steps per
-// streamed piece of rows longer than 512 bytes (the longest whose operands
-// stay in registers across a column sweep).
+// operand vector of one step (KSTEP / 4 elements), and the steps per k
+// piece of rows longer than 512 bytes (32 bytes a lane of each of the four
+// tiles a block holds in registers).
 template <int DT> struct JoinT;
 template <> struct JoinT<DT_F32> {
-    static constexpr int ES = 4, KSTEP = 16, PIECE = 4;
+    static constexpr int ES = 4, KSTEP = 16, PIECE = 2;
     using V = uint4;
 };
 template <> struct JoinT<DT_BF16> {
-    static constexpr int ES = 2, KSTEP = 32, PIECE = 4;
+    static constexpr int ES = 2, KSTEP = 32, PIECE = 2;
     using V = uint4;
 };
 template <> struct JoinT<DT_FP8> {
-    static constexpr int ES = 1, KSTEP = 32, PIECE = 8;
+    static constexpr int ES = 1, KSTEP = 32, PIECE = 4;
     using V = uint2;
 };
-
-__device__ inline void vzero(uint4 &v) { v = make_uint4(0, 0, 0, 0); }
-__device__ inline void vzero(uint2 &v) { v = make_uint2(0, 0); }
 
 // Join rows go to the MFMA straight from HBM, without LDS: in the 16x16
 // operand layout lane (r = lane & 15, g = lane >> 4) feeds row r's elements
 // [KSTEP t + KSTEP/4 g, + KSTEP/4) at step t, bytes no other lane reads.
-// ops[j] is lane (r, g)'s vector of step t0 + j (j < TS, t0 + j < T) of row
-// list[r0 + r]; rows at or past `count` and elements at or past kdim read
-// as zero.
-template <int DT, int TS>
-__device__ inline void load_ops(const NndDev &a, const uint32_t *list, int r0, int count, int T, int t0,
-                                typename JoinT<DT>::V ops[TS]) {
-    using V = typename JoinT<DT>::V;
-    constexpr int KSTEP = JoinT<DT>::KSTEP, ES = JoinT<DT>::ES;
-    const int lane = lane_id(), r = lane & 15, g = lane >> 4;
-    const bool live = r0 + r < count;
-    const uint8_t *row = a.xq + (size_t)(live ? list[r0 + r] : 0u) * a.xq_stride;
-#pragma unroll
-    for (int j = 0; j < TS; ++j) {
-        const uint32_t e0 = (uint32_t)(KSTEP * (t0 + j) + (KSTEP / 4) * g);
-        vzero(ops[j]);
-        if (live && t0 + j < T && e0 < a.kdim) ops[j] = *reinterpret_cast<const V *>(row + (size_t)e0 * ES);
-    }
-}
-
-// Branch-free form of load_ops for a whole row tile of TS steps (rows of at
-// most 512 bytes), so that waits on these loads can count them: rows at or
-// past `count` read row 0 and steps at or past T repeat step T - 1; their
-// products reach only outputs no proposal reads or steps no MFMA takes.
+// A whole row tile of TS steps (rows of at most 512 bytes), branch-free so
+// that waits on these loads can count them: rows at or past `count` read
+// row 0 and steps at or past T repeat step T - 1; their products reach only
+// outputs no proposal reads or steps no MFMA takes.
 // FP32 elements at or past kdim (padded widths that are not a multiple of
 // 16) read the row's first vector and are zeroed at the MFMA (ops_mfma_k);
 // BF16 and FP8 rows are padded to a multiple of 32 and have none.
@@ -749,6 +727,27 @@ __device__ inline void load_tile(const NndDev &a, const uint32_t *list, int r0, 
 #pragma unroll
     for (int j = 0; j < TS; ++j) {
         const uint32_t e0 = (uint32_t)(KSTEP * min(j, T - 1) + (KSTEP / 4) * g);
+        ops[j] = *reinterpret_cast<const V *>(row + (size_t)(e0 < a.kdim ? e0 : 0u) * ES);
+    }
+}
+
+// A k piece of a tile for the streamed form: steps t0 .. t0+PS-1 of rows
+// list[r0 .. r0+16), branch-free like load_tile (rows at or past `count`
+// read row 0, steps at or past T repeat step T - 1 and are not multiplied,
+// FP32 elements at or past kdim read the row's first vector and are zeroed
+// at the MFMA).  list is read up to 16 entries past its end (LDS of the
+// same wavefront; unused there).
+template <int DT, int PS>
+__device__ inline void load_piece(const NndDev &a, const uint32_t *list, int r0, int count, int T, int t0,
+                                  typename JoinT<DT>::V ops[PS]) {
+    using V = typename JoinT<DT>::V;
+    constexpr int KSTEP = JoinT<DT>::KSTEP, ES = JoinT<DT>::ES;
+    const int lane = lane_id(), r = lane & 15, g = lane >> 4;
+    const uint32_t li = list[r0 + r];
+    const uint8_t *row = a.xq + (size_t)(r0 + r < count ? li : 0u) * a.xq_stride;
+#pragma unroll
+    for (int j = 0; j < PS; ++j) {
+        const uint32_t e0 = (uint32_t)(KSTEP * min(t0 + j, T - 1) + (KSTEP / 4) * g);
         ops[j] = *reinterpret_cast<const V *>(row + (size_t)(e0 < a.kdim ? e0 : 0u) * ES);
     }
 }
@@ -786,7 +785,7 @@ __device__ inline floatx4 ops_mfma(const typename JoinT<DT>::V A[TS], const type
 }
 
 // ops_mfma for tiles from load_tile: FP32 step T - 1's elements at or past
-// kdim are zeroed on both operands (as load_ops leaves them), only when the
+// kdim are zeroed on both operands (they add nothing), only when the
 // padded width is not a multiple of 16 (a wavefront-uniform test).
 template <int DT, int TS>
 __device__ inline floatx4 ops_mfma_k(const typename JoinT<DT>::V A[TS], const typename JoinT<DT>::V B[TS], int T,
@@ -1123,31 +1122,71 @@ nnd_join_kernel(NndDev a) {
         }
     } else {
         put_records();
-        // longer rows: per tile pair, pieces of PIECE steps streamed through
-        // registers, the next piece loading during the current one's MFMAs
+        // Longer rows, k outermost over blocks of four tiles: slots 0-1 hold
+        // row tiles rt0 and rt0 + 1, slots 2-3 column tiles ct0 and ct0 + 1
+        // (from rt0 + 2 on, two at a time).  Every k piece of the four tiles
+        // is loaded once and meets each pair of them the join needs, so a
+        // listed row is gathered once per block instead of once per tile
+        // pair (for lists of up to four tiles, once).  A pair's steps are
+        // accumulated in step order, as tile by tile.  Tiles past the list
+        // read row 0 (loads stay unconditional); their products are never
+        // used.
         using V = typename JoinT<DT>::V;
         constexpr int PS = JoinT<DT>::PIECE;
         const int T = (int)((kdim + KSTEP - 1) / KSTEP);
-        for (int rt = 0; rt < nrt; ++rt) {
-            for (int ct = rt; ct < nct; ++ct) {
-                floatx4 acc = {0.f, 0.f, 0.f, 0.f};
-                V A[PS], B[PS], An[PS], Bn[PS];
-                load_ops<DT, PS>(a, col, rt * 16, (int)nn, T, 0, A);
-                load_ops<DT, PS>(a, col, ct * 16, (int)nc, T, 0, B);
-                for (int t0 = 0; t0 < T; t0 += PS) {
-                    if (t0 + PS < T) {
-                        load_ops<DT, PS>(a, col, rt * 16, (int)nn, T, t0 + PS, An);
-                        load_ops<DT, PS>(a, col, ct * 16, (int)nc, T, t0 + PS, Bn);
-                    }
-                    acc = ops_mfma<DT, PS>(A, B, min(PS, T - t0), acc);
+        const floatx4 z = {0.f, 0.f, 0.f, 0.f};
+        for (int rt0 = 0; rt0 < nrt; rt0 += 2) {
+            const bool r1 = rt0 + 1 < nrt;   // slot 1 is a row tile
+            const bool c1 = rt0 + 1 < nct;   // slot 1 is a column tile
+            for (int ct0 = rt0 + 2; ct0 == rt0 + 2 || ct0 < nct; ct0 += 2) {
+                const bool first = ct0 == rt0 + 2;  // the block that also pairs slots 0-1
+                const bool c2 = ct0 < nct, c3 = ct0 + 1 < nct;
+                // 0: (0,0)  1: (0,1)  2: (1,1)  3: (0,2)  4: (1,2)  5: (0,3)  6: (1,3)
+                floatx4 acc[7];
 #pragma unroll
-                    for (int j = 0; j < PS; ++j) {
-                        A[j] = An[j];
-                        B[j] = Bn[j];
+                for (int i = 0; i < 7; ++i) acc[i] = z;
+                for (int t0 = 0; t0 < T; t0 += PS) {
+                    V S0[PS], S1[PS], S2[PS], S3[PS];
+                    load_piece<DT, PS>(a, col, rt0 * 16, (int)nc, T, t0, S0);
+                    load_piece<DT, PS>(a, col, (rt0 + 1) * 16, (int)nc, T, t0, S1);
+                    load_piece<DT, PS>(a, col, ct0 * 16, (int)nc, T, t0, S2);
+                    load_piece<DT, PS>(a, col, (ct0 + 1) * 16, (int)nc, T, t0, S3);
+                    const int n = min(PS, T - t0);
+                    // FP32 only: this lane's last step runs past kdim
+                    bool zt = false;
+                    if (DT == DT_F32 && t0 + n == T && (kdim & (KSTEP - 1)))
+                        zt = (uint32_t)(KSTEP * (T - 1) + (KSTEP / 4) * (lane >> 4)) >= kdim;
+                    if (first) {
+                        acc[0] = ops_mfma<DT, PS>(S0, S0, n, acc[0], zt);
+                        if (c1) acc[1] = ops_mfma<DT, PS>(S0, S1, n, acc[1], zt);
+                        if (r1) acc[2] = ops_mfma<DT, PS>(S1, S1, n, acc[2], zt);
+                    }
+                    if (c2) {
+                        acc[3] = ops_mfma<DT, PS>(S0, S2, n, acc[3], zt);
+                        if (r1) acc[4] = ops_mfma<DT, PS>(S1, S2, n, acc[4], zt);
+                    }
+                    if (c3) {
+                        acc[5] = ops_mfma<DT, PS>(S0, S3, n, acc[5], zt);
+                        if (r1) acc[6] = ops_mfma<DT, PS>(S1, S3, n, acc[6], zt);
                     }
                 }
-                tile_proposals<DT>(a, Q, col, cthr, cnorm, cscale, nn, nc, rt, ct, acc, pairs, props);
-                props += jq_drain(a, Q, col);
+                auto prop = [&](int rt, int ct, const floatx4 &x) {
+                    tile_proposals<DT>(a, Q, col, cthr, cnorm, cscale, nn, nc, rt, ct, x, pairs, props);
+                    props += jq_drain(a, Q, col);
+                };
+                if (first) {
+                    prop(rt0, rt0, acc[0]);
+                    if (c1) prop(rt0, rt0 + 1, acc[1]);
+                    if (r1) prop(rt0 + 1, rt0 + 1, acc[2]);
+                }
+                if (c2) {
+                    prop(rt0, ct0, acc[3]);
+                    if (r1) prop(rt0 + 1, ct0, acc[4]);
+                }
+                if (c3) {
+                    prop(rt0, ct0 + 1, acc[5]);
+                    if (r1) prop(rt0 + 1, ct0 + 1, acc[6]);
+                }
             }
         }
     }
