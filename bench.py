@@ -280,11 +280,11 @@ def join_distance_error(base, ids, deg, dtype, sample, seed=0):
 C3_SETTINGS = dict(K=64, L=64, S=16, R=64, iters=12, rerank_mult=2)
 
 
-def c3_leg(ctx, Index, datagen):
+def c3_leg(ctx, Index, datagen, at95_sweep=True):
     """Precision: NN-Descent on 1M x 960 with FP32, BF16 and FP8 joins (FP32
     rerank of each pool's top 2K), graph recall and distance error vs FP64,
-    at the workload's 12 rounds and at the fewest rounds that reach graph
-    recall 0.95 (each dtype's own count)."""
+    at the workload's 12 rounds and (with at95_sweep) at the fewest rounds
+    that reach graph recall 0.95 (each dtype's own count)."""
     t0 = time.time()
     base, _ = datagen.gmm_c3(n_query=1)
     N, d = base.shape
@@ -295,6 +295,13 @@ def c3_leg(ctx, Index, datagen):
     log(f"c3 exact FP64 10-NN of {ctl} control nodes: {gt_ms:.1f} ms")
     c = C3_SETTINGS
     per = {}
+    traffic = {}
+    tp = os.path.join(ROOT, "profiles", "nndescent_c3_join_traffic.json")
+    if os.path.exists(tp):
+        try:
+            traffic = json.load(open(tp)).get("hbm_bytes_per_launch", {})
+        except (ValueError, OSError):
+            traffic = {}
     for dt in ("f32", "bf16", "fp8"):
         st = ix.build_nndescent(c["K"], c["L"], c["S"], c["R"], c["iters"], seed=1, dtype=dt,
                                 rerank_mult=c["rerank_mult"])
@@ -311,7 +318,7 @@ def c3_leg(ctx, Index, datagen):
         srel = np.abs(stored[ok] - ex[ok]) / ex[ok]
         kd = {"f32": (d + 3) // 4 * 4, "bf16": (d + 31) // 32 * 32, "fp8": (d + 31) // 32 * 32}[dt]
         per[dt] = dict(build_s=round(st["total_ms"] / 1e3, 4), join_ms=round(st["join_ms"], 1),
-                       graph_recall=round(gr, 4), roofline=join_roofline(st, N, d, kd, c["L"], dt),
+                       graph_recall=round(gr, 4), roofline=join_roofline(st, N, d, kd, c["L"], dt, traffic.get(dt)),
                        join_distance_rel_err_max=jmax, join_distance_rel_err_mean=jmean,
                        graph_distance_rel_err_max=float(srel.max()),
                        graph_distance_rel_err_mean=float(srel.mean()), error_pairs=npairs)
@@ -319,7 +326,7 @@ def c3_leg(ctx, Index, datagen):
             f"mean {jmean:.3e}; graph distances (FP32 rerank) max {srel.max():.3e}")
     # build seconds at graph recall 0.95: each dtype's fewest rounds reaching it
     at95 = {}
-    for dt in ("f32", "bf16", "fp8"):
+    for dt in ("f32", "bf16", "fp8") if at95_sweep else ():
         for it in range(2, c["iters"] + 1):
             st = ix.build_nndescent(c["K"], c["L"], c["S"], c["R"], it, seed=1, dtype=dt,
                                     rerank_mult=c["rerank_mult"])
@@ -340,13 +347,14 @@ def c3_leg(ctx, Index, datagen):
             "per_dtype": per,
             "bf16_speedup_vs_f32": round(per["f32"]["build_s"] / per["bf16"]["build_s"], 3),
             "fp8_speedup_vs_f32": round(per["f32"]["build_s"] / per["fp8"]["build_s"], 3),
-            "at_graph_recall_0_95": at95,
-            "bf16_speedup_vs_f32_at_0_95": round(at95["f32"]["build_s"] / at95["bf16"]["build_s"], 3)}
+            "at_graph_recall_0_95": at95 or None,
+            "bf16_speedup_vs_f32_at_0_95": round(at95["f32"]["build_s"] / at95["bf16"]["build_s"], 3)
+            if at95 else None}
 
 
 def run_c3(args, ctx, Index, datagen):
     """The precision configuration alone, as its own JSON line."""
-    leg = c3_leg(ctx, Index, datagen)
+    leg = c3_leg(ctx, Index, datagen, at95_sweep=not args.c3_builds_only)
     out = {"metric": "NN-Descent precision: graph recall and distance error vs FP64 per join dtype",
            "value": leg["per_dtype"]["bf16"]["graph_recall"], "unit": "graph recall@10 (BF16 join)", "n_gpus": 1,
            "higher_is_better": True, "data": "synthetic, seeded (gmm_c3), 1000000x960",
@@ -741,6 +749,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the host-core baseline")
     ap.add_argument("--no-nnd-leg", action="store_true", help="skip the 1M x 128 NN-Descent leg")
     ap.add_argument("--no-c3-leg", action="store_true", help="skip the 1M x 960 precision leg")
+    ap.add_argument("--c3-builds-only", action="store_true",
+                    help="--workload c3: the 12-round build of each dtype only (profiling runs)")
     ap.add_argument("--fixed-L", type=int, default=0,
                     help="profiling run: plain search at this L only (no L sweep, learned stop, second "
                          "target, NN-Descent legs or batching leg)")
