@@ -58,6 +58,24 @@ def test_efanna_bit_exact_initial_graph(ctx, small_int, stride, L, S, it):
     assert np.array_equal(gdeg, rdeg) and np.array_equal(gi, ri) and np.array_equal(gd, rd)
 
 
+@pytest.mark.parametrize("d, dtype", [(200, "f32"), (300, "bf16")])
+def test_efanna_bit_exact_streamed_rows(ctx, d, dtype):
+    """Rows longer than 512 bytes (the join's streamed form) with S=35:
+    the first round's new lists hold 70 samples and more (five row tiles).
+    Small integers keep every FP32 and BF16 quantity exact."""
+    rng = np.random.default_rng(d)
+    centres = rng.integers(0, 16, size=(8, d))
+    base = np.clip(centres[rng.integers(0, 8, size=700)] + rng.integers(-2, 3, size=(700, d)), 0, 15)
+    base = base.astype(np.float32)
+    K, L, S, R, it = 24, 64, 35, 32, 4
+    print("efanna streamed rows", d, dtype, flush=True)
+    gi, gd, gdeg, st = _build(ctx, base, K, L, S, R, it, seed=7, dtype=dtype, cand_slots=128)
+    ri, rd, rdeg, rst = O.nndescent_efanna(base, K, L, S, R, it, seed=7, cand_slots=128, with_stats=True)
+    assert st["pairs"] == sum(p for p, _ in rst)
+    assert st["proposals"] == sum(a for _, a in rst)
+    assert np.array_equal(gdeg, rdeg) and np.array_equal(gi, ri) and np.array_equal(gd, rd)
+
+
 @pytest.mark.parametrize("dtype", ["f32", "bf16"])
 def test_efanna_quality(ctx, small_float, dtype):
     base, _ = small_float
