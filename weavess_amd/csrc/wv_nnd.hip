@@ -735,8 +735,10 @@ __device__ inline void load_tile(const NndDev &a, const uint32_t *list, int r0, 
 // list[r0 .. r0+16), branch-free like load_tile (rows at or past `count`
 // read row 0, steps at or past T repeat step T - 1 and are not multiplied,
 // FP32 elements at or past kdim read the row's first vector and are zeroed
-// at the MFMA).  list is read up to 16 entries past its end (LDS of the
-// same wavefront; unused there).
+// at the MFMA).  Tiles past the list read list entries past its end, at
+// most 64 (the block's last tile): LDS of the same wavefront's region,
+// which holds the thresholds, norms and proposal ring after the list, and
+// the values read there are never used.
 template <int DT, int PS>
 __device__ inline void load_piece(const NndDev &a, const uint32_t *list, int r0, int count, int T, int t0,
                                   typename JoinT<DT>::V ops[PS]) {
