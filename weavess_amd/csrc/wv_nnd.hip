@@ -288,10 +288,60 @@ __device__ inline uint32_t wave_gen_random(uint64_t s, uint32_t size, uint32_t N
     return (uint32_t)lane < size ? (uint32_t)(((uint64_t)(pm + lane) + off) % N) : ID_NONE;
 }
 
+// keys_out[r] = (d, ids[r]) for r < m with d node n's distance to ids[r]
+// by the join's formula on the join-precision rows: |a|^2 + |b|^2 - 2 a.b
+// (the FP8 dot times both rows' scales), FP32 products and sums, clamped at
+// 0.  Four lanes a row, 16 rows a pass; lane s of a row takes the row's
+// 16-byte vectors s, s + 4, ...; rows past m read node n's own row and are
+// not stored.
+template <int DT>
+__device__ inline void lp_rows_dist(const NndDev &a, uint32_t n, const uint32_t *ids, int m, uint64_t *keys_out) {
+    const int lane = lane_id(), sub = lane & 3, grp = lane >> 2;
+    const uint32_t nv = a.xq_stride / 16;  // whole vectors: kdim is a multiple of 32
+    const uint4 *qa = reinterpret_cast<const uint4 *>(a.xq + (size_t)n * a.xq_stride);
+    const float na = a.rec[n].norm, sa = a.rec[n].scale;
+    for (int r0 = 0; r0 < m; r0 += 16) {
+        const int row = r0 + grp;
+        const bool live = row < m;
+        const uint32_t id = live ? ids[row] : n;
+        const uint4 *qb = reinterpret_cast<const uint4 *>(a.xq + (size_t)id * a.xq_stride);
+        float acc = 0.f;
+#pragma unroll 4
+        for (uint32_t v = (uint32_t)sub; v < nv; v += 4) {
+            const uint4 x = qa[v], y = qb[v];
+            const uint32_t xs[4] = {x.x, x.y, x.z, x.w}, ys[4] = {y.x, y.y, y.z, y.w};
+#pragma unroll
+            for (int w = 0; w < 4; ++w) {
+                if (DT == DT_BF16) {
+                    acc = acc + __uint_as_float(xs[w] << 16) * __uint_as_float(ys[w] << 16);
+                    acc = acc + __uint_as_float(xs[w] & 0xFFFF0000u) * __uint_as_float(ys[w] & 0xFFFF0000u);
+                } else {
+                    const int xw = (int)xs[w], yw = (int)ys[w];
+                    acc = acc + __builtin_amdgcn_cvt_f32_fp8(xw, 0) * __builtin_amdgcn_cvt_f32_fp8(yw, 0);
+                    acc = acc + __builtin_amdgcn_cvt_f32_fp8(xw, 1) * __builtin_amdgcn_cvt_f32_fp8(yw, 1);
+                    acc = acc + __builtin_amdgcn_cvt_f32_fp8(xw, 2) * __builtin_amdgcn_cvt_f32_fp8(yw, 2);
+                    acc = acc + __builtin_amdgcn_cvt_f32_fp8(xw, 3) * __builtin_amdgcn_cvt_f32_fp8(yw, 3);
+                }
+            }
+        }
+        acc = acc + __shfl_xor(acc, 1);
+        acc = acc + __shfl_xor(acc, 2);
+        if (sub == 0 && live) {
+            float dot = acc;
+            if (DT == DT_FP8) dot = dot * (sa * a.rec[id].scale);
+            const float d = fmaxf((na + a.rec[id].norm) - 2.0f * dot, 0.0f);
+            keys_out[row] = make_key(d, id);
+        }
+    }
+}
+
 // InitRandom: per node, S draws of GenRandom on the node's stream (as in
 // SearchEntryRand, a prefix-max scan forces them strictly increasing), self
-// skipped, build-order FP32 distances, ascending pool, all new, M = S.
-template <int CPL>
+// skipped, distances, ascending pool, all new, M = S.  FP32 builds take
+// build-order FP32 distances (rows of CPL chunks a lane); BF16 and FP8
+// builds the join's formula on their join rows (half or a quarter of the
+// bytes), as every later distance in their pools.
+template <int CPL, int DT>
 __global__ void __launch_bounds__(256) nnd_init_kernel(NndDev a, const uint32_t *init_ids,
                                                        const uint32_t *init_cnt) {
     extern __shared__ __align__(16) uint8_t smem[];
@@ -314,16 +364,21 @@ __global__ void __launch_bounds__(256) nnd_init_kernel(NndDev a, const uint32_t 
     const int m = __popcll(bm);
     if (ok) cid[lanes_below(bm)] = id;
     wave_sync();
-    float4 qv[CPL];
-    {
+    if constexpr (DT == DT_F32) {
+        float4 qv[CPL];
         const float4 *qp = reinterpret_cast<const float4 *>(a.base + (size_t)n * a.dp);
 #pragma unroll
         for (int c = 0; c < CPL; ++c) {
             const int ch = (lane & 15) + 16 * c;
             qv[c] = ch < (int)(a.dp / 4) ? qp[ch] : make_float4(0.f, 0.f, 0.f, 0.f);
         }
+        // long rows gather one row group at a time: four groups of 8-16
+        // chunks a lane would take 180-256 VGPRs and leave one or two
+        // wavefronts a SIMD (the sums are the same either way)
+        rows_dist<CPL, (CPL >= 8 ? 1 : 4)>(a.base, a.dp, a.dp / 4, qv, cid, m, ckey);
+    } else {
+        lp_rows_dist<DT>(a, n, cid, m, ckey);
     }
-    rows_dist<CPL, 4>(a.base, a.dp, a.dp / 4, qv, cid, m, ckey);
     wave_sync();
     uint64_t key = lane < m ? ckey[lane] : KEY_EMPTY;
     key = wave_sort64(key);
@@ -1685,7 +1740,13 @@ hipError_t set_lds(K kernel, size_t bytes) {
 
 template <int CPL>
 hipError_t launch_init(const NndDev &a, const uint32_t *ii, const uint32_t *ic, uint32_t blocks, hipStream_t s) {
-    hipLaunchKernelGGL(nnd_init_kernel<CPL>, dim3(blocks), dim3(64 * WPB), WPB * (256 + 512), s, a, ii, ic);
+    hipLaunchKernelGGL((nnd_init_kernel<CPL, DT_F32>), dim3(blocks), dim3(64 * WPB), WPB * (256 + 512), s, a, ii,
+                       ic);
+    return hipGetLastError();
+}
+template <int DT>
+hipError_t launch_init_lp(const NndDev &a, const uint32_t *ii, const uint32_t *ic, uint32_t blocks, hipStream_t s) {
+    hipLaunchKernelGGL((nnd_init_kernel<1, DT>), dim3(blocks), dim3(64 * WPB), WPB * (256 + 512), s, a, ii, ic);
     return hipGetLastError();
 }
 template <int CPL>
@@ -1875,6 +1936,10 @@ int nndescent_build(wv_index *ix, const wv_nnd_params *p, const uint32_t *init_i
         case 8: NND_HIP(launch_init_rows<8>(a, d_init, d_icnt, ist, blocks, s)); break;
         default: NND_HIP(launch_init_rows<16>(a, d_init, d_icnt, ist, blocks, s)); break;
         }
+    } else if (dt == DT_BF16) {
+        NND_HIP(launch_init_lp<DT_BF16>(a, d_init, d_icnt, blocks, s));
+    } else if (dt == DT_FP8) {
+        NND_HIP(launch_init_lp<DT_FP8>(a, d_init, d_icnt, blocks, s));
     } else {
         switch (cpl) {
         case 1: NND_HIP(launch_init<1>(a, d_init, d_icnt, blocks, s)); break;
