@@ -1459,24 +1459,36 @@ __global__ void __launch_bounds__(256) nnd_merge_kernel(NndDev a, uint32_t up2, 
         pk[r] = i < cnt ? k : KEY_EMPTY;
         pfl[r] = i < cnt ? f : (uint8_t)0;
     }
-    for (uint32_t i = lane; i < hp2; i += WAVE) hk[i] = ID_NONE;
-    wave_sync();
+    // Slot winners are never ids already in the pool: the join probed each
+    // proposal against its target's pool-id hash before the slot update.
+    // Pools of at most 64 with at most 64 winners (the register path below)
+    // therefore need no hash of the old pool; the general path builds one
+    // for the flags it carries over.
+    uint32_t nwin = 0;
 #pragma unroll
-    for (int r = 0; r < MERGE_REGS; ++r) {
-        const uint32_t i = lane + r * WAVE;
-        if (i >= cnt) continue;
-        u[i] = pk[r];
-        uint32_t h = fmix32(key_id(pk[r])) & hmask;
-        while (atomicCAS(&hk[h], ID_NONE, key_id(pk[r])) != ID_NONE) h = (h + 1) & hmask;
-        hf[h] = pfl[r];
+    for (int r = 0; r < MERGE_REGS; ++r)
+        if ((uint32_t)r < cper) nwin += (uint32_t)__popcll(__ballot(ck[r] != KEY_EMPTY));
+    const bool in_regs = a.L <= (uint32_t)WAVE && nwin <= (uint32_t)WAVE;
+    if (!in_regs) {
+        for (uint32_t i = lane; i < hp2; i += WAVE) hk[i] = ID_NONE;
+        wave_sync();
+#pragma unroll
+        for (int r = 0; r < MERGE_REGS; ++r) {
+            const uint32_t i = lane + r * WAVE;
+            if (i >= cnt) continue;
+            u[i] = pk[r];
+            uint32_t h = fmix32(key_id(pk[r])) & hmask;
+            while (atomicCAS(&hk[h], ID_NONE, key_id(pk[r])) != ID_NONE) h = (h + 1) & hmask;
+            hf[h] = pfl[r];
+        }
+        wave_sync();
     }
-    wave_sync();
     // reset every slot for the next join with whole-line stores (resetting
     // only the filled ones leaves scattered partial-line writes)
 #pragma unroll
     for (int r = 0; r < MERGE_REGS; ++r)
         if ((uint32_t)r < cper) cs[lane + r * WAVE] = KEY_EMPTY;
-    // winners not already pooled, compacted behind the pool keys by ballot
+    // winners (not already pooled), compacted behind the pool keys by ballot
     // (slot hashing gives each id one slot, so they are distinct)
     uint32_t nw = 0;
 #pragma unroll
@@ -1488,11 +1500,13 @@ __global__ void __launch_bounds__(256) nnd_merge_kernel(NndDev a, uint32_t up2, 
             const uint32_t id = key_id(k);
             uint32_t h = fmix32(id) & hmask;
             take = true;
-            for (;;) {
-                const uint32_t e = hk[h];
-                if (e == id) { take = false; break; }
-                if (e == ID_NONE) break;
-                h = (h + 1) & hmask;
+            if (!in_regs) {
+                for (;;) {
+                    const uint32_t e = hk[h];
+                    if (e == id) { take = false; break; }
+                    if (e == ID_NONE) break;
+                    h = (h + 1) & hmask;
+                }
             }
         }
         const uint64_t bm = __ballot(take);
