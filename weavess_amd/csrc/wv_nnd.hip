@@ -1709,7 +1709,7 @@ struct Bufs {
     }
 };
 
-constexpr int WPB = 1;       // wavefronts (nodes) per workgroup: one, as in the join
+constexpr int WPB = 1;       // wavefronts (nodes) per workgroup: one, as in the join (four were 1-8 % slower in every stage)
 constexpr int JOIN_WPB = 1;  // one node a workgroup: a finished wavefront frees its slot at once
 
 // The join kernel for this element type: register operands of at most ts
