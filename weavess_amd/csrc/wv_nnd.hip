@@ -468,7 +468,7 @@ __global__ void __launch_bounds__(256) nnd_init_rows_kernel(NndDev a, const uint
         const int m = __popcll(bm);
         if (ok) cid[lanes_below(bm)] = id;
         wave_sync();
-        rows_dist<CPL, 4>(a.base, a.dp, a.dp / 4, qv, cid, m, ckey);
+        rows_dist<CPL, (CPL >= 8 ? 1 : 4)>(a.base, a.dp, a.dp / 4, qv, cid, m, ckey);  // as in nnd_init_kernel
         wave_sync();
         if (lane < m) keys[tot + lane] = ckey[lane];
         tot += (uint32_t)m;
@@ -1673,7 +1673,8 @@ __global__ void __launch_bounds__(256) nnd_finalize_kernel(NndDev a, uint32_t rp
             qv[c] = ch < (int)(a.dp / 4) ? qp[ch] : make_float4(0.f, 0.f, 0.f, 0.f);
         }
     }
-    rows_dist<CPL, 4>(a.base, a.dp, a.dp / 4, qv, ids, (int)rr, keys);
+    // long rows one row group at a time, as in nnd_init_kernel
+    rows_dist<CPL, (CPL >= 8 ? 1 : 4)>(a.base, a.dp, a.dp / 4, qv, ids, (int)rr, keys);
     for (uint32_t i = rr + lane; i < rp2; i += WAVE) keys[i] = KEY_EMPTY;
     wave_sync();
     lds_sort64(keys, (int)rp2);
