@@ -1208,23 +1208,23 @@ nnd_join_kernel(NndDev a) {
                     load_piece<DT, PS>(a, col, (rt0 + 1) * 16, (int)nc, T, t0, S1);
                     load_piece<DT, PS>(a, col, ct0 * 16, (int)nc, T, t0, S2);
                     load_piece<DT, PS>(a, col, (ct0 + 1) * 16, (int)nc, T, t0, S3);
-                    const int n = min(PS, T - t0);
+                    const int ns = min(PS, T - t0);  // steps of this piece
                     // FP32 only: this lane's last step runs past kdim
                     bool zt = false;
-                    if (DT == DT_F32 && t0 + n == T && (kdim & (KSTEP - 1)))
+                    if (DT == DT_F32 && t0 + ns == T && (kdim & (KSTEP - 1)))
                         zt = (uint32_t)(KSTEP * (T - 1) + (KSTEP / 4) * (lane >> 4)) >= kdim;
                     if (first) {
-                        acc[0] = ops_mfma<DT, PS>(S0, S0, n, acc[0], zt);
-                        if (c1) acc[1] = ops_mfma<DT, PS>(S0, S1, n, acc[1], zt);
-                        if (r1) acc[2] = ops_mfma<DT, PS>(S1, S1, n, acc[2], zt);
+                        acc[0] = ops_mfma<DT, PS>(S0, S0, ns, acc[0], zt);
+                        if (c1) acc[1] = ops_mfma<DT, PS>(S0, S1, ns, acc[1], zt);
+                        if (r1) acc[2] = ops_mfma<DT, PS>(S1, S1, ns, acc[2], zt);
                     }
                     if (c2) {
-                        acc[3] = ops_mfma<DT, PS>(S0, S2, n, acc[3], zt);
-                        if (r1) acc[4] = ops_mfma<DT, PS>(S1, S2, n, acc[4], zt);
+                        acc[3] = ops_mfma<DT, PS>(S0, S2, ns, acc[3], zt);
+                        if (r1) acc[4] = ops_mfma<DT, PS>(S1, S2, ns, acc[4], zt);
                     }
                     if (c3) {
-                        acc[5] = ops_mfma<DT, PS>(S0, S3, n, acc[5], zt);
-                        if (r1) acc[6] = ops_mfma<DT, PS>(S1, S3, n, acc[6], zt);
+                        acc[5] = ops_mfma<DT, PS>(S0, S3, ns, acc[5], zt);
+                        if (r1) acc[6] = ops_mfma<DT, PS>(S1, S3, ns, acc[6], zt);
                     }
                 }
                 auto prop = [&](int rt, int ct, const floatx4 &x) {
