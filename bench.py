@@ -201,12 +201,13 @@ def c2_search(ctx, ix, base, q, g, dg):
                         "iters=12), fixed-L greedy search k=10 from seeded random entries", "sweep": rows}
 
 
-def nndescent_leg(ctx, Index, datagen):
+def nndescent_leg(ctx, Index, datagen, extras=True):
     """FP32 and BF16 NN-Descent on the 1M x 128 mixture with the leg's
     settings, with the join's roofline; each dtype's build at the fewest
     rounds (from 8) that reach graph recall 0.95; and the BF16 build's graph
     recall at larger candidate tables (towards the reference's exact
-    insertion)."""
+    insertion).  Without `extras`, the two 12-round builds and the search
+    sweep only (profiling runs)."""
     w = NND_LEG
     base, cq = getattr(datagen, w["gen"])(n_query=10_000)
     N, d = base.shape
@@ -245,7 +246,7 @@ def nndescent_leg(ctx, Index, datagen):
     out["bf16_join_speedup_vs_f32"] = round(out["f32"]["join_ms"] / out["bf16"]["join_ms"], 3)
     # build seconds at graph recall 0.95: each dtype's fewest rounds reaching it
     at95 = {}
-    for dt in ("bf16", "f32"):
+    for dt in ("bf16", "f32") if extras else ():
         for it in range(8, w["iters"] + 1):
             st = ix.build_nndescent(w["K"], w["L"], w["S"], w["R"], it, seed=1, dtype=dt,
                                     cand_slots=w["cand_slots"])
@@ -255,11 +256,11 @@ def nndescent_leg(ctx, Index, datagen):
                 at95[dt] = dict(iters=it, build_s=round(st["total_ms"] / 1e3, 4), join_ms=round(st["join_ms"], 1),
                                 graph_recall=round(gr, 4), reached_0_95=gr >= 0.95)
                 break
-    out["at_graph_recall_0_95"] = at95
-    out["bf16_speedup_vs_f32_at_0_95"] = round(at95["f32"]["build_s"] / at95["bf16"]["build_s"], 3)
+    out["at_graph_recall_0_95"] = at95 or None
+    out["bf16_speedup_vs_f32_at_0_95"] = round(at95["f32"]["build_s"] / at95["bf16"]["build_s"], 3) if at95 else None
     log(f"nndescent leg at graph recall 0.95: {at95}")
     slots = {}
-    for c in (256, 1024):
+    for c in (256, 1024) if extras else ():
         st = ix.build_nndescent(w["K"], w["L"], w["S"], w["R"], w["iters"], seed=1, dtype="bf16", cand_slots=c)
         g, _, dg = ix.get_graph()
         slots[str(c)] = dict(build_s=round(st["total_ms"] / 1e3, 4), graph_recall=round(graph_recall(g[:ctl], dg[:ctl], gt), 4))
@@ -357,7 +358,7 @@ def c3_leg(ctx, Index, datagen, at95_sweep=True):
 
 def run_c3(args, ctx, Index, datagen):
     """The precision configuration alone, as its own JSON line."""
-    leg = c3_leg(ctx, Index, datagen, at95_sweep=not args.c3_builds_only)
+    leg = c3_leg(ctx, Index, datagen, at95_sweep=not args.builds_only)
     out = {"metric": "NN-Descent precision: graph recall and distance error vs FP64 per join dtype",
            "value": leg["per_dtype"]["bf16"]["graph_recall"], "unit": "graph recall@10 (BF16 join)", "n_gpus": 1,
            "higher_is_better": True, "data": "synthetic, seeded (gmm_c3), 1000000x960",
@@ -432,7 +433,7 @@ def run_graphs(args, ctx, Index, datagen):
 
 def run_nnd(args, ctx, Index, datagen):
     """The NN-Descent leg alone (FP32 and BF16 builds of the 1M x 128 mixture)."""
-    leg = nndescent_leg(ctx, Index, datagen)
+    leg = nndescent_leg(ctx, Index, datagen, extras=not args.builds_only)
     out = {"metric": "NN-Descent build s @ 0.95 graph recall", "value": leg["bf16"]["build_s"], "unit": "s (BF16 join)",
            "n_gpus": 1, "higher_is_better": False, "dtype": "bf16", "data": "synthetic, seeded (gmm_c2)",
            "config": {"workload": "nnd: " + NND_LEG["desc"]}, "nndescent": leg}
@@ -752,8 +753,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the host-core baseline")
     ap.add_argument("--no-nnd-leg", action="store_true", help="skip the 1M x 128 NN-Descent leg")
     ap.add_argument("--no-c3-leg", action="store_true", help="skip the 1M x 960 precision leg")
-    ap.add_argument("--c3-builds-only", action="store_true",
-                    help="--workload c3: the 12-round build of each dtype only (profiling runs)")
+    ap.add_argument("--builds-only", action="store_true",
+                    help="--workload c3 / nnd: the 12-round build of each dtype only (profiling runs)")
     ap.add_argument("--fixed-L", type=int, default=0,
                     help="profiling run: plain search at this L only (no L sweep, learned stop, second "
                          "target, NN-Descent legs or batching leg)")
