@@ -83,6 +83,26 @@ __device__ inline void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// v of lane (this lane XOR m).  m = 1, 2, 4, 8 move within a row of 16
+// lanes by DPP (no LDS round trip): quad_perm [1,0,3,2] and [2,3,0,1];
+// row_half_mirror then quad_perm [3,2,1,0] (lane i of 8 takes 7 - (i ^ 3)
+// = i ^ 4); row_ror:8.  Other m go through ds_bpermute.  Every lane must be
+// active (a DPP read of an inactive lane is not its register).
+__device__ inline uint32_t xor_lane(uint32_t v, int m) {
+    const int x = (int)v;
+    if (m == 1) return (uint32_t)__builtin_amdgcn_update_dpp(0, x, 0xB1, 0xF, 0xF, false);
+    if (m == 2) return (uint32_t)__builtin_amdgcn_update_dpp(0, x, 0x4E, 0xF, 0xF, false);
+    if (m == 4)
+        return (uint32_t)__builtin_amdgcn_update_dpp(0, __builtin_amdgcn_update_dpp(0, x, 0x141, 0xF, 0xF, false),
+                                                     0x1B, 0xF, 0xF, false);
+    if (m == 8) return (uint32_t)__builtin_amdgcn_update_dpp(0, x, 0x128, 0xF, 0xF, false);
+    return (uint32_t)__shfl_xor(x, m);
+}
+__device__ inline float xor_lane(float v, int m) { return __uint_as_float(xor_lane(__float_as_uint(v), m)); }
+__device__ inline uint64_t xor_lane64(uint64_t v, int m) {
+    return ((uint64_t)xor_lane((uint32_t)(v >> 32), m) << 32) | xor_lane((uint32_t)v, m);
+}
+
 __device__ inline uint64_t shfl_xor64(uint64_t v, int m) {
     uint32_t lo = (uint32_t)v, hi = (uint32_t)(v >> 32);
     lo = (uint32_t)__shfl_xor((int)lo, m);
@@ -109,14 +129,19 @@ __device__ inline uint32_t lanes_below(uint64_t mask) {
                                      __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
 }
 
-// Ascending bitonic sort of one 64-bit key per lane across the wavefront.
+// Ascending bitonic sort of one 64-bit key per lane across the wavefront
+// (every lane active, as for all wavefront-wide exchanges here).  DPP:
+// exchanges within 16 lanes by DPP (xor_lane) instead of ds_bpermute,
+// which shortens a latency-bound wavefront's chain but costs VALU issue
+// where the SIMD is busy.
+template <bool DPP = false>
 __device__ inline uint64_t wave_sort64(uint64_t key) {
     const int lane = lane_id();
 #pragma unroll
     for (int k = 2; k <= WAVE; k <<= 1) {
 #pragma unroll
         for (int j = k >> 1; j > 0; j >>= 1) {
-            uint64_t o = shfl_xor64(key, j);
+            uint64_t o = DPP ? xor_lane64(key, j) : shfl_xor64(key, j);
             bool up = (lane & k) == 0;
             bool lower = (lane & j) == 0;
             bool keep_min = (lower == up);
@@ -129,11 +154,12 @@ __device__ inline uint64_t wave_sort64(uint64_t key) {
 
 // Ascending bitonic sort of the keys in lanes [0, w), w a power of two in
 // [2, 64] (lanes from w up keep KEY_EMPTY-sorted garbage of their own block).
+template <bool DPP = false>
 __device__ inline uint64_t wave_sort64_n(uint64_t key, int w) {
     const int lane = lane_id();
     for (int k = 2; k <= w; k <<= 1) {
         for (int j = k >> 1; j > 0; j >>= 1) {
-            uint64_t o = shfl_xor64(key, j);
+            uint64_t o = DPP ? xor_lane64(key, j) : shfl_xor64(key, j);
             bool up = (lane & k) == 0;
             bool lower = (lane & j) == 0;
             bool keep_min = (lower == up);
@@ -231,11 +257,12 @@ __device__ inline float chunk_sq(const float4 a, const float4 b) {
 }
 // Fold of the 16 accumulators of a 16-lane group: acc[j] += acc[j+h] for
 // h = 8, 4, 2, 1.  Every lane of the group ends with the same value.
+template <bool DPP = false>
 __device__ inline float fold16(float acc) {
-    acc = acc + __shfl_xor(acc, 8);
-    acc = acc + __shfl_xor(acc, 4);
-    acc = acc + __shfl_xor(acc, 2);
-    acc = acc + __shfl_xor(acc, 1);
+    acc = acc + (DPP ? xor_lane(acc, 8) : __shfl_xor(acc, 8));
+    acc = acc + (DPP ? xor_lane(acc, 4) : __shfl_xor(acc, 4));
+    acc = acc + (DPP ? xor_lane(acc, 2) : __shfl_xor(acc, 2));
+    acc = acc + (DPP ? xor_lane(acc, 1) : __shfl_xor(acc, 1));
     return acc;
 }
 
@@ -263,8 +290,9 @@ __device__ inline void load_query(const float *q, uint32_t nch, float4 (&qv)[CPL
 
 // keys_out[r] = (dist(query, base[ids[r]]), ids[r]) for r < m.  row_lanes
 // lanes per row; G row groups of 64 / row_lanes rows are gathered before
-// any is reduced, so that all their loads are in flight.
-template <int CPL, int G>
+// any is reduced, so that all their loads are in flight.  DPP: the folds
+// exchange by DPP (see wave_sort64); the sums are the same.
+template <int CPL, int G, bool DPP = false>
 __device__ inline void rows_dist(const float *__restrict__ base, uint32_t dp, uint32_t nch,
                                  const float4 (&qv)[CPL], const uint32_t *ids, int m,
                                  uint64_t *keys_out) {
@@ -295,7 +323,7 @@ __device__ inline void rows_dist(const float *__restrict__ base, uint32_t dp, ui
 #pragma unroll
                 for (int c = 0; c < CPL; ++c)
                     if (sub + 16 * c < (int)nch) acc = acc + chunk_sq(qv[c], v[g][c]);
-                acc = fold16(acc);
+                acc = fold16<DPP>(acc);
             } else {
                 float a0 = 0.f, a1 = 0.f;  // accumulators sub and sub + 8
 #pragma unroll
@@ -306,9 +334,9 @@ __device__ inline void rows_dist(const float *__restrict__ base, uint32_t dp, ui
                     }
                 }
                 acc = a0 + a1;
-                acc = acc + __shfl_xor(acc, 4);
-                acc = acc + __shfl_xor(acc, 2);
-                acc = acc + __shfl_xor(acc, 1);
+                acc = acc + (DPP ? xor_lane(acc, 4) : __shfl_xor(acc, 4));
+                acc = acc + (DPP ? xor_lane(acc, 2) : __shfl_xor(acc, 2));
+                acc = acc + (DPP ? xor_lane(acc, 1) : __shfl_xor(acc, 1));
             }
             if (sub == 0 && row < m) keys_out[row] = make_key(acc, ids[row]);
         }

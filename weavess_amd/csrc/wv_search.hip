@@ -150,7 +150,7 @@ __device__ inline uint64_t sort_survivors(uint64_t key, uint64_t lm, int c, uint
     wave_sync();
     int w = 2;
     while (w < c) w <<= 1;
-    key = wave_sort64_n(lane < c ? blk[lane] : KEY_EMPTY, w);
+    key = wave_sort64_n<true>(lane < c ? blk[lane] : KEY_EMPTY, w);
     wave_sync();
     if (lane < c) blk[lane] = key;
     wave_sync();
@@ -205,10 +205,10 @@ __device__ inline void search_query(const SearchArgs &a, uint8_t *my, const uint
             const int cnt = (int)min((uint32_t)WAVE, L - c0);
             if (lane < cnt) cid[lane] = (uint32_t)pool[c0 + lane];
             wave_sync();
-            rows_dist<CPL, G>(a.base, dp, nch, qv, cid, cnt, ckey);
+            rows_dist<CPL, G, true>(a.base, dp, nch, qv, cid, cnt, ckey);
             wave_sync();
             if (COLLECT) collect_stage(full, flen, (int)a.C, ckey, cnt, fstg, fsn, fblk);
-            const uint64_t key = wave_sort64(lane < cnt ? ckey[lane] : KEY_EMPTY);
+            const uint64_t key = wave_sort64<true>(lane < cnt ? ckey[lane] : KEY_EMPTY);
             wave_sync();
             ckey[lane] = key;
             wave_sync();
@@ -287,7 +287,7 @@ __device__ inline void search_query(const SearchArgs &a, uint8_t *my, const uint
                 int c = 0;
                 uint64_t key = KEY_EMPTY;
                 if (m > 0) {
-                    rows_dist<CPL, G>(a.base, dp, nch, qv, cid, m, ckey);
+                    rows_dist<CPL, G, true>(a.base, dp, nch, qv, cid, m, ckey);
                     evals += (uint32_t)m;
                     if (evals - L > a.H) probes = VIS_PROBES_FULL;
                     wave_sync();
@@ -414,7 +414,7 @@ __device__ inline void search_query(const SearchArgs &a, uint8_t *my, const uint
                 if (fresh) cid[lanes_below(bm)] = id;
                 wave_sync();
                 if (m == 0) continue;
-                rows_dist<CPL, G>(a.base, dp, nch, qv, cid, m, ckey);
+                rows_dist<CPL, G, true>(a.base, dp, nch, qv, cid, m, ckey);
                 evals += (uint32_t)m;
                 // more ids inserted than the table has slots: it forgets either
                 // way, so stop probing deeply
@@ -586,9 +586,9 @@ __global__ void __launch_bounds__(256) greedy_search_coop_kernel(SearchArgs a) {
             const int cnt = (int)min((uint32_t)WAVE, L - c0);
             if (lane < cnt) cid[lane] = (uint32_t)pool[c0 + lane];
             wave_sync();
-            rows_dist<CPL, G>(a.base, dp, nch, qv, cid, cnt, ckey);
+            rows_dist<CPL, G, true>(a.base, dp, nch, qv, cid, cnt, ckey);
             wave_sync();
-            const uint64_t key = wave_sort64(lane < cnt ? ckey[lane] : KEY_EMPTY);
+            const uint64_t key = wave_sort64<true>(lane < cnt ? ckey[lane] : KEY_EMPTY);
             wave_sync();
             ckey[lane] = key;
             wave_sync();
@@ -655,7 +655,7 @@ __global__ void __launch_bounds__(256) greedy_search_coop_kernel(SearchArgs a) {
             const int m = __popcll(bm);
             if (fresh) cid[lanes_below(bm)] = id;
             wave_sync();
-            if (m > 0) rows_dist<CPL, G>(a.base, dp, nch, qv, cid, m, ckey);
+            if (m > 0) rows_dist<CPL, G, true>(a.base, dp, nch, qv, cid, m, ckey);
             if (lane == 0) ctl[3 + wid] = (uint32_t)m;
             __syncthreads();
             if (wid == 0) {
