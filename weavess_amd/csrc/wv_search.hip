@@ -85,6 +85,37 @@ __device__ inline int first_flag(const uint64_t *a, int from, int n, uint64_t &k
     return n;
 }
 
+// first_flag over a[fa..na) and b[fb..nb) together: the first 64-slot chunk
+// of each is read before either is tested, so the two LDS reads travel
+// together; a search that does not end in its first chunk goes on as
+// first_flag.  nb may be 0 (then ib = 0, kb = KEY_EMPTY).
+__device__ inline void first_flag2(const uint64_t *a, int fa, int na, int &ia, uint64_t &ka, const uint64_t *b,
+                                   int fb, int nb, int &ib, uint64_t &kb) {
+    const int lane = lane_id();
+    const int ta = fa & ~(WAVE - 1), tb = fb & ~(WAVE - 1);
+    const int pa = ta + lane, pb = tb + lane;
+    const uint64_t ea = pa >= fa && pa < na ? a[pa] : 0;
+    const uint64_t eb = pb >= fb && pb < nb ? b[pb] : 0;
+    const uint64_t ma = __ballot((ea & FBIT) != 0), mb = __ballot((eb & FBIT) != 0);
+    if (ma) {
+        const int i = __ffsll((unsigned long long)ma) - 1;
+        ka = canon(readlane64(ea, i));
+        ia = ta + i;
+    } else {
+        ia = first_flag(a, ta + WAVE, na, ka);
+    }
+    if (mb) {
+        const int i = __ffsll((unsigned long long)mb) - 1;
+        kb = canon(readlane64(eb, i));
+        ib = tb + i;
+    } else if (nb == 0) {
+        kb = KEY_EMPTY;
+        ib = 0;
+    } else {
+        ib = first_flag(b, tb + WAVE, nb, kb);
+    }
+}
+
 // The k-th smallest (0-based) key of the union of the sorted, disjoint entry
 // arrays a[0..na) and b[0..nb), k < na + nb: the split that takes i keys of b
 // and k+1-i of a, with i the largest value for which b[i-1] < a[k+1-i].
@@ -298,9 +329,9 @@ __device__ inline void search_query(const SearchArgs &a, uint8_t *my, const uint
                 }
                 bool next_new = false;  // the next node is the smallest survivor
                 if (bi + 1 == nblk) {
-                    uint64_t ka, kbk = KEY_EMPTY;
-                    const int kk = first_flag(pool, kp, Li, ka);
-                    const int jb = nb ? first_flag(buf, kb, nb, kbk) : 0;
+                    uint64_t ka, kbk;
+                    int kk, jb;
+                    first_flag2(pool, kp, Li, kk, ka, buf, kb, nb, jb, kbk);
                     const bool from_buf = kbk < ka;
                     const uint64_t x = from_buf ? kbk : ka;
                     const uint64_t y = c > 0 ? readlane64(key, 0) : KEY_EMPTY;
