@@ -151,12 +151,16 @@ __device__ inline int buffer_insert(uint64_t *buf, int nb, uint64_t skey, int c,
                                     uint64_t &worst) {
     const int lane = lane_id();
     constexpr int R = 4;  // entries a lane holds: B <= 256
-    uint64_t b[R];
+    const int nn = nb + c, hi = nn < Li ? nn : Li;
+    uint64_t b[R], pt[R];
     int np[R];
+    // the pool's last entries (read first: the merge below does not change
+    // the pool, and their LDS reads then travel with the buffer's)
 #pragma unroll
     for (int r = 0; r < R; ++r) {
         const int j = lane + r * WAVE;
         b[r] = j < nb ? buf[j] : KEY_EMPTY;
+        pt[r] = j < Li && r * WAVE <= hi ? canon(pool[Li - 1 - j]) : 0;  // entry taken (<= hi) may be read
         np[r] = j;
     }
     int spos = 0;
@@ -179,14 +183,12 @@ __device__ inline int buffer_insert(uint64_t *buf, int nb, uint64_t skey, int c,
         if (lane + r * WAVE < nb) buf[np[r]] = b[r];
     if (lane < c) buf[spos] = skey | FBIT;
     wave_sync();
-    const int nn = nb + c, hi = nn < Li ? nn : Li;
-    uint64_t bn[R], pt[R];
+    uint64_t bn[R];
     int taken = 0;
 #pragma unroll
     for (int r = 0; r < R; ++r) {
         const int j = lane + r * WAVE;
         bn[r] = j < nn ? canon(buf[j]) : KEY_EMPTY;
-        pt[r] = j < Li ? canon(pool[Li - 1 - j]) : 0;
     }
 #pragma unroll
     for (int r = 0; r < R; ++r) taken += (int)__popcll(__ballot(lane + r * WAVE < hi && bn[r] < pt[r]));
