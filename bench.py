@@ -1054,7 +1054,7 @@ def main():
     if os.path.exists(tp):
         try:
             tj = json.load(open(tp))
-            if tj.get("method") == method:
+            if tj.get("method") == method and tj.get("L") == L_plain:  # measured on this launch's shape
                 traffic = tj.get("hbm_bytes_per_launch")
         except (ValueError, OSError):
             traffic = None
