@@ -353,6 +353,48 @@ def test_search_waves_per_query_parity(ctx, small_float, small_float_graph, wpq,
         assert np.array_equal(got2[i], ref2[i]), i
 
 
+@pytest.mark.parametrize("d,tiles", [(16, (8, 16)), (128, (4, 8, 16)), (96, (4, 8, 16)), (200, (4, 8)),
+                                     (512, (8,)), (960, (8,))])
+def test_search_look_ahead_loop(ctx, d, tiles):
+    """Launches whose LDS allows at most 2 wavefronts a SIMD (a 16,384-slot
+    table, or a pool of about 2,000 keys) on graphs of at most 64 slots a
+    node take the loop that tests the next node's neighbours and issues
+    their rows before the current node's merge: ids, distances, evaluations
+    and hops equal the oracle's for every gather tile, with and without the
+    merge buffer; a degree-80 graph under the same LDS keeps the loop without
+    look-ahead and equals the oracle too."""
+    from weavess_amd import Index
+    rng = np.random.default_rng(d)
+    base = rng.normal(size=(2500, d)).astype(np.float32)
+    q = rng.normal(size=(40, d)).astype(np.float32)
+    g, _, deg = O.nndescent(base, 24, 32, 8, 32, 4, seed=1)
+    ix = Index(ctx, base)
+    ix.set_graph(g, deg)
+    for L, vs in ((200, 16384), (1990, 0)):
+        ref = O.search_greedy(g, deg, base, q, L, 10, seed=6)
+        for rg in tiles:
+            for mb in (1, 64):
+                ix.set_search_config(row_groups=rg, visited_slots=vs, merge_buffer=mb)
+                got = ix.search(q, L, 10, seed=6)
+                for i in (0, 1, 3):
+                    assert np.array_equal(got[i], ref[i]), (L, rg, mb, i)
+                if vs:  # a table larger than the base forgets nothing
+                    assert np.array_equal(got[2], ref[2]), (L, rg, mb)
+                else:
+                    assert np.all(got[2] >= ref[2])
+    ix.close()
+    g80, _ = O.exact_knn(base, base, 80, self_offset=0)
+    deg80 = np.full(len(base), 80, np.uint32)
+    ref = O.search_greedy(g80, deg80, base, q[:10], 200, 10, seed=6)
+    ix = Index(ctx, base)
+    ix.set_graph(g80, deg80)
+    ix.set_search_config(visited_slots=16384)
+    got = ix.search(q[:10], 200, 10, seed=6)
+    ix.close()
+    for i in range(4):
+        assert np.array_equal(got[i], ref[i]), i
+
+
 @pytest.mark.parametrize("wpq", [2, 4])
 def test_search_waves_per_query_wide_rows_and_degrees(ctx, wpq):
     """d = 960 rows (16 chunks a lane) and a degree-128 graph (more than one

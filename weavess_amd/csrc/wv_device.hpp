@@ -10,6 +10,7 @@
 namespace wv {
 
 constexpr int WAVE = 64;
+constexpr size_t LDS_PER_CU = 160 * 1024;  // gfx950: LDS bytes a compute unit holds
 constexpr uint64_t KEY_EMPTY = ~0ull;
 constexpr uint32_t ID_NONE = 0xFFFFFFFFu;
 constexpr uint64_t GOLDEN = 0x9E3779B97F4A7C15ull;
@@ -288,6 +289,77 @@ __device__ inline void load_query(const float *q, uint32_t nch, float4 (&qv)[CPL
     }
 }
 
+// rows_dist in two halves: rows_issue gathers the first G row groups (the
+// rows a wavefront holds at once) into v; rows_finish reduces them and then
+// gathers and reduces any further rows.  Between the two a caller can do
+// other work while the loads travel.
+template <int CPL, int G>
+__device__ inline void rows_issue(const float *__restrict__ base, uint32_t dp, uint32_t nch, const uint32_t *ids,
+                                  int m, int r0, float4 (&v)[G][CPL]) {
+    constexpr int LPR = row_lanes<CPL>(), RPG = WAVE / LPR;
+    const int lane = lane_id(), sub = lane % LPR, grp = lane / LPR;
+    // groups wholly past m (the same for the whole wavefront) are skipped
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+        if (r0 + RPG * g >= m) break;
+        const int row = r0 + RPG * g + grp;
+        const uint32_t id = row < m ? ids[row] : 0u;
+        const float4 *p = reinterpret_cast<const float4 *>(base + (size_t)id * dp);
+#pragma unroll
+        for (int c = 0; c < CPL; ++c) {
+            const int ch = sub + LPR * c;
+            v[g][c] = (row < m && ch < (int)nch) ? p[ch] : make_float4(0.f, 0.f, 0.f, 0.f);
+        }
+    }
+}
+
+template <int CPL, int G, bool DPP = false>
+__device__ inline void rows_reduce(uint32_t nch, const float4 (&qv)[CPL], const uint32_t *ids, int m, int r0,
+                                   const float4 (&v)[G][CPL], uint64_t *keys_out) {
+    constexpr int LPR = row_lanes<CPL>(), RPG = WAVE / LPR;
+    const int lane = lane_id(), sub = lane % LPR, grp = lane / LPR;
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+        if (r0 + RPG * g >= m) break;
+        const int row = r0 + RPG * g + grp;
+        float acc;
+        if constexpr (LPR == 16) {
+            acc = 0.f;
+#pragma unroll
+            for (int c = 0; c < CPL; ++c)
+                if (sub + 16 * c < (int)nch) acc = acc + chunk_sq(qv[c], v[g][c]);
+            acc = fold16<DPP>(acc);
+        } else {
+            float a0 = 0.f, a1 = 0.f;  // accumulators sub and sub + 8
+#pragma unroll
+            for (int c = 0; c < CPL; ++c) {
+                if (sub + 8 * c < (int)nch) {
+                    if (c & 1) a1 = a1 + chunk_sq(qv[c], v[g][c]);
+                    else a0 = a0 + chunk_sq(qv[c], v[g][c]);
+                }
+            }
+            acc = a0 + a1;
+            acc = acc + (DPP ? xor_lane(acc, 4) : __shfl_xor(acc, 4));
+            acc = acc + (DPP ? xor_lane(acc, 2) : __shfl_xor(acc, 2));
+            acc = acc + (DPP ? xor_lane(acc, 1) : __shfl_xor(acc, 1));
+        }
+        if (sub == 0 && row < m) keys_out[row] = make_key(acc, ids[row]);
+    }
+}
+
+template <int CPL, int G, bool DPP = false>
+__device__ inline void rows_finish(const float *__restrict__ base, uint32_t dp, uint32_t nch,
+                                   const float4 (&qv)[CPL], const uint32_t *ids, int m, const float4 (&v0)[G][CPL],
+                                   uint64_t *keys_out) {
+    constexpr int RPG = WAVE / row_lanes<CPL>();
+    rows_reduce<CPL, G, DPP>(nch, qv, ids, m, 0, v0, keys_out);
+    for (int r0 = RPG * G; r0 < m; r0 += RPG * G) {
+        float4 v[G][CPL];
+        rows_issue<CPL, G>(base, dp, nch, ids, m, r0, v);
+        rows_reduce<CPL, G, DPP>(nch, qv, ids, m, r0, v, keys_out);
+    }
+}
+
 // keys_out[r] = (dist(query, base[ids[r]]), ids[r]) for r < m.  row_lanes
 // lanes per row; G row groups of 64 / row_lanes rows are gathered before
 // any is reduced, so that all their loads are in flight.  DPP: the folds
@@ -296,50 +368,11 @@ template <int CPL, int G, bool DPP = false>
 __device__ inline void rows_dist(const float *__restrict__ base, uint32_t dp, uint32_t nch,
                                  const float4 (&qv)[CPL], const uint32_t *ids, int m,
                                  uint64_t *keys_out) {
-    constexpr int LPR = row_lanes<CPL>(), RPG = WAVE / LPR;
-    const int lane = lane_id(), sub = lane % LPR, grp = lane / LPR;
+    constexpr int RPG = WAVE / row_lanes<CPL>();
     for (int r0 = 0; r0 < m; r0 += RPG * G) {
         float4 v[G][CPL];
-        // groups wholly past m (the same for the whole wavefront) are skipped
-#pragma unroll
-        for (int g = 0; g < G; ++g) {
-            if (r0 + RPG * g >= m) break;
-            const int row = r0 + RPG * g + grp;
-            const uint32_t id = row < m ? ids[row] : 0u;
-            const float4 *p = reinterpret_cast<const float4 *>(base + (size_t)id * dp);
-#pragma unroll
-            for (int c = 0; c < CPL; ++c) {
-                const int ch = sub + LPR * c;
-                v[g][c] = (row < m && ch < (int)nch) ? p[ch] : make_float4(0.f, 0.f, 0.f, 0.f);
-            }
-        }
-#pragma unroll
-        for (int g = 0; g < G; ++g) {
-            if (r0 + RPG * g >= m) break;
-            const int row = r0 + RPG * g + grp;
-            float acc;
-            if constexpr (LPR == 16) {
-                acc = 0.f;
-#pragma unroll
-                for (int c = 0; c < CPL; ++c)
-                    if (sub + 16 * c < (int)nch) acc = acc + chunk_sq(qv[c], v[g][c]);
-                acc = fold16<DPP>(acc);
-            } else {
-                float a0 = 0.f, a1 = 0.f;  // accumulators sub and sub + 8
-#pragma unroll
-                for (int c = 0; c < CPL; ++c) {
-                    if (sub + 8 * c < (int)nch) {
-                        if (c & 1) a1 = a1 + chunk_sq(qv[c], v[g][c]);
-                        else a0 = a0 + chunk_sq(qv[c], v[g][c]);
-                    }
-                }
-                acc = a0 + a1;
-                acc = acc + (DPP ? xor_lane(acc, 4) : __shfl_xor(acc, 4));
-                acc = acc + (DPP ? xor_lane(acc, 2) : __shfl_xor(acc, 2));
-                acc = acc + (DPP ? xor_lane(acc, 1) : __shfl_xor(acc, 1));
-            }
-            if (sub == 0 && row < m) keys_out[row] = make_key(acc, ids[row]);
-        }
+        rows_issue<CPL, G>(base, dp, nch, ids, m, r0, v);
+        rows_reduce<CPL, G, DPP>(nch, qv, ids, m, r0, v, keys_out);
     }
 }
 

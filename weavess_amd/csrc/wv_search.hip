@@ -193,7 +193,7 @@ __device__ inline uint64_t sort_survivors(uint64_t key, uint64_t lm, int c, uint
 // use the variant without that code.
 // One query, by one wavefront, in its LDS region `my` (every slot it reads
 // is written first, so a region can serve query after query).
-template <int CPL, int G, bool COLLECT, bool WATCH>
+template <int CPL, int G, bool COLLECT, bool WATCH, bool LOOK = false>
 __device__ inline void search_query(const SearchArgs &a, uint8_t *my, const uint32_t q) {
     const int lane = lane_id();
     uint64_t *pool = reinterpret_cast<uint64_t *>(my);         // P keys (P >= L, multiple of 64)
@@ -272,7 +272,107 @@ __device__ inline void search_query(const SearchArgs &a, uint8_t *my, const uint
     float kprev = r0;  // k-th distance at the last stop-rule evaluation (the entries' at first)
     uint32_t settle = 0;  // last expansion that changed the top-k
     int probes = VIS_PROBES;
-    if constexpr (!COLLECT) {
+    if constexpr (LOOK) {
+        // The plain loop for graphs of at most 64 neighbour slots a node (one
+        // block a node), one step further ahead: once the next node is
+        // chosen, its neighbours are tested in the visited table and their
+        // rows' loads issued before this node's survivors are merged, so the
+        // gathers travel during the merge.  The visited table and the merge
+        // touch disjoint state and each keeps its order of operations, so
+        // expansions, evaluations and results are the loop's below.
+        static_assert(!COLLECT && !WATCH, "the look-ahead loop is the plain search's");
+        uint32_t id0 = ID_NONE, du = 0;
+        auto start = [&](uint32_t node) {
+            id0 = (uint32_t)lane < a.deg_max ? a.graph[(size_t)node * a.deg_max + lane] : ID_NONE;
+            du = a.deg[node];
+        };
+        int m = 0;
+        float4 v[G][CPL];
+        auto mark = [&]() {  // step 1 of the chosen node, then its first rows' loads
+            const uint32_t d = du < a.deg_max ? du : a.deg_max;
+            const uint32_t id = (uint32_t)lane < d ? id0 : ID_NONE;
+            const bool fresh = id < N && visit_first(vis, hmask, id, probes) != 0;
+            const uint64_t bm = __ballot(fresh);
+            m = __popcll(bm);
+            if (fresh) cid[lanes_below(bm)] = id;
+            wave_sync();
+            if (m > 0) rows_issue<CPL, G>(a.base, dp, nch, cid, m, 0, v);
+        };
+        bool live;
+        {
+            uint64_t cur;
+            const int kk = first_flag(pool, kp, Li, cur);  // the buffer is empty
+            live = !(cur == KEY_EMPTY || cur > worst);
+            if (live) {
+                wave_sync();
+                if (lane == 0) pool[kk] = cur;  // expanded: stored without bit 31
+                kp = kk + 1;
+                start(key_id(cur));
+                mark();
+            }
+        }
+        while (live) {
+            ++hops;
+            int c = 0;
+            uint64_t key = KEY_EMPTY;
+            if (m > 0) {
+                rows_finish<CPL, G, true>(a.base, dp, nch, qv, cid, m, v, ckey);
+                evals += (uint32_t)m;
+                if (evals - L > a.H) probes = VIS_PROBES_FULL;
+                wave_sync();
+                key = survivor(lane < m ? ckey[lane] : KEY_EMPTY, pool, Li, buf, nb, worst);
+                const uint64_t lm = __ballot(key != KEY_EMPTY);
+                c = __popcll(lm);
+                if (c > 0) key = sort_survivors(key, lm, c, ckey);
+            }
+            bool next_new;
+            {
+                uint64_t ka, kbk;
+                int kk, jb;
+                first_flag2(pool, kp, Li, kk, ka, buf, kb, nb, jb, kbk);
+                const bool from_buf = kbk < ka;
+                const uint64_t x = from_buf ? kbk : ka;
+                const uint64_t y = c > 0 ? readlane64(key, 0) : KEY_EMPTY;
+                next_new = y < x;
+                const uint64_t cur = next_new ? y : x;
+                live = !(cur == KEY_EMPTY || cur > worst);
+                if (live) {
+                    if (!next_new) {
+                        wave_sync();
+                        if (lane == 0) {
+                            if (from_buf) buf[jb] = cur;
+                            else pool[kk] = cur;
+                        }
+                        if (from_buf) kb = jb + 1;
+                        else kp = kk + 1;
+                    }
+                    start(key_id(cur));
+                    mark();
+                }
+            }
+            if (c == 0) continue;
+            if (B == 0) {
+                const int pos0 = merge_into(pool, Li, Li, ckey, c, true);
+                if (pos0 < kp) kp = pos0;
+                if (pos0 <= kidx) settle = hops;
+                if (next_new && lane == 0) pool[pos0] = ckey[0];  // pos0 < L: ckey[0] < worst
+                worst = canon(pool[Li - 1]);
+            } else {
+                if (nb + c > B) {
+                    const int pos0 = merge_into(pool, Li, Li, buf, nb, false);
+                    if (pos0 < kp) kp = pos0;
+                    nb = 0;
+                    kb = 0;
+                }
+                const int bp = buffer_insert(buf, nb, key, c, pool, Li, worst);
+                nb += c;
+                if (bp < kb) kb = bp;
+                if (next_new && lane == 0) buf[bp] = ckey[0];
+                if (track_settle && bp + lower_bound_e(pool, Li, ckey[0]) <= kidx) settle = hops;
+            }
+            wave_sync();
+        }
+    } else if constexpr (!COLLECT) {
         // The loop below with the next node chosen before the merge: once
         // the last block of u's survivors is sorted, the next node is the
         // smaller of U's first unexpanded key and the smallest survivor (both
@@ -500,8 +600,9 @@ __device__ inline void search_query(const SearchArgs &a, uint8_t *my, const uint
 }
 
 // Wavefronts per SIMD the plain grid kernel is compiled for: 3 (at most 168
-// VGPRs, no spills) for the 64-row gather tile of rows of 17-24 chunks,
-// which otherwise takes a few more and drops to 2 where LDS would allow 3.
+// VGPRs, one or two values spilled) for the 64-row gather tile of rows of
+// 17-24 chunks, which otherwise takes a few more and drops to 2 where LDS
+// would allow 3.  The look-ahead loop is compiled for 2.
 // (The persistent form keeps more state live and would spill under it.)
 template <int CPL, int G, bool EXTRA> struct SearchWaves {
     static constexpr int value = EXTRA ? 1
@@ -510,14 +611,14 @@ template <int CPL, int G, bool EXTRA> struct SearchWaves {
 };
 
 // One wavefront per query: query = its global wavefront index.
-template <int CPL, int G, bool COLLECT, bool WATCH>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SearchWaves<CPL, G, COLLECT || WATCH>::value)))
+template <int CPL, int G, bool COLLECT, bool WATCH, bool LOOK = false>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(LOOK ? 2 : SearchWaves<CPL, G, COLLECT || WATCH>::value)))
 greedy_search_kernel(SearchArgs a) {
     extern __shared__ __align__(16) uint8_t smem[];
     const int wid = (int)(threadIdx.x >> 6);
     const uint32_t q = blockIdx.x * (blockDim.x >> 6) + wid;
     if (q >= a.Q) return;
-    search_query<CPL, G, COLLECT, WATCH>(a, smem + (size_t)wid * a.wave_lds, q);
+    search_query<CPL, G, COLLECT, WATCH, LOOK>(a, smem + (size_t)wid * a.wave_lds, q);
 }
 
 // Persistent form: a grid of about as many wavefronts as the GPU holds at
@@ -544,18 +645,18 @@ __global__ void __launch_bounds__(256) greedy_search_persistent_kernel(SearchArg
     }
 }
 
-template <int CPL, int G, bool COLLECT, bool WATCH>
+template <int CPL, int G, bool COLLECT, bool WATCH, bool LOOK = false>
 hipError_t launch_w(const SearchArgs &a, int waves_per_block, hipStream_t s) {
     const size_t lds = (size_t)a.wave_lds * waves_per_block;
     const uint32_t blocks = (a.Q + waves_per_block - 1) / waves_per_block;
     if (lds > 65536) {
         hipError_t e = hipFuncSetAttribute(
-            reinterpret_cast<const void *>(&greedy_search_kernel<CPL, G, COLLECT, WATCH>),
+            reinterpret_cast<const void *>(&greedy_search_kernel<CPL, G, COLLECT, WATCH, LOOK>),
             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (e != hipSuccess) return e;
     }
-    hipLaunchKernelGGL((greedy_search_kernel<CPL, G, COLLECT, WATCH>), dim3(blocks), dim3(WAVE * waves_per_block),
-                       lds, s, a);
+    hipLaunchKernelGGL((greedy_search_kernel<CPL, G, COLLECT, WATCH, LOOK>), dim3(blocks),
+                       dim3(WAVE * waves_per_block), lds, s, a);
     return hipGetLastError();
 }
 // ---------------------------------------------------------------- cooperative
@@ -802,6 +903,18 @@ hipError_t launch_t(const SearchArgs &a, int waves_per_block, uint32_t cu_count,
                      : launch_p<CPL, G, false>(a, waves_per_block, cu_count, s);
     }
     if (!COLLECT && watch) return launch_w<CPL, G, false, true>(a, waves_per_block, s);
+    if constexpr (!COLLECT && !(CPL == 4 && G == 8)) {
+        // The look-ahead loop holds the next node's rows in registers through
+        // the merge (123-248 VGPRs by tile, 200 for rows of 17-24 chunks in
+        // 64-row tiles; the 4-chunk 8-group tile would spill and keeps the
+        // plain loop): taken for graphs of at most 64 slots a node when the
+        // LDS already keeps the launch at 2 wavefronts a SIMD or fewer.  At
+        // 2M x 96, L = 2075 it is 2.6 % faster than the loop without it.
+        const size_t wg_lds = (size_t)a.wave_lds * waves_per_block;
+        const size_t lds_waves = wg_lds ? (LDS_PER_CU / wg_lds) * waves_per_block : 0;
+        if (a.deg_max <= (uint32_t)WAVE && lds_waves <= 8)
+            return launch_w<CPL, G, false, false, true>(a, waves_per_block, s);
+    }
     return launch_w<CPL, G, COLLECT, false>(a, waves_per_block, s);
 }
 
